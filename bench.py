@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark of the SBP-SAT Poisson hot path on one MI355X.
+
+Workload (one "step"): one mixed-precision MG-PCG solve (FP32 inner
+MG-preconditioned CG inside FP64 iterative refinement) of the 4097 x 4097,
+SBP order 4 problem with a seeded random Fourier-mode source (64 modes) and
+boundary data (8 modes per face), from x0 = 0 to a relative residual that
+gives a relative H-norm error <= 1e-10 against a tightly converged FP64
+reference solution.  The same problem is also solved with the plain FP64
+MG-PCG (reported beside it), and the matrix-free apply is timed alone with
+rotating buffers larger than the 256 MB Infinity Cache for the roofline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4097] [--p 4]
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+REL_ERR_TARGET = 1e-10
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=4097)
+    ap.add_argument("--p", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=2022)
+    ap.add_argument("--rtol", type=float, default=1e-11, help="stopping rule ||r|| < rtol ||b||")
+    ap.add_argument("--apply-reps", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--only-apply", action="store_true", help="time the apply only (profiling runs)")
+    return ap.parse_args()
+
+
+def fourier_problem(h, N, seed, torch):
+    """Device-side data generator for the seeded Fourier-mode problem (the
+    mode list is the same splitmix64 stream the tests use)."""
+    from oracle.sbp_oracle import fourier_modes
+    src, bnd = fourier_modes(seed)
+    dev = h.device
+    t = torch.linspace(0.0, 1.0, N, dtype=torch.float64, device=dev)
+    yi = t[:, None]   # fast index i -> y   (tensor is [j, i] row-major; here build [i, j] then transpose)
+    xj = t[None, :]
+    F = torch.zeros((N, N), dtype=torch.float64, device=dev)
+    for kx, ky, a, ph in src:
+        F += a * torch.cos(torch.pi * (kx * xj + ky * yi) + ph)
+    f = F.t().contiguous().reshape(-1)          # element (i, j) at i + j N
+    g = []
+    for modes in bnd:
+        v = torch.zeros(N, dtype=torch.float64, device=dev)
+        for kk, a, ph in modes:
+            v += a * torch.cos(torch.pi * kk * t + ph)
+        g.append(v)
+    return h.assemble_rhs(f, *g)
+
+
+def time_apply(h, torch, dtype, reps, nbuf):
+    """Average duration of the stencil apply kernel over reps launches that
+    rotate over nbuf input/output pairs (HIP events on the launch stream)."""
+    n = h.n
+    ins = [torch.randn(n, dtype=dtype, device=h.device) for _ in range(nbuf)]
+    outs = [torch.empty(n, dtype=dtype, device=h.device) for _ in range(nbuf)]
+    for k in range(nbuf):
+        h.apply(ins[k], outs[k])
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for r in range(reps):
+        k = r % nbuf
+        h.apply(ins[k], outs[k])
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del ins, outs
+    return ms
+
+
+def time_copy(h, torch, reps, nbytes, nbuf):
+    src = [torch.empty(nbytes // 8, dtype=torch.float64, device=h.device).fill_(1.0) for _ in range(nbuf)]
+    dst = [torch.empty_like(src[0]) for _ in range(nbuf)]
+    for k in range(nbuf):
+        h.copy(src[k], dst[k])
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for r in range(reps):
+        h.copy(src[r % nbuf], dst[r % nbuf])
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def cpu_baseline(N, p, seed, threads, rtol):
+    """The C oracle's FP64 MG-PCG (same V-cycle, OpenMP) on the host cores,
+    one solve of the same problem: a bounded sample of the workload."""
+    from oracle import c_oracle
+    cores = threads or min(16, os.cpu_count() or 1)
+    res = c_oracle.time_mgpcg_fourier(N, p, seed, rtol, cores)
+    return {"value": 1.0 / res["seconds"], "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"1 FP64 MG-PCG solve of the same {N}^2 p={p} problem by oracle/sbp_oracle.c "
+                      f"({res['iters']} iterations, {res['seconds']:.2f} s, OpenMP {cores} threads)"}
+
+
+def traffic_from_profiles(dtype_tag):
+    path = os.path.join(ROOT, "profiles", "apply_traffic.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return d.get(dtype_tag)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": a.gpus}), flush=True)
+        sys.exit(2)
+    import torch
+    from sc_2022_amd.solver import SbpPoisson
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    N, p = a.n, a.p
+    n = N * N
+    h = SbpPoisson(N, p, device=dev)
+
+    # ---- apply roofline (FP64 and FP32), rotating > 1 GB of buffers ----
+    nbuf64 = max(2, int((2 * 1024 ** 3) // (2 * n * 8)) + 1)
+    nbuf32 = max(2, int((2 * 1024 ** 3) // (2 * n * 4)) + 1)
+    apply_ms64 = time_apply(h, torch, torch.float64, a.apply_reps, nbuf64)
+    apply_ms32 = time_apply(h, torch, torch.float32, a.apply_reps, nbuf32)
+    bytes64 = 2 * n * 8
+    bytes32 = 2 * n * 4
+    gbs64 = bytes64 / (apply_ms64 * 1e-3) / 1e9
+    gbs32 = bytes32 / (apply_ms32 * 1e-3) / 1e9
+    copy_ms = time_copy(h, torch, 100, n * 8, nbuf64)
+    copy_gbs = 2 * n * 8 / (copy_ms * 1e-3) / 1e9
+    if a.only_apply:
+        print(json.dumps({"apply_f64_us": apply_ms64 * 1e3, "apply_f64_GBs": gbs64,
+                          "apply_f32_us": apply_ms32 * 1e3, "apply_f32_GBs": gbs32,
+                          "copy_GBs": copy_gbs}), flush=True)
+        return
+
+    # ---- problem and FP64 reference solution (untimed) ----
+    b = fourier_problem(h, N, a.seed, torch)
+    bnorm = float(torch.linalg.vector_norm(b))
+    xref, rref = h.mgpcg(b, abstol=1e-14 * bnorm, maxiter=200, raise_on_fail=False)
+    href = h.hnorm(xref)
+
+    def rel_err(x):
+        return h.hnorm(x - xref) / href
+
+    abstol = a.rtol * bnorm
+    x = torch.zeros_like(b)
+
+    def solve_mixed():
+        x.zero_()
+        return h.mgpcg_mixed(b, x, abstol=abstol)[1]
+
+    def solve_f64():
+        x.zero_()
+        return h.mgpcg(b, x, abstol=abstol, maxiter=200)[1]
+
+    # ---- FP64 MG-PCG (reported beside the headline) ----
+    for _ in range(max(1, a.warmup)):
+        r64 = solve_f64()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        r64 = solve_f64()
+    torch.cuda.synchronize()
+    t64 = (time.perf_counter() - t0) / a.steps
+    err64 = rel_err(x)
+
+    # ---- headline: mixed-precision solves ----
+    for _ in range(a.warmup):
+        rmx = solve_mixed()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        rmx = solve_mixed()
+    torch.cuda.synchronize()
+    tmx = (time.perf_counter() - t0) / a.steps
+    errmx = rel_err(x)
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(N, p, a.seed, a.cpu_threads, a.rtol)
+        except Exception as e:  # reported, never silently replaced
+            cpu = {"error": f"{type(e).__name__}: {e}"}
+
+    out = {
+        "metric": "mixed-precision MG-PCG solves/s at 4097^2 SBP4 (rel err <= 1e-10 vs FP64), + apply HBM GB/s",
+        "value": 1.0 / tmx,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": tmx * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 outer",
+        "data": "synthetic: seeded Fourier-mode source (64 modes) and boundary data (8 modes/face)",
+        "config": {"workload": f"MG-PCG solve, N={N} (n={n}), SBP p={p}, V-cycle 2+2 damped Jacobi to 17^2, "
+                               f"stop ||r|| < {a.rtol:g} ||b||", "seed": a.seed},
+        "rel_err_mixed": errmx,
+        "rel_err_f64": err64,
+        "meets_rel_err_target": bool(errmx <= REL_ERR_TARGET),
+        "mixed": {"solves_per_s": 1.0 / tmx, "outer": rmx.iters, "inner": rmx.inner_iters,
+                  "ms": tmx * 1e3},
+        "f64": {"solves_per_s": 1.0 / t64, "iters": r64.iters, "ms": t64 * 1e3},
+        "speedup_mixed_vs_f64": t64 / tmx,
+        "apply": {"f64_us": apply_ms64 * 1e3, "f64_GBs": gbs64, "f32_us": apply_ms32 * 1e3, "f32_GBs": gbs32,
+                  "copy_GBs": copy_gbs, "rotating_buffers": nbuf64},
+        "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs64 / HBM_PEAK_GBS, "traffic": traffic_from_profiles("f64"),
+                     "kernel": "stencil_pass FP64 apply (OutStore), algorithmic bytes 2*n*8 per launch"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * sbp.h -- C ABI of the MI355X (gfx950) matrix-free SBP-SAT 2-D Poisson solver.
+ *
+ * Grid and layout (the reference's): a level-k problem has N = 2^k + 1 points
+ * per side on [0,1]^2 (the apply and CG accept any N >= 2*closure; the
+ * multigrid entry points need N = 2^k + 1 >= 17).  Grid functions are dense
+ * device arrays of N*N elements with element (i, j) at i + j*N, i fast
+ * (Julia column-major (Nx, Ny)).  The fast index i carries the Neumann faces
+ * (i = 0 and i = N-1), the slow index j the Dirichlet faces (j = 0, j = N-1).
+ *
+ * The operator is the reference's SPD matrix
+ *     A = -H~ (D2 + SAT_W + SAT_E + SAT_S + SAT_N)      (tau_D = -13/h, tau_N = -1)
+ * (preconditioned_CG/level_2_multigrid_new.jl:141-185) for SBP order p = 2 or 4.
+ *
+ * Conventions: every pointer argument that names a grid function is a device
+ * pointer owned by the caller; work is ordered on the handle's stream (the
+ * null stream unless sbp_set_stream was called); scalar outputs (iteration
+ * counts, norms) are host pointers and the call synchronises the stream before
+ * it returns them.  A handle is not thread-safe; use one handle per stream.
+ * Every entry point returns an sbp_status.  No entry point falls back to host
+ * computation.
+ */
+#ifndef SBP_H
+#define SBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    SBP_OK = 0,
+    SBP_ERR_INVALID_SIZE = 1,   /* N too small, or not 2^k+1 where multigrid needs it */
+    SBP_ERR_INVALID_ORDER = 2,  /* p not in {2, 4} */
+    SBP_ERR_INVALID_ARG = 3,    /* null pointer, bad dtype/sign/level, bad parameter */
+    SBP_ERR_HIP = 4,            /* a HIP runtime call failed */
+    SBP_ERR_NONFINITE = 5,      /* NaN/Inf met in a residual or inner product */
+    SBP_ERR_BREAKDOWN = 6,      /* p'Ap <= 0 (operator or preconditioner not SPD) */
+    SBP_ERR_NOT_CONVERGED = 7,  /* maxiter reached; x holds the last iterate */
+    SBP_ERR_NO_DEVICE = 8       /* no HIP device visible */
+} sbp_status;
+
+typedef enum { SBP_F64 = 0, SBP_F32 = 1 } sbp_dtype;
+
+typedef struct {
+    int nu_pre;              /* pre-smoothing sweeps per level (default 2) */
+    int nu_post;             /* post-smoothing sweeps per level (default 2) */
+    double omega;            /* damped-Jacobi weight (default 2/3) */
+    int coarse_n;            /* coarsest grid side, 2^k+1 (default 17) */
+    double restrict_scale;   /* coarse rhs = restrict_scale * full_weighting(r);
+                                4 (default) gives R = P^T, 1 the reference's scale */
+    int smoother;            /* 0 = damped Jacobi (default), 1 = Richardson */
+    double richardson_omega; /* Richardson weight (reference: 0.15) */
+    int poll_every;          /* host convergence poll interval in iterations (default 4) */
+    double inner_rtol;       /* mixed precision: inner FP32 solve relative tolerance (default 1e-5) */
+    int inner_maxiter;       /* mixed precision: inner iteration cap (default 50) */
+    int max_outer;           /* mixed precision: refinement step cap (default 10) */
+    int tile_i;              /* apply tile width in i (0 = autotuned) */
+    int tile_j;              /* apply rows per thread (0 = autotuned) */
+} sbp_params;
+
+typedef struct sbp_handle_s* sbp_handle;
+
+/* Library / handle management ------------------------------------------- */
+const char* sbp_version(void);
+const char* sbp_status_string(int status);
+void sbp_default_params(sbp_params* prm);
+/* Builds the operator tables for side N and order p and allocates the fine
+ * work vectors and, when N = 2^k+1 >= 17, the multigrid hierarchy.
+ * Replaces the per-call set-up of Assembling_matrix (level_2_multigrid_new.jl:125)
+ * and the per-call CuArray allocations in initial_guess_cg.jl:152-161. */
+int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out);
+int sbp_destroy(sbp_handle h);
+int sbp_set_stream(sbp_handle h, void* hip_stream);
+int sbp_get_info(sbp_handle h, int* N, int* p, int* nlevels);
+int sbp_set_tiles(sbp_handle h, int level, int tile_i, int tile_j);
+
+/* Operator --------------------------------------------------------------- */
+/* Au = sign * A u on multigrid level `level` (0 = finest).  sign = -1 gives
+ * the reference's matrix_free_A_full_GPU(idata, odata) (split_matrix_free.jl:444-486),
+ * which returns -A u; sign = +1 gives the SPD A u.  dtype selects the element
+ * type of both u and Au. */
+int sbp_apply(sbp_handle h, int level, const void* u, void* Au, int sign, int dtype);
+/* Au = A u and *dot = u'Au in one pass (FP64). */
+int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot);
+/* d = diag(A) on the finest level (FP64). */
+int sbp_diag(sbp_handle h, double* d);
+/* b = -H~ (f + SAT_W_r gW + SAT_E_r gE + SAT_S_r gS + SAT_N_r gN)
+ * (level_2_multigrid_new.jl:164-167,181,185).  f: N*N; gW, gE: N values
+ * indexed by i; gS, gN: N values indexed by j.  Any g may be NULL (zero). */
+int sbp_assemble_rhs(sbp_handle h, const double* f, const double* gW, const double* gE,
+                     const double* gS, const double* gN, double* b);
+/* Transpose of sbp_assemble_rhs: given lambda, df = -H~ lambda and
+ * dg_X = -(H~ SAT_X_r)^T lambda.  Any output may be NULL. */
+int sbp_rhs_transpose(sbp_handle h, const double* lam, double* df, double* dgW,
+                      double* dgE, double* dgS, double* dgN);
+/* *out = sqrt(v' H~ v) (matrix_free_mg_cg.jl:464). */
+int sbp_hnorm(sbp_handle h, const double* v, double* out);
+/* *out = u'v and *nrm = ||u||_2 (either may be NULL). */
+int sbp_dot(sbp_handle h, const double* u, const double* v, double* out);
+
+/* Multigrid components (level l has side N_l = (N_{l-1}+1)/2) ------------ */
+/* rc = restrict_scale * FW(rf): full weighting [1 2 1;2 4 2;1 2 1]/16 with zero
+ * padding (restriction_2D_kernel, initial_guess_cg.jl:98-131), fine level l -> l+1. */
+int sbp_restrict(sbp_handle h, int level, const void* rf, void* rc, int dtype);
+/* xf += P ec, bilinear (prolongation_2D_kernel, initial_guess_cg.jl:41-77), l+1 -> l. */
+int sbp_prolong_add(sbp_handle h, int level, const void* ec, void* xf, int dtype);
+/* nsweeps damped-Jacobi sweeps x <- x + omega D^-1 (b - A x) on level l
+ * (jacobi_brittany!, level_2_multigrid_new.jl:335-344); smoother = 1 uses
+ * Richardson x <- x + w (b - A x) (matrix_free_richardson, initial_guess_cg.jl:134-144). */
+int sbp_smooth(sbp_handle h, int level, const void* b, void* x, int nsweeps, int dtype);
+/* Exact coarsest-level solve x = A_c^-1 b in FP64 (replaces the UMFPACK
+ * A_2h \ f of initial_guess_cg.jl:170). */
+int sbp_coarse_solve(sbp_handle h, const double* b, double* x);
+/* One V-cycle z ~ A^-1 r from a zero initial guess (matrix_free_Two_level_multigrid,
+ * initial_guess_cg.jl:147-180, generalised to all levels). */
+int sbp_vcycle(sbp_handle h, const void* r, void* z, int dtype);
+
+/* Solvers (x holds the initial guess on entry and the solution on exit) --- */
+/* Matrix-free CG on the SPD A, stop when ||r||_2 < abstol
+ * (CG_Matrix_Free_GPU_v2 / CG_full_GPU, split_matrix_free.jl:647-714, with the
+ * sign fixed).  hist (nullable, maxiter+1 doubles) receives ||r_k||_2. */
+int sbp_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter,
+           int* iters, double* resnorm, double* hist);
+/* FP64 matrix-free MG-preconditioned CG (matrix_free_MGCG, initial_guess_cg.jl:303-346). */
+int sbp_mgpcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter,
+              int* iters, double* resnorm, double* hist);
+/* FP32 inner MG-PCG inside FP64 iterative refinement: the outer residual
+ * r = b - A x, the update x += d, the norms and the coarse solve are FP64;
+ * the inner preconditioned solve of A d = r is FP32.  Stops when the FP64
+ * ||r||_2 < abstol.  inner_iters (nullable) receives the total inner count. */
+int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double abstol, int max_outer,
+                    int* outer_iters, int* inner_iters, double* resnorm);
+
+/* CSR comparison path (the assembled operator, as CG_GPU_sparse,
+ * split_matrix_free.jl:744-769) ------------------------------------------ */
+/* Fills device CSR arrays (int32 indices) of the assembled A; call first with
+ * row_ptr = col = val = NULL to get *nnz. */
+int sbp_csr_assemble(sbp_handle h, int32_t* row_ptr, int32_t* col, double* val, int64_t* nnz);
+/* y = A x with a hand-written CSR SpMV kernel. */
+int sbp_csr_spmv(sbp_handle h, const int32_t* row_ptr, const int32_t* col, const double* val,
+                 const double* x, double* y);
+/* CG with the CSR SpMV in place of the matrix-free apply. */
+int sbp_csr_cg(sbp_handle h, const int32_t* row_ptr, const int32_t* col, const double* val,
+               const double* b, double* x, double abstol, int maxiter, int* iters,
+               double* resnorm);
+
+/* Streaming-copy calibration kernel (copy_naive!, split_matrix_free.jl:51-77). */
+int sbp_copy(sbp_handle h, const void* src, void* dst, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SBP_H */

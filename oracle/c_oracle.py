@@ -1,0 +1,129 @@
+"""ctypes wrapper of the C oracle (oracle/sbp_oracle.c -> oracle/liboracle.so).
+
+Test infrastructure and CPU baseline only.  ``build()`` runs the Makefile in
+this directory (gcc, OpenMP)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import time
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_apply.argtypes = [C.c_int, C.c_int, dp, dp]
+        L.orc_diag.argtypes = [C.c_int, C.c_int, dp]
+        L.orc_csr.restype = C.c_long
+        L.orc_csr.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_rhs.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 5 + [dp]
+        L.orc_cg_csr.argtypes = [C.c_long, ip, ip, dp, dp, dp, C.c_double, C.c_int, C.POINTER(C.c_double)]
+        L.orc_mgpcg.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
+                                C.c_double, C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def set_threads(t):
+    lib().orc_set_threads(int(t))
+
+
+def apply(p, N, u):
+    out = np.empty(N * N)
+    assert lib().orc_apply(p, N, np.ascontiguousarray(u, dtype=np.float64), out) == 0
+    return out
+
+
+def diag(p, N):
+    d = np.empty(N * N)
+    assert lib().orc_diag(p, N, d) == 0
+    return d
+
+
+def csr(p, N):
+    L = lib()
+    nnz = L.orc_csr(p, N, None, None, None)
+    assert nnz > 0
+    rp = np.empty(N * N + 1, dtype=np.int32)
+    col = np.empty(nnz, dtype=np.int32)
+    val = np.empty(nnz)
+    L.orc_csr(p, N, rp.ctypes.data, col.ctypes.data, val.ctypes.data)
+    return rp, col, val
+
+
+def rhs(p, N, f=None, gW=None, gE=None, gS=None, gN=None):
+    b = np.empty(N * N)
+    arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (f, gW, gE, gS, gN)]
+    args = [None if a is None else a.ctypes.data for a in arrs]
+    assert lib().orc_rhs(p, N, *args, b) == 0
+    return b
+
+
+def cg_csr(A, b, abstol, maxiter=None, x=None):
+    rp, col, val = A
+    n = len(rp) - 1
+    x = np.zeros(n) if x is None else np.array(x, dtype=np.float64)
+    rn = C.c_double()
+    it = lib().orc_cg_csr(n, rp, col, val, np.ascontiguousarray(b, dtype=np.float64), x, abstol,
+                          n if maxiter is None else maxiter, C.byref(rn))
+    return x, it, rn.value
+
+
+def mgpcg(p, N, b, abstol, maxiter=200, nu_pre=2, nu_post=2, omega=2.0 / 3.0, rscale=4.0, x=None):
+    x = np.zeros(N * N) if x is None else np.array(x, dtype=np.float64)
+    rn = C.c_double()
+    it = lib().orc_mgpcg(p, N, np.ascontiguousarray(b, dtype=np.float64), x, abstol, maxiter, nu_pre, nu_post,
+                         omega, rscale, C.byref(rn))
+    if it < 0:
+        raise ValueError("orc_mgpcg: unsupported size")
+    return x, it, rn.value
+
+
+def fourier_rhs(p, N, seed):
+    """Right-hand side of the seeded Fourier-mode problem (same mode list as
+    sbp_oracle.fourier_modes), built on the host."""
+    from .sbp_oracle import fourier_modes
+    src, bnd = fourier_modes(seed)
+    t = np.linspace(0.0, 1.0, N)
+    F = np.zeros((N, N))            # F[j, i]
+    xj = t[:, None]; yi = t[None, :]
+    for kx, ky, a, ph in src:
+        F += a * np.cos(np.pi * (kx * xj + ky * yi) + ph)
+    g = []
+    for modes in bnd:
+        v = np.zeros(N)
+        for kk, a, ph in modes:
+            v += a * np.cos(np.pi * kk * t + ph)
+        g.append(v)
+    return rhs(p, N, F.reshape(-1), *g)
+
+
+def time_mgpcg_fourier(N, p, seed, rtol, threads):
+    """One FP64 MG-PCG solve of the seeded Fourier problem on `threads` host
+    threads (the bench's CPU-baseline sample)."""
+    set_threads(threads)
+    b = fourier_rhs(p, N, seed)
+    abstol = rtol * float(np.linalg.norm(b))
+    t0 = time.perf_counter()
+    x, it, rn = mgpcg(p, N, b, abstol)
+    dt = time.perf_counter() - t0
+    return {"seconds": dt, "iters": it, "resnorm": rn}

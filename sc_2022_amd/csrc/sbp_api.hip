@@ -1,0 +1,853 @@
+// sbp_api.hip -- C ABI (include/sbp.h) over the gfx950 kernels in sbp_kernels.h.
+//
+// The handle owns everything the reference re-allocated per call (the MG
+// hierarchy buffers of initial_guess_cg.jl:152-161, the apply temporaries of
+// split_matrix_free.jl:459-462, the coarse LU of initial_guess_cg.jl:521),
+// allocated once in sbp_create.  Krylov scalars live on the device; kernels
+// of an iteration test a device "done" flag and fall through once the
+// iteration has converged, so the host only polls every poll_every
+// iterations while the iteration count keeps the reference's semantics
+// (the first k with ||r_k||_2 < abstol).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/sbp.h"
+#include "sbp_kernels.h"
+#include "sbp_launch.h"
+#include "sbp_tables.h"
+
+using namespace sbp;
+
+namespace {
+
+constexpr int kMaxLevels = 16;
+constexpr int kEwThreads = 256;
+
+struct Level {
+    int N = 0;
+    size_t n = 0;
+    OpTab tb{};
+    int ti = 0, tj = 0;
+    // per precision (0 = f64, 1 = f32): rhs, two sweep buffers, output, residual
+    void* b[2] = {nullptr, nullptr};
+    void* xa[2] = {nullptr, nullptr};
+    void* xb[2] = {nullptr, nullptr};
+    void* xo[2] = {nullptr, nullptr};
+    void* r[2] = {nullptr, nullptr};
+};
+
+}  // namespace
+
+struct sbp_handle_s {
+    int N = 0, p = 0, nlev = 0;
+    bool mg = false;
+    sbp_params prm{};
+    hipStream_t st = nullptr;
+    RhsTab rt{};
+    Level L[kMaxLevels];
+    double* Ainv = nullptr;
+    int nc = 0;
+    // fine Krylov vectors
+    double *r = nullptr, *pv = nullptr, *Ap = nullptr, *z = nullptr;
+    float *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *z32 = nullptr, *d32 = nullptr;
+    double* partial = nullptr;
+    size_t npartial = 0;
+    Scalars* sc = nullptr;      // outer / FP64 solver
+    Scalars* sc_in = nullptr;   // inner FP32 solver
+    Scalars* sc_host = nullptr; // pinned mirror
+    double* dtmp = nullptr;     // small device scalars
+    bool have32 = false;
+};
+
+namespace {
+
+#define HIPCHK(x)                                   \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) return SBP_ERR_HIP;   \
+    } while (0)
+
+#define STCHK(x)                                    \
+    do {                                            \
+        int s_ = (x);                               \
+        if (s_ != SBP_OK) return s_;                \
+    } while (0)
+
+inline int ew_blocks(size_t n) {
+    size_t b = (n + kEwThreads - 1) / kEwThreads;
+    return (int)std::max<size_t>(1, std::min<size_t>(b, 4096));
+}
+
+// Tile shapes that are compiled in.
+inline bool tile_ok(int ti, int tj) {
+#define SBP_TILE_OK(TI_, J_) if (ti == TI_ && tj == J_) return true;
+    SBP_TILE_SHAPES(SBP_TILE_OK)
+#undef SBP_TILE_OK
+    return false;
+}
+
+inline void default_tile(int N, int* ti, int* tj) {
+    if (N >= 1025) { *ti = 256; *tj = 8; }
+    else if (N >= 129) { *ti = 128; *tj = 8; }
+    else { *ti = 64; *tj = 4; }
+}
+
+inline size_t pass_blocks(int N, int ti, int tj) {
+    return (size_t)((N + ti - 1) / ti) * (size_t)((N + tj - 1) / tj);
+}
+
+template <typename T, class IN, class OUT>
+int launch_pass(int p, const Level& L, const IN& in, const OUT& out, const int* done, double* partial,
+                hipStream_t st, int* nparts) {
+    if (nparts) *nparts = (int)pass_blocks(L.N, L.ti, L.tj);
+#define SBP_TILE_CASE(TI_, J_) \
+    if (L.ti == TI_ && L.tj == J_) return launch_tile<TI_, J_, T, IN, OUT>(p, L.tb, in, out, done, partial, st);
+    SBP_TILE_SHAPES(SBP_TILE_CASE)
+#undef SBP_TILE_CASE
+    return SBP_ERR_INVALID_ARG;
+}
+
+template <typename T> inline int prec_index();
+template <> inline int prec_index<double>() { return 0; }
+template <> inline int prec_index<float>() { return 1; }
+
+int alloc_level_buffers(sbp_handle h, int prec) {
+    const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
+    for (int l = 0; l < h->nlev; ++l) {
+        Level& L = h->L[l];
+        const size_t bytes = L.n * es;
+        if (l > 0) HIPCHK(hipMalloc(&L.b[prec], bytes));
+        HIPCHK(hipMalloc(&L.xo[prec], bytes));
+        if (l < h->nlev - 1) {
+            HIPCHK(hipMalloc(&L.xa[prec], bytes));
+            HIPCHK(hipMalloc(&L.xb[prec], bytes));
+            HIPCHK(hipMalloc(&L.r[prec], bytes));
+        }
+    }
+    return SBP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Smoothing / V-cycle
+// ---------------------------------------------------------------------------
+// One smoothing pass: dst = S(operand) with the operand formed by IN.
+template <typename T, class IN>
+int smooth_pass(sbp_handle h, const Level& L, const IN& in, const T* b, T* dst, const T* dotv, const int* done,
+                double* partial) {
+    const bool rich = h->prm.smoother == 1;
+    OutSmooth<T> o{b, dst, (T)(rich ? h->prm.richardson_omega : h->prm.omega), dotv, rich ? 0 : 1};
+    return launch_pass<T>(h->p, L, in, o, done, partial, h->st, nullptr);
+}
+
+// V-cycle on level l: x_out ~ A_l^-1 b (zero initial guess).  When dotv is
+// non-null the last pass also writes block partials of dotv' x_out.
+template <typename T>
+int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const int* done, double* partial) {
+    const int pi = prec_index<T>();
+    Level& L = h->L[l];
+    hipStream_t st = h->st;
+    if (l == h->nlev - 1) {
+        hipLaunchKernelGGL((coarse_solve<T>), dim3(1), dim3(320), 0, st, h->Ainv, b, out, (int)L.n, done);
+        HIPCHK(hipGetLastError());
+        if (dotv) {
+            hipLaunchKernelGGL((dot_partial<T, kEwThreads>), dim3(1), dim3(kEwThreads), 0, st, dotv, (const T*)out,
+                               L.n, partial, nullptr);
+            HIPCHK(hipGetLastError());
+        }
+        return SBP_OK;
+    }
+    T* xa = (T*)L.xa[pi];
+    T* xb = (T*)L.xb[pi];
+    T* rr = (T*)L.r[pi];
+    const int nu1 = h->prm.nu_pre, nu2 = h->prm.nu_post;
+    const T om = (T)(h->prm.smoother == 1 ? h->prm.richardson_omega : h->prm.omega);
+    // Pre-smoothing from zero.  The first sweep x1 = w D^-1 b is formed on
+    // the fly inside the second sweep's pass.
+    T* cur = xa;
+    if (nu1 == 0) {
+        hipLaunchKernelGGL((fill<T, kEwThreads>), dim3(ew_blocks(L.n)), dim3(kEwThreads), 0, st, xa, T(0), L.n, done);
+        HIPCHK(hipGetLastError());
+    } else {
+        const InSmooth0<T> in0{b, om, h->prm.smoother == 1 ? 0 : 1};
+        if (nu1 == 1) {
+            STCHK(launch_pass<T>(h->p, L, in0, OutCopy<T>{xa}, done, nullptr, st, nullptr));
+        } else {
+            STCHK(smooth_pass<T>(h, L, in0, b, xa, nullptr, done, nullptr));
+            for (int s = 2; s < nu1; ++s) {
+                T* nxt = (cur == xa) ? xb : xa;
+                STCHK(smooth_pass<T>(h, L, InPlain<T>{cur}, b, nxt, nullptr, done, nullptr));
+                cur = nxt;
+            }
+        }
+    }
+    // Residual and restriction.
+    {
+        OutResid<T, T> o{b, rr};
+        STCHK(launch_pass<T>(h->p, L, InPlain<T>{cur}, o, done, nullptr, st, nullptr));
+        Level& Lc = h->L[l + 1];
+        dim3 g((Lc.N + 255) / 256, Lc.N);
+        hipLaunchKernelGGL((restrict_fw<T>), g, dim3(256), 0, st, (const T*)rr, (T*)Lc.b[pi], L.N, Lc.N,
+                           (T)h->prm.restrict_scale, done);
+        HIPCHK(hipGetLastError());
+    }
+    Level& Lc = h->L[l + 1];
+    T* ec = (T*)Lc.xo[pi];
+    STCHK(vcycle_level<T>(h, l + 1, (const T*)Lc.b[pi], ec, nullptr, done, nullptr));
+    // Prolongation folded into the first post-smoothing pass.
+    if (nu2 == 0) {
+        hipLaunchKernelGGL((prolong_add<T>), dim3((L.N + 255) / 256, L.N), dim3(256), 0, st, (const T*)ec, cur, L.N,
+                           Lc.N, done);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(L.n)), dim3(kEwThreads), 0, st, (const T*)cur,
+                           out, L.n, done);
+        HIPCHK(hipGetLastError());
+        if (dotv) {
+            hipLaunchKernelGGL((dot_partial<T, kEwThreads>), dim3(ew_blocks(L.n)), dim3(kEwThreads), 0, st, dotv,
+                               (const T*)out, L.n, partial, nullptr);
+            HIPCHK(hipGetLastError());
+        }
+        return SBP_OK;
+    }
+    T* other = (cur == xa) ? xb : xa;
+    {
+        T* dst = (nu2 == 1) ? out : other;
+        STCHK(smooth_pass<T>(h, L, InProlong<T>{cur, ec, Lc.N}, b, dst, nu2 == 1 ? dotv : nullptr, done,
+                             nu2 == 1 ? partial : nullptr));
+        cur = dst;
+    }
+    for (int s = 1; s < nu2; ++s) {
+        const bool last = (s == nu2 - 1);
+        T* dst = last ? out : ((cur == xa) ? xb : xa);
+        STCHK(smooth_pass<T>(h, L, InPlain<T>{cur}, b, dst, last ? dotv : nullptr, done, last ? partial : nullptr));
+        cur = dst;
+    }
+    return SBP_OK;
+}
+
+// Number of partials the V-cycle's final pass writes on level 0.
+template <typename T>
+int vcycle_nparts(sbp_handle h) {
+    const Level& L = h->L[0];
+    if (h->nlev == 1) return 1;
+    if (h->prm.nu_post == 0) return ew_blocks(L.n);
+    return (int)pass_blocks(L.N, L.ti, L.tj);
+}
+
+int poll(sbp_handle h, Scalars* sc) {
+    HIPCHK(hipMemcpyAsync(h->sc_host, sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SBP_OK;
+}
+
+int finish_status(const Scalars& s, int maxiter_hit) {
+    if (s.status == 5) return SBP_ERR_NONFINITE;
+    if (s.status == 6) return SBP_ERR_BREAKDOWN;
+    if (!s.done && maxiter_hit) return SBP_ERR_NOT_CONVERGED;
+    return SBP_OK;
+}
+
+int ensure32(sbp_handle h) {
+    if (h->have32) return SBP_OK;
+    const size_t n = h->L[0].n;
+    HIPCHK(hipMalloc(&h->r32, n * sizeof(float)));
+    HIPCHK(hipMalloc(&h->p32, n * sizeof(float)));
+    HIPCHK(hipMalloc(&h->Ap32, n * sizeof(float)));
+    HIPCHK(hipMalloc(&h->z32, n * sizeof(float)));
+    HIPCHK(hipMalloc(&h->d32, n * sizeof(float)));
+    if (h->mg) STCHK(alloc_level_buffers(h, 1));
+    h->have32 = true;
+    return SBP_OK;
+}
+
+// Preconditioned CG body shared by FP64 and FP32: solves A x = b given the
+// initial residual r (already in r, with sc->rr / abstol / done set by the
+// caller's init kernel).  prec = V-cycle when use_mg, identity otherwise.
+template <typename T>
+int pcg_loop(sbp_handle h, T* x, T* r, T* pv, T* Ap, T* z, Scalars* sc, int maxiter, bool use_mg, double* hist,
+             int* iters_out) {
+    hipStream_t st = h->st;
+    const Level& L0 = h->L[0];
+    const size_t n = L0.n;
+    const int* done = &sc->done;
+    int np = 0;
+    if (use_mg) {
+        STCHK(vcycle_level<T>(h, 0, r, z, r, done, h->partial));
+        hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(256), 0, st, h->partial, vcycle_nparts<T>(h), sc);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, (const T*)z, pv, n,
+                           done);
+    } else {
+        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, (const T*)r, pv, n,
+                           done);
+    }
+    HIPCHK(hipGetLastError());
+    const int poll_every = std::max(1, h->prm.poll_every);
+    int it = 0;
+    for (; it < maxiter; ++it) {
+        STCHK(launch_pass<T>(h->p, L0, InPlain<T>{pv}, OutStoreDot<T>{Ap}, done, h->partial, st, &np));
+        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(256), 0, st, h->partial, np, sc, use_mg ? 1 : 0);
+        const int eb = ew_blocks(n);
+        hipLaunchKernelGGL((update_xr<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, x, r, (const T*)pv,
+                           (const T*)Ap, (const Scalars*)sc, n, h->partial);
+        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(256), 0, st, h->partial, eb, sc, it, hist, use_mg ? 0 : 1);
+        HIPCHK(hipGetLastError());
+        if (use_mg) {
+            STCHK(vcycle_level<T>(h, 0, r, z, r, done, h->partial));
+            hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(256), 0, st, h->partial, vcycle_nparts<T>(h), sc);
+            hipLaunchKernelGGL((update_p<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, pv, (const T*)z,
+                               (const Scalars*)sc, n);
+        } else {
+            hipLaunchKernelGGL((update_p<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, pv, (const T*)r,
+                               (const Scalars*)sc, n);
+        }
+        HIPCHK(hipGetLastError());
+        if ((it + 1) % poll_every == 0) {
+            STCHK(poll(h, sc));
+            if (h->sc_host->done) { ++it; break; }
+        }
+    }
+    STCHK(poll(h, sc));
+    if (iters_out) *iters_out = h->sc_host->iters;
+    return finish_status(*h->sc_host, !h->sc_host->done);
+}
+
+}  // namespace
+
+namespace {
+template <typename T>
+int smooth_t(sbp_handle h, int level, const T* b, T* x, int nsweeps) {
+    const int pi = prec_index<T>();
+    Level& L = h->L[level];
+    // Ping-pong through a level scratch buffer (the finest level borrows Ap/Ap32).
+    T* tmp;
+    if (level < h->nlev - 1 && L.xb[pi]) tmp = (T*)L.xb[pi];
+    else if (level == 0) tmp = pi == 0 ? (T*)(void*)h->Ap : (T*)(void*)h->Ap32;
+    else tmp = (T*)L.xo[pi];
+    if (!tmp || tmp == x || (const T*)tmp == b) return SBP_ERR_INVALID_ARG;
+    T* cur = x;
+    for (int s = 0; s < nsweeps; ++s) {
+        T* dst = (cur == x) ? tmp : x;
+        STCHK(smooth_pass<T>(h, L, InPlain<T>{cur}, b, dst, nullptr, nullptr, nullptr));
+        cur = dst;
+    }
+    if (cur != x) {
+        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(L.n)), dim3(kEwThreads), 0, h->st, (const T*)cur,
+                           x, L.n, nullptr);
+        HIPCHK(hipGetLastError());
+    }
+    return SBP_OK;
+}
+}  // namespace
+
+namespace {
+// r = b - A x ; rr / ||r0|| check into sc.
+int init_residual(sbp_handle h, const double* b, const double* x, double abstol, double* hist) {
+    int np = 0;
+    STCHK(launch_pass<double>(h->p, h->L[0], InPlain<double>{x}, OutResid<double, double>{b, h->r}, nullptr,
+                              h->partial, h->st, &np));
+    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(256), 0, h->st, h->partial, np, h->sc, abstol, hist);
+    HIPCHK(hipGetLastError());
+    return SBP_OK;
+}
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* sbp_version(void) { return "sc_2022_amd 0.1 (gfx950)"; }
+
+const char* sbp_status_string(int s) {
+    switch (s) {
+        case SBP_OK: return "ok";
+        case SBP_ERR_INVALID_SIZE: return "invalid size";
+        case SBP_ERR_INVALID_ORDER: return "invalid SBP order (p must be 2 or 4)";
+        case SBP_ERR_INVALID_ARG: return "invalid argument";
+        case SBP_ERR_HIP: return "HIP runtime error";
+        case SBP_ERR_NONFINITE: return "non-finite value in residual or inner product";
+        case SBP_ERR_BREAKDOWN: return "breakdown: p'Ap <= 0";
+        case SBP_ERR_NOT_CONVERGED: return "not converged within maxiter";
+        case SBP_ERR_NO_DEVICE: return "no HIP device";
+        default: return "unknown status";
+    }
+}
+
+void sbp_default_params(sbp_params* prm) {
+    if (!prm) return;
+    std::memset(prm, 0, sizeof(*prm));
+    prm->nu_pre = 2;
+    prm->nu_post = 2;
+    prm->omega = 2.0 / 3.0;
+    prm->coarse_n = 17;
+    prm->restrict_scale = 4.0;
+    prm->smoother = 0;
+    prm->richardson_omega = 0.15;
+    prm->poll_every = 4;
+    prm->inner_rtol = 1e-5;
+    prm->inner_maxiter = 50;
+    prm->max_outer = 10;
+    prm->tile_i = 0;
+    prm->tile_j = 0;
+}
+
+int sbp_destroy(sbp_handle h) {
+    if (!h) return SBP_ERR_INVALID_ARG;
+    for (int l = 0; l < kMaxLevels; ++l) {
+        Level& L = h->L[l];
+        for (int q = 0; q < 2; ++q) {
+            if (L.b[q] && l > 0) (void)hipFree(L.b[q]);
+            if (L.xa[q]) (void)hipFree(L.xa[q]);
+            if (L.xb[q]) (void)hipFree(L.xb[q]);
+            if (L.xo[q]) (void)hipFree(L.xo[q]);
+            if (L.r[q]) (void)hipFree(L.r[q]);
+        }
+    }
+    void* ptrs[] = {h->Ainv, h->r, h->pv, h->Ap, h->z, h->r32, h->p32, h->Ap32, h->z32, h->d32,
+                    h->partial, h->sc, h->sc_in, h->dtmp};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    if (h->sc_host) (void)hipHostFree(h->sc_host);
+    delete h;
+    return SBP_OK;
+}
+
+int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) {
+    if (!out) return SBP_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (p != 2 && p != 4) return SBP_ERR_INVALID_ORDER;
+    {
+        OpTab t0;
+        if (!build_optab(p, N, &t0, nullptr)) return SBP_ERR_INVALID_SIZE;
+        if (N > 46340) return SBP_ERR_INVALID_SIZE;   // N*N must fit a 32-bit block index space
+    }
+    if (prm && (prm->nu_pre < 0 || prm->nu_post < 0 || prm->poll_every < 1 || prm->max_outer < 1 ||
+                prm->inner_maxiter < 1 || !(prm->omega > 0) || !(prm->restrict_scale > 0) ||
+                (prm->smoother != 0 && prm->smoother != 1)))
+        return SBP_ERR_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SBP_ERR_NO_DEVICE;
+    sbp_handle h = new (std::nothrow) sbp_handle_s();
+    if (!h) return SBP_ERR_HIP;
+    h->N = N;
+    h->p = p;
+    if (prm) h->prm = *prm; else sbp_default_params(&h->prm);
+    if (!build_optab(p, N, &h->L[0].tb, &h->rt)) { delete h; return SBP_ERR_INVALID_SIZE; }
+    h->L[0].N = N;
+    h->L[0].n = (size_t)N * N;
+    // Multigrid hierarchy when N = 2^k+1 and coarse_n = 2^c+1 <= N.
+    int nlev = 1;
+    const int cn = h->prm.coarse_n;
+    auto pow2p1 = [](int v) { return v >= 3 && ((v - 1) & (v - 2)) == 0; };
+    if (pow2p1(N) && pow2p1(cn) && cn <= N && cn <= 33) {
+        int n = N;
+        while (n > cn && nlev < kMaxLevels) {
+            n = (n + 1) / 2;
+            Level& L = h->L[nlev];
+            L.N = n;
+            L.n = (size_t)n * n;
+            if (!build_optab(p, n, &L.tb, nullptr)) break;
+            ++nlev;
+        }
+        if (h->L[nlev - 1].N == cn) h->mg = true;
+        else nlev = 1;
+    }
+    h->nlev = nlev;
+    for (int l = 0; l < nlev; ++l) {
+        Level& L = h->L[l];
+        if (l == 0 && h->prm.tile_i > 0) {
+            if (!tile_ok(h->prm.tile_i, h->prm.tile_j)) { delete h; return SBP_ERR_INVALID_ARG; }
+            L.ti = h->prm.tile_i;
+            L.tj = h->prm.tile_j;
+        } else {
+            default_tile(L.N, &L.ti, &L.tj);
+        }
+    }
+    int rc = SBP_OK;
+    do {
+        const size_t n = h->L[0].n;
+        if (hipMalloc(&h->r, n * 8) != hipSuccess || hipMalloc(&h->pv, n * 8) != hipSuccess ||
+            hipMalloc(&h->Ap, n * 8) != hipSuccess || hipMalloc(&h->z, n * 8) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+        // Enough partial slots for the finest pass at any compiled tile shape.
+        h->npartial = std::max<size_t>(pass_blocks(N, 64, 4), 4096) + 64;
+        if (hipMalloc(&h->partial, h->npartial * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+        if (hipMalloc(&h->sc, sizeof(Scalars)) != hipSuccess || hipMalloc(&h->sc_in, sizeof(Scalars)) != hipSuccess ||
+            hipMalloc(&h->dtmp, 64 * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+        if (hipHostMalloc((void**)&h->sc_host, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+        if (hipMemset(h->sc, 0, sizeof(Scalars)) != hipSuccess || hipMemset(h->sc_in, 0, sizeof(Scalars)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+        if (h->mg) {
+            h->L[0].b[0] = nullptr;
+            if ((rc = alloc_level_buffers(h, 0)) != SBP_OK) break;
+            // Coarsest level: FP64 inverse of the rediscretised operator via Cholesky.
+            const Level& Lc = h->L[nlev - 1];
+            std::vector<double> A;
+            dense_operator(Lc.tb, A);
+            const int nc = (int)Lc.n;
+            if (!spd_inverse(A, nc)) { rc = SBP_ERR_BREAKDOWN; break; }
+            h->nc = nc;
+            if (hipMalloc(&h->Ainv, (size_t)nc * nc * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+            if (hipMemcpy(h->Ainv, A.data(), (size_t)nc * nc * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+                rc = SBP_ERR_HIP; break;
+            }
+        }
+    } while (0);
+    if (rc != SBP_OK) { sbp_destroy(h); return rc; }
+    *out = h;
+    return SBP_OK;
+}
+
+int sbp_set_stream(sbp_handle h, void* s) {
+    if (!h) return SBP_ERR_INVALID_ARG;
+    h->st = (hipStream_t)s;
+    return SBP_OK;
+}
+
+int sbp_get_info(sbp_handle h, int* N, int* p, int* nlevels) {
+    if (!h) return SBP_ERR_INVALID_ARG;
+    if (N) *N = h->N;
+    if (p) *p = h->p;
+    if (nlevels) *nlevels = h->mg ? h->nlev : 0;
+    return SBP_OK;
+}
+
+int sbp_set_tiles(sbp_handle h, int level, int ti, int tj) {
+    if (!h || level < 0 || level >= h->nlev || !tile_ok(ti, tj)) return SBP_ERR_INVALID_ARG;
+    h->L[level].ti = ti;
+    h->L[level].tj = tj;
+    return SBP_OK;
+}
+
+int sbp_apply(sbp_handle h, int level, const void* u, void* Au, int sign, int dtype) {
+    if (!h || !u || !Au || (sign != 1 && sign != -1) || level < 0 || level >= h->nlev) return SBP_ERR_INVALID_ARG;
+    const Level& L = h->L[level];
+    if (dtype == SBP_F64)
+        return launch_pass<double>(h->p, L, InPlain<double>{(const double*)u}, OutStore<double>{(double*)Au, (double)sign},
+                                   nullptr, nullptr, h->st, nullptr);
+    if (dtype == SBP_F32)
+        return launch_pass<float>(h->p, L, InPlain<float>{(const float*)u}, OutStore<float>{(float*)Au, (float)sign},
+                                  nullptr, nullptr, h->st, nullptr);
+    return SBP_ERR_INVALID_ARG;
+}
+
+int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot) {
+    if (!h || !u || !Au || !dot) return SBP_ERR_INVALID_ARG;
+    int np = 0;
+    STCHK(launch_pass<double>(h->p, h->L[0], InPlain<double>{u}, OutStoreDot<double>{Au}, nullptr, h->partial, h->st, &np));
+    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(256), 0, h->st, h->partial, np, h->dtmp, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dot, h->dtmp, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SBP_OK;
+}
+
+int sbp_diag(sbp_handle h, double* d) {
+    if (!h || !d) return SBP_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(diag_kernel, dim3(ew_blocks(h->L[0].n)), dim3(256), 0, h->st, d, h->L[0].tb);
+    HIPCHK(hipGetLastError());
+    return SBP_OK;
+}
+
+int sbp_assemble_rhs(sbp_handle h, const double* f, const double* gW, const double* gE, const double* gS,
+                     const double* gN, double* b) {
+    if (!h || !b) return SBP_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(assemble_rhs, dim3(ew_blocks(h->L[0].n)), dim3(256), 0, h->st, f, gW, gE, gS, gN, b,
+                       h->L[0].tb, h->rt);
+    HIPCHK(hipGetLastError());
+    return SBP_OK;
+}
+
+int sbp_rhs_transpose(sbp_handle h, const double* lam, double* df, double* dgW, double* dgE, double* dgS,
+                      double* dgN) {
+    if (!h || !lam) return SBP_ERR_INVALID_ARG;
+    if (df) {
+        hipLaunchKernelGGL(rhs_t_f, dim3(ew_blocks(h->L[0].n)), dim3(256), 0, h->st, lam, df, h->L[0].tb, h->rt);
+        HIPCHK(hipGetLastError());
+    }
+    if (dgW || dgE || dgS || dgN) {
+        hipLaunchKernelGGL(rhs_t_g, dim3((h->N + 255) / 256), dim3(256), 0, h->st, lam, dgW, dgE, dgS, dgN, h->L[0].tb,
+                           h->rt);
+        HIPCHK(hipGetLastError());
+    }
+    return SBP_OK;
+}
+
+int sbp_hnorm(sbp_handle h, const double* v, double* out) {
+    if (!h || !v || !out) return SBP_ERR_INVALID_ARG;
+    const int nb = ew_blocks(h->L[0].n);
+    const double hh = 1.0 / (h->N - 1);
+    hipLaunchKernelGGL((hnorm_partial<256>), dim3(nb), dim3(256), 0, h->st, v, h->L[0].tb, hh * hh, h->partial);
+    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(256), 0, h->st, h->partial, nb, h->dtmp, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, h->dtmp, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SBP_OK;
+}
+
+int sbp_dot(sbp_handle h, const double* u, const double* v, double* out) {
+    if (!h || !u || !v || !out) return SBP_ERR_INVALID_ARG;
+    const int nb = ew_blocks(h->L[0].n);
+    hipLaunchKernelGGL((dot_partial<double, 256>), dim3(nb), dim3(256), 0, h->st, u, v, h->L[0].n, h->partial, nullptr);
+    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(256), 0, h->st, h->partial, nb, h->dtmp, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, h->dtmp, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SBP_OK;
+}
+
+int sbp_restrict(sbp_handle h, int level, const void* rf, void* rc, int dtype) {
+    if (!h || !rf || !rc || level < 0 || level + 1 >= h->nlev) return SBP_ERR_INVALID_ARG;
+    const int Nf = h->L[level].N, Nc = h->L[level + 1].N;
+    dim3 g((Nc + 255) / 256, Nc);
+    if (dtype == SBP_F64)
+        hipLaunchKernelGGL((restrict_fw<double>), g, dim3(256), 0, h->st, (const double*)rf, (double*)rc, Nf, Nc,
+                           h->prm.restrict_scale, nullptr);
+    else if (dtype == SBP_F32)
+        hipLaunchKernelGGL((restrict_fw<float>), g, dim3(256), 0, h->st, (const float*)rf, (float*)rc, Nf, Nc,
+                           (float)h->prm.restrict_scale, nullptr);
+    else return SBP_ERR_INVALID_ARG;
+    HIPCHK(hipGetLastError());
+    return SBP_OK;
+}
+
+int sbp_prolong_add(sbp_handle h, int level, const void* ec, void* xf, int dtype) {
+    if (!h || !ec || !xf || level < 0 || level + 1 >= h->nlev) return SBP_ERR_INVALID_ARG;
+    const int Nf = h->L[level].N, Nc = h->L[level + 1].N;
+    dim3 g((Nf + 255) / 256, Nf);
+    if (dtype == SBP_F64)
+        hipLaunchKernelGGL((prolong_add<double>), g, dim3(256), 0, h->st, (const double*)ec, (double*)xf, Nf, Nc, nullptr);
+    else if (dtype == SBP_F32)
+        hipLaunchKernelGGL((prolong_add<float>), g, dim3(256), 0, h->st, (const float*)ec, (float*)xf, Nf, Nc, nullptr);
+    else return SBP_ERR_INVALID_ARG;
+    HIPCHK(hipGetLastError());
+    return SBP_OK;
+}
+
+
+int sbp_smooth(sbp_handle h, int level, const void* b, void* x, int nsweeps, int dtype) {
+    if (!h || !b || !x || level < 0 || level >= h->nlev || nsweeps < 0) return SBP_ERR_INVALID_ARG;
+    if (dtype == SBP_F64) return smooth_t<double>(h, level, (const double*)b, (double*)x, nsweeps);
+    if (dtype == SBP_F32) {
+        STCHK(ensure32(h));
+        return smooth_t<float>(h, level, (const float*)b, (float*)x, nsweeps);
+    }
+    return SBP_ERR_INVALID_ARG;
+}
+
+int sbp_coarse_solve(sbp_handle h, const double* b, double* x) {
+    if (!h || !b || !x || !h->mg) return SBP_ERR_INVALID_ARG;
+    hipLaunchKernelGGL((coarse_solve<double>), dim3(1), dim3(320), 0, h->st, h->Ainv, b, x, h->nc, nullptr);
+    HIPCHK(hipGetLastError());
+    return SBP_OK;
+}
+
+int sbp_vcycle(sbp_handle h, const void* r, void* z, int dtype) {
+    if (!h || !r || !z || !h->mg) return SBP_ERR_INVALID_ARG;
+    if (dtype == SBP_F64) return vcycle_level<double>(h, 0, (const double*)r, (double*)z, nullptr, nullptr, nullptr);
+    if (dtype == SBP_F32) {
+        STCHK(ensure32(h));
+        return vcycle_level<float>(h, 0, (const float*)r, (float*)z, nullptr, nullptr, nullptr);
+    }
+    return SBP_ERR_INVALID_ARG;
+}
+
+
+int sbp_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
+           double* hist) {
+    if (!h || !b || !x || maxiter < 0 || !(abstol >= 0)) return SBP_ERR_INVALID_ARG;
+    STCHK(init_residual(h, b, x, abstol, hist));
+    int it = 0;
+    int rc = pcg_loop<double>(h, x, h->r, h->pv, h->Ap, h->z, h->sc, maxiter, false, hist, &it);
+    if (iters) *iters = it;
+    if (resnorm) *resnorm = h->sc_host->resnorm;
+    return rc;
+}
+
+int sbp_mgpcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
+              double* hist) {
+    if (!h || !b || !x || maxiter < 0 || !(abstol >= 0)) return SBP_ERR_INVALID_ARG;
+    if (!h->mg) return SBP_ERR_INVALID_SIZE;
+    STCHK(init_residual(h, b, x, abstol, hist));
+    int it = 0;
+    int rc = pcg_loop<double>(h, x, h->r, h->pv, h->Ap, h->z, h->sc, maxiter, true, hist, &it);
+    if (iters) *iters = it;
+    if (resnorm) *resnorm = h->sc_host->resnorm;
+    return rc;
+}
+
+}  // extern "C"
+
+namespace {
+// Outer refinement check: resnorm = ||r||; converged -> both done; else arm
+// the inner solver with abstol = inner_rtol * ||r||.
+__global__ __launch_bounds__(256) void ir_check(const double* partial, int np, Scalars* so, Scalars* si,
+                                               double inner_rtol, int outer) {
+    double rs = sum_partials(partial, np);
+    if (threadIdx.x == 0) {
+        const double nr = sqrt(rs);
+        so->resnorm = nr;
+        so->rr = rs;
+        si->iters = 0;
+        si->status = 0;
+        si->alpha = si->beta = 0.0;
+        si->rr = rs;
+        si->resnorm = nr;
+        if (!isfinite(rs)) { so->status = 5; so->done = 1; si->done = 1; return; }
+        if (nr < so->abstol) { so->done = 1; si->done = 1; return; }
+        so->iters = outer + 1;
+        si->abstol = inner_rtol * nr;
+        si->done = 0;
+    }
+}
+
+__global__ void ir_init(Scalars* so, double abstol) {
+    so->abstol = abstol;
+    so->done = 0;
+    so->status = 0;
+    so->iters = 0;
+}
+}  // namespace
+
+extern "C" int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double abstol, int max_outer,
+                               int* outer_iters, int* inner_iters, double* resnorm) {
+    if (!h || !b || !x || max_outer < 0 || !(abstol >= 0)) return SBP_ERR_INVALID_ARG;
+    if (!h->mg) return SBP_ERR_INVALID_SIZE;
+    STCHK(ensure32(h));
+    hipStream_t st = h->st;
+    const size_t n = h->L[0].n;
+    hipLaunchKernelGGL(ir_init, dim3(1), dim3(1), 0, st, h->sc, abstol);
+    HIPCHK(hipGetLastError());
+    int total_inner = 0;
+    int rc = SBP_OK;
+    for (int o = 0; o <= max_outer; ++o) {
+        int np = 0;
+        // FP64 residual, cast to FP32 for the inner solve.
+        STCHK(launch_pass<double>(h->p, h->L[0], InPlain<double>{x}, OutResid<double, float>{b, h->r32}, &h->sc->done,
+                                  h->partial, st, &np));
+        hipLaunchKernelGGL(ir_check, dim3(1), dim3(256), 0, st, h->partial, np, h->sc, h->sc_in, h->prm.inner_rtol, o);
+        HIPCHK(hipGetLastError());
+        STCHK(poll(h, h->sc));
+        if (h->sc_host->done || o == max_outer) break;
+        hipLaunchKernelGGL((fill<float, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, h->d32, 0.0f, n,
+                           &h->sc_in->done);
+        HIPCHK(hipGetLastError());
+        int it = 0;
+        int irc = pcg_loop<float>(h, h->d32, h->r32, h->p32, h->Ap32, h->z32, h->sc_in, h->prm.inner_maxiter, true,
+                                  nullptr, &it);
+        if (irc == SBP_ERR_NONFINITE || irc == SBP_ERR_BREAKDOWN) { rc = irc; break; }
+        total_inner += it;
+        hipLaunchKernelGGL((add_f32_to_f64<kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, x,
+                           (const float*)h->d32, n, nullptr);
+        HIPCHK(hipGetLastError());
+    }
+    STCHK(poll(h, h->sc));
+    if (outer_iters) *outer_iters = h->sc_host->iters;
+    if (inner_iters) *inner_iters = total_inner;
+    if (resnorm) *resnorm = h->sc_host->resnorm;
+    if (rc != SBP_OK) return rc;
+    if (h->sc_host->status == 5) return SBP_ERR_NONFINITE;
+    return h->sc_host->done ? SBP_OK : SBP_ERR_NOT_CONVERGED;
+}
+
+// ---------------------------------------------------------------------------
+// CSR comparison path
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void exclusive_scan_serial(const int32_t* counts, int32_t* rp, int n, int64_t* total) {
+    // One thread; used once per assembly.
+    int64_t s = 0;
+    for (int q = 0; q < n; ++q) { rp[q] = (int32_t)s; s += counts[q]; }
+    rp[n] = (int32_t)s;
+    *total = s;
+}
+}  // namespace
+
+extern "C" int sbp_csr_assemble(sbp_handle h, int32_t* row_ptr, int32_t* col, double* val, int64_t* nnz) {
+    if (!h || !nnz) return SBP_ERR_INVALID_ARG;
+    const int n = (int)h->L[0].n;
+    int32_t* counts = nullptr;
+    int32_t* rp = row_ptr;
+    int64_t* tot = nullptr;
+    HIPCHK(hipMalloc(&counts, (size_t)n * sizeof(int32_t)));
+    HIPCHK(hipMalloc(&tot, sizeof(int64_t)));
+    if (!rp) HIPCHK(hipMalloc(&rp, (size_t)(n + 1) * sizeof(int32_t)));
+    const int nb = (n + 255) / 256;
+    hipLaunchKernelGGL(csr_fill, dim3(nb), dim3(256), 0, h->st, rp, col, val, h->L[0].tb, 1, counts);
+    hipLaunchKernelGGL(exclusive_scan_serial, dim3(1), dim3(1), 0, h->st, counts, rp, n, tot);
+    if (col && val && row_ptr)
+        hipLaunchKernelGGL(csr_fill, dim3(nb), dim3(256), 0, h->st, rp, col, val, h->L[0].tb, 0, counts);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(nnz, tot, sizeof(int64_t), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    (void)hipFree(counts);
+    (void)hipFree(tot);
+    if (!row_ptr) (void)hipFree(rp);
+    return SBP_OK;
+}
+
+extern "C" int sbp_csr_spmv(sbp_handle h, const int32_t* rp, const int32_t* col, const double* val, const double* x,
+                            double* y) {
+    if (!h || !rp || !col || !val || !x || !y) return SBP_ERR_INVALID_ARG;
+    const int n = (int)h->L[0].n;
+    hipLaunchKernelGGL(csr_spmv, dim3((n + 255) / 256), dim3(256), 0, h->st, rp, col, val, x, y, n, nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+    return SBP_OK;
+}
+
+extern "C" int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, const double* val, const double* b,
+                          double* x, double abstol, int maxiter, int* iters, double* resnorm) {
+    if (!h || !rp || !col || !val || !b || !x || maxiter < 0) return SBP_ERR_INVALID_ARG;
+    hipStream_t st = h->st;
+    const int n = (int)h->L[0].n;
+    const int nbs = (n + 255) / 256;
+    if ((size_t)nbs > h->npartial) return SBP_ERR_INVALID_SIZE;
+    Scalars* sc = h->sc;
+    // r = b - A x with block partials of r'r
+    hipLaunchKernelGGL(csr_resid, dim3(nbs), dim3(256), 0, st, rp, col, val, x, b, h->r, n, h->partial);
+    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(256), 0, st, h->partial, nbs, sc, abstol, nullptr);
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st,
+                       (const double*)h->r, h->pv, (size_t)n, &sc->done);
+    HIPCHK(hipGetLastError());
+    const int poll_every = std::max(1, h->prm.poll_every);
+    for (int it = 0; it < maxiter; ++it) {
+        hipLaunchKernelGGL(csr_spmv, dim3(nbs), dim3(256), 0, st, rp, col, val, (const double*)h->pv, h->Ap, n,
+                           h->partial, &sc->done);
+        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(256), 0, st, h->partial, nbs, sc, 0);
+        const int eb = ew_blocks(n);
+        hipLaunchKernelGGL((update_xr<double, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, x, h->r,
+                           (const double*)h->pv, (const double*)h->Ap, (const Scalars*)sc, (size_t)n, h->partial);
+        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(256), 0, st, h->partial, eb, sc, it, nullptr, 1);
+        hipLaunchKernelGGL((update_p<double, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, h->pv,
+                           (const double*)h->r, (const Scalars*)sc, (size_t)n);
+        HIPCHK(hipGetLastError());
+        if ((it + 1) % poll_every == 0) {
+            STCHK(poll(h, sc));
+            if (h->sc_host->done) break;
+        }
+    }
+    STCHK(poll(h, sc));
+    if (iters) *iters = h->sc_host->iters;
+    if (resnorm) *resnorm = h->sc_host->resnorm;
+    return finish_status(*h->sc_host, !h->sc_host->done);
+}
+
+namespace {
+__global__ __launch_bounds__(256) void copy_bytes(const uint4* __restrict__ s, uint4* __restrict__ d, size_t n16) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n16; q += stride) d[q] = s[q];
+}
+}  // namespace
+
+extern "C" int sbp_copy(sbp_handle h, const void* src, void* dst, size_t bytes) {
+    if (!h || !src || !dst || (bytes % 16) != 0) return SBP_ERR_INVALID_ARG;
+    const size_t n16 = bytes / 16;
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>((n16 + 255) / 256, 8192));
+    hipLaunchKernelGGL(copy_bytes, dim3(nb), dim3(256), 0, h->st, (const uint4*)src, (uint4*)dst, n16);
+    HIPCHK(hipGetLastError());
+    return SBP_OK;
+}

@@ -1,0 +1,432 @@
+// sbp_kernels.h -- gfx950 kernels of the Krylov loops, transfers, coarse
+// solve, right-hand side and CSR comparison path (the stencil pass itself is
+// in sbp_stencil.h).  Included by exactly one translation unit.
+#pragma once
+#include "sbp_stencil.h"
+
+namespace sbp {
+
+// ---------------------------------------------------------------------------
+// Transfers
+// ---------------------------------------------------------------------------
+// rc(I,J) = scale * sum_{a,b in -1..1} w_a w_b rf(2I+a, 2J+b) / 4, w_0 = 1, w_+-1 = 1/2,
+// fine points outside the grid count as zero.
+template <typename T>
+__global__ __launch_bounds__(256) void restrict_fw(const T* __restrict__ rf, T* __restrict__ rc, int Nf, int Nc,
+                                                  T scale, const int* done) {
+    if (done && *done) return;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Jc = blockIdx.y;
+    if (I >= Nc) return;
+    const int fi = 2 * I, fj = 2 * Jc;
+    T s = T(0);
+#pragma unroll
+    for (int b = -1; b <= 1; ++b) {
+        const int jj = fj + b;
+        if (jj < 0 || jj >= Nf) continue;
+        const T wb = b == 0 ? T(1) : T(0.5);
+        const T* row = rf + (size_t)jj * Nf;
+        T rs = row[fi];
+        if (fi - 1 >= 0) rs += T(0.5) * row[fi - 1];
+        if (fi + 1 < Nf) rs += T(0.5) * row[fi + 1];
+        s += wb * rs;
+    }
+    rc[(size_t)Jc * Nc + I] = scale * T(0.25) * s;
+}
+
+// xf += P ec
+template <typename T>
+__global__ __launch_bounds__(256) void prolong_add(const T* __restrict__ ec, T* __restrict__ xf, int Nf, int Nc,
+                                                  const int* done) {
+    if (done && *done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= Nf) return;
+    const int ic = i >> 1, jc = j >> 1;
+    const T* e = ec + (size_t)jc * Nc + ic;
+    T v;
+    if (!(i & 1)) v = (!(j & 1)) ? e[0] : (e[0] + e[Nc]) * T(0.5);
+    else v = (!(j & 1)) ? (e[0] + e[1]) * T(0.5) : (e[0] + e[1] + e[Nc] + e[Nc + 1]) * T(0.25);
+    xf[(size_t)j * Nf + i] += v;
+}
+
+// Coarsest-level exact solve x = Ainv b with the FP64 inverse (symmetric,
+// stored n x n).  One workgroup; b is staged in LDS.
+template <typename T>
+__global__ __launch_bounds__(1024) void coarse_solve(const double* __restrict__ Ainv, const T* __restrict__ b,
+                                                    T* __restrict__ x, int n, const int* done) {
+    if (done && *done) return;
+    __shared__ double bs[1089];   // up to 33 x 33
+    for (int k = threadIdx.x; k < n; k += blockDim.x) bs[k] = (double)b[k];
+    __syncthreads();
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) s += Ainv[(size_t)k * n + r] * bs[k];
+        x[r] = (T)s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise / reduction kernels for the Krylov loops
+// ---------------------------------------------------------------------------
+struct Scalars {           // device-resident solver state
+    double rr;             // r'r (CG) or latest ||r||^2 (PCG)
+    double rz;             // r'z (PCG)
+    double pAp;
+    double alpha, beta;
+    double abstol;
+    double resnorm;        // latest ||r||_2
+    double bnorm;
+    int iters;
+    int done;
+    int status;
+    int pad;
+};
+
+// Deterministic single-workgroup sum of the block partials.
+__device__ __forceinline__ double sum_partials(const double* partial, int np) {
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < np; k += blockDim.x) s += partial[k];
+    s = wave_sum(s);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = s;
+    __syncthreads();
+    double tot = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)((blockDim.x + 63) / 64); ++k) tot += red[k];
+    return tot;   // valid on thread 0
+}
+
+// Stage 2 of a plain dot product: out = sum.
+__global__ __launch_bounds__(256) void reduce_to(const double* partial, int np, double* out, int take_sqrt) {
+    double s = sum_partials(partial, np);
+    if (threadIdx.x == 0) *out = take_sqrt ? sqrt(s) : s;
+}
+
+// alpha = num / pAp where num = rr (CG, use_rz = 0) or rz (PCG, use_rz = 1).
+__global__ __launch_bounds__(256) void krylov_alpha(const double* partial, int np, Scalars* sc, int use_rz) {
+    if (sc->done) return;
+    double pAp = sum_partials(partial, np);
+    if (threadIdx.x == 0) {
+        sc->pAp = pAp;
+        if (!(pAp > 0.0) || !isfinite(pAp)) {
+            sc->status = isfinite(pAp) ? 6 : 5;
+            sc->done = 1;
+            return;
+        }
+        sc->alpha = (use_rz ? sc->rz : sc->rr) / pAp;
+    }
+}
+
+// rs = ||r||^2; record history; converge test ||r|| < abstol; for CG also beta.
+__global__ __launch_bounds__(256) void krylov_check(const double* partial, int np, Scalars* sc, int it,
+                                                   double* hist, int cg_beta) {
+    if (sc->done) return;
+    double rs = sum_partials(partial, np);
+    if (threadIdx.x == 0) {
+        const double nr = sqrt(rs);
+        sc->resnorm = nr;
+        if (hist) hist[it + 1] = nr;
+        sc->iters = it + 1;
+        if (!isfinite(rs)) { sc->status = 5; sc->done = 1; return; }
+        if (nr < sc->abstol) { sc->done = 1; return; }
+        if (cg_beta) { sc->beta = rs / sc->rr; }
+        sc->rr = rs;
+    }
+}
+
+// rz_new = r'z; beta = rz_new / rz.
+__global__ __launch_bounds__(256) void krylov_beta(const double* partial, int np, Scalars* sc) {
+    if (sc->done) return;
+    double rz = sum_partials(partial, np);
+    if (threadIdx.x == 0) {
+        if (!isfinite(rz)) { sc->status = 5; sc->done = 1; return; }
+        sc->beta = rz / sc->rz;
+        sc->rz = rz;
+    }
+}
+
+// Initial state: rr = sum partial; rz optional second set; test ||r0|| < abstol.
+__global__ __launch_bounds__(256) void krylov_init(const double* partial, int np, Scalars* sc, double abstol,
+                                                  double* hist) {
+    double rs = sum_partials(partial, np);
+    if (threadIdx.x == 0) {
+        sc->abstol = abstol;
+        sc->rr = rs;
+        sc->resnorm = sqrt(rs);
+        sc->iters = 0;
+        sc->status = 0;
+        sc->done = 0;
+        sc->alpha = sc->beta = 0.0;
+        if (hist) hist[0] = sqrt(rs);
+        if (!isfinite(rs)) { sc->status = 5; sc->done = 1; return; }
+        if (sqrt(rs) < abstol) sc->done = 1;
+    }
+}
+
+// rz0 = r'z (stage 2) without touching done.
+__global__ __launch_bounds__(256) void krylov_set_rz(const double* partial, int np, Scalars* sc) {
+    if (sc->done) return;
+    double rz = sum_partials(partial, np);
+    if (threadIdx.x == 0) {
+        if (!isfinite(rz)) { sc->status = 5; sc->done = 1; return; }
+        sc->rz = rz;
+    }
+}
+
+// x += alpha p ; r -= alpha Ap ; partial = block sums of r'r.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void update_xr(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                                               const T* __restrict__ Ap, const Scalars* sc, size_t n,
+                                               double* partial) {
+    if (sc->done) return;
+    const T a = (T)sc->alpha;
+    T acc = T(0);
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) {
+        x[q] += a * p[q];
+        T rv = r[q] - a * Ap[q];
+        r[q] = rv;
+        acc += rv * rv;
+    }
+    T s = block_sum<T, NT>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = (double)s;
+}
+
+// p = z + beta p
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void update_p(T* __restrict__ p, const T* __restrict__ z, const Scalars* sc,
+                                              size_t n) {
+    if (sc->done) return;
+    const T bt = (T)sc->beta;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) p[q] = z[q] + bt * p[q];
+}
+
+// partial = block sums of u'v (v may equal u).
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void dot_partial(const T* __restrict__ u, const T* __restrict__ v, size_t n,
+                                                 double* partial, const double* __restrict__ wt) {
+    T acc = T(0);
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) acc += u[q] * v[q];
+    T s = block_sum<T, NT>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = (double)s;
+}
+
+// partial = block sums of v' H~ v with H~(i,j) = h^2 hw[i] hw[j].
+template <int NT>
+__global__ __launch_bounds__(NT) void hnorm_partial(const double* __restrict__ v, OpTab t, double h2,
+                                                   double* partial) {
+    const size_t n = (size_t)t.N * t.N;
+    double acc = 0.0;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) {
+        const int i = (int)(q % t.N), j = (int)(q / t.N);
+        acc += hw_of<double>(t, i) * hw_of<double>(t, j) * v[q] * v[q];
+    }
+    double s = block_sum<double, NT>(acc) * h2;
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Type conversion y = (TO) x.
+template <typename TI_, typename TO, int NT>
+__global__ __launch_bounds__(NT) void convert(const TI_* __restrict__ x, TO* __restrict__ y, size_t n) {
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) y[q] = (TO)x[q];
+}
+
+// x64 += (double) d32
+template <int NT>
+__global__ __launch_bounds__(NT) void add_f32_to_f64(double* __restrict__ x, const float* __restrict__ d, size_t n,
+                                                    const int* done) {
+    if (done && *done) return;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) x[q] += (double)d[q];
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void fill(T* __restrict__ x, T v, size_t n, const int* done) {
+    if (done && *done) return;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) x[q] = v;
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void copy_vec(const T* __restrict__ x, T* __restrict__ y, size_t n, const int* done) {
+    if (done && *done) return;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) y[q] = x[q];
+}
+
+// diag(A)
+__global__ __launch_bounds__(256) void diag_kernel(double* d, OpTab t) {
+    const size_t n = (size_t)t.N * t.N;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += stride) {
+        const int i = (int)(q % t.N), j = (int)(q / t.N);
+        d[q] = diag_of<double>(t, i, j);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Right-hand side assembly and its transpose (closed form of
+// b = -H~ (f + SAT_W_r gW + SAT_E_r gE + SAT_S_r gS + SAT_N_r gN)):
+//   b(i,j) = -h^2 hw_i hw_j f(i,j)
+//          + [j < nbs]        hw_i (13 d_{j,0} - bs_j) gW(i)
+//          + [N-1-j < nbs]    hw_i (13 d_{N-1-j,0} - bs_{N-1-j}) gE(i)
+//          + [i == 0]         h hw_j gS(j)
+//          + [i == N-1]       h hw_j gN(j)
+// ---------------------------------------------------------------------------
+struct RhsTab {
+    double bs[kMaxR];   // boundary-derivative row (dimensionless, h * BS)
+    int nbs;
+    double h;
+};
+
+__device__ __forceinline__ double dir_w(const RhsTab& r, int k) {
+    if (k >= r.nbs) return 0.0;
+    return (k == 0 ? 13.0 : 0.0) - r.bs[k];
+}
+
+__global__ __launch_bounds__(256) void assemble_rhs(const double* f, const double* gW, const double* gE,
+                                                   const double* gS, const double* gN, double* b, OpTab t,
+                                                   RhsTab rt) {
+    const int N = t.N;
+    const size_t n = (size_t)N * N;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += stride) {
+        const int i = (int)(q % N), j = (int)(q / N);
+        const double hi = hw_of<double>(t, i), hj = hw_of<double>(t, j);
+        double v = f ? -rt.h * rt.h * hi * hj * f[q] : 0.0;
+        if (gW && j < rt.nbs) v += hi * dir_w(rt, j) * gW[i];
+        if (gE && N - 1 - j < rt.nbs) v += hi * dir_w(rt, N - 1 - j) * gE[i];
+        if (gS && i == 0) v += rt.h * hj * gS[j];
+        if (gN && i == N - 1) v += rt.h * hj * gN[j];
+        b[q] = v;
+    }
+}
+
+// df = -h^2 hw_i hw_j lam
+__global__ __launch_bounds__(256) void rhs_t_f(const double* lam, double* df, OpTab t, RhsTab rt) {
+    const int N = t.N;
+    const size_t n = (size_t)N * N;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += stride) {
+        const int i = (int)(q % N), j = (int)(q / N);
+        df[q] = -rt.h * rt.h * hw_of<double>(t, i) * hw_of<double>(t, j) * lam[q];
+    }
+}
+
+// dgW(i) = hw_i sum_k dir_w(k) lam(i,k); dgE mirrored; dgS(j) = h hw_j lam(0,j); dgN(j) = h hw_j lam(N-1,j)
+__global__ __launch_bounds__(256) void rhs_t_g(const double* lam, double* dgW, double* dgE, double* dgS,
+                                              double* dgN, OpTab t, RhsTab rt) {
+    const int N = t.N;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= N) return;
+    const double hq = hw_of<double>(t, q);
+    if (dgW) {
+        double s = 0.0;
+        for (int k = 0; k < rt.nbs; ++k) s += dir_w(rt, k) * lam[gidx(q, k, N)];
+        dgW[q] = hq * s;
+    }
+    if (dgE) {
+        double s = 0.0;
+        for (int k = 0; k < rt.nbs; ++k) s += dir_w(rt, k) * lam[gidx(q, N - 1 - k, N)];
+        dgE[q] = hq * s;
+    }
+    if (dgS) dgS[q] = rt.h * hq * lam[gidx(0, q, N)];
+    if (dgN) dgN[q] = rt.h * hq * lam[gidx(N - 1, q, N)];
+}
+
+// ---------------------------------------------------------------------------
+// CSR comparison path
+// ---------------------------------------------------------------------------
+// One row per thread over rows of the assembled operator (nnz/row <= 11).
+__global__ __launch_bounds__(256) void csr_spmv(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                               const double* __restrict__ val, const double* __restrict__ x,
+                                               double* __restrict__ y, int nrows, double* partial, const int* done) {
+    if (done && *done) return;
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (r < nrows) {
+        double s = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) s += val[k] * x[col[k]];
+        y[r] = s;
+        acc = x[r] * s;
+    }
+    if (partial) {
+        double bsum = block_sum<double, 256>(acc);
+        if (threadIdx.x == 0) partial[blockIdx.x] = bsum;
+    }
+}
+
+// r = b - A x with block partials of r'r.
+__global__ __launch_bounds__(256) void csr_resid(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                const double* __restrict__ val, const double* __restrict__ x,
+                                                const double* __restrict__ b, double* __restrict__ r, int nrows,
+                                                double* partial) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (q < nrows) {
+        double s = 0.0;
+        for (int k = rp[q]; k < rp[q + 1]; ++k) s += val[k] * x[col[k]];
+        const double rv = b[q] - s;
+        r[q] = rv;
+        acc = rv * rv;
+    }
+    double bsum = block_sum<double, 256>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bsum;
+}
+
+// Fills the CSR arrays of A row by row from the tensor form (rows in natural order).
+__global__ __launch_bounds__(256) void csr_fill(int32_t* rp, int32_t* col, double* val, OpTab t, int count_only,
+                                               int32_t* counts) {
+    const int N = t.N;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= N * N) return;
+    const int i = q % N, j = q / N;
+    // Collect (column, value) pairs for row (i,j): k-part along j, m-part along i.
+    int cj[kMaxC + 5]; double vj[kMaxC + 5]; int nj = 0;
+    int ci[kMaxC + 5]; double vi[kMaxC + 5]; int ni = 0;
+    auto row1d = [&](int r, int* cc, double* vv, int& nn, bool is_k) {
+        if (r < t.R) {
+            for (int c = 0; c < t.C; ++c) { double v = is_k ? t.kb[r][c] : t.mb[r][c]; if (v != 0.0) { cc[nn] = c; vv[nn++] = v; } }
+        } else if (r >= N - t.R) {
+            const int rr = N - 1 - r;
+            for (int c = t.C - 1; c >= 0; --c) { double v = is_k ? t.kb[rr][c] : t.mb[rr][c]; if (v != 0.0) { cc[nn] = N - 1 - c; vv[nn++] = v; } }
+        } else {
+            const int H = (t.C == 6) ? 2 : 1;
+            for (int s = -H; s <= H; ++s) {
+                double v = (s == 0) ? t.mid : (H == 1 ? -1.0 : ((s == 1 || s == -1) ? -4.0 / 3.0 : 1.0 / 12.0));
+                cc[nn] = r + s; vv[nn++] = v;
+            }
+        }
+    };
+    row1d(j, cj, vj, nj, true);
+    row1d(i, ci, vi, ni, false);
+    const double hwi = hw_of<double>(t, i), hwj = hw_of<double>(t, j);
+    // Merge into sorted global columns: entries (i, cj[k]) -> col = i + cj*N; (ci[k], j) -> ci + j*N.
+    int cnt = 0;
+    int outc[2 * (kMaxC + 5)]; double outv[2 * (kMaxC + 5)];
+    // k-part columns with cj < j come first (they are < j*N), then row j block, then cj > j.
+    for (int a = 0; a < nj; ++a) {
+        if (cj[a] < j) { outc[cnt] = i + cj[a] * N; outv[cnt++] = hwi * vj[a]; }
+    }
+    double diagk = 0.0;
+    for (int a = 0; a < nj; ++a) if (cj[a] == j) diagk = hwi * vj[a];
+    for (int a = 0; a < ni; ++a) {
+        double v = hwj * vi[a];
+        if (ci[a] == i) v += diagk;
+        outc[cnt] = ci[a] + j * N; outv[cnt++] = v;
+    }
+    for (int a = 0; a < nj; ++a) {
+        if (cj[a] > j) { outc[cnt] = i + cj[a] * N; outv[cnt++] = hwi * vj[a]; }
+    }
+    if (count_only) { counts[q] = cnt; return; }
+    const int base = rp[q];
+    for (int a = 0; a < cnt; ++a) { col[base + a] = outc[a]; val[base + a] = outv[a]; }
+}
+
+}  // namespace sbp

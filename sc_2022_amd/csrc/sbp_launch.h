@@ -1,0 +1,44 @@
+// sbp_launch.h -- launch entry points of the stencil pass.  Each compiled
+// tile shape (TI x J) lives in its own translation unit (sbp_tile.hip built
+// once per shape) so that the shapes compile in parallel; the API dispatches
+// on the shape chosen for a level.
+#pragma once
+#include "sbp_stencil.h"
+
+namespace sbp {
+
+using ResidDD = OutResid<double, double>;
+using ResidDF = OutResid<double, float>;
+using ResidFF = OutResid<float, float>;
+
+// Every (element type, input policy, output policy) the solvers use.
+#define SBP_STENCIL_COMBOS(X)                     \
+    X(double, InPlain<double>, OutStore<double>)    \
+    X(double, InPlain<double>, OutStoreDot<double>) \
+    X(double, InAxpy<double>, OutWriteDot<double>)  \
+    X(double, InPlain<double>, ResidDD)             \
+    X(double, InPlain<double>, ResidDF)             \
+    X(double, InSmooth0<double>, OutSmooth<double>) \
+    X(double, InPlain<double>, OutSmooth<double>)   \
+    X(double, InProlong<double>, OutSmooth<double>) \
+    X(double, InSmooth0<double>, OutCopy<double>)   \
+    X(float, InPlain<float>, OutStore<float>)       \
+    X(float, InPlain<float>, OutStoreDot<float>)    \
+    X(float, InAxpy<float>, OutWriteDot<float>)     \
+    X(float, InPlain<float>, ResidFF)               \
+    X(float, InSmooth0<float>, OutSmooth<float>)    \
+    X(float, InPlain<float>, OutSmooth<float>)      \
+    X(float, InProlong<float>, OutSmooth<float>)    \
+    X(float, InSmooth0<float>, OutCopy<float>)
+
+// Compiled tile shapes (TI threads along i, J rows per thread).
+#define SBP_TILE_SHAPES(X) X(64, 4) X(64, 8) X(128, 8) X(256, 4) X(256, 8) X(256, 16)
+
+inline int nparts_of(int N, int ti, int tj) { return ((N + ti - 1) / ti) * ((N + tj - 1) / tj); }
+
+// Launches stencil_pass<p, TI, J, T, IN, OUT> over the whole N x N grid of tb.
+template <int TI, int J, typename T, class IN, class OUT>
+int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, const int* done, double* partial,
+                hipStream_t st);
+
+}  // namespace sbp

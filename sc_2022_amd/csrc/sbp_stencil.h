@@ -1,0 +1,349 @@
+// sbp_stencil.h -- gfx950 stencil pass for the matrix-free SBP-SAT Poisson path.
+//
+// The operator (SPD A = -H~(D2 + SAT), preconditioned_CG/level_2_multigrid_new.jl:141-185)
+// is applied through its tensor form
+//     (A u)(i,j) = hw[i] * sum_j' k[j][j'] u(i,j') + hw[j] * sum_i' m[i][i'] u(i',j)
+// with dimensionless 1-D factors built on the host from the reference's SBP
+// tables (see sbp_tables.h).  Away from the faces both factors are the
+// interior stencil -d (p=2: [-1 2 -1], p=4: [1/12 -4/3 5/2 -4/3 1/12]) and
+// hw = 1, so the interior is a (2p+1)-point cross; near the faces the first
+// R rows of k and m are a dense R x C closure block (mirrored at the far face).
+//
+// Kernel shape (the hot path): a workgroup owns a tile of TI consecutive i
+// (fast, coalesced) by J consecutive j.  Each thread loads its own column
+// u(i, j0-H .. j0+J-1+H) into registers up front (all loads independent, so
+// a wave keeps J+2H row loads in flight), stores the J centre values in an
+// LDS tile (plus H halo columns each side), waits once, and then forms the
+// j-direction stencil from registers and the i-direction stencil from LDS.
+// HBM sees each element once; the H-row halo re-reads of j-adjacent tiles are
+// served by the XCD's L2 because the block index is remapped so that each XCD
+// walks a contiguous band of tiles (dispatch is round-robin over the 8 XCDs).
+//
+// What the pass does with (u, Au) is a policy: IN says how u(i,j) is formed
+// (plain load, Jacobi-from-zero, x + P e_c, r + beta p), OUT what is written
+// (sign * Au, residual, Jacobi update, fused inner products).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sbp {
+
+constexpr int kMaxR = 4;   // closure rows
+constexpr int kMaxC = 6;   // closure columns
+
+struct OpTab {
+    int N;      // points per side
+    int R;      // closure rows of k and m (p=2: 3, p=4: 4)
+    int C;      // closure columns (p=2: 4, p=4: 6)
+    int RH;     // number of non-unit H weights (p=2: 1, p=4: 4)
+    double hw[kMaxR];
+    double mb[kMaxR][kMaxC];
+    double kb[kMaxR][kMaxC];
+    double md[kMaxR];   // mb[r][r]
+    double kd[kMaxR];   // kb[r][r]
+    double mid;         // interior diagonal of the 1-D factor (2 or 5/2)
+};
+
+// Interior 1-D stencil s[-H..H] of -d.
+template <int P> struct Stencil;
+template <> struct Stencil<2> {
+    static constexpr int H = 1;
+    template <typename T> __device__ static T c(int t) { return t == 0 ? T(2) : T(-1); }
+};
+template <> struct Stencil<4> {
+    static constexpr int H = 2;
+    template <typename T> __device__ static T c(int t) {
+        return t == 0 ? T(2.5) : ((t == 1 || t == -1) ? T(-4.0 / 3.0) : T(1.0 / 12.0));
+    }
+};
+
+__device__ __forceinline__ size_t gidx(int i, int j, int N) { return (size_t)j * (size_t)N + (size_t)i; }
+
+template <typename T>
+__device__ __forceinline__ T hw_of(const OpTab& t, int r) {
+    if (r < t.RH) return (T)t.hw[r];
+    if (r >= t.N - t.RH) return (T)t.hw[t.N - 1 - r];
+    return T(1);
+}
+
+template <typename T>
+__device__ __forceinline__ T diag_of(const OpTab& t, int i, int j) {
+    T kj = (j < t.R) ? (T)t.kd[j] : (j >= t.N - t.R ? (T)t.kd[t.N - 1 - j] : (T)t.mid);
+    T mi = (i < t.R) ? (T)t.md[i] : (i >= t.N - t.R ? (T)t.md[t.N - 1 - i] : (T)t.mid);
+    return hw_of<T>(t, i) * kj + hw_of<T>(t, j) * mi;
+}
+
+// Rows 0..N-1 are remapped so that the 8 XCDs (round-robin dispatch) each get
+// a contiguous run of logical blocks.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int full = nb & ~7;
+    if (b >= full) return b;
+    const int per = full >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Input policies: value of the operand at (i, j)
+// ---------------------------------------------------------------------------
+template <typename T> struct InPlain {
+    const T* u;
+    __device__ void init() {}
+    __device__ __forceinline__ T load(const OpTab& t, int i, int j) const { return u[gidx(i, j, t.N)]; }
+};
+
+// First smoothing sweep from x = 0: x1 = w D^-1 b (damped Jacobi) or
+// x1 = w b (Richardson, use_diag = false).
+template <typename T> struct InSmooth0 {
+    const T* b; T omega; int use_diag;
+    __device__ void init() {}
+    __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
+        const T v = omega * b[gidx(i, j, t.N)];
+        return use_diag ? v / diag_of<T>(t, i, j) : v;
+    }
+};
+
+// x + P e_c with P the bilinear prolongation from the coarse side Nc = (N+1)/2.
+template <typename T> struct InProlong {
+    const T* x; const T* ec; int Nc;
+    __device__ void init() {}
+    __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
+        const int ic = i >> 1, jc = j >> 1;
+        const T* e = ec + gidx(ic, jc, Nc);
+        T v;
+        if (!(i & 1)) {
+            v = (!(j & 1)) ? e[0] : (e[0] + e[Nc]) * T(0.5);
+        } else {
+            v = (!(j & 1)) ? (e[0] + e[1]) * T(0.5) : (e[0] + e[1] + e[Nc] + e[Nc + 1]) * T(0.25);
+        }
+        return x[gidx(i, j, t.N)] + v;
+    }
+};
+
+// r + beta * p with beta read from the device scalar block.
+template <typename T> struct InAxpy {
+    const T* r; const T* p; const double* beta; T bt;
+    __device__ void init() { bt = (T)(*beta); }
+    __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
+        const size_t q = gidx(i, j, t.N);
+        return r[q] + bt * p[q];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Output policies
+// ---------------------------------------------------------------------------
+// acc_t: accumulator type of fused inner products (the element type).
+template <typename T> struct OutStore {            // o = s * Au
+    T* o; T s;
+    static constexpr bool kDot = false;
+    using Aux = T;
+    __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T, T Au, Aux, T&) const { o[q] = s * Au; }
+};
+
+template <typename T> struct OutStoreDot {         // o = Au ; acc += u Au
+    T* o;
+    static constexpr bool kDot = true;
+    using Aux = T;
+    __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux, T& acc) const {
+        o[q] = Au; acc += u * Au;
+    }
+};
+
+template <typename T> struct OutWriteDot {         // w = u (materialise the formed operand); o = Au ; acc += u Au
+    T* o; T* w;
+    static constexpr bool kDot = true;
+    using Aux = T;
+    __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux, T& acc) const {
+        w[q] = u; o[q] = Au; acc += u * Au;
+    }
+};
+
+template <typename T, typename TO> struct OutResid {  // r = b - Au (optionally cast), acc += r^2
+    const T* b; TO* r;
+    static constexpr bool kDot = true;
+    using Aux = T;
+    __device__ Aux pre(const OpTab&, size_t q) const { return b[q]; }
+    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T, T Au, Aux bq, T& acc) const {
+        T rv = bq - Au; r[q] = (TO)rv; acc += rv * rv;
+    }
+};
+
+// Smoothing sweep x_new = u + w (b - Au) / diag (damped Jacobi,
+// jacobi_brittany!) or u + w (b - Au) (Richardson, use_diag = 0);
+// optional acc += dotv * x_new.
+template <typename T> struct OutSmooth {
+    const T* b; T* xo; T omega; const T* dotv; int use_diag;
+    static constexpr bool kDot = true;
+    struct Aux { T b, d; };
+    __device__ Aux pre(const OpTab&, size_t q) const { Aux a; a.b = b[q]; a.d = dotv ? dotv[q] : T(0); return a; }
+    __device__ __forceinline__ void emit(const OpTab& t, int i, int j, size_t q, T u, T Au, Aux a, T& acc) const {
+        T c = omega * (a.b - Au);
+        if (use_diag) c = c / diag_of<T>(t, i, j);
+        const T xn = u + c;
+        xo[q] = xn; acc += a.d * xn;
+    }
+};
+
+// o = u (materialises a formed operand).
+template <typename T> struct OutCopy {
+    T* o;
+    static constexpr bool kDot = false;
+    using Aux = T;
+    __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T, Aux, T&) const { o[q] = u; }
+};
+
+// ---------------------------------------------------------------------------
+// Block reduction helpers (wave64 shuffles, then LDS across waves)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ T block_sum(T v) {
+    constexpr int NW = (NT + 63) / 64;
+    __shared__ T red[NW];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    T s = T(0);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) s += red[k];
+    }
+    return s;  // valid on thread 0
+}
+
+// ---------------------------------------------------------------------------
+// The stencil pass
+// ---------------------------------------------------------------------------
+// Closure sums via direct (cache-served) loads; used only for face rows/columns.
+template <typename T, class IN>
+__device__ __forceinline__ T k_closure(const OpTab& t, const IN& in, int i, int j) {
+    T s = T(0);
+    if (j < t.R) {
+#pragma unroll
+        for (int c = 0; c < kMaxC; ++c)
+            if (c < t.C) s += (T)t.kb[j][c] * in.load(t, i, c);
+    } else {
+        const int r = t.N - 1 - j;
+#pragma unroll
+        for (int c = 0; c < kMaxC; ++c)
+            if (c < t.C) s += (T)t.kb[r][c] * in.load(t, i, t.N - 1 - c);
+    }
+    return s;
+}
+
+template <typename T, class IN>
+__device__ __forceinline__ T m_closure(const OpTab& t, const IN& in, int i, int j) {
+    T s = T(0);
+    if (i < t.R) {
+#pragma unroll
+        for (int c = 0; c < kMaxC; ++c)
+            if (c < t.C) s += (T)t.mb[i][c] * in.load(t, c, j);
+    } else {
+        const int r = t.N - 1 - i;
+#pragma unroll
+        for (int c = 0; c < kMaxC; ++c)
+            if (c < t.C) s += (T)t.mb[r][c] * in.load(t, t.N - 1 - c, j);
+    }
+    return s;
+}
+
+struct PassCtl {
+    const int* done;     // nullable: skip the pass when *done != 0
+    double* partial;     // nullable: per-block inner-product partials
+    int nbx, nby;        // tile grid
+};
+
+template <int P, int TI, int J, typename T, class IN, class OUT>
+__global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, PassCtl ctl) {
+    constexpr int H = Stencil<P>::H;
+    constexpr int W = J + 2 * H;
+    if (ctl.done && *ctl.done) return;
+    const int nb = ctl.nbx * ctl.nby;
+    const int b = xcd_remap((int)blockIdx.x, nb);
+    const int bx = b % ctl.nbx, by = b / ctl.nbx;
+    const int tid = threadIdx.x;
+    const int N = t.N;
+    const int i0 = bx * TI, i = i0 + tid;
+    const int j0 = by * J;
+    const bool act = i < N;
+    in.init();
+
+    __shared__ T tile[J][TI + 2 * H];
+
+    // Column window u(i, j0-H .. j0+J-1+H).
+    T w[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int jj = j0 - H + k;
+        w[k] = (act && jj >= 0 && jj < N) ? in.load(t, i, jj) : T(0);
+    }
+    // Halo columns of the LDS tile.
+    if (tid < 2 * H) {
+        const int side = tid / H, off = tid % H;
+        const int ii = side == 0 ? i0 - H + off : i0 + TI + off;
+        const int col = side == 0 ? off : TI + H + off;
+#pragma unroll
+        for (int r = 0; r < J; ++r) {
+            const int jj = j0 + r;
+            tile[r][col] = (ii >= 0 && ii < N && jj < N) ? in.load(t, ii, jj) : T(0);
+        }
+    }
+    typename OUT::Aux aux[J];
+#pragma unroll
+    for (int r = 0; r < J; ++r) {
+        tile[r][tid + H] = w[r + H];
+        const int jj = j0 + r;
+        if (act && jj < N) aux[r] = out.pre(t, gidx(i, jj, N));
+    }
+    __syncthreads();
+
+    T acc = T(0);
+    if (act) {
+        const bool iint = (i >= t.R) && (i < N - t.R);
+        const T hwi = hw_of<T>(t, i);
+#pragma unroll
+        for (int r = 0; r < J; ++r) {
+            const int j = j0 + r;
+            if (j >= N) break;
+            const bool jint = (j >= t.R) && (j < N - t.R);
+            T kp, mp;
+            if (jint) {
+                kp = Stencil<P>::template c<T>(0) * w[r + H];
+#pragma unroll
+                for (int s = 1; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * (w[r + H - s] + w[r + H + s]);
+            } else {
+                kp = k_closure<T>(t, in, i, j);
+            }
+            if (iint) {
+                const T* row = &tile[r][tid + H];
+                mp = Stencil<P>::template c<T>(0) * row[0];
+#pragma unroll
+                for (int s = 1; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * (row[-s] + row[s]);
+            } else {
+                mp = m_closure<T>(t, in, i, j);
+            }
+            const T Au = (iint ? kp : hwi * kp) + (jint ? mp : hw_of<T>(t, j) * mp);
+            out.emit(t, i, j, gidx(i, j, N), w[r + H], Au, aux[r], acc);
+        }
+    }
+    if constexpr (OUT::kDot) {
+        if (ctl.partial) {
+            __syncthreads();
+            T s = block_sum<T, TI>(acc);
+            if (tid == 0) ctl.partial[b] = (double)s;
+        }
+    }
+}
+
+}  // namespace sbp

@@ -1,0 +1,115 @@
+"""The reference's GPU call surface, on torch device tensors.
+
+Each function keeps the name, argument order and in-place behaviour of the
+Julia function it stands in for, and calls libsbp.so through ``SbpPoisson``.
+Arrays are (N, N) or flat N*N device tensors in the reference's layout
+(element (i, j) at i + j*N).  Differences from the reference are deliberate
+and listed per function.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from .solver import SbpPoisson
+
+_HANDLES: dict = {}
+
+
+def handle(N, p=2, device=None, **params):
+    """Cached solver handle for (device, N, p, params)."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    key = (str(dev), int(N), int(p), tuple(sorted(params.items())))
+    h = _HANDLES.get(key)
+    if h is None:
+        h = SbpPoisson(N, p, device=dev, **params)
+        _HANDLES[key] = h
+    return h
+
+
+def _side(t):
+    n = t.numel()
+    N = int(round(math.sqrt(n)))
+    if N * N != n:
+        raise ValueError("grid arrays must hold N*N elements")
+    return N
+
+
+def matrix_free_A_full_GPU(idata, odata, p=2):
+    """odata = -A idata (split_matrix_free.jl:444-486).  The reference exists for
+    p = 2 only; p = 4 is the same operator family at fourth order."""
+    h = handle(_side(idata), p, device=idata.device)
+    h.apply(idata.reshape(-1), odata.reshape(-1), sign=-1)
+    return None
+
+
+def matrix_free_prolongation_2d_GPU(idata, odata):
+    """odata = P idata, bilinear, coarse (n, n) -> fine (2n-1, 2n-1)
+    (initial_guess_cg.jl:41-77)."""
+    Nc = _side(idata)
+    h = handle(2 * Nc - 1, 2, device=idata.device)
+    odata.zero_()
+    h.prolong_add(idata.reshape(-1), odata.reshape(-1), level=0)
+    return None
+
+
+def matrix_free_restriction_2d_GPU(idata, odata):
+    """odata = full weighting [1 2 1;2 4 2;1 2 1]/16 of idata with zero padding
+    (initial_guess_cg.jl:98-131), without the padded host copy."""
+    h = handle(_side(idata), 2, device=idata.device, restrict_scale=1.0)
+    h.restrict(idata.reshape(-1), level=0, out=odata.reshape(-1))
+    return None
+
+
+def matrix_free_richardson(idata, odata, b, maxiter=3, omega=0.15, p=2):
+    """maxiter Richardson sweeps x <- x + omega (b - A x) starting from idata;
+    idata and odata both end holding the result (initial_guess_cg.jl:134-144)."""
+    h = handle(_side(idata), p, device=idata.device, smoother=1, richardson_omega=float(omega))
+    if maxiter == 0:
+        odata.copy_(idata)
+        return None
+    h.smooth(b.reshape(-1), idata.reshape(-1), int(maxiter))
+    odata.copy_(idata)
+    return None
+
+
+def matrix_free_MGCG(b, x, maxiter=None, abstol=None, nu=2, p=2):
+    """MG-preconditioned CG (initial_guess_cg.jl:303-346) -> (iters, norms, x).
+    Differences: the preconditioner is the multi-level V-cycle with nu
+    damped-Jacobi sweeps down to 17x17 (the reference: a two-level cycle with
+    Richardson and a CPU LU); abstol defaults to sqrt(eps) ||b|| as at its
+    call site (matrix_free_mg_cg.jl:466-468)."""
+    h = handle(_side(b), p, device=b.device, nu_pre=nu, nu_post=nu)
+    bb = b.reshape(-1)
+    if abstol is None:
+        abstol = math.sqrt(2.220446049250313e-16) * float(torch.linalg.vector_norm(bb))
+    maxiter = bb.numel() if maxiter is None else maxiter
+    xs, res = h.mgpcg(bb, x.reshape(-1), abstol=abstol, maxiter=maxiter, history=True)
+    return res.iters, res.history, x
+
+
+def CG_Matrix_Free_GPU_v2(x, Ap, b, Nx, Ny, abstol=None, p=2):
+    """CG with the matrix-free apply (split_matrix_free.jl:688-714) ->
+    (iters, abstol, last ||r||).  Solves the SPD system A x = b; the reference
+    applies its -A kernel here and so converges to -A^-1 b."""
+    h = handle(Nx, p, device=b.device)
+    abstol = math.sqrt(2.220446049250313e-16) if abstol is None else abstol
+    _, res = h.cg(b.reshape(-1), x.reshape(-1), abstol=abstol, maxiter=Nx * Ny)
+    return res.iters, abstol, res.resnorm
+
+
+def CG_full_GPU(b, x, abstol=None, p=2):
+    """CG_full_GPU (split_matrix_free.jl:647-686) -> (iters, abstol, last ||r||)."""
+    N = _side(b)
+    return CG_Matrix_Free_GPU_v2(x, None, b, N, N, abstol=abstol, p=p)
+
+
+def CG_GPU_sparse(x, A_csr, b, abstol=None, p=2):
+    """CG with a CSR SpMV of the assembled operator (split_matrix_free.jl:744-769)
+    -> (iters, abstol, last ||r||).  A_csr is the (row_ptr, col, val) triple of
+    SbpPoisson.csr_assemble."""
+    h = handle(_side(b), p, device=b.device)
+    abstol = math.sqrt(2.220446049250313e-16) if abstol is None else abstol
+    _, res = h.csr_cg(A_csr, b.reshape(-1), x.reshape(-1), abstol=abstol)
+    return res.iters, abstol, res.resnorm

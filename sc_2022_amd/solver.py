@@ -1,0 +1,257 @@
+"""Python host side of the MI355X SBP-SAT Poisson solver.
+
+``SbpPoisson`` owns one C-ABI handle (operator tables, multigrid hierarchy,
+work vectors) and exposes the path on torch device tensors.  Grid functions
+are torch tensors on a HIP device with N*N elements in the reference's layout
+(element (i, j) at i + j*N, i fast); a row-major (N, N) tensor t is the
+transpose view of the reference's column-major (Nx, Ny) array: t[j, i] = u(i, j).
+
+Every call goes to libsbp.so; nothing here computes on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _capi as capi
+
+_DT = {torch.float64: capi.SBP_F64, torch.float32: capi.SBP_F32}
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+@dataclass
+class SolveResult:
+    iters: int
+    resnorm: float
+    status: int
+    history: list | None = None
+    inner_iters: int | None = None
+
+    @property
+    def converged(self):
+        return self.status == capi.SBP_OK
+
+
+class SbpPoisson:
+    """Matrix-free SBP-SAT operator of order p on an N x N grid (N = 2^k+1 for
+    multigrid).  Keyword arguments override sbp_params fields."""
+
+    def __init__(self, N: int, p: int = 2, device=None, **params):
+        self.lib = capi.load()
+        self.N = int(N)
+        self.p = int(p)
+        self.n = self.N * self.N
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.params = capi.default_params(**params)
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            capi.check(self.lib.sbp_create(self.N, self.p, C.byref(self.params), C.byref(h)), "sbp_create")
+        self._h = h
+        nl = C.c_int()
+        self.lib.sbp_get_info(self._h, None, None, C.byref(nl))
+        self.nlevels = nl.value
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.sbp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _sync_stream(self):
+        s = torch.cuda.current_stream(self.device)
+        self.lib.sbp_set_stream(self._h, C.c_void_p(s.cuda_stream))
+
+    def _chk(self, t, n=None, dtype=None, name="array"):
+        n = self.n if n is None else n
+        if t.device != self.device or not t.is_contiguous() or t.numel() != n:
+            raise ValueError(f"{name}: need a contiguous tensor of {n} elements on {self.device}")
+        if dtype is not None and t.dtype != dtype:
+            raise ValueError(f"{name}: need dtype {dtype}, got {t.dtype}")
+        return t
+
+    def level_size(self, level):
+        N = self.N
+        for _ in range(level):
+            N = (N + 1) // 2
+        return N
+
+    def set_tiles(self, level, ti, tj):
+        capi.check(self.lib.sbp_set_tiles(self._h, level, ti, tj), "sbp_set_tiles")
+
+    # -- operator ----------------------------------------------------------
+    def apply(self, u, out=None, sign=1, level=0):
+        """out = sign * A u (sign = -1 is the reference's matrix_free_A_full_GPU)."""
+        n = self.level_size(level) ** 2
+        self._chk(u, n, name="u")
+        out = torch.empty_like(u) if out is None else self._chk(out, n, u.dtype, "out")
+        self._sync_stream()
+        capi.check(self.lib.sbp_apply(self._h, level, _ptr(u), _ptr(out), sign, _DT[u.dtype]), "sbp_apply")
+        return out
+
+    def apply_dot(self, u, out=None):
+        self._chk(u, dtype=torch.float64, name="u")
+        out = torch.empty_like(u) if out is None else self._chk(out, dtype=torch.float64, name="out")
+        d = C.c_double()
+        self._sync_stream()
+        capi.check(self.lib.sbp_apply_dot(self._h, _ptr(u), _ptr(out), C.byref(d)), "sbp_apply_dot")
+        return out, d.value
+
+    def diag(self):
+        d = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        self._sync_stream()
+        capi.check(self.lib.sbp_diag(self._h, _ptr(d)), "sbp_diag")
+        return d
+
+    def assemble_rhs(self, f=None, gW=None, gE=None, gS=None, gN=None, out=None):
+        """b = -H~ (f + SAT_W_r gW + SAT_E_r gE + SAT_S_r gS + SAT_N_r gN), f = Laplace(u)."""
+        for t, nm, n in ((f, "f", self.n), (gW, "gW", self.N), (gE, "gE", self.N), (gS, "gS", self.N),
+                         (gN, "gN", self.N)):
+            if t is not None:
+                self._chk(t, n, torch.float64, nm)
+        b = torch.empty(self.n, dtype=torch.float64, device=self.device) if out is None else out
+        self._sync_stream()
+        capi.check(self.lib.sbp_assemble_rhs(self._h, _ptr(f), _ptr(gW), _ptr(gE), _ptr(gS), _ptr(gN), _ptr(b)),
+                   "sbp_assemble_rhs")
+        return b
+
+    def rhs_transpose(self, lam):
+        self._chk(lam, dtype=torch.float64, name="lam")
+        z = lambda n: torch.empty(n, dtype=torch.float64, device=self.device)
+        df, gW, gE, gS, gN = z(self.n), z(self.N), z(self.N), z(self.N), z(self.N)
+        self._sync_stream()
+        capi.check(self.lib.sbp_rhs_transpose(self._h, _ptr(lam), _ptr(df), _ptr(gW), _ptr(gE), _ptr(gS), _ptr(gN)),
+                   "sbp_rhs_transpose")
+        return df, gW, gE, gS, gN
+
+    def hnorm(self, v):
+        self._chk(v, dtype=torch.float64, name="v")
+        d = C.c_double()
+        self._sync_stream()
+        capi.check(self.lib.sbp_hnorm(self._h, _ptr(v), C.byref(d)), "sbp_hnorm")
+        return d.value
+
+    def dot(self, u, v):
+        self._chk(u, dtype=torch.float64, name="u")
+        self._chk(v, dtype=torch.float64, name="v")
+        d = C.c_double()
+        self._sync_stream()
+        capi.check(self.lib.sbp_dot(self._h, _ptr(u), _ptr(v), C.byref(d)), "sbp_dot")
+        return d.value
+
+    # -- multigrid components ---------------------------------------------
+    def restrict(self, rf, level=0, out=None):
+        Nc = self.level_size(level + 1)
+        self._chk(rf, self.level_size(level) ** 2, name="rf")
+        out = torch.empty(Nc * Nc, dtype=rf.dtype, device=self.device) if out is None else out
+        self._sync_stream()
+        capi.check(self.lib.sbp_restrict(self._h, level, _ptr(rf), _ptr(out), _DT[rf.dtype]), "sbp_restrict")
+        return out
+
+    def prolong_add(self, ec, xf, level=0):
+        self._chk(ec, self.level_size(level + 1) ** 2, name="ec")
+        self._chk(xf, self.level_size(level) ** 2, ec.dtype, "xf")
+        self._sync_stream()
+        capi.check(self.lib.sbp_prolong_add(self._h, level, _ptr(ec), _ptr(xf), _DT[ec.dtype]), "sbp_prolong_add")
+        return xf
+
+    def smooth(self, b, x, nsweeps, level=0):
+        n = self.level_size(level) ** 2
+        self._chk(b, n, name="b")
+        self._chk(x, n, b.dtype, "x")
+        self._sync_stream()
+        capi.check(self.lib.sbp_smooth(self._h, level, _ptr(b), _ptr(x), nsweeps, _DT[b.dtype]), "sbp_smooth")
+        return x
+
+    def coarse_solve(self, b):
+        x = torch.empty_like(b)
+        self._sync_stream()
+        capi.check(self.lib.sbp_coarse_solve(self._h, _ptr(b), _ptr(x)), "sbp_coarse_solve")
+        return x
+
+    def vcycle(self, r, out=None):
+        self._chk(r, name="r")
+        out = torch.empty_like(r) if out is None else out
+        self._sync_stream()
+        capi.check(self.lib.sbp_vcycle(self._h, _ptr(r), _ptr(out), _DT[r.dtype]), "sbp_vcycle")
+        return out
+
+    # -- solvers -----------------------------------------------------------
+    def _solve(self, fn, name, b, x, abstol, maxiter, history, raise_on_fail):
+        self._chk(b, dtype=torch.float64, name="b")
+        x = torch.zeros_like(b) if x is None else self._chk(x, dtype=torch.float64, name="x")
+        it = C.c_int()
+        rn = C.c_double()
+        hist = torch.zeros(maxiter + 1, dtype=torch.float64, device=self.device) if history else None
+        self._sync_stream()
+        st = fn(self._h, _ptr(b), _ptr(x), float(abstol), int(maxiter), C.byref(it), C.byref(rn), _ptr(hist))
+        if raise_on_fail and st not in (capi.SBP_OK, capi.SBP_ERR_NOT_CONVERGED):
+            capi.check(st, name)
+        hl = hist[: it.value + 1].cpu().tolist() if history else None
+        return x, SolveResult(it.value, rn.value, st, hl)
+
+    def cg(self, b, x=None, abstol=1e-8, maxiter=None, history=False, raise_on_fail=True):
+        maxiter = self.n if maxiter is None else maxiter
+        return self._solve(self.lib.sbp_cg, "sbp_cg", b, x, abstol, maxiter, history, raise_on_fail)
+
+    def mgpcg(self, b, x=None, abstol=1e-8, maxiter=500, history=False, raise_on_fail=True):
+        return self._solve(self.lib.sbp_mgpcg, "sbp_mgpcg", b, x, abstol, maxiter, history, raise_on_fail)
+
+    def mgpcg_mixed(self, b, x=None, abstol=1e-8, max_outer=None, raise_on_fail=True):
+        self._chk(b, dtype=torch.float64, name="b")
+        x = torch.zeros_like(b) if x is None else self._chk(x, dtype=torch.float64, name="x")
+        max_outer = self.params.max_outer if max_outer is None else max_outer
+        oi, ii, rn = C.c_int(), C.c_int(), C.c_double()
+        self._sync_stream()
+        st = self.lib.sbp_mgpcg_mixed(self._h, _ptr(b), _ptr(x), float(abstol), int(max_outer), C.byref(oi),
+                                      C.byref(ii), C.byref(rn))
+        if raise_on_fail and st not in (capi.SBP_OK, capi.SBP_ERR_NOT_CONVERGED):
+            capi.check(st, "sbp_mgpcg_mixed")
+        return x, SolveResult(oi.value, rn.value, st, None, ii.value)
+
+    # -- CSR comparison path -------------------------------------------------
+    def csr_assemble(self):
+        nnz = C.c_int64()
+        self._sync_stream()
+        capi.check(self.lib.sbp_csr_assemble(self._h, None, None, None, C.byref(nnz)), "sbp_csr_assemble")
+        rp = torch.empty(self.n + 1, dtype=torch.int32, device=self.device)
+        col = torch.empty(nnz.value, dtype=torch.int32, device=self.device)
+        val = torch.empty(nnz.value, dtype=torch.float64, device=self.device)
+        capi.check(self.lib.sbp_csr_assemble(self._h, _ptr(rp), _ptr(col), _ptr(val), C.byref(nnz)),
+                   "sbp_csr_assemble")
+        return rp, col, val
+
+    def csr_spmv(self, csr, x, out=None):
+        rp, col, val = csr
+        out = torch.empty_like(x) if out is None else out
+        self._sync_stream()
+        capi.check(self.lib.sbp_csr_spmv(self._h, _ptr(rp), _ptr(col), _ptr(val), _ptr(x), _ptr(out)),
+                   "sbp_csr_spmv")
+        return out
+
+    def csr_cg(self, csr, b, x=None, abstol=1e-8, maxiter=None, raise_on_fail=True):
+        rp, col, val = csr
+        maxiter = self.n if maxiter is None else maxiter
+        x = torch.zeros_like(b) if x is None else x
+        it, rn = C.c_int(), C.c_double()
+        self._sync_stream()
+        st = self.lib.sbp_csr_cg(self._h, _ptr(rp), _ptr(col), _ptr(val), _ptr(b), _ptr(x), float(abstol),
+                                 int(maxiter), C.byref(it), C.byref(rn))
+        if raise_on_fail and st not in (capi.SBP_OK, capi.SBP_ERR_NOT_CONVERGED):
+            capi.check(st, "sbp_csr_cg")
+        return x, SolveResult(it.value, rn.value, st)
+
+    def copy(self, src, dst):
+        self._sync_stream()
+        capi.check(self.lib.sbp_copy(self._h, _ptr(src), _ptr(dst), src.numel() * src.element_size()), "sbp_copy")
+        return dst

@@ -1,0 +1,226 @@
+"""Pins the CPU oracle (oracle/sbp_oracle.py) against facts that do not come
+from the oracle itself: polynomial exactness and the SBP property of the 1-D
+operators, the literal restatement of the reference GPU kernel, the closed
+tensor form, transfer-operator identities, direct solves, manufactured-solution
+convergence rates and the adjoint (dot-product) identity.  Each check is
+chosen so that a plausible slip (a dropped closure term, a wrong sign or index,
+a transposed kron) fails it."""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import sbp_oracle as O
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_d2_polynomial_exactness(p):
+    N = 40
+    D2, S0, SN, HI, H, r = O.diagonal_sbp_D2(p, N, xc=(0, 1))
+    D2 = D2.toarray()
+    bm = 1 if p == 2 else 4
+    kb = 2 if p == 2 else 3          # boundary accuracy degree
+    for k in range(0, p + 2):
+        v = r ** k
+        exact = k * (k - 1) * r ** (k - 2) if k >= 2 else np.zeros_like(r)
+        err = np.abs(D2 @ v - exact)
+        assert err[bm:-bm].max() < 1e-7 * max(1, k ** 2), (p, k)
+        if k <= kb:
+            assert err.max() < 1e-7 * max(1, k ** 2), (p, k)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_d2_sbp_property(p):
+    """H D2 = -M + BS with M symmetric positive semidefinite and M 1 = 0
+    (diagonal_sbp.jl:159-160)."""
+    N = 30
+    D2, S0, SN, HI, H, r = O.diagonal_sbp_D2(p, N, xc=(0, 1))
+    BS = (SN - S0).toarray()
+    M = BS - (H @ D2).toarray()
+    assert np.abs(M - M.T).max() < 1e-9 * np.abs(M).max()
+    assert np.abs(M @ np.ones(N + 1)).max() < 1e-9 * np.abs(M).max()
+    ev = np.linalg.eigvalsh(0.5 * (M + M.T))
+    assert ev.min() > -1e-9 * np.abs(ev).max()
+    assert abs(H.diagonal().sum() - 1.0) < 1e-14
+    # S0/SN approximate -/+ first derivatives at the ends: exact on linears
+    # (BS row = outward normal derivative)
+    assert abs((S0 @ r)[0] - 1.0) < 1e-12 and abs((SN @ r)[-1] - 1.0) < 1e-12
+    assert abs((BS @ r ** 2)[0] - 0.0) < 1e-9 and abs((BS @ r ** 2)[-1] - 2.0) < 1e-9
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_d1_sbp_property(p):
+    N = 30
+    D, HI, H, r = O.diagonal_sbp_D1(p, N, xc=(0, 1))
+    Q = (H @ D).toarray()
+    B = np.zeros((N + 1, N + 1)); B[0, 0] = -1; B[-1, -1] = 1
+    assert np.abs(Q + Q.T - B).max() < 1e-12
+    for k in range(0, p // 2 + 1):
+        v = r ** k
+        exact = k * r ** (k - 1) if k >= 1 else np.zeros_like(r)
+        assert np.abs(D @ v - exact).max() < 1e-9
+    # D1's norm equals D2's norm (the assembly mixes them)
+    _, _, _, _, H2, _ = O.diagonal_sbp_D2(p, N, xc=(0, 1))
+    assert np.abs(H.diagonal() - H2.diagonal()).max() < 1e-15
+
+
+def test_invalid_order_and_size():
+    with pytest.raises(ValueError):
+        O.diagonal_sbp_D2(6 + 1, 20)
+    with pytest.raises(ValueError):
+        O.diagonal_sbp_D2(4, 5)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("level", [3, 4])
+def test_operator_spd_and_closed_form(level, p):
+    A = O.assemble_operator(level, p)
+    Ad = A.toarray()
+    assert np.abs(Ad - Ad.T).max() < 1e-13 * np.abs(Ad).max()
+    assert np.linalg.eigvalsh(Ad).min() > 0
+    C = O.closed_form_operator(level, p)
+    assert abs(A - C).max() < 1e-13 * abs(A).max()
+    # A commutes with the 180-degree reversal of the grid
+    R = np.eye(A.shape[0])[::-1]
+    assert np.abs(R @ Ad @ R - Ad).max() < 1e-13 * np.abs(Ad).max()
+
+
+@pytest.mark.parametrize("level", [3, 4, 5])
+def test_reference_gpu_kernel_is_minus_A(level):
+    """The literal restatement of matrix_free_A_full_GPU (p=2) equals -A on
+    every unit vector: two independent texts of the reference agree."""
+    N = 2 ** level + 1
+    A = O.assemble_operator(level, 2).toarray()
+    I = np.eye(N * N)
+    K = np.column_stack([O.reference_gpu_apply_p2(I[:, c], N) for c in range(N * N)])
+    assert np.abs(K + A).max() < 1e-12
+
+
+@pytest.mark.parametrize("p,rate_lo,rate_hi,e129", [(2, 1.95, 2.05, 1.77e-5), (4, 3.6, 4.3, 2.58e-9)])
+def test_manufactured_convergence(p, rate_lo, rate_hi, e129):
+    errs = []
+    for level in (4, 5, 6, 7):
+        A = O.assemble_operator(level, p)
+        f, gW, gE, gS, gN, u = O.baseline_data(level)
+        b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
+        x = spla.spsolve(A.tocsc(), b)
+        errs.append(O.h_norm(x - u, O.h_tilde_diag(level, p)))
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(3)]
+    assert rate_lo < rates[-1] < rate_hi, rates
+    assert abs(errs[-1] / e129 - 1) < 0.02
+    # the reference's own manufactured solution converges too
+    A = O.assemble_operator(6, p)
+    Ab, b, Ht, _, _, u = O.assembling_matrix(6, p)
+    x = spla.spsolve(Ab.tocsc(), b)
+    assert O.h_norm(x - u, Ht.diagonal()) < 1e-3
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_rhs_matches_manufactured_limit(p):
+    """b built from the data of u equals A u up to the truncation error: the
+    SAT data maps carry the right signs and scalings."""
+    for level, tol in ((6, 5e-2), (7, 2.5e-2)):
+        A = O.assemble_operator(level, p)
+        f, gW, gE, gS, gN, u = O.baseline_data(level)
+        b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
+        hd = O.h_tilde_diag(level, p)
+        rel = np.linalg.norm((A @ u - b) / np.sqrt(hd)) / np.linalg.norm(b / np.sqrt(hd))
+        assert rel < tol
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_rhs_transpose_identity(p):
+    level = 4
+    N = 2 ** level + 1
+    rng = np.random.default_rng(3)
+    f, gW, gE, gS, gN = (rng.standard_normal(N * N), *rng.standard_normal((4, N)))
+    lam = rng.standard_normal(N * N)
+    b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
+    df, dW, dE, dS, dN = O.rhs_transpose(level, p, lam)
+    lhs = b @ lam
+    rhs = f @ df + gW @ dW + gE @ dE + gS @ dS + gN @ dN
+    assert abs(lhs - rhs) < 1e-12 * (abs(lhs) + 1)
+
+
+def test_transfers():
+    for Nc in (5, 9, 17):
+        Nf = 2 * Nc - 1
+        P = O.prolongation_2d(Nc)
+        R = O.restriction_2d(Nf)
+        assert abs(R - P.T / 4).max() < 1e-15
+        # P exact on bilinear functions
+        tc = np.linspace(0, 1, Nc); tf = np.linspace(0, 1, Nf)
+        g = lambda a, b: 1 + 2 * a + 3 * b + 4 * a * b
+        Ec = g(tc[:, None], tc[None, :]).ravel(order="F")
+        Ef = g(tf[:, None], tf[None, :]).ravel(order="F")
+        assert np.abs(P @ Ec - Ef).max() < 1e-13
+        # full weighting keeps constants at interior coarse points
+        rc = (R @ np.ones(Nf * Nf)).reshape(Nc, Nc, order="F")
+        assert np.abs(rc[1:-1, 1:-1] - 1).max() < 1e-15
+        assert abs(rc[0, 0] - 9 / 16) < 1e-15
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_cg_cpu_against_direct(p):
+    level = 4
+    A = O.assemble_operator(level, p)
+    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
+    xd = spla.spsolve(A.tocsc(), b)
+    x, it, norms = O.cg_cpu(A, b, abstol=1e-12 * np.linalg.norm(b))
+    assert norms[-1] < 1e-12 * np.linalg.norm(b)
+    assert np.linalg.norm(x - xd) < 1e-9 * np.linalg.norm(xd)
+    assert 10 < it < A.shape[0]
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_mgpcg_oracle_against_direct(p):
+    level = 6
+    hier = O.Hierarchy(level, p)
+    A = hier.A[0]
+    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
+    xd = spla.spsolve(A.tocsc(), b)
+    x, it, norms = O.mgpcg(hier, b, rtol=1e-12)
+    assert np.linalg.norm(x - xd) < 1e-9 * np.linalg.norm(xd)
+    assert it < 40
+    # the V-cycle is a symmetric operator (needed by PCG)
+    n = A.shape[0]
+    rng = np.random.default_rng(5)
+    a, c = rng.standard_normal(n), rng.standard_normal(n)
+    assert abs(a @ O.vcycle(hier, c) - c @ O.vcycle(hier, a)) < 1e-10 * abs(a @ O.vcycle(hier, c))
+
+
+def test_cg_iteration_count_129_p2():
+    """Expected count for the 129^2 CG case (relative residual 1e-8)."""
+    A = O.assemble_operator(7, 2)
+    f, gW, gE, gS, gN, u = O.baseline_data(7)
+    b = O.assemble_rhs(7, 2, f, gW, gE, gS, gN)
+    x, it, norms = O.cg_cpu(A, b, abstol=1e-8 * np.linalg.norm(b))
+    assert 540 <= it <= 580
+
+
+def test_fourier_modes_deterministic():
+    s1, b1 = O.fourier_modes(123)
+    s2, b2 = O.fourier_modes(123)
+    assert s1 == s2 and b1 == b2
+    assert len(s1) == 64 and all(len(m) == 8 for m in b1)
+    for kx, ky, a, ph in s1:
+        assert 1 <= kx <= 32 and 1 <= ky <= 32 and 0 <= ph < 2 * np.pi
+        assert abs(a - 1 / (kx * kx + ky * ky + 1)) < 1e-15
+    assert O.fourier_modes(124)[0] != s1
+
+
+@pytest.mark.parametrize("level,p", [(4, 2), (4, 4), (5, 2), (5, 4)])
+def test_golden_fixtures(golden_dir, level, p):
+    d = np.load(os.path.join(golden_dir, f"sbp_L{level}_p{p}.npz"))
+    A = O.assemble_operator(level, p)
+    A0 = sp.csr_matrix((d["A_data"], d["A_indices"], d["A_indptr"]), shape=A.shape)
+    assert abs(A - A0).max() < 1e-14
+    assert np.abs(A @ d["u"] - d["Au"]).max() < 1e-12
+    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    assert np.abs(O.assemble_rhs(level, p, f, gW, gE, gS, gN) - d["b"]).max() < 1e-14
+    assert np.abs(A @ d["x"] - d["b"]).max() < 1e-10
