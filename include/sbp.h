@@ -59,6 +59,7 @@ typedef struct {
     int max_outer;           /* mixed precision: refinement step cap (default 10) */
     int tile_i;              /* apply tile width in i (0 = autotuned) */
     int tile_j;              /* apply rows per thread (0 = autotuned) */
+    int use_graph;           /* replay each Krylov iteration from a captured hipGraph (default 1) */
 } sbp_params;
 
 typedef struct sbp_handle_s* sbp_handle;
@@ -75,7 +76,9 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out);
 int sbp_destroy(sbp_handle h);
 int sbp_set_stream(sbp_handle h, void* hip_stream);
 int sbp_get_info(sbp_handle h, int* N, int* p, int* nlevels);
-int sbp_set_tiles(sbp_handle h, int level, int tile_i, int tile_j);
+/* Stencil tile shape (tile_i threads along i x tile_j rows per thread) used
+ * on `level` for element type `dtype`; shapes not compiled in are rejected. */
+int sbp_set_tiles(sbp_handle h, int level, int dtype, int tile_i, int tile_j);
 
 /* Operator --------------------------------------------------------------- */
 /* Au = sign * A u on multigrid level `level` (0 = finest).  sign = -1 gives

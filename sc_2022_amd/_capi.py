@@ -47,6 +47,7 @@ class SbpParams(C.Structure):
         ("max_outer", C.c_int),
         ("tile_i", C.c_int),
         ("tile_j", C.c_int),
+        ("use_graph", C.c_int),
     ]
 
 
@@ -58,7 +59,7 @@ SIGNATURES = {
     "sbp_destroy": (_i, [_vp]),
     "sbp_set_stream": (_i, [_vp, _vp]),
     "sbp_get_info": (_i, [_vp, _pi, _pi, _pi]),
-    "sbp_set_tiles": (_i, [_vp, _i, _i, _i]),
+    "sbp_set_tiles": (_i, [_vp, _i, _i, _i, _i]),
     "sbp_apply": (_i, [_vp, _i, _vp, _vp, _i, _i]),
     "sbp_apply_dot": (_i, [_vp, _vp, _vp, _pd]),
     "sbp_diag": (_i, [_vp, _vp]),

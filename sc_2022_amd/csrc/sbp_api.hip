@@ -33,7 +33,8 @@ struct Level {
     int N = 0;
     size_t n = 0;
     OpTab tb{};
-    int ti = 0, tj = 0;
+    OpTab* dtab = nullptr;                // device copy of tb (tb.dev points here)
+    int ti[2] = {0, 0}, tj[2] = {0, 0};   // tile shape per precision (0 = f64, 1 = f32)
     // per precision (0 = f64, 1 = f32): rhs, two sweep buffers, output, residual
     void* b[2] = {nullptr, nullptr};
     void* xa[2] = {nullptr, nullptr};
@@ -62,7 +63,12 @@ struct sbp_handle_s {
     Scalars* sc_in = nullptr;   // inner FP32 solver
     Scalars* sc_host = nullptr; // pinned mirror
     double* dtmp = nullptr;     // small device scalars
+    double* x = nullptr;        // internal FP64 iterate (solvers work on it, then copy out)
     bool have32 = false;
+    // Captured Krylov iterations: [precision][0 = CG, 1 = MG-PCG]
+    hipStream_t cap = nullptr;  // capture stream (default priority)
+    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
 
 namespace {
@@ -102,20 +108,21 @@ inline size_t pass_blocks(int N, int ti, int tj) {
     return (size_t)((N + ti - 1) / ti) * (size_t)((N + tj - 1) / tj);
 }
 
+template <typename T> inline int prec_index();
+template <> inline int prec_index<double>() { return 0; }
+template <> inline int prec_index<float>() { return 1; }
+
 template <typename T, class IN, class OUT>
 int launch_pass(int p, const Level& L, const IN& in, const OUT& out, const int* done, double* partial,
                 hipStream_t st, int* nparts) {
-    if (nparts) *nparts = (int)pass_blocks(L.N, L.ti, L.tj);
+    const int ti = L.ti[prec_index<T>()], tj = L.tj[prec_index<T>()];
+    if (nparts) *nparts = (int)pass_blocks(L.N, ti, tj);
 #define SBP_TILE_CASE(TI_, J_) \
-    if (L.ti == TI_ && L.tj == J_) return launch_tile<TI_, J_, T, IN, OUT>(p, L.tb, in, out, done, partial, st);
+    if (ti == TI_ && tj == J_) return launch_tile<TI_, J_, T, IN, OUT>(p, L.tb, in, out, done, partial, st);
     SBP_TILE_SHAPES(SBP_TILE_CASE)
 #undef SBP_TILE_CASE
     return SBP_ERR_INVALID_ARG;
 }
-
-template <typename T> inline int prec_index();
-template <> inline int prec_index<double>() { return 0; }
-template <> inline int prec_index<float>() { return 1; }
 
 int alloc_level_buffers(sbp_handle h, int prec) {
     const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
@@ -236,7 +243,17 @@ int vcycle_nparts(sbp_handle h) {
     const Level& L = h->L[0];
     if (h->nlev == 1) return 1;
     if (h->prm.nu_post == 0) return ew_blocks(L.n);
-    return (int)pass_blocks(L.N, L.ti, L.tj);
+    return (int)pass_blocks(L.N, L.ti[prec_index<T>()], L.tj[prec_index<T>()]);
+}
+
+void drop_graphs(sbp_handle h) {
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            if (h->gexec[a][b]) (void)hipGraphExecDestroy(h->gexec[a][b]);
+            if (h->graph[a][b]) (void)hipGraphDestroy(h->graph[a][b]);
+            h->gexec[a][b] = nullptr;
+            h->graph[a][b] = nullptr;
+        }
 }
 
 int poll(sbp_handle h, Scalars* sc) {
@@ -265,17 +282,73 @@ int ensure32(sbp_handle h) {
     return SBP_OK;
 }
 
-// Preconditioned CG body shared by FP64 and FP32: solves A x = b given the
-// initial residual r (already in r, with sc->rr / abstol / done set by the
-// caller's init kernel).  prec = V-cycle when use_mg, identity otherwise.
+// One Krylov iteration on handle-owned buffers (x is the internal iterate):
+// apply + p'Ap, alpha, x/r update + r'r, convergence test, preconditioner,
+// beta, p update.  Every kernel tests the device "done" flag.
 template <typename T>
-int pcg_loop(sbp_handle h, T* x, T* r, T* pv, T* Ap, T* z, Scalars* sc, int maxiter, bool use_mg, double* hist,
-             int* iters_out) {
+int issue_iteration(sbp_handle h, T* x, T* r, T* pv, T* Ap, T* z, Scalars* sc, bool use_mg) {
     hipStream_t st = h->st;
     const Level& L0 = h->L[0];
     const size_t n = L0.n;
     const int* done = &sc->done;
     int np = 0;
+    STCHK(launch_pass<T>(h->p, L0, InPlain<T>{pv}, OutStoreDot<T>{Ap}, done, h->partial, st, &np));
+    hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(256), 0, st, h->partial, np, sc, use_mg ? 1 : 0);
+    const int eb = ew_blocks(n);
+    hipLaunchKernelGGL((update_xr<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, x, r, (const T*)pv,
+                       (const T*)Ap, (const Scalars*)sc, n, h->partial);
+    hipLaunchKernelGGL(krylov_check, dim3(1), dim3(256), 0, st, h->partial, eb, sc, use_mg ? 0 : 1);
+    HIPCHK(hipGetLastError());
+    if (use_mg) {
+        STCHK(vcycle_level<T>(h, 0, r, z, r, done, h->partial));
+        hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(256), 0, st, h->partial, vcycle_nparts<T>(h), sc);
+        hipLaunchKernelGGL((update_p<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, pv, (const T*)z,
+                           (const Scalars*)sc, n);
+    } else {
+        hipLaunchKernelGGL((update_p<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, pv, (const T*)r,
+                           (const Scalars*)sc, n);
+    }
+    HIPCHK(hipGetLastError());
+    return SBP_OK;
+}
+
+// Captures issue_iteration once per (precision, solver) into a hipGraph on
+// the handle's own capture stream; replays go to the handle's stream.
+template <typename T>
+int iteration_graph(sbp_handle h, T* x, T* r, T* pv, T* Ap, T* z, Scalars* sc, bool use_mg, hipGraphExec_t* out) {
+    const int a = prec_index<T>(), b = use_mg ? 1 : 0;
+    if (!h->gexec[a][b]) {
+        if (!h->cap) HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+        HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+        hipStream_t saved = h->st;
+        h->st = h->cap;
+        const int rc = issue_iteration<T>(h, x, r, pv, Ap, z, sc, use_mg);
+        h->st = saved;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(h->cap, &g);
+        if (rc != SBP_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+        HIPCHK(e);
+        hipGraphExec_t ex = nullptr;
+        if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return SBP_ERR_HIP;
+        }
+        h->graph[a][b] = g;
+        h->gexec[a][b] = ex;
+    }
+    *out = h->gexec[a][b];
+    return SBP_OK;
+}
+
+// Preconditioned CG body shared by FP64 and FP32: solves A x = b given the
+// initial residual in r (sc->rr / abstol / done already set by the caller's
+// init kernel); the preconditioner is the V-cycle when use_mg, else identity.
+template <typename T>
+int pcg_loop(sbp_handle h, T* x, T* r, T* pv, T* Ap, T* z, Scalars* sc, int maxiter, bool use_mg,
+             int* iters_out) {
+    hipStream_t st = h->st;
+    const size_t n = h->L[0].n;
+    const int* done = &sc->done;
     if (use_mg) {
         STCHK(vcycle_level<T>(h, 0, r, z, r, done, h->partial));
         hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(256), 0, st, h->partial, vcycle_nparts<T>(h), sc);
@@ -287,29 +360,15 @@ int pcg_loop(sbp_handle h, T* x, T* r, T* pv, T* Ap, T* z, Scalars* sc, int maxi
                            done);
     }
     HIPCHK(hipGetLastError());
+    hipGraphExec_t ex = nullptr;
+    if (h->prm.use_graph) STCHK(iteration_graph<T>(h, x, r, pv, Ap, z, sc, use_mg, &ex));
     const int poll_every = std::max(1, h->prm.poll_every);
-    int it = 0;
-    for (; it < maxiter; ++it) {
-        STCHK(launch_pass<T>(h->p, L0, InPlain<T>{pv}, OutStoreDot<T>{Ap}, done, h->partial, st, &np));
-        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(256), 0, st, h->partial, np, sc, use_mg ? 1 : 0);
-        const int eb = ew_blocks(n);
-        hipLaunchKernelGGL((update_xr<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, x, r, (const T*)pv,
-                           (const T*)Ap, (const Scalars*)sc, n, h->partial);
-        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(256), 0, st, h->partial, eb, sc, it, hist, use_mg ? 0 : 1);
-        HIPCHK(hipGetLastError());
-        if (use_mg) {
-            STCHK(vcycle_level<T>(h, 0, r, z, r, done, h->partial));
-            hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(256), 0, st, h->partial, vcycle_nparts<T>(h), sc);
-            hipLaunchKernelGGL((update_p<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, pv, (const T*)z,
-                               (const Scalars*)sc, n);
-        } else {
-            hipLaunchKernelGGL((update_p<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, pv, (const T*)r,
-                               (const Scalars*)sc, n);
-        }
-        HIPCHK(hipGetLastError());
-        if ((it + 1) % poll_every == 0) {
+    for (int it = 0; it < maxiter; ++it) {
+        if (ex) HIPCHK(hipGraphLaunch(ex, st));
+        else STCHK(issue_iteration<T>(h, x, r, pv, Ap, z, sc, use_mg));
+        if ((it + 1) % poll_every == 0 && it + 1 < maxiter) {
             STCHK(poll(h, sc));
-            if (h->sc_host->done) { ++it; break; }
+            if (h->sc_host->done) break;
         }
     }
     STCHK(poll(h, sc));
@@ -357,6 +416,27 @@ int init_residual(sbp_handle h, const double* b, const double* x, double abstol,
 }
 }  // namespace
 
+namespace {
+// Runs the FP64 (P)CG on the internal iterate: x -> h->x, iterate, h->x -> x.
+int krylov64(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
+             double* hist, bool use_mg) {
+    const size_t n = h->L[0].n;
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, h->st,
+                       (const double*)x, h->x, n, nullptr);
+    HIPCHK(hipGetLastError());
+    STCHK(init_residual(h, b, h->x, abstol, hist));
+    int it = 0;
+    int rc = pcg_loop<double>(h, h->x, h->r, h->pv, h->Ap, h->z, h->sc, maxiter, use_mg, &it);
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, h->st,
+                       (const double*)h->x, x, n, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (iters) *iters = it;
+    if (resnorm) *resnorm = h->sc_host->resnorm;
+    return rc;
+}
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -395,12 +475,14 @@ void sbp_default_params(sbp_params* prm) {
     prm->max_outer = 10;
     prm->tile_i = 0;
     prm->tile_j = 0;
+    prm->use_graph = 1;
 }
 
 int sbp_destroy(sbp_handle h) {
     if (!h) return SBP_ERR_INVALID_ARG;
     for (int l = 0; l < kMaxLevels; ++l) {
         Level& L = h->L[l];
+        if (L.dtab) (void)hipFree(L.dtab);
         for (int q = 0; q < 2; ++q) {
             if (L.b[q] && l > 0) (void)hipFree(L.b[q]);
             if (L.xa[q]) (void)hipFree(L.xa[q]);
@@ -409,7 +491,9 @@ int sbp_destroy(sbp_handle h) {
             if (L.r[q]) (void)hipFree(L.r[q]);
         }
     }
-    void* ptrs[] = {h->Ainv, h->r, h->pv, h->Ap, h->z, h->r32, h->p32, h->Ap32, h->z32, h->d32,
+    drop_graphs(h);
+    if (h->cap) (void)hipStreamDestroy(h->cap);
+    void* ptrs[] = {h->Ainv, h->r, h->pv, h->Ap, h->z, h->x, h->r32, h->p32, h->Ap32, h->z32, h->d32,
                     h->partial, h->sc, h->sc_in, h->dtmp};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -461,19 +545,29 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) {
     h->nlev = nlev;
     for (int l = 0; l < nlev; ++l) {
         Level& L = h->L[l];
-        if (l == 0 && h->prm.tile_i > 0) {
-            if (!tile_ok(h->prm.tile_i, h->prm.tile_j)) { delete h; return SBP_ERR_INVALID_ARG; }
-            L.ti = h->prm.tile_i;
-            L.tj = h->prm.tile_j;
-        } else {
-            default_tile(L.N, &L.ti, &L.tj);
+        for (int q = 0; q < 2; ++q) {
+            if (l == 0 && h->prm.tile_i > 0) {
+                if (!tile_ok(h->prm.tile_i, h->prm.tile_j)) { delete h; return SBP_ERR_INVALID_ARG; }
+                L.ti[q] = h->prm.tile_i;
+                L.tj[q] = h->prm.tile_j;
+            } else {
+                default_tile(L.N, &L.ti[q], &L.tj[q]);
+            }
         }
     }
     int rc = SBP_OK;
     do {
+        for (int l = 0; l < nlev && rc == SBP_OK; ++l) {
+            Level& L = h->L[l];
+            if (hipMalloc(&L.dtab, sizeof(OpTab)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+            L.tb.dev = L.dtab;
+            if (hipMemcpy(L.dtab, &L.tb, sizeof(OpTab), hipMemcpyHostToDevice) != hipSuccess) rc = SBP_ERR_HIP;
+        }
+        if (rc != SBP_OK) break;
         const size_t n = h->L[0].n;
         if (hipMalloc(&h->r, n * 8) != hipSuccess || hipMalloc(&h->pv, n * 8) != hipSuccess ||
-            hipMalloc(&h->Ap, n * 8) != hipSuccess || hipMalloc(&h->z, n * 8) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+            hipMalloc(&h->Ap, n * 8) != hipSuccess || hipMalloc(&h->z, n * 8) != hipSuccess ||
+            hipMalloc(&h->x, n * 8) != hipSuccess) { rc = SBP_ERR_HIP; break; }
         // Enough partial slots for the finest pass at any compiled tile shape.
         h->npartial = std::max<size_t>(pass_blocks(N, 64, 4), 4096) + 64;
         if (hipMalloc(&h->partial, h->npartial * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
@@ -516,10 +610,12 @@ int sbp_get_info(sbp_handle h, int* N, int* p, int* nlevels) {
     return SBP_OK;
 }
 
-int sbp_set_tiles(sbp_handle h, int level, int ti, int tj) {
-    if (!h || level < 0 || level >= h->nlev || !tile_ok(ti, tj)) return SBP_ERR_INVALID_ARG;
-    h->L[level].ti = ti;
-    h->L[level].tj = tj;
+int sbp_set_tiles(sbp_handle h, int level, int dtype, int ti, int tj) {
+    if (!h || level < 0 || level >= h->nlev || !tile_ok(ti, tj) || (dtype != SBP_F64 && dtype != SBP_F32))
+        return SBP_ERR_INVALID_ARG;
+    if (h->L[level].ti[dtype] != ti || h->L[level].tj[dtype] != tj) drop_graphs(h);   // launch shapes change
+    h->L[level].ti[dtype] = ti;
+    h->L[level].tj[dtype] = tj;
     return SBP_OK;
 }
 
@@ -657,27 +753,18 @@ int sbp_vcycle(sbp_handle h, const void* r, void* z, int dtype) {
 }
 
 
+
 int sbp_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
            double* hist) {
     if (!h || !b || !x || maxiter < 0 || !(abstol >= 0)) return SBP_ERR_INVALID_ARG;
-    STCHK(init_residual(h, b, x, abstol, hist));
-    int it = 0;
-    int rc = pcg_loop<double>(h, x, h->r, h->pv, h->Ap, h->z, h->sc, maxiter, false, hist, &it);
-    if (iters) *iters = it;
-    if (resnorm) *resnorm = h->sc_host->resnorm;
-    return rc;
+    return krylov64(h, b, x, abstol, maxiter, iters, resnorm, hist, false);
 }
 
 int sbp_mgpcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
               double* hist) {
     if (!h || !b || !x || maxiter < 0 || !(abstol >= 0)) return SBP_ERR_INVALID_ARG;
     if (!h->mg) return SBP_ERR_INVALID_SIZE;
-    STCHK(init_residual(h, b, x, abstol, hist));
-    int it = 0;
-    int rc = pcg_loop<double>(h, x, h->r, h->pv, h->Ap, h->z, h->sc, maxiter, true, hist, &it);
-    if (iters) *iters = it;
-    if (resnorm) *resnorm = h->sc_host->resnorm;
-    return rc;
+    return krylov64(h, b, x, abstol, maxiter, iters, resnorm, hist, true);
 }
 
 }  // extern "C"
@@ -738,7 +825,7 @@ extern "C" int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double 
         HIPCHK(hipGetLastError());
         int it = 0;
         int irc = pcg_loop<float>(h, h->d32, h->r32, h->p32, h->Ap32, h->z32, h->sc_in, h->prm.inner_maxiter, true,
-                                  nullptr, &it);
+                                  &it);
         if (irc == SBP_ERR_NONFINITE || irc == SBP_ERR_BREAKDOWN) { rc = irc; break; }
         total_inner += it;
         hipLaunchKernelGGL((add_f32_to_f64<kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, x,
@@ -821,7 +908,7 @@ extern "C" int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, c
         const int eb = ew_blocks(n);
         hipLaunchKernelGGL((update_xr<double, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, x, h->r,
                            (const double*)h->pv, (const double*)h->Ap, (const Scalars*)sc, (size_t)n, h->partial);
-        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(256), 0, st, h->partial, eb, sc, it, nullptr, 1);
+        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(256), 0, st, h->partial, eb, sc, 1);
         hipLaunchKernelGGL((update_p<double, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, h->pv,
                            (const double*)h->r, (const Scalars*)sc, (size_t)n);
         HIPCHK(hipGetLastError());
@@ -837,17 +924,35 @@ extern "C" int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, c
 }
 
 namespace {
-__global__ __launch_bounds__(256) void copy_bytes(const uint4* __restrict__ s, uint4* __restrict__ d, size_t n16) {
+__global__ __launch_bounds__(256) void copy_vec16(const uint4* __restrict__ s, uint4* __restrict__ d, size_t n16) {
     const size_t stride = (size_t)gridDim.x * 256;
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n16; q += stride) d[q] = s[q];
 }
+__global__ __launch_bounds__(256) void copy_u8(const unsigned char* __restrict__ s, unsigned char* __restrict__ d,
+                                              size_t n) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += stride) d[q] = s[q];
+}
 }  // namespace
 
+// Streaming copy: 16-byte vector body when both ends are 16-byte aligned,
+// bytes for the tail (or everything when unaligned).
 extern "C" int sbp_copy(sbp_handle h, const void* src, void* dst, size_t bytes) {
-    if (!h || !src || !dst || (bytes % 16) != 0) return SBP_ERR_INVALID_ARG;
-    const size_t n16 = bytes / 16;
-    const int nb = (int)std::max<size_t>(1, std::min<size_t>((n16 + 255) / 256, 8192));
-    hipLaunchKernelGGL(copy_bytes, dim3(nb), dim3(256), 0, h->st, (const uint4*)src, (uint4*)dst, n16);
-    HIPCHK(hipGetLastError());
+    if (!h || !src || !dst) return SBP_ERR_INVALID_ARG;
+    if (bytes == 0) return SBP_OK;
+    const bool aligned = (((uintptr_t)src | (uintptr_t)dst) & 15) == 0;
+    const size_t n16 = aligned ? bytes / 16 : 0;
+    if (n16) {
+        const int nb = (int)std::max<size_t>(1, std::min<size_t>((n16 + 255) / 256, 8192));
+        hipLaunchKernelGGL(copy_vec16, dim3(nb), dim3(256), 0, h->st, (const uint4*)src, (uint4*)dst, n16);
+        HIPCHK(hipGetLastError());
+    }
+    const size_t done = n16 * 16, rest = bytes - done;
+    if (rest) {
+        const int nb = (int)std::max<size_t>(1, std::min<size_t>((rest + 255) / 256, 8192));
+        hipLaunchKernelGGL(copy_u8, dim3(nb), dim3(256), 0, h->st, (const unsigned char*)src + done,
+                           (unsigned char*)dst + done, rest);
+        HIPCHK(hipGetLastError());
+    }
     return SBP_OK;
 }

@@ -77,7 +77,8 @@ struct Scalars {           // device-resident solver state
     double abstol;
     double resnorm;        // latest ||r||_2
     double bnorm;
-    int iters;
+    double* hist;          // nullable residual-norm history, hist[k] = ||r_k||_2
+    int iters;             // iterations completed (advanced on the device)
     int done;
     int status;
     int pad;
@@ -119,16 +120,18 @@ __global__ __launch_bounds__(256) void krylov_alpha(const double* partial, int n
     }
 }
 
-// rs = ||r||^2; record history; converge test ||r|| < abstol; for CG also beta.
-__global__ __launch_bounds__(256) void krylov_check(const double* partial, int np, Scalars* sc, int it,
-                                                   double* hist, int cg_beta) {
+// rs = ||r||^2; iteration counter and history advance on the device (so a
+// captured iteration replays unchanged); converge test ||r|| < abstol; for CG
+// also beta.
+__global__ __launch_bounds__(256) void krylov_check(const double* partial, int np, Scalars* sc, int cg_beta) {
     if (sc->done) return;
     double rs = sum_partials(partial, np);
     if (threadIdx.x == 0) {
         const double nr = sqrt(rs);
+        const int k = sc->iters + 1;
+        sc->iters = k;
         sc->resnorm = nr;
-        if (hist) hist[it + 1] = nr;
-        sc->iters = it + 1;
+        if (sc->hist) sc->hist[k] = nr;
         if (!isfinite(rs)) { sc->status = 5; sc->done = 1; return; }
         if (nr < sc->abstol) { sc->done = 1; return; }
         if (cg_beta) { sc->beta = rs / sc->rr; }
@@ -155,6 +158,7 @@ __global__ __launch_bounds__(256) void krylov_init(const double* partial, int np
         sc->abstol = abstol;
         sc->rr = rs;
         sc->resnorm = sqrt(rs);
+        sc->hist = hist;
         sc->iters = 0;
         sc->status = 0;
         sc->done = 0;
@@ -392,10 +396,10 @@ __global__ __launch_bounds__(256) void csr_fill(int32_t* rp, int32_t* col, doubl
     int ci[kMaxC + 5]; double vi[kMaxC + 5]; int ni = 0;
     auto row1d = [&](int r, int* cc, double* vv, int& nn, bool is_k) {
         if (r < t.R) {
-            for (int c = 0; c < t.C; ++c) { double v = is_k ? t.kb[r][c] : t.mb[r][c]; if (v != 0.0) { cc[nn] = c; vv[nn++] = v; } }
+            for (int c = 0; c < t.C; ++c) { double v = is_k ? t.dev->kb[r][c] : t.dev->mb[r][c]; if (v != 0.0) { cc[nn] = c; vv[nn++] = v; } }
         } else if (r >= N - t.R) {
             const int rr = N - 1 - r;
-            for (int c = t.C - 1; c >= 0; --c) { double v = is_k ? t.kb[rr][c] : t.mb[rr][c]; if (v != 0.0) { cc[nn] = N - 1 - c; vv[nn++] = v; } }
+            for (int c = t.C - 1; c >= 0; --c) { double v = is_k ? t.dev->kb[rr][c] : t.dev->mb[rr][c]; if (v != 0.0) { cc[nn] = N - 1 - c; vv[nn++] = v; } }
         } else {
             const int H = (t.C == 6) ? 2 : 1;
             for (int s = -H; s <= H; ++s) {

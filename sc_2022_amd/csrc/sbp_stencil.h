@@ -41,8 +41,24 @@ struct OpTab {
     double kb[kMaxR][kMaxC];
     double md[kMaxR];   // mb[r][r]
     double kd[kMaxR];   // kb[r][r]
+    double mrs[kMaxR];  // row sums of mb (exactly 0: m 1 = 0)
+    double krs[kMaxR];  // row sums of kb (the Dirichlet SAT "mass")
     double mid;         // interior diagonal of the 1-D factor (2 or 5/2)
+    double idint;       // 1 / (2 mid): inverse diagonal of A away from the faces
+    // Device-memory copy of this table.  Kernels take the table by value in
+    // the kernel-argument segment for its scalar fields; lookups with a
+    // per-thread row index go through this pointer (an indexed access into
+    // the argument segment would make the compiler copy the table to scratch).
+    const OpTab* dev;
 };
+
+// Every row of the 1-D factors is evaluated in difference form,
+//     sum_c a[r][c] u_c = rowsum_r u_r + sum_{c != r} a[r][c] (u_c - u_r),
+// which is exact algebra but keeps the zero row sums of the interior and of
+// the Neumann-side closure exact in any precision: with rounded FP32
+// coefficients the plain sum leaves a spurious O(1e-8) diagonal term, as
+// large as the smallest eigenvalue of A at N ~ 4k, while differences of
+// neighbouring values of a smooth field are formed almost exactly.
 
 // Interior 1-D stencil s[-H..H] of -d.
 template <int P> struct Stencil;
@@ -61,15 +77,16 @@ __device__ __forceinline__ size_t gidx(int i, int j, int N) { return (size_t)j *
 
 template <typename T>
 __device__ __forceinline__ T hw_of(const OpTab& t, int r) {
-    if (r < t.RH) return (T)t.hw[r];
-    if (r >= t.N - t.RH) return (T)t.hw[t.N - 1 - r];
-    return T(1);
+    const int rr = r < t.N - 1 - r ? r : t.N - 1 - r;   // distance to the nearer face
+    return rr < t.RH ? (T)t.dev->hw[rr] : T(1);
 }
 
 template <typename T>
 __device__ __forceinline__ T diag_of(const OpTab& t, int i, int j) {
-    T kj = (j < t.R) ? (T)t.kd[j] : (j >= t.N - t.R ? (T)t.kd[t.N - 1 - j] : (T)t.mid);
-    T mi = (i < t.R) ? (T)t.md[i] : (i >= t.N - t.R ? (T)t.md[t.N - 1 - i] : (T)t.mid);
+    const int jj = j < t.N - 1 - j ? j : t.N - 1 - j;
+    const int ii = i < t.N - 1 - i ? i : t.N - 1 - i;
+    const T kj = jj < t.R ? (T)t.dev->kd[jj] : (T)t.mid;
+    const T mi = ii < t.R ? (T)t.dev->md[ii] : (T)t.mid;
     return hw_of<T>(t, i) * kj + hw_of<T>(t, j) * mi;
 }
 
@@ -88,6 +105,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 template <typename T> struct InPlain {
     const T* u;
     __device__ void init() {}
+    template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const { return u[gidx(i, j, t.N)]; }
 };
 
@@ -96,9 +114,11 @@ template <typename T> struct InPlain {
 template <typename T> struct InSmooth0 {
     const T* b; T omega; int use_diag;
     __device__ void init() {}
+    template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
         const T v = omega * b[gidx(i, j, t.N)];
-        return use_diag ? v / diag_of<T>(t, i, j) : v;
+        if (!use_diag) return v;
+        return INT ? v * (T)t.idint : v / diag_of<T>(t, i, j);
     }
 };
 
@@ -106,6 +126,7 @@ template <typename T> struct InSmooth0 {
 template <typename T> struct InProlong {
     const T* x; const T* ec; int Nc;
     __device__ void init() {}
+    template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
         const int ic = i >> 1, jc = j >> 1;
         const T* e = ec + gidx(ic, jc, Nc);
@@ -123,6 +144,7 @@ template <typename T> struct InProlong {
 template <typename T> struct InAxpy {
     const T* r; const T* p; const double* beta; T bt;
     __device__ void init() { bt = (T)(*beta); }
+    template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
         const size_t q = gidx(i, j, t.N);
         return r[q] + bt * p[q];
@@ -138,6 +160,7 @@ template <typename T> struct OutStore {            // o = s * Au
     static constexpr bool kDot = false;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    template <bool INT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T, T Au, Aux, T&) const { o[q] = s * Au; }
 };
 
@@ -146,6 +169,7 @@ template <typename T> struct OutStoreDot {         // o = Au ; acc += u Au
     static constexpr bool kDot = true;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    template <bool INT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux, T& acc) const {
         o[q] = Au; acc += u * Au;
     }
@@ -156,6 +180,7 @@ template <typename T> struct OutWriteDot {         // w = u (materialise the for
     static constexpr bool kDot = true;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    template <bool INT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux, T& acc) const {
         w[q] = u; o[q] = Au; acc += u * Au;
     }
@@ -166,6 +191,7 @@ template <typename T, typename TO> struct OutResid {  // r = b - Au (optionally 
     static constexpr bool kDot = true;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t q) const { return b[q]; }
+    template <bool INT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T, T Au, Aux bq, T& acc) const {
         T rv = bq - Au; r[q] = (TO)rv; acc += rv * rv;
     }
@@ -179,9 +205,10 @@ template <typename T> struct OutSmooth {
     static constexpr bool kDot = true;
     struct Aux { T b, d; };
     __device__ Aux pre(const OpTab&, size_t q) const { Aux a; a.b = b[q]; a.d = dotv ? dotv[q] : T(0); return a; }
+    template <bool INT = false>
     __device__ __forceinline__ void emit(const OpTab& t, int i, int j, size_t q, T u, T Au, Aux a, T& acc) const {
         T c = omega * (a.b - Au);
-        if (use_diag) c = c / diag_of<T>(t, i, j);
+        if (use_diag) c = INT ? c * (T)t.idint : c / diag_of<T>(t, i, j);
         const T xn = u + c;
         xo[q] = xn; acc += a.d * xn;
     }
@@ -193,6 +220,7 @@ template <typename T> struct OutCopy {
     static constexpr bool kDot = false;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    template <bool INT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T, Aux, T&) const { o[q] = u; }
 };
 
@@ -225,36 +253,39 @@ __device__ __forceinline__ T block_sum(T v) {
 // ---------------------------------------------------------------------------
 // The stencil pass
 // ---------------------------------------------------------------------------
-// Closure sums via direct (cache-served) loads; used only for face rows/columns.
+// Closure rows via direct (cache-served) loads, used only for face rows and
+// columns; uc is the operand at (i, j).
 template <typename T, class IN>
-__device__ __forceinline__ T k_closure(const OpTab& t, const IN& in, int i, int j) {
-    T s = T(0);
+__device__ __forceinline__ T k_closure(const OpTab& t, const IN& in, int i, int j, T uc) {
     if (j < t.R) {
+        T s = (T)t.dev->krs[j] * uc;
 #pragma unroll
         for (int c = 0; c < kMaxC; ++c)
-            if (c < t.C) s += (T)t.kb[j][c] * in.load(t, i, c);
-    } else {
-        const int r = t.N - 1 - j;
-#pragma unroll
-        for (int c = 0; c < kMaxC; ++c)
-            if (c < t.C) s += (T)t.kb[r][c] * in.load(t, i, t.N - 1 - c);
+            if (c < t.C && c != j) s += (T)t.dev->kb[j][c] * (in.load(t, i, c) - uc);
+        return s;
     }
+    const int r = t.N - 1 - j;
+    T s = (T)t.dev->krs[r] * uc;
+#pragma unroll
+    for (int c = 0; c < kMaxC; ++c)
+        if (c < t.C && c != r) s += (T)t.dev->kb[r][c] * (in.load(t, i, t.N - 1 - c) - uc);
     return s;
 }
 
 template <typename T, class IN>
-__device__ __forceinline__ T m_closure(const OpTab& t, const IN& in, int i, int j) {
-    T s = T(0);
+__device__ __forceinline__ T m_closure(const OpTab& t, const IN& in, int i, int j, T uc) {
     if (i < t.R) {
+        T s = (T)t.dev->mrs[i] * uc;
 #pragma unroll
         for (int c = 0; c < kMaxC; ++c)
-            if (c < t.C) s += (T)t.mb[i][c] * in.load(t, c, j);
-    } else {
-        const int r = t.N - 1 - i;
-#pragma unroll
-        for (int c = 0; c < kMaxC; ++c)
-            if (c < t.C) s += (T)t.mb[r][c] * in.load(t, t.N - 1 - c, j);
+            if (c < t.C && c != i) s += (T)t.dev->mb[i][c] * (in.load(t, c, j) - uc);
+        return s;
     }
+    const int r = t.N - 1 - i;
+    T s = (T)t.dev->mrs[r] * uc;
+#pragma unroll
+    for (int c = 0; c < kMaxC; ++c)
+        if (c < t.C && c != r) s += (T)t.dev->mb[r][c] * (in.load(t, t.N - 1 - c, j) - uc);
     return s;
 }
 
@@ -264,31 +295,72 @@ struct PassCtl {
     int nbx, nby;        // tile grid
 };
 
+// Interior tile: every point of the TI x J tile and its halo lies inside the
+// grid and away from the closure rows, so the pass is straight-line code
+// with no bounds tests (the common case: all but the first/last tile
+// column and row of tiles).
 template <int P, int TI, int J, typename T, class IN, class OUT>
-__global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, PassCtl ctl) {
+__device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const OpTab& t, int i0, int j0,
+                                           T (*tile)[TI + 2 * Stencil<P>::H]) {
     constexpr int H = Stencil<P>::H;
     constexpr int W = J + 2 * H;
-    if (ctl.done && *ctl.done) return;
-    const int nb = ctl.nbx * ctl.nby;
-    const int b = xcd_remap((int)blockIdx.x, nb);
-    const int bx = b % ctl.nbx, by = b / ctl.nbx;
+    const int tid = threadIdx.x;
+    const int i = i0 + tid;
+    const int N = t.N;
+    T w[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) w[k] = in.template load<true>(t, i, j0 - H + k);
+    // 2H halo columns x J rows, spread over the first threads.
+#pragma unroll
+    for (int e0 = 0; e0 < 2 * H * J; e0 += TI) {
+        const int e = e0 + tid;
+        if (e < 2 * H * J) {
+            const int side = e / (H * J), rem = e % (H * J);
+            const int off = rem / J, r = rem % J;
+            const int ii = side == 0 ? i0 - H + off : i0 + TI + off;
+            tile[r][side == 0 ? off : TI + H + off] = in.template load<true>(t, ii, j0 + r);
+        }
+    }
+    typename OUT::Aux aux[J];
+#pragma unroll
+    for (int r = 0; r < J; ++r) {
+        tile[r][tid + H] = w[r + H];
+        aux[r] = out.pre(t, gidx(i, j0 + r, N));
+    }
+    __syncthreads();
+    T acc = T(0);
+#pragma unroll
+    for (int r = 0; r < J; ++r) {
+        const T uc = w[r + H];
+        const T* row = &tile[r][tid + H];
+        T kp = Stencil<P>::template c<T>(1) * ((w[r + H - 1] - uc) + (w[r + H + 1] - uc));
+        T mp = Stencil<P>::template c<T>(1) * ((row[-1] - uc) + (row[1] - uc));
+#pragma unroll
+        for (int s = 2; s <= H; ++s) {
+            kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
+            mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
+        }
+        out.template emit<true>(t, i, j0 + r, gidx(i, j0 + r, N), uc, kp + mp, aux[r], acc);
+    }
+    return acc;
+}
+
+// General tile: bounds tests, face closures and H weights.
+template <int P, int TI, int J, typename T, class IN, class OUT>
+__device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const OpTab& t, int i0, int j0,
+                                          T (*tile)[TI + 2 * Stencil<P>::H]) {
+    constexpr int H = Stencil<P>::H;
+    constexpr int W = J + 2 * H;
     const int tid = threadIdx.x;
     const int N = t.N;
-    const int i0 = bx * TI, i = i0 + tid;
-    const int j0 = by * J;
+    const int i = i0 + tid;
     const bool act = i < N;
-    in.init();
-
-    __shared__ T tile[J][TI + 2 * H];
-
-    // Column window u(i, j0-H .. j0+J-1+H).
     T w[W];
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         const int jj = j0 - H + k;
         w[k] = (act && jj >= 0 && jj < N) ? in.load(t, i, jj) : T(0);
     }
-    // Halo columns of the LDS tile.
     if (tid < 2 * H) {
         const int side = tid / H, off = tid % H;
         const int ii = side == 0 ? i0 - H + off : i0 + TI + off;
@@ -307,7 +379,6 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
         if (act && jj < N) aux[r] = out.pre(t, gidx(i, jj, N));
     }
     __syncthreads();
-
     T acc = T(0);
     if (act) {
         const bool iint = (i >= t.R) && (i < N - t.R);
@@ -317,31 +388,50 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
             const int j = j0 + r;
             if (j >= N) break;
             const bool jint = (j >= t.R) && (j < N - t.R);
+            const T uc = w[r + H];
             T kp, mp;
             if (jint) {
-                kp = Stencil<P>::template c<T>(0) * w[r + H];
+                kp = Stencil<P>::template c<T>(1) * ((w[r + H - 1] - uc) + (w[r + H + 1] - uc));
 #pragma unroll
-                for (int s = 1; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * (w[r + H - s] + w[r + H + s]);
+                for (int s = 2; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
             } else {
-                kp = k_closure<T>(t, in, i, j);
+                kp = k_closure<T>(t, in, i, j, uc);
             }
             if (iint) {
                 const T* row = &tile[r][tid + H];
-                mp = Stencil<P>::template c<T>(0) * row[0];
+                mp = Stencil<P>::template c<T>(1) * ((row[-1] - uc) + (row[1] - uc));
 #pragma unroll
-                for (int s = 1; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * (row[-s] + row[s]);
+                for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
             } else {
-                mp = m_closure<T>(t, in, i, j);
+                mp = m_closure<T>(t, in, i, j, uc);
             }
             const T Au = (iint ? kp : hwi * kp) + (jint ? mp : hw_of<T>(t, j) * mp);
-            out.emit(t, i, j, gidx(i, j, N), w[r + H], Au, aux[r], acc);
+            out.emit(t, i, j, gidx(i, j, N), uc, Au, aux[r], acc);
         }
     }
+    return acc;
+}
+
+template <int P, int TI, int J, typename T, class IN, class OUT>
+__global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, PassCtl ctl) {
+    constexpr int H = Stencil<P>::H;
+    if (ctl.done && *ctl.done) return;
+    const int nb = ctl.nbx * ctl.nby;
+    const int b = xcd_remap((int)blockIdx.x, nb);
+    const int bx = b % ctl.nbx, by = b / ctl.nbx;
+    const int N = t.N;
+    const int i0 = bx * TI, j0 = by * J;
+    in.init();
+    __shared__ T tile[J][TI + 2 * H];
+    // every point the tile loads (halo included) lies outside the closure rows
+    const bool interior = i0 - H >= t.R && i0 + TI + H <= N - t.R && j0 - H >= t.R && j0 + J + H <= N - t.R;
+    const T acc = interior ? pass_interior<P, TI, J, T>(in, out, t, i0, j0, tile)
+                           : pass_general<P, TI, J, T>(in, out, t, i0, j0, tile);
     if constexpr (OUT::kDot) {
         if (ctl.partial) {
             __syncthreads();
             T s = block_sum<T, TI>(acc);
-            if (tid == 0) ctl.partial[b] = (double)s;
+            if (threadIdx.x == 0) ctl.partial[b] = (double)s;
         }
     }
 }

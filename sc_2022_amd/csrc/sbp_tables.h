@@ -115,8 +115,18 @@ inline bool build_optab(int p, int N, OpTab* t, RhsTab* rt) {
             if (m[r * n0 + c] != 0.0 || k[r * n0 + c] != 0.0) return false;
         t->md[r] = m[r * n0 + r];
         t->kd[r] = k[r * n0 + r];
+        double ms = 0.0, ks = 0.0, mmax = 0.0;
+        for (int c = 0; c < C; ++c) {
+            ms += t->mb[r][c];
+            ks += t->kb[r][c];
+            mmax = std::fmax(mmax, std::fabs(t->mb[r][c]));
+        }
+        if (std::fabs(ms) > 1e-12 * mmax) return false;   // m 1 = 0 must hold
+        t->mrs[r] = 0.0;
+        t->krs[r] = ks;
     }
     t->mid = -s.d[p / 2];
+    t->idint = 1.0 / (2.0 * t->mid);
     if (rt) {
         std::memset(rt, 0, sizeof(*rt));
         rt->nbs = s.nbs;

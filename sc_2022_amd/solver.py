@@ -41,7 +41,7 @@ class SbpPoisson:
     """Matrix-free SBP-SAT operator of order p on an N x N grid (N = 2^k+1 for
     multigrid).  Keyword arguments override sbp_params fields."""
 
-    def __init__(self, N: int, p: int = 2, device=None, **params):
+    def __init__(self, N: int, p: int = 2, device=None, tuned: bool = True, **params):
         self.lib = capi.load()
         self.N = int(N)
         self.p = int(p)
@@ -55,6 +55,8 @@ class SbpPoisson:
         nl = C.c_int()
         self.lib.sbp_get_info(self._h, None, None, C.byref(nl))
         self.nlevels = nl.value
+        if tuned and not (params.get("tile_i") or params.get("tile_j")):
+            self.apply_tuned_tiles()
 
     # -- lifecycle ---------------------------------------------------------
     def close(self):
@@ -86,8 +88,22 @@ class SbpPoisson:
             N = (N + 1) // 2
         return N
 
-    def set_tiles(self, level, ti, tj):
-        capi.check(self.lib.sbp_set_tiles(self._h, level, ti, tj), "sbp_set_tiles")
+    def set_tiles(self, level, ti, tj, dtype=None):
+        """Tile shape for `level`; both precisions unless dtype is given."""
+        dts = [capi.SBP_F64, capi.SBP_F32] if dtype is None else [_DT[dtype]]
+        for d in dts:
+            capi.check(self.lib.sbp_set_tiles(self._h, level, d, ti, tj), "sbp_set_tiles")
+
+    def apply_tuned_tiles(self, cache=None):
+        """Applies the autotuner's choices (tiles_gfx950.json) to every level."""
+        from . import autotune
+        c = autotune.load_cache() if cache is None else cache
+        for level in range(max(1, self.nlevels)):
+            Nl = self.level_size(level)
+            for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+                t = autotune.lookup(self.p, tag, Nl, c)
+                if t:
+                    self.set_tiles(level, t[0], t[1], dt)
 
     # -- operator ----------------------------------------------------------
     def apply(self, u, out=None, sign=1, level=0):

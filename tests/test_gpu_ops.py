@@ -73,7 +73,7 @@ def test_apply_golden(SP, golden_dir, p):
 
 
 @pytest.mark.parametrize("p", [2, 4])
-@pytest.mark.parametrize("tiles", [(64, 4), (64, 8), (128, 8), (256, 4), (256, 8), (256, 16)])
+@pytest.mark.parametrize("tiles", [(64, 4), (64, 8), (64, 16), (128, 8), (128, 16), (256, 4), (256, 8), (256, 16)])
 def test_apply_every_tile_shape(SP, p, tiles):
     level = 7
     N = 2 ** level + 1

@@ -166,3 +166,28 @@ def test_status_codes(SP):
     hr = SP(40, 2)
     with pytest.raises(Exception):
         hr.mgpcg(T(np.ones(1600)), abstol=1e-8)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_graph_replay_matches_eager(SP, p):
+    """Iterations replayed from a captured hipGraph give bit-identical results
+    to eager launches, over several solves with eager work in between."""
+    level = 7
+    N = 2 ** level + 1
+    A, b, _, _ = problem(level, p, "fourier")
+    tol = 1e-12 * np.linalg.norm(b)
+    hg = SP(N, p, use_graph=1)
+    he = SP(N, p, use_graph=0)
+    for rep in range(3):
+        xg, rg = hg.mgpcg(T(b), abstol=tol, history=True)
+        xe, re_ = he.mgpcg(T(b), abstol=tol, history=True)
+        assert rg.iters == re_.iters and rg.history == re_.history
+        assert torch.equal(xg, xe)
+        cg_g, r1 = hg.cg(T(b), abstol=1e-6 * np.linalg.norm(b))
+        cg_e, r2 = he.cg(T(b), abstol=1e-6 * np.linalg.norm(b))
+        assert r1.iters == r2.iters and torch.equal(cg_g, cg_e)
+        mg, m1 = hg.mgpcg_mixed(T(b), abstol=tol)
+        me, m2 = he.mgpcg_mixed(T(b), abstol=tol)
+        assert (m1.iters, m1.inner_iters) == (m2.iters, m2.inner_iters) and torch.equal(mg, me)
+        hg.apply(xg)                      # eager work between replays
+        torch.cuda.synchronize()
