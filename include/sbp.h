@@ -59,7 +59,10 @@ typedef struct {
     int max_outer;           /* mixed precision: refinement step cap (default 10) */
     int tile_i;              /* apply tile width in i (0 = autotuned) */
     int tile_j;              /* apply rows per thread (0 = autotuned) */
-    int use_graph;           /* replay each Krylov iteration from a captured hipGraph (default 1) */
+    int use_graph;           /* replay Krylov iterations from a captured hipGraph (default 1) */
+    double ir_rtol;          /* mixed precision: also stop when the correction-based error
+                                estimate ||d_k|| min(1, ||d_k||/||d_k-1||) <= ir_rtol ||x||
+                                (0 = residual test only, the default) */
 } sbp_params;
 
 typedef struct sbp_handle_s* sbp_handle;
@@ -132,10 +135,24 @@ int sbp_mgpcg(sbp_handle h, const double* b, double* x, double abstol, int maxit
               int* iters, double* resnorm, double* hist);
 /* FP32 inner MG-PCG inside FP64 iterative refinement: the outer residual
  * r = b - A x, the update x += d, the norms and the coarse solve are FP64;
- * the inner preconditioned solve of A d = r is FP32.  Stops when the FP64
- * ||r||_2 < abstol.  inner_iters (nullable) receives the total inner count. */
+ * the inner preconditioned solve of A d = r is FP32 (to inner_rtol ||r||).
+ * Stops when the FP64 ||r||_2 < abstol, or (ir_rtol > 0) when the error
+ * estimate from successive corrections drops below ir_rtol ||x||.
+ * outer_iters receives the number of corrections applied, inner_iters
+ * (nullable) the total inner iterations, resnorm the last FP64 residual norm
+ * evaluated. */
 int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double abstol, int max_outer,
                     int* outer_iters, int* inner_iters, double* resnorm);
+
+/* Gradient of L = 1/2 (u - u_obs)' H~ (u - u_obs) with u = A^-1 b(f, g)
+ * (u: the forward solution): the adjoint solve A^T lambda = H~ (u - u_obs)
+ * (FP64 MG-PCG; A is symmetric) to ||r|| < abstol, then dL/df, dL/dg_X from
+ * the transpose of sbp_assemble_rhs.  lam (nullable) receives lambda; loss
+ * (nullable, host) receives L; any gradient output may be NULL.  No
+ * counterpart in the reference (its solves are not differentiated). */
+int sbp_adjoint_grad(sbp_handle h, const double* u, const double* u_obs, double abstol, int maxiter,
+                     double* loss, double* lam, double* df, double* dgW, double* dgE, double* dgS,
+                     double* dgN, int* iters);
 
 /* CSR comparison path (the assembled operator, as CG_GPU_sparse,
  * split_matrix_free.jl:744-769) ------------------------------------------ */

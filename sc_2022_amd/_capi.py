@@ -48,6 +48,7 @@ class SbpParams(C.Structure):
         ("tile_i", C.c_int),
         ("tile_j", C.c_int),
         ("use_graph", C.c_int),
+        ("ir_rtol", C.c_double),
     ]
 
 
@@ -75,6 +76,7 @@ SIGNATURES = {
     "sbp_cg": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd, _vp]),
     "sbp_mgpcg": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd, _vp]),
     "sbp_mgpcg_mixed": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pi, _pd]),
+    "sbp_adjoint_grad": (_i, [_vp, _vp, _vp, _d, _i, _pd, _vp, _vp, _vp, _vp, _vp, _vp, _pi]),
     "sbp_csr_assemble": (_i, [_vp, _vp, _vp, _vp, C.POINTER(C.c_int64)]),
     "sbp_csr_spmv": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sbp_csr_cg": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _pi, _pd]),

@@ -55,8 +55,8 @@ struct sbp_handle_s {
     double* Ainv = nullptr;
     int nc = 0;
     // fine Krylov vectors
-    double *r = nullptr, *pv = nullptr, *Ap = nullptr, *z = nullptr;
-    float *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *z32 = nullptr, *d32 = nullptr;
+    double *r = nullptr, *pv = nullptr, *pv2 = nullptr, *Ap = nullptr, *z = nullptr;
+    float *r32 = nullptr, *p32 = nullptr, *p32b = nullptr, *Ap32 = nullptr, *z32 = nullptr, *d32 = nullptr;
     double* partial = nullptr;
     size_t npartial = 0;
     Scalars* sc = nullptr;      // outer / FP64 solver
@@ -64,6 +64,8 @@ struct sbp_handle_s {
     Scalars* sc_host = nullptr; // pinned mirror
     double* dtmp = nullptr;     // small device scalars
     double* x = nullptr;        // internal FP64 iterate (solvers work on it, then copy out)
+    double* w = nullptr;        // adjoint source (allocated on first use)
+    double* lam = nullptr;      // adjoint state when the caller passes none
     bool have32 = false;
     // Captured Krylov iterations: [precision][0 = CG, 1 = MG-PCG]
     hipStream_t cap = nullptr;  // capture stream (default priority)
@@ -140,6 +142,16 @@ int alloc_level_buffers(sbp_handle h, int prec) {
     return SBP_OK;
 }
 
+template <typename T>
+int launch_rr(int p, const Level& L, const T* u, const T* b, T* rc, int Nc, T scale, const int* done, hipStream_t st) {
+    const int ti = L.ti[prec_index<T>()], tj = L.tj[prec_index<T>()];
+#define SBP_RR_CASE(TI_, J_) \
+    if (ti == TI_ && tj == J_) return launch_resid_restrict<TI_, J_, T>(p, L.tb, u, b, rc, Nc, scale, done, st);
+    SBP_TILE_SHAPES(SBP_RR_CASE)
+#undef SBP_RR_CASE
+    return SBP_ERR_INVALID_ARG;
+}
+
 // ---------------------------------------------------------------------------
 // Smoothing / V-cycle
 // ---------------------------------------------------------------------------
@@ -148,8 +160,13 @@ template <typename T, class IN>
 int smooth_pass(sbp_handle h, const Level& L, const IN& in, const T* b, T* dst, const T* dotv, const int* done,
                 double* partial) {
     const bool rich = h->prm.smoother == 1;
-    OutSmooth<T> o{b, dst, (T)(rich ? h->prm.richardson_omega : h->prm.omega), dotv, rich ? 0 : 1};
-    return launch_pass<T>(h->p, L, in, o, done, partial, h->st, nullptr);
+    const T om = (T)(rich ? h->prm.richardson_omega : h->prm.omega);
+    if (dotv) {
+        OutSmoothDot<T> o{b, dst, om, dotv, rich ? 0 : 1};
+        return launch_pass<T>(h->p, L, in, o, done, partial, h->st, nullptr);
+    }
+    OutSmooth<T> o{b, dst, om, rich ? 0 : 1};
+    return launch_pass<T>(h->p, L, in, o, done, nullptr, h->st, nullptr);
 }
 
 // V-cycle on level l: x_out ~ A_l^-1 b (zero initial guess).  When dotv is
@@ -160,7 +177,8 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const i
     Level& L = h->L[l];
     hipStream_t st = h->st;
     if (l == h->nlev - 1) {
-        hipLaunchKernelGGL((coarse_solve<T>), dim3(1), dim3(320), 0, st, h->Ainv, b, out, (int)L.n, done);
+        hipLaunchKernelGGL((coarse_solve<T>), dim3(((int)L.n + kCoarseRows - 1) / kCoarseRows), dim3(256), 0, st,
+                           h->Ainv, b, out, (int)L.n, done);
         HIPCHK(hipGetLastError());
         if (dotv) {
             hipLaunchKernelGGL((dot_partial<T, kEwThreads>), dim3(1), dim3(kEwThreads), 0, st, dotv, (const T*)out,
@@ -193,15 +211,10 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const i
             }
         }
     }
-    // Residual and restriction.
+    // Residual and full-weighting restriction, fused (the fine residual stays on chip).
     {
-        OutResid<T, T> o{b, rr};
-        STCHK(launch_pass<T>(h->p, L, InPlain<T>{cur}, o, done, nullptr, st, nullptr));
         Level& Lc = h->L[l + 1];
-        dim3 g((Lc.N + 255) / 256, Lc.N);
-        hipLaunchKernelGGL((restrict_fw<T>), g, dim3(256), 0, st, (const T*)rr, (T*)Lc.b[pi], L.N, Lc.N,
-                           (T)h->prm.restrict_scale, done);
-        HIPCHK(hipGetLastError());
+        STCHK(launch_rr<T>(h->p, L, (const T*)cur, b, (T*)Lc.b[pi], Lc.N, (T)h->prm.restrict_scale, done, st));
     }
     Level& Lc = h->L[l + 1];
     T* ec = (T*)Lc.xo[pi];
@@ -265,6 +278,7 @@ int poll(sbp_handle h, Scalars* sc) {
 int finish_status(const Scalars& s, int maxiter_hit) {
     if (s.status == 5) return SBP_ERR_NONFINITE;
     if (s.status == 6) return SBP_ERR_BREAKDOWN;
+    if (s.status == 7) return SBP_ERR_NOT_CONVERGED;
     if (!s.done && maxiter_hit) return SBP_ERR_NOT_CONVERGED;
     return SBP_OK;
 }
@@ -274,6 +288,7 @@ int ensure32(sbp_handle h) {
     const size_t n = h->L[0].n;
     HIPCHK(hipMalloc(&h->r32, n * sizeof(float)));
     HIPCHK(hipMalloc(&h->p32, n * sizeof(float)));
+    HIPCHK(hipMalloc(&h->p32b, n * sizeof(float)));
     HIPCHK(hipMalloc(&h->Ap32, n * sizeof(float)));
     HIPCHK(hipMalloc(&h->z32, n * sizeof(float)));
     HIPCHK(hipMalloc(&h->d32, n * sizeof(float)));
@@ -282,47 +297,59 @@ int ensure32(sbp_handle h) {
     return SBP_OK;
 }
 
-// One Krylov iteration on handle-owned buffers (x is the internal iterate):
-// apply + p'Ap, alpha, x/r update + r'r, convergence test, preconditioner,
-// beta, p update.  Every kernel tests the device "done" flag.
+// Handle-owned Krylov work vectors of one precision.
 template <typename T>
-int issue_iteration(sbp_handle h, T* x, T* r, T* pv, T* Ap, T* z, Scalars* sc, bool use_mg) {
+struct KVecs {
+    T* x;      // iterate
+    T* r;      // residual
+    T* p[2];   // search directions (ping-pong: iteration k writes p[k % 2])
+    T* Ap;
+    T* z;      // preconditioned residual
+};
+
+// Iteration k (1-based, k % 2 == par): p_k = z_{k-1} + beta p_{k-1} formed on
+// the fly inside the apply, which also writes A p_k, accumulates p_k'A p_k and
+// applies the pending x += alpha_{k-1} p_{k-1}; then alpha, r -= alpha A p_k
+// with r'r, the convergence test, and (MG) z = V(r) with r'z and beta.
+// Every kernel tests the device "done" flag.
+template <typename T>
+int issue_iteration(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, int par) {
     hipStream_t st = h->st;
     const Level& L0 = h->L[0];
     const size_t n = L0.n;
     const int* done = &sc->done;
+    T* pold = v.p[par ^ 1];
+    T* pnew = v.p[par];
+    const T* zz = use_mg ? v.z : v.r;
     int np = 0;
-    STCHK(launch_pass<T>(h->p, L0, InPlain<T>{pv}, OutStoreDot<T>{Ap}, done, h->partial, st, &np));
-    hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(256), 0, st, h->partial, np, sc, use_mg ? 1 : 0);
+    STCHK(launch_pass<T>(h->p, L0, InAxpy<T>{zz, pold, &sc->beta, T(0)},
+                         OutPUpd<T>{pnew, v.Ap, v.x, pold, &sc->alpha, T(0)}, done, h->partial, st, &np));
+    hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, use_mg ? 1 : 0);
     const int eb = ew_blocks(n);
-    hipLaunchKernelGGL((update_xr<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, x, r, (const T*)pv,
-                       (const T*)Ap, (const Scalars*)sc, n, h->partial);
-    hipLaunchKernelGGL(krylov_check, dim3(1), dim3(256), 0, st, h->partial, eb, sc, use_mg ? 0 : 1);
+    hipLaunchKernelGGL((update_r<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, v.r, (const T*)v.Ap,
+                       (const Scalars*)sc, n, h->partial);
+    hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, eb, sc, use_mg ? 0 : 1);
     HIPCHK(hipGetLastError());
     if (use_mg) {
-        STCHK(vcycle_level<T>(h, 0, r, z, r, done, h->partial));
-        hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(256), 0, st, h->partial, vcycle_nparts<T>(h), sc);
-        hipLaunchKernelGGL((update_p<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, pv, (const T*)z,
-                           (const Scalars*)sc, n);
-    } else {
-        hipLaunchKernelGGL((update_p<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, pv, (const T*)r,
-                           (const Scalars*)sc, n);
+        STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, done, h->partial));
+        hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(kRedThreads), 0, st, h->partial, vcycle_nparts<T>(h), sc);
+        HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipGetLastError());
     return SBP_OK;
 }
 
-// Captures issue_iteration once per (precision, solver) into a hipGraph on
-// the handle's own capture stream; replays go to the handle's stream.
+// Captures two iterations (both parities) once per (precision, solver) into a
+// hipGraph on the handle's own capture stream; replays go to the handle's stream.
 template <typename T>
-int iteration_graph(sbp_handle h, T* x, T* r, T* pv, T* Ap, T* z, Scalars* sc, bool use_mg, hipGraphExec_t* out) {
+int iteration_graph(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, hipGraphExec_t* out) {
     const int a = prec_index<T>(), b = use_mg ? 1 : 0;
     if (!h->gexec[a][b]) {
         if (!h->cap) HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
         HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
         hipStream_t saved = h->st;
         h->st = h->cap;
-        const int rc = issue_iteration<T>(h, x, r, pv, Ap, z, sc, use_mg);
+        int rc = issue_iteration<T>(h, v, sc, use_mg, 1);
+        if (rc == SBP_OK) rc = issue_iteration<T>(h, v, sc, use_mg, 0);
         h->st = saved;
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(h->cap, &g);
@@ -341,39 +368,55 @@ int iteration_graph(sbp_handle h, T* x, T* r, T* pv, T* Ap, T* z, Scalars* sc, b
 }
 
 // Preconditioned CG body shared by FP64 and FP32: solves A x = b given the
-// initial residual in r (sc->rr / abstol / done already set by the caller's
-// init kernel); the preconditioner is the V-cycle when use_mg, else identity.
+// initial residual in v.r (sc->rr / abstol / maxiter / done already set by the
+// caller's init kernel); the preconditioner is the V-cycle when use_mg, else
+// the identity.  The host polls every poll_every iterations.
 template <typename T>
-int pcg_loop(sbp_handle h, T* x, T* r, T* pv, T* Ap, T* z, Scalars* sc, int maxiter, bool use_mg,
-             int* iters_out) {
+int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use_mg, int* iters_out) {
     hipStream_t st = h->st;
     const size_t n = h->L[0].n;
     const int* done = &sc->done;
+    const T* z0 = v.r;
     if (use_mg) {
-        STCHK(vcycle_level<T>(h, 0, r, z, r, done, h->partial));
-        hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(256), 0, st, h->partial, vcycle_nparts<T>(h), sc);
+        STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, done, h->partial));
+        hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(kRedThreads), 0, st, h->partial, vcycle_nparts<T>(h), sc);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, (const T*)z, pv, n,
-                           done);
-    } else {
-        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, (const T*)r, pv, n,
-                           done);
+        z0 = v.z;
     }
+    // p_0 := z_0 (any finite values serve: alpha = beta = 0 in the first pass)
+    hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, z0, v.p[0], n, done);
     HIPCHK(hipGetLastError());
     hipGraphExec_t ex = nullptr;
-    if (h->prm.use_graph) STCHK(iteration_graph<T>(h, x, r, pv, Ap, z, sc, use_mg, &ex));
-    const int poll_every = std::max(1, h->prm.poll_every);
-    for (int it = 0; it < maxiter; ++it) {
-        if (ex) HIPCHK(hipGraphLaunch(ex, st));
-        else STCHK(issue_iteration<T>(h, x, r, pv, Ap, z, sc, use_mg));
-        if ((it + 1) % poll_every == 0 && it + 1 < maxiter) {
+    if (h->prm.use_graph) STCHK(iteration_graph<T>(h, v, sc, use_mg, &ex));
+    const int poll_every = std::max(2, h->prm.poll_every + (h->prm.poll_every & 1));
+    for (int it = 0; it < maxiter; it += 2) {
+        if (ex) {
+            HIPCHK(hipGraphLaunch(ex, st));
+        } else {
+            STCHK(issue_iteration<T>(h, v, sc, use_mg, 1));
+            STCHK(issue_iteration<T>(h, v, sc, use_mg, 0));
+        }
+        if ((it + 2) % poll_every == 0 && it + 2 < maxiter) {
             STCHK(poll(h, sc));
             if (h->sc_host->done) break;
         }
     }
+    hipLaunchKernelGGL((flush_x<T, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, v.x,
+                       (const T*)v.p[0], (const T*)v.p[1], (const Scalars*)sc, n);
+    HIPCHK(hipGetLastError());
     STCHK(poll(h, sc));
     if (iters_out) *iters_out = h->sc_host->iters;
     return finish_status(*h->sc_host, !h->sc_host->done);
+}
+
+template <typename T> KVecs<T> kvecs(sbp_handle h);
+template <> KVecs<double> kvecs<double>(sbp_handle h) {
+    KVecs<double> v{h->x, h->r, {h->pv, h->pv2}, h->Ap, h->z};
+    return v;
+}
+template <> KVecs<float> kvecs<float>(sbp_handle h) {
+    KVecs<float> v{h->d32, h->r32, {h->p32, h->p32b}, h->Ap32, h->z32};
+    return v;
 }
 
 }  // namespace
@@ -406,11 +449,11 @@ int smooth_t(sbp_handle h, int level, const T* b, T* x, int nsweeps) {
 
 namespace {
 // r = b - A x ; rr / ||r0|| check into sc.
-int init_residual(sbp_handle h, const double* b, const double* x, double abstol, double* hist) {
+int init_residual(sbp_handle h, const double* b, const double* x, double abstol, double* hist, int maxiter) {
     int np = 0;
     STCHK(launch_pass<double>(h->p, h->L[0], InPlain<double>{x}, OutResid<double, double>{b, h->r}, nullptr,
                               h->partial, h->st, &np));
-    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(256), 0, h->st, h->partial, np, h->sc, abstol, hist);
+    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(kRedThreads), 0, h->st, h->partial, np, h->sc, abstol, hist, maxiter);
     HIPCHK(hipGetLastError());
     return SBP_OK;
 }
@@ -424,9 +467,9 @@ int krylov64(sbp_handle h, const double* b, double* x, double abstol, int maxite
     hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, h->st,
                        (const double*)x, h->x, n, nullptr);
     HIPCHK(hipGetLastError());
-    STCHK(init_residual(h, b, h->x, abstol, hist));
+    STCHK(init_residual(h, b, h->x, abstol, hist, maxiter));
     int it = 0;
-    int rc = pcg_loop<double>(h, h->x, h->r, h->pv, h->Ap, h->z, h->sc, maxiter, use_mg, &it);
+    int rc = pcg_loop<double>(h, kvecs<double>(h), h->sc, maxiter, use_mg, &it);
     hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, h->st,
                        (const double*)h->x, x, n, nullptr);
     HIPCHK(hipGetLastError());
@@ -476,6 +519,7 @@ void sbp_default_params(sbp_params* prm) {
     prm->tile_i = 0;
     prm->tile_j = 0;
     prm->use_graph = 1;
+    prm->ir_rtol = 0.0;
 }
 
 int sbp_destroy(sbp_handle h) {
@@ -493,7 +537,7 @@ int sbp_destroy(sbp_handle h) {
     }
     drop_graphs(h);
     if (h->cap) (void)hipStreamDestroy(h->cap);
-    void* ptrs[] = {h->Ainv, h->r, h->pv, h->Ap, h->z, h->x, h->r32, h->p32, h->Ap32, h->z32, h->d32,
+    void* ptrs[] = {h->w, h->lam, h->Ainv, h->r, h->pv, h->pv2, h->Ap, h->z, h->x, h->r32, h->p32, h->p32b, h->Ap32, h->z32, h->d32,
                     h->partial, h->sc, h->sc_in, h->dtmp};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -567,7 +611,7 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) {
         const size_t n = h->L[0].n;
         if (hipMalloc(&h->r, n * 8) != hipSuccess || hipMalloc(&h->pv, n * 8) != hipSuccess ||
             hipMalloc(&h->Ap, n * 8) != hipSuccess || hipMalloc(&h->z, n * 8) != hipSuccess ||
-            hipMalloc(&h->x, n * 8) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+            hipMalloc(&h->x, n * 8) != hipSuccess || hipMalloc(&h->pv2, n * 8) != hipSuccess) { rc = SBP_ERR_HIP; break; }
         // Enough partial slots for the finest pass at any compiled tile shape.
         h->npartial = std::max<size_t>(pass_blocks(N, 64, 4), 4096) + 64;
         if (hipMalloc(&h->partial, h->npartial * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
@@ -635,7 +679,7 @@ int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot) {
     if (!h || !u || !Au || !dot) return SBP_ERR_INVALID_ARG;
     int np = 0;
     STCHK(launch_pass<double>(h->p, h->L[0], InPlain<double>{u}, OutStoreDot<double>{Au}, nullptr, h->partial, h->st, &np));
-    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(256), 0, h->st, h->partial, np, h->dtmp, 0);
+    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(kRedThreads), 0, h->st, h->partial, np, h->dtmp, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(dot, h->dtmp, sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
@@ -678,7 +722,7 @@ int sbp_hnorm(sbp_handle h, const double* v, double* out) {
     const int nb = ew_blocks(h->L[0].n);
     const double hh = 1.0 / (h->N - 1);
     hipLaunchKernelGGL((hnorm_partial<256>), dim3(nb), dim3(256), 0, h->st, v, h->L[0].tb, hh * hh, h->partial);
-    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(256), 0, h->st, h->partial, nb, h->dtmp, 1);
+    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(kRedThreads), 0, h->st, h->partial, nb, h->dtmp, 1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, h->dtmp, sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
@@ -689,7 +733,7 @@ int sbp_dot(sbp_handle h, const double* u, const double* v, double* out) {
     if (!h || !u || !v || !out) return SBP_ERR_INVALID_ARG;
     const int nb = ew_blocks(h->L[0].n);
     hipLaunchKernelGGL((dot_partial<double, 256>), dim3(nb), dim3(256), 0, h->st, u, v, h->L[0].n, h->partial, nullptr);
-    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(256), 0, h->st, h->partial, nb, h->dtmp, 0);
+    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(kRedThreads), 0, h->st, h->partial, nb, h->dtmp, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, h->dtmp, sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
@@ -737,7 +781,8 @@ int sbp_smooth(sbp_handle h, int level, const void* b, void* x, int nsweeps, int
 
 int sbp_coarse_solve(sbp_handle h, const double* b, double* x) {
     if (!h || !b || !x || !h->mg) return SBP_ERR_INVALID_ARG;
-    hipLaunchKernelGGL((coarse_solve<double>), dim3(1), dim3(320), 0, h->st, h->Ainv, b, x, h->nc, nullptr);
+    hipLaunchKernelGGL((coarse_solve<double>), dim3((h->nc + kCoarseRows - 1) / kCoarseRows), dim3(256), 0, h->st,
+                       h->Ainv, b, x, h->nc, nullptr);
     HIPCHK(hipGetLastError());
     return SBP_OK;
 }
@@ -771,9 +816,9 @@ int sbp_mgpcg(sbp_handle h, const double* b, double* x, double abstol, int maxit
 
 namespace {
 // Outer refinement check: resnorm = ||r||; converged -> both done; else arm
-// the inner solver with abstol = inner_rtol * ||r||.
-__global__ __launch_bounds__(256) void ir_check(const double* partial, int np, Scalars* so, Scalars* si,
-                                               double inner_rtol, int outer) {
+// the inner solver with abstol = inner_rtol * ||r|| and its iteration cap.
+__global__ __launch_bounds__(1024) void ir_check(const double* partial, int np, Scalars* so, Scalars* si,
+                                               double inner_rtol, int inner_maxiter) {
     double rs = sum_partials(partial, np);
     if (threadIdx.x == 0) {
         const double nr = sqrt(rs);
@@ -784,9 +829,11 @@ __global__ __launch_bounds__(256) void ir_check(const double* partial, int np, S
         si->alpha = si->beta = 0.0;
         si->rr = rs;
         si->resnorm = nr;
+        si->hist = nullptr;
+        si->maxiter = inner_maxiter;
+        if (so->done) { si->done = 1; return; }
         if (!isfinite(rs)) { so->status = 5; so->done = 1; si->done = 1; return; }
         if (nr < so->abstol) { so->done = 1; si->done = 1; return; }
-        so->iters = outer + 1;
         si->abstol = inner_rtol * nr;
         si->done = 0;
     }
@@ -797,6 +844,51 @@ __global__ void ir_init(Scalars* so, double abstol) {
     so->done = 0;
     so->status = 0;
     so->iters = 0;
+    so->dprev = 0.0;
+    so->est = 0.0;
+}
+
+// x64 += (double) d32 with block partials of ||d||^2 (slot 2b) and ||x||^2 (slot 2b+1).
+template <int NT>
+__global__ __launch_bounds__(NT) void ir_update(double* __restrict__ x, const float* __restrict__ d, size_t n,
+                                               double* partial) {
+    double dd = 0.0, xx = 0.0;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) {
+        const double dv = (double)d[q];
+        const double xv = x[q] + dv;
+        x[q] = xv;
+        dd += dv * dv;
+        xx += xv * xv;
+    }
+    const double sd = block_sum<double, NT>(dd);
+    __syncthreads();
+    const double sx = block_sum<double, NT>(xx);
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = sd; partial[2 * blockIdx.x + 1] = sx; }
+}
+
+// Error estimate after correction k: ||e_k|| ~ ||d_k|| * min(1, ||d_k|| / ||d_{k-1}||)
+// (the observed contraction of the refinement).  Stops the outer loop when
+// est <= ir_rtol * ||x|| (ir_rtol > 0).
+__global__ __launch_bounds__(1024) void ir_estimate(const double* partial, int nb, Scalars* so, double ir_rtol) {
+    __shared__ double red[2][16];
+    double sd = 0.0, sx = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) { sd += partial[2 * k]; sx += partial[2 * k + 1]; }
+    sd = wave_sum(sd);
+    sx = wave_sum(sx);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[0][w] = sd; red[1][w] = sx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double td = 0.0, tx = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { td += red[0][k]; tx += red[1][k]; }
+        const double dn = sqrt(td), xn = sqrt(tx);
+        so->iters += 1;
+        const double ratio = so->dprev > 0.0 ? fmin(1.0, dn / so->dprev) : 1.0;
+        so->est = dn * ratio;
+        so->dprev = dn;
+        if (ir_rtol > 0.0 && so->iters >= 2 && so->est <= ir_rtol * xn) so->done = 1;
+    }
 }
 }  // namespace
 
@@ -807,29 +899,33 @@ extern "C" int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double 
     STCHK(ensure32(h));
     hipStream_t st = h->st;
     const size_t n = h->L[0].n;
+    const int ub = std::min<int>(ew_blocks(n), (int)(h->npartial / 2));
     hipLaunchKernelGGL(ir_init, dim3(1), dim3(1), 0, st, h->sc, abstol);
     HIPCHK(hipGetLastError());
     int total_inner = 0;
     int rc = SBP_OK;
     for (int o = 0; o <= max_outer; ++o) {
         int np = 0;
-        // FP64 residual, cast to FP32 for the inner solve.
+        // FP64 residual r = b - A x, cast to FP32 for the inner solve.
         STCHK(launch_pass<double>(h->p, h->L[0], InPlain<double>{x}, OutResid<double, float>{b, h->r32}, &h->sc->done,
                                   h->partial, st, &np));
-        hipLaunchKernelGGL(ir_check, dim3(1), dim3(256), 0, st, h->partial, np, h->sc, h->sc_in, h->prm.inner_rtol, o);
+        hipLaunchKernelGGL(ir_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, h->sc, h->sc_in, h->prm.inner_rtol,
+                           h->prm.inner_maxiter);
         HIPCHK(hipGetLastError());
         STCHK(poll(h, h->sc));
         if (h->sc_host->done || o == max_outer) break;
+        // Inner FP32 MG-PCG on A d = r from d = 0.
         hipLaunchKernelGGL((fill<float, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, h->d32, 0.0f, n,
                            &h->sc_in->done);
         HIPCHK(hipGetLastError());
         int it = 0;
-        int irc = pcg_loop<float>(h, h->d32, h->r32, h->p32, h->Ap32, h->z32, h->sc_in, h->prm.inner_maxiter, true,
-                                  &it);
+        const int irc = pcg_loop<float>(h, kvecs<float>(h), h->sc_in, h->prm.inner_maxiter, true, &it);
         if (irc == SBP_ERR_NONFINITE || irc == SBP_ERR_BREAKDOWN) { rc = irc; break; }
         total_inner += it;
-        hipLaunchKernelGGL((add_f32_to_f64<kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, x,
-                           (const float*)h->d32, n, nullptr);
+        // FP64 update and the error estimate from the correction norms.
+        hipLaunchKernelGGL((ir_update<kEwThreads>), dim3(ub), dim3(kEwThreads), 0, st, x, (const float*)h->d32, n,
+                           h->partial);
+        hipLaunchKernelGGL(ir_estimate, dim3(1), dim3(kRedThreads), 0, st, h->partial, ub, h->sc, h->prm.ir_rtol);
         HIPCHK(hipGetLastError());
     }
     STCHK(poll(h, h->sc));
@@ -839,6 +935,39 @@ extern "C" int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double 
     if (rc != SBP_OK) return rc;
     if (h->sc_host->status == 5) return SBP_ERR_NONFINITE;
     return h->sc_host->done ? SBP_OK : SBP_ERR_NOT_CONVERGED;
+}
+
+// Adjoint gradient: L = 1/2 (u - u_obs)' H~ (u - u_obs), u = A^-1 b(f, g).
+extern "C" int sbp_adjoint_grad(sbp_handle h, const double* u, const double* u_obs, double abstol, int maxiter,
+                                double* loss, double* lam, double* df, double* dgW, double* dgE, double* dgS,
+                                double* dgN, int* iters) {
+    if (!h || !u || !u_obs || maxiter < 0 || !(abstol >= 0)) return SBP_ERR_INVALID_ARG;
+    if (!h->mg) return SBP_ERR_INVALID_SIZE;
+    const size_t n = h->L[0].n;
+    if (!h->w) HIPCHK(hipMalloc(&h->w, n * sizeof(double)));
+    double* lm = lam;
+    if (!lm) {
+        if (!h->lam) HIPCHK(hipMalloc(&h->lam, n * sizeof(double)));
+        lm = h->lam;
+    }
+    const int nb = ew_blocks(n);
+    const double hh = 1.0 / (h->N - 1);
+    hipLaunchKernelGGL(adjoint_source, dim3(nb), dim3(256), 0, h->st, u, u_obs, h->w, h->L[0].tb, hh * hh, h->partial);
+    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(kRedThreads), 0, h->st, h->partial, nb, h->dtmp, 0);
+    hipLaunchKernelGGL((fill<double, kEwThreads>), dim3(nb), dim3(kEwThreads), 0, h->st, lm, 0.0, n, nullptr);
+    HIPCHK(hipGetLastError());
+    if (loss) {
+        HIPCHK(hipMemcpyAsync(loss, h->dtmp, sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+        *loss *= 0.5;
+    }
+    // A is symmetric, so the adjoint solve A^T lambda = w is the forward MG-PCG.
+    int it = 0;
+    double rn = 0.0;
+    const int rc = krylov64(h, h->w, lm, abstol, maxiter, &it, &rn, nullptr, true);
+    if (iters) *iters = it;
+    if (rc != SBP_OK) return rc;
+    return sbp_rhs_transpose(h, lm, df, dgW, dgE, dgS, dgN);
 }
 
 // ---------------------------------------------------------------------------
@@ -896,7 +1025,7 @@ extern "C" int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, c
     Scalars* sc = h->sc;
     // r = b - A x with block partials of r'r
     hipLaunchKernelGGL(csr_resid, dim3(nbs), dim3(256), 0, st, rp, col, val, x, b, h->r, n, h->partial);
-    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(256), 0, st, h->partial, nbs, sc, abstol, nullptr);
+    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(kRedThreads), 0, st, h->partial, nbs, sc, abstol, nullptr, maxiter);
     hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st,
                        (const double*)h->r, h->pv, (size_t)n, &sc->done);
     HIPCHK(hipGetLastError());
@@ -904,11 +1033,11 @@ extern "C" int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, c
     for (int it = 0; it < maxiter; ++it) {
         hipLaunchKernelGGL(csr_spmv, dim3(nbs), dim3(256), 0, st, rp, col, val, (const double*)h->pv, h->Ap, n,
                            h->partial, &sc->done);
-        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(256), 0, st, h->partial, nbs, sc, 0);
+        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, nbs, sc, 0);
         const int eb = ew_blocks(n);
         hipLaunchKernelGGL((update_xr<double, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, x, h->r,
                            (const double*)h->pv, (const double*)h->Ap, (const Scalars*)sc, (size_t)n, h->partial);
-        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(256), 0, st, h->partial, eb, sc, 1);
+        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, eb, sc, 1);
         hipLaunchKernelGGL((update_p<double, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, h->pv,
                            (const double*)h->r, (const Scalars*)sc, (size_t)n);
         HIPCHK(hipGetLastError());

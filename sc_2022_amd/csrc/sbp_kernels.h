@@ -51,18 +51,27 @@ __global__ __launch_bounds__(256) void prolong_add(const T* __restrict__ ec, T* 
 }
 
 // Coarsest-level exact solve x = Ainv b with the FP64 inverse (symmetric,
-// stored n x n).  One workgroup; b is staged in LDS.
+// stored n x n).  Blocks of 4 waves own kCoarseRows consecutive rows; each
+// wave forms its rows' dot products across its 64 lanes (coalesced row
+// reads, all loads of a row issued together), then reduces across lanes.
+constexpr int kCoarseRows = 8;
 template <typename T>
-__global__ __launch_bounds__(1024) void coarse_solve(const double* __restrict__ Ainv, const T* __restrict__ b,
-                                                    T* __restrict__ x, int n, const int* done) {
+__global__ __launch_bounds__(256) void coarse_solve(const double* __restrict__ Ainv, const T* __restrict__ b,
+                                                   T* __restrict__ x, int n, const int* done) {
     if (done && *done) return;
     __shared__ double bs[1089];   // up to 33 x 33
     for (int k = threadIdx.x; k < n; k += blockDim.x) bs[k] = (double)b[k];
     __syncthreads();
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int rr = 0; rr < kCoarseRows / 4; ++rr) {
+        const int r = blockIdx.x * kCoarseRows + rr * 4 + wv;
+        if (r >= n) break;
+        const double* row = Ainv + (size_t)r * n;
         double s = 0.0;
-        for (int k = 0; k < n; ++k) s += Ainv[(size_t)k * n + r] * bs[k];
-        x[r] = (T)s;
+        for (int k = lane; k < n; k += 64) s += row[k] * bs[k];
+        s = wave_sum(s);
+        if (lane == 0) x[r] = (T)s;
     }
 }
 
@@ -78,35 +87,50 @@ struct Scalars {           // device-resident solver state
     double resnorm;        // latest ||r||_2
     double bnorm;
     double* hist;          // nullable residual-norm history, hist[k] = ||r_k||_2
+    double dprev;          // refinement: norm of the previous correction
+    double est;            // refinement: a-posteriori error estimate
+    int maxiter;           // iteration cap (tested on the device)
     int iters;             // iterations completed (advanced on the device)
     int done;
     int status;
     int pad;
 };
 
-// Deterministic single-workgroup sum of the block partials.
+// Deterministic single-workgroup sum of the block partials (fixed order:
+// thread t sums t, t+NT, ... with four independent accumulators, then a
+// tree across lanes and waves).  Launched with kRedThreads threads.
+constexpr int kRedThreads = 1024;
 __device__ __forceinline__ double sum_partials(const double* partial, int np) {
-    __shared__ double red[16];
-    double s = 0.0;
-    for (int k = threadIdx.x; k < np; k += blockDim.x) s += partial[k];
+    __shared__ double red[kRedThreads / 64];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = threadIdx.x;
+    const int st = blockDim.x;
+    for (; k + 3 * st < np; k += 4 * st) {
+        a0 += partial[k];
+        a1 += partial[k + st];
+        a2 += partial[k + 2 * st];
+        a3 += partial[k + 3 * st];
+    }
+    for (; k < np; k += st) a0 += partial[k];
+    double s = (a0 + a1) + (a2 + a3);
     s = wave_sum(s);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) red[w] = s;
     __syncthreads();
     double tot = 0.0;
     if (threadIdx.x == 0)
-        for (int k = 0; k < (int)((blockDim.x + 63) / 64); ++k) tot += red[k];
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) tot += red[q];
     return tot;   // valid on thread 0
 }
 
 // Stage 2 of a plain dot product: out = sum.
-__global__ __launch_bounds__(256) void reduce_to(const double* partial, int np, double* out, int take_sqrt) {
+__global__ __launch_bounds__(1024) void reduce_to(const double* partial, int np, double* out, int take_sqrt) {
     double s = sum_partials(partial, np);
     if (threadIdx.x == 0) *out = take_sqrt ? sqrt(s) : s;
 }
 
 // alpha = num / pAp where num = rr (CG, use_rz = 0) or rz (PCG, use_rz = 1).
-__global__ __launch_bounds__(256) void krylov_alpha(const double* partial, int np, Scalars* sc, int use_rz) {
+__global__ __launch_bounds__(1024) void krylov_alpha(const double* partial, int np, Scalars* sc, int use_rz) {
     if (sc->done) return;
     double pAp = sum_partials(partial, np);
     if (threadIdx.x == 0) {
@@ -123,7 +147,7 @@ __global__ __launch_bounds__(256) void krylov_alpha(const double* partial, int n
 // rs = ||r||^2; iteration counter and history advance on the device (so a
 // captured iteration replays unchanged); converge test ||r|| < abstol; for CG
 // also beta.
-__global__ __launch_bounds__(256) void krylov_check(const double* partial, int np, Scalars* sc, int cg_beta) {
+__global__ __launch_bounds__(1024) void krylov_check(const double* partial, int np, Scalars* sc, int cg_beta) {
     if (sc->done) return;
     double rs = sum_partials(partial, np);
     if (threadIdx.x == 0) {
@@ -134,13 +158,14 @@ __global__ __launch_bounds__(256) void krylov_check(const double* partial, int n
         if (sc->hist) sc->hist[k] = nr;
         if (!isfinite(rs)) { sc->status = 5; sc->done = 1; return; }
         if (nr < sc->abstol) { sc->done = 1; return; }
+        if (k >= sc->maxiter) { sc->status = 7; sc->done = 1; return; }
         if (cg_beta) { sc->beta = rs / sc->rr; }
         sc->rr = rs;
     }
 }
 
 // rz_new = r'z; beta = rz_new / rz.
-__global__ __launch_bounds__(256) void krylov_beta(const double* partial, int np, Scalars* sc) {
+__global__ __launch_bounds__(1024) void krylov_beta(const double* partial, int np, Scalars* sc) {
     if (sc->done) return;
     double rz = sum_partials(partial, np);
     if (threadIdx.x == 0) {
@@ -150,12 +175,13 @@ __global__ __launch_bounds__(256) void krylov_beta(const double* partial, int np
     }
 }
 
-// Initial state: rr = sum partial; rz optional second set; test ||r0|| < abstol.
-__global__ __launch_bounds__(256) void krylov_init(const double* partial, int np, Scalars* sc, double abstol,
-                                                  double* hist) {
+// Initial state: rr = sum partial; test ||r0|| < abstol (or maxiter = 0).
+__global__ __launch_bounds__(1024) void krylov_init(const double* partial, int np, Scalars* sc, double abstol,
+                                                  double* hist, int maxiter) {
     double rs = sum_partials(partial, np);
     if (threadIdx.x == 0) {
         sc->abstol = abstol;
+        sc->maxiter = maxiter;
         sc->rr = rs;
         sc->resnorm = sqrt(rs);
         sc->hist = hist;
@@ -165,12 +191,13 @@ __global__ __launch_bounds__(256) void krylov_init(const double* partial, int np
         sc->alpha = sc->beta = 0.0;
         if (hist) hist[0] = sqrt(rs);
         if (!isfinite(rs)) { sc->status = 5; sc->done = 1; return; }
-        if (sqrt(rs) < abstol) sc->done = 1;
+        if (sqrt(rs) < abstol) { sc->done = 1; return; }
+        if (maxiter <= 0) { sc->status = 7; sc->done = 1; }
     }
 }
 
 // rz0 = r'z (stage 2) without touching done.
-__global__ __launch_bounds__(256) void krylov_set_rz(const double* partial, int np, Scalars* sc) {
+__global__ __launch_bounds__(1024) void krylov_set_rz(const double* partial, int np, Scalars* sc) {
     if (sc->done) return;
     double rz = sum_partials(partial, np);
     if (threadIdx.x == 0) {
@@ -196,6 +223,38 @@ __global__ __launch_bounds__(NT) void update_xr(T* __restrict__ x, T* __restrict
     }
     T s = block_sum<T, NT>(acc);
     if (threadIdx.x == 0) partial[blockIdx.x] = (double)s;
+}
+
+// r -= alpha Ap ; partial = block sums of r'r.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void update_r(T* __restrict__ r, const T* __restrict__ Ap, const Scalars* sc, size_t n,
+                                              double* partial) {
+    if (sc->done) return;
+    const T a = (T)sc->alpha;
+    T acc = T(0);
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) {
+        const T rv = r[q] - a * Ap[q];
+        r[q] = rv;
+        acc += rv * rv;
+    }
+    T s = block_sum<T, NT>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = (double)s;
+}
+
+// The update x += alpha_K p_K still pending after the last iteration K (the
+// fused search-direction pass of iteration K+1 would have applied it).
+// p_K sits in pbuf[K % 2]; nothing is pending after a breakdown or when no
+// iteration ran.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void flush_x(T* __restrict__ x, const T* __restrict__ p0, const T* __restrict__ p1,
+                                             const Scalars* sc, size_t n) {
+    const int K = sc->iters;
+    if (K < 1 || sc->status == 5 || sc->status == 6) return;
+    const T* pk = (K & 1) ? p1 : p0;
+    const T a = (T)sc->alpha;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) x[q] += a * pk[q];
 }
 
 // p = z + beta p
@@ -310,6 +369,23 @@ __global__ __launch_bounds__(256) void assemble_rhs(const double* f, const doubl
         if (gN && i == N - 1) v += rt.h * hj * gN[j];
         b[q] = v;
     }
+}
+
+// Adjoint source w = H~ (u - u_obs) and block partials of (u - u_obs)' H~ (u - u_obs).
+__global__ __launch_bounds__(256) void adjoint_source(const double* __restrict__ u, const double* __restrict__ uo,
+                                                     double* __restrict__ w, OpTab t, double h2, double* partial) {
+    const size_t n = (size_t)t.N * t.N;
+    double acc = 0.0;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += stride) {
+        const int i = (int)(q % t.N), j = (int)(q / t.N);
+        const double e = u[q] - uo[q];
+        const double we = h2 * hw_of<double>(t, i) * hw_of<double>(t, j) * e;
+        w[q] = we;
+        acc += we * e;
+    }
+    const double s = block_sum<double, 256>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // df = -h^2 hw_i hw_j lam

@@ -20,7 +20,10 @@ using ResidFF = OutResid<float, float>;
     X(double, InPlain<double>, ResidDF)             \
     X(double, InSmooth0<double>, OutSmooth<double>) \
     X(double, InPlain<double>, OutSmooth<double>)   \
+    X(double, InPlain<double>, OutSmoothDot<double>) \
     X(double, InProlong<double>, OutSmooth<double>) \
+    X(double, InProlong<double>, OutSmoothDot<double>) \
+    X(double, InAxpy<double>, OutPUpd<double>)      \
     X(double, InSmooth0<double>, OutCopy<double>)   \
     X(float, InPlain<float>, OutStore<float>)       \
     X(float, InPlain<float>, OutStoreDot<float>)    \
@@ -28,13 +31,23 @@ using ResidFF = OutResid<float, float>;
     X(float, InPlain<float>, ResidFF)               \
     X(float, InSmooth0<float>, OutSmooth<float>)    \
     X(float, InPlain<float>, OutSmooth<float>)      \
+    X(float, InPlain<float>, OutSmoothDot<float>)   \
     X(float, InProlong<float>, OutSmooth<float>)    \
+    X(float, InProlong<float>, OutSmoothDot<float>) \
+    X(float, InAxpy<float>, OutPUpd<float>)         \
     X(float, InSmooth0<float>, OutCopy<float>)
 
 // Compiled tile shapes (TI threads along i, J rows per thread).
 #define SBP_TILE_SHAPES(X) X(64, 4) X(64, 8) X(64, 16) X(128, 8) X(128, 16) X(256, 4) X(256, 8) X(256, 16)
 
 inline int nparts_of(int N, int ti, int tj) { return ((N + ti - 1) / ti) * ((N + tj - 1) / tj); }
+
+inline int rr_blocks(int Nc, int ti, int tj) { return ((Nc + ti / 2 - 2) / (ti / 2 - 1)) * ((Nc + tj / 2 - 1) / (tj / 2)); }
+
+// rc = scale * FW(b - A u): residual and restriction in one pass (tf = fine level table).
+template <int TI, int J, typename T>
+int launch_resid_restrict(int p, const OpTab& tf, const T* u, const T* b, T* rc, int Nc, T scale, const int* done,
+                          hipStream_t st);
 
 // Launches stencil_pass<p, TI, J, T, IN, OUT> over the whole N x N grid of tb.
 template <int TI, int J, typename T, class IN, class OUT>

@@ -104,6 +104,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 // ---------------------------------------------------------------------------
 template <typename T> struct InPlain {
     const T* u;
+    static constexpr bool kCol = false;
     __device__ void init() {}
     template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const { return u[gidx(i, j, t.N)]; }
@@ -113,6 +114,7 @@ template <typename T> struct InPlain {
 // x1 = w b (Richardson, use_diag = false).
 template <typename T> struct InSmooth0 {
     const T* b; T omega; int use_diag;
+    static constexpr bool kCol = false;
     __device__ void init() {}
     template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
@@ -125,7 +127,29 @@ template <typename T> struct InSmooth0 {
 // x + P e_c with P the bilinear prolongation from the coarse side Nc = (N+1)/2.
 template <typename T> struct InProlong {
     const T* x; const T* ec; int Nc;
+    static constexpr bool kCol = true;
     __device__ void init() {}
+    // Column window w[k] = x(i, jbeg+k) + (P ec)(i, jbeg+k), k < W, with jbeg of
+    // parity PAR: each coarse row is loaded once (W/2+2 rows) instead of four
+    // coarse loads per fine point.  Interior tiles only (no bounds tests).
+    template <int W, int PAR>
+    __device__ __forceinline__ void load_col(const OpTab& t, int i, int jbeg, T* w) const {
+        constexpr int NC = ((PAR + W - 1) >> 1) + 2;
+        const int ic = i >> 1, jc0 = jbeg >> 1;
+        const bool odd = i & 1;
+        T c[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            const T* e = ec + gidx(ic, jc0 + m, Nc);
+            c[m] = odd ? (e[0] + e[1]) * T(0.5) : e[0];
+        }
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const int s = PAR + k;
+            const T v = (s & 1) ? (c[s >> 1] + c[(s >> 1) + 1]) * T(0.5) : c[s >> 1];
+            w[k] = x[gidx(i, jbeg + k, t.N)] + v;
+        }
+    }
     template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
         const int ic = i >> 1, jc = j >> 1;
@@ -143,6 +167,7 @@ template <typename T> struct InProlong {
 // r + beta * p with beta read from the device scalar block.
 template <typename T> struct InAxpy {
     const T* r; const T* p; const double* beta; T bt;
+    static constexpr bool kCol = false;
     __device__ void init() { bt = (T)(*beta); }
     template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
@@ -198,19 +223,47 @@ template <typename T, typename TO> struct OutResid {  // r = b - Au (optionally 
 };
 
 // Smoothing sweep x_new = u + w (b - Au) / diag (damped Jacobi,
-// jacobi_brittany!) or u + w (b - Au) (Richardson, use_diag = 0);
-// optional acc += dotv * x_new.
+// jacobi_brittany!) or u + w (b - Au) (Richardson, use_diag = 0).
 template <typename T> struct OutSmooth {
+    const T* b; T* xo; T omega; int use_diag;
+    static constexpr bool kDot = false;
+    using Aux = T;
+    __device__ Aux pre(const OpTab&, size_t q) const { return b[q]; }
+    template <bool INT = false>
+    __device__ __forceinline__ void emit(const OpTab& t, int i, int j, size_t q, T u, T Au, Aux bq, T&) const {
+        T c = omega * (bq - Au);
+        if (use_diag) c = INT ? c * (T)t.idint : c / diag_of<T>(t, i, j);
+        xo[q] = u + c;
+    }
+};
+
+// The same sweep, also accumulating dotv' x_new (r'z of the last pass of a V-cycle).
+template <typename T> struct OutSmoothDot {
     const T* b; T* xo; T omega; const T* dotv; int use_diag;
     static constexpr bool kDot = true;
     struct Aux { T b, d; };
-    __device__ Aux pre(const OpTab&, size_t q) const { Aux a; a.b = b[q]; a.d = dotv ? dotv[q] : T(0); return a; }
+    __device__ Aux pre(const OpTab&, size_t q) const { Aux a; a.b = b[q]; a.d = dotv[q]; return a; }
     template <bool INT = false>
     __device__ __forceinline__ void emit(const OpTab& t, int i, int j, size_t q, T u, T Au, Aux a, T& acc) const {
         T c = omega * (a.b - Au);
         if (use_diag) c = INT ? c * (T)t.idint : c / diag_of<T>(t, i, j);
         const T xn = u + c;
         xo[q] = xn; acc += a.d * xn;
+    }
+};
+
+// Fused Krylov step on the search direction: the operand is p_new = z + beta p_old
+// (InAxpy); writes p_new and A p_new, applies the pending x += alpha p_old of
+// the previous iteration, and accumulates p_new' A p_new.
+template <typename T> struct OutPUpd {
+    T* pnew; T* Ap; T* x; const T* pold; const double* alpha; T al;
+    static constexpr bool kDot = true;
+    struct Aux { T x, p; };
+    __device__ void init() { al = (T)(*alpha); }
+    __device__ Aux pre(const OpTab&, size_t q) const { Aux a; a.x = x[q]; a.p = pold[q]; return a; }
+    template <bool INT = false>
+    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux a, T& acc) const {
+        pnew[q] = u; Ap[q] = Au; x[q] = a.x + al * a.p; acc += u * Au;
     }
 };
 
@@ -289,6 +342,9 @@ __device__ __forceinline__ T m_closure(const OpTab& t, const IN& in, int i, int 
     return s;
 }
 
+template <class X, class = void> struct has_init { static constexpr bool value = false; };
+template <class X> struct has_init<X, decltype(void(&X::init))> { static constexpr bool value = true; };
+
 struct PassCtl {
     const int* done;     // nullable: skip the pass when *done != 0
     double* partial;     // nullable: per-block inner-product partials
@@ -308,8 +364,12 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
     const int i = i0 + tid;
     const int N = t.N;
     T w[W];
+    if constexpr (IN::kCol) {
+        in.template load_col<W, (H & 1)>(t, i, j0 - H, w);   // j0 is a multiple of the even J
+    } else {
 #pragma unroll
-    for (int k = 0; k < W; ++k) w[k] = in.template load<true>(t, i, j0 - H + k);
+        for (int k = 0; k < W; ++k) w[k] = in.template load<true>(t, i, j0 - H + k);
+    }
     // 2H halo columns x J rows, spread over the first threads.
 #pragma unroll
     for (int e0 = 0; e0 < 2 * H * J; e0 += TI) {
@@ -422,6 +482,7 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
     const int N = t.N;
     const int i0 = bx * TI, j0 = by * J;
     in.init();
+    if constexpr (has_init<OUT>::value) out.init();
     __shared__ T tile[J][TI + 2 * H];
     // every point the tile loads (halo included) lies outside the closure rows
     const bool interior = i0 - H >= t.R && i0 + TI + H <= N - t.R && j0 - H >= t.R && j0 + J + H <= N - t.R;
@@ -433,6 +494,128 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
             T s = block_sum<T, TI>(acc);
             if (threadIdx.x == 0) ctl.partial[b] = (double)s;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Residual + full-weighting restriction in one pass
+// ---------------------------------------------------------------------------
+// (A u)(i, j) at a single point, closure-aware, all loads through IN.
+template <int P, typename T, class IN>
+__device__ __forceinline__ T au_point(const OpTab& t, const IN& in, int i, int j) {
+    constexpr int H = Stencil<P>::H;
+    const int N = t.N;
+    const T uc = in.load(t, i, j);
+    const bool jint = (j >= t.R) && (j < N - t.R);
+    const bool iint = (i >= t.R) && (i < N - t.R);
+    T kp = T(0), mp = T(0);
+    if (jint) {
+#pragma unroll
+        for (int s = 1; s <= H; ++s)
+            kp += Stencil<P>::template c<T>(s) * ((in.load(t, i, j - s) - uc) + (in.load(t, i, j + s) - uc));
+    } else {
+        kp = k_closure<T>(t, in, i, j, uc);
+    }
+    if (iint) {
+#pragma unroll
+        for (int s = 1; s <= H; ++s)
+            mp += Stencil<P>::template c<T>(s) * ((in.load(t, i - s, j) - uc) + (in.load(t, i + s, j) - uc));
+    } else {
+        mp = m_closure<T>(t, in, i, j, uc);
+    }
+    return (iint ? kp : hw_of<T>(t, i) * kp) + (jint ? mp : hw_of<T>(t, j) * mp);
+}
+
+struct RestrictCtl {
+    const int* done;
+    int Nc;
+    int nbx, nby;    // block grid over coarse points: TI/2-1 columns x J/2 rows per block
+};
+
+// rc = scale * FW(b - A u) on the coarse grid of side Nc = (N+1)/2.  A block
+// owns TI/2-1 coarse columns and J/2 coarse rows; its TI threads form the fine
+// residual on the TI x (J+1) fine points those coarse points weigh (one
+// column/row overlap with the neighbouring blocks), keep it in LDS and
+// restrict it there, so the fine residual never goes to HBM.
+template <int P, int TI, int J, typename T>
+__global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, const T* __restrict__ b,
+                                                    T* __restrict__ rc, T scale, OpTab t, RestrictCtl ctl) {
+    constexpr int H = Stencil<P>::H;
+    constexpr int JR = J + 1;            // fine residual rows per block
+    constexpr int W = JR + 2 * H;        // column window
+    constexpr int CI = TI / 2 - 1;       // coarse columns per block
+    constexpr int CJ = J / 2;            // coarse rows per block
+    if (ctl.done && *ctl.done) return;
+    const int nb = ctl.nbx * ctl.nby;
+    const int blk = xcd_remap((int)blockIdx.x, nb);
+    const int bx = blk % ctl.nbx, by = blk / ctl.nbx;
+    const int N = t.N, Nc = ctl.Nc;
+    const int Ic0 = bx * CI, Jc0 = by * CJ;
+    const int tid = threadIdx.x;
+    const int fi0 = 2 * Ic0 - 1, fj0 = 2 * Jc0 - 1;   // fine origin (may be -1)
+    const int fi = fi0 + tid;
+    const InPlain<T> in{u};
+    __shared__ T ut[JR][TI + 2 * H];
+    T (*rt)[TI + 2 * H] = ut;           // the residual reuses the operand tile after a barrier
+    T res[JR];
+    const bool interior = fi0 - H >= t.R && fi0 + TI + H <= N - t.R && fj0 - H >= t.R && fj0 + JR + H <= N - t.R;
+    if (interior) {
+        T w[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) w[k] = u[gidx(fi, fj0 - H + k, N)];
+#pragma unroll
+        for (int e0 = 0; e0 < 2 * H * JR; e0 += TI) {
+            const int e = e0 + tid;
+            if (e < 2 * H * JR) {
+                const int side = e / (H * JR), rem = e % (H * JR);
+                const int off = rem / JR, r = rem % JR;
+                const int ii = side == 0 ? fi0 - H + off : fi0 + TI + off;
+                ut[r][side == 0 ? off : TI + H + off] = u[gidx(ii, fj0 + r, N)];
+            }
+        }
+        T bq[JR];
+#pragma unroll
+        for (int r = 0; r < JR; ++r) {
+            ut[r][tid + H] = w[r + H];
+            bq[r] = b[gidx(fi, fj0 + r, N)];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < JR; ++r) {
+            const T uc = w[r + H];
+            const T* row = &ut[r][tid + H];
+            T kp = Stencil<P>::template c<T>(1) * ((w[r + H - 1] - uc) + (w[r + H + 1] - uc));
+            T mp = Stencil<P>::template c<T>(1) * ((row[-1] - uc) + (row[1] - uc));
+#pragma unroll
+            for (int s = 2; s <= H; ++s) {
+                kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
+                mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
+            }
+            res[r] = bq[r] - (kp + mp);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < JR; ++r) {
+            const int fj = fj0 + r;
+            T v = T(0);
+            if (fi >= 0 && fi < N && fj >= 0 && fj < N) v = b[gidx(fi, fj, N)] - au_point<P, T>(t, in, fi, fj);
+            res[r] = v;
+        }
+    }
+    __syncthreads();                     // every thread is done reading the operand tile
+#pragma unroll
+    for (int r = 0; r < JR; ++r) rt[r][tid] = res[r];
+    __syncthreads();
+    // Full weighting [1 2 1; 2 4 2; 1 2 1] / 16 (times scale) around fine (2I, 2Jc).
+    for (int o = tid; o < CI * CJ; o += TI) {
+        const int cI = o % CI, cJ = o / CI;
+        const int I = Ic0 + cI, Jc = Jc0 + cJ;
+        if (I >= Nc || Jc >= Nc) continue;
+        const int lc = 2 * cI + 1, lr = 2 * cJ + 1;
+        const T s0 = rt[lr][lc] + T(0.5) * (rt[lr][lc - 1] + rt[lr][lc + 1]);
+        const T sm = rt[lr - 1][lc] + T(0.5) * (rt[lr - 1][lc - 1] + rt[lr - 1][lc + 1]);
+        const T sp = rt[lr + 1][lc] + T(0.5) * (rt[lr + 1][lc - 1] + rt[lr + 1][lc + 1]);
+        rc[gidx(I, Jc, Nc)] = scale * T(0.25) * (s0 + T(0.5) * (sm + sp));
     }
 }
 

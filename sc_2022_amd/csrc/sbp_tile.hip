@@ -23,6 +23,26 @@ int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, const int*
     return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
+template <int TI, int J, typename T>
+int launch_resid_restrict(int p, const OpTab& tf, const T* u, const T* b, T* rc, int Nc, T scale, const int* done,
+                          hipStream_t st) {
+    RestrictCtl c;
+    c.done = done;
+    c.Nc = Nc;
+    c.nbx = (Nc + TI / 2 - 2) / (TI / 2 - 1);
+    c.nby = (Nc + J / 2 - 1) / (J / 2);
+    if (p == 2)
+        hipLaunchKernelGGL((resid_restrict<2, TI, J, T>), dim3(c.nbx * c.nby), dim3(TI), 0, st, u, b, rc, scale, tf, c);
+    else
+        hipLaunchKernelGGL((resid_restrict<4, TI, J, T>), dim3(c.nbx * c.nby), dim3(TI), 0, st, u, b, rc, scale, tf, c);
+    return hipGetLastError() == hipSuccess ? 0 : 4;
+}
+
+template int launch_resid_restrict<SBP_TI, SBP_TJ, double>(int, const OpTab&, const double*, const double*, double*,
+                                                           int, double, const int*, hipStream_t);
+template int launch_resid_restrict<SBP_TI, SBP_TJ, float>(int, const OpTab&, const float*, const float*, float*, int,
+                                                          float, const int*, hipStream_t);
+
 #define SBP_INST(T, IN, OUT)                                                                                \
     template int launch_tile<SBP_TI, SBP_TJ, T, IN, OUT>(int, const OpTab&, const IN&, const OUT&, const int*, \
                                                          double*, hipStream_t);

@@ -235,6 +235,26 @@ class SbpPoisson:
             capi.check(st, "sbp_mgpcg_mixed")
         return x, SolveResult(oi.value, rn.value, st, None, ii.value)
 
+    def adjoint_grad(self, u, u_obs, abstol=None, maxiter=500, lam=None):
+        """Gradient of L = 1/2 (u - u_obs)' H~ (u - u_obs), u the forward solution,
+        through the adjoint solve.  Returns (L, lam, df, dgW, dgE, dgS, dgN, iters)."""
+        self._chk(u, dtype=torch.float64, name="u")
+        self._chk(u_obs, dtype=torch.float64, name="u_obs")
+        z = lambda n: torch.empty(n, dtype=torch.float64, device=self.device)
+        lam = z(self.n) if lam is None else self._chk(lam, dtype=torch.float64, name="lam")
+        df, gW, gE, gS, gN = z(self.n), z(self.N), z(self.N), z(self.N), z(self.N)
+        if abstol is None:
+            # 1e-13 relative to the adjoint source H~ (u - u_obs)
+            d = u - u_obs
+            abstol = 1e-13 * float(torch.linalg.vector_norm(d)) * (1.0 / (self.N - 1)) ** 2
+        loss = C.c_double()
+        it = C.c_int()
+        self._sync_stream()
+        capi.check(self.lib.sbp_adjoint_grad(self._h, _ptr(u), _ptr(u_obs), float(abstol), int(maxiter),
+                                             C.byref(loss), _ptr(lam), _ptr(df), _ptr(gW), _ptr(gE), _ptr(gS),
+                                             _ptr(gN), C.byref(it)), "sbp_adjoint_grad")
+        return loss.value, lam, df, gW, gE, gS, gN, it.value
+
     # -- CSR comparison path -------------------------------------------------
     def csr_assemble(self):
         nnz = C.c_int64()
@@ -271,3 +291,36 @@ class SbpPoisson:
         self._sync_stream()
         capi.check(self.lib.sbp_copy(self._h, _ptr(src), _ptr(dst), src.numel() * src.element_size()), "sbp_copy")
         return dst
+
+
+class PoissonSolve(torch.autograd.Function):
+    """u = A^-1 b(f, gW, gE, gS, gN) as a differentiable torch operation.
+
+    forward: right-hand side assembly + FP64 MG-PCG; backward: the adjoint
+    solve lambda = A^-T grad_u (A is symmetric, so the same MG-PCG) and the
+    transposed right-hand-side map.  The Python counterpart of a
+    ChainRulesCore.rrule for the solve."""
+
+    @staticmethod
+    def forward(ctx, op, f, gW, gE, gS, gN, rtol=1e-13):
+        b = op.assemble_rhs(f, gW, gE, gS, gN)
+        bn = float(torch.linalg.vector_norm(b))
+        u, res = op.mgpcg(b, abstol=rtol * bn)
+        ctx.op = op
+        ctx.rtol = rtol
+        return u
+
+    @staticmethod
+    def backward(ctx, gu):
+        op = ctx.op
+        gu = gu.contiguous()
+        gn = float(torch.linalg.vector_norm(gu))
+        lam = torch.zeros_like(gu)
+        if gn > 0:
+            lam, _ = op.mgpcg(gu, abstol=ctx.rtol * gn)
+        df, gW, gE, gS, gN = op.rhs_transpose(lam)
+        return None, df, gW, gE, gS, gN, None
+
+
+def poisson_solve(op, f, gW, gE, gS, gN, rtol=1e-13):
+    return PoissonSolve.apply(op, f, gW, gE, gS, gN, rtol)

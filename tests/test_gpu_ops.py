@@ -257,3 +257,18 @@ def test_copy_kernel(SP):
     h.copy(a, b)
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_vcycle_large_against_oracle(SP, p):
+    """Level 9 (513^2): every V-cycle kernel runs its interior-tile path
+    (including the fused residual + restriction)."""
+    level = 9
+    N = 2 ** level + 1
+    h = SP(N, p)
+    hier = O.Hierarchy(level, p)
+    r = np.random.default_rng(21).standard_normal(N * N)
+    z = h.vcycle(T(r)).cpu().numpy()
+    assert relmax(z, O.vcycle(hier, r)) < 1e-12
+    z32 = h.vcycle(T(r, torch.float32)).cpu().double().numpy()
+    assert relmax(z32, O.vcycle(hier, r)) < 1e-4

@@ -98,6 +98,12 @@ def test_mixed_precision_ir(SP, p):
     rel = O.h_norm(xg - xd, hd) / O.h_norm(xd, hd)
     assert rel <= 1e-10, rel
     assert 1 <= res.iters <= 6 and res.inner_iters >= res.iters
+    # stopping on the correction-based error estimate
+    he = SP(2 ** level + 1, p, ir_rtol=1e-11)
+    x, res = he.mgpcg_mixed(T(b), abstol=1e-14 * np.linalg.norm(b))
+    assert res.converged and res.iters <= 3, res
+    rel = O.h_norm(x.cpu().numpy() - xd, hd) / O.h_norm(xd, hd)
+    assert rel <= 1e-10, rel
 
 
 @pytest.mark.parametrize("p", [2, 4])
