@@ -38,7 +38,9 @@ def parse():
     ap.add_argument("--n", type=int, default=4097)
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--seed", type=int, default=2022)
-    ap.add_argument("--rtol", type=float, default=1e-11, help="stopping rule ||r|| < rtol ||b||")
+    ap.add_argument("--rtol", type=float, default=1e-12, help="stopping rule ||r|| < rtol ||b||")
+    ap.add_argument("--inner-rtol", type=float, default=1e-6,
+                    help="mixed: relative tolerance of each FP32 inner solve")
     ap.add_argument("--ir-rtol", type=float, default=1e-11,
                     help="mixed: stop when the correction-based error estimate <= ir_rtol ||x||")
     ap.add_argument("--apply-reps", type=int, default=200)
@@ -142,7 +144,7 @@ def main():
     torch.cuda.set_device(dev)
     N, p = a.n, a.p
     n = N * N
-    h = SbpPoisson(N, p, device=dev, ir_rtol=a.ir_rtol)
+    h = SbpPoisson(N, p, device=dev, ir_rtol=a.ir_rtol, inner_rtol=a.inner_rtol)
 
     # ---- apply roofline (FP64 and FP32), rotating > 1 GB of buffers ----
     nbuf64 = max(2, int((2 * 1024 ** 3) // (2 * n * 8)) + 1)
@@ -224,7 +226,8 @@ def main():
         "dtype": "f32 inner / f64 outer",
         "data": "synthetic: seeded Fourier-mode source (64 modes) and boundary data (8 modes/face)",
         "config": {"workload": f"MG-PCG solve, N={N} (n={n}), SBP p={p}, V-cycle 2+2 damped Jacobi to 17^2, "
-                               f"stop ||r|| < {a.rtol:g} ||b|| (mixed also: error estimate < {a.ir_rtol:g} ||x||)",
+                               f"stop ||r|| < {a.rtol:g} ||b|| (mixed also: error estimate < {a.ir_rtol:g} ||x||, "
+                               f"inner FP32 solves to {a.inner_rtol:g})",
                    "seed": a.seed},
         "rel_err_mixed": errmx,
         "rel_err_f64": err64,

@@ -102,10 +102,19 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 // ---------------------------------------------------------------------------
 // Input policies: value of the operand at (i, j)
 // ---------------------------------------------------------------------------
+// Every input policy splits a value into a raw fetch (plain loads only) and a
+// transform, so that the guarded face-tile paths issue all their loads before
+// the first use (a transform inside a guarded load would serialise the
+// round trips).  load() = make(fetch()).
 template <typename T> struct InPlain {
     const T* u;
     static constexpr bool kCol = false;
+    using Raw = T;
     __device__ void init() {}
+    template <bool INT = false>
+    __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const { return u[gidx(i, j, t.N)]; }
+    template <bool INT = false>
+    __device__ __forceinline__ T make(const OpTab&, int, int, const Raw& r) const { return r; }
     template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const { return u[gidx(i, j, t.N)]; }
 };
@@ -115,12 +124,19 @@ template <typename T> struct InPlain {
 template <typename T> struct InSmooth0 {
     const T* b; T omega; int use_diag;
     static constexpr bool kCol = false;
+    using Raw = T;
     __device__ void init() {}
     template <bool INT = false>
-    __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
-        const T v = omega * b[gidx(i, j, t.N)];
+    __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const { return b[gidx(i, j, t.N)]; }
+    template <bool INT = false>
+    __device__ __forceinline__ T make(const OpTab& t, int i, int j, const Raw& r) const {
+        const T v = omega * r;
         if (!use_diag) return v;
         return INT ? v * (T)t.idint : v / diag_of<T>(t, i, j);
+    }
+    template <bool INT = false>
+    __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
+        return make<INT>(t, i, j, fetch<INT>(t, i, j));
     }
 };
 
@@ -150,17 +166,29 @@ template <typename T> struct InProlong {
             w[k] = x[gidx(i, jbeg + k, t.N)] + v;
         }
     }
+    struct Raw { T x, e00, e10, e01, e11; };
+    template <bool INT = false>
+    __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const {
+        const int ic = i >> 1, jc = j >> 1;
+        const int ic1 = ic + 1 < Nc ? ic + 1 : ic, jc1 = jc + 1 < Nc ? jc + 1 : jc;
+        Raw r;
+        r.x = x[gidx(i, j, t.N)];
+        r.e00 = ec[gidx(ic, jc, Nc)];
+        r.e10 = ec[gidx(ic1, jc, Nc)];
+        r.e01 = ec[gidx(ic, jc1, Nc)];
+        r.e11 = ec[gidx(ic1, jc1, Nc)];
+        return r;
+    }
+    template <bool INT = false>
+    __device__ __forceinline__ T make(const OpTab&, int i, int j, const Raw& r) const {
+        T v;
+        if (!(i & 1)) v = (!(j & 1)) ? r.e00 : (r.e00 + r.e01) * T(0.5);
+        else v = (!(j & 1)) ? (r.e00 + r.e10) * T(0.5) : (r.e00 + r.e10 + r.e01 + r.e11) * T(0.25);
+        return r.x + v;
+    }
     template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
-        const int ic = i >> 1, jc = j >> 1;
-        const T* e = ec + gidx(ic, jc, Nc);
-        T v;
-        if (!(i & 1)) {
-            v = (!(j & 1)) ? e[0] : (e[0] + e[Nc]) * T(0.5);
-        } else {
-            v = (!(j & 1)) ? (e[0] + e[1]) * T(0.5) : (e[0] + e[1] + e[Nc] + e[Nc + 1]) * T(0.25);
-        }
-        return x[gidx(i, j, t.N)] + v;
+        return make<INT>(t, i, j, fetch<INT>(t, i, j));
     }
 };
 
@@ -169,6 +197,17 @@ template <typename T> struct InAxpy {
     const T* r; const T* p; const double* beta; T bt;
     static constexpr bool kCol = false;
     __device__ void init() { bt = (T)(*beta); }
+    struct Raw { T r, p; };
+    template <bool INT = false>
+    __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const {
+        const size_t q = gidx(i, j, t.N);
+        Raw v;
+        v.r = r[q];
+        v.p = p[q];
+        return v;
+    }
+    template <bool INT = false>
+    __device__ __forceinline__ T make(const OpTab&, int, int, const Raw& v) const { return v.r + bt * v.p; }
     template <bool INT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
         const size_t q = gidx(i, j, t.N);
@@ -310,40 +349,33 @@ __device__ __forceinline__ T block_sum(T v) {
 // columns; uc is the operand at (i, j).
 template <typename T, class IN>
 __device__ __forceinline__ T k_closure(const OpTab& t, const IN& in, int i, int j, T uc) {
-    if (j < t.R) {
-        T s = (T)t.dev->krs[j] * uc;
+    const bool lo = j < t.R;
+    const int r = lo ? j : t.N - 1 - j;
+    typename IN::Raw raw[kMaxC];
 #pragma unroll
-        for (int c = 0; c < kMaxC; ++c)
-            if (c < t.C && c != j) s += (T)t.dev->kb[j][c] * (in.load(t, i, c) - uc);
-        return s;
-    }
-    const int r = t.N - 1 - j;
+    for (int c = 0; c < kMaxC; ++c)
+        if (c < t.C) raw[c] = in.fetch(t, i, lo ? c : t.N - 1 - c);
     T s = (T)t.dev->krs[r] * uc;
 #pragma unroll
     for (int c = 0; c < kMaxC; ++c)
-        if (c < t.C && c != r) s += (T)t.dev->kb[r][c] * (in.load(t, i, t.N - 1 - c) - uc);
+        if (c < t.C && c != r) s += (T)t.dev->kb[r][c] * (in.make(t, i, lo ? c : t.N - 1 - c, raw[c]) - uc);
     return s;
 }
 
 template <typename T, class IN>
 __device__ __forceinline__ T m_closure(const OpTab& t, const IN& in, int i, int j, T uc) {
-    if (i < t.R) {
-        T s = (T)t.dev->mrs[i] * uc;
+    const bool lo = i < t.R;
+    const int r = lo ? i : t.N - 1 - i;
+    typename IN::Raw raw[kMaxC];
 #pragma unroll
-        for (int c = 0; c < kMaxC; ++c)
-            if (c < t.C && c != i) s += (T)t.dev->mb[i][c] * (in.load(t, c, j) - uc);
-        return s;
-    }
-    const int r = t.N - 1 - i;
+    for (int c = 0; c < kMaxC; ++c)
+        if (c < t.C) raw[c] = in.fetch(t, lo ? c : t.N - 1 - c, j);
     T s = (T)t.dev->mrs[r] * uc;
 #pragma unroll
     for (int c = 0; c < kMaxC; ++c)
-        if (c < t.C && c != r) s += (T)t.dev->mb[r][c] * (in.load(t, t.N - 1 - c, j) - uc);
+        if (c < t.C && c != r) s += (T)t.dev->mb[r][c] * (in.make(t, lo ? c : t.N - 1 - c, j, raw[c]) - uc);
     return s;
 }
-
-template <class X, class = void> struct has_init { static constexpr bool value = false; };
-template <class X> struct has_init<X, decltype(void(&X::init))> { static constexpr bool value = true; };
 
 struct PassCtl {
     const int* done;     // nullable: skip the pass when *done != 0
@@ -351,24 +383,48 @@ struct PassCtl {
     int nbx, nby;        // tile grid
 };
 
-// Interior tile: every point of the TI x J tile and its halo lies inside the
-// grid and away from the closure rows, so the pass is straight-line code
-// with no bounds tests (the common case: all but the first/last tile
-// column and row of tiles).
-template <int P, int TI, int J, typename T, class IN, class OUT>
+// Copies the device table into LDS (blockDim.x threads, one word each).
+__device__ __forceinline__ void stage_table(const OpTab& t, OpTab* l) {
+    static_assert(sizeof(OpTab) % 8 == 0, "table is copied in 8-byte words");
+    const unsigned long long* src = reinterpret_cast<const unsigned long long*>(t.dev);
+    unsigned long long* dst = reinterpret_cast<unsigned long long*>(l);
+    for (int k = threadIdx.x; k < (int)(sizeof(OpTab) / 8); k += blockDim.x) dst[k] = src[k];
+    __syncthreads();
+}
+
+template <class X, class = void> struct has_init { static constexpr bool value = false; };
+template <class X> struct has_init<X, decltype(void(&X::init))> { static constexpr bool value = true; };
+
+// Tile whose rows (and their halo) lie inside the grid, away from the closure
+// rows in j.  With ICHK = false the columns are interior too: straight-line
+// code with no bounds tests and a constant inverse diagonal (the common
+// case).  With ICHK = true (the first/last column of tiles) each thread tests
+// its own column: out-of-grid threads idle, threads on the i-closure rows take
+// the closure sum and the H weight; everything else stays on the fast path.
+template <int P, int TI, int J, bool ICHK, typename T, class IN, class OUT>
 __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const OpTab& t, int i0, int j0,
                                            T (*tile)[TI + 2 * Stencil<P>::H]) {
     constexpr int H = Stencil<P>::H;
     constexpr int W = J + 2 * H;
+    constexpr bool INT = !ICHK;
     const int tid = threadIdx.x;
     const int i = i0 + tid;
     const int N = t.N;
+    const bool act = !ICHK || i < N;
     T w[W];
-    if constexpr (IN::kCol) {
-        in.template load_col<W, (H & 1)>(t, i, j0 - H, w);   // j0 is a multiple of the even J
+    if (act) {
+        if constexpr (IN::kCol && !ICHK) {
+            in.template load_col<W, (H & 1)>(t, i, j0 - H, w);   // j0 is a multiple of the even J
+        } else {
+            typename IN::Raw raw[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) raw[k] = in.template fetch<INT>(t, i, j0 - H + k);
+#pragma unroll
+            for (int k = 0; k < W; ++k) w[k] = in.template make<INT>(t, i, j0 - H + k, raw[k]);
+        }
     } else {
 #pragma unroll
-        for (int k = 0; k < W; ++k) w[k] = in.template load<true>(t, i, j0 - H + k);
+        for (int k = 0; k < W; ++k) w[k] = T(0);
     }
     // 2H halo columns x J rows, spread over the first threads.
 #pragma unroll
@@ -378,29 +434,39 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
             const int side = e / (H * J), rem = e % (H * J);
             const int off = rem / J, r = rem % J;
             const int ii = side == 0 ? i0 - H + off : i0 + TI + off;
-            tile[r][side == 0 ? off : TI + H + off] = in.template load<true>(t, ii, j0 + r);
+            T v = T(0);
+            if (!ICHK || (ii >= 0 && ii < N)) v = in.template load<INT>(t, ii, j0 + r);
+            tile[r][side == 0 ? off : TI + H + off] = v;
         }
     }
     typename OUT::Aux aux[J];
 #pragma unroll
     for (int r = 0; r < J; ++r) {
         tile[r][tid + H] = w[r + H];
-        aux[r] = out.pre(t, gidx(i, j0 + r, N));
+        if (act) aux[r] = out.pre(t, gidx(i, j0 + r, N));
     }
     __syncthreads();
     T acc = T(0);
+    if (!act) return acc;
+    const bool iint = !ICHK || ((i >= t.R) && (i < N - t.R));
+    const T hwi = ICHK ? hw_of<T>(t, i) : T(1);
 #pragma unroll
     for (int r = 0; r < J; ++r) {
         const T uc = w[r + H];
-        const T* row = &tile[r][tid + H];
         T kp = Stencil<P>::template c<T>(1) * ((w[r + H - 1] - uc) + (w[r + H + 1] - uc));
-        T mp = Stencil<P>::template c<T>(1) * ((row[-1] - uc) + (row[1] - uc));
 #pragma unroll
-        for (int s = 2; s <= H; ++s) {
-            kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
-            mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
+        for (int s = 2; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
+        T mp;
+        if (iint) {
+            const T* row = &tile[r][tid + H];
+            mp = Stencil<P>::template c<T>(1) * ((row[-1] - uc) + (row[1] - uc));
+#pragma unroll
+            for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
+        } else {
+            mp = m_closure<T>(t, in, i, j0 + r, uc);
         }
-        out.template emit<true>(t, i, j0 + r, gidx(i, j0 + r, N), uc, kp + mp, aux[r], acc);
+        const T Au = ICHK ? hwi * kp + mp : kp + mp;   // hw_j = 1 on interior rows
+        out.template emit<INT>(t, i, j0 + r, gidx(i, j0 + r, N), uc, Au, aux[r], acc);
     }
     return acc;
 }
@@ -416,19 +482,31 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
     const int i = i0 + tid;
     const bool act = i < N;
     T w[W];
+    {
+        typename IN::Raw raw[W];
 #pragma unroll
-    for (int k = 0; k < W; ++k) {
-        const int jj = j0 - H + k;
-        w[k] = (act && jj >= 0 && jj < N) ? in.load(t, i, jj) : T(0);
+        for (int k = 0; k < W; ++k) {
+            const int jj = j0 - H + k;
+            if (act && jj >= 0 && jj < N) raw[k] = in.fetch(t, i, jj);
+        }
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const int jj = j0 - H + k;
+            w[k] = (act && jj >= 0 && jj < N) ? in.make(t, i, jj, raw[k]) : T(0);
+        }
     }
     if (tid < 2 * H) {
         const int side = tid / H, off = tid % H;
         const int ii = side == 0 ? i0 - H + off : i0 + TI + off;
         const int col = side == 0 ? off : TI + H + off;
+        typename IN::Raw raw[J];
+#pragma unroll
+        for (int r = 0; r < J; ++r)
+            if (ii >= 0 && ii < N && j0 + r < N) raw[r] = in.fetch(t, ii, j0 + r);
 #pragma unroll
         for (int r = 0; r < J; ++r) {
             const int jj = j0 + r;
-            tile[r][col] = (ii >= 0 && ii < N && jj < N) ? in.load(t, ii, jj) : T(0);
+            tile[r][col] = (ii >= 0 && ii < N && jj < N) ? in.make(t, ii, jj, raw[r]) : T(0);
         }
     }
     typename OUT::Aux aux[J];
@@ -485,9 +563,22 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
     if constexpr (has_init<OUT>::value) out.init();
     __shared__ T tile[J][TI + 2 * H];
     // every point the tile loads (halo included) lies outside the closure rows
-    const bool interior = i0 - H >= t.R && i0 + TI + H <= N - t.R && j0 - H >= t.R && j0 + J + H <= N - t.R;
-    const T acc = interior ? pass_interior<P, TI, J, T>(in, out, t, i0, j0, tile)
-                           : pass_general<P, TI, J, T>(in, out, t, i0, j0, tile);
+    const bool jin = j0 - H >= t.R && j0 + J + H <= N - t.R;
+    const bool iin = i0 - H >= t.R && i0 + TI + H <= N - t.R;
+    T acc;
+    if (jin && iin) {
+        acc = pass_interior<P, TI, J, false, T>(in, out, t, i0, j0, tile);
+    } else {
+        // Face tiles look the closure tables up per thread: stage them in LDS
+        // first (one load per thread) so those lookups cost LDS, not global,
+        // latency -- on the small multigrid levels every tile is a face tile.
+        __shared__ OpTab ltab;
+        stage_table(t, &ltab);
+        OpTab tl = t;
+        tl.dev = &ltab;
+        if (jin) acc = pass_interior<P, TI, J, true, T>(in, out, tl, i0, j0, tile);
+        else acc = pass_general<P, TI, J, T>(in, out, tl, i0, j0, tile);
+    }
     if constexpr (OUT::kDot) {
         if (ctl.partial) {
             __syncthreads();
@@ -558,11 +649,21 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
     __shared__ T ut[JR][TI + 2 * H];
     T (*rt)[TI + 2 * H] = ut;           // the residual reuses the operand tile after a barrier
     T res[JR];
-    const bool interior = fi0 - H >= t.R && fi0 + TI + H <= N - t.R && fj0 - H >= t.R && fj0 + JR + H <= N - t.R;
-    if (interior) {
+    const bool jin = fj0 - H >= t.R && fj0 + JR + H <= N - t.R;
+    const bool iin = fi0 - H >= t.R && fi0 + TI + H <= N - t.R;
+    __shared__ OpTab ltab;
+    OpTab tl = t;
+    if (!(jin && iin)) {
+        stage_table(t, &ltab);   // per-thread closure lookups from LDS
+        tl.dev = &ltab;
+    }
+    if (jin) {
+        // Rows interior: register window along j, LDS tile along i; with
+        // columns at the faces (iin false) each thread tests its own column.
+        const bool act = iin || (fi >= 0 && fi < N);
         T w[W];
 #pragma unroll
-        for (int k = 0; k < W; ++k) w[k] = u[gidx(fi, fj0 - H + k, N)];
+        for (int k = 0; k < W; ++k) w[k] = act ? u[gidx(fi, fj0 - H + k, N)] : T(0);
 #pragma unroll
         for (int e0 = 0; e0 < 2 * H * JR; e0 += TI) {
             const int e = e0 + tid;
@@ -570,35 +671,41 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
                 const int side = e / (H * JR), rem = e % (H * JR);
                 const int off = rem / JR, r = rem % JR;
                 const int ii = side == 0 ? fi0 - H + off : fi0 + TI + off;
-                ut[r][side == 0 ? off : TI + H + off] = u[gidx(ii, fj0 + r, N)];
+                ut[r][side == 0 ? off : TI + H + off] = (iin || (ii >= 0 && ii < N)) ? u[gidx(ii, fj0 + r, N)] : T(0);
             }
         }
         T bq[JR];
 #pragma unroll
         for (int r = 0; r < JR; ++r) {
             ut[r][tid + H] = w[r + H];
-            bq[r] = b[gidx(fi, fj0 + r, N)];
+            bq[r] = act ? b[gidx(fi, fj0 + r, N)] : T(0);
         }
         __syncthreads();
+        const bool iint = iin || ((fi >= t.R) && (fi < N - t.R));
+        const T hwi = iin ? T(1) : (act ? hw_of<T>(tl, fi) : T(0));
 #pragma unroll
         for (int r = 0; r < JR; ++r) {
             const T uc = w[r + H];
-            const T* row = &ut[r][tid + H];
             T kp = Stencil<P>::template c<T>(1) * ((w[r + H - 1] - uc) + (w[r + H + 1] - uc));
-            T mp = Stencil<P>::template c<T>(1) * ((row[-1] - uc) + (row[1] - uc));
 #pragma unroll
-            for (int s = 2; s <= H; ++s) {
-                kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
-                mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
+            for (int s = 2; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
+            T mp;
+            if (iint) {
+                const T* row = &ut[r][tid + H];
+                mp = Stencil<P>::template c<T>(1) * ((row[-1] - uc) + (row[1] - uc));
+#pragma unroll
+                for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
+            } else {
+                mp = act ? m_closure<T>(tl, in, fi, fj0 + r, uc) : T(0);
             }
-            res[r] = bq[r] - (kp + mp);
+            res[r] = act ? bq[r] - (hwi * kp + mp) : T(0);
         }
     } else {
 #pragma unroll
         for (int r = 0; r < JR; ++r) {
             const int fj = fj0 + r;
             T v = T(0);
-            if (fi >= 0 && fi < N && fj >= 0 && fj < N) v = b[gidx(fi, fj, N)] - au_point<P, T>(t, in, fi, fj);
+            if (fi >= 0 && fi < N && fj >= 0 && fj < N) v = b[gidx(fi, fj, N)] - au_point<P, T>(tl, in, fi, fj);
             res[r] = v;
         }
     }
