@@ -23,6 +23,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -47,29 +49,15 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--only-apply", action="store_true", help="time the apply only (profiling runs)")
+    ap.add_argument("--batch", type=int, default=256, help="problems in the batched leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=3)
     return ap.parse_args()
 
 
-def fourier_problem(h, N, seed, torch):
-    """Device-side data generator for the seeded Fourier-mode problem (the
-    mode list is the same splitmix64 stream the tests use)."""
-    from oracle.sbp_oracle import fourier_modes
-    src, bnd = fourier_modes(seed)
-    dev = h.device
-    t = torch.linspace(0.0, 1.0, N, dtype=torch.float64, device=dev)
-    yi = t[:, None]   # fast index i -> y   (tensor is [j, i] row-major; here build [i, j] then transpose)
-    xj = t[None, :]
-    F = torch.zeros((N, N), dtype=torch.float64, device=dev)
-    for kx, ky, a, ph in src:
-        F += a * torch.cos(torch.pi * (kx * xj + ky * yi) + ph)
-    f = F.t().contiguous().reshape(-1)          # element (i, j) at i + j N
-    g = []
-    for modes in bnd:
-        v = torch.zeros(N, dtype=torch.float64, device=dev)
-        for kk, a, ph in modes:
-            v += a * torch.cos(torch.pi * kk * t + ph)
-        g.append(v)
-    return h.assemble_rhs(f, *g)
+def fourier_problem(h, seed):
+    """b of the seeded Fourier-mode problem, generated on the device."""
+    from sc_2022_amd.data import fourier_rhs
+    return fourier_rhs(h, seed)
 
 
 def time_apply(h, torch, dtype, reps, nbuf):
@@ -108,6 +96,50 @@ def time_copy(h, torch, reps, nbytes, nbuf):
     e1.record(s)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def batch_leg(a, dev, torch):
+    """Uneven batch: a.batch problems, N = 2^k+1 with k uniform in 6..10 and p
+    in {2, 4} drawn from the seed, each with its own Fourier-mode data; mixed
+    MG-PCG with per-problem masks.  Reports batch solves/s, p50/p99 of the
+    per-problem device times and the worst relative H-norm error against a
+    tightly converged FP64 solve of the same problems."""
+    from sc_2022_amd.batch import UnevenBatch
+    from sc_2022_amd.data import batch_specs
+    specs = batch_specs(a.seed, count=a.batch)
+    ub = UnevenBatch([(N, p) for N, p, _ in specs], device=dev, ir_rtol=a.ir_rtol, inner_rtol=a.inner_rtol)
+    tol_ref, tol = [], []
+    for q, (N, p, s) in enumerate(specs):
+        ub.set_fourier(q, s)
+    torch.cuda.synchronize()
+    for q in range(len(specs)):
+        bn = float(torch.linalg.vector_norm(ub.rhs(q)))
+        tol_ref.append(1e-14 * bn)
+        tol.append(a.rtol * bn)
+    ub.solve(tol_ref, mixed=False, maxiter=200)
+    xref = [ub.solution(q).clone() for q in range(len(specs))]
+    for _ in range(max(1, a.warmup)):
+        out = ub.solve(tol, mixed=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.batch_steps):
+        out = ub.solve(tol, mixed=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.batch_steps
+    errs = []
+    for q in range(len(specs)):
+        op = ub.op(q)
+        errs.append(op.hnorm(ub.solution(q) - xref[q]) / op.hnorm(xref[q]))
+    sec = sorted(o["seconds"] for o in out)
+    pts = sum(N * N for N, _, _ in specs)
+    res = {"problems": len(specs), "groups": len(ub.groups), "points": pts,
+           "solves_per_s": len(specs) / dt, "ms_per_batch": dt * 1e3,
+           "p50_ms": 1e3 * float(np.percentile(sec, 50)), "p99_ms": 1e3 * float(np.percentile(sec, 99)),
+           "max_rel_err": max(errs), "all_converged": all(o["status"] == 0 for o in out),
+           "sizes": {f"{N}^2 p{p}": sum(1 for n, pp, _ in specs if (n, pp) == (N, p))
+                     for N, p in sorted({(n, pp) for n, pp, _ in specs})}}
+    ub.close()
+    return res
 
 
 def cpu_baseline(N, p, seed, threads, rtol):
@@ -164,7 +196,7 @@ def main():
         return
 
     # ---- problem and FP64 reference solution (untimed) ----
-    b = fourier_problem(h, N, a.seed, torch)
+    b = fourier_problem(h, a.seed)
     bnorm = float(torch.linalg.vector_norm(b))
     xref, rref = h.mgpcg(b, abstol=1e-14 * bnorm, maxiter=200, raise_on_fail=False)
     href = h.hnorm(xref)
@@ -205,6 +237,13 @@ def main():
     tmx = (time.perf_counter() - t0) / a.steps
     errmx = rel_err(x)
 
+    batched = None
+    if a.batch > 0:
+        try:
+            batched = batch_leg(a, dev, torch)
+        except Exception as e:  # reported, never silently replaced
+            batched = {"error": f"{type(e).__name__}: {e}"}
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -241,6 +280,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs64 / HBM_PEAK_GBS, "traffic": traffic_from_profiles("f64"),
                      "kernel": "stencil_pass FP64 apply (OutStore), algorithmic bytes 2*n*8 per launch"},
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -19,6 +19,14 @@
  * it returns them.  A handle is not thread-safe; use one handle per stream.
  * Every entry point returns an sbp_status.  No entry point falls back to host
  * computation.
+ *
+ * Batches: a handle made by sbp_create_batch holds nprob independent problems
+ * of the same N and p.  Their grid functions are stacked in one device array,
+ * problem q at element offset q*N_l*N_l on level l, and the operator,
+ * multigrid-component and batched-solver entry points act on all of them in
+ * each launch (per-problem scalars are arrays of nprob).  sbp_diag,
+ * sbp_assemble_rhs, sbp_rhs_transpose, sbp_hnorm, sbp_dot and the CSR path act
+ * on one problem's arrays; the single-problem solvers need nprob = 1.
  */
 #ifndef SBP_H
 #define SBP_H
@@ -76,6 +84,11 @@ void sbp_default_params(sbp_params* prm);
  * Replaces the per-call set-up of Assembling_matrix (level_2_multigrid_new.jl:125)
  * and the per-call CuArray allocations in initial_guess_cg.jl:152-161. */
 int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out);
+/* A handle for nprob stacked problems of side N and order p (sbp_create is
+ * nprob = 1).  No counterpart in the reference, which solves one problem per
+ * call of matrix_free_MGCG (initial_guess_cg.jl:303-346). */
+int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle* out);
+int sbp_get_batch(sbp_handle h, int* nprob);
 int sbp_destroy(sbp_handle h);
 int sbp_set_stream(sbp_handle h, void* hip_stream);
 int sbp_get_info(sbp_handle h, int* N, int* p, int* nlevels);
@@ -89,7 +102,7 @@ int sbp_set_tiles(sbp_handle h, int level, int dtype, int tile_i, int tile_j);
  * which returns -A u; sign = +1 gives the SPD A u.  dtype selects the element
  * type of both u and Au. */
 int sbp_apply(sbp_handle h, int level, const void* u, void* Au, int sign, int dtype);
-/* Au = A u and *dot = u'Au in one pass (FP64). */
+/* Au = A u and dot[q] = u_q'A u_q in one pass (FP64; dot: nprob host doubles). */
 int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot);
 /* d = diag(A) on the finest level (FP64). */
 int sbp_diag(sbp_handle h, double* d);
@@ -104,7 +117,7 @@ int sbp_rhs_transpose(sbp_handle h, const double* lam, double* df, double* dgW,
                       double* dgE, double* dgS, double* dgN);
 /* *out = sqrt(v' H~ v) (matrix_free_mg_cg.jl:464). */
 int sbp_hnorm(sbp_handle h, const double* v, double* out);
-/* *out = u'v and *nrm = ||u||_2 (either may be NULL). */
+/* *out = u'v (FP64). */
 int sbp_dot(sbp_handle h, const double* u, const double* v, double* out);
 
 /* Multigrid components (level l has side N_l = (N_{l-1}+1)/2) ------------ */
@@ -143,6 +156,20 @@ int sbp_mgpcg(sbp_handle h, const double* b, double* x, double abstol, int maxit
  * evaluated. */
 int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double abstol, int max_outer,
                     int* outer_iters, int* inner_iters, double* resnorm);
+/* Batched forms of sbp_mgpcg and sbp_mgpcg_mixed over the handle's nprob
+ * problems: one launch sequence drives all of them, each with its own
+ * tolerance abstol[q] and convergence mask (a converged problem's work is
+ * skipped on the device while the others iterate).  Host outputs are arrays of
+ * nprob (each nullable); seconds[q] is the device wall-clock time from the
+ * solve's start to the moment problem q stopped iterating (-1 if it never
+ * did), status[q] its own sbp_status.  Returns the worst status over the
+ * problems.  No counterpart in the
+ * reference (one problem per matrix_free_MGCG call, initial_guess_cg.jl:303-346). */
+int sbp_mgpcg_batch(sbp_handle h, const double* b, double* x, const double* abstol, int maxiter,
+                    int* iters, double* resnorm, double* seconds, int* status);
+int sbp_mgpcg_mixed_batch(sbp_handle h, const double* b, double* x, const double* abstol,
+                          int max_outer, int* outer_iters, int* inner_iters, double* resnorm,
+                          double* seconds, int* status);
 
 /* Gradient of L = 1/2 (u - u_obs)' H~ (u - u_obs) with u = A^-1 b(f, g)
  * (u: the forward solution): the adjoint solve A^T lambda = H~ (u - u_obs)

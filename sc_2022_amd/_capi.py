@@ -57,6 +57,8 @@ SIGNATURES = {
     "sbp_status_string": (C.c_char_p, [_i]),
     "sbp_default_params": (None, [C.POINTER(SbpParams)]),
     "sbp_create": (_i, [_i, _i, C.POINTER(SbpParams), C.POINTER(_vp)]),
+    "sbp_create_batch": (_i, [_i, _i, _i, C.POINTER(SbpParams), C.POINTER(_vp)]),
+    "sbp_get_batch": (_i, [_vp, _pi]),
     "sbp_destroy": (_i, [_vp]),
     "sbp_set_stream": (_i, [_vp, _vp]),
     "sbp_get_info": (_i, [_vp, _pi, _pi, _pi]),
@@ -76,6 +78,8 @@ SIGNATURES = {
     "sbp_cg": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd, _vp]),
     "sbp_mgpcg": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd, _vp]),
     "sbp_mgpcg_mixed": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pi, _pd]),
+    "sbp_mgpcg_batch": (_i, [_vp, _vp, _vp, _pd, _i, _pi, _pd, _pd, _pi]),
+    "sbp_mgpcg_mixed_batch": (_i, [_vp, _vp, _vp, _pd, _i, _pi, _pi, _pd, _pd, _pi]),
     "sbp_adjoint_grad": (_i, [_vp, _vp, _vp, _d, _i, _pd, _vp, _vp, _vp, _vp, _vp, _vp, _pi]),
     "sbp_csr_assemble": (_i, [_vp, _vp, _vp, _vp, C.POINTER(C.c_int64)]),
     "sbp_csr_spmv": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),

@@ -3,11 +3,13 @@
 // The handle owns everything the reference re-allocated per call (the MG
 // hierarchy buffers of initial_guess_cg.jl:152-161, the apply temporaries of
 // split_matrix_free.jl:459-462, the coarse LU of initial_guess_cg.jl:521),
-// allocated once in sbp_create.  Krylov scalars live on the device; kernels
-// of an iteration test a device "done" flag and fall through once the
-// iteration has converged, so the host only polls every poll_every
-// iterations while the iteration count keeps the reference's semantics
-// (the first k with ||r_k||_2 < abstol).
+// allocated once at creation.  A handle holds nprob >= 1 independent problems
+// of one size and order stacked in its buffers (problem q at offset q * n of
+// every grid function); every launch covers all of them.  Krylov scalars live
+// on the device, one block per problem; kernels test that problem's "done"
+// flag and fall through once it has converged, so the host only polls every
+// poll_every iterations while each problem's iteration count keeps the
+// reference's semantics (the first k with ||r_k||_2 < abstol).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,11 +33,12 @@ constexpr int kEwThreads = 256;
 
 struct Level {
     int N = 0;
-    size_t n = 0;
+    size_t n = 0;                         // points of one problem
     OpTab tb{};
     OpTab* dtab = nullptr;                // device copy of tb (tb.dev points here)
     int ti[2] = {0, 0}, tj[2] = {0, 0};   // tile shape per precision (0 = f64, 1 = f32)
-    // per precision (0 = f64, 1 = f32): rhs, two sweep buffers, output, residual
+    // per precision (0 = f64, 1 = f32), nprob problems each: rhs, two sweep
+    // buffers, output, residual
     void* b[2] = {nullptr, nullptr};
     void* xa[2] = {nullptr, nullptr};
     void* xb[2] = {nullptr, nullptr};
@@ -46,7 +49,7 @@ struct Level {
 }  // namespace
 
 struct sbp_handle_s {
-    int N = 0, p = 0, nlev = 0;
+    int N = 0, p = 0, nlev = 0, nprob = 1;
     bool mg = false;
     sbp_params prm{};
     hipStream_t st = nullptr;
@@ -54,21 +57,23 @@ struct sbp_handle_s {
     Level L[kMaxLevels];
     double* Ainv = nullptr;
     int nc = 0;
-    // fine Krylov vectors
-    double *r = nullptr, *pv = nullptr, *pv2 = nullptr, *Ap = nullptr, *z = nullptr;
+    // fine Krylov vectors (nprob problems each)
+    double *r = nullptr, *pv = nullptr, *pv2 = nullptr, *Ap = nullptr, *z = nullptr, *x = nullptr;
     float *r32 = nullptr, *p32 = nullptr, *p32b = nullptr, *Ap32 = nullptr, *z32 = nullptr, *d32 = nullptr;
-    double* partial = nullptr;
-    size_t npartial = 0;
-    Scalars* sc = nullptr;      // outer / FP64 solver
-    Scalars* sc_in = nullptr;   // inner FP32 solver
-    Scalars* sc_host = nullptr; // pinned mirror
-    double* dtmp = nullptr;     // small device scalars
-    double* x = nullptr;        // internal FP64 iterate (solvers work on it, then copy out)
-    double* w = nullptr;        // adjoint source (allocated on first use)
-    double* lam = nullptr;      // adjoint state when the caller passes none
+    double* partial = nullptr;   // block partials, nprob * npp slots
+    size_t npp = 0;
+    Scalars* sc = nullptr;       // outer / FP64 solver, nprob blocks
+    Scalars* sc_in = nullptr;    // inner FP32 solver, nprob blocks
+    Scalars* sc_host = nullptr;  // pinned mirror, nprob blocks
+    double* dabs = nullptr;      // device per-problem tolerances
+    double* habs = nullptr;      // pinned staging for them
+    double* dtmp = nullptr;      // small device scalars (nprob)
+    double* w = nullptr;         // adjoint source (allocated on first use)
+    double* lam = nullptr;       // adjoint state when the caller passes none
     bool have32 = false;
+    double clock_khz = 0.0;      // wall_clock64 rate
     // Captured Krylov iterations: [precision][0 = CG, 1 = MG-PCG]
-    hipStream_t cap = nullptr;  // capture stream (default priority)
+    hipStream_t cap = nullptr;   // capture stream (default priority)
     hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
@@ -114,15 +119,52 @@ template <typename T> inline int prec_index();
 template <> inline int prec_index<double>() { return 0; }
 template <> inline int prec_index<float>() { return 1; }
 
+inline Batch batch_of(const sbp_handle h, int l) {
+    Batch bt;
+    bt.nprob = h->nprob;
+    bt.sf = (long long)h->L[l].n;
+    bt.sc = l + 1 < h->nlev ? (long long)h->L[l + 1].n : 0;
+    bt.ss = kScalarDoubles;
+    return bt;
+}
+
+inline dim3 ew_grid(const sbp_handle h, size_t n) { return dim3(ew_blocks(n), h->nprob); }
+
+// One stencil pass on level l over every problem; *nparts receives the
+// number of block partials per problem.
 template <typename T, class IN, class OUT>
-int launch_pass(int p, const Level& L, const IN& in, const OUT& out, const int* done, double* partial,
-                hipStream_t st, int* nparts) {
+int launch_pass(sbp_handle h, int l, const IN& in, const OUT& out, const Scalars* scb, double* partial,
+                int* nparts) {
+    const Level& L = h->L[l];
     const int ti = L.ti[prec_index<T>()], tj = L.tj[prec_index<T>()];
     if (nparts) *nparts = (int)pass_blocks(L.N, ti, tj);
+    PassCtl c;
+    c.done = scb ? &scb->done : nullptr;
+    c.ds = kScalarInts;
+    c.partial = partial;
+    c.nbx = c.nby = 0;
+    c.bt = batch_of(h, l);
 #define SBP_TILE_CASE(TI_, J_) \
-    if (ti == TI_ && tj == J_) return launch_tile<TI_, J_, T, IN, OUT>(p, L.tb, in, out, done, partial, st);
+    if (ti == TI_ && tj == J_) return launch_tile<TI_, J_, T, IN, OUT>(h->p, L.tb, in, out, c, h->st);
     SBP_TILE_SHAPES(SBP_TILE_CASE)
 #undef SBP_TILE_CASE
+    return SBP_ERR_INVALID_ARG;
+}
+
+// rc = scale * FW(b - A u) from level l to l+1, every problem.
+template <typename T>
+int launch_rr(sbp_handle h, int l, const T* u, const T* b, T* rc, const Scalars* scb) {
+    const Level& L = h->L[l];
+    const int ti = L.ti[prec_index<T>()], tj = L.tj[prec_index<T>()];
+    const Batch bt = batch_of(h, l);
+    const int Nc = h->L[l + 1].N;
+    const T scale = (T)h->prm.restrict_scale;
+    const int* done = scb ? &scb->done : nullptr;
+#define SBP_RR_CASE(TI_, J_) \
+    if (ti == TI_ && tj == J_)   \
+        return launch_resid_restrict<TI_, J_, T>(h->p, L.tb, u, b, rc, Nc, scale, done, kScalarInts, bt, h->st);
+    SBP_TILE_SHAPES(SBP_RR_CASE)
+#undef SBP_RR_CASE
     return SBP_ERR_INVALID_ARG;
 }
 
@@ -130,7 +172,7 @@ int alloc_level_buffers(sbp_handle h, int prec) {
     const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
     for (int l = 0; l < h->nlev; ++l) {
         Level& L = h->L[l];
-        const size_t bytes = L.n * es;
+        const size_t bytes = L.n * es * h->nprob;
         if (l > 0) HIPCHK(hipMalloc(&L.b[prec], bytes));
         HIPCHK(hipMalloc(&L.xo[prec], bytes));
         if (l < h->nlev - 1) {
@@ -142,94 +184,80 @@ int alloc_level_buffers(sbp_handle h, int prec) {
     return SBP_OK;
 }
 
-template <typename T>
-int launch_rr(int p, const Level& L, const T* u, const T* b, T* rc, int Nc, T scale, const int* done, hipStream_t st) {
-    const int ti = L.ti[prec_index<T>()], tj = L.tj[prec_index<T>()];
-#define SBP_RR_CASE(TI_, J_) \
-    if (ti == TI_ && tj == J_) return launch_resid_restrict<TI_, J_, T>(p, L.tb, u, b, rc, Nc, scale, done, st);
-    SBP_TILE_SHAPES(SBP_RR_CASE)
-#undef SBP_RR_CASE
-    return SBP_ERR_INVALID_ARG;
-}
-
 // ---------------------------------------------------------------------------
 // Smoothing / V-cycle
 // ---------------------------------------------------------------------------
 // One smoothing pass: dst = S(operand) with the operand formed by IN.
 template <typename T, class IN>
-int smooth_pass(sbp_handle h, const Level& L, const IN& in, const T* b, T* dst, const T* dotv, const int* done,
+int smooth_pass(sbp_handle h, int l, const IN& in, const T* b, T* dst, const T* dotv, const Scalars* scb,
                 double* partial) {
     const bool rich = h->prm.smoother == 1;
     const T om = (T)(rich ? h->prm.richardson_omega : h->prm.omega);
     if (dotv) {
         OutSmoothDot<T> o{b, dst, om, dotv, rich ? 0 : 1};
-        return launch_pass<T>(h->p, L, in, o, done, partial, h->st, nullptr);
+        return launch_pass<T>(h, l, in, o, scb, partial, nullptr);
     }
     OutSmooth<T> o{b, dst, om, rich ? 0 : 1};
-    return launch_pass<T>(h->p, L, in, o, done, nullptr, h->st, nullptr);
+    return launch_pass<T>(h, l, in, o, scb, nullptr, nullptr);
 }
 
-// V-cycle on level l: x_out ~ A_l^-1 b (zero initial guess).  When dotv is
-// non-null the last pass also writes block partials of dotv' x_out.
+// V-cycle on level l: x_out ~ A_l^-1 b (zero initial guess), every problem.
+// When dotv is non-null the last pass also writes block partials of dotv' x_out.
 template <typename T>
-int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const int* done, double* partial) {
+int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const Scalars* scb, double* partial) {
     const int pi = prec_index<T>();
     Level& L = h->L[l];
     hipStream_t st = h->st;
     if (l == h->nlev - 1) {
-        hipLaunchKernelGGL((coarse_solve<T>), dim3(((int)L.n + kCoarseRows - 1) / kCoarseRows), dim3(256), 0, st,
-                           h->Ainv, b, out, (int)L.n, done);
+        hipLaunchKernelGGL((coarse_solve<T>), dim3(((int)L.n + kCoarseRows - 1) / kCoarseRows, h->nprob), dim3(256), 0,
+                           st, h->Ainv, b, out, (int)L.n, scb);
         HIPCHK(hipGetLastError());
         if (dotv) {
-            hipLaunchKernelGGL((dot_partial<T, kEwThreads>), dim3(1), dim3(kEwThreads), 0, st, dotv, (const T*)out,
-                               L.n, partial, nullptr);
+            hipLaunchKernelGGL((dot_partial<T, kEwThreads>), dim3(1, h->nprob), dim3(kEwThreads), 0, st, dotv,
+                               (const T*)out, L.n, partial);
             HIPCHK(hipGetLastError());
         }
         return SBP_OK;
     }
     T* xa = (T*)L.xa[pi];
     T* xb = (T*)L.xb[pi];
-    T* rr = (T*)L.r[pi];
     const int nu1 = h->prm.nu_pre, nu2 = h->prm.nu_post;
     const T om = (T)(h->prm.smoother == 1 ? h->prm.richardson_omega : h->prm.omega);
     // Pre-smoothing from zero.  The first sweep x1 = w D^-1 b is formed on
     // the fly inside the second sweep's pass.
     T* cur = xa;
     if (nu1 == 0) {
-        hipLaunchKernelGGL((fill<T, kEwThreads>), dim3(ew_blocks(L.n)), dim3(kEwThreads), 0, st, xa, T(0), L.n, done);
+        hipLaunchKernelGGL((fill<T, kEwThreads>), ew_grid(h, L.n), dim3(kEwThreads), 0, st, xa, T(0), L.n, scb);
         HIPCHK(hipGetLastError());
     } else {
         const InSmooth0<T> in0{b, om, h->prm.smoother == 1 ? 0 : 1};
         if (nu1 == 1) {
-            STCHK(launch_pass<T>(h->p, L, in0, OutCopy<T>{xa}, done, nullptr, st, nullptr));
+            STCHK(launch_pass<T>(h, l, in0, OutCopy<T>{xa}, scb, nullptr, nullptr));
         } else {
-            STCHK(smooth_pass<T>(h, L, in0, b, xa, nullptr, done, nullptr));
+            STCHK(smooth_pass<T>(h, l, in0, b, xa, nullptr, scb, nullptr));
             for (int s = 2; s < nu1; ++s) {
                 T* nxt = (cur == xa) ? xb : xa;
-                STCHK(smooth_pass<T>(h, L, InPlain<T>{cur}, b, nxt, nullptr, done, nullptr));
+                STCHK(smooth_pass<T>(h, l, InPlain<T>{cur}, b, nxt, nullptr, scb, nullptr));
                 cur = nxt;
             }
         }
     }
     // Residual and full-weighting restriction, fused (the fine residual stays on chip).
-    {
-        Level& Lc = h->L[l + 1];
-        STCHK(launch_rr<T>(h->p, L, (const T*)cur, b, (T*)Lc.b[pi], Lc.N, (T)h->prm.restrict_scale, done, st));
-    }
     Level& Lc = h->L[l + 1];
+    STCHK(launch_rr<T>(h, l, (const T*)cur, b, (T*)Lc.b[pi], scb));
     T* ec = (T*)Lc.xo[pi];
-    STCHK(vcycle_level<T>(h, l + 1, (const T*)Lc.b[pi], ec, nullptr, done, nullptr));
+    STCHK(vcycle_level<T>(h, l + 1, (const T*)Lc.b[pi], ec, nullptr, scb, nullptr));
     // Prolongation folded into the first post-smoothing pass.
     if (nu2 == 0) {
-        hipLaunchKernelGGL((prolong_add<T>), dim3((L.N + 255) / 256, L.N), dim3(256), 0, st, (const T*)ec, cur, L.N,
-                           Lc.N, done);
+        hipLaunchKernelGGL((prolong_add<T>), dim3((L.N + 255) / 256, L.N, h->nprob), dim3(256), 0, st, (const T*)ec,
+                           cur, L.N, Lc.N, scb);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(L.n)), dim3(kEwThreads), 0, st, (const T*)cur,
-                           out, L.n, done);
+        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), ew_grid(h, L.n), dim3(kEwThreads), 0, st, (const T*)cur, out,
+                           L.n, scb);
         HIPCHK(hipGetLastError());
         if (dotv) {
-            hipLaunchKernelGGL((dot_partial<T, kEwThreads>), dim3(ew_blocks(L.n)), dim3(kEwThreads), 0, st, dotv,
-                               (const T*)out, L.n, partial, nullptr);
+            hipLaunchKernelGGL((dot_partial<T, kEwThreads>), ew_grid(h, L.n), dim3(kEwThreads), 0, st, dotv,
+                               (const T*)out, L.n, partial);
             HIPCHK(hipGetLastError());
         }
         return SBP_OK;
@@ -237,20 +265,20 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const i
     T* other = (cur == xa) ? xb : xa;
     {
         T* dst = (nu2 == 1) ? out : other;
-        STCHK(smooth_pass<T>(h, L, InProlong<T>{cur, ec, Lc.N}, b, dst, nu2 == 1 ? dotv : nullptr, done,
+        STCHK(smooth_pass<T>(h, l, InProlong<T>{cur, ec, Lc.N}, b, dst, nu2 == 1 ? dotv : nullptr, scb,
                              nu2 == 1 ? partial : nullptr));
         cur = dst;
     }
     for (int s = 1; s < nu2; ++s) {
         const bool last = (s == nu2 - 1);
         T* dst = last ? out : ((cur == xa) ? xb : xa);
-        STCHK(smooth_pass<T>(h, L, InPlain<T>{cur}, b, dst, last ? dotv : nullptr, done, last ? partial : nullptr));
+        STCHK(smooth_pass<T>(h, l, InPlain<T>{cur}, b, dst, last ? dotv : nullptr, scb, last ? partial : nullptr));
         cur = dst;
     }
     return SBP_OK;
 }
 
-// Number of partials the V-cycle's final pass writes on level 0.
+// Number of partials per problem the V-cycle's final pass writes on level 0.
 template <typename T>
 int vcycle_nparts(sbp_handle h) {
     const Level& L = h->L[0];
@@ -269,23 +297,40 @@ void drop_graphs(sbp_handle h) {
         }
 }
 
-int poll(sbp_handle h, Scalars* sc) {
-    HIPCHK(hipMemcpyAsync(h->sc_host, sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->st));
+// Copies every problem's scalar block to the pinned mirror; *all_done (when
+// given) tells whether every problem is done.
+int poll(sbp_handle h, const Scalars* sc, bool* all_done = nullptr) {
+    HIPCHK(hipMemcpyAsync(h->sc_host, sc, sizeof(Scalars) * h->nprob, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
+    if (all_done) {
+        bool d = true;
+        for (int q = 0; q < h->nprob; ++q) d = d && h->sc_host[q].done;
+        *all_done = d;
+    }
     return SBP_OK;
 }
 
-int finish_status(const Scalars& s, int maxiter_hit) {
+int status_of(const Scalars& s) {
     if (s.status == 5) return SBP_ERR_NONFINITE;
     if (s.status == 6) return SBP_ERR_BREAKDOWN;
-    if (s.status == 7) return SBP_ERR_NOT_CONVERGED;
-    if (!s.done && maxiter_hit) return SBP_ERR_NOT_CONVERGED;
+    if (s.status == 7 || !s.done) return SBP_ERR_NOT_CONVERGED;
     return SBP_OK;
+}
+
+// Worst status over the problems (errors first, then not-converged).
+int batch_status(sbp_handle h) {
+    int worst = SBP_OK;
+    for (int q = 0; q < h->nprob; ++q) {
+        const int s = status_of(h->sc_host[q]);
+        if (s == SBP_ERR_NONFINITE || s == SBP_ERR_BREAKDOWN) return s;
+        if (s != SBP_OK) worst = s;
+    }
+    return worst;
 }
 
 int ensure32(sbp_handle h) {
     if (h->have32) return SBP_OK;
-    const size_t n = h->L[0].n;
+    const size_t n = h->L[0].n * h->nprob;
     HIPCHK(hipMalloc(&h->r32, n * sizeof(float)));
     HIPCHK(hipMalloc(&h->p32, n * sizeof(float)));
     HIPCHK(hipMalloc(&h->p32b, n * sizeof(float)));
@@ -307,32 +352,42 @@ struct KVecs {
     T* z;      // preconditioned residual
 };
 
+template <typename T> KVecs<T> kvecs(sbp_handle h);
+template <> KVecs<double> kvecs<double>(sbp_handle h) {
+    KVecs<double> v{h->x, h->r, {h->pv, h->pv2}, h->Ap, h->z};
+    return v;
+}
+template <> KVecs<float> kvecs<float>(sbp_handle h) {
+    KVecs<float> v{h->d32, h->r32, {h->p32, h->p32b}, h->Ap32, h->z32};
+    return v;
+}
+
 // Iteration k (1-based, k % 2 == par): p_k = z_{k-1} + beta p_{k-1} formed on
 // the fly inside the apply, which also writes A p_k, accumulates p_k'A p_k and
 // applies the pending x += alpha_{k-1} p_{k-1}; then alpha, r -= alpha A p_k
 // with r'r, the convergence test, and (MG) z = V(r) with r'z and beta.
-// Every kernel tests the device "done" flag.
+// Every kernel tests its problem's "done" flag.
 template <typename T>
 int issue_iteration(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, int par) {
     hipStream_t st = h->st;
-    const Level& L0 = h->L[0];
-    const size_t n = L0.n;
-    const int* done = &sc->done;
+    const size_t n = h->L[0].n;
     T* pold = v.p[par ^ 1];
     T* pnew = v.p[par];
     const T* zz = use_mg ? v.z : v.r;
     int np = 0;
-    STCHK(launch_pass<T>(h->p, L0, InAxpy<T>{zz, pold, &sc->beta, T(0)},
-                         OutPUpd<T>{pnew, v.Ap, v.x, pold, &sc->alpha, T(0)}, done, h->partial, st, &np));
-    hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, use_mg ? 1 : 0);
-    const int eb = ew_blocks(n);
-    hipLaunchKernelGGL((update_r<T, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, v.r, (const T*)v.Ap,
+    STCHK(launch_pass<T>(h, 0, InAxpy<T>{zz, pold, &sc->beta, T(0)},
+                         OutPUpd<T>{pnew, v.Ap, v.x, pold, &sc->alpha, T(0)}, sc, h->partial, &np));
+    hipLaunchKernelGGL(krylov_alpha, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, np, sc, use_mg ? 1 : 0);
+    const dim3 g = ew_grid(h, n);
+    hipLaunchKernelGGL((update_r<T, kEwThreads>), g, dim3(kEwThreads), 0, st, v.r, (const T*)v.Ap,
                        (const Scalars*)sc, n, h->partial);
-    hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, eb, sc, use_mg ? 0 : 1);
+    hipLaunchKernelGGL(krylov_check, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc,
+                       use_mg ? 0 : 1);
     HIPCHK(hipGetLastError());
     if (use_mg) {
-        STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, done, h->partial));
-        hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(kRedThreads), 0, st, h->partial, vcycle_nparts<T>(h), sc);
+        STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, sc, h->partial));
+        hipLaunchKernelGGL(krylov_beta, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, vcycle_nparts<T>(h),
+                           sc);
         HIPCHK(hipGetLastError());
     }
     return SBP_OK;
@@ -367,24 +422,26 @@ int iteration_graph(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, h
     return SBP_OK;
 }
 
-// Preconditioned CG body shared by FP64 and FP32: solves A x = b given the
-// initial residual in v.r (sc->rr / abstol / maxiter / done already set by the
+// Preconditioned CG body shared by FP64 and FP32: solves A x = b for every
+// problem given the initial residuals in v.r (scalar blocks already set by the
 // caller's init kernel); the preconditioner is the V-cycle when use_mg, else
-// the identity.  The host polls every poll_every iterations.
+// the identity.  The host polls every poll_every iterations and stops issuing
+// once every problem is done.  Leaves the scalar blocks in h->sc_host.
 template <typename T>
-int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use_mg, int* iters_out) {
+int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use_mg) {
     hipStream_t st = h->st;
     const size_t n = h->L[0].n;
-    const int* done = &sc->done;
     const T* z0 = v.r;
     if (use_mg) {
-        STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, done, h->partial));
-        hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(kRedThreads), 0, st, h->partial, vcycle_nparts<T>(h), sc);
+        STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, sc, h->partial));
+        hipLaunchKernelGGL(krylov_set_rz, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, vcycle_nparts<T>(h),
+                           sc);
         HIPCHK(hipGetLastError());
         z0 = v.z;
     }
     // p_0 := z_0 (any finite values serve: alpha = beta = 0 in the first pass)
-    hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, z0, v.p[0], n, done);
+    hipLaunchKernelGGL((copy_vec<T, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, z0, v.p[0], n,
+                       (const Scalars*)sc);
     HIPCHK(hipGetLastError());
     hipGraphExec_t ex = nullptr;
     if (h->prm.use_graph) STCHK(iteration_graph<T>(h, v, sc, use_mg, &ex));
@@ -397,36 +454,229 @@ int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use
             STCHK(issue_iteration<T>(h, v, sc, use_mg, 0));
         }
         if ((it + 2) % poll_every == 0 && it + 2 < maxiter) {
-            STCHK(poll(h, sc));
-            if (h->sc_host->done) break;
+            bool all = false;
+            STCHK(poll(h, sc, &all));
+            if (all) break;
         }
     }
-    hipLaunchKernelGGL((flush_x<T, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, v.x,
-                       (const T*)v.p[0], (const T*)v.p[1], (const Scalars*)sc, n);
+    hipLaunchKernelGGL((flush_x<T, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, v.x, (const T*)v.p[0],
+                       (const T*)v.p[1], (const Scalars*)sc, n);
     HIPCHK(hipGetLastError());
-    STCHK(poll(h, sc));
-    if (iters_out) *iters_out = h->sc_host->iters;
-    return finish_status(*h->sc_host, !h->sc_host->done);
+    return poll(h, sc);
 }
 
-template <typename T> KVecs<T> kvecs(sbp_handle h);
-template <> KVecs<double> kvecs<double>(sbp_handle h) {
-    KVecs<double> v{h->x, h->r, {h->pv, h->pv2}, h->Ap, h->z};
-    return v;
-}
-template <> KVecs<float> kvecs<float>(sbp_handle h) {
-    KVecs<float> v{h->d32, h->r32, {h->p32, h->p32b}, h->Ap32, h->z32};
-    return v;
+// Per-problem tolerances to the device (through pinned staging, stream-ordered).
+int put_abstol(sbp_handle h, const double* abstol) {
+    HIPCHK(hipStreamSynchronize(h->st));   // the staging buffer may still feed an earlier copy
+    for (int q = 0; q < h->nprob; ++q) h->habs[q] = abstol[q];
+    HIPCHK(hipMemcpyAsync(h->dabs, h->habs, sizeof(double) * h->nprob, hipMemcpyHostToDevice, h->st));
+    return SBP_OK;
 }
 
-}  // namespace
+// Per-problem results from the pinned mirror (seconds: device wall clock from
+// the solve's start to the moment the problem stopped, -1 when it never did).
+void get_results(sbp_handle h, int* iters, double* resnorm, double* seconds) {
+    for (int q = 0; q < h->nprob; ++q) {
+        const Scalars& s = h->sc_host[q];
+        if (iters) iters[q] = s.iters;
+        if (resnorm) resnorm[q] = s.resnorm;
+        if (seconds)
+            seconds[q] = (s.t1 > s.t0 && h->clock_khz > 0) ? (double)(s.t1 - s.t0) / (h->clock_khz * 1e3) : -1.0;
+    }
+}
 
-namespace {
+// FP64 (P)CG on the internal iterate: x -> h->x, init, iterate, h->x -> x.
+int krylov64(sbp_handle h, const double* b, double* x, const double* abstol, int maxiter, double* hist,
+             bool use_mg) {
+    const size_t n = h->L[0].n;
+    hipStream_t st = h->st;
+    STCHK(put_abstol(h, abstol));
+    hipLaunchKernelGGL(stamp_start, dim3(1), dim3(256), 0, st, h->sc, h->nprob);
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, x, h->x, n,
+                       (const Scalars*)nullptr);
+    HIPCHK(hipGetLastError());
+    int np = 0;
+    STCHK(launch_pass<double>(h, 0, InPlain<double>{h->x}, OutResid<double, double>{b, h->r}, nullptr, h->partial,
+                              &np));
+    hipLaunchKernelGGL(krylov_init, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, np, h->sc,
+                       (const double*)h->dabs, hist, maxiter);
+    HIPCHK(hipGetLastError());
+    STCHK(pcg_loop<double>(h, kvecs<double>(h), h->sc, maxiter, use_mg));
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, (const double*)h->x,
+                       x, n, (const Scalars*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return SBP_OK;
+}
+
+// Outer refinement check: resnorm = ||r||; converged -> both done; else arm
+// the inner solver with abstol = inner_rtol * ||r|| and its iteration cap.
+__global__ __launch_bounds__(1024) void ir_check(const double* partial, int np, Scalars* sob, Scalars* sib,
+                                                double inner_rtol, int inner_maxiter) {
+    Scalars* so = sob + blockIdx.x;
+    Scalars* si = sib + blockIdx.x;
+    double rs = sum_partials(partial + (size_t)blockIdx.x * np, np);
+    if (threadIdx.x == 0) {
+        si->iters = 0;
+        si->status = 0;
+        si->alpha = si->beta = 0.0;
+        si->hist = nullptr;
+        si->maxiter = inner_maxiter;
+        if (so->done) { si->done = 1; return; }
+        const double nr = sqrt(rs);
+        so->resnorm = nr;
+        so->rr = rs;
+        si->rr = rs;
+        si->resnorm = nr;
+        if (!isfinite(rs)) { so->status = 5; mark_done(so); si->done = 1; return; }
+        if (nr < so->abstol) { mark_done(so); si->done = 1; return; }
+        si->abstol = inner_rtol * nr;
+        si->done = 0;
+    }
+}
+
+__global__ void ir_init(Scalars* so, const double* abstol, int nprob) {
+    const unsigned long long t = wall_clock64();
+    for (int q = threadIdx.x; q < nprob; q += blockDim.x) {
+        so[q].abstol = abstol[q];
+        so[q].done = 0;
+        so[q].status = 0;
+        so[q].iters = 0;
+        so[q].dprev = 0.0;
+        so[q].est = 0.0;
+        so[q].t0 = t;
+        so[q].t1 = 0;
+    }
+}
+
+// x64 += (double) d32 with block partials of ||d||^2 (slot 2b) and ||x||^2
+// (slot 2b+1).  Problems whose refinement has finished are left alone; a
+// problem whose inner solve failed (non-finite or breakdown) takes that status
+// and stops without the update.
+template <int NT>
+__global__ __launch_bounds__(NT) void ir_update(double* __restrict__ x, const float* __restrict__ d, size_t n,
+                                               Scalars* sob, const Scalars* sib, double* partial) {
+    const int q = blockIdx.y;
+    if (sob[q].done) return;
+    const int ist = sib[q].status;
+    if (ist == 5 || ist == 6) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sob[q].status = ist;
+            mark_done(sob + q);
+        }
+        return;
+    }
+    x += q * n;
+    d += q * n;
+    double dd = 0.0, xx = 0.0;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) {
+        const double dv = (double)d[k];
+        const double xv = x[k] + dv;
+        x[k] = xv;
+        dd += dv * dv;
+        xx += xv * xv;
+    }
+    const double sd = block_sum<double, NT>(dd);
+    __syncthreads();
+    const double sx = block_sum<double, NT>(xx);
+    if (threadIdx.x == 0) {
+        double* pq = partial + (size_t)q * 2 * gridDim.x;
+        pq[2 * blockIdx.x] = sd;
+        pq[2 * blockIdx.x + 1] = sx;
+    }
+}
+
+// Error estimate after correction k: ||e_k|| ~ ||d_k|| * min(1, ||d_k|| / ||d_{k-1}||)
+// (the observed contraction of the refinement).  Stops that problem's
+// refinement when est <= ir_rtol * ||x|| (ir_rtol > 0).
+__global__ __launch_bounds__(1024) void ir_estimate(const double* partial, int nb, Scalars* sob, double ir_rtol) {
+    Scalars* so = sob + blockIdx.x;
+    if (so->done) return;
+    const double* pq = partial + (size_t)blockIdx.x * 2 * nb;
+    __shared__ double red[2][16];
+    double sd = 0.0, sx = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) { sd += pq[2 * k]; sx += pq[2 * k + 1]; }
+    sd = wave_sum(sd);
+    sx = wave_sum(sx);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[0][w] = sd; red[1][w] = sx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double td = 0.0, tx = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { td += red[0][k]; tx += red[1][k]; }
+        const double dn = sqrt(td), xn = sqrt(tx);
+        so->iters += 1;
+        const double ratio = so->dprev > 0.0 ? fmin(1.0, dn / so->dprev) : 1.0;
+        so->est = dn * ratio;
+        so->dprev = dn;
+        if (ir_rtol > 0.0 && so->iters >= 2 && so->est <= ir_rtol * xn) mark_done(so);
+    }
+}
+
+// Mixed-precision refinement over every problem.  inner_tot (host, nprob)
+// receives each problem's total inner iterations.  The final scalar blocks
+// are left in h->sc_host.
+int mixed_solve(sbp_handle h, const double* b, double* x, const double* abstol, int max_outer, int* inner_tot) {
+    STCHK(ensure32(h));
+    hipStream_t st = h->st;
+    const size_t n = h->L[0].n;
+    const int ub = std::min<int>(ew_blocks(n), (int)(h->npp / 2));
+    STCHK(put_abstol(h, abstol));
+    hipLaunchKernelGGL(ir_init, dim3(1), dim3(256), 0, st, h->sc, (const double*)h->dabs, h->nprob);
+    HIPCHK(hipGetLastError());
+    for (int q = 0; q < h->nprob; ++q) inner_tot[q] = 0;
+    for (int o = 0; o <= max_outer; ++o) {
+        int np = 0;
+        // FP64 residual r = b - A x, cast to FP32 for the inner solve.
+        STCHK(launch_pass<double>(h, 0, InPlain<double>{x}, OutResid<double, float>{b, h->r32}, h->sc, h->partial,
+                                  &np));
+        hipLaunchKernelGGL(ir_check, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, np, h->sc, h->sc_in,
+                           h->prm.inner_rtol, h->prm.inner_maxiter);
+        HIPCHK(hipGetLastError());
+        bool all = false;
+        STCHK(poll(h, h->sc, &all));
+        if (all || o == max_outer) break;
+        // Inner FP32 MG-PCG on A d = r from d = 0.
+        hipLaunchKernelGGL((fill<float, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, h->d32, 0.0f, n,
+                           (const Scalars*)h->sc_in);
+        HIPCHK(hipGetLastError());
+        STCHK(pcg_loop<float>(h, kvecs<float>(h), h->sc_in, h->prm.inner_maxiter, true));
+        for (int q = 0; q < h->nprob; ++q) inner_tot[q] += h->sc_host[q].iters;
+        // FP64 update and the error estimate from the correction norms.
+        hipLaunchKernelGGL((ir_update<kEwThreads>), dim3(ub, h->nprob), dim3(kEwThreads), 0, st, x,
+                           (const float*)h->d32, n, h->sc, (const Scalars*)h->sc_in, h->partial);
+        hipLaunchKernelGGL(ir_estimate, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, ub, h->sc,
+                           h->prm.ir_rtol);
+        HIPCHK(hipGetLastError());
+    }
+    return poll(h, h->sc);
+}
+
+// Status of one finished refinement.
+int mixed_status_of(const Scalars& s) {
+    if (s.status == 5) return SBP_ERR_NONFINITE;
+    if (s.status == 6) return SBP_ERR_BREAKDOWN;
+    return s.done ? SBP_OK : SBP_ERR_NOT_CONVERGED;
+}
+
+// Worst refinement status over the problems (errors first, then not-converged).
+int mixed_status(sbp_handle h) {
+    int worst = SBP_OK;
+    for (int q = 0; q < h->nprob; ++q) {
+        const int st = mixed_status_of(h->sc_host[q]);
+        if (st == SBP_ERR_NONFINITE || st == SBP_ERR_BREAKDOWN) return st;
+        if (st != SBP_OK) worst = st;
+    }
+    return worst;
+}
+
+// Smoothing sweeps on level l for every problem (ping-pong through a level
+// scratch buffer; the finest level borrows Ap / Ap32).
 template <typename T>
 int smooth_t(sbp_handle h, int level, const T* b, T* x, int nsweeps) {
     const int pi = prec_index<T>();
     Level& L = h->L[level];
-    // Ping-pong through a level scratch buffer (the finest level borrows Ap/Ap32).
     T* tmp;
     if (level < h->nlev - 1 && L.xb[pi]) tmp = (T*)L.xb[pi];
     else if (level == 0) tmp = pi == 0 ? (T*)(void*)h->Ap : (T*)(void*)h->Ap32;
@@ -435,49 +685,35 @@ int smooth_t(sbp_handle h, int level, const T* b, T* x, int nsweeps) {
     T* cur = x;
     for (int s = 0; s < nsweeps; ++s) {
         T* dst = (cur == x) ? tmp : x;
-        STCHK(smooth_pass<T>(h, L, InPlain<T>{cur}, b, dst, nullptr, nullptr, nullptr));
+        STCHK(smooth_pass<T>(h, level, InPlain<T>{cur}, b, dst, nullptr, nullptr, nullptr));
         cur = dst;
     }
     if (cur != x) {
-        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(L.n)), dim3(kEwThreads), 0, h->st, (const T*)cur,
-                           x, L.n, nullptr);
+        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), ew_grid(h, L.n), dim3(kEwThreads), 0, h->st, (const T*)cur, x,
+                           L.n, (const Scalars*)nullptr);
         HIPCHK(hipGetLastError());
     }
     return SBP_OK;
 }
-}  // namespace
 
-namespace {
-// r = b - A x ; rr / ||r0|| check into sc.
-int init_residual(sbp_handle h, const double* b, const double* x, double abstol, double* hist, int maxiter) {
-    int np = 0;
-    STCHK(launch_pass<double>(h->p, h->L[0], InPlain<double>{x}, OutResid<double, double>{b, h->r}, nullptr,
-                              h->partial, h->st, &np));
-    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(kRedThreads), 0, h->st, h->partial, np, h->sc, abstol, hist, maxiter);
-    HIPCHK(hipGetLastError());
-    return SBP_OK;
+__global__ void exclusive_scan_serial(const int32_t* counts, int32_t* rp, int n, int64_t* total) {
+    // One thread; used once per assembly.
+    int64_t s = 0;
+    for (int q = 0; q < n; ++q) { rp[q] = (int32_t)s; s += counts[q]; }
+    rp[n] = (int32_t)s;
+    *total = s;
 }
-}  // namespace
 
-namespace {
-// Runs the FP64 (P)CG on the internal iterate: x -> h->x, iterate, h->x -> x.
-int krylov64(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
-             double* hist, bool use_mg) {
-    const size_t n = h->L[0].n;
-    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, h->st,
-                       (const double*)x, h->x, n, nullptr);
-    HIPCHK(hipGetLastError());
-    STCHK(init_residual(h, b, h->x, abstol, hist, maxiter));
-    int it = 0;
-    int rc = pcg_loop<double>(h, kvecs<double>(h), h->sc, maxiter, use_mg, &it);
-    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, h->st,
-                       (const double*)h->x, x, n, nullptr);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(h->st));
-    if (iters) *iters = it;
-    if (resnorm) *resnorm = h->sc_host->resnorm;
-    return rc;
+__global__ __launch_bounds__(256) void copy_vec16(const uint4* __restrict__ s, uint4* __restrict__ d, size_t n16) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n16; q += stride) d[q] = s[q];
 }
+__global__ __launch_bounds__(256) void copy_u8(const unsigned char* __restrict__ s, unsigned char* __restrict__ d,
+                                              size_t n) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += stride) d[q] = s[q];
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -485,7 +721,7 @@ int krylov64(sbp_handle h, const double* b, double* x, double abstol, int maxite
 // ===========================================================================
 extern "C" {
 
-const char* sbp_version(void) { return "sc_2022_amd 0.1 (gfx950)"; }
+const char* sbp_version(void) { return "sc_2022_amd 0.2 (gfx950)"; }
 
 const char* sbp_status_string(int s) {
     switch (s) {
@@ -537,16 +773,17 @@ int sbp_destroy(sbp_handle h) {
     }
     drop_graphs(h);
     if (h->cap) (void)hipStreamDestroy(h->cap);
-    void* ptrs[] = {h->w, h->lam, h->Ainv, h->r, h->pv, h->pv2, h->Ap, h->z, h->x, h->r32, h->p32, h->p32b, h->Ap32, h->z32, h->d32,
-                    h->partial, h->sc, h->sc_in, h->dtmp};
+    void* ptrs[] = {h->w, h->lam, h->Ainv, h->r, h->pv, h->pv2, h->Ap, h->z, h->x, h->r32, h->p32, h->p32b, h->Ap32,
+                    h->z32, h->d32, h->partial, h->sc, h->sc_in, h->dtmp, h->dabs};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (h->sc_host) (void)hipHostFree(h->sc_host);
+    if (h->habs) (void)hipHostFree(h->habs);
     delete h;
     return SBP_OK;
 }
 
-int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) {
+int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle* out) {
     if (!out) return SBP_ERR_INVALID_ARG;
     *out = nullptr;
     if (p != 2 && p != 4) return SBP_ERR_INVALID_ORDER;
@@ -555,6 +792,8 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) {
         if (!build_optab(p, N, &t0, nullptr)) return SBP_ERR_INVALID_SIZE;
         if (N > 46340) return SBP_ERR_INVALID_SIZE;   // N*N must fit a 32-bit block index space
     }
+    if (nprob < 1 || nprob > 65535) return SBP_ERR_INVALID_ARG;   // problems ride grid.y of the elementwise kernels
+    if ((long long)N * N * nprob > (1ll << 40)) return SBP_ERR_INVALID_SIZE;
     if (prm && (prm->nu_pre < 0 || prm->nu_post < 0 || prm->poll_every < 1 || prm->max_outer < 1 ||
                 prm->inner_maxiter < 1 || !(prm->omega > 0) || !(prm->restrict_scale > 0) ||
                 (prm->smoother != 0 && prm->smoother != 1)))
@@ -565,6 +804,7 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) {
     if (!h) return SBP_ERR_HIP;
     h->N = N;
     h->p = p;
+    h->nprob = nprob;
     if (prm) h->prm = *prm; else sbp_default_params(&h->prm);
     if (!build_optab(p, N, &h->L[0].tb, &h->rt)) { delete h; return SBP_ERR_INVALID_SIZE; }
     h->L[0].N = N;
@@ -599,6 +839,10 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) {
             }
         }
     }
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess)
+        h->clock_khz = (double)khz;
     int rc = SBP_OK;
     do {
         for (int l = 0; l < nlev && rc == SBP_OK; ++l) {
@@ -608,19 +852,26 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) {
             if (hipMemcpy(L.dtab, &L.tb, sizeof(OpTab), hipMemcpyHostToDevice) != hipSuccess) rc = SBP_ERR_HIP;
         }
         if (rc != SBP_OK) break;
-        const size_t n = h->L[0].n;
+        const size_t n = h->L[0].n * (size_t)nprob;
         if (hipMalloc(&h->r, n * 8) != hipSuccess || hipMalloc(&h->pv, n * 8) != hipSuccess ||
             hipMalloc(&h->Ap, n * 8) != hipSuccess || hipMalloc(&h->z, n * 8) != hipSuccess ||
             hipMalloc(&h->x, n * 8) != hipSuccess || hipMalloc(&h->pv2, n * 8) != hipSuccess) { rc = SBP_ERR_HIP; break; }
-        // Enough partial slots for the finest pass at any compiled tile shape.
-        h->npartial = std::max<size_t>(pass_blocks(N, 64, 4), 4096) + 64;
-        if (hipMalloc(&h->partial, h->npartial * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
-        if (hipMalloc(&h->sc, sizeof(Scalars)) != hipSuccess || hipMalloc(&h->sc_in, sizeof(Scalars)) != hipSuccess ||
-            hipMalloc(&h->dtmp, 64 * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
-        if (hipHostMalloc((void**)&h->sc_host, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) { rc = SBP_ERR_HIP; break; }
-        if (hipMemset(h->sc, 0, sizeof(Scalars)) != hipSuccess || hipMemset(h->sc_in, 0, sizeof(Scalars)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+        // Partial slots per problem: the finest pass at any compiled tile shape,
+        // and the elementwise kernels (two per block in the refinement update).
+        h->npp = std::max<size_t>(pass_blocks(N, 64, 4), 2 * 4096) + 64;
+        if (hipMalloc(&h->partial, h->npp * nprob * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+        if (hipMalloc(&h->sc, sizeof(Scalars) * nprob) != hipSuccess ||
+            hipMalloc(&h->sc_in, sizeof(Scalars) * nprob) != hipSuccess ||
+            hipMalloc(&h->dtmp, nprob * sizeof(double)) != hipSuccess ||
+            hipMalloc(&h->dabs, nprob * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+        if (hipHostMalloc((void**)&h->sc_host, sizeof(Scalars) * nprob, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&h->habs, sizeof(double) * nprob, hipHostMallocDefault) != hipSuccess) {
+            rc = SBP_ERR_HIP;
+            break;
+        }
+        if (hipMemset(h->sc, 0, sizeof(Scalars) * nprob) != hipSuccess ||
+            hipMemset(h->sc_in, 0, sizeof(Scalars) * nprob) != hipSuccess) { rc = SBP_ERR_HIP; break; }
         if (h->mg) {
-            h->L[0].b[0] = nullptr;
             if ((rc = alloc_level_buffers(h, 0)) != SBP_OK) break;
             // Coarsest level: FP64 inverse of the rediscretised operator via Cholesky.
             const Level& Lc = h->L[nlev - 1];
@@ -631,7 +882,8 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) {
             h->nc = nc;
             if (hipMalloc(&h->Ainv, (size_t)nc * nc * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
             if (hipMemcpy(h->Ainv, A.data(), (size_t)nc * nc * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-                rc = SBP_ERR_HIP; break;
+                rc = SBP_ERR_HIP;
+                break;
             }
         }
     } while (0);
@@ -639,6 +891,8 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) {
     *out = h;
     return SBP_OK;
 }
+
+int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) { return sbp_create_batch(N, p, 1, prm, out); }
 
 int sbp_set_stream(sbp_handle h, void* s) {
     if (!h) return SBP_ERR_INVALID_ARG;
@@ -654,6 +908,12 @@ int sbp_get_info(sbp_handle h, int* N, int* p, int* nlevels) {
     return SBP_OK;
 }
 
+int sbp_get_batch(sbp_handle h, int* nprob) {
+    if (!h || !nprob) return SBP_ERR_INVALID_ARG;
+    *nprob = h->nprob;
+    return SBP_OK;
+}
+
 int sbp_set_tiles(sbp_handle h, int level, int dtype, int ti, int tj) {
     if (!h || level < 0 || level >= h->nlev || !tile_ok(ti, tj) || (dtype != SBP_F64 && dtype != SBP_F32))
         return SBP_ERR_INVALID_ARG;
@@ -665,23 +925,22 @@ int sbp_set_tiles(sbp_handle h, int level, int dtype, int ti, int tj) {
 
 int sbp_apply(sbp_handle h, int level, const void* u, void* Au, int sign, int dtype) {
     if (!h || !u || !Au || (sign != 1 && sign != -1) || level < 0 || level >= h->nlev) return SBP_ERR_INVALID_ARG;
-    const Level& L = h->L[level];
     if (dtype == SBP_F64)
-        return launch_pass<double>(h->p, L, InPlain<double>{(const double*)u}, OutStore<double>{(double*)Au, (double)sign},
-                                   nullptr, nullptr, h->st, nullptr);
+        return launch_pass<double>(h, level, InPlain<double>{(const double*)u},
+                                   OutStore<double>{(double*)Au, (double)sign}, nullptr, nullptr, nullptr);
     if (dtype == SBP_F32)
-        return launch_pass<float>(h->p, L, InPlain<float>{(const float*)u}, OutStore<float>{(float*)Au, (float)sign},
-                                  nullptr, nullptr, h->st, nullptr);
+        return launch_pass<float>(h, level, InPlain<float>{(const float*)u}, OutStore<float>{(float*)Au, (float)sign},
+                                  nullptr, nullptr, nullptr);
     return SBP_ERR_INVALID_ARG;
 }
 
 int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot) {
     if (!h || !u || !Au || !dot) return SBP_ERR_INVALID_ARG;
     int np = 0;
-    STCHK(launch_pass<double>(h->p, h->L[0], InPlain<double>{u}, OutStoreDot<double>{Au}, nullptr, h->partial, h->st, &np));
-    hipLaunchKernelGGL(reduce_to, dim3(1), dim3(kRedThreads), 0, h->st, h->partial, np, h->dtmp, 0);
+    STCHK(launch_pass<double>(h, 0, InPlain<double>{u}, OutStoreDot<double>{Au}, nullptr, h->partial, &np));
+    hipLaunchKernelGGL(reduce_to, dim3(h->nprob), dim3(kRedThreads), 0, h->st, h->partial, np, h->dtmp, 0);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(dot, h->dtmp, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(dot, h->dtmp, sizeof(double) * h->nprob, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return SBP_OK;
 }
@@ -732,7 +991,7 @@ int sbp_hnorm(sbp_handle h, const double* v, double* out) {
 int sbp_dot(sbp_handle h, const double* u, const double* v, double* out) {
     if (!h || !u || !v || !out) return SBP_ERR_INVALID_ARG;
     const int nb = ew_blocks(h->L[0].n);
-    hipLaunchKernelGGL((dot_partial<double, 256>), dim3(nb), dim3(256), 0, h->st, u, v, h->L[0].n, h->partial, nullptr);
+    hipLaunchKernelGGL((dot_partial<double, 256>), dim3(nb, 1), dim3(256), 0, h->st, u, v, h->L[0].n, h->partial);
     hipLaunchKernelGGL(reduce_to, dim3(1), dim3(kRedThreads), 0, h->st, h->partial, nb, h->dtmp, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, h->dtmp, sizeof(double), hipMemcpyDeviceToHost, h->st));
@@ -743,13 +1002,13 @@ int sbp_dot(sbp_handle h, const double* u, const double* v, double* out) {
 int sbp_restrict(sbp_handle h, int level, const void* rf, void* rc, int dtype) {
     if (!h || !rf || !rc || level < 0 || level + 1 >= h->nlev) return SBP_ERR_INVALID_ARG;
     const int Nf = h->L[level].N, Nc = h->L[level + 1].N;
-    dim3 g((Nc + 255) / 256, Nc);
+    const dim3 g((Nc + 255) / 256, Nc, h->nprob);
     if (dtype == SBP_F64)
         hipLaunchKernelGGL((restrict_fw<double>), g, dim3(256), 0, h->st, (const double*)rf, (double*)rc, Nf, Nc,
-                           h->prm.restrict_scale, nullptr);
+                           h->prm.restrict_scale, (const Scalars*)nullptr);
     else if (dtype == SBP_F32)
         hipLaunchKernelGGL((restrict_fw<float>), g, dim3(256), 0, h->st, (const float*)rf, (float*)rc, Nf, Nc,
-                           (float)h->prm.restrict_scale, nullptr);
+                           (float)h->prm.restrict_scale, (const Scalars*)nullptr);
     else return SBP_ERR_INVALID_ARG;
     HIPCHK(hipGetLastError());
     return SBP_OK;
@@ -758,16 +1017,17 @@ int sbp_restrict(sbp_handle h, int level, const void* rf, void* rc, int dtype) {
 int sbp_prolong_add(sbp_handle h, int level, const void* ec, void* xf, int dtype) {
     if (!h || !ec || !xf || level < 0 || level + 1 >= h->nlev) return SBP_ERR_INVALID_ARG;
     const int Nf = h->L[level].N, Nc = h->L[level + 1].N;
-    dim3 g((Nf + 255) / 256, Nf);
+    const dim3 g((Nf + 255) / 256, Nf, h->nprob);
     if (dtype == SBP_F64)
-        hipLaunchKernelGGL((prolong_add<double>), g, dim3(256), 0, h->st, (const double*)ec, (double*)xf, Nf, Nc, nullptr);
+        hipLaunchKernelGGL((prolong_add<double>), g, dim3(256), 0, h->st, (const double*)ec, (double*)xf, Nf, Nc,
+                           (const Scalars*)nullptr);
     else if (dtype == SBP_F32)
-        hipLaunchKernelGGL((prolong_add<float>), g, dim3(256), 0, h->st, (const float*)ec, (float*)xf, Nf, Nc, nullptr);
+        hipLaunchKernelGGL((prolong_add<float>), g, dim3(256), 0, h->st, (const float*)ec, (float*)xf, Nf, Nc,
+                           (const Scalars*)nullptr);
     else return SBP_ERR_INVALID_ARG;
     HIPCHK(hipGetLastError());
     return SBP_OK;
 }
-
 
 int sbp_smooth(sbp_handle h, int level, const void* b, void* x, int nsweeps, int dtype) {
     if (!h || !b || !x || level < 0 || level >= h->nlev || nsweeps < 0) return SBP_ERR_INVALID_ARG;
@@ -781,8 +1041,8 @@ int sbp_smooth(sbp_handle h, int level, const void* b, void* x, int nsweeps, int
 
 int sbp_coarse_solve(sbp_handle h, const double* b, double* x) {
     if (!h || !b || !x || !h->mg) return SBP_ERR_INVALID_ARG;
-    hipLaunchKernelGGL((coarse_solve<double>), dim3((h->nc + kCoarseRows - 1) / kCoarseRows), dim3(256), 0, h->st,
-                       h->Ainv, b, x, h->nc, nullptr);
+    hipLaunchKernelGGL((coarse_solve<double>), dim3((h->nc + kCoarseRows - 1) / kCoarseRows, h->nprob), dim3(256), 0,
+                       h->st, h->Ainv, b, x, h->nc, (const Scalars*)nullptr);
     HIPCHK(hipGetLastError());
     return SBP_OK;
 }
@@ -797,151 +1057,67 @@ int sbp_vcycle(sbp_handle h, const void* r, void* z, int dtype) {
     return SBP_ERR_INVALID_ARG;
 }
 
-
-
 int sbp_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
            double* hist) {
-    if (!h || !b || !x || maxiter < 0 || !(abstol >= 0)) return SBP_ERR_INVALID_ARG;
-    return krylov64(h, b, x, abstol, maxiter, iters, resnorm, hist, false);
+    if (!h || !b || !x || maxiter < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
+    STCHK(krylov64(h, b, x, &abstol, maxiter, hist, false));
+    get_results(h, iters, resnorm, nullptr);
+    return batch_status(h);
 }
 
 int sbp_mgpcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
               double* hist) {
-    if (!h || !b || !x || maxiter < 0 || !(abstol >= 0)) return SBP_ERR_INVALID_ARG;
+    if (!h || !b || !x || maxiter < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
     if (!h->mg) return SBP_ERR_INVALID_SIZE;
-    return krylov64(h, b, x, abstol, maxiter, iters, resnorm, hist, true);
+    STCHK(krylov64(h, b, x, &abstol, maxiter, hist, true));
+    get_results(h, iters, resnorm, nullptr);
+    return batch_status(h);
 }
 
-}  // extern "C"
-
-namespace {
-// Outer refinement check: resnorm = ||r||; converged -> both done; else arm
-// the inner solver with abstol = inner_rtol * ||r|| and its iteration cap.
-__global__ __launch_bounds__(1024) void ir_check(const double* partial, int np, Scalars* so, Scalars* si,
-                                               double inner_rtol, int inner_maxiter) {
-    double rs = sum_partials(partial, np);
-    if (threadIdx.x == 0) {
-        const double nr = sqrt(rs);
-        so->resnorm = nr;
-        so->rr = rs;
-        si->iters = 0;
-        si->status = 0;
-        si->alpha = si->beta = 0.0;
-        si->rr = rs;
-        si->resnorm = nr;
-        si->hist = nullptr;
-        si->maxiter = inner_maxiter;
-        if (so->done) { si->done = 1; return; }
-        if (!isfinite(rs)) { so->status = 5; so->done = 1; si->done = 1; return; }
-        if (nr < so->abstol) { so->done = 1; si->done = 1; return; }
-        si->abstol = inner_rtol * nr;
-        si->done = 0;
-    }
-}
-
-__global__ void ir_init(Scalars* so, double abstol) {
-    so->abstol = abstol;
-    so->done = 0;
-    so->status = 0;
-    so->iters = 0;
-    so->dprev = 0.0;
-    so->est = 0.0;
-}
-
-// x64 += (double) d32 with block partials of ||d||^2 (slot 2b) and ||x||^2 (slot 2b+1).
-template <int NT>
-__global__ __launch_bounds__(NT) void ir_update(double* __restrict__ x, const float* __restrict__ d, size_t n,
-                                               double* partial) {
-    double dd = 0.0, xx = 0.0;
-    const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) {
-        const double dv = (double)d[q];
-        const double xv = x[q] + dv;
-        x[q] = xv;
-        dd += dv * dv;
-        xx += xv * xv;
-    }
-    const double sd = block_sum<double, NT>(dd);
-    __syncthreads();
-    const double sx = block_sum<double, NT>(xx);
-    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = sd; partial[2 * blockIdx.x + 1] = sx; }
-}
-
-// Error estimate after correction k: ||e_k|| ~ ||d_k|| * min(1, ||d_k|| / ||d_{k-1}||)
-// (the observed contraction of the refinement).  Stops the outer loop when
-// est <= ir_rtol * ||x|| (ir_rtol > 0).
-__global__ __launch_bounds__(1024) void ir_estimate(const double* partial, int nb, Scalars* so, double ir_rtol) {
-    __shared__ double red[2][16];
-    double sd = 0.0, sx = 0.0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) { sd += partial[2 * k]; sx += partial[2 * k + 1]; }
-    sd = wave_sum(sd);
-    sx = wave_sum(sx);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) { red[0][w] = sd; red[1][w] = sx; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double td = 0.0, tx = 0.0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { td += red[0][k]; tx += red[1][k]; }
-        const double dn = sqrt(td), xn = sqrt(tx);
-        so->iters += 1;
-        const double ratio = so->dprev > 0.0 ? fmin(1.0, dn / so->dprev) : 1.0;
-        so->est = dn * ratio;
-        so->dprev = dn;
-        if (ir_rtol > 0.0 && so->iters >= 2 && so->est <= ir_rtol * xn) so->done = 1;
-    }
-}
-}  // namespace
-
-extern "C" int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double abstol, int max_outer,
-                               int* outer_iters, int* inner_iters, double* resnorm) {
-    if (!h || !b || !x || max_outer < 0 || !(abstol >= 0)) return SBP_ERR_INVALID_ARG;
+int sbp_mgpcg_batch(sbp_handle h, const double* b, double* x, const double* abstol, int maxiter, int* iters,
+                    double* resnorm, double* seconds, int* status) {
+    if (!h || !b || !x || !abstol || maxiter < 0) return SBP_ERR_INVALID_ARG;
+    for (int q = 0; q < h->nprob; ++q)
+        if (!(abstol[q] >= 0)) return SBP_ERR_INVALID_ARG;
     if (!h->mg) return SBP_ERR_INVALID_SIZE;
-    STCHK(ensure32(h));
-    hipStream_t st = h->st;
-    const size_t n = h->L[0].n;
-    const int ub = std::min<int>(ew_blocks(n), (int)(h->npartial / 2));
-    hipLaunchKernelGGL(ir_init, dim3(1), dim3(1), 0, st, h->sc, abstol);
-    HIPCHK(hipGetLastError());
-    int total_inner = 0;
-    int rc = SBP_OK;
-    for (int o = 0; o <= max_outer; ++o) {
-        int np = 0;
-        // FP64 residual r = b - A x, cast to FP32 for the inner solve.
-        STCHK(launch_pass<double>(h->p, h->L[0], InPlain<double>{x}, OutResid<double, float>{b, h->r32}, &h->sc->done,
-                                  h->partial, st, &np));
-        hipLaunchKernelGGL(ir_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, h->sc, h->sc_in, h->prm.inner_rtol,
-                           h->prm.inner_maxiter);
-        HIPCHK(hipGetLastError());
-        STCHK(poll(h, h->sc));
-        if (h->sc_host->done || o == max_outer) break;
-        // Inner FP32 MG-PCG on A d = r from d = 0.
-        hipLaunchKernelGGL((fill<float, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st, h->d32, 0.0f, n,
-                           &h->sc_in->done);
-        HIPCHK(hipGetLastError());
-        int it = 0;
-        const int irc = pcg_loop<float>(h, kvecs<float>(h), h->sc_in, h->prm.inner_maxiter, true, &it);
-        if (irc == SBP_ERR_NONFINITE || irc == SBP_ERR_BREAKDOWN) { rc = irc; break; }
-        total_inner += it;
-        // FP64 update and the error estimate from the correction norms.
-        hipLaunchKernelGGL((ir_update<kEwThreads>), dim3(ub), dim3(kEwThreads), 0, st, x, (const float*)h->d32, n,
-                           h->partial);
-        hipLaunchKernelGGL(ir_estimate, dim3(1), dim3(kRedThreads), 0, st, h->partial, ub, h->sc, h->prm.ir_rtol);
-        HIPCHK(hipGetLastError());
-    }
-    STCHK(poll(h, h->sc));
-    if (outer_iters) *outer_iters = h->sc_host->iters;
-    if (inner_iters) *inner_iters = total_inner;
-    if (resnorm) *resnorm = h->sc_host->resnorm;
-    if (rc != SBP_OK) return rc;
-    if (h->sc_host->status == 5) return SBP_ERR_NONFINITE;
-    return h->sc_host->done ? SBP_OK : SBP_ERR_NOT_CONVERGED;
+    STCHK(krylov64(h, b, x, abstol, maxiter, nullptr, true));
+    get_results(h, iters, resnorm, seconds);
+    if (status)
+        for (int q = 0; q < h->nprob; ++q) status[q] = status_of(h->sc_host[q]);
+    return batch_status(h);
+}
+
+int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double abstol, int max_outer, int* outer_iters,
+                    int* inner_iters, double* resnorm) {
+    if (!h || !b || !x || max_outer < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
+    if (!h->mg) return SBP_ERR_INVALID_SIZE;
+    int inner = 0;
+    STCHK(mixed_solve(h, b, x, &abstol, max_outer, &inner));
+    get_results(h, outer_iters, resnorm, nullptr);
+    if (inner_iters) *inner_iters = inner;
+    return mixed_status(h);
+}
+
+int sbp_mgpcg_mixed_batch(sbp_handle h, const double* b, double* x, const double* abstol, int max_outer,
+                          int* outer_iters, int* inner_iters, double* resnorm, double* seconds, int* status) {
+    if (!h || !b || !x || !abstol || max_outer < 0) return SBP_ERR_INVALID_ARG;
+    for (int q = 0; q < h->nprob; ++q)
+        if (!(abstol[q] >= 0)) return SBP_ERR_INVALID_ARG;
+    if (!h->mg) return SBP_ERR_INVALID_SIZE;
+    std::vector<int> inner(h->nprob, 0);
+    STCHK(mixed_solve(h, b, x, abstol, max_outer, inner.data()));
+    get_results(h, outer_iters, resnorm, seconds);
+    if (inner_iters)
+        for (int q = 0; q < h->nprob; ++q) inner_iters[q] = inner[q];
+    if (status)
+        for (int q = 0; q < h->nprob; ++q) status[q] = mixed_status_of(h->sc_host[q]);
+    return mixed_status(h);
 }
 
 // Adjoint gradient: L = 1/2 (u - u_obs)' H~ (u - u_obs), u = A^-1 b(f, g).
-extern "C" int sbp_adjoint_grad(sbp_handle h, const double* u, const double* u_obs, double abstol, int maxiter,
-                                double* loss, double* lam, double* df, double* dgW, double* dgE, double* dgS,
-                                double* dgN, int* iters) {
-    if (!h || !u || !u_obs || maxiter < 0 || !(abstol >= 0)) return SBP_ERR_INVALID_ARG;
+int sbp_adjoint_grad(sbp_handle h, const double* u, const double* u_obs, double abstol, int maxiter, double* loss,
+                     double* lam, double* df, double* dgW, double* dgE, double* dgS, double* dgN, int* iters) {
+    if (!h || !u || !u_obs || maxiter < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
     if (!h->mg) return SBP_ERR_INVALID_SIZE;
     const size_t n = h->L[0].n;
     if (!h->w) HIPCHK(hipMalloc(&h->w, n * sizeof(double)));
@@ -954,7 +1130,8 @@ extern "C" int sbp_adjoint_grad(sbp_handle h, const double* u, const double* u_o
     const double hh = 1.0 / (h->N - 1);
     hipLaunchKernelGGL(adjoint_source, dim3(nb), dim3(256), 0, h->st, u, u_obs, h->w, h->L[0].tb, hh * hh, h->partial);
     hipLaunchKernelGGL(reduce_to, dim3(1), dim3(kRedThreads), 0, h->st, h->partial, nb, h->dtmp, 0);
-    hipLaunchKernelGGL((fill<double, kEwThreads>), dim3(nb), dim3(kEwThreads), 0, h->st, lm, 0.0, n, nullptr);
+    hipLaunchKernelGGL((fill<double, kEwThreads>), dim3(nb, 1), dim3(kEwThreads), 0, h->st, lm, 0.0, n,
+                       (const Scalars*)nullptr);
     HIPCHK(hipGetLastError());
     if (loss) {
         HIPCHK(hipMemcpyAsync(loss, h->dtmp, sizeof(double), hipMemcpyDeviceToHost, h->st));
@@ -962,28 +1139,17 @@ extern "C" int sbp_adjoint_grad(sbp_handle h, const double* u, const double* u_o
         *loss *= 0.5;
     }
     // A is symmetric, so the adjoint solve A^T lambda = w is the forward MG-PCG.
-    int it = 0;
-    double rn = 0.0;
-    const int rc = krylov64(h, h->w, lm, abstol, maxiter, &it, &rn, nullptr, true);
-    if (iters) *iters = it;
+    STCHK(krylov64(h, h->w, lm, &abstol, maxiter, nullptr, true));
+    get_results(h, iters, nullptr, nullptr);
+    const int rc = batch_status(h);
     if (rc != SBP_OK) return rc;
     return sbp_rhs_transpose(h, lm, df, dgW, dgE, dgS, dgN);
 }
 
 // ---------------------------------------------------------------------------
-// CSR comparison path
+// CSR comparison path (one problem)
 // ---------------------------------------------------------------------------
-namespace {
-__global__ void exclusive_scan_serial(const int32_t* counts, int32_t* rp, int n, int64_t* total) {
-    // One thread; used once per assembly.
-    int64_t s = 0;
-    for (int q = 0; q < n; ++q) { rp[q] = (int32_t)s; s += counts[q]; }
-    rp[n] = (int32_t)s;
-    *total = s;
-}
-}  // namespace
-
-extern "C" int sbp_csr_assemble(sbp_handle h, int32_t* row_ptr, int32_t* col, double* val, int64_t* nnz) {
+int sbp_csr_assemble(sbp_handle h, int32_t* row_ptr, int32_t* col, double* val, int64_t* nnz) {
     if (!h || !nnz) return SBP_ERR_INVALID_ARG;
     const int n = (int)h->L[0].n;
     int32_t* counts = nullptr;
@@ -1006,67 +1172,60 @@ extern "C" int sbp_csr_assemble(sbp_handle h, int32_t* row_ptr, int32_t* col, do
     return SBP_OK;
 }
 
-extern "C" int sbp_csr_spmv(sbp_handle h, const int32_t* rp, const int32_t* col, const double* val, const double* x,
-                            double* y) {
+int sbp_csr_spmv(sbp_handle h, const int32_t* rp, const int32_t* col, const double* val, const double* x,
+                 double* y) {
     if (!h || !rp || !col || !val || !x || !y) return SBP_ERR_INVALID_ARG;
     const int n = (int)h->L[0].n;
-    hipLaunchKernelGGL(csr_spmv, dim3((n + 255) / 256), dim3(256), 0, h->st, rp, col, val, x, y, n, nullptr, nullptr);
+    hipLaunchKernelGGL(csr_spmv, dim3((n + 255) / 256), dim3(256), 0, h->st, rp, col, val, x, y, n, nullptr,
+                       (const Scalars*)nullptr);
     HIPCHK(hipGetLastError());
     return SBP_OK;
 }
 
-extern "C" int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, const double* val, const double* b,
-                          double* x, double abstol, int maxiter, int* iters, double* resnorm) {
-    if (!h || !rp || !col || !val || !b || !x || maxiter < 0) return SBP_ERR_INVALID_ARG;
+int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, const double* val, const double* b, double* x,
+               double abstol, int maxiter, int* iters, double* resnorm) {
+    if (!h || !rp || !col || !val || !b || !x || maxiter < 0 || !(abstol >= 0) || h->nprob != 1)
+        return SBP_ERR_INVALID_ARG;
     hipStream_t st = h->st;
     const int n = (int)h->L[0].n;
     const int nbs = (n + 255) / 256;
-    if ((size_t)nbs > h->npartial) return SBP_ERR_INVALID_SIZE;
+    if ((size_t)nbs > h->npp) return SBP_ERR_INVALID_SIZE;
     Scalars* sc = h->sc;
+    STCHK(put_abstol(h, &abstol));
+    hipLaunchKernelGGL(stamp_start, dim3(1), dim3(256), 0, st, sc, 1);
     // r = b - A x with block partials of r'r
     hipLaunchKernelGGL(csr_resid, dim3(nbs), dim3(256), 0, st, rp, col, val, x, b, h->r, n, h->partial);
-    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(kRedThreads), 0, st, h->partial, nbs, sc, abstol, nullptr, maxiter);
-    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n)), dim3(kEwThreads), 0, st,
-                       (const double*)h->r, h->pv, (size_t)n, &sc->done);
+    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(kRedThreads), 0, st, h->partial, nbs, sc, (const double*)h->dabs,
+                       (double*)nullptr, maxiter);
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n), 1), dim3(kEwThreads), 0, st,
+                       (const double*)h->r, h->pv, (size_t)n, (const Scalars*)sc);
     HIPCHK(hipGetLastError());
     const int poll_every = std::max(1, h->prm.poll_every);
+    const dim3 g(ew_blocks(n), 1);
     for (int it = 0; it < maxiter; ++it) {
         hipLaunchKernelGGL(csr_spmv, dim3(nbs), dim3(256), 0, st, rp, col, val, (const double*)h->pv, h->Ap, n,
-                           h->partial, &sc->done);
+                           h->partial, (const Scalars*)sc);
         hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, nbs, sc, 0);
-        const int eb = ew_blocks(n);
-        hipLaunchKernelGGL((update_xr<double, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, x, h->r,
-                           (const double*)h->pv, (const double*)h->Ap, (const Scalars*)sc, (size_t)n, h->partial);
-        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, eb, sc, 1);
-        hipLaunchKernelGGL((update_p<double, kEwThreads>), dim3(eb), dim3(kEwThreads), 0, st, h->pv,
-                           (const double*)h->r, (const Scalars*)sc, (size_t)n);
+        hipLaunchKernelGGL((update_xr<double, kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, (const double*)h->pv,
+                           (const double*)h->Ap, (const Scalars*)sc, (size_t)n, h->partial);
+        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 1);
+        hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)h->r,
+                           (const Scalars*)sc, (size_t)n);
         HIPCHK(hipGetLastError());
         if ((it + 1) % poll_every == 0) {
-            STCHK(poll(h, sc));
-            if (h->sc_host->done) break;
+            bool all = false;
+            STCHK(poll(h, sc, &all));
+            if (all) break;
         }
     }
     STCHK(poll(h, sc));
-    if (iters) *iters = h->sc_host->iters;
-    if (resnorm) *resnorm = h->sc_host->resnorm;
-    return finish_status(*h->sc_host, !h->sc_host->done);
+    get_results(h, iters, resnorm, nullptr);
+    return batch_status(h);
 }
-
-namespace {
-__global__ __launch_bounds__(256) void copy_vec16(const uint4* __restrict__ s, uint4* __restrict__ d, size_t n16) {
-    const size_t stride = (size_t)gridDim.x * 256;
-    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n16; q += stride) d[q] = s[q];
-}
-__global__ __launch_bounds__(256) void copy_u8(const unsigned char* __restrict__ s, unsigned char* __restrict__ d,
-                                              size_t n) {
-    const size_t stride = (size_t)gridDim.x * 256;
-    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += stride) d[q] = s[q];
-}
-}  // namespace
 
 // Streaming copy: 16-byte vector body when both ends are 16-byte aligned,
 // bytes for the tail (or everything when unaligned).
-extern "C" int sbp_copy(sbp_handle h, const void* src, void* dst, size_t bytes) {
+int sbp_copy(sbp_handle h, const void* src, void* dst, size_t bytes) {
     if (!h || !src || !dst) return SBP_ERR_INVALID_ARG;
     if (bytes == 0) return SBP_OK;
     const bool aligned = (((uintptr_t)src | (uintptr_t)dst) & 15) == 0;
@@ -1085,3 +1244,5 @@ extern "C" int sbp_copy(sbp_handle h, const void* src, void* dst, size_t bytes) 
     }
     return SBP_OK;
 }
+
+}  // extern "C"

@@ -7,14 +7,43 @@
 namespace sbp {
 
 // ---------------------------------------------------------------------------
-// Transfers
+// Device-resident Krylov state, one block per stacked problem
+// ---------------------------------------------------------------------------
+struct Scalars {
+    double rr;             // r'r (CG) or latest ||r||^2 (PCG)
+    double rz;             // r'z (PCG)
+    double pAp;
+    double alpha, beta;
+    double abstol;
+    double resnorm;        // latest ||r||_2
+    double bnorm;
+    double* hist;          // nullable residual-norm history, hist[k] = ||r_k||_2
+    double dprev;          // refinement: norm of the previous correction
+    double est;            // refinement: a-posteriori error estimate
+    unsigned long long t0; // wall clock at the solve's start
+    unsigned long long t1; // wall clock when this problem converged
+    int maxiter;           // iteration cap (tested on the device)
+    int iters;             // iterations completed (advanced on the device)
+    int done;              // convergence mask: kernels of a done problem fall through
+    int status;
+};
+constexpr int kScalarInts = (int)(sizeof(Scalars) / sizeof(int));
+constexpr int kScalarDoubles = (int)(sizeof(Scalars) / sizeof(double));
+
+__device__ __forceinline__ bool skip(const Scalars* sc, int q) { return sc && sc[q].done; }
+
+// ---------------------------------------------------------------------------
+// Transfers (blockIdx.z = problem)
 // ---------------------------------------------------------------------------
 // rc(I,J) = scale * sum_{a,b in -1..1} w_a w_b rf(2I+a, 2J+b) / 4, w_0 = 1, w_+-1 = 1/2,
 // fine points outside the grid count as zero.
 template <typename T>
 __global__ __launch_bounds__(256) void restrict_fw(const T* __restrict__ rf, T* __restrict__ rc, int Nf, int Nc,
-                                                  T scale, const int* done) {
-    if (done && *done) return;
+                                                  T scale, const Scalars* sc) {
+    const int q = blockIdx.z;
+    if (skip(sc, q)) return;
+    rf += (size_t)q * Nf * Nf;
+    rc += (size_t)q * Nc * Nc;
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     const int Jc = blockIdx.y;
     if (I >= Nc) return;
@@ -37,8 +66,11 @@ __global__ __launch_bounds__(256) void restrict_fw(const T* __restrict__ rf, T* 
 // xf += P ec
 template <typename T>
 __global__ __launch_bounds__(256) void prolong_add(const T* __restrict__ ec, T* __restrict__ xf, int Nf, int Nc,
-                                                  const int* done) {
-    if (done && *done) return;
+                                                  const Scalars* sc) {
+    const int q = blockIdx.z;
+    if (skip(sc, q)) return;
+    ec += (size_t)q * Nc * Nc;
+    xf += (size_t)q * Nf * Nf;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= Nf) return;
@@ -51,14 +83,17 @@ __global__ __launch_bounds__(256) void prolong_add(const T* __restrict__ ec, T* 
 }
 
 // Coarsest-level exact solve x = Ainv b with the FP64 inverse (symmetric,
-// stored n x n).  Blocks of 4 waves own kCoarseRows consecutive rows; each
-// wave forms its rows' dot products across its 64 lanes (coalesced row
-// reads, all loads of a row issued together), then reduces across lanes.
+// stored n x n); blockIdx.y = problem.  Blocks of 4 waves own kCoarseRows
+// consecutive rows; each wave forms its rows' dot products across its 64
+// lanes (coalesced row reads, all loads of a row issued together).
 constexpr int kCoarseRows = 8;
 template <typename T>
 __global__ __launch_bounds__(256) void coarse_solve(const double* __restrict__ Ainv, const T* __restrict__ b,
-                                                   T* __restrict__ x, int n, const int* done) {
-    if (done && *done) return;
+                                                   T* __restrict__ x, int n, const Scalars* sc) {
+    const int q = blockIdx.y;
+    if (skip(sc, q)) return;
+    b += (size_t)q * n;
+    x += (size_t)q * n;
     __shared__ double bs[1089];   // up to 33 x 33
     for (int k = threadIdx.x; k < n; k += blockDim.x) bs[k] = (double)b[k];
     __syncthreads();
@@ -76,29 +111,12 @@ __global__ __launch_bounds__(256) void coarse_solve(const double* __restrict__ A
 }
 
 // ---------------------------------------------------------------------------
-// Elementwise / reduction kernels for the Krylov loops
+// Reductions: one workgroup per problem (blockIdx.x = problem), each summing
+// that problem's np block partials at partial[q * np ...]
 // ---------------------------------------------------------------------------
-struct Scalars {           // device-resident solver state
-    double rr;             // r'r (CG) or latest ||r||^2 (PCG)
-    double rz;             // r'z (PCG)
-    double pAp;
-    double alpha, beta;
-    double abstol;
-    double resnorm;        // latest ||r||_2
-    double bnorm;
-    double* hist;          // nullable residual-norm history, hist[k] = ||r_k||_2
-    double dprev;          // refinement: norm of the previous correction
-    double est;            // refinement: a-posteriori error estimate
-    int maxiter;           // iteration cap (tested on the device)
-    int iters;             // iterations completed (advanced on the device)
-    int done;
-    int status;
-    int pad;
-};
-
-// Deterministic single-workgroup sum of the block partials (fixed order:
-// thread t sums t, t+NT, ... with four independent accumulators, then a
-// tree across lanes and waves).  Launched with kRedThreads threads.
+// Deterministic sum (fixed order: thread t sums t, t+NT, ... with four
+// independent accumulators, then a tree across lanes and waves).  Launched
+// with kRedThreads threads.
 constexpr int kRedThreads = 1024;
 __device__ __forceinline__ double sum_partials(const double* partial, int np) {
     __shared__ double red[kRedThreads / 64];
@@ -123,21 +141,34 @@ __device__ __forceinline__ double sum_partials(const double* partial, int np) {
     return tot;   // valid on thread 0
 }
 
-// Stage 2 of a plain dot product: out = sum.
+__device__ __forceinline__ void mark_done(Scalars* s) {
+    s->done = 1;
+    s->t1 = wall_clock64();
+}
+
+// out[q] = sum (or sqrt of it) of problem q's partials.
 __global__ __launch_bounds__(1024) void reduce_to(const double* partial, int np, double* out, int take_sqrt) {
-    double s = sum_partials(partial, np);
-    if (threadIdx.x == 0) *out = take_sqrt ? sqrt(s) : s;
+    const int q = blockIdx.x;
+    double s = sum_partials(partial + (size_t)q * np, np);
+    if (threadIdx.x == 0) out[q] = take_sqrt ? sqrt(s) : s;
+}
+
+// Wall-clock stamp of the solve's start for every problem.
+__global__ void stamp_start(Scalars* sc, int nprob) {
+    const unsigned long long t = wall_clock64();
+    for (int q = threadIdx.x; q < nprob; q += blockDim.x) { sc[q].t0 = t; sc[q].t1 = 0; }
 }
 
 // alpha = num / pAp where num = rr (CG, use_rz = 0) or rz (PCG, use_rz = 1).
-__global__ __launch_bounds__(1024) void krylov_alpha(const double* partial, int np, Scalars* sc, int use_rz) {
+__global__ __launch_bounds__(1024) void krylov_alpha(const double* partial, int np, Scalars* scb, int use_rz) {
+    Scalars* sc = scb + blockIdx.x;
     if (sc->done) return;
-    double pAp = sum_partials(partial, np);
+    double pAp = sum_partials(partial + (size_t)blockIdx.x * np, np);
     if (threadIdx.x == 0) {
         sc->pAp = pAp;
         if (!(pAp > 0.0) || !isfinite(pAp)) {
             sc->status = isfinite(pAp) ? 6 : 5;
-            sc->done = 1;
+            mark_done(sc);
             return;
         }
         sc->alpha = (use_rz ? sc->rz : sc->rr) / pAp;
@@ -147,40 +178,43 @@ __global__ __launch_bounds__(1024) void krylov_alpha(const double* partial, int 
 // rs = ||r||^2; iteration counter and history advance on the device (so a
 // captured iteration replays unchanged); converge test ||r|| < abstol; for CG
 // also beta.
-__global__ __launch_bounds__(1024) void krylov_check(const double* partial, int np, Scalars* sc, int cg_beta) {
+__global__ __launch_bounds__(1024) void krylov_check(const double* partial, int np, Scalars* scb, int cg_beta) {
+    Scalars* sc = scb + blockIdx.x;
     if (sc->done) return;
-    double rs = sum_partials(partial, np);
+    double rs = sum_partials(partial + (size_t)blockIdx.x * np, np);
     if (threadIdx.x == 0) {
         const double nr = sqrt(rs);
         const int k = sc->iters + 1;
         sc->iters = k;
         sc->resnorm = nr;
         if (sc->hist) sc->hist[k] = nr;
-        if (!isfinite(rs)) { sc->status = 5; sc->done = 1; return; }
-        if (nr < sc->abstol) { sc->done = 1; return; }
-        if (k >= sc->maxiter) { sc->status = 7; sc->done = 1; return; }
+        if (!isfinite(rs)) { sc->status = 5; mark_done(sc); return; }
+        if (nr < sc->abstol) { mark_done(sc); return; }
+        if (k >= sc->maxiter) { sc->status = 7; mark_done(sc); return; }
         if (cg_beta) { sc->beta = rs / sc->rr; }
         sc->rr = rs;
     }
 }
 
 // rz_new = r'z; beta = rz_new / rz.
-__global__ __launch_bounds__(1024) void krylov_beta(const double* partial, int np, Scalars* sc) {
+__global__ __launch_bounds__(1024) void krylov_beta(const double* partial, int np, Scalars* scb) {
+    Scalars* sc = scb + blockIdx.x;
     if (sc->done) return;
-    double rz = sum_partials(partial, np);
+    double rz = sum_partials(partial + (size_t)blockIdx.x * np, np);
     if (threadIdx.x == 0) {
-        if (!isfinite(rz)) { sc->status = 5; sc->done = 1; return; }
+        if (!isfinite(rz)) { sc->status = 5; mark_done(sc); return; }
         sc->beta = rz / sc->rz;
         sc->rz = rz;
     }
 }
 
-// Initial state: rr = sum partial; test ||r0|| < abstol (or maxiter = 0).
-__global__ __launch_bounds__(1024) void krylov_init(const double* partial, int np, Scalars* sc, double abstol,
-                                                  double* hist, int maxiter) {
-    double rs = sum_partials(partial, np);
+// Initial state: rr = sum partial; test ||r0|| < abstol[q] (or maxiter = 0).
+__global__ __launch_bounds__(1024) void krylov_init(const double* partial, int np, Scalars* scb,
+                                                   const double* abstol, double* hist, int maxiter) {
+    Scalars* sc = scb + blockIdx.x;
+    double rs = sum_partials(partial + (size_t)blockIdx.x * np, np);
     if (threadIdx.x == 0) {
-        sc->abstol = abstol;
+        sc->abstol = abstol[blockIdx.x];
         sc->maxiter = maxiter;
         sc->rr = rs;
         sc->resnorm = sqrt(rs);
@@ -190,56 +224,85 @@ __global__ __launch_bounds__(1024) void krylov_init(const double* partial, int n
         sc->done = 0;
         sc->alpha = sc->beta = 0.0;
         if (hist) hist[0] = sqrt(rs);
-        if (!isfinite(rs)) { sc->status = 5; sc->done = 1; return; }
-        if (sqrt(rs) < abstol) { sc->done = 1; return; }
-        if (maxiter <= 0) { sc->status = 7; sc->done = 1; }
+        if (!isfinite(rs)) { sc->status = 5; mark_done(sc); return; }
+        if (sqrt(rs) < sc->abstol) { mark_done(sc); return; }
+        if (maxiter <= 0) { sc->status = 7; mark_done(sc); }
     }
 }
 
 // rz0 = r'z (stage 2) without touching done.
-__global__ __launch_bounds__(1024) void krylov_set_rz(const double* partial, int np, Scalars* sc) {
+__global__ __launch_bounds__(1024) void krylov_set_rz(const double* partial, int np, Scalars* scb) {
+    Scalars* sc = scb + blockIdx.x;
     if (sc->done) return;
-    double rz = sum_partials(partial, np);
+    double rz = sum_partials(partial + (size_t)blockIdx.x * np, np);
     if (threadIdx.x == 0) {
-        if (!isfinite(rz)) { sc->status = 5; sc->done = 1; return; }
+        if (!isfinite(rz)) { sc->status = 5; mark_done(sc); return; }
         sc->rz = rz;
     }
 }
 
-// x += alpha p ; r -= alpha Ap ; partial = block sums of r'r.
+// ---------------------------------------------------------------------------
+// Elementwise kernels: grid (blocks per problem, problems); problem q owns
+// elements [q n, (q+1) n) and partial slots [q gridDim.x, (q+1) gridDim.x).
+// ---------------------------------------------------------------------------
+// r -= alpha Ap ; block partials of r'r.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void update_xr(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
-                                               const T* __restrict__ Ap, const Scalars* sc, size_t n,
-                                               double* partial) {
+__global__ __launch_bounds__(NT) void update_r(T* __restrict__ r, const T* __restrict__ Ap, const Scalars* scb,
+                                              size_t n, double* partial) {
+    const int q = blockIdx.y;
+    const Scalars* sc = scb + q;
     if (sc->done) return;
+    r += q * n;
+    Ap += q * n;
     const T a = (T)sc->alpha;
     T acc = T(0);
     const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) {
-        x[q] += a * p[q];
-        T rv = r[q] - a * Ap[q];
-        r[q] = rv;
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) {
+        const T rv = r[k] - a * Ap[k];
+        r[k] = rv;
         acc += rv * rv;
     }
     T s = block_sum<T, NT>(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = (double)s;
+    if (threadIdx.x == 0) partial[(size_t)q * gridDim.x + blockIdx.x] = (double)s;
 }
 
-// r -= alpha Ap ; partial = block sums of r'r.
+// Unfused CG updates (CSR comparison path): x += alpha p ; r -= alpha Ap with
+// block partials of r'r, then p = r + beta p.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void update_r(T* __restrict__ r, const T* __restrict__ Ap, const Scalars* sc, size_t n,
-                                              double* partial) {
+__global__ __launch_bounds__(NT) void update_xr(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                                               const T* __restrict__ Ap, const Scalars* scb, size_t n,
+                                               double* partial) {
+    const int q = blockIdx.y;
+    const Scalars* sc = scb + q;
     if (sc->done) return;
+    x += q * n;
+    r += q * n;
+    p += q * n;
+    Ap += q * n;
     const T a = (T)sc->alpha;
     T acc = T(0);
     const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) {
-        const T rv = r[q] - a * Ap[q];
-        r[q] = rv;
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) {
+        x[k] += a * p[k];
+        const T rv = r[k] - a * Ap[k];
+        r[k] = rv;
         acc += rv * rv;
     }
     T s = block_sum<T, NT>(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = (double)s;
+    if (threadIdx.x == 0) partial[(size_t)q * gridDim.x + blockIdx.x] = (double)s;
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void update_p(T* __restrict__ p, const T* __restrict__ r, const Scalars* scb,
+                                              size_t n) {
+    const int q = blockIdx.y;
+    const Scalars* sc = scb + q;
+    if (sc->done) return;
+    p += q * n;
+    r += q * n;
+    const T b = (T)sc->beta;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) p[k] = r[k] + b * p[k];
 }
 
 // The update x += alpha_K p_K still pending after the last iteration K (the
@@ -248,37 +311,33 @@ __global__ __launch_bounds__(NT) void update_r(T* __restrict__ r, const T* __res
 // iteration ran.
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void flush_x(T* __restrict__ x, const T* __restrict__ p0, const T* __restrict__ p1,
-                                             const Scalars* sc, size_t n) {
+                                             const Scalars* scb, size_t n) {
+    const int q = blockIdx.y;
+    const Scalars* sc = scb + q;
     const int K = sc->iters;
     if (K < 1 || sc->status == 5 || sc->status == 6) return;
-    const T* pk = (K & 1) ? p1 : p0;
+    const T* pk = ((K & 1) ? p1 : p0) + q * n;
+    x += q * n;
     const T a = (T)sc->alpha;
     const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) x[q] += a * pk[q];
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) x[k] += a * pk[k];
 }
 
-// p = z + beta p
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void update_p(T* __restrict__ p, const T* __restrict__ z, const Scalars* sc,
-                                              size_t n) {
-    if (sc->done) return;
-    const T bt = (T)sc->beta;
-    const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) p[q] = z[q] + bt * p[q];
-}
-
-// partial = block sums of u'v (v may equal u).
+// block partials of u'v (v may equal u).
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void dot_partial(const T* __restrict__ u, const T* __restrict__ v, size_t n,
-                                                 double* partial, const double* __restrict__ wt) {
+                                                 double* partial) {
+    const int q = blockIdx.y;
+    u += q * n;
+    v += q * n;
     T acc = T(0);
     const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) acc += u[q] * v[q];
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) acc += u[k] * v[k];
     T s = block_sum<T, NT>(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = (double)s;
+    if (threadIdx.x == 0) partial[(size_t)q * gridDim.x + blockIdx.x] = (double)s;
 }
 
-// partial = block sums of v' H~ v with H~(i,j) = h^2 hw[i] hw[j].
+// block partials of v' H~ v with H~(i,j) = h^2 hw[i] hw[j] (single problem).
 template <int NT>
 __global__ __launch_bounds__(NT) void hnorm_partial(const double* __restrict__ v, OpTab t, double h2,
                                                    double* partial) {
@@ -293,34 +352,24 @@ __global__ __launch_bounds__(NT) void hnorm_partial(const double* __restrict__ v
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// Type conversion y = (TO) x.
-template <typename TI_, typename TO, int NT>
-__global__ __launch_bounds__(NT) void convert(const TI_* __restrict__ x, TO* __restrict__ y, size_t n) {
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void fill(T* __restrict__ x, T v, size_t n, const Scalars* sc) {
+    const int q = blockIdx.y;
+    if (skip(sc, q)) return;
+    x += q * n;
     const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) y[q] = (TO)x[q];
-}
-
-// x64 += (double) d32
-template <int NT>
-__global__ __launch_bounds__(NT) void add_f32_to_f64(double* __restrict__ x, const float* __restrict__ d, size_t n,
-                                                    const int* done) {
-    if (done && *done) return;
-    const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) x[q] += (double)d[q];
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) x[k] = v;
 }
 
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void fill(T* __restrict__ x, T v, size_t n, const int* done) {
-    if (done && *done) return;
+__global__ __launch_bounds__(NT) void copy_vec(const T* __restrict__ x, T* __restrict__ y, size_t n,
+                                              const Scalars* sc) {
+    const int q = blockIdx.y;
+    if (skip(sc, q)) return;
+    x += q * n;
+    y += q * n;
     const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) x[q] = v;
-}
-
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void copy_vec(const T* __restrict__ x, T* __restrict__ y, size_t n, const int* done) {
-    if (done && *done) return;
-    const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) y[q] = x[q];
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) y[k] = x[k];
 }
 
 // diag(A)
@@ -426,8 +475,8 @@ __global__ __launch_bounds__(256) void rhs_t_g(const double* lam, double* dgW, d
 // One row per thread over rows of the assembled operator (nnz/row <= 11).
 __global__ __launch_bounds__(256) void csr_spmv(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                const double* __restrict__ val, const double* __restrict__ x,
-                                               double* __restrict__ y, int nrows, double* partial, const int* done) {
-    if (done && *done) return;
+                                               double* __restrict__ y, int nrows, double* partial, const Scalars* sc) {
+    if (sc && sc->done) return;
     const int r = blockIdx.x * 256 + threadIdx.x;
     double acc = 0.0;
     if (r < nrows) {

@@ -44,14 +44,15 @@ inline int nparts_of(int N, int ti, int tj) { return ((N + ti - 1) / ti) * ((N +
 
 inline int rr_blocks(int Nc, int ti, int tj) { return ((Nc + ti / 2 - 2) / (ti / 2 - 1)) * ((Nc + tj / 2 - 1) / (tj / 2)); }
 
-// rc = scale * FW(b - A u): residual and restriction in one pass (tf = fine level table).
+// rc = scale * FW(b - A u): residual and restriction in one pass (tf = fine
+// level table), for bt.nprob stacked problems (done: per-problem flags, stride ds ints).
 template <int TI, int J, typename T>
 int launch_resid_restrict(int p, const OpTab& tf, const T* u, const T* b, T* rc, int Nc, T scale, const int* done,
-                          hipStream_t st);
+                          int ds, const Batch& bt, hipStream_t st);
 
-// Launches stencil_pass<p, TI, J, T, IN, OUT> over the whole N x N grid of tb.
+// Launches stencil_pass<p, TI, J, T, IN, OUT> over the whole N x N grid of tb
+// for every stacked problem (c.done/ds/partial/bt given; the tile grid is set here).
 template <int TI, int J, typename T, class IN, class OUT>
-int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, const int* done, double* partial,
-                hipStream_t st);
+int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, PassCtl c, hipStream_t st);
 
 }  // namespace sbp

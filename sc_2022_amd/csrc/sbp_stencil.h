@@ -60,6 +60,17 @@ struct OpTab {
 // large as the smallest eigenvalue of A at N ~ 4k, while differences of
 // neighbouring values of a smooth field are formed almost exactly.
 
+// Problems stacked in one launch (a batched handle): problem q's grid
+// functions start q * sf elements (this level) or q * sc elements (the next
+// coarser level) after problem 0's, and its device scalar block q * ss doubles
+// after problem 0's.  A single problem is nprob = 1.
+struct Batch {
+    int nprob;
+    long long sf;
+    long long sc;
+    int ss;
+};
+
 // Interior 1-D stencil s[-H..H] of -d.
 template <int P> struct Stencil;
 template <> struct Stencil<2> {
@@ -108,6 +119,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 // round trips).  load() = make(fetch()).
 template <typename T> struct InPlain {
     const T* u;
+    __device__ void at(int q, const Batch& bt) { u += q * bt.sf; }
     static constexpr bool kCol = false;
     using Raw = T;
     __device__ void init() {}
@@ -123,6 +135,7 @@ template <typename T> struct InPlain {
 // x1 = w b (Richardson, use_diag = false).
 template <typename T> struct InSmooth0 {
     const T* b; T omega; int use_diag;
+    __device__ void at(int q, const Batch& bt) { b += q * bt.sf; }
     static constexpr bool kCol = false;
     using Raw = T;
     __device__ void init() {}
@@ -143,6 +156,7 @@ template <typename T> struct InSmooth0 {
 // x + P e_c with P the bilinear prolongation from the coarse side Nc = (N+1)/2.
 template <typename T> struct InProlong {
     const T* x; const T* ec; int Nc;
+    __device__ void at(int q, const Batch& bt) { x += q * bt.sf; ec += q * bt.sc; }
     static constexpr bool kCol = true;
     __device__ void init() {}
     // Column window w[k] = x(i, jbeg+k) + (P ec)(i, jbeg+k), k < W, with jbeg of
@@ -195,6 +209,7 @@ template <typename T> struct InProlong {
 // r + beta * p with beta read from the device scalar block.
 template <typename T> struct InAxpy {
     const T* r; const T* p; const double* beta; T bt;
+    __device__ void at(int q, const Batch& bt) { r += q * bt.sf; p += q * bt.sf; beta += q * bt.ss; }
     static constexpr bool kCol = false;
     __device__ void init() { bt = (T)(*beta); }
     struct Raw { T r, p; };
@@ -221,6 +236,7 @@ template <typename T> struct InAxpy {
 // acc_t: accumulator type of fused inner products (the element type).
 template <typename T> struct OutStore {            // o = s * Au
     T* o; T s;
+    __device__ void at(int q, const Batch& bt) { o += q * bt.sf; }
     static constexpr bool kDot = false;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
@@ -230,6 +246,7 @@ template <typename T> struct OutStore {            // o = s * Au
 
 template <typename T> struct OutStoreDot {         // o = Au ; acc += u Au
     T* o;
+    __device__ void at(int q, const Batch& bt) { o += q * bt.sf; }
     static constexpr bool kDot = true;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
@@ -241,6 +258,7 @@ template <typename T> struct OutStoreDot {         // o = Au ; acc += u Au
 
 template <typename T> struct OutWriteDot {         // w = u (materialise the formed operand); o = Au ; acc += u Au
     T* o; T* w;
+    __device__ void at(int q, const Batch& bt) { o += q * bt.sf; w += q * bt.sf; }
     static constexpr bool kDot = true;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
@@ -252,6 +270,7 @@ template <typename T> struct OutWriteDot {         // w = u (materialise the for
 
 template <typename T, typename TO> struct OutResid {  // r = b - Au (optionally cast), acc += r^2
     const T* b; TO* r;
+    __device__ void at(int q, const Batch& bt) { b += q * bt.sf; r += q * bt.sf; }
     static constexpr bool kDot = true;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t q) const { return b[q]; }
@@ -265,6 +284,7 @@ template <typename T, typename TO> struct OutResid {  // r = b - Au (optionally 
 // jacobi_brittany!) or u + w (b - Au) (Richardson, use_diag = 0).
 template <typename T> struct OutSmooth {
     const T* b; T* xo; T omega; int use_diag;
+    __device__ void at(int q, const Batch& bt) { b += q * bt.sf; xo += q * bt.sf; }
     static constexpr bool kDot = false;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t q) const { return b[q]; }
@@ -279,6 +299,7 @@ template <typename T> struct OutSmooth {
 // The same sweep, also accumulating dotv' x_new (r'z of the last pass of a V-cycle).
 template <typename T> struct OutSmoothDot {
     const T* b; T* xo; T omega; const T* dotv; int use_diag;
+    __device__ void at(int q, const Batch& bt) { b += q * bt.sf; xo += q * bt.sf; dotv += q * bt.sf; }
     static constexpr bool kDot = true;
     struct Aux { T b, d; };
     __device__ Aux pre(const OpTab&, size_t q) const { Aux a; a.b = b[q]; a.d = dotv[q]; return a; }
@@ -296,6 +317,7 @@ template <typename T> struct OutSmoothDot {
 // the previous iteration, and accumulates p_new' A p_new.
 template <typename T> struct OutPUpd {
     T* pnew; T* Ap; T* x; const T* pold; const double* alpha; T al;
+    __device__ void at(int q, const Batch& bt) { pnew += q * bt.sf; Ap += q * bt.sf; x += q * bt.sf; pold += q * bt.sf; alpha += q * bt.ss; }
     static constexpr bool kDot = true;
     struct Aux { T x, p; };
     __device__ void init() { al = (T)(*alpha); }
@@ -309,6 +331,7 @@ template <typename T> struct OutPUpd {
 // o = u (materialises a formed operand).
 template <typename T> struct OutCopy {
     T* o;
+    __device__ void at(int q, const Batch& bt) { o += q * bt.sf; }
     static constexpr bool kDot = false;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
@@ -378,9 +401,11 @@ __device__ __forceinline__ T m_closure(const OpTab& t, const IN& in, int i, int 
 }
 
 struct PassCtl {
-    const int* done;     // nullable: skip the pass when *done != 0
-    double* partial;     // nullable: per-block inner-product partials
-    int nbx, nby;        // tile grid
+    const int* done;     // nullable: problem q skips the pass when done[q * ds] != 0
+    int ds;              // ints between problems' done flags
+    double* partial;     // nullable: problem q's block partials at partial[q * nbx * nby + tile]
+    int nbx, nby;        // tile grid of one problem
+    Batch bt;
 };
 
 // Copies the device table into LDS (blockDim.x threads, one word each).
@@ -553,12 +578,15 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
 template <int P, int TI, int J, typename T, class IN, class OUT>
 __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, PassCtl ctl) {
     constexpr int H = Stencil<P>::H;
-    if (ctl.done && *ctl.done) return;
-    const int nb = ctl.nbx * ctl.nby;
-    const int b = xcd_remap((int)blockIdx.x, nb);
+    const int nbt = ctl.nbx * ctl.nby;
+    const int ball = xcd_remap((int)blockIdx.x, nbt * ctl.bt.nprob);
+    const int q = ball / nbt, b = ball - q * nbt;
+    if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
     const int bx = b % ctl.nbx, by = b / ctl.nbx;
     const int N = t.N;
     const int i0 = bx * TI, j0 = by * J;
+    in.at(q, ctl.bt);
+    out.at(q, ctl.bt);
     in.init();
     if constexpr (has_init<OUT>::value) out.init();
     __shared__ T tile[J][TI + 2 * H];
@@ -583,7 +611,7 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
         if (ctl.partial) {
             __syncthreads();
             T s = block_sum<T, TI>(acc);
-            if (threadIdx.x == 0) ctl.partial[b] = (double)s;
+            if (threadIdx.x == 0) ctl.partial[(size_t)q * nbt + b] = (double)s;
         }
     }
 }
@@ -618,9 +646,11 @@ __device__ __forceinline__ T au_point(const OpTab& t, const IN& in, int i, int j
 }
 
 struct RestrictCtl {
-    const int* done;
+    const int* done;   // nullable, per problem (stride ds ints)
+    int ds;
     int Nc;
-    int nbx, nby;    // block grid over coarse points: TI/2-1 columns x J/2 rows per block
+    int nbx, nby;      // block grid over coarse points: TI/2-1 columns x J/2 rows per block
+    Batch bt;          // sf: fine elements, sc: coarse elements per problem
 };
 
 // rc = scale * FW(b - A u) on the coarse grid of side Nc = (N+1)/2.  A block
@@ -636,9 +666,13 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
     constexpr int W = JR + 2 * H;        // column window
     constexpr int CI = TI / 2 - 1;       // coarse columns per block
     constexpr int CJ = J / 2;            // coarse rows per block
-    if (ctl.done && *ctl.done) return;
     const int nb = ctl.nbx * ctl.nby;
-    const int blk = xcd_remap((int)blockIdx.x, nb);
+    const int ball = xcd_remap((int)blockIdx.x, nb * ctl.bt.nprob);
+    const int q = ball / nb, blk = ball - q * nb;
+    if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
+    u += q * ctl.bt.sf;
+    b += q * ctl.bt.sf;
+    rc += q * ctl.bt.sc;
     const int bx = blk % ctl.nbx, by = blk / ctl.nbx;
     const int N = t.N, Nc = ctl.Nc;
     const int Ic0 = bx * CI, Jc0 = by * CJ;

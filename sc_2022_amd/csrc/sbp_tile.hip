@@ -9,43 +9,42 @@
 namespace sbp {
 
 template <int TI, int J, typename T, class IN, class OUT>
-int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, const int* done, double* partial,
-                hipStream_t st) {
-    PassCtl c;
-    c.done = done;
-    c.partial = partial;
+int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, PassCtl c, hipStream_t st) {
     c.nbx = (tb.N + TI - 1) / TI;
     c.nby = (tb.N + J - 1) / J;
+    const dim3 g(c.nbx * c.nby * c.bt.nprob);
     if (p == 2)
-        hipLaunchKernelGGL((stencil_pass<2, TI, J, T, IN, OUT>), dim3(c.nbx * c.nby), dim3(TI), 0, st, in, out, tb, c);
+        hipLaunchKernelGGL((stencil_pass<2, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c);
     else
-        hipLaunchKernelGGL((stencil_pass<4, TI, J, T, IN, OUT>), dim3(c.nbx * c.nby), dim3(TI), 0, st, in, out, tb, c);
+        hipLaunchKernelGGL((stencil_pass<4, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c);
     return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
 template <int TI, int J, typename T>
 int launch_resid_restrict(int p, const OpTab& tf, const T* u, const T* b, T* rc, int Nc, T scale, const int* done,
-                          hipStream_t st) {
+                          int ds, const Batch& bt, hipStream_t st) {
     RestrictCtl c;
     c.done = done;
+    c.ds = ds;
     c.Nc = Nc;
     c.nbx = (Nc + TI / 2 - 2) / (TI / 2 - 1);
     c.nby = (Nc + J / 2 - 1) / (J / 2);
+    c.bt = bt;
+    const dim3 g(c.nbx * c.nby * bt.nprob);
     if (p == 2)
-        hipLaunchKernelGGL((resid_restrict<2, TI, J, T>), dim3(c.nbx * c.nby), dim3(TI), 0, st, u, b, rc, scale, tf, c);
+        hipLaunchKernelGGL((resid_restrict<2, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c);
     else
-        hipLaunchKernelGGL((resid_restrict<4, TI, J, T>), dim3(c.nbx * c.nby), dim3(TI), 0, st, u, b, rc, scale, tf, c);
+        hipLaunchKernelGGL((resid_restrict<4, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c);
     return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
 template int launch_resid_restrict<SBP_TI, SBP_TJ, double>(int, const OpTab&, const double*, const double*, double*,
-                                                           int, double, const int*, hipStream_t);
+                                                           int, double, const int*, int, const Batch&, hipStream_t);
 template int launch_resid_restrict<SBP_TI, SBP_TJ, float>(int, const OpTab&, const float*, const float*, float*, int,
-                                                          float, const int*, hipStream_t);
+                                                          float, const int*, int, const Batch&, hipStream_t);
 
-#define SBP_INST(T, IN, OUT)                                                                                \
-    template int launch_tile<SBP_TI, SBP_TJ, T, IN, OUT>(int, const OpTab&, const IN&, const OUT&, const int*, \
-                                                         double*, hipStream_t);
+#define SBP_INST(T, IN, OUT) \
+    template int launch_tile<SBP_TI, SBP_TJ, T, IN, OUT>(int, const OpTab&, const IN&, const OUT&, PassCtl, hipStream_t);
 SBP_STENCIL_COMBOS(SBP_INST)
 
 }  // namespace sbp
