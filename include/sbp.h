@@ -71,6 +71,13 @@ typedef struct {
     double ir_rtol;          /* mixed precision: also stop when the correction-based error
                                 estimate ||d_k|| min(1, ||d_k||/||d_k-1||) <= ir_rtol ||x||
                                 (0 = residual test only, the default) */
+    int bottom_n;            /* multigrid levels of side <= bottom_n (at most 65) run as one
+                                single-workgroup kernel per problem with the level data in
+                                LDS, when the coarsest side is <= 17 (default 33, 0 = off) */
+    int fuse_n;              /* with nu_pre = 2 / nu_post = 2, levels of side <= fuse_n run their
+                                pre-smoothing + residual + restriction, and their prolongation +
+                                post-smoothing, as one temporally blocked kernel each
+                                (default 1025, 0 = off) */
 } sbp_params;
 
 typedef struct sbp_handle_s* sbp_handle;

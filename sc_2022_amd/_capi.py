@@ -49,6 +49,8 @@ class SbpParams(C.Structure):
         ("tile_j", C.c_int),
         ("use_graph", C.c_int),
         ("ir_rtol", C.c_double),
+        ("bottom_n", C.c_int),
+        ("fuse_n", C.c_int),
     ]
 
 

@@ -51,7 +51,8 @@ def _run(cmd):
 
 def build(jobs: int | None = None, verbose: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
-    units = [(CSRC / "sbp_api.hip", OBJ / "sbp_api.o", [])]
+    units = [(CSRC / "sbp_api.hip", OBJ / "sbp_api.o", []),
+             (CSRC / "sbp_fused.hip", OBJ / "sbp_fused.o", [])]
     for ti, tj in tile_shapes():
         units.append((CSRC / "sbp_tile.hip", OBJ / f"sbp_tile_{ti}x{tj}.o",
                       [f"-DSBP_TI={ti}", f"-DSBP_TJ={tj}"]))

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/sbp.h"
+#include "sbp_bottom.h"
 #include "sbp_kernels.h"
 #include "sbp_launch.h"
 #include "sbp_tables.h"
@@ -37,6 +38,7 @@ struct Level {
     OpTab tb{};
     OpTab* dtab = nullptr;                // device copy of tb (tb.dev points here)
     int ti[2] = {0, 0}, tj[2] = {0, 0};   // tile shape per precision (0 = f64, 1 = f32)
+    int fti[2] = {128, 128};              // width of the fused two-sweep blocks per precision
     // per precision (0 = f64, 1 = f32), nprob problems each: rhs, two sweep
     // buffers, output, residual
     void* b[2] = {nullptr, nullptr};
@@ -51,6 +53,7 @@ struct Level {
 struct sbp_handle_s {
     int N = 0, p = 0, nlev = 0, nprob = 1;
     bool mg = false;
+    int bottom_l0 = -1;          // first level run by the single-workgroup bottom kernel (-1: none)
     sbp_params prm{};
     hipStream_t st = nullptr;
     RhsTab rt{};
@@ -201,6 +204,32 @@ int smooth_pass(sbp_handle h, int l, const IN& in, const T* b, T* dst, const T* 
     return launch_pass<T>(h, l, in, o, scb, nullptr, nullptr);
 }
 
+// Levels bottom_l0 .. nlev-1 of the V-cycle in one launch (one workgroup per problem).
+template <typename T>
+int launch_bottom(sbp_handle h, const T* b, T* out, const T* dotv, const Scalars* scb, double* partial) {
+    const int l0 = h->bottom_l0;
+    BottomCtl c{};
+    c.nlev = h->nlev - l0;
+    for (int k = 0; k < c.nlev; ++k) c.tab[k] = h->L[l0 + k].dtab;
+    c.Ainv = h->Ainv;
+    c.nu1 = h->prm.nu_pre;
+    c.nu2 = h->prm.nu_post;
+    c.use_diag = h->prm.smoother == 1 ? 0 : 1;
+    c.omega = h->prm.smoother == 1 ? h->prm.richardson_omega : h->prm.omega;
+    c.scale = h->prm.restrict_scale;
+    c.done = scb ? &scb->done : nullptr;
+    c.ds = kScalarInts;
+    c.sb = (long long)h->L[l0].n;
+    c.ss = 1;
+    c.partial = partial;
+    if (h->p == 2)
+        hipLaunchKernelGGL((vcycle_bottom<2, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, b, out, dotv, c);
+    else
+        hipLaunchKernelGGL((vcycle_bottom<4, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, b, out, dotv, c);
+    HIPCHK(hipGetLastError());
+    return SBP_OK;
+}
+
 // V-cycle on level l: x_out ~ A_l^-1 b (zero initial guess), every problem.
 // When dotv is non-null the last pass also writes block partials of dotv' x_out.
 template <typename T>
@@ -208,6 +237,7 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
     const int pi = prec_index<T>();
     Level& L = h->L[l];
     hipStream_t st = h->st;
+    if (l == h->bottom_l0) return launch_bottom<T>(h, b, out, dotv, scb, partial);
     if (l == h->nlev - 1) {
         hipLaunchKernelGGL((coarse_solve<T>), dim3(((int)L.n + kCoarseRows - 1) / kCoarseRows, h->nprob), dim3(256), 0,
                            st, h->Ainv, b, out, (int)L.n, scb);
@@ -226,7 +256,16 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
     // Pre-smoothing from zero.  The first sweep x1 = w D^-1 b is formed on
     // the fly inside the second sweep's pass.
     T* cur = xa;
-    if (nu1 == 0) {
+    const int use_diag = h->prm.smoother == 1 ? 0 : 1;
+    const int* done = scb ? &scb->done : nullptr;
+    const bool fused_pre = nu1 == 2 && L.N <= h->prm.fuse_n;
+    const bool fused_post = nu2 == 2 && L.N <= h->prm.fuse_n;
+    if (fused_pre) {
+        // x2 = S(S(0)) and the restricted residual in one pass
+        const int rc = launch_smooth2_restrict<T>(h->p, L.fti[pi], L.tb, b, xa, (T*)h->L[l + 1].b[pi], om, use_diag,
+                                                  (T)h->prm.restrict_scale, done, kScalarInts, batch_of(h, l), st);
+        if (rc != SBP_OK) return rc;
+    } else if (nu1 == 0) {
         hipLaunchKernelGGL((fill<T, kEwThreads>), ew_grid(h, L.n), dim3(kEwThreads), 0, st, xa, T(0), L.n, scb);
         HIPCHK(hipGetLastError());
     } else {
@@ -244,10 +283,13 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
     }
     // Residual and full-weighting restriction, fused (the fine residual stays on chip).
     Level& Lc = h->L[l + 1];
-    STCHK(launch_rr<T>(h, l, (const T*)cur, b, (T*)Lc.b[pi], scb));
+    if (!fused_pre) STCHK(launch_rr<T>(h, l, (const T*)cur, b, (T*)Lc.b[pi], scb));
     T* ec = (T*)Lc.xo[pi];
     STCHK(vcycle_level<T>(h, l + 1, (const T*)Lc.b[pi], ec, nullptr, scb, nullptr));
     // Prolongation folded into the first post-smoothing pass.
+    if (fused_post)
+        return launch_prolong_smooth2<T>(h->p, L.fti[pi], L.tb, (const T*)cur, (const T*)ec, b, out, dotv, om, use_diag,
+                                         done, kScalarInts, partial, batch_of(h, l), st);
     if (nu2 == 0) {
         hipLaunchKernelGGL((prolong_add<T>), dim3((L.N + 255) / 256, L.N, h->nprob), dim3(256), 0, st, (const T*)ec,
                            cur, L.N, Lc.N, scb);
@@ -282,7 +324,8 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
 template <typename T>
 int vcycle_nparts(sbp_handle h) {
     const Level& L = h->L[0];
-    if (h->nlev == 1) return 1;
+    if (h->nlev == 1 || h->bottom_l0 == 0) return 1;
+    if (h->prm.nu_post == 2 && L.N <= h->prm.fuse_n) return fused_post_blocks(L.N, L.fti[prec_index<T>()]);
     if (h->prm.nu_post == 0) return ew_blocks(L.n);
     return (int)pass_blocks(L.N, L.ti[prec_index<T>()], L.tj[prec_index<T>()]);
 }
@@ -756,6 +799,8 @@ void sbp_default_params(sbp_params* prm) {
     prm->tile_j = 0;
     prm->use_graph = 1;
     prm->ir_rtol = 0.0;
+    prm->bottom_n = 33;
+    prm->fuse_n = 1025;
 }
 
 int sbp_destroy(sbp_handle h) {
@@ -796,7 +841,8 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
     if ((long long)N * N * nprob > (1ll << 40)) return SBP_ERR_INVALID_SIZE;
     if (prm && (prm->nu_pre < 0 || prm->nu_post < 0 || prm->poll_every < 1 || prm->max_outer < 1 ||
                 prm->inner_maxiter < 1 || !(prm->omega > 0) || !(prm->restrict_scale > 0) ||
-                (prm->smoother != 0 && prm->smoother != 1)))
+                (prm->smoother != 0 && prm->smoother != 1) || prm->bottom_n < 0 ||
+                prm->fuse_n < 0))
         return SBP_ERR_INVALID_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SBP_ERR_NO_DEVICE;
@@ -827,6 +873,14 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
         else nlev = 1;
     }
     h->nlev = nlev;
+    // Bottom kernel: from the first level of side <= bottom_n down, when the
+    // coarsest system is small enough for one workgroup's direct solve.
+    if (h->mg && h->L[nlev - 1].n <= (size_t)kBottomMaxCoarse) {
+        const int bn = std::min(h->prm.bottom_n, kBottomN);
+        for (int l = 0; l < nlev; ++l)
+            if (h->L[l].N <= bn) { h->bottom_l0 = l; break; }
+        if (h->bottom_l0 >= 0 && nlev - h->bottom_l0 > kBottomMaxLevels) h->bottom_l0 = -1;
+    }
     for (int l = 0; l < nlev; ++l) {
         Level& L = h->L[l];
         for (int q = 0; q < 2; ++q) {

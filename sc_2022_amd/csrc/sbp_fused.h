@@ -1,0 +1,389 @@
+// sbp_fused.h -- temporally blocked V-cycle passes.
+//
+// A V-cycle level with two pre- and two post-smoothing sweeps does, on the
+// fine grid, (pre) x2 = S(S(0)) then the residual b - A x2 and its
+// restriction, and (post) out = S(S(x2 + P e)).  Run as separate stencil
+// passes this streams x2, the intermediate sweep and b through HBM several
+// times, and on the small levels every pass is a kernel boundary (~5 us
+// whatever the size).  Here each block loads its tile of the inputs once,
+// with a halo two stencil radii wide, runs both sweeps (and the residual)
+// in LDS, recomputing the halo points the neighbouring blocks also compute,
+// and writes only the results:
+//
+//   smooth2_restrict  reads b; writes x2 and rc = scale * FW(b - A x2)
+//   prolong_smooth2   reads x2, e_c, b (and dotv); writes out (and dotv'out)
+//
+// Geometry.  A block's output region C is TI columns x JC rows; the first
+// sweep is evaluated on region B = C widened by H (the stencil radius) and
+// its operand on region A = C widened by 2H.  Blocks are placed on a regular
+// grid except that the last block in each direction is shifted back so its
+// region C ends at the grid's far face; together with C >= closure width this
+// guarantees that every closure row/column a face point reads lies inside
+// the block's regions.  Each block writes only the points (and coarse points)
+// it owns under the unshifted tiling, so shifted blocks overlap their
+// neighbours' work but never their writes.
+#pragma once
+#include "sbp_stencil.h"
+
+namespace sbp {
+
+// Operand read from an LDS 2-D array holding columns [i0, i0 + W) and rows
+// [j0, ...) of the grid (row stride W).
+template <typename T, int W> struct InGrid {
+    const T* a;
+    int i0, j0;
+    using Raw = T;
+    template <bool INT = false, bool LT = false>
+    __device__ __forceinline__ Raw fetch(const OpTab&, int i, int j) const { return a[(j - j0) * W + (i - i0)]; }
+    template <bool INT = false, bool LT = false>
+    __device__ __forceinline__ T make(const OpTab&, int, int, const Raw& r) const { return r; }
+    template <bool INT = false, bool LT = false>
+    __device__ __forceinline__ T load(const OpTab&, int i, int j) const { return a[(j - j0) * W + (i - i0)]; }
+};
+
+__device__ __forceinline__ bool is_interior(const OpTab& t, int i, int j) {
+    return i >= t.R && i < t.N - t.R && j >= t.R && j < t.N - t.R;
+}
+
+// (A u) at an interior point of an LDS 2-D array of row stride W, k the
+// point's index: the (2p+1)-point cross, same order of operations as the
+// tiled passes' interior path (j-direction part first, h-weights 1).
+template <int P, typename T, int W>
+__device__ __forceinline__ T au_interior(const T* a, int k) {
+    constexpr int H = Stencil<P>::H;
+    const T uc = a[k];
+    T kp = Stencil<P>::template c<T>(1) * ((a[k - W] - uc) + (a[k + W] - uc));
+#pragma unroll
+    for (int s = 2; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * ((a[k - s * W] - uc) + (a[k + s * W] - uc));
+    T mp = Stencil<P>::template c<T>(1) * ((a[k - 1] - uc) + (a[k + 1] - uc));
+#pragma unroll
+    for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((a[k - s] - uc) + (a[k + s] - uc));
+    return kp + mp;
+}
+
+// (A u)(i, j) for the operand in an LDS array covering columns [i0, i0 + W),
+// rows [j0, ...): the interior cross where it applies, else the closure-aware
+// general form (INNER: every point is interior, the general form is not
+// compiled).
+template <int P, typename T, bool LT, bool INNER, int W>
+__device__ __forceinline__ T au_grid(const OpTab& t, const T* a, int i0, int j0, int i, int j) {
+    const int k = (j - j0) * W + (i - i0);
+    if (INNER || is_interior(t, i, j)) return au_interior<P, T, W>(a, k);
+    const InGrid<T, W> in{a, i0, j0};
+    return au_point<P, T, LT>(t, in, i, j);
+}
+
+// Scale of the smoothing correction w (b - Au): w / diag (damped Jacobi; the
+// inverse diagonal is the constant idint away from the faces) or w (Richardson).
+template <typename T, bool LT, bool INNER>
+__device__ __forceinline__ T smooth_corr(const OpTab& t, int i, int j, T r, T omega, int use_diag) {
+    T c = omega * r;
+    if (!use_diag) return c;
+    if (INNER || is_interior(t, i, j)) return c * (T)t.idint;
+    return c / diag_of<T, LT>(t, i, j);
+}
+
+struct FusedCtl {
+    const int* done;    // nullable: per-problem flags, stride ds ints
+    int ds;
+    int nbx, nby;       // block grid of one problem
+    double* partial;    // prolong_smooth2 with dotv: per-block partials of dotv'out
+    Batch bt;           // sf: fine elements, sc: coarse elements per problem
+};
+
+// ---------------------------------------------------------------------------
+// x2 = S(S(0)) and rc = scale * FW(b - A x2)
+// ---------------------------------------------------------------------------
+template <int P, int TI, int J, int NT, typename T, bool LT, bool INNER>
+__device__ __forceinline__ void sr_body(const T* __restrict__ b, T* __restrict__ x2g, T* __restrict__ rc,
+                                        T omega, int use_diag, T scale, const OpTab& t, int bx, int by,
+                                        int Nc, T* Bt, T* U, T* V) {
+    constexpr int H = Stencil<P>::H;
+    constexpr int JR = J + 1;                     // fine residual rows (region C)
+    constexpr int CI = TI / 2 - 1, CJ = J / 2;    // coarse points owned per block
+    constexpr int AW = TI + 4 * H, AH = JR + 4 * H;
+    constexpr int BW = TI + 2 * H, BH = JR + 2 * H;
+    const int N = t.N;
+    const int tid = threadIdx.x;
+    int Ic0 = bx * CI < Nc - CI ? bx * CI : Nc - CI;
+    int Jc0 = by * CJ < Nc - CJ ? by * CJ : Nc - CJ;
+    if (Ic0 < 0) Ic0 = 0;
+    if (Jc0 < 0) Jc0 = 0;
+    const int ci0 = 2 * Ic0 - 1, cj0 = 2 * Jc0 - 1;   // region C origin
+    const int ai0 = ci0 - 2 * H, aj0 = cj0 - 2 * H;   // region A origin
+    const int bi0 = ci0 - H, bj0 = cj0 - H;           // region B origin
+    // A: b and x1 = w D^-1 b (all loads issued before the first use)
+    {
+        constexpr int NA = (AW * AH + NT - 1) / NT;
+        T v[NA];
+#pragma unroll
+        for (int m = 0; m < NA; ++m) {
+            const int k = tid + m * NT;
+            const int r = k / AW, c = k - r * AW;
+            const int i = ai0 + c, j = aj0 + r;
+            v[m] = (k < AW * AH && i >= 0 && i < N && j >= 0 && j < N) ? b[gidx(i, j, N)] : T(0);
+        }
+#pragma unroll
+        for (int m = 0; m < NA; ++m) {
+            const int k = tid + m * NT;
+            if (k < AW * AH) {
+                const int r = k / AW, c = k - r * AW;
+                const int i = ai0 + c, j = aj0 + r;
+                Bt[k] = v[m];
+                U[k] = (i >= 0 && i < N && j >= 0 && j < N) ? smooth_corr<T, LT, INNER>(t, i, j, v[m], omega, use_diag)
+                                                            : T(0);
+            }
+        }
+    }
+    __syncthreads();
+    // B: x2 = x1 + w D^-1 (b - A x1)
+    for (int k = tid; k < BW * BH; k += NT) {
+        const int r = k / BW, c = k - r * BW;
+        const int i = bi0 + c, j = bj0 + r;
+        T x2 = T(0);
+        if (INNER || (i >= 0 && i < N && j >= 0 && j < N)) {
+            const int ka = (r + H) * AW + (c + H);
+            const T Au = au_grid<P, T, LT, INNER, AW>(t, U, ai0, aj0, i, j);
+            x2 = U[ka] + smooth_corr<T, LT, INNER>(t, i, j, Bt[ka] - Au, omega, use_diag);
+        }
+        V[k] = x2;
+    }
+    __syncthreads();
+    // x2 at the fine points this block owns
+    {
+        const int oi0 = 2 * bx * CI, oj0 = 2 * by * CJ;
+        const int oi1 = oi0 + 2 * CI < N ? oi0 + 2 * CI : N;
+        const int oj1 = oj0 + 2 * CJ < N ? oj0 + 2 * CJ : N;
+        const int ow = oi1 - oi0;
+        for (int k = tid; k < ow * (oj1 - oj0); k += NT) {
+            const int r = k / ow, c = k - r * ow;
+            const int i = oi0 + c, j = oj0 + r;
+            x2g[gidx(i, j, N)] = V[(j - bj0) * BW + (i - bi0)];
+        }
+    }
+    // C: residual r = b - A x2 into U's storage (x1 is no longer read)
+    T* R = U;
+    for (int k = tid; k < TI * JR; k += NT) {
+        const int r = k / TI, c = k - r * TI;
+        const int i = ci0 + c, j = cj0 + r;
+        T rv = T(0);
+        if (INNER || (i >= 0 && i < N && j >= 0 && j < N))
+            rv = Bt[(r + 2 * H) * AW + (c + 2 * H)] - au_grid<P, T, LT, INNER, BW>(t, V, bi0, bj0, i, j);
+        R[k] = rv;
+    }
+    __syncthreads();
+    // Full weighting [1 2 1; 2 4 2; 1 2 1] / 16 (times scale) of the owned coarse points.
+    {
+        const int I0 = bx * CI, J0 = by * CJ;
+        const int I1 = I0 + CI < Nc ? I0 + CI : Nc, J1 = J0 + CJ < Nc ? J0 + CJ : Nc;
+        const int cw = I1 - I0;
+        for (int o = tid; o < cw * (J1 - J0); o += NT) {
+            const int rr = o / cw, cc = o - rr * cw;
+            const int I = I0 + cc, Jc = J0 + rr;
+            const int lc = 2 * I - ci0, lr = 2 * Jc - cj0;   // fine (2I, 2Jc) in region C
+            const T* r0 = R + lr * TI + lc;
+            const T s0 = r0[0] + T(0.5) * (r0[-1] + r0[1]);
+            const T sm = r0[-TI] + T(0.5) * (r0[-TI - 1] + r0[-TI + 1]);
+            const T sp = r0[TI] + T(0.5) * (r0[TI - 1] + r0[TI + 1]);
+            rc[gidx(I, Jc, Nc)] = scale * T(0.25) * (s0 + T(0.5) * (sm + sp));
+        }
+    }
+}
+
+template <int P, int TI, int J, int NT, typename T>
+__global__ __launch_bounds__(NT) void smooth2_restrict(const T* __restrict__ b, T* __restrict__ x2g,
+                                                      T* __restrict__ rc, T omega, int use_diag, T scale, OpTab t,
+                                                      FusedCtl ctl) {
+    constexpr int H = Stencil<P>::H;
+    constexpr int JR = J + 1;
+    constexpr int AW = TI + 4 * H, AH = JR + 4 * H;
+    constexpr int BW = TI + 2 * H, BH = JR + 2 * H;
+    const int nb = ctl.nbx * ctl.nby;
+    const int ball = xcd_remap((int)blockIdx.x, nb * ctl.bt.nprob);
+    const int q = ball / nb, blk = ball - q * nb;
+    if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
+    b += q * ctl.bt.sf;
+    x2g += q * ctl.bt.sf;
+    rc += q * ctl.bt.sc;
+    const int bx = blk % ctl.nbx, by = blk / ctl.nbx;
+    const int Nc = (t.N + 1) / 2;
+    __shared__ T Bt[AW * AH];
+    __shared__ T U[AW * AH];
+    __shared__ T V[BW * BH];
+    constexpr int CI = TI / 2 - 1, CJ = J / 2;
+    // Interior block: regions A lie inside the closure-free zone.
+    int Ic0 = bx * CI < Nc - CI ? bx * CI : Nc - CI;
+    int Jc0 = by * CJ < Nc - CJ ? by * CJ : Nc - CJ;
+    if (Ic0 < 0) Ic0 = 0;
+    if (Jc0 < 0) Jc0 = 0;
+    const int ai0 = 2 * Ic0 - 1 - 2 * H, aj0 = 2 * Jc0 - 1 - 2 * H;
+    const bool inner = ai0 >= t.R && ai0 + AW <= t.N - t.R && aj0 >= t.R && aj0 + AH <= t.N - t.R;
+    if (inner) {
+        sr_body<P, TI, J, NT, T, false, true>(b, x2g, rc, omega, use_diag, scale, t, bx, by, Nc, Bt, U, V);
+    } else {
+        __shared__ OpTab ltab;
+        stage_table(t, &ltab);
+        sr_body<P, TI, J, NT, T, true, false>(b, x2g, rc, omega, use_diag, scale, ltab, bx, by, Nc, Bt, U, V);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// out = S(S(x + P e)), optionally with block partials of dotv' out
+// ---------------------------------------------------------------------------
+template <int P, int TI, int J, int NT, typename T, bool LT, bool INNER>
+__device__ __forceinline__ T ps_body(const T* __restrict__ x, const T* __restrict__ ec, const T* __restrict__ b,
+                                     T* __restrict__ out, const T* __restrict__ dotv, T omega, int use_diag,
+                                     const OpTab& t, int bx, int by, T* U, T* E, T* Bt, T* V) {
+    constexpr int H = Stencil<P>::H;
+    constexpr int AW = TI + 4 * H, AH = J + 4 * H;
+    constexpr int BW = TI + 2 * H, BH = J + 2 * H;
+    constexpr int EW = AW / 2 + 2, EH = AH / 2 + 2;
+    const int N = t.N, Nc = (N + 1) / 2;
+    const int tid = threadIdx.x;
+    int i0 = bx * TI < N - TI ? bx * TI : N - TI;
+    int j0 = by * J < N - J ? by * J : N - J;
+    if (i0 < 0) i0 = 0;
+    if (j0 < 0) j0 = 0;
+    const int ai0 = i0 - 2 * H, aj0 = j0 - 2 * H;
+    const int bi0 = i0 - H, bj0 = j0 - H;
+    const int ei0 = ai0 >> 1, ej0 = aj0 >> 1;   // coarse patch origin (floor)
+    // Coarse patch, x on A and b on B: every load issued before the first use.
+    {
+        constexpr int NE = (EW * EH + NT - 1) / NT;
+        constexpr int NA = (AW * AH + NT - 1) / NT;
+        constexpr int NB = (BW * BH + NT - 1) / NT;
+        T ve[NE], va[NA], vb[NB];
+#pragma unroll
+        for (int m = 0; m < NE; ++m) {
+            const int k = tid + m * NT;
+            const int r = k / EW, c = k - r * EW;
+            const int I = ei0 + c, Jc = ej0 + r;
+            ve[m] = (k < EW * EH && I >= 0 && I < Nc && Jc >= 0 && Jc < Nc) ? ec[gidx(I, Jc, Nc)] : T(0);
+        }
+#pragma unroll
+        for (int m = 0; m < NA; ++m) {
+            const int k = tid + m * NT;
+            const int r = k / AW, c = k - r * AW;
+            const int i = ai0 + c, j = aj0 + r;
+            va[m] = (k < AW * AH && i >= 0 && i < N && j >= 0 && j < N) ? x[gidx(i, j, N)] : T(0);
+        }
+#pragma unroll
+        for (int m = 0; m < NB; ++m) {
+            const int k = tid + m * NT;
+            const int r = k / BW, c = k - r * BW;
+            const int i = bi0 + c, j = bj0 + r;
+            vb[m] = (k < BW * BH && i >= 0 && i < N && j >= 0 && j < N) ? b[gidx(i, j, N)] : T(0);
+        }
+#pragma unroll
+        for (int m = 0; m < NE; ++m) {
+            const int k = tid + m * NT;
+            if (k < EW * EH) E[k] = ve[m];
+        }
+#pragma unroll
+        for (int m = 0; m < NB; ++m) {
+            const int k = tid + m * NT;
+            if (k < BW * BH) Bt[k] = vb[m];
+        }
+        __syncthreads();
+        // U = x + P e on A (bilinear prolongation from the coarse patch)
+#pragma unroll
+        for (int m = 0; m < NA; ++m) {
+            const int k = tid + m * NT;
+            if (k < AW * AH) {
+                const int r = k / AW, c = k - r * AW;
+                const int i = ai0 + c, j = aj0 + r;
+                T u = T(0);
+                if (i >= 0 && i < N && j >= 0 && j < N) {
+                    const int ic = i >> 1, jc = j >> 1;
+                    const int ic1 = ic + 1 < Nc ? ic + 1 : ic, jc1 = jc + 1 < Nc ? jc + 1 : jc;
+                    const T e00 = E[(jc - ej0) * EW + (ic - ei0)], e10 = E[(jc - ej0) * EW + (ic1 - ei0)];
+                    const T e01 = E[(jc1 - ej0) * EW + (ic - ei0)], e11 = E[(jc1 - ej0) * EW + (ic1 - ei0)];
+                    T pv;
+                    if (!(i & 1)) pv = (!(j & 1)) ? e00 : (e00 + e01) * T(0.5);
+                    else pv = (!(j & 1)) ? (e00 + e10) * T(0.5) : (e00 + e10 + e01 + e11) * T(0.25);
+                    u = va[m] + pv;
+                }
+                U[k] = u;
+            }
+        }
+    }
+    __syncthreads();
+    // B: v1 = u + w D^-1 (b - A u)
+    for (int k = tid; k < BW * BH; k += NT) {
+        const int r = k / BW, c = k - r * BW;
+        const int i = bi0 + c, j = bj0 + r;
+        T v1 = T(0);
+        if (INNER || (i >= 0 && i < N && j >= 0 && j < N)) {
+            const T Au = au_grid<P, T, LT, INNER, AW>(t, U, ai0, aj0, i, j);
+            v1 = U[(r + H) * AW + (c + H)] + smooth_corr<T, LT, INNER>(t, i, j, Bt[k] - Au, omega, use_diag);
+        }
+        V[k] = v1;
+    }
+    __syncthreads();
+    // C: out = v1 + w D^-1 (b - A v1) on the owned points
+    T acc = T(0);
+    {
+        const int oi0 = bx * TI, oj0 = by * J;
+        const int oi1 = oi0 + TI < N ? oi0 + TI : N, oj1 = oj0 + J < N ? oj0 + J : N;
+        const int ow = oi1 - oi0;
+        for (int k = tid; k < ow * (oj1 - oj0); k += NT) {
+            const int r = k / ow, c = k - r * ow;
+            const int i = oi0 + c, j = oj0 + r;
+            const int kb = (j - bj0) * BW + (i - bi0);
+            const T Av = au_grid<P, T, LT, INNER, BW>(t, V, bi0, bj0, i, j);
+            const T o = V[kb] + smooth_corr<T, LT, INNER>(t, i, j, Bt[kb] - Av, omega, use_diag);
+            const size_t g = gidx(i, j, N);
+            out[g] = o;
+            if (dotv) acc += dotv[g] * o;
+        }
+    }
+    return acc;
+}
+
+template <int P, int TI, int J, int NT, typename T>
+__global__ __launch_bounds__(NT) void prolong_smooth2(const T* __restrict__ x, const T* __restrict__ ec,
+                                                     const T* __restrict__ b, T* __restrict__ out,
+                                                     const T* __restrict__ dotv, T omega, int use_diag, OpTab t,
+                                                     FusedCtl ctl) {
+    constexpr int H = Stencil<P>::H;
+    constexpr int AW = TI + 4 * H, AH = J + 4 * H;
+    constexpr int BW = TI + 2 * H, BH = J + 2 * H;
+    constexpr int EW = AW / 2 + 2, EH = AH / 2 + 2;
+    const int nb = ctl.nbx * ctl.nby;
+    const int ball = xcd_remap((int)blockIdx.x, nb * ctl.bt.nprob);
+    const int q = ball / nb, blk = ball - q * nb;
+    if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
+    x += q * ctl.bt.sf;
+    b += q * ctl.bt.sf;
+    out += q * ctl.bt.sf;
+    ec += q * ctl.bt.sc;
+    if (dotv) dotv += q * ctl.bt.sf;
+    const int bx = blk % ctl.nbx, by = blk / ctl.nbx;
+    const int N = t.N;
+    __shared__ T U[AW * AH];
+    __shared__ T E[EW * EH];
+    __shared__ T Bt[BW * BH];
+    __shared__ T V[BW * BH];
+    int i0 = bx * TI < N - TI ? bx * TI : N - TI;
+    int j0 = by * J < N - J ? by * J : N - J;
+    if (i0 < 0) i0 = 0;
+    if (j0 < 0) j0 = 0;
+    const int ai0 = i0 - 2 * H, aj0 = j0 - 2 * H;
+    const bool inner = ai0 >= t.R && ai0 + AW <= N - t.R && aj0 >= t.R && aj0 + AH <= N - t.R;
+    T acc;
+    if (inner) {
+        acc = ps_body<P, TI, J, NT, T, false, true>(x, ec, b, out, dotv, omega, use_diag, t, bx, by, U, E, Bt, V);
+    } else {
+        __shared__ OpTab ltab;
+        stage_table(t, &ltab);
+        acc = ps_body<P, TI, J, NT, T, true, false>(x, ec, b, out, dotv, omega, use_diag, ltab, bx, by, U, E, Bt,
+                                                     V);
+    }
+    if (dotv) {
+        __syncthreads();
+        const T s = block_sum<T, NT>(acc);
+        if (threadIdx.x == 0) ctl.partial[(size_t)q * nb + blk] = (double)s;
+    }
+}
+
+}  // namespace sbp

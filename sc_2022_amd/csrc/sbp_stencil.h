@@ -86,19 +86,28 @@ template <> struct Stencil<4> {
 
 __device__ __forceinline__ size_t gidx(int i, int j, int N) { return (size_t)j * (size_t)N + (size_t)i; }
 
-template <typename T>
-__device__ __forceinline__ T hw_of(const OpTab& t, int r) {
-    const int rr = r < t.N - 1 - r ? r : t.N - 1 - r;   // distance to the nearer face
-    return rr < t.RH ? (T)t.dev->hw[rr] : T(1);
+// Where per-thread table lookups go: LT = false, the table's device-memory
+// copy (t.dev); LT = true, the table object itself, which the caller has
+// placed in LDS (its fields are then plain LDS reads).
+template <bool LT>
+__device__ __forceinline__ const OpTab& lut(const OpTab& t) {
+    if constexpr (LT) return t;
+    else return *t.dev;
 }
 
-template <typename T>
+template <typename T, bool LT = false>
+__device__ __forceinline__ T hw_of(const OpTab& t, int r) {
+    const int rr = r < t.N - 1 - r ? r : t.N - 1 - r;   // distance to the nearer face
+    return rr < t.RH ? (T)lut<LT>(t).hw[rr] : T(1);
+}
+
+template <typename T, bool LT = false>
 __device__ __forceinline__ T diag_of(const OpTab& t, int i, int j) {
     const int jj = j < t.N - 1 - j ? j : t.N - 1 - j;
     const int ii = i < t.N - 1 - i ? i : t.N - 1 - i;
-    const T kj = jj < t.R ? (T)t.dev->kd[jj] : (T)t.mid;
-    const T mi = ii < t.R ? (T)t.dev->md[ii] : (T)t.mid;
-    return hw_of<T>(t, i) * kj + hw_of<T>(t, j) * mi;
+    const T kj = jj < t.R ? (T)lut<LT>(t).kd[jj] : (T)t.mid;
+    const T mi = ii < t.R ? (T)lut<LT>(t).md[ii] : (T)t.mid;
+    return hw_of<T, LT>(t, i) * kj + hw_of<T, LT>(t, j) * mi;
 }
 
 // Rows 0..N-1 are remapped so that the 8 XCDs (round-robin dispatch) each get
@@ -123,11 +132,11 @@ template <typename T> struct InPlain {
     static constexpr bool kCol = false;
     using Raw = T;
     __device__ void init() {}
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const { return u[gidx(i, j, t.N)]; }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T make(const OpTab&, int, int, const Raw& r) const { return r; }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const { return u[gidx(i, j, t.N)]; }
 };
 
@@ -139,17 +148,17 @@ template <typename T> struct InSmooth0 {
     static constexpr bool kCol = false;
     using Raw = T;
     __device__ void init() {}
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const { return b[gidx(i, j, t.N)]; }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T make(const OpTab& t, int i, int j, const Raw& r) const {
         const T v = omega * r;
         if (!use_diag) return v;
-        return INT ? v * (T)t.idint : v / diag_of<T>(t, i, j);
+        return INT ? v * (T)t.idint : v / diag_of<T, LT>(t, i, j);
     }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
-        return make<INT>(t, i, j, fetch<INT>(t, i, j));
+        return make<INT, LT>(t, i, j, fetch<INT, LT>(t, i, j));
     }
 };
 
@@ -181,7 +190,7 @@ template <typename T> struct InProlong {
         }
     }
     struct Raw { T x, e00, e10, e01, e11; };
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const {
         const int ic = i >> 1, jc = j >> 1;
         const int ic1 = ic + 1 < Nc ? ic + 1 : ic, jc1 = jc + 1 < Nc ? jc + 1 : jc;
@@ -193,16 +202,16 @@ template <typename T> struct InProlong {
         r.e11 = ec[gidx(ic1, jc1, Nc)];
         return r;
     }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T make(const OpTab&, int i, int j, const Raw& r) const {
         T v;
         if (!(i & 1)) v = (!(j & 1)) ? r.e00 : (r.e00 + r.e01) * T(0.5);
         else v = (!(j & 1)) ? (r.e00 + r.e10) * T(0.5) : (r.e00 + r.e10 + r.e01 + r.e11) * T(0.25);
         return r.x + v;
     }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
-        return make<INT>(t, i, j, fetch<INT>(t, i, j));
+        return make<INT, LT>(t, i, j, fetch<INT, LT>(t, i, j));
     }
 };
 
@@ -213,7 +222,7 @@ template <typename T> struct InAxpy {
     static constexpr bool kCol = false;
     __device__ void init() { bt = (T)(*beta); }
     struct Raw { T r, p; };
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const {
         const size_t q = gidx(i, j, t.N);
         Raw v;
@@ -221,9 +230,9 @@ template <typename T> struct InAxpy {
         v.p = p[q];
         return v;
     }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T make(const OpTab&, int, int, const Raw& v) const { return v.r + bt * v.p; }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
         const size_t q = gidx(i, j, t.N);
         return r[q] + bt * p[q];
@@ -240,7 +249,7 @@ template <typename T> struct OutStore {            // o = s * Au
     static constexpr bool kDot = false;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T, T Au, Aux, T&) const { o[q] = s * Au; }
 };
 
@@ -250,7 +259,7 @@ template <typename T> struct OutStoreDot {         // o = Au ; acc += u Au
     static constexpr bool kDot = true;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux, T& acc) const {
         o[q] = Au; acc += u * Au;
     }
@@ -262,7 +271,7 @@ template <typename T> struct OutWriteDot {         // w = u (materialise the for
     static constexpr bool kDot = true;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux, T& acc) const {
         w[q] = u; o[q] = Au; acc += u * Au;
     }
@@ -274,7 +283,7 @@ template <typename T, typename TO> struct OutResid {  // r = b - Au (optionally 
     static constexpr bool kDot = true;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t q) const { return b[q]; }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T, T Au, Aux bq, T& acc) const {
         T rv = bq - Au; r[q] = (TO)rv; acc += rv * rv;
     }
@@ -288,10 +297,10 @@ template <typename T> struct OutSmooth {
     static constexpr bool kDot = false;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t q) const { return b[q]; }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ void emit(const OpTab& t, int i, int j, size_t q, T u, T Au, Aux bq, T&) const {
         T c = omega * (bq - Au);
-        if (use_diag) c = INT ? c * (T)t.idint : c / diag_of<T>(t, i, j);
+        if (use_diag) c = INT ? c * (T)t.idint : c / diag_of<T, LT>(t, i, j);
         xo[q] = u + c;
     }
 };
@@ -303,10 +312,10 @@ template <typename T> struct OutSmoothDot {
     static constexpr bool kDot = true;
     struct Aux { T b, d; };
     __device__ Aux pre(const OpTab&, size_t q) const { Aux a; a.b = b[q]; a.d = dotv[q]; return a; }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ void emit(const OpTab& t, int i, int j, size_t q, T u, T Au, Aux a, T& acc) const {
         T c = omega * (a.b - Au);
-        if (use_diag) c = INT ? c * (T)t.idint : c / diag_of<T>(t, i, j);
+        if (use_diag) c = INT ? c * (T)t.idint : c / diag_of<T, LT>(t, i, j);
         const T xn = u + c;
         xo[q] = xn; acc += a.d * xn;
     }
@@ -322,7 +331,7 @@ template <typename T> struct OutPUpd {
     struct Aux { T x, p; };
     __device__ void init() { al = (T)(*alpha); }
     __device__ Aux pre(const OpTab&, size_t q) const { Aux a; a.x = x[q]; a.p = pold[q]; return a; }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux a, T& acc) const {
         pnew[q] = u; Ap[q] = Au; x[q] = a.x + al * a.p; acc += u * Au;
     }
@@ -335,7 +344,7 @@ template <typename T> struct OutCopy {
     static constexpr bool kDot = false;
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
-    template <bool INT = false>
+    template <bool INT = false, bool LT = false>
     __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T, Aux, T&) const { o[q] = u; }
 };
 
@@ -370,33 +379,35 @@ __device__ __forceinline__ T block_sum(T v) {
 // ---------------------------------------------------------------------------
 // Closure rows via direct (cache-served) loads, used only for face rows and
 // columns; uc is the operand at (i, j).
-template <typename T, class IN>
+template <typename T, bool LT = false, class IN>
 __device__ __forceinline__ T k_closure(const OpTab& t, const IN& in, int i, int j, T uc) {
     const bool lo = j < t.R;
     const int r = lo ? j : t.N - 1 - j;
     typename IN::Raw raw[kMaxC];
 #pragma unroll
     for (int c = 0; c < kMaxC; ++c)
-        if (c < t.C) raw[c] = in.fetch(t, i, lo ? c : t.N - 1 - c);
-    T s = (T)t.dev->krs[r] * uc;
+        if (c < t.C) raw[c] = in.template fetch<false, LT>(t, i, lo ? c : t.N - 1 - c);
+    T s = (T)lut<LT>(t).krs[r] * uc;
 #pragma unroll
     for (int c = 0; c < kMaxC; ++c)
-        if (c < t.C && c != r) s += (T)t.dev->kb[r][c] * (in.make(t, i, lo ? c : t.N - 1 - c, raw[c]) - uc);
+        if (c < t.C && c != r)
+            s += (T)lut<LT>(t).kb[r][c] * (in.template make<false, LT>(t, i, lo ? c : t.N - 1 - c, raw[c]) - uc);
     return s;
 }
 
-template <typename T, class IN>
+template <typename T, bool LT = false, class IN>
 __device__ __forceinline__ T m_closure(const OpTab& t, const IN& in, int i, int j, T uc) {
     const bool lo = i < t.R;
     const int r = lo ? i : t.N - 1 - i;
     typename IN::Raw raw[kMaxC];
 #pragma unroll
     for (int c = 0; c < kMaxC; ++c)
-        if (c < t.C) raw[c] = in.fetch(t, lo ? c : t.N - 1 - c, j);
-    T s = (T)t.dev->mrs[r] * uc;
+        if (c < t.C) raw[c] = in.template fetch<false, LT>(t, lo ? c : t.N - 1 - c, j);
+    T s = (T)lut<LT>(t).mrs[r] * uc;
 #pragma unroll
     for (int c = 0; c < kMaxC; ++c)
-        if (c < t.C && c != r) s += (T)t.dev->mb[r][c] * (in.make(t, lo ? c : t.N - 1 - c, j, raw[c]) - uc);
+        if (c < t.C && c != r)
+            s += (T)lut<LT>(t).mb[r][c] * (in.template make<false, LT>(t, lo ? c : t.N - 1 - c, j, raw[c]) - uc);
     return s;
 }
 
@@ -426,7 +437,7 @@ template <class X> struct has_init<X, decltype(void(&X::init))> { static constex
 // case).  With ICHK = true (the first/last column of tiles) each thread tests
 // its own column: out-of-grid threads idle, threads on the i-closure rows take
 // the closure sum and the H weight; everything else stays on the fast path.
-template <int P, int TI, int J, bool ICHK, typename T, class IN, class OUT>
+template <int P, int TI, int J, bool ICHK, typename T, bool LT = false, class IN, class OUT>
 __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const OpTab& t, int i0, int j0,
                                            T (*tile)[TI + 2 * Stencil<P>::H]) {
     constexpr int H = Stencil<P>::H;
@@ -443,9 +454,9 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
         } else {
             typename IN::Raw raw[W];
 #pragma unroll
-            for (int k = 0; k < W; ++k) raw[k] = in.template fetch<INT>(t, i, j0 - H + k);
+            for (int k = 0; k < W; ++k) raw[k] = in.template fetch<INT, LT>(t, i, j0 - H + k);
 #pragma unroll
-            for (int k = 0; k < W; ++k) w[k] = in.template make<INT>(t, i, j0 - H + k, raw[k]);
+            for (int k = 0; k < W; ++k) w[k] = in.template make<INT, LT>(t, i, j0 - H + k, raw[k]);
         }
     } else {
 #pragma unroll
@@ -460,7 +471,7 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
             const int off = rem / J, r = rem % J;
             const int ii = side == 0 ? i0 - H + off : i0 + TI + off;
             T v = T(0);
-            if (!ICHK || (ii >= 0 && ii < N)) v = in.template load<INT>(t, ii, j0 + r);
+            if (!ICHK || (ii >= 0 && ii < N)) v = in.template load<INT, LT>(t, ii, j0 + r);
             tile[r][side == 0 ? off : TI + H + off] = v;
         }
     }
@@ -474,7 +485,7 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
     T acc = T(0);
     if (!act) return acc;
     const bool iint = !ICHK || ((i >= t.R) && (i < N - t.R));
-    const T hwi = ICHK ? hw_of<T>(t, i) : T(1);
+    const T hwi = ICHK ? hw_of<T, LT>(t, i) : T(1);
 #pragma unroll
     for (int r = 0; r < J; ++r) {
         const T uc = w[r + H];
@@ -488,16 +499,16 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
 #pragma unroll
             for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
         } else {
-            mp = m_closure<T>(t, in, i, j0 + r, uc);
+            mp = m_closure<T, LT>(t, in, i, j0 + r, uc);
         }
         const T Au = ICHK ? hwi * kp + mp : kp + mp;   // hw_j = 1 on interior rows
-        out.template emit<INT>(t, i, j0 + r, gidx(i, j0 + r, N), uc, Au, aux[r], acc);
+        out.template emit<INT, LT>(t, i, j0 + r, gidx(i, j0 + r, N), uc, Au, aux[r], acc);
     }
     return acc;
 }
 
 // General tile: bounds tests, face closures and H weights.
-template <int P, int TI, int J, typename T, class IN, class OUT>
+template <int P, int TI, int J, typename T, bool LT = false, class IN, class OUT>
 __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const OpTab& t, int i0, int j0,
                                           T (*tile)[TI + 2 * Stencil<P>::H]) {
     constexpr int H = Stencil<P>::H;
@@ -512,12 +523,12 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
 #pragma unroll
         for (int k = 0; k < W; ++k) {
             const int jj = j0 - H + k;
-            if (act && jj >= 0 && jj < N) raw[k] = in.fetch(t, i, jj);
+            if (act && jj >= 0 && jj < N) raw[k] = in.template fetch<false, LT>(t, i, jj);
         }
 #pragma unroll
         for (int k = 0; k < W; ++k) {
             const int jj = j0 - H + k;
-            w[k] = (act && jj >= 0 && jj < N) ? in.make(t, i, jj, raw[k]) : T(0);
+            w[k] = (act && jj >= 0 && jj < N) ? in.template make<false, LT>(t, i, jj, raw[k]) : T(0);
         }
     }
     if (tid < 2 * H) {
@@ -527,11 +538,11 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
         typename IN::Raw raw[J];
 #pragma unroll
         for (int r = 0; r < J; ++r)
-            if (ii >= 0 && ii < N && j0 + r < N) raw[r] = in.fetch(t, ii, j0 + r);
+            if (ii >= 0 && ii < N && j0 + r < N) raw[r] = in.template fetch<false, LT>(t, ii, j0 + r);
 #pragma unroll
         for (int r = 0; r < J; ++r) {
             const int jj = j0 + r;
-            tile[r][col] = (ii >= 0 && ii < N && jj < N) ? in.make(t, ii, jj, raw[r]) : T(0);
+            tile[r][col] = (ii >= 0 && ii < N && jj < N) ? in.template make<false, LT>(t, ii, jj, raw[r]) : T(0);
         }
     }
     typename OUT::Aux aux[J];
@@ -545,7 +556,7 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
     T acc = T(0);
     if (act) {
         const bool iint = (i >= t.R) && (i < N - t.R);
-        const T hwi = hw_of<T>(t, i);
+        const T hwi = hw_of<T, LT>(t, i);
 #pragma unroll
         for (int r = 0; r < J; ++r) {
             const int j = j0 + r;
@@ -558,7 +569,7 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
 #pragma unroll
                 for (int s = 2; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
             } else {
-                kp = k_closure<T>(t, in, i, j, uc);
+                kp = k_closure<T, LT>(t, in, i, j, uc);
             }
             if (iint) {
                 const T* row = &tile[r][tid + H];
@@ -566,10 +577,10 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
 #pragma unroll
                 for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
             } else {
-                mp = m_closure<T>(t, in, i, j, uc);
+                mp = m_closure<T, LT>(t, in, i, j, uc);
             }
-            const T Au = (iint ? kp : hwi * kp) + (jint ? mp : hw_of<T>(t, j) * mp);
-            out.emit(t, i, j, gidx(i, j, N), uc, Au, aux[r], acc);
+            const T Au = (iint ? kp : hwi * kp) + (jint ? mp : hw_of<T, LT>(t, j) * mp);
+            out.template emit<false, LT>(t, i, j, gidx(i, j, N), uc, Au, aux[r], acc);
         }
     }
     return acc;
@@ -602,10 +613,8 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
         // latency -- on the small multigrid levels every tile is a face tile.
         __shared__ OpTab ltab;
         stage_table(t, &ltab);
-        OpTab tl = t;
-        tl.dev = &ltab;
-        if (jin) acc = pass_interior<P, TI, J, true, T>(in, out, tl, i0, j0, tile);
-        else acc = pass_general<P, TI, J, T>(in, out, tl, i0, j0, tile);
+        if (jin) acc = pass_interior<P, TI, J, true, T, true>(in, out, ltab, i0, j0, tile);
+        else acc = pass_general<P, TI, J, T, true>(in, out, ltab, i0, j0, tile);
     }
     if constexpr (OUT::kDot) {
         if (ctl.partial) {
@@ -620,29 +629,29 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
 // Residual + full-weighting restriction in one pass
 // ---------------------------------------------------------------------------
 // (A u)(i, j) at a single point, closure-aware, all loads through IN.
-template <int P, typename T, class IN>
+template <int P, typename T, bool LT = false, class IN>
 __device__ __forceinline__ T au_point(const OpTab& t, const IN& in, int i, int j) {
     constexpr int H = Stencil<P>::H;
     const int N = t.N;
-    const T uc = in.load(t, i, j);
+    const T uc = in.template load<false, LT>(t, i, j);
     const bool jint = (j >= t.R) && (j < N - t.R);
     const bool iint = (i >= t.R) && (i < N - t.R);
     T kp = T(0), mp = T(0);
     if (jint) {
 #pragma unroll
         for (int s = 1; s <= H; ++s)
-            kp += Stencil<P>::template c<T>(s) * ((in.load(t, i, j - s) - uc) + (in.load(t, i, j + s) - uc));
+            kp += Stencil<P>::template c<T>(s) * ((in.template load<false, LT>(t, i, j - s) - uc) + (in.template load<false, LT>(t, i, j + s) - uc));
     } else {
-        kp = k_closure<T>(t, in, i, j, uc);
+        kp = k_closure<T, LT>(t, in, i, j, uc);
     }
     if (iint) {
 #pragma unroll
         for (int s = 1; s <= H; ++s)
-            mp += Stencil<P>::template c<T>(s) * ((in.load(t, i - s, j) - uc) + (in.load(t, i + s, j) - uc));
+            mp += Stencil<P>::template c<T>(s) * ((in.template load<false, LT>(t, i - s, j) - uc) + (in.template load<false, LT>(t, i + s, j) - uc));
     } else {
-        mp = m_closure<T>(t, in, i, j, uc);
+        mp = m_closure<T, LT>(t, in, i, j, uc);
     }
-    return (iint ? kp : hw_of<T>(t, i) * kp) + (jint ? mp : hw_of<T>(t, j) * mp);
+    return (iint ? kp : hw_of<T, LT>(t, i) * kp) + (jint ? mp : hw_of<T, LT>(t, j) * mp);
 }
 
 struct RestrictCtl {
@@ -686,11 +695,8 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
     const bool jin = fj0 - H >= t.R && fj0 + JR + H <= N - t.R;
     const bool iin = fi0 - H >= t.R && fi0 + TI + H <= N - t.R;
     __shared__ OpTab ltab;
-    OpTab tl = t;
-    if (!(jin && iin)) {
-        stage_table(t, &ltab);   // per-thread closure lookups from LDS
-        tl.dev = &ltab;
-    }
+    if (!(jin && iin)) stage_table(t, &ltab);   // per-thread closure lookups from LDS
+    const OpTab& tl = ltab;
     if (jin) {
         // Rows interior: register window along j, LDS tile along i; with
         // columns at the faces (iin false) each thread tests its own column.
@@ -716,7 +722,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
         }
         __syncthreads();
         const bool iint = iin || ((fi >= t.R) && (fi < N - t.R));
-        const T hwi = iin ? T(1) : (act ? hw_of<T>(tl, fi) : T(0));
+        const T hwi = iin ? T(1) : (act ? hw_of<T, true>(tl, fi) : T(0));
 #pragma unroll
         for (int r = 0; r < JR; ++r) {
             const T uc = w[r + H];
@@ -730,7 +736,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
 #pragma unroll
                 for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
             } else {
-                mp = act ? m_closure<T>(tl, in, fi, fj0 + r, uc) : T(0);
+                mp = act ? m_closure<T, true>(tl, in, fi, fj0 + r, uc) : T(0);
             }
             res[r] = act ? bq[r] - (hwi * kp + mp) : T(0);
         }
@@ -739,7 +745,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
         for (int r = 0; r < JR; ++r) {
             const int fj = fj0 + r;
             T v = T(0);
-            if (fi >= 0 && fi < N && fj >= 0 && fj < N) v = b[gidx(fi, fj, N)] - au_point<P, T>(tl, in, fi, fj);
+            if (fi >= 0 && fi < N && fj >= 0 && fj < N) v = b[gidx(fi, fj, N)] - au_point<P, T, true>(tl, in, fi, fj);
             res[r] = v;
         }
     }

@@ -206,11 +206,14 @@ def test_smoother_matches_oracle(SP, p):
     assert relmax(x.cpu().numpy(), O.modified_richardson(A, x0, b, 3, 0.15)) < 1e-13
 
 
+@pytest.mark.parametrize("bottom", [0, 65])
 @pytest.mark.parametrize("p", [2, 4])
-def test_coarse_solve_and_vcycle(SP, p):
+def test_coarse_solve_and_vcycle(SP, p, bottom):
+    """65^2: with bottom_n = 65 the whole V-cycle is the single-workgroup
+    bottom kernel; with 0 every level runs the tiled passes."""
     level = 6
     N = 2 ** level + 1
-    h = SP(N, p)
+    h = SP(N, p, bottom_n=bottom)
     assert h.nlevels == 3
     Ac = A_of(4, p)
     bc = np.random.default_rng(8).standard_normal(17 * 17)
@@ -224,11 +227,12 @@ def test_coarse_solve_and_vcycle(SP, p):
     assert relmax(z32, O.vcycle(hier, r)) < 1e-4
 
 
+@pytest.mark.parametrize("bottom", [0, 65])
 @pytest.mark.parametrize("nu", [(1, 1), (3, 0), (0, 3), (2, 2)])
-def test_vcycle_sweep_counts(SP, nu):
+def test_vcycle_sweep_counts(SP, nu, bottom):
     level, p = 5, 4
     N = 2 ** level + 1
-    h = SP(N, p, nu_pre=nu[0], nu_post=nu[1])
+    h = SP(N, p, nu_pre=nu[0], nu_post=nu[1], bottom_n=bottom)
     hier = O.Hierarchy(level, p)
     r = np.random.default_rng(10).standard_normal(N * N)
     z = h.vcycle(T(r)).cpu().numpy()
@@ -259,16 +263,43 @@ def test_copy_kernel(SP):
     assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("bottom", [0, 33, 65])
 @pytest.mark.parametrize("p", [2, 4])
-def test_vcycle_large_against_oracle(SP, p):
+def test_vcycle_large_against_oracle(SP, p, bottom):
     """Level 9 (513^2): every V-cycle kernel runs its interior-tile path
-    (including the fused residual + restriction)."""
+    (including the fused residual + restriction); the levels at or below
+    bottom_n run in the single-workgroup bottom kernel."""
     level = 9
     N = 2 ** level + 1
-    h = SP(N, p)
+    h = SP(N, p, bottom_n=bottom)
     hier = O.Hierarchy(level, p)
     r = np.random.default_rng(21).standard_normal(N * N)
     z = h.vcycle(T(r)).cpu().numpy()
     assert relmax(z, O.vcycle(hier, r)) < 1e-12
     z32 = h.vcycle(T(r, torch.float32)).cpu().double().numpy()
     assert relmax(z32, O.vcycle(hier, r)) < 1e-4
+
+
+@pytest.mark.parametrize("level", [5, 7, 10])
+@pytest.mark.parametrize("p", [2, 4])
+def test_vcycle_fused_passes_match_separate_passes(SP, p, level):
+    """The temporally blocked two-sweep kernels (pre-smoothing + residual +
+    restriction; prolongation + post-smoothing) against the pass-per-sweep
+    V-cycle and the oracle: level 10 (1025^2) has interior blocks, shifted
+    last blocks and face blocks on every level down to 33^2."""
+    N = 2 ** level + 1
+    r = np.random.default_rng(30 + level).standard_normal(N * N)
+    hf = SP(N, p, fuse_n=1 << 20, bottom_n=0)
+    hs = SP(N, p, fuse_n=0, bottom_n=0)
+    zf = hf.vcycle(T(r)).cpu().numpy()
+    zs = hs.vcycle(T(r)).cpu().numpy()
+    assert relmax(zf, zs) < 1e-13
+    if level <= 7:
+        assert relmax(zf, O.vcycle(O.Hierarchy(level, p), r)) < 1e-12
+    zf32 = hf.vcycle(T(r, torch.float32)).cpu().double().numpy()
+    zs32 = hs.vcycle(T(r, torch.float32)).cpu().double().numpy()
+    assert relmax(zf32, zs32) < 1e-5
+    # Richardson smoother through the same kernels
+    hr = SP(N, p, fuse_n=1 << 20, smoother=1, richardson_omega=0.15, bottom_n=0)
+    hr0 = SP(N, p, fuse_n=0, smoother=1, richardson_omega=0.15, bottom_n=0)
+    assert relmax(hr.vcycle(T(r)).cpu().numpy(), hr0.vcycle(T(r)).cpu().numpy()) < 1e-13

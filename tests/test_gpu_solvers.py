@@ -197,3 +197,40 @@ def test_graph_replay_matches_eager(SP, p):
         assert (m1.iters, m1.inner_iters) == (m2.iters, m2.inner_iters) and torch.equal(mg, me)
         hg.apply(xg)                      # eager work between replays
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_mgpcg_bottom_kernel_matches_tiled_levels(SP, p):
+    """The single-workgroup bottom of the V-cycle and the all-tiled V-cycle
+    give the same MG-PCG iteration count and solution to rounding."""
+    level = 8
+    N = 2 ** level + 1
+    A, b, u, _ = problem(level, p, data="fourier")
+    tol = 1e-11 * np.linalg.norm(b)
+    h0 = SP(N, p, bottom_n=0)
+    h1 = SP(N, p)
+    x0, r0 = h0.mgpcg(T(b), abstol=tol)
+    x1, r1 = h1.mgpcg(T(b), abstol=tol)
+    assert r0.converged and r1.converged
+    assert r0.iters == r1.iters
+    assert torch.linalg.vector_norm(x0 - x1) <= 1e-11 * torch.linalg.vector_norm(x0)
+    xm, rm = h1.mgpcg_mixed(T(b), abstol=tol)
+    assert rm.converged
+    assert torch.linalg.vector_norm(xm - x0) <= 1e-9 * torch.linalg.vector_norm(x0)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_mgpcg_fused_passes_match_separate_passes(SP, p):
+    level = 9
+    N = 2 ** level + 1
+    A, b, u, _ = problem(level, p, data="fourier")
+    tol = 1e-11 * np.linalg.norm(b)
+    hf = SP(N, p, fuse_n=1 << 20)
+    hs = SP(N, p, fuse_n=0)
+    xf, rf = hf.mgpcg(T(b), abstol=tol)
+    xs, rs = hs.mgpcg(T(b), abstol=tol)
+    assert rf.converged and rs.converged and rf.iters == rs.iters
+    assert torch.linalg.vector_norm(xf - xs) <= 1e-11 * torch.linalg.vector_norm(xs)
+    xm, rm = hf.mgpcg_mixed(T(b), abstol=tol)
+    assert rm.converged
+    assert torch.linalg.vector_norm(xm - xs) <= 1e-9 * torch.linalg.vector_norm(xs)
