@@ -51,6 +51,8 @@ typedef enum {
 } sbp_status;
 
 typedef enum { SBP_F64 = 0, SBP_F32 = 1 } sbp_dtype;
+/* Or-ed into sbp_set_tiles' dtype: the shape of the fused two-sweep kernels. */
+#define SBP_FUSED 2
 
 typedef struct {
     int nu_pre;              /* pre-smoothing sweeps per level (default 2) */
@@ -73,11 +75,11 @@ typedef struct {
                                 (0 = residual test only, the default) */
     int bottom_n;            /* multigrid levels of side <= bottom_n (at most 65) run as one
                                 single-workgroup kernel per problem with the level data in
-                                LDS, when the coarsest side is <= 17 (default 33, 0 = off) */
-    int fuse_n;              /* with nu_pre = 2 / nu_post = 2, levels of side <= fuse_n run their
-                                pre-smoothing + residual + restriction, and their prolongation +
-                                post-smoothing, as one temporally blocked kernel each
-                                (default 1025, 0 = off) */
+                                LDS, when the coarsest side is <= 17 (default 0 = off) */
+    int fuse_n;              /* with nu_pre = 2 / nu_post = 2, levels whose points over all the
+                                handle's problems number at most fuse_n^2 run their pre-smoothing
+                                + residual + restriction, and their prolongation + post-smoothing,
+                                as one temporally blocked kernel each (default 1025, 0 = off) */
 } sbp_params;
 
 typedef struct sbp_handle_s* sbp_handle;
@@ -100,7 +102,9 @@ int sbp_destroy(sbp_handle h);
 int sbp_set_stream(sbp_handle h, void* hip_stream);
 int sbp_get_info(sbp_handle h, int* N, int* p, int* nlevels);
 /* Stencil tile shape (tile_i threads along i x tile_j rows per thread) used
- * on `level` for element type `dtype`; shapes not compiled in are rejected. */
+ * on `level` for element type `dtype`; shapes not compiled in are rejected.
+ * With SBP_FUSED or-ed into dtype: the block shape (tile_i output columns x
+ * tile_j fine rows) of the level's fused two-sweep kernels. */
 int sbp_set_tiles(sbp_handle h, int level, int dtype, int tile_i, int tile_j);
 
 /* Operator --------------------------------------------------------------- */

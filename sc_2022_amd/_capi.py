@@ -23,6 +23,7 @@ SBP_ERR_NO_DEVICE = 8
 
 SBP_F64 = 0
 SBP_F32 = 1
+SBP_FUSED = 2   # or-ed into sbp_set_tiles' dtype: shape of the fused two-sweep kernels
 
 # Every exported entry point: name -> (restype, argtypes)
 _vp = C.c_void_p

@@ -38,7 +38,7 @@ struct Level {
     OpTab tb{};
     OpTab* dtab = nullptr;                // device copy of tb (tb.dev points here)
     int ti[2] = {0, 0}, tj[2] = {0, 0};   // tile shape per precision (0 = f64, 1 = f32)
-    int fti[2] = {128, 128};              // width of the fused two-sweep blocks per precision
+    int fti[2] = {128, 128}, ftj[2] = {8, 8};   // fused two-sweep block shape per precision
     // per precision (0 = f64, 1 = f32), nprob problems each: rhs, two sweep
     // buffers, output, residual
     void* b[2] = {nullptr, nullptr};
@@ -204,6 +204,15 @@ int smooth_pass(sbp_handle h, int l, const IN& in, const T* b, T* dst, const T* 
     return launch_pass<T>(h, l, in, o, scb, nullptr, nullptr);
 }
 
+// Whether level l runs the fused two-sweep kernels: when the level's points
+// over all stacked problems are at most fuse_n^2 (launch-latency-bound
+// levels; above that the redundant halo work of the fused blocks costs more
+// than the passes it saves).
+inline bool fuse_level(const sbp_handle h, int l) {
+    const long long pts = (long long)h->L[l].n * h->nprob;
+    return pts <= (long long)h->prm.fuse_n * h->prm.fuse_n;
+}
+
 // Levels bottom_l0 .. nlev-1 of the V-cycle in one launch (one workgroup per problem).
 template <typename T>
 int launch_bottom(sbp_handle h, const T* b, T* out, const T* dotv, const Scalars* scb, double* partial) {
@@ -258,11 +267,11 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
     T* cur = xa;
     const int use_diag = h->prm.smoother == 1 ? 0 : 1;
     const int* done = scb ? &scb->done : nullptr;
-    const bool fused_pre = nu1 == 2 && L.N <= h->prm.fuse_n;
-    const bool fused_post = nu2 == 2 && L.N <= h->prm.fuse_n;
+    const bool fused_pre = nu1 == 2 && fuse_level(h, l);
+    const bool fused_post = nu2 == 2 && fuse_level(h, l);
     if (fused_pre) {
         // x2 = S(S(0)) and the restricted residual in one pass
-        const int rc = launch_smooth2_restrict<T>(h->p, L.fti[pi], L.tb, b, xa, (T*)h->L[l + 1].b[pi], om, use_diag,
+        const int rc = launch_smooth2_restrict<T>(h->p, L.fti[pi], L.ftj[pi], L.tb, b, xa, (T*)h->L[l + 1].b[pi], om, use_diag,
                                                   (T)h->prm.restrict_scale, done, kScalarInts, batch_of(h, l), st);
         if (rc != SBP_OK) return rc;
     } else if (nu1 == 0) {
@@ -288,7 +297,7 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
     STCHK(vcycle_level<T>(h, l + 1, (const T*)Lc.b[pi], ec, nullptr, scb, nullptr));
     // Prolongation folded into the first post-smoothing pass.
     if (fused_post)
-        return launch_prolong_smooth2<T>(h->p, L.fti[pi], L.tb, (const T*)cur, (const T*)ec, b, out, dotv, om, use_diag,
+        return launch_prolong_smooth2<T>(h->p, L.fti[pi], L.ftj[pi], L.tb, (const T*)cur, (const T*)ec, b, out, dotv, om, use_diag,
                                          done, kScalarInts, partial, batch_of(h, l), st);
     if (nu2 == 0) {
         hipLaunchKernelGGL((prolong_add<T>), dim3((L.N + 255) / 256, L.N, h->nprob), dim3(256), 0, st, (const T*)ec,
@@ -325,7 +334,8 @@ template <typename T>
 int vcycle_nparts(sbp_handle h) {
     const Level& L = h->L[0];
     if (h->nlev == 1 || h->bottom_l0 == 0) return 1;
-    if (h->prm.nu_post == 2 && L.N <= h->prm.fuse_n) return fused_post_blocks(L.N, L.fti[prec_index<T>()]);
+    if (h->prm.nu_post == 2 && fuse_level(h, 0))
+        return fused_post_blocks(L.N, L.fti[prec_index<T>()], L.ftj[prec_index<T>()]);
     if (h->prm.nu_post == 0) return ew_blocks(L.n);
     return (int)pass_blocks(L.N, L.ti[prec_index<T>()], L.tj[prec_index<T>()]);
 }
@@ -799,7 +809,7 @@ void sbp_default_params(sbp_params* prm) {
     prm->tile_j = 0;
     prm->use_graph = 1;
     prm->ir_rtol = 0.0;
-    prm->bottom_n = 33;
+    prm->bottom_n = 0;
     prm->fuse_n = 1025;
 }
 
@@ -891,6 +901,9 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
             } else {
                 default_tile(L.N, &L.ti[q], &L.tj[q]);
             }
+            // fused two-sweep blocks: small blocks keep the per-block chain short
+            L.fti[q] = 64;
+            L.ftj[q] = (long long)L.n * nprob >= 1025ll * 1025 ? 8 : 4;
         }
     }
     int dev = 0, khz = 0;
@@ -969,11 +982,17 @@ int sbp_get_batch(sbp_handle h, int* nprob) {
 }
 
 int sbp_set_tiles(sbp_handle h, int level, int dtype, int ti, int tj) {
-    if (!h || level < 0 || level >= h->nlev || !tile_ok(ti, tj) || (dtype != SBP_F64 && dtype != SBP_F32))
+    if (!h || level < 0 || level >= h->nlev) return SBP_ERR_INVALID_ARG;
+    const bool fused = (dtype & SBP_FUSED) != 0;
+    const int d = dtype & ~SBP_FUSED;
+    if ((d != SBP_F64 && d != SBP_F32) || (fused ? !fused_shape_ok(ti, tj) : !tile_ok(ti, tj)))
         return SBP_ERR_INVALID_ARG;
-    if (h->L[level].ti[dtype] != ti || h->L[level].tj[dtype] != tj) drop_graphs(h);   // launch shapes change
-    h->L[level].ti[dtype] = ti;
-    h->L[level].tj[dtype] = tj;
+    Level& L = h->L[level];
+    int* pti = fused ? &L.fti[d] : &L.ti[d];
+    int* ptj = fused ? &L.ftj[d] : &L.tj[d];
+    if (*pti != ti || *ptj != tj) drop_graphs(h);   // launch shapes change
+    *pti = ti;
+    *ptj = tj;
     return SBP_OK;
 }
 

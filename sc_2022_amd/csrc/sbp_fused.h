@@ -14,14 +14,14 @@
 //   prolong_smooth2   reads x2, e_c, b (and dotv); writes out (and dotv'out)
 //
 // Geometry.  A block's output region C is TI columns x JC rows; the first
-// sweep is evaluated on region B = C widened by H (the stencil radius) and
-// its operand on region A = C widened by 2H.  Blocks are placed on a regular
-// grid except that the last block in each direction is shifted back so its
-// region C ends at the grid's far face; together with C >= closure width this
-// guarantees that every closure row/column a face point reads lies inside
-// the block's regions.  Each block writes only the points (and coarse points)
-// it owns under the unshifted tiling, so shifted blocks overlap their
-// neighbours' work but never their writes.
+// sweep is evaluated on region B = C widened by HB and its operand on region
+// A = B widened by HA - HB (FusedHalo: R-1 each, R the closure rows, which
+// covers both the stencil radius and the closure rows that reach C columns
+// from a face).  Blocks are placed on a regular grid except that the last
+// block in each direction is shifted back so its region C ends at the grid's
+// far face; each block writes only the points (and coarse points) it owns
+// under the unshifted tiling, so shifted blocks overlap their neighbours'
+// work but never their writes.
 #pragma once
 #include "sbp_stencil.h"
 
@@ -61,10 +61,11 @@ __device__ __forceinline__ T au_interior(const T* a, int k) {
     return kp + mp;
 }
 
-// (A u)(i, j) for the operand in an LDS array covering columns [i0, i0 + W),
-// rows [j0, ...): the interior cross where it applies, else the closure-aware
-// general form (INNER: every point is interior, the general form is not
-// compiled).
+// (A u)(i, j) for the operand in an LDS tile covering columns [i0, i0 + W),
+// rows [j0, ...): the interior cross where it applies, else the
+// closure-aware general form (INNER: every point is interior, the general
+// form is not compiled).  The caller's regions contain every point either
+// form reads.
 template <int P, typename T, bool LT, bool INNER, int W>
 __device__ __forceinline__ T au_grid(const OpTab& t, const T* a, int i0, int j0, int i, int j) {
     const int k = (j - j0) * W + (i - i0);
@@ -72,6 +73,17 @@ __device__ __forceinline__ T au_grid(const OpTab& t, const T* a, int i0, int j0,
     const InGrid<T, W> in{a, i0, j0};
     return au_point<P, T, LT>(t, in, i, j);
 }
+
+// Halo widths of the fused blocks.  A closure row reads C columns from the
+// face, so a region whose points include closure rows must reach the face:
+// with R closure rows the first sweep's region B needs R-1 points of halo
+// around the output region C and the operand's region A another R-1 (both
+// >= the stencil radius H, which interior points need).
+template <int P> struct FusedHalo {
+    static constexpr int R = P == 2 ? 3 : 4;
+    static constexpr int HB = R - 1;
+    static constexpr int HA = 2 * (R - 1);
+};
 
 // Scale of the smoothing correction w (b - Au): w / diag (damped Jacobi; the
 // inverse diagonal is the constant idint away from the faces) or w (Richardson).
@@ -98,11 +110,11 @@ template <int P, int TI, int J, int NT, typename T, bool LT, bool INNER>
 __device__ __forceinline__ void sr_body(const T* __restrict__ b, T* __restrict__ x2g, T* __restrict__ rc,
                                         T omega, int use_diag, T scale, const OpTab& t, int bx, int by,
                                         int Nc, T* Bt, T* U, T* V) {
-    constexpr int H = Stencil<P>::H;
+    constexpr int HA = FusedHalo<P>::HA, HB = FusedHalo<P>::HB;
     constexpr int JR = J + 1;                     // fine residual rows (region C)
     constexpr int CI = TI / 2 - 1, CJ = J / 2;    // coarse points owned per block
-    constexpr int AW = TI + 4 * H, AH = JR + 4 * H;
-    constexpr int BW = TI + 2 * H, BH = JR + 2 * H;
+    constexpr int AW = TI + 2 * HA, AH = JR + 2 * HA;
+    constexpr int BW = TI + 2 * HB, BH = JR + 2 * HB;
     const int N = t.N;
     const int tid = threadIdx.x;
     int Ic0 = bx * CI < Nc - CI ? bx * CI : Nc - CI;
@@ -110,8 +122,8 @@ __device__ __forceinline__ void sr_body(const T* __restrict__ b, T* __restrict__
     if (Ic0 < 0) Ic0 = 0;
     if (Jc0 < 0) Jc0 = 0;
     const int ci0 = 2 * Ic0 - 1, cj0 = 2 * Jc0 - 1;   // region C origin
-    const int ai0 = ci0 - 2 * H, aj0 = cj0 - 2 * H;   // region A origin
-    const int bi0 = ci0 - H, bj0 = cj0 - H;           // region B origin
+    const int ai0 = ci0 - HA, aj0 = cj0 - HA;   // region A origin
+    const int bi0 = ci0 - HB, bj0 = cj0 - HB;   // region B origin
     // A: b and x1 = w D^-1 b (all loads issued before the first use)
     {
         constexpr int NA = (AW * AH + NT - 1) / NT;
@@ -142,7 +154,7 @@ __device__ __forceinline__ void sr_body(const T* __restrict__ b, T* __restrict__
         const int i = bi0 + c, j = bj0 + r;
         T x2 = T(0);
         if (INNER || (i >= 0 && i < N && j >= 0 && j < N)) {
-            const int ka = (r + H) * AW + (c + H);
+            const int ka = (r + HA - HB) * AW + (c + HA - HB);
             const T Au = au_grid<P, T, LT, INNER, AW>(t, U, ai0, aj0, i, j);
             x2 = U[ka] + smooth_corr<T, LT, INNER>(t, i, j, Bt[ka] - Au, omega, use_diag);
         }
@@ -168,7 +180,7 @@ __device__ __forceinline__ void sr_body(const T* __restrict__ b, T* __restrict__
         const int i = ci0 + c, j = cj0 + r;
         T rv = T(0);
         if (INNER || (i >= 0 && i < N && j >= 0 && j < N))
-            rv = Bt[(r + 2 * H) * AW + (c + 2 * H)] - au_grid<P, T, LT, INNER, BW>(t, V, bi0, bj0, i, j);
+            rv = Bt[(r + HA) * AW + (c + HA)] - au_grid<P, T, LT, INNER, BW>(t, V, bi0, bj0, i, j);
         R[k] = rv;
     }
     __syncthreads();
@@ -194,10 +206,10 @@ template <int P, int TI, int J, int NT, typename T>
 __global__ __launch_bounds__(NT) void smooth2_restrict(const T* __restrict__ b, T* __restrict__ x2g,
                                                       T* __restrict__ rc, T omega, int use_diag, T scale, OpTab t,
                                                       FusedCtl ctl) {
-    constexpr int H = Stencil<P>::H;
+    constexpr int HA = FusedHalo<P>::HA, HB = FusedHalo<P>::HB;
     constexpr int JR = J + 1;
-    constexpr int AW = TI + 4 * H, AH = JR + 4 * H;
-    constexpr int BW = TI + 2 * H, BH = JR + 2 * H;
+    constexpr int AW = TI + 2 * HA, AH = JR + 2 * HA;
+    constexpr int BW = TI + 2 * HB, BH = JR + 2 * HB;
     const int nb = ctl.nbx * ctl.nby;
     const int ball = xcd_remap((int)blockIdx.x, nb * ctl.bt.nprob);
     const int q = ball / nb, blk = ball - q * nb;
@@ -216,7 +228,7 @@ __global__ __launch_bounds__(NT) void smooth2_restrict(const T* __restrict__ b, 
     int Jc0 = by * CJ < Nc - CJ ? by * CJ : Nc - CJ;
     if (Ic0 < 0) Ic0 = 0;
     if (Jc0 < 0) Jc0 = 0;
-    const int ai0 = 2 * Ic0 - 1 - 2 * H, aj0 = 2 * Jc0 - 1 - 2 * H;
+    const int ai0 = 2 * Ic0 - 1 - HA, aj0 = 2 * Jc0 - 1 - HA;
     const bool inner = ai0 >= t.R && ai0 + AW <= t.N - t.R && aj0 >= t.R && aj0 + AH <= t.N - t.R;
     if (inner) {
         sr_body<P, TI, J, NT, T, false, true>(b, x2g, rc, omega, use_diag, scale, t, bx, by, Nc, Bt, U, V);
@@ -234,9 +246,9 @@ template <int P, int TI, int J, int NT, typename T, bool LT, bool INNER>
 __device__ __forceinline__ T ps_body(const T* __restrict__ x, const T* __restrict__ ec, const T* __restrict__ b,
                                      T* __restrict__ out, const T* __restrict__ dotv, T omega, int use_diag,
                                      const OpTab& t, int bx, int by, T* U, T* E, T* Bt, T* V) {
-    constexpr int H = Stencil<P>::H;
-    constexpr int AW = TI + 4 * H, AH = J + 4 * H;
-    constexpr int BW = TI + 2 * H, BH = J + 2 * H;
+    constexpr int HA = FusedHalo<P>::HA, HB = FusedHalo<P>::HB;
+    constexpr int AW = TI + 2 * HA, AH = J + 2 * HA;
+    constexpr int BW = TI + 2 * HB, BH = J + 2 * HB;
     constexpr int EW = AW / 2 + 2, EH = AH / 2 + 2;
     const int N = t.N, Nc = (N + 1) / 2;
     const int tid = threadIdx.x;
@@ -244,8 +256,8 @@ __device__ __forceinline__ T ps_body(const T* __restrict__ x, const T* __restric
     int j0 = by * J < N - J ? by * J : N - J;
     if (i0 < 0) i0 = 0;
     if (j0 < 0) j0 = 0;
-    const int ai0 = i0 - 2 * H, aj0 = j0 - 2 * H;
-    const int bi0 = i0 - H, bj0 = j0 - H;
+    const int ai0 = i0 - HA, aj0 = j0 - HA;
+    const int bi0 = i0 - HB, bj0 = j0 - HB;
     const int ei0 = ai0 >> 1, ej0 = aj0 >> 1;   // coarse patch origin (floor)
     // Coarse patch, x on A and b on B: every load issued before the first use.
     {
@@ -315,7 +327,7 @@ __device__ __forceinline__ T ps_body(const T* __restrict__ x, const T* __restric
         T v1 = T(0);
         if (INNER || (i >= 0 && i < N && j >= 0 && j < N)) {
             const T Au = au_grid<P, T, LT, INNER, AW>(t, U, ai0, aj0, i, j);
-            v1 = U[(r + H) * AW + (c + H)] + smooth_corr<T, LT, INNER>(t, i, j, Bt[k] - Au, omega, use_diag);
+            v1 = U[(r + HA - HB) * AW + (c + HA - HB)] + smooth_corr<T, LT, INNER>(t, i, j, Bt[k] - Au, omega, use_diag);
         }
         V[k] = v1;
     }
@@ -345,9 +357,9 @@ __global__ __launch_bounds__(NT) void prolong_smooth2(const T* __restrict__ x, c
                                                      const T* __restrict__ b, T* __restrict__ out,
                                                      const T* __restrict__ dotv, T omega, int use_diag, OpTab t,
                                                      FusedCtl ctl) {
-    constexpr int H = Stencil<P>::H;
-    constexpr int AW = TI + 4 * H, AH = J + 4 * H;
-    constexpr int BW = TI + 2 * H, BH = J + 2 * H;
+    constexpr int HA = FusedHalo<P>::HA, HB = FusedHalo<P>::HB;
+    constexpr int AW = TI + 2 * HA, AH = J + 2 * HA;
+    constexpr int BW = TI + 2 * HB, BH = J + 2 * HB;
     constexpr int EW = AW / 2 + 2, EH = AH / 2 + 2;
     const int nb = ctl.nbx * ctl.nby;
     const int ball = xcd_remap((int)blockIdx.x, nb * ctl.bt.nprob);
@@ -368,7 +380,7 @@ __global__ __launch_bounds__(NT) void prolong_smooth2(const T* __restrict__ x, c
     int j0 = by * J < N - J ? by * J : N - J;
     if (i0 < 0) i0 = 0;
     if (j0 < 0) j0 = 0;
-    const int ai0 = i0 - 2 * H, aj0 = j0 - 2 * H;
+    const int ai0 = i0 - HA, aj0 = j0 - HA;
     const bool inner = ai0 >= t.R && ai0 + AW <= N - t.R && aj0 >= t.R && aj0 + AH <= N - t.R;
     T acc;
     if (inner) {

@@ -8,75 +8,88 @@
 namespace sbp {
 
 namespace {
-constexpr int kFusedJ = 8;      // fine rows of a block's output region
 constexpr int kFusedNT = 256;   // threads per block
 
-template <int P, int TI, typename T>
+template <int P, int TI, int J, typename T>
 void sr_launch(const OpTab& tf, const T* b, T* x2, T* rc, T omega, int use_diag, T scale, FusedCtl c,
                hipStream_t st) {
-    hipLaunchKernelGGL((smooth2_restrict<P, TI, kFusedJ, kFusedNT, T>), dim3(c.nbx * c.nby * c.bt.nprob),
-                       dim3(kFusedNT), 0, st, b, x2, rc, omega, use_diag, scale, tf, c);
+    hipLaunchKernelGGL((smooth2_restrict<P, TI, J, kFusedNT, T>), dim3(c.nbx * c.nby * c.bt.nprob), dim3(kFusedNT),
+                       0, st, b, x2, rc, omega, use_diag, scale, tf, c);
 }
 
-template <int P, int TI, typename T>
+template <int P, int TI, int J, typename T>
 void ps_launch(const OpTab& tf, const T* x, const T* ec, const T* b, T* out, const T* dotv, T omega, int use_diag,
                FusedCtl c, hipStream_t st) {
-    hipLaunchKernelGGL((prolong_smooth2<P, TI, kFusedJ, kFusedNT, T>), dim3(c.nbx * c.nby * c.bt.nprob),
-                       dim3(kFusedNT), 0, st, x, ec, b, out, dotv, omega, use_diag, tf, c);
+    hipLaunchKernelGGL((prolong_smooth2<P, TI, J, kFusedNT, T>), dim3(c.nbx * c.nby * c.bt.nprob), dim3(kFusedNT), 0,
+                       st, x, ec, b, out, dotv, omega, use_diag, tf, c);
 }
 }  // namespace
 
-int fused_post_blocks(int N, int ti) { return ((N + ti - 1) / ti) * ((N + kFusedJ - 1) / kFusedJ); }
+// Compiled block shapes (output columns x fine rows).
+#define SBP_FUSED_SHAPES(X) X(64, 4) X(64, 8) X(128, 8) X(256, 8)
 
-bool fused_width_ok(int ti) { return ti == 128 || ti == 256; }
+int fused_post_blocks(int N, int ti, int tj) { return ((N + ti - 1) / ti) * ((N + tj - 1) / tj); }
+
+bool fused_shape_ok(int ti, int tj) {
+#define SBP_FUSED_OK(TI_, J_) if (ti == TI_ && tj == J_) return true;
+    SBP_FUSED_SHAPES(SBP_FUSED_OK)
+#undef SBP_FUSED_OK
+    return false;
+}
 
 template <typename T>
-int launch_smooth2_restrict(int p, int ti, const OpTab& tf, const T* b, T* x2, T* rc, T omega, int use_diag,
-                            T scale, const int* done, int ds, const Batch& bt, hipStream_t st) {
+int launch_smooth2_restrict(int p, int ti, int tj, const OpTab& tf, const T* b, T* x2, T* rc, T omega,
+                            int use_diag, T scale, const int* done, int ds, const Batch& bt, hipStream_t st) {
     const int Nc = (tf.N + 1) / 2;
     FusedCtl c{};
     c.done = done;
     c.ds = ds;
     c.nbx = (Nc + ti / 2 - 2) / (ti / 2 - 1);
-    c.nby = (Nc + kFusedJ / 2 - 1) / (kFusedJ / 2);
+    c.nby = (Nc + tj / 2 - 1) / (tj / 2);
     c.partial = nullptr;
     c.bt = bt;
-    if (p == 2 && ti == 128) sr_launch<2, 128, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);
-    else if (p == 2 && ti == 256) sr_launch<2, 256, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);
-    else if (p == 4 && ti == 128) sr_launch<4, 128, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);
-    else if (p == 4 && ti == 256) sr_launch<4, 256, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);
-    else return 3;
-    return hipGetLastError() == hipSuccess ? 0 : 4;
+#define SBP_SR_CASE(TI_, J_)                                                                  \
+    if (ti == TI_ && tj == J_) {                                                              \
+        if (p == 2) sr_launch<2, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);  \
+        else sr_launch<4, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);         \
+        return hipGetLastError() == hipSuccess ? 0 : 4;                                       \
+    }
+    SBP_FUSED_SHAPES(SBP_SR_CASE)
+#undef SBP_SR_CASE
+    return 3;
 }
 
 template <typename T>
-int launch_prolong_smooth2(int p, int ti, const OpTab& tf, const T* x, const T* ec, const T* b, T* out,
+int launch_prolong_smooth2(int p, int ti, int tj, const OpTab& tf, const T* x, const T* ec, const T* b, T* out,
                            const T* dotv, T omega, int use_diag, const int* done, int ds, double* partial,
                            const Batch& bt, hipStream_t st) {
     FusedCtl c{};
     c.done = done;
     c.ds = ds;
     c.nbx = (tf.N + ti - 1) / ti;
-    c.nby = (tf.N + kFusedJ - 1) / kFusedJ;
+    c.nby = (tf.N + tj - 1) / tj;
     c.partial = partial;
     c.bt = bt;
-    if (p == 2 && ti == 128) ps_launch<2, 128, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st);
-    else if (p == 2 && ti == 256) ps_launch<2, 256, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st);
-    else if (p == 4 && ti == 128) ps_launch<4, 128, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st);
-    else if (p == 4 && ti == 256) ps_launch<4, 256, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st);
-    else return 3;
-    return hipGetLastError() == hipSuccess ? 0 : 4;
+#define SBP_PS_CASE(TI_, J_)                                                                   \
+    if (ti == TI_ && tj == J_) {                                                               \
+        if (p == 2) ps_launch<2, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st); \
+        else ps_launch<4, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st);        \
+        return hipGetLastError() == hipSuccess ? 0 : 4;                                        \
+    }
+    SBP_FUSED_SHAPES(SBP_PS_CASE)
+#undef SBP_PS_CASE
+    return 3;
 }
 
-template int launch_smooth2_restrict<double>(int, int, const OpTab&, const double*, double*, double*, double, int,
-                                             double, const int*, int, const Batch&, hipStream_t);
-template int launch_smooth2_restrict<float>(int, int, const OpTab&, const float*, float*, float*, float, int, float,
-                                            const int*, int, const Batch&, hipStream_t);
-template int launch_prolong_smooth2<double>(int, int, const OpTab&, const double*, const double*, const double*,
-                                            double*, const double*, double, int, const int*, int, double*,
-                                            const Batch&, hipStream_t);
-template int launch_prolong_smooth2<float>(int, int, const OpTab&, const float*, const float*, const float*, float*,
-                                           const float*, float, int, const int*, int, double*, const Batch&,
+template int launch_smooth2_restrict<double>(int, int, int, const OpTab&, const double*, double*, double*, double,
+                                             int, double, const int*, int, const Batch&, hipStream_t);
+template int launch_smooth2_restrict<float>(int, int, int, const OpTab&, const float*, float*, float*, float, int,
+                                            float, const int*, int, const Batch&, hipStream_t);
+template int launch_prolong_smooth2<double>(int, int, int, const OpTab&, const double*, const double*,
+                                            const double*, double*, const double*, double, int, const int*, int,
+                                            double*, const Batch&, hipStream_t);
+template int launch_prolong_smooth2<float>(int, int, int, const OpTab&, const float*, const float*, const float*,
+                                           float*, const float*, float, int, const int*, int, double*, const Batch&,
                                            hipStream_t);
 
 }  // namespace sbp

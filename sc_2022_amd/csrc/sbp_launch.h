@@ -55,17 +55,17 @@ int launch_resid_restrict(int p, const OpTab& tf, const T* u, const T* b, T* rc,
 template <int TI, int J, typename T, class IN, class OUT>
 int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, PassCtl c, hipStream_t st);
 
-// Temporally blocked V-cycle passes (two sweeps per kernel), block width ti
-// in {128, 256} (fused_width_ok).
-int fused_post_blocks(int N, int ti);
-bool fused_width_ok(int ti);
+// Temporally blocked V-cycle passes (two sweeps per kernel); block shape ti
+// output columns x tj fine rows, one of the compiled shapes (fused_shape_ok).
+int fused_post_blocks(int N, int ti, int tj);
+bool fused_shape_ok(int ti, int tj);
 // x2 = S(S(0)) on the fine level and rc = scale * FW(b - A x2).
 template <typename T>
-int launch_smooth2_restrict(int p, int ti, const OpTab& tf, const T* b, T* x2, T* rc, T omega, int use_diag,
-                            T scale, const int* done, int ds, const Batch& bt, hipStream_t st);
+int launch_smooth2_restrict(int p, int ti, int tj, const OpTab& tf, const T* b, T* x2, T* rc, T omega,
+                            int use_diag, T scale, const int* done, int ds, const Batch& bt, hipStream_t st);
 // out = S(S(x + P ec)); with dotv, per-block partials of dotv' out (fused_post_blocks per problem).
 template <typename T>
-int launch_prolong_smooth2(int p, int ti, const OpTab& tf, const T* x, const T* ec, const T* b, T* out,
+int launch_prolong_smooth2(int p, int ti, int tj, const OpTab& tf, const T* x, const T* ec, const T* b, T* out,
                            const T* dotv, T omega, int use_diag, const int* done, int ds, double* partial,
                            const Batch& bt, hipStream_t st);
 

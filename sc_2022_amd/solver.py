@@ -88,11 +88,13 @@ class SbpPoisson:
             N = (N + 1) // 2
         return N
 
-    def set_tiles(self, level, ti, tj, dtype=None):
-        """Tile shape for `level`; both precisions unless dtype is given."""
+    def set_tiles(self, level, ti, tj, dtype=None, fused=False):
+        """Tile shape for `level` (fused: the block shape of the level's
+        two-sweep kernels); both precisions unless dtype is given."""
         dts = [capi.SBP_F64, capi.SBP_F32] if dtype is None else [_DT[dtype]]
         for d in dts:
-            capi.check(self.lib.sbp_set_tiles(self._h, level, d, ti, tj), "sbp_set_tiles")
+            capi.check(self.lib.sbp_set_tiles(self._h, level, d | (capi.SBP_FUSED if fused else 0), ti, tj),
+                       "sbp_set_tiles")
 
     def apply_tuned_tiles(self, cache=None):
         """Applies the autotuner's choices (tiles_gfx950.json) to every level."""

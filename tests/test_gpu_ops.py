@@ -280,16 +280,21 @@ def test_vcycle_large_against_oracle(SP, p, bottom):
     assert relmax(z32, O.vcycle(hier, r)) < 1e-4
 
 
+@pytest.mark.parametrize("shape", [(64, 4), (64, 8), (128, 8), (256, 8)])
 @pytest.mark.parametrize("level", [5, 7, 10])
 @pytest.mark.parametrize("p", [2, 4])
-def test_vcycle_fused_passes_match_separate_passes(SP, p, level):
+def test_vcycle_fused_passes_match_separate_passes(SP, p, level, shape):
     """The temporally blocked two-sweep kernels (pre-smoothing + residual +
     restriction; prolongation + post-smoothing) against the pass-per-sweep
-    V-cycle and the oracle: level 10 (1025^2) has interior blocks, shifted
-    last blocks and face blocks on every level down to 33^2."""
+    V-cycle and the oracle, for every compiled block shape: level 10 (1025^2)
+    has interior blocks, shifted last blocks and face blocks on every level
+    down to 33^2; with 4-row blocks the second block row holds closure rows
+    whose reads leave the block's tiles (recomputed from global memory)."""
     N = 2 ** level + 1
     r = np.random.default_rng(30 + level).standard_normal(N * N)
     hf = SP(N, p, fuse_n=1 << 20, bottom_n=0)
+    for lv in range(hf.nlevels - 1):
+        hf.set_tiles(lv, shape[0], shape[1], fused=True)
     hs = SP(N, p, fuse_n=0, bottom_n=0)
     zf = hf.vcycle(T(r)).cpu().numpy()
     zs = hs.vcycle(T(r)).cpu().numpy()
@@ -301,5 +306,7 @@ def test_vcycle_fused_passes_match_separate_passes(SP, p, level):
     assert relmax(zf32, zs32) < 1e-5
     # Richardson smoother through the same kernels
     hr = SP(N, p, fuse_n=1 << 20, smoother=1, richardson_omega=0.15, bottom_n=0)
+    for lv in range(hr.nlevels - 1):
+        hr.set_tiles(lv, shape[0], shape[1], fused=True)
     hr0 = SP(N, p, fuse_n=0, smoother=1, richardson_omega=0.15, bottom_n=0)
     assert relmax(hr.vcycle(T(r)).cpu().numpy(), hr0.vcycle(T(r)).cpu().numpy()) < 1e-13
