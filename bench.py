@@ -40,17 +40,24 @@ def parse():
     ap.add_argument("--n", type=int, default=4097)
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--seed", type=int, default=2022)
-    ap.add_argument("--rtol", type=float, default=1e-12, help="stopping rule ||r|| < rtol ||b||")
-    ap.add_argument("--inner-rtol", type=float, default=1e-6,
+    ap.add_argument("--rtol", type=float, default=1e-11,
+                    help="FP64 leg: stop when ||r|| < rtol ||b||")
+    ap.add_argument("--mixed-rtol", type=float, default=1e-10,
+                    help="mixed leg: stop when the FP64 ||r|| < mixed_rtol ||b||")
+    ap.add_argument("--inner-rtol", type=float, default=1e-5,
                     help="mixed: relative tolerance of each FP32 inner solve")
-    ap.add_argument("--ir-rtol", type=float, default=1e-11,
-                    help="mixed: stop when the correction-based error estimate <= ir_rtol ||x||")
+    ap.add_argument("--ir-rtol", type=float, default=0.0,
+                    help="mixed: also stop when the correction-based error estimate <= ir_rtol ||x||")
+    ap.add_argument("--tol-sweep", action="store_true",
+                    help="print iterations, error and time per stopping rule for both legs, then exit")
     ap.add_argument("--apply-reps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--only-apply", action="store_true", help="time the apply only (profiling runs)")
     ap.add_argument("--batch", type=int, default=256, help="problems in the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--batch-rtol", type=float, default=1e-11,
+                    help="batched leg: stop each problem when its FP64 ||r|| < batch_rtol ||b||")
     return ap.parse_args()
 
 
@@ -115,7 +122,7 @@ def batch_leg(a, dev, torch):
     for q in range(len(specs)):
         bn = float(torch.linalg.vector_norm(ub.rhs(q)))
         tol_ref.append(1e-14 * bn)
-        tol.append(a.rtol * bn)
+        tol.append(a.batch_rtol * bn)
     ub.solve(tol_ref, mixed=False, maxiter=200)
     xref = [ub.solution(q).clone() for q in range(len(specs))]
     for _ in range(max(1, a.warmup)):
@@ -140,6 +147,40 @@ def batch_leg(a, dev, torch):
                      for N, p in sorted({(n, pp) for n, pp, _ in specs})}}
     ub.close()
     return res
+
+
+def tol_sweep(a, h, b, x, bnorm, rel_err, torch):
+    """Iterations, relative H-norm error and time of both legs over a grid of
+    stopping rules (one JSON line each), to pick the loosest rule per leg
+    that meets the error target."""
+    from sc_2022_amd.solver import SbpPoisson
+
+    def timeit(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, r
+
+    for rt in (1e-10, 3e-11, 1e-11, 3e-12, 1e-12):
+        def f():
+            x.zero_()
+            return h.mgpcg(b, x, abstol=rt * bnorm, maxiter=200, raise_on_fail=False)[1]
+        dt, r = timeit(f)
+        print(json.dumps({"leg": "f64", "rtol": rt, "iters": r.iters, "rel_err": rel_err(x), "ms": dt * 1e3}),
+              flush=True)
+    for irt in (1e-5, 3e-6, 1e-6):
+        for rt in (1e-10, 1e-11, 1e-12):
+            hm = SbpPoisson(h.N, h.p, device=h.device, inner_rtol=irt, ir_rtol=a.ir_rtol)
+            def f():
+                x.zero_()
+                return hm.mgpcg_mixed(b, x, abstol=rt * bnorm, raise_on_fail=False)[1]
+            dt, r = timeit(f)
+            print(json.dumps({"leg": "mixed", "inner_rtol": irt, "rtol": rt, "outer": r.iters,
+                              "inner": r.inner_iters, "rel_err": rel_err(x), "ms": dt * 1e3}), flush=True)
+            hm.close()
 
 
 def cpu_baseline(N, p, seed, threads, rtol):
@@ -204,16 +245,18 @@ def main():
     def rel_err(x):
         return h.hnorm(x - xref) / href
 
-    abstol = a.rtol * bnorm
     x = torch.zeros_like(b)
+    if a.tol_sweep:
+        tol_sweep(a, h, b, x, bnorm, rel_err, torch)
+        return
 
     def solve_mixed():
         x.zero_()
-        return h.mgpcg_mixed(b, x, abstol=abstol)[1]
+        return h.mgpcg_mixed(b, x, abstol=a.mixed_rtol * bnorm)[1]
 
     def solve_f64():
         x.zero_()
-        return h.mgpcg(b, x, abstol=abstol, maxiter=200)[1]
+        return h.mgpcg(b, x, abstol=a.rtol * bnorm, maxiter=200)[1]
 
     # ---- FP64 MG-PCG (reported beside the headline) ----
     for _ in range(max(1, a.warmup)):
@@ -264,13 +307,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 inner / f64 outer",
         "data": "synthetic: seeded Fourier-mode source (64 modes) and boundary data (8 modes/face)",
-        "config": {"workload": f"MG-PCG solve, N={N} (n={n}), SBP p={p}, V-cycle 2+2 damped Jacobi to 17^2, "
-                               f"stop ||r|| < {a.rtol:g} ||b|| (mixed also: error estimate < {a.ir_rtol:g} ||x||, "
-                               f"inner FP32 solves to {a.inner_rtol:g})",
+        "config": {"workload": f"MG-PCG solve, N={N} (n={n}), SBP p={p}, V-cycle 2+2 damped Jacobi to 17^2; "
+                               f"each leg stops at the loosest rule meeting rel err <= {REL_ERR_TARGET:g} "
+                               f"(bench.py --tol-sweep): FP64 ||r|| < {a.rtol:g} ||b||, mixed FP64 ||r|| < "
+                               f"{a.mixed_rtol:g} ||b|| with FP32 inner solves to {a.inner_rtol:g}"
+                               + (f" and error estimate < {a.ir_rtol:g} ||x||" if a.ir_rtol > 0 else ""),
                    "seed": a.seed},
         "rel_err_mixed": errmx,
         "rel_err_f64": err64,
-        "meets_rel_err_target": bool(errmx <= REL_ERR_TARGET),
+        "meets_rel_err_target": bool(errmx <= REL_ERR_TARGET and err64 <= REL_ERR_TARGET),
         "mixed": {"solves_per_s": 1.0 / tmx, "outer": rmx.iters, "inner": rmx.inner_iters,
                   "ms": tmx * 1e3},
         "f64": {"solves_per_s": 1.0 / t64, "iters": r64.iters, "ms": t64 * 1e3},
