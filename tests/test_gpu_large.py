@@ -1,0 +1,86 @@
+"""GPU solves at the benchmark sizes, checked through size-independent
+properties: residuals recomputed by the oracle's C matrix-free apply on the
+host, manufactured-solution convergence rates (SBP p = 2: 2, p = 4: ~4), and
+the mixed-precision solve at 8193^2 against a tightly converged FP64 solve."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import c_oracle
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _oracle_lib():
+    c_oracle.lib()           # builds oracle/liboracle.so only if it is missing
+    c_oracle.set_threads(16)
+
+
+def manufactured_error(N, p, rtol):
+    """H-norm error of the FP64 MG-PCG solution of the manufactured problem
+    u = sin(pi x) sin(pi y) + x y, and the oracle-recomputed residual."""
+    from sc_2022_amd.data import manufactured_fields
+    from sc_2022_amd.solver import SbpPoisson
+    h = SbpPoisson(N, p, device=DEV)
+    f, gW, gE, gS, gN, u = manufactured_fields(N, DEV)
+    b = h.assemble_rhs(f, gW, gE, gS, gN)
+    bn = float(torch.linalg.vector_norm(b))
+    x, res = h.mgpcg(b, abstol=rtol * bn, maxiter=100)
+    assert res.converged, res
+    r = b.cpu().numpy() - c_oracle.apply(p, N, x.cpu().numpy())
+    err = h.hnorm(x - u)
+    h.close()
+    return err, float(np.linalg.norm(r)) / bn, res.iters
+
+
+@pytest.mark.parametrize("p,levels,rate", [(2, (10, 11, 12), 2.0), (4, (7, 8, 9), 4.0)])
+def test_manufactured_convergence_rate(p, levels, rate):
+    errs = []
+    for lv in levels:
+        err, rres, it = manufactured_error(2 ** lv + 1, p, 1e-13)
+        assert rres < 1.01e-13, (lv, rres)      # the oracle agrees the solve met its tolerance
+        assert it <= 20, (lv, it)               # grid-independent multigrid convergence
+        errs.append(err)
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(len(errs) - 1)]
+    tol = 0.06 if p == 2 else 0.35
+    assert all(abs(r - rate) < tol for r in rates), (errs, rates)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_mgpcg_4097_residual_by_oracle(p):
+    """configs[1] at its largest size: FP64 MG-PCG on the Fourier-mode problem
+    stops at ||r|| < 1e-12 ||b||; the host oracle's apply confirms it."""
+    from sc_2022_amd.data import fourier_rhs
+    from sc_2022_amd.solver import SbpPoisson
+    N = 4097
+    h = SbpPoisson(N, p, device=DEV)
+    b = fourier_rhs(h, 2022)
+    bn = float(torch.linalg.vector_norm(b))
+    x, res = h.mgpcg(b, abstol=1e-12 * bn, maxiter=100)
+    assert res.converged and res.iters <= 20, res
+    r = b.cpu().numpy() - c_oracle.apply(p, N, x.cpu().numpy())
+    assert np.linalg.norm(r) < 1.01e-12 * bn
+
+
+def test_mixed_precision_8193():
+    """FP32 inner MG-PCG inside FP64 refinement at 8193^2, p = 4,
+    to rel H-norm error <= 1e-10 against the FP64 solve converged to
+    ||r|| < 1e-14 ||b||; the host oracle recomputes the mixed solution's residual."""
+    from sc_2022_amd.data import fourier_rhs
+    from sc_2022_amd.solver import SbpPoisson
+    N, p = 8193, 4
+    h = SbpPoisson(N, p, device=DEV, inner_rtol=1e-5)
+    b = fourier_rhs(h, 2022)
+    bn = float(torch.linalg.vector_norm(b))
+    xref, r64 = h.mgpcg(b, abstol=1e-14 * bn, maxiter=200, raise_on_fail=False)
+    assert r64.resnorm < 1e-13 * bn
+    x, res = h.mgpcg_mixed(b, abstol=1e-10 * bn)
+    assert res.converged and res.iters <= 4, res
+    rel = h.hnorm(x - xref) / h.hnorm(xref)
+    assert rel <= 1e-10, rel
+    r = b.cpu().numpy() - c_oracle.apply(p, N, x.cpu().numpy())
+    assert np.linalg.norm(r) < 1.01e-10 * bn
