@@ -48,6 +48,7 @@ def parse():
                     help="mixed: relative tolerance of each FP32 inner solve")
     ap.add_argument("--ir-rtol", type=float, default=0.0,
                     help="mixed: also stop when the correction-based error estimate <= ir_rtol ||x||")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 8193^2 and CSR comparison legs")
     ap.add_argument("--tol-sweep", action="store_true",
                     help="print iterations, error and time per stopping rule for both legs, then exit")
     ap.add_argument("--apply-reps", type=int, default=200)
@@ -56,7 +57,7 @@ def parse():
     ap.add_argument("--only-apply", action="store_true", help="time the apply only (profiling runs)")
     ap.add_argument("--batch", type=int, default=256, help="problems in the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--batch-rtol", type=float, default=1e-11,
+    ap.add_argument("--batch-rtol", type=float, default=1e-12,
                     help="batched leg: stop each problem when its FP64 ||r|| < batch_rtol ||b||")
     return ap.parse_args()
 
@@ -147,6 +148,66 @@ def batch_leg(a, dev, torch):
                      for N, p in sorted({(n, pp) for n, pp, _ in specs})}}
     ub.close()
     return res
+
+
+def csr_leg(h, torch, apply_us):
+    """The assembled CSR operator (the comparison path of the reference's
+    CG_GPU_sparse) through the hand-written CSR SpMV, against the matrix-free
+    apply: time per product and algorithmic bytes (values 8 B and column
+    indices 4 B per nonzero, row pointers, x read once, y written)."""
+    rp, col, val = h.csr_assemble()
+    n, nnz = h.n, col.numel()
+    x = torch.randn(n, dtype=torch.float64, device=h.device)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        h.csr_spmv((rp, col, val), x, y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 50
+    e0.record(torch.cuda.current_stream())
+    for _ in range(reps):
+        h.csr_spmv((rp, col, val), x, y)
+    e1.record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / reps * 1e3
+    nbytes = 12 * nnz + 4 * (n + 1) + 16 * n
+    res = {"nnz": nnz, "spmv_us": us, "spmv_GBs": nbytes / (us * 1e-6) / 1e9, "bytes": nbytes,
+           "matrix_free_apply_us": apply_us, "matrix_free_speedup": us / apply_us}
+    del rp, col, val
+    return res
+
+
+def leg_8193(a, dev, torch):
+    """Mixed precision and FP64 MG-PCG at 8193^2 (configs[2] size) with the
+    headline stopping rules; errors against the FP64 solve to 1e-14."""
+    from sc_2022_amd.data import fourier_rhs
+    from sc_2022_amd.solver import SbpPoisson
+    N = 8193
+    h = SbpPoisson(N, a.p, device=dev, ir_rtol=a.ir_rtol, inner_rtol=a.inner_rtol)
+    b = fourier_rhs(h, a.seed)
+    bn = float(torch.linalg.vector_norm(b))
+    xref, _ = h.mgpcg(b, abstol=1e-14 * bn, maxiter=200, raise_on_fail=False)
+    href = h.hnorm(xref)
+    x = torch.zeros_like(b)
+    out = {}
+    for leg in ("mixed", "f64"):
+        def run():
+            x.zero_()
+            if leg == "mixed":
+                return h.mgpcg_mixed(b, x, abstol=a.mixed_rtol * bn)[1]
+            return h.mgpcg(b, x, abstol=a.rtol * bn, maxiter=200)[1]
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            r = run()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 3
+        out[leg] = {"ms": dt * 1e3, "solves_per_s": 1.0 / dt, "rel_err": h.hnorm(x - xref) / href,
+                    "iters": r.iters, "inner": r.inner_iters}
+    out["speedup_mixed_vs_f64"] = out["f64"]["ms"] / out["mixed"]["ms"]
+    h.close()
+    return out
 
 
 def tol_sweep(a, h, b, x, bnorm, rel_err, torch):
@@ -280,6 +341,15 @@ def main():
     tmx = (time.perf_counter() - t0) / a.steps
     errmx = rel_err(x)
 
+    extra = {}
+    if not a.no_extra:
+        for name, fn in (("csr_spmv", lambda: csr_leg(h, torch, apply_ms64 * 1e3)),
+                         ("n8193", lambda: leg_8193(a, dev, torch))):
+            try:
+                extra[name] = fn()
+            except Exception as e:  # reported, never silently replaced
+                extra[name] = {"error": f"{type(e).__name__}: {e}"}
+
     batched = None
     if a.batch > 0:
         try:
@@ -326,6 +396,7 @@ def main():
                      "frac": gbs64 / HBM_PEAK_GBS, "traffic": traffic_from_profiles("f64"),
                      "kernel": "stencil_pass FP64 apply (OutStore), algorithmic bytes 2*n*8 per launch"},
         "batched": batched,
+        **extra,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

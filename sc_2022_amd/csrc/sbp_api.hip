@@ -749,12 +749,63 @@ int smooth_t(sbp_handle h, int level, const T* b, T* x, int nsweeps) {
     return SBP_OK;
 }
 
-__global__ void exclusive_scan_serial(const int32_t* counts, int32_t* rp, int n, int64_t* total) {
-    // One thread; used once per assembly.
-    int64_t s = 0;
-    for (int q = 0; q < n; ++q) { rp[q] = (int32_t)s; s += counts[q]; }
-    rp[n] = (int32_t)s;
-    *total = s;
+// Exclusive scan of the CSR row counts into row pointers, three passes:
+// per-block scans of kScanPer elements, a scan of the block totals in one
+// block, and the block offsets added back (rp[n] = nnz).
+constexpr int kScanThreads = 256, kScanItems = 8, kScanPer = kScanThreads * kScanItems;
+
+__global__ __launch_bounds__(kScanThreads) void scan_rows(const int32_t* counts, int32_t* rp, int n, int64_t* bsum) {
+    __shared__ int64_t sh[kScanThreads];
+    const int tid = threadIdx.x;
+    const int base = blockIdx.x * kScanPer + tid * kScanItems;
+    int64_t loc[kScanItems];
+    int64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        loc[k] = acc;
+        acc += base + k < n ? counts[base + k] : 0;
+    }
+    sh[tid] = acc;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int64_t v = tid >= off ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += v;
+        __syncthreads();
+    }
+    const int64_t excl = sh[tid] - acc;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) rp[base + k] = (int32_t)(excl + loc[k]);
+    if (tid == kScanThreads - 1) bsum[blockIdx.x] = sh[tid];
+}
+
+__global__ __launch_bounds__(1024) void scan_block_sums(int64_t* bsum, int nb, int64_t* total) {
+    __shared__ int64_t sh[1024];
+    const int tid = threadIdx.x;
+    int64_t carry = 0;
+    for (int start = 0; start < nb; start += 1024) {
+        const int idx = start + tid;
+        const int64_t v = idx < nb ? bsum[idx] : 0;
+        sh[tid] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int64_t u = tid >= off ? sh[tid - off] : 0;
+            __syncthreads();
+            sh[tid] += u;
+            __syncthreads();
+        }
+        if (idx < nb) bsum[idx] = carry + sh[tid] - v;
+        carry += sh[1023];
+        __syncthreads();
+    }
+    if (tid == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(256) void scan_add(int32_t* rp, const int64_t* bsum, int n, const int64_t* total) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx < n) rp[idx] += (int32_t)bsum[idx / kScanPer];
+    if (idx == 0) rp[n] = (int32_t)*total;
 }
 
 __global__ __launch_bounds__(256) void copy_vec16(const uint4* __restrict__ s, uint4* __restrict__ d, size_t n16) {
@@ -1228,12 +1279,17 @@ int sbp_csr_assemble(sbp_handle h, int32_t* row_ptr, int32_t* col, double* val, 
     int32_t* counts = nullptr;
     int32_t* rp = row_ptr;
     int64_t* tot = nullptr;
+    int64_t* bsum = nullptr;
+    const int nsb = (n + kScanPer - 1) / kScanPer;
     HIPCHK(hipMalloc(&counts, (size_t)n * sizeof(int32_t)));
     HIPCHK(hipMalloc(&tot, sizeof(int64_t)));
+    HIPCHK(hipMalloc(&bsum, (size_t)nsb * sizeof(int64_t)));
     if (!rp) HIPCHK(hipMalloc(&rp, (size_t)(n + 1) * sizeof(int32_t)));
     const int nb = (n + 255) / 256;
     hipLaunchKernelGGL(csr_fill, dim3(nb), dim3(256), 0, h->st, rp, col, val, h->L[0].tb, 1, counts);
-    hipLaunchKernelGGL(exclusive_scan_serial, dim3(1), dim3(1), 0, h->st, counts, rp, n, tot);
+    hipLaunchKernelGGL(scan_rows, dim3(nsb), dim3(kScanThreads), 0, h->st, (const int32_t*)counts, rp, n, bsum);
+    hipLaunchKernelGGL(scan_block_sums, dim3(1), dim3(1024), 0, h->st, bsum, nsb, tot);
+    hipLaunchKernelGGL(scan_add, dim3(nb), dim3(256), 0, h->st, rp, (const int64_t*)bsum, n, (const int64_t*)tot);
     if (col && val && row_ptr)
         hipLaunchKernelGGL(csr_fill, dim3(nb), dim3(256), 0, h->st, rp, col, val, h->L[0].tb, 0, counts);
     HIPCHK(hipGetLastError());
@@ -1241,6 +1297,7 @@ int sbp_csr_assemble(sbp_handle h, int32_t* row_ptr, int32_t* col, double* val, 
     HIPCHK(hipStreamSynchronize(h->st));
     (void)hipFree(counts);
     (void)hipFree(tot);
+    (void)hipFree(bsum);
     if (!row_ptr) (void)hipFree(rp);
     return SBP_OK;
 }
@@ -1249,7 +1306,8 @@ int sbp_csr_spmv(sbp_handle h, const int32_t* rp, const int32_t* col, const doub
                  double* y) {
     if (!h || !rp || !col || !val || !x || !y) return SBP_ERR_INVALID_ARG;
     const int n = (int)h->L[0].n;
-    hipLaunchKernelGGL(csr_spmv, dim3((n + 255) / 256), dim3(256), 0, h->st, rp, col, val, x, y, n, nullptr,
+    const int nb = std::min(kCsrMaxBlocks, (n + 256 / kCsrLanes - 1) / (256 / kCsrLanes));
+    hipLaunchKernelGGL(csr_spmv, dim3(nb), dim3(256), 0, h->st, rp, col, val, x, y, n, nullptr,
                        (const Scalars*)nullptr);
     HIPCHK(hipGetLastError());
     return SBP_OK;
@@ -1275,10 +1333,11 @@ int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, const double
     HIPCHK(hipGetLastError());
     const int poll_every = std::max(1, h->prm.poll_every);
     const dim3 g(ew_blocks(n), 1);
+    const int nbv = std::min(kCsrMaxBlocks, (n + 256 / kCsrLanes - 1) / (256 / kCsrLanes));
     for (int it = 0; it < maxiter; ++it) {
-        hipLaunchKernelGGL(csr_spmv, dim3(nbs), dim3(256), 0, st, rp, col, val, (const double*)h->pv, h->Ap, n,
+        hipLaunchKernelGGL(csr_spmv, dim3(nbv), dim3(256), 0, st, rp, col, val, (const double*)h->pv, h->Ap, n,
                            h->partial, (const Scalars*)sc);
-        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, nbs, sc, 0);
+        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, nbv, sc, 0);
         hipLaunchKernelGGL((update_xr<double, kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, (const double*)h->pv,
                            (const double*)h->Ap, (const Scalars*)sc, (size_t)n, h->partial);
         hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 1);

@@ -472,18 +472,28 @@ __global__ __launch_bounds__(256) void rhs_t_g(const double* lam, double* dgW, d
 // ---------------------------------------------------------------------------
 // CSR comparison path
 // ---------------------------------------------------------------------------
-// One row per thread over rows of the assembled operator (nnz/row <= 11).
+// kCsrLanes lanes per row (rows hold at most 11 nonzeros), so a wave reads
+// 8 rows' values and column indices as contiguous runs; a grid-stride loop
+// over rows keeps the number of blocks (and block partials of x'Ax) bounded.
+constexpr int kCsrLanes = 8;
+constexpr int kCsrMaxBlocks = 4096;
 __global__ __launch_bounds__(256) void csr_spmv(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                const double* __restrict__ val, const double* __restrict__ x,
                                                double* __restrict__ y, int nrows, double* partial, const Scalars* sc) {
     if (sc && sc->done) return;
-    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & (kCsrLanes - 1);
+    const int ng = gridDim.x * (256 / kCsrLanes);
     double acc = 0.0;
-    if (r < nrows) {
+    for (int r = (blockIdx.x * 256 + threadIdx.x) / kCsrLanes; r < nrows; r += ng) {
+        const int k1 = rp[r + 1];
         double s = 0.0;
-        for (int k = rp[r]; k < rp[r + 1]; ++k) s += val[k] * x[col[k]];
-        y[r] = s;
-        acc = x[r] * s;
+        for (int k = rp[r] + lane; k < k1; k += kCsrLanes) s += val[k] * x[col[k]];
+#pragma unroll
+        for (int off = kCsrLanes / 2; off > 0; off >>= 1) s += __shfl_down(s, off, kCsrLanes);
+        if (lane == 0) {
+            y[r] = s;
+            acc += x[r] * s;
+        }
     }
     if (partial) {
         double bsum = block_sum<double, 256>(acc);
