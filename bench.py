@@ -48,7 +48,7 @@ def parse():
                     help="mixed: relative tolerance of each FP32 inner solve")
     ap.add_argument("--ir-rtol", type=float, default=0.0,
                     help="mixed: also stop when the correction-based error estimate <= ir_rtol ||x||")
-    ap.add_argument("--no-extra", action="store_true", help="skip the 8193^2 and CSR comparison legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 8193^2, CSR comparison and initial-guess CG legs")
     ap.add_argument("--tol-sweep", action="store_true",
                     help="print iterations, error and time per stopping rule for both legs, then exit")
     ap.add_argument("--apply-reps", type=int, default=200)
@@ -210,6 +210,51 @@ def leg_8193(a, dev, torch):
     return out
 
 
+def nested_leg(a, dev, torch, N=1025, p=2, rtol=1e-10):
+    """Initial-guess interpolation CG (the reference's
+    initial_guess_interpolation_CG family) on the manufactured problem at
+    N^2, p = 2, against matrix-free CG from zero and MG-PCG, all stopped at
+    ||r|| < rtol ||b||: fine-grid iterations, time per solve (the nested
+    solves include their coarse solves), H-norm error against u."""
+    from sc_2022_amd.data import manufactured_fields
+    from sc_2022_amd.solver import SbpPoisson
+    hs = [SbpPoisson(N, p, device=dev), SbpPoisson((N + 1) // 2, p, device=dev),
+          SbpPoisson((N + 3) // 4, p, device=dev)]
+    bs, u = [], None
+    for k, hk in enumerate(hs):
+        f, gW, gE, gS, gN, uk = manufactured_fields(hk.N, dev)
+        bs.append(hk.assemble_rhs(f, gW, gE, gS, gN))
+        if k == 0:
+            u = uk
+    h = hs[0]
+    tol = rtol * float(torch.linalg.vector_norm(bs[0]))
+    x = torch.zeros_like(bs[0])
+    runs = {
+        "cg_from_zero": lambda: h.cg(bs[0], x.zero_(), abstol=tol)[1],
+        "interp_2level": lambda: h.cg_interp(bs[0], bs[1], x=x, abstol=tol)[1],
+        "interp_3level": lambda: h.cg_interp(bs[0], bs[1], bs[2], x=x, abstol=tol)[1],
+        "mgpcg": lambda: h.mgpcg(bs[0], x.zero_(), abstol=tol)[1],
+    }
+    out = {"workload": f"N={N}, p={p}, u = sin(pi x) sin(pi y) + x y, stop ||r|| < {rtol:g} ||b||"}
+    uh = h.hnorm(u)
+    for name, fn in runs.items():
+        r = fn()
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = {"ms": dt * 1e3, "iters": r.iters, "converged": r.converged,
+                     "h_err_vs_u": h.hnorm(x - u) / uh}
+        if r.level_iters:
+            out[name]["level_iters"] = r.level_iters
+    for hk in hs:
+        hk.close()
+    return out
+
+
 def tol_sweep(a, h, b, x, bnorm, rel_err, torch):
     """Iterations, relative H-norm error and time of both legs over a grid of
     stopping rules (one JSON line each), to pick the loosest rule per leg
@@ -344,7 +389,8 @@ def main():
     extra = {}
     if not a.no_extra:
         for name, fn in (("csr_spmv", lambda: csr_leg(h, torch, apply_ms64 * 1e3)),
-                         ("n8193", lambda: leg_8193(a, dev, torch))):
+                         ("n8193", lambda: leg_8193(a, dev, torch)),
+                         ("initial_guess_cg", lambda: nested_leg(a, dev, torch))):
             try:
                 extra[name] = fn()
             except Exception as e:  # reported, never silently replaced

@@ -192,6 +192,25 @@ int sbp_adjoint_grad(sbp_handle h, const double* u, const double* u_obs, double 
                      double* loss, double* lam, double* df, double* dgW, double* dgE, double* dgS,
                      double* dgN, int* iters);
 
+/* Initial-guess interpolation CG (initial_guess_interpolation_CG /
+ * _CG_GPU / _CG_Matrix_Free_GPU, initial_guess_cg.jl:394-430; with nlev = 3,
+ * initial_guess_interpolation_three_level_CG_GPU, :447-456).  nlev = 2: solve
+ * A_2h x_2h = b2h exactly on the grid of side (N+1)/2 (the reference: sparse
+ * LU; here MG-PCG to ||r|| < 1e-13 ||b2h||), set x = P x_2h (bilinear) and run
+ * matrix-free CG on A x = b from there to ||r||_2 < abstol.  nlev = 3: the
+ * exact solve is on the 4h grid (b4h), CG on the 2h grid from P x_4h to
+ * abstol, then CG on this grid from P x_2h.  b2h / b4h are the right-hand
+ * sides assembled on the coarser grids (the reference's Assembling_matrix at
+ * level - 1 / - 2).  iters and resnorm (nullable, nlev - 1 entries, coarse to
+ * fine) receive the CG iterations and final ||r|| per CG grid; hist (nullable,
+ * maxiter + 1 doubles) the fine grid's ||r_k||.  x is output only.  Needs
+ * N = 2^k+1 with (N+1)/2 (and for nlev = 3 also (N+3)/4) >= coarse_n; the
+ * reference's reversal of the interpolated guess (:416) maps between its
+ * kernel's array orientation and its assembled ordering, which coincide in
+ * this layout. */
+int sbp_cg_interp(sbp_handle h, int nlev, const double* b, const double* b2h, const double* b4h,
+                  double* x, double abstol, int maxiter, int* iters, double* resnorm, double* hist);
+
 /* CSR comparison path (the assembled operator, as CG_GPU_sparse,
  * split_matrix_free.jl:744-769) ------------------------------------------ */
 /* Fills device CSR arrays (int32 indices) of the assembled A; call first with

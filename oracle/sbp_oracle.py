@@ -21,6 +21,8 @@ operator, literally (same kron products, same SAT terms):
 * ``vcycle`` / ``mgpcg`` -- the multi-level V-cycle and MG-preconditioned CG
   (mg_preconditioned_CG, level_2_multigrid_new.jl:480-524, generalised from two
   levels to a hierarchy down to 17x17 with damped-Jacobi smoothing).
+* ``initial_guess_interpolation_cg`` -- initial_guess_interpolation_CG and its
+  three-level form, initial_guess_cg.jl:394-399, 447-456.
 
 Grid convention (the reference's): level k gives N = 2^k + 1 points per side
 on [0,1]^2, h = 1/2^k.  A grid function is a flat vector with element (i, j)
@@ -33,8 +35,10 @@ independent fact): D1/D2 exactness on polynomials and the SBP property,
 sum(H) = 1, A symmetric positive definite, the literal restatement of the
 reference GPU kernel equal to -A, the closed form A = K (x) H + H (x) M,
 transfer operators against their closed forms (R = P^T / 4, P exact on
-bilinears), CG and MG-PCG against the sparse direct solve, and the
-manufactured-solution convergence rates (2 for p = 2, about 4 for p = 4).
+bilinears), CG and MG-PCG against the sparse direct solve, the
+manufactured-solution convergence rates (2 for p = 2, about 4 for p = 4), and
+the interpolated initial guess against a bilinear solution, which every level
+represents exactly (so the guess must equal it to round-off).
 """
 from __future__ import annotations
 
@@ -575,6 +579,50 @@ def mgpcg(hier, b, x=None, rtol=1e-8, maxiter=200, nu_pre=2, nu_post=2, omega=2.
         p = z + (rznew / rz) * p
         rz = rznew
     return x, it, norms
+
+
+def initial_guess_interpolation_cg(A, b, A2h, b2h, abstol, maxiter=None, A4h=None, b4h=None):
+    """initial_guess_interpolation_CG (initial_guess_cg.jl:394-399): x_2h =
+    A_2h \\ b_2h (sparse direct solve), x0 = P x_2h (bilinear), then CG on
+    A x = b from x0 to ||r|| < abstol.  With A4h/b4h, the three-level form
+    (initial_guess_cg.jl:447-456): the direct solve is on the 4h grid and CG
+    on the 2h grid from P x_4h (same abstol) supplies x_2h.
+    Returns (x, iterations per CG grid (coarse to fine), final ||r|| per CG
+    grid, x0)."""
+    N = int(round(math.sqrt(len(b))))
+    N2 = (N + 1) // 2
+    iters, norms = [], []
+    if A4h is None:
+        x2 = spla.spsolve(sp.csc_matrix(A2h), np.asarray(b2h, dtype=np.float64))
+    else:
+        N4 = (N2 + 1) // 2
+        x4 = spla.spsolve(sp.csc_matrix(A4h), np.asarray(b4h, dtype=np.float64))
+        x2, it2, nr2 = cg_cpu(A2h, b2h, prolongation_2d(N4) @ x4, maxiter=maxiter, abstol=abstol)
+        iters.append(it2)
+        norms.append(nr2[-1])
+    x0 = prolongation_2d(N2) @ x2
+    x, it, nr = cg_cpu(A, b, x0.copy(), maxiter=maxiter, abstol=abstol)
+    iters.append(it)
+    norms.append(nr[-1])
+    return x, iters, norms, x0
+
+
+def bilinear_data(level, c=(0.5, -1.0, 2.0, 3.0)):
+    """u = c0 + c1 x + c2 y + c3 x y (x slow / Dirichlet, y fast / Neumann)
+    and its data: f = Laplace(u) = 0, gW/gE = u on x = 0/1, gS = -du/dy at
+    y = 0, gN = du/dy at y = 1.  SBP D2 and the boundary derivative are exact
+    on linear functions, so u is the discrete solution on every grid."""
+    t = grid(level)
+    yi = t[:, None]
+    xj = t[None, :]
+    U = c[0] + c[1] * xj + c[2] * yi + c[3] * xj * yi
+    u = U.ravel(order="F")
+    f = np.zeros_like(u)
+    gW = c[0] + c[2] * t
+    gE = c[0] + c[1] + (c[2] + c[3]) * t
+    gS = -(c[2] + c[3] * t)
+    gN = c[2] + c[3] * t
+    return f, gW, gE, gS, gN, u
 
 
 def h_norm(v, hdiag):

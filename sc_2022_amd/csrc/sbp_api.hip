@@ -79,6 +79,11 @@ struct sbp_handle_s {
     hipStream_t cap = nullptr;   // capture stream (default priority)
     hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    // Initial-guess interpolation: the handle of the next coarser grid
+    // ((N+1)/2, same order and parameters) and the coarse solution on it,
+    // both made on first use.
+    sbp_handle child = nullptr;
+    double* xc = nullptr;
 };
 
 namespace {
@@ -879,7 +884,8 @@ int sbp_destroy(sbp_handle h) {
     }
     drop_graphs(h);
     if (h->cap) (void)hipStreamDestroy(h->cap);
-    void* ptrs[] = {h->w, h->lam, h->Ainv, h->r, h->pv, h->pv2, h->Ap, h->z, h->x, h->r32, h->p32, h->p32b, h->Ap32,
+    if (h->child) (void)sbp_destroy(h->child);
+    void* ptrs[] = {h->xc, h->w, h->lam, h->Ainv, h->r, h->pv, h->pv2, h->Ap, h->z, h->x, h->r32, h->p32, h->p32b, h->Ap32,
                     h->z32, h->d32, h->partial, h->sc, h->sc_in, h->dtmp, h->dabs};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -1268,6 +1274,91 @@ int sbp_adjoint_grad(sbp_handle h, const double* u, const double* u_obs, double 
     const int rc = batch_status(h);
     if (rc != SBP_OK) return rc;
     return sbp_rhs_transpose(h, lm, df, dgW, dgE, dgS, dgN);
+}
+
+// ---------------------------------------------------------------------------
+// Initial-guess interpolation CG (nested iteration, one problem)
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr double kExactRtol = 1e-13;   // coarsest solve: ||r|| < 1e-13 ||b||
+constexpr int kExactMaxiter = 100;
+
+// Handle of the next coarser grid ((N+1)/2, same order and parameters, level
+// l keeping h's tile shapes for level l+1), made on first use; it runs on h's
+// stream.
+int coarser(sbp_handle h, sbp_handle* out) {
+    if (h->nprob != 1 || !h->mg || h->nlev < 2) return SBP_ERR_INVALID_SIZE;
+    if (!h->child) {
+        sbp_handle c = nullptr;
+        STCHK(sbp_create(h->L[1].N, h->p, &h->prm, &c));
+        for (int l = 0; l < c->nlev && l + 1 < h->nlev; ++l)
+            for (int q = 0; q < 2; ++q) {
+                c->L[l].ti[q] = h->L[l + 1].ti[q];
+                c->L[l].tj[q] = h->L[l + 1].tj[q];
+                c->L[l].fti[q] = h->L[l + 1].fti[q];
+                c->L[l].ftj[q] = h->L[l + 1].ftj[q];
+            }
+        if (hipMalloc(&h->xc, c->L[0].n * sizeof(double)) != hipSuccess) {
+            (void)sbp_destroy(c);
+            return SBP_ERR_HIP;
+        }
+        h->child = c;
+    }
+    h->child->st = h->st;
+    *out = h->child;
+    return SBP_OK;
+}
+
+// x = A^-1 b on handle c, for the reference's sparse LU solve A_2h \ b_2h:
+// MG-PCG from zero to ||r|| < 1e-13 ||b|| (round-off level for FP64).  An
+// iterate short of that after kExactMaxiter iterations is still returned as
+// the guess; HIP errors, NaN/Inf and breakdown are reported.
+int exact_solve(sbp_handle c, const double* b, double* x) {
+    HIPCHK(hipMemsetAsync(x, 0, c->L[0].n * sizeof(double), c->st));
+    double bb = 0.0;
+    STCHK(sbp_dot(c, b, b, &bb));
+    if (!std::isfinite(bb)) return SBP_ERR_NONFINITE;
+    if (bb == 0.0) return SBP_OK;
+    int it = 0;
+    double rn = 0.0;
+    const int rc = sbp_mgpcg(c, b, x, kExactRtol * std::sqrt(bb), kExactMaxiter, &it, &rn, nullptr);
+    return rc == SBP_ERR_NOT_CONVERGED ? SBP_OK : rc;
+}
+
+// bs[0] is this grid's right-hand side, bs[1] the next coarser grid's, ...
+// The coarsest of the nlev grids is solved exactly; every finer one starts
+// CG from the bilinear prolongation of the solution one level down.
+// iters / resnorm [nlev - 2] receive this grid's CG results.
+int nested_cg(sbp_handle h, int nlev, const double* const* bs, double* x, double abstol, int maxiter, int* iters,
+              double* resnorm, double* hist) {
+    sbp_handle c = nullptr;
+    STCHK(coarser(h, &c));
+    if (nlev == 2) {
+        STCHK(exact_solve(c, bs[1], h->xc));
+    } else {
+        const int rc = nested_cg(c, nlev - 1, bs + 1, h->xc, abstol, maxiter, iters, resnorm, nullptr);
+        if (rc != SBP_OK && rc != SBP_ERR_NOT_CONVERGED) return rc;
+    }
+    HIPCHK(hipMemsetAsync(x, 0, h->L[0].n * sizeof(double), h->st));
+    STCHK(sbp_prolong_add(h, 0, h->xc, x, SBP_F64));
+    int it = 0;
+    double rn = 0.0;
+    const int rc = sbp_cg(h, bs[0], x, abstol, maxiter, &it, &rn, hist);
+    if (iters) iters[nlev - 2] = it;
+    if (resnorm) resnorm[nlev - 2] = rn;
+    return rc;
+}
+
+}  // namespace
+
+int sbp_cg_interp(sbp_handle h, int nlev, const double* b, const double* b2h, const double* b4h, double* x,
+                  double abstol, int maxiter, int* iters, double* resnorm, double* hist) {
+    if (!h || !b || !b2h || !x || (nlev != 2 && nlev != 3) || (nlev == 3 && !b4h) || maxiter < 0 ||
+        !(abstol >= 0) || h->nprob != 1)
+        return SBP_ERR_INVALID_ARG;
+    const double* bs[3] = {b, b2h, b4h};
+    return nested_cg(h, nlev, bs, x, abstol, maxiter, iters, resnorm, hist);
 }
 
 // ---------------------------------------------------------------------------

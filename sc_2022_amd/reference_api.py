@@ -113,3 +113,41 @@ def CG_GPU_sparse(x, A_csr, b, abstol=None, p=2):
     abstol = math.sqrt(2.220446049250313e-16) if abstol is None else abstol
     _, res = h.csr_cg(A_csr, b.reshape(-1), x.reshape(-1), abstol=abstol)
     return res.iters, abstol, res.resnorm
+
+
+def initial_guess_interpolation_CG_Matrix_Free_GPU(A_GPU, b_GPU, b_2h, x, Nx_2h, abstol=None, p=2):
+    """Coarse exact solve, bilinear interpolation, matrix-free CG
+    (initial_guess_cg.jl:408-430) -> (x, iters, final ||r||).  A_GPU is unused:
+    the operator is the handle's.  The reference's CG here stops at the
+    absolute sqrt(eps); its reversal of the guess (:416) is a layout change
+    that this layout does not need.  x receives the solution."""
+    N = _side(b_GPU)
+    if (N + 1) // 2 != Nx_2h:
+        raise ValueError("b_2h must be on the grid of side (N+1)/2")
+    h = handle(N, p, device=b_GPU.device)
+    abstol = math.sqrt(2.220446049250313e-16) if abstol is None else abstol
+    _, res = h.cg_interp(b_GPU.reshape(-1), b_2h.reshape(-1), x=x.reshape(-1), abstol=abstol)
+    return x, res.iters, res.resnorm
+
+
+def initial_guess_interpolation_CG_GPU(A_GPU, b_GPU, b_2h, x, Nx_2h, abstol=None, p=2):
+    """initial_guess_interpolation_CG_GPU (initial_guess_cg.jl:401-406): the
+    same nested solve (the reference runs its CG on the CSR matrix A_GPU; here
+    the matrix-free apply of the same operator) -> (x, iters, final ||r||)."""
+    return initial_guess_interpolation_CG_Matrix_Free_GPU(A_GPU, b_GPU, b_2h, x, Nx_2h, abstol=abstol, p=p)
+
+
+def initial_guess_interpolation_three_level_CG_GPU(A_GPU, A_2h_GPU, b_GPU, b_2h_GPU, b_2h, b_4h, x, Nx_2h, Nx_4h,
+                                                   abstol=None, p=2):
+    """Three-level nested solve (initial_guess_cg.jl:447-456): exact solve on
+    the 4h grid, CG on the 2h grid from its interpolation, CG on the fine grid
+    from that -> (x, iters_2h, iters_h, ||r||_2h, ||r||_h).  b_2h duplicates
+    b_2h_GPU in the reference (host copy for its LU) and is unused here."""
+    N = _side(b_GPU)
+    if (N + 1) // 2 != Nx_2h or (Nx_2h + 1) // 2 != Nx_4h:
+        raise ValueError("coarse grids must have sides (N+1)/2 and (N+3)/4")
+    h = handle(N, p, device=b_GPU.device)
+    abstol = math.sqrt(2.220446049250313e-16) if abstol is None else abstol
+    b4 = torch.as_tensor(b_4h, dtype=torch.float64, device=b_GPU.device).reshape(-1)
+    _, res = h.cg_interp(b_GPU.reshape(-1), b_2h_GPU.reshape(-1), b4h=b4, x=x.reshape(-1), abstol=abstol)
+    return x, res.level_iters[0], res.level_iters[1], res.level_resnorms[0], res.level_resnorms[1]

@@ -31,6 +31,8 @@ class SolveResult:
     status: int
     history: list | None = None
     inner_iters: int | None = None
+    level_iters: list | None = None      # nested iteration: CG iterations per grid, coarse to fine
+    level_resnorms: list | None = None
 
     @property
     def converged(self):
@@ -224,6 +226,30 @@ class SbpPoisson:
 
     def mgpcg(self, b, x=None, abstol=1e-8, maxiter=500, history=False, raise_on_fail=True):
         return self._solve(self.lib.sbp_mgpcg, "sbp_mgpcg", b, x, abstol, maxiter, history, raise_on_fail)
+
+    def cg_interp(self, b, b2h, b4h=None, x=None, abstol=1e-8, maxiter=None, history=False, raise_on_fail=True):
+        """Initial-guess interpolation CG: exact solve on the coarsest grid
+        (b2h on side (N+1)/2, or b4h on the next one down when given), bilinear
+        prolongation, CG on every finer grid from the interpolated solution.
+        Returns (x, SolveResult) with the fine grid's iterations and the
+        per-grid ones in level_iters."""
+        nlev = 2 if b4h is None else 3
+        self._chk(b, dtype=torch.float64, name="b")
+        self._chk(b2h, self.level_size(1) ** 2, torch.float64, "b2h")
+        if b4h is not None:
+            self._chk(b4h, self.level_size(2) ** 2, torch.float64, "b4h")
+        x = torch.empty_like(b) if x is None else self._chk(x, dtype=torch.float64, name="x")
+        maxiter = self.n if maxiter is None else maxiter
+        its = (C.c_int * (nlev - 1))()
+        rns = (C.c_double * (nlev - 1))()
+        hist = torch.zeros(maxiter + 1, dtype=torch.float64, device=self.device) if history else None
+        self._sync_stream()
+        st = self.lib.sbp_cg_interp(self._h, nlev, _ptr(b), _ptr(b2h), _ptr(b4h), _ptr(x), float(abstol),
+                                    int(maxiter), its, rns, _ptr(hist))
+        if raise_on_fail and st not in (capi.SBP_OK, capi.SBP_ERR_NOT_CONVERGED):
+            capi.check(st, "sbp_cg_interp")
+        hl = hist[: its[nlev - 2] + 1].cpu().tolist() if history else None
+        return x, SolveResult(its[nlev - 2], rns[nlev - 2], st, hl, None, list(its), list(rns))
 
     def mgpcg_mixed(self, b, x=None, abstol=1e-8, max_outer=None, raise_on_fail=True):
         self._chk(b, dtype=torch.float64, name="b")

@@ -203,6 +203,52 @@ def test_cg_iteration_count_129_p2():
     assert 540 <= it <= 580
 
 
+@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("three", [False, True])
+def test_initial_guess_interpolation_bilinear(p, three):
+    """A bilinear u is the discrete solution on every grid (D2 and the
+    boundary derivative are exact on linears), so the nested solve's
+    interpolated guess must reproduce it to round-off; the same guess rotated
+    by 180 degrees (the reference's matrix-free variant reverses its guess)
+    must not."""
+    level = 6
+    A, A2, A4 = (O.assemble_operator(level - k, p) for k in range(3))
+    bs = []
+    for k in range(3):
+        f, gW, gE, gS, gN, u_k = O.bilinear_data(level - k)
+        bs.append(O.assemble_rhs(level - k, p, f, gW, gE, gS, gN))
+        if k == 0:
+            u = u_k
+    assert np.linalg.norm(bs[0] - A @ u) < 1e-13 * np.linalg.norm(bs[0])
+    kw = dict(A4h=A4, b4h=bs[2]) if three else {}
+    x, iters, norms, x0 = O.initial_guess_interpolation_cg(A, bs[0], A2, bs[1], 1e-10 * np.linalg.norm(bs[0]), **kw)
+    assert np.abs(x0 - u).max() < 1e-12
+    assert np.abs(x0[::-1] - u).max() > 0.5
+    assert len(iters) == (2 if three else 1)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_initial_guess_interpolation_cuts_cg_iterations(p):
+    """On the manufactured problem the nested solve converges to the direct
+    solution in fewer fine-grid CG iterations than CG from zero."""
+    level = 6
+    A, A2 = O.assemble_operator(level, p), O.assemble_operator(level - 1, p)
+    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
+    f2, gW2, gE2, gS2, gN2, _ = O.baseline_data(level - 1)
+    b2 = O.assemble_rhs(level - 1, p, f2, gW2, gE2, gS2, gN2)
+    tol = 1e-10 * np.linalg.norm(b)
+    x, iters, norms, x0 = O.initial_guess_interpolation_cg(A, b, A2, b2, tol)
+    xd = spla.spsolve(A.tocsc(), b)
+    _, it0, _ = O.cg_cpu(A, b, abstol=tol)
+    assert norms[-1] < tol and np.linalg.norm(A @ x - b) < tol * 1.000001
+    assert np.linalg.norm(x - xd) < 1e-8 * np.linalg.norm(xd)
+    assert iters[0] < it0
+    # the guess is O(h^2)-close to the fine solution already
+    hd = O.h_tilde_diag(level, p)
+    assert O.h_norm(x0 - xd, hd) < 1e-3 * O.h_norm(xd, hd)
+
+
 def test_fourier_modes_deterministic():
     s1, b1 = O.fourier_modes(123)
     s2, b2 = O.fourier_modes(123)
