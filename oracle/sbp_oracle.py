@@ -7,9 +7,9 @@ The product path (``sc_2022_amd``) never routes through it.
 It restates, with numpy/scipy.sparse, the reference's CPU construction of the
 operator, literally (same kron products, same SAT terms):
 
-* ``diagonal_sbp_D1`` -- diagonal_sbp.jl:56-147 (p = 2, 4)
+* ``diagonal_sbp_D1`` -- diagonal_sbp.jl:56-147 (p = 2, 4, 6, 8)
 * ``diagonal_sbp_D2`` -- diagonal_sbp.jl:189-454 (p = 2 :191-197, p = 4
-  :198-208, H/HI :421-428, S0/SN :446-448)
+  :198-208, p = 6 :210-222, p = 8 :224-311, H/HI :421-428, S0/SN :446-448)
 * ``operators_2d`` / ``assembling_matrix`` -- level_2_multigrid_new.jl:48-188
 * ``prolongation_matrix`` / ``restriction_matrix_normal`` /
   ``prolongation_2d`` / ``restriction_2d`` -- level_2_multigrid_new.jl:190-246
@@ -63,6 +63,32 @@ _D1_TABLES = {
             [4 / 43, -59 / 86, 0, 59 / 86, -4 / 43, 0],
             [3 / 98, 0, -59 / 98, 0, 32 / 49, -4 / 49]],
     ),
+    6: dict(  # diagonal_sbp.jl:69-79
+        bhinv=[43200 / 13649, 8640 / 12013, 4320 / 2711, 4320 / 5359, 8640 / 7877, 43200 / 43801],
+        d=[-1 / 60, 3 / 20, -3 / 4, 0.0, 3 / 4, -3 / 20, 1 / 60],
+        bd=[
+            [-21600 / 13649, 104009 / 54596, 30443 / 81894, -33311 / 27298, 16863 / 27298, -15025 / 163788, 0.0, 0.0, 0.0],
+            [-104009 / 240260, 0.0, -311 / 72078, 20229 / 24026, -24337 / 48052, 36661 / 360390, 0.0, 0.0, 0.0],
+            [-30443 / 162660, 311 / 32532, 0.0, -11155 / 16266, 41287 / 32532, -21999 / 54220, 0.0, 0.0, 0.0],
+            [33311 / 107180, -20229 / 21436, 485 / 1398, 0.0, 4147 / 21436, 25427 / 321540, 72 / 5359, 0.0, 0.0],
+            [-16863 / 78770, 24337 / 31508, -41287 / 47262, -4147 / 15754, 0.0, 342523 / 472620, -1296 / 7877, 144 / 7877, 0.0],
+            [15025 / 525612, -36661 / 262806, 21999 / 87602, -25427 / 262806, -342523 / 525612, 0.0, 32400 / 43801, -6480 / 43801, 720 / 43801],
+        ],
+    ),
+    8: dict(  # diagonal_sbp.jl:80-91
+        bhinv=[5080320 / 1498139, 725760 / 1107307, 80640 / 20761, 725760 / 1304999, 725760 / 299527, 80640 / 103097, 725760 / 670091, 5080320 / 5127739],
+        d=[1 / 280, -4 / 105, 1 / 5, -4 / 5, 0.0, 4 / 5, -1 / 5, 4 / 105, -1 / 280],
+        bd=[
+            [-2540160 / 1498139, 699846290793 / 311403172540, -10531586157 / 311403172540, -145951651079 / 186841903524, 398124597 / 15570158627, 39152001 / 113858564, -80631892229 / 934209517620, -6230212503 / 311403172540, 0.0, 0.0, 0.0, 0.0],
+            [-24132630717 / 55557028660, 0.0, 2113176981 / 23016483302, 5686186719 / 11508241651, -3408473341 / 138098899812, -39291999 / 210388330, 607046586 / 11508241651, 3460467023 / 483346149342, 0.0, 0.0, 0.0, 0.0],
+            [3510528719 / 90623010660, -704392327 / 1294614438, 0.0, 503511235 / 1294614438, 354781619 / 2589228876, 407439 / 3944590, -2986842 / 16597621, 169381493 / 3020767022, 0.0, 0.0, 0.0, 0.0],
+            [145951651079 / 1139279786988, -5686186719 / 13562854607, -1510533705 / 27125709214, 0.0, 6763379967 / 54251418428, 13948923 / 49589962, -1603900430 / 40688563821, -3742312557 / 189879964498, 0.0, 0.0, 0.0, 0.0],
+            [-398124597 / 21790888777, 3408473341 / 37355809332, -1064344857 / 12451936444, -6763379967 / 12451936444, 0.0, 763665 / 1198108, -1282435899 / 12451936444, 7822226819 / 261490665324, -2592 / 299527, 0.0, 0.0, 0.0],
+            [-1864381 / 23506116, 13097333 / 58765290, -407439 / 19588430, -4649641 / 11753058, -254555 / 1237164, 0.0, 5346432 / 9794215, -923328 / 9794215, 3072 / 103097, -288 / 103097, 0.0, 0.0],
+            [11518841747 / 417855345780, -607046586 / 6964255763, 1192698 / 23768791, 1603900430 / 20892767289, 1282435899 / 27857023052, -48117888 / 63658645, 0.0, 301190400 / 366539777, -145152 / 670091, 27648 / 670091, -2592 / 670091, 0.0],
+            [6230212503 / 1065851828540, -3460467023 / 319755548562, -1524433437 / 106585182854, 3742312557 / 106585182854, -7822226819 / 639511097124, 58169664 / 487135205, -2108332800 / 2804873233, 0.0, 4064256 / 5127739, -1016064 / 5127739, 193536 / 5127739, -18144 / 5127739],
+        ],
+    ),
 }
 
 _D2_TABLES = {
@@ -78,11 +104,39 @@ _D2_TABLES = {
             [-1 / 49, 0, 59 / 49, -118 / 49, 64 / 49, -4 / 49]],
         BS=[11 / 6, -3.0, 3 / 2, -1 / 3],
     ),
+    6: dict(  # diagonal_sbp.jl:210-222
+        bhinv=[43200 / 13649, 8640 / 12013, 4320 / 2711, 4320 / 5359, 8640 / 7877, 43200 / 43801],
+        d=[1 / 90, -3 / 20, 3 / 2, -49 / 18, 3 / 2, -3 / 20, 1 / 90],
+        bd=[
+            [114170 / 40947, -438107 / 54596, 336409 / 40947, -276997 / 81894, 3747 / 13649, 21035 / 163788, 0.0, 0.0, 0.0],
+            [6173 / 5860, -2066 / 879, 3283 / 1758, -303 / 293, 2111 / 3516, -601 / 4395, 0.0, 0.0, 0.0],
+            [-52391 / 81330, 134603 / 32532, -21982 / 2711, 112915 / 16266, -46969 / 16266, 30409 / 54220, 0.0, 0.0, 0.0],
+            [68603 / 321540, -12423 / 10718, 112915 / 32154, -75934 / 16077, 53369 / 21436, -54899 / 160770, 48 / 5359, 0.0, 0.0],
+            [-7053 / 39385, 86551 / 94524, -46969 / 23631, 53369 / 15754, -87904 / 23631, 820271 / 472620, -1296 / 7877, 96 / 7877, 0.0],
+            [21035 / 525612, -24641 / 131403, 30409 / 87602, -54899 / 131403, 820271 / 525612, -117600 / 43801, 64800 / 43801, -6480 / 43801, 480 / 43801],
+        ],
+        BS=[25 / 12, -4.0, 3.0, -4 / 3, 1 / 4],
+    ),
+    8: dict(  # diagonal_sbp.jl:224-311
+        bhinv=[5080320 / 1498139, 725760 / 1107307, 80640 / 20761, 725760 / 1304999, 725760 / 299527, 80640 / 103097, 725760 / 670091, 5080320 / 5127739],
+        d=[-1 / 560, 8 / 315, -1 / 5, 8 / 5, -205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560],
+        bd=[
+            [4870382994799 / 1358976868290, -893640087518 / 75498714905, 926594825119 / 60398971924, -1315109406200 / 135897686829, 39126983272 / 15099742981, 12344491342 / 75498714905, -451560522577 / 2717953736580, 0.0, 0.0, 0.0, 0.0, 0.0],
+            [333806012194 / 390619153855, -154646272029 / 111605472530, 1168338040 / 33481641759, 82699112501 / 133926567036, -171562838 / 11160547253, -28244698346 / 167408208795, 11904122576 / 167408208795, -2598164715 / 312495323084, 0.0, 0.0, 0.0, 0.0],
+            [7838984095 / 52731029988, 1168338040 / 5649753213, -88747895 / 144865467, 423587231 / 627750357, -43205598281 / 22599012852, 4876378562 / 1883251071, -5124426509 / 3766502142, 10496900965 / 39548272491, 0.0, 0.0, 0.0, 0.0],
+            [-94978241528 / 828644350023, 82699112501 / 157837019052, 1270761693 / 13153084921, -167389605005 / 118377764289, 48242560214 / 39459254763, -31673996013 / 52612339684, 43556319241 / 118377764289, -44430275135 / 552429566682, 0.0, 0.0, 0.0, 0.0],
+            [1455067816 / 21132528431, -171562838 / 3018932633, -43205598281 / 36227191596, 48242560214 / 9056797899, -52276055645 / 6037865266, 57521587238 / 9056797899, -80321706377 / 36227191596, 8078087158 / 21132528431, -1296 / 299527, 0.0, 0.0, 0.0],
+            [10881504334 / 327321118845, -28244698346 / 140280479505, 4876378562 / 9352031967, -10557998671 / 12469375956, 57521587238 / 28056095901, -278531401019 / 93520319670, 73790130002 / 46760159835, -137529995233 / 785570685228, 2048 / 103097, -144 / 103097, 0.0, 0.0],
+            [-135555328849 / 8509847458140, 11904122576 / 101307707835, -5124426509 / 13507694378, 43556319241 / 60784624701, -80321706377 / 81046166268, 73790130002 / 33769235945, -950494905688 / 303923123505, 239073018673 / 141830790969, -145152 / 670091, 18432 / 670091, -1296 / 670091, 0.0],
+            [0.0, -2598164715 / 206729925524, 10496900965 / 155047444143, -44430275135 / 310094888286, 425162482 / 2720130599, -137529995233 / 620189776572, 239073018673 / 155047444143, -144648000000 / 51682481381, 8128512 / 5127739, -1016064 / 5127739, 129024 / 5127739, -9072 / 5127739],
+        ],
+        BS=[4723 / 2100, -839 / 175, 157 / 35, -278 / 105, 103 / 140, 1 / 175, -6 / 175],
+    ),
 }
 
 
 def _check_order(p):
-    if p not in (2, 4):
+    if p not in (2, 4, 6, 8):
         raise ValueError(f"Operators for order {p} are not implemented")
 
 
@@ -110,7 +164,7 @@ def _banded(p, N, d, bd, mirror_sign):
 
 
 def diagonal_sbp_D1(p, N, xc=(-1.0, 1.0)):
-    """(D, HI, H, r) -- diagonal_sbp.jl:56-147 (p = 2, 4 only)."""
+    """(D, HI, H, r) -- diagonal_sbp.jl:56-147 (p = 2, 4, 6, 8)."""
     _check_order(p)
     t = _D1_TABLES[p]
     rows, cols, vals, bm = _banded(p, N, t["d"], t["bd"], -1.0)
@@ -129,7 +183,7 @@ def diagonal_sbp_D1(p, N, xc=(-1.0, 1.0)):
 
 
 def diagonal_sbp_D2(p, N, xc=(-1.0, 1.0)):
-    """(D, S0, SN, HI, H, r) -- diagonal_sbp.jl:189-454 (p = 2, 4 only)."""
+    """(D, S0, SN, HI, H, r) -- diagonal_sbp.jl:189-454 (p = 2, 4, 6, 8)."""
     _check_order(p)
     t = _D2_TABLES[p]
     rows, cols, vals, bm = _banded(p, N, t["d"], t["bd"], +1.0)

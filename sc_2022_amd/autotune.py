@@ -23,9 +23,11 @@ HERE = Path(__file__).resolve().parent
 CACHE = HERE / "tiles_gfx950.json"
 
 
-def compiled_shapes():
+def compiled_shapes(p=2):
+    """Tile shapes libsbp.so has for order p (p = 6, 8: a subset)."""
     text = (HERE / "csrc" / "sbp_launch.h").read_text()
-    line = re.search(r"#define SBP_TILE_SHAPES\(X\)(.*)", text).group(1)
+    name = "SBP_TILE_SHAPES" if p <= 4 else "SBP_HI_TILE_SHAPES"
+    line = re.search(r"#define " + name + r"\(X\)(.*)", text).group(1)
     return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+),\s*(\d+)\)", line)]
 
 
@@ -76,9 +78,9 @@ def time_level(h, level, dtype, shapes, reps=None):
 def tune(N, orders=(2, 4), out=None, verbose=True):
     import torch
     from .solver import SbpPoisson
-    shapes = compiled_shapes()
     cache = load_cache(out)
     for p in orders:
+        shapes = compiled_shapes(p)
         h = SbpPoisson(N, p)
         nlev = max(1, h.nlevels)
         for level in range(nlev):

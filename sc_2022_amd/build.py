@@ -35,9 +35,9 @@ FLAGS = [
 ]
 
 
-def tile_shapes():
+def tile_shapes(name="SBP_TILE_SHAPES"):
     text = (CSRC / "sbp_launch.h").read_text()
-    line = re.search(r"#define SBP_TILE_SHAPES\(X\)(.*)", text).group(1)
+    line = re.search(r"#define " + name + r"\(X\)(.*)", text).group(1)
     return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+),\s*(\d+)\)", line)]
 
 
@@ -53,9 +53,10 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
     units = [(CSRC / "sbp_api.hip", OBJ / "sbp_api.o", []),
              (CSRC / "sbp_fused.hip", OBJ / "sbp_fused.o", [])]
+    hi = set(tile_shapes("SBP_HI_TILE_SHAPES"))
     for ti, tj in tile_shapes():
         units.append((CSRC / "sbp_tile.hip", OBJ / f"sbp_tile_{ti}x{tj}.o",
-                      [f"-DSBP_TI={ti}", f"-DSBP_TJ={tj}"]))
+                      [f"-DSBP_TI={ti}", f"-DSBP_TJ={tj}", f"-DSBP_HI={int((ti, tj) in hi)}"]))
     jobs = jobs or min(len(units), os.cpu_count() or 4)
     cmds = [[HIPCC, *FLAGS, *defs, "-c", str(src), "-o", str(obj)] for src, obj, defs in units]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -106,8 +106,12 @@ inline int ew_blocks(size_t n) {
 }
 
 // Tile shapes that are compiled in.
-inline bool tile_ok(int ti, int tj) {
+inline bool tile_ok(int p, int ti, int tj) {
 #define SBP_TILE_OK(TI_, J_) if (ti == TI_ && tj == J_) return true;
+    if (p > 4) {
+        SBP_HI_TILE_SHAPES(SBP_TILE_OK)
+        return false;
+    }
     SBP_TILE_SHAPES(SBP_TILE_OK)
 #undef SBP_TILE_OK
     return false;
@@ -214,6 +218,7 @@ int smooth_pass(sbp_handle h, int l, const IN& in, const T* b, T* dst, const T* 
 // levels; above that the redundant halo work of the fused blocks costs more
 // than the passes it saves).
 inline bool fuse_level(const sbp_handle h, int l) {
+    if (h->p > 4) return false;   // the two-sweep kernels are compiled for p = 2, 4
     const long long pts = (long long)h->L[l].n * h->nprob;
     return pts <= (long long)h->prm.fuse_n * h->prm.fuse_n;
 }
@@ -238,8 +243,10 @@ int launch_bottom(sbp_handle h, const T* b, T* out, const T* dotv, const Scalars
     c.partial = partial;
     if (h->p == 2)
         hipLaunchKernelGGL((vcycle_bottom<2, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, b, out, dotv, c);
-    else
+    else if (h->p == 4)
         hipLaunchKernelGGL((vcycle_bottom<4, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, b, out, dotv, c);
+    else
+        return SBP_ERR_INVALID_ORDER;   // not compiled for p = 6, 8 (never selected for them)
     HIPCHK(hipGetLastError());
     return SBP_OK;
 }
@@ -898,7 +905,7 @@ int sbp_destroy(sbp_handle h) {
 int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle* out) {
     if (!out) return SBP_ERR_INVALID_ARG;
     *out = nullptr;
-    if (p != 2 && p != 4) return SBP_ERR_INVALID_ORDER;
+    if (p != 2 && p != 4 && p != 6 && p != 8) return SBP_ERR_INVALID_ORDER;
     {
         OpTab t0;
         if (!build_optab(p, N, &t0, nullptr)) return SBP_ERR_INVALID_SIZE;
@@ -919,6 +926,9 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
     h->p = p;
     h->nprob = nprob;
     if (prm) h->prm = *prm; else sbp_default_params(&h->prm);
+    // p = 6, 8 closures need at least 2 C = 18 / 24 points per side on every
+    // level: their coarsest grid is 33 x 33 at the least.
+    if (p >= 6 && h->prm.coarse_n < 33) h->prm.coarse_n = 33;
     if (!build_optab(p, N, &h->L[0].tb, &h->rt)) { delete h; return SBP_ERR_INVALID_SIZE; }
     h->L[0].N = N;
     h->L[0].n = (size_t)N * N;
@@ -942,7 +952,7 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
     h->nlev = nlev;
     // Bottom kernel: from the first level of side <= bottom_n down, when the
     // coarsest system is small enough for one workgroup's direct solve.
-    if (h->mg && h->L[nlev - 1].n <= (size_t)kBottomMaxCoarse) {
+    if (h->mg && p <= 4 && h->L[nlev - 1].n <= (size_t)kBottomMaxCoarse) {
         const int bn = std::min(h->prm.bottom_n, kBottomN);
         for (int l = 0; l < nlev; ++l)
             if (h->L[l].N <= bn) { h->bottom_l0 = l; break; }
@@ -952,7 +962,7 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
         Level& L = h->L[l];
         for (int q = 0; q < 2; ++q) {
             if (l == 0 && h->prm.tile_i > 0) {
-                if (!tile_ok(h->prm.tile_i, h->prm.tile_j)) { delete h; return SBP_ERR_INVALID_ARG; }
+                if (!tile_ok(p, h->prm.tile_i, h->prm.tile_j)) { delete h; return SBP_ERR_INVALID_ARG; }
                 L.ti[q] = h->prm.tile_i;
                 L.tj[q] = h->prm.tile_j;
             } else {
@@ -1002,12 +1012,18 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
             std::vector<double> A;
             dense_operator(Lc.tb, A);
             const int nc = (int)Lc.n;
-            if (!spd_inverse(A, nc)) { rc = SBP_ERR_BREAKDOWN; break; }
-            h->nc = nc;
-            if (hipMalloc(&h->Ainv, (size_t)nc * nc * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
-            if (hipMemcpy(h->Ainv, A.data(), (size_t)nc * nc * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-                rc = SBP_ERR_HIP;
-                break;
+            if (!spd_inverse(A, nc)) {
+                // Not positive definite (p = 8 with the reference's penalty):
+                // no multigrid; the apply and plain CG stay available.
+                h->mg = false;
+            } else {
+                h->nc = nc;
+                if (hipMalloc(&h->Ainv, (size_t)nc * nc * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+                if (hipMemcpy(h->Ainv, A.data(), (size_t)nc * nc * sizeof(double), hipMemcpyHostToDevice) !=
+                    hipSuccess) {
+                    rc = SBP_ERR_HIP;
+                    break;
+                }
             }
         }
     } while (0);
@@ -1042,7 +1058,7 @@ int sbp_set_tiles(sbp_handle h, int level, int dtype, int ti, int tj) {
     if (!h || level < 0 || level >= h->nlev) return SBP_ERR_INVALID_ARG;
     const bool fused = (dtype & SBP_FUSED) != 0;
     const int d = dtype & ~SBP_FUSED;
-    if ((d != SBP_F64 && d != SBP_F32) || (fused ? !fused_shape_ok(ti, tj) : !tile_ok(ti, tj)))
+    if ((d != SBP_F64 && d != SBP_F32) || (fused ? !fused_shape_ok(ti, tj) : !tile_ok(h->p, ti, tj)))
         return SBP_ERR_INVALID_ARG;
     Level& L = h->L[level];
     int* pti = fused ? &L.fti[d] : &L.ti[d];

@@ -51,7 +51,8 @@ int launch_smooth2_restrict(int p, int ti, int tj, const OpTab& tf, const T* b, 
 #define SBP_SR_CASE(TI_, J_)                                                                  \
     if (ti == TI_ && tj == J_) {                                                              \
         if (p == 2) sr_launch<2, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);  \
-        else sr_launch<4, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);         \
+        else if (p == 4) sr_launch<4, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st); \
+        else return 3;                                                                        \
         return hipGetLastError() == hipSuccess ? 0 : 4;                                       \
     }
     SBP_FUSED_SHAPES(SBP_SR_CASE)
@@ -73,7 +74,8 @@ int launch_prolong_smooth2(int p, int ti, int tj, const OpTab& tf, const T* x, c
 #define SBP_PS_CASE(TI_, J_)                                                                   \
     if (ti == TI_ && tj == J_) {                                                               \
         if (p == 2) ps_launch<2, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st); \
-        else ps_launch<4, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st);        \
+        else if (p == 4) ps_launch<4, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st); \
+        else return 3;                                                                         \
         return hipGetLastError() == hipSuccess ? 0 : 4;                                        \
     }
     SBP_FUSED_SHAPES(SBP_PS_CASE)

@@ -536,10 +536,9 @@ __global__ __launch_bounds__(256) void csr_fill(int32_t* rp, int32_t* col, doubl
             const int rr = N - 1 - r;
             for (int c = t.C - 1; c >= 0; --c) { double v = is_k ? t.dev->kb[rr][c] : t.dev->mb[rr][c]; if (v != 0.0) { cc[nn] = N - 1 - c; vv[nn++] = v; } }
         } else {
-            const int H = (t.C == 6) ? 2 : 1;
-            for (int s = -H; s <= H; ++s) {
-                double v = (s == 0) ? t.mid : (H == 1 ? -1.0 : ((s == 1 || s == -1) ? -4.0 / 3.0 : 1.0 / 12.0));
-                cc[nn] = r + s; vv[nn++] = v;
+            for (int s = -t.H; s <= t.H; ++s) {
+                cc[nn] = r + s;
+                vv[nn++] = t.cint[s < 0 ? -s : s];
             }
         }
     };

@@ -28,14 +28,16 @@
 
 namespace sbp {
 
-constexpr int kMaxR = 4;   // closure rows
-constexpr int kMaxC = 6;   // closure columns
+constexpr int kMaxR = 8;   // closure rows
+constexpr int kMaxC = 12;  // closure columns
 
 struct OpTab {
     int N;      // points per side
-    int R;      // closure rows of k and m (p=2: 3, p=4: 4)
-    int C;      // closure columns (p=2: 4, p=4: 6)
-    int RH;     // number of non-unit H weights (p=2: 1, p=4: 4)
+    int R;      // closure rows of k and m (p = 2, 4, 6, 8: 3, 4, 6, 8)
+    int C;      // closure columns (4, 6, 9, 12)
+    int RH;     // number of non-unit H weights (1, 4, 6, 8)
+    int H;      // half-width of the interior stencil (p / 2)
+    double cint[5];   // interior stencil of the 1-D factors: cint[s] at offsets +-s
     double hw[kMaxR];
     double mb[kMaxR][kMaxC];
     double kb[kMaxR][kMaxC];
@@ -71,16 +73,35 @@ struct Batch {
     int ss;
 };
 
-// Interior 1-D stencil s[-H..H] of -d.
+// Interior 1-D stencil s[-H..H] of -d (diagonal_sbp.jl:193, 201, 213, 227),
+// and the closure block width C of the order (columns of the first R rows).
 template <int P> struct Stencil;
 template <> struct Stencil<2> {
-    static constexpr int H = 1;
+    static constexpr int H = 1, C = 4;
     template <typename T> __device__ static T c(int t) { return t == 0 ? T(2) : T(-1); }
 };
 template <> struct Stencil<4> {
-    static constexpr int H = 2;
+    static constexpr int H = 2, C = 6;
     template <typename T> __device__ static T c(int t) {
         return t == 0 ? T(2.5) : ((t == 1 || t == -1) ? T(-4.0 / 3.0) : T(1.0 / 12.0));
+    }
+};
+template <> struct Stencil<6> {
+    static constexpr int H = 3, C = 9;
+    template <typename T> __device__ static T c(int t) {
+        const int a = t < 0 ? -t : t;
+        return a == 0 ? T(49.0 / 18.0) : a == 1 ? T(-3.0 / 2.0) : a == 2 ? T(3.0 / 20.0) : T(-1.0 / 90.0);
+    }
+};
+template <> struct Stencil<8> {
+    static constexpr int H = 4, C = 12;
+    template <typename T> __device__ static T c(int t) {
+        const int a = t < 0 ? -t : t;
+        return a == 0 ? T(205.0 / 72.0)
+             : a == 1 ? T(-8.0 / 5.0)
+             : a == 2 ? T(1.0 / 5.0)
+             : a == 3 ? T(-8.0 / 315.0)
+                      : T(1.0 / 560.0);
     }
 };
 
@@ -379,34 +400,36 @@ __device__ __forceinline__ T block_sum(T v) {
 // ---------------------------------------------------------------------------
 // Closure rows via direct (cache-served) loads, used only for face rows and
 // columns; uc is the operand at (i, j).
-template <typename T, bool LT = false, class IN>
+// (The closure width is the order's compile-time C, so the unrolled loops
+// carry no dead iterations for the narrower orders.)
+template <int P, typename T, bool LT = false, class IN>
 __device__ __forceinline__ T k_closure(const OpTab& t, const IN& in, int i, int j, T uc) {
+    constexpr int CW = Stencil<P>::C;
     const bool lo = j < t.R;
     const int r = lo ? j : t.N - 1 - j;
-    typename IN::Raw raw[kMaxC];
+    typename IN::Raw raw[CW];
 #pragma unroll
-    for (int c = 0; c < kMaxC; ++c)
-        if (c < t.C) raw[c] = in.template fetch<false, LT>(t, i, lo ? c : t.N - 1 - c);
+    for (int c = 0; c < CW; ++c) raw[c] = in.template fetch<false, LT>(t, i, lo ? c : t.N - 1 - c);
     T s = (T)lut<LT>(t).krs[r] * uc;
 #pragma unroll
-    for (int c = 0; c < kMaxC; ++c)
-        if (c < t.C && c != r)
+    for (int c = 0; c < CW; ++c)
+        if (c != r)
             s += (T)lut<LT>(t).kb[r][c] * (in.template make<false, LT>(t, i, lo ? c : t.N - 1 - c, raw[c]) - uc);
     return s;
 }
 
-template <typename T, bool LT = false, class IN>
+template <int P, typename T, bool LT = false, class IN>
 __device__ __forceinline__ T m_closure(const OpTab& t, const IN& in, int i, int j, T uc) {
+    constexpr int CW = Stencil<P>::C;
     const bool lo = i < t.R;
     const int r = lo ? i : t.N - 1 - i;
-    typename IN::Raw raw[kMaxC];
+    typename IN::Raw raw[CW];
 #pragma unroll
-    for (int c = 0; c < kMaxC; ++c)
-        if (c < t.C) raw[c] = in.template fetch<false, LT>(t, lo ? c : t.N - 1 - c, j);
+    for (int c = 0; c < CW; ++c) raw[c] = in.template fetch<false, LT>(t, lo ? c : t.N - 1 - c, j);
     T s = (T)lut<LT>(t).mrs[r] * uc;
 #pragma unroll
-    for (int c = 0; c < kMaxC; ++c)
-        if (c < t.C && c != r)
+    for (int c = 0; c < CW; ++c)
+        if (c != r)
             s += (T)lut<LT>(t).mb[r][c] * (in.template make<false, LT>(t, lo ? c : t.N - 1 - c, j, raw[c]) - uc);
     return s;
 }
@@ -499,7 +522,7 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
 #pragma unroll
             for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
         } else {
-            mp = m_closure<T, LT>(t, in, i, j0 + r, uc);
+            mp = m_closure<P, T, LT>(t, in, i, j0 + r, uc);
         }
         const T Au = ICHK ? hwi * kp + mp : kp + mp;   // hw_j = 1 on interior rows
         out.template emit<INT, LT>(t, i, j0 + r, gidx(i, j0 + r, N), uc, Au, aux[r], acc);
@@ -569,7 +592,7 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
 #pragma unroll
                 for (int s = 2; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
             } else {
-                kp = k_closure<T, LT>(t, in, i, j, uc);
+                kp = k_closure<P, T, LT>(t, in, i, j, uc);
             }
             if (iint) {
                 const T* row = &tile[r][tid + H];
@@ -577,7 +600,7 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
 #pragma unroll
                 for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
             } else {
-                mp = m_closure<T, LT>(t, in, i, j, uc);
+                mp = m_closure<P, T, LT>(t, in, i, j, uc);
             }
             const T Au = (iint ? kp : hwi * kp) + (jint ? mp : hw_of<T, LT>(t, j) * mp);
             out.template emit<false, LT>(t, i, j, gidx(i, j, N), uc, Au, aux[r], acc);
@@ -642,14 +665,14 @@ __device__ __forceinline__ T au_point(const OpTab& t, const IN& in, int i, int j
         for (int s = 1; s <= H; ++s)
             kp += Stencil<P>::template c<T>(s) * ((in.template load<false, LT>(t, i, j - s) - uc) + (in.template load<false, LT>(t, i, j + s) - uc));
     } else {
-        kp = k_closure<T, LT>(t, in, i, j, uc);
+        kp = k_closure<P, T, LT>(t, in, i, j, uc);
     }
     if (iint) {
 #pragma unroll
         for (int s = 1; s <= H; ++s)
             mp += Stencil<P>::template c<T>(s) * ((in.template load<false, LT>(t, i - s, j) - uc) + (in.template load<false, LT>(t, i + s, j) - uc));
     } else {
-        mp = m_closure<T, LT>(t, in, i, j, uc);
+        mp = m_closure<P, T, LT>(t, in, i, j, uc);
     }
     return (iint ? kp : hw_of<T, LT>(t, i) * kp) + (jint ? mp : hw_of<T, LT>(t, j) * mp);
 }
@@ -736,7 +759,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
 #pragma unroll
                 for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
             } else {
-                mp = act ? m_closure<T, true>(tl, in, fi, fj0 + r, uc) : T(0);
+                mp = act ? m_closure<P, T, true>(tl, in, fi, fj0 + r, uc) : T(0);
             }
             res[r] = act ? bq[r] - (hwi * kp + mp) : T(0);
         }

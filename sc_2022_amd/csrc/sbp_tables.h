@@ -1,7 +1,8 @@
 // sbp_tables.h -- host-side construction of the dimensionless 1-D factors.
 //
 // From the reference's diagonal-norm SBP second-derivative tables
-// (diagonal_sbp.jl:191-208: bhinv, interior d, closure bd, boundary row BS)
+// (diagonal_sbp.jl:191-311, orders p = 2, 4, 6, 8: bhinv, interior d,
+// closure bd, boundary row BS)
 // this builds, for a side of N points,
 //     hw = H / h                        (H = diag(h / bhinv) near the faces, h inside)
 //     m  = h (BS - H D2)                (the 1-D "stiffness", symmetric PSD, m 1 = 0)
@@ -10,7 +11,11 @@
 // equals  k (x) hw  +  hw (x) m  (first factor on the slow index j).
 // The Neumann SAT with tau_N = -1 cancels the BS term of the fast index exactly.
 // Only the first R rows / C columns differ from the interior stencil; they
-// are extracted from a small dense build and checked for shape.
+// are extracted from a small dense build and checked for shape.  With the
+// reference's tau = 13 the Dirichlet factor k is positive definite for
+// p <= 6 but not for p = 8 (its smallest eigenvalue is about -2.7; the
+// penalty would have to exceed about 800): the p = 8 operator is symmetric
+// indefinite, so its apply is exact but CG-type solves may break down.
 #pragma once
 #include <cmath>
 #include <cstring>
@@ -22,25 +27,33 @@ namespace sbp {
 
 struct Sbp1D {
     int p = 0, bm = 0, bn = 0, nbs = 0;
-    double bhinv[4] = {0, 0, 0, 0};
-    double d[5] = {0, 0, 0, 0, 0};       // interior D2 stencil (times h^2)
-    double bd[4][6] = {};                 // closure of D2 (times h^2)
-    double bs[4] = {0, 0, 0, 0};          // boundary derivative row (times h)
+    double bhinv[kMaxR] = {};
+    double d[9] = {};                    // interior D2 stencil (times h^2), p + 1 entries
+    double bd[kMaxR][kMaxC] = {};        // closure of D2 (times h^2), bm x bn
+    double bs[kMaxR] = {};               // boundary derivative row (times h), nbs entries
 };
 
+// The reference's rationals (diagonal_sbp.jl:191-197, 198-208, 210-222, 224-311).
 inline bool sbp_1d(int p, Sbp1D* s) {
-    std::memset(s, 0, sizeof(*s));
+    *s = Sbp1D{};
     s->p = p;
+    auto fill = [s](int bm, int bn, int nbs, const double* bh, const double* d, const double* bd, const double* bs) {
+        s->bm = bm; s->bn = bn; s->nbs = nbs;
+        for (int k = 0; k < bm; ++k) s->bhinv[k] = bh[k];
+        for (int k = 0; k <= s->p; ++k) s->d[k] = d[k];
+        for (int r = 0; r < bm; ++r)
+            for (int c = 0; c < bn; ++c) s->bd[r][c] = bd[r * bn + c];
+        for (int k = 0; k < nbs; ++k) s->bs[k] = bs[k];
+    };
     if (p == 2) {
-        s->bm = 1; s->bn = 3; s->nbs = 3;
-        s->bhinv[0] = 2.0;
+        const double bh[1] = {2.0};
         const double d[3] = {1.0, -2.0, 1.0};
-        for (int k = 0; k < 3; ++k) { s->d[k] = d[k]; s->bd[0][k] = d[k]; }
-        s->bs[0] = 3.0 / 2.0; s->bs[1] = -2.0; s->bs[2] = 1.0 / 2.0;
+        const double bd[1][3] = {{1.0, -2.0, 1.0}};
+        const double bs[3] = {3.0 / 2.0, -2.0, 1.0 / 2.0};
+        fill(1, 3, 3, bh, d, &bd[0][0], bs);
         return true;
     }
     if (p == 4) {
-        s->bm = 4; s->bn = 6; s->nbs = 4;
         const double bh[4] = {48.0 / 17.0, 48.0 / 59.0, 48.0 / 43.0, 48.0 / 49.0};
         const double d[5] = {-1.0 / 12.0, 4.0 / 3.0, -5.0 / 2.0, 4.0 / 3.0, -1.0 / 12.0};
         const double bd[4][6] = {
@@ -48,11 +61,38 @@ inline bool sbp_1d(int p, Sbp1D* s) {
             {1.0, -2.0, 1.0, 0.0, 0.0, 0.0},
             {-4.0 / 43.0, 59.0 / 43.0, -110.0 / 43.0, 59.0 / 43.0, -4.0 / 43.0, 0.0},
             {-1.0 / 49.0, 0.0, 59.0 / 49.0, -118.0 / 49.0, 64.0 / 49.0, -4.0 / 49.0}};
-        for (int k = 0; k < 4; ++k) s->bhinv[k] = bh[k];
-        for (int k = 0; k < 5; ++k) s->d[k] = d[k];
-        for (int r = 0; r < 4; ++r)
-            for (int c = 0; c < 6; ++c) s->bd[r][c] = bd[r][c];
-        s->bs[0] = 11.0 / 6.0; s->bs[1] = -3.0; s->bs[2] = 3.0 / 2.0; s->bs[3] = -1.0 / 3.0;
+        const double bs[4] = {11.0 / 6.0, -3.0, 3.0 / 2.0, -1.0 / 3.0};
+        fill(4, 6, 4, bh, d, &bd[0][0], bs);
+        return true;
+    }
+    if (p == 6) {
+        const double bh[6] = {43200.0 / 13649.0, 8640.0 / 12013.0, 4320.0 / 2711.0, 4320.0 / 5359.0, 8640.0 / 7877.0, 43200.0 / 43801.0};
+        const double d[7] = {1.0 / 90.0, -3.0 / 20.0, 3.0 / 2.0, -49.0 / 18.0, 3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0};
+        const double bd[6][9] = {
+            {114170.0 / 40947.0, -438107.0 / 54596.0, 336409.0 / 40947.0, -276997.0 / 81894.0, 3747.0 / 13649.0, 21035.0 / 163788.0, 0.0, 0.0, 0.0},
+            {6173.0 / 5860.0, -2066.0 / 879.0, 3283.0 / 1758.0, -303.0 / 293.0, 2111.0 / 3516.0, -601.0 / 4395.0, 0.0, 0.0, 0.0},
+            {-52391.0 / 81330.0, 134603.0 / 32532.0, -21982.0 / 2711.0, 112915.0 / 16266.0, -46969.0 / 16266.0, 30409.0 / 54220.0, 0.0, 0.0, 0.0},
+            {68603.0 / 321540.0, -12423.0 / 10718.0, 112915.0 / 32154.0, -75934.0 / 16077.0, 53369.0 / 21436.0, -54899.0 / 160770.0, 48.0 / 5359.0, 0.0, 0.0},
+            {-7053.0 / 39385.0, 86551.0 / 94524.0, -46969.0 / 23631.0, 53369.0 / 15754.0, -87904.0 / 23631.0, 820271.0 / 472620.0, -1296.0 / 7877.0, 96.0 / 7877.0, 0.0},
+            {21035.0 / 525612.0, -24641.0 / 131403.0, 30409.0 / 87602.0, -54899.0 / 131403.0, 820271.0 / 525612.0, -117600.0 / 43801.0, 64800.0 / 43801.0, -6480.0 / 43801.0, 480.0 / 43801.0}};
+        const double bs[5] = {25.0 / 12.0, -4.0, 3.0, -4.0 / 3.0, 1.0 / 4.0};
+        fill(6, 9, 5, bh, d, &bd[0][0], bs);
+        return true;
+    }
+    if (p == 8) {
+        const double bh[8] = {5080320.0 / 1498139.0, 725760.0 / 1107307.0, 80640.0 / 20761.0, 725760.0 / 1304999.0, 725760.0 / 299527.0, 80640.0 / 103097.0, 725760.0 / 670091.0, 5080320.0 / 5127739.0};
+        const double d[9] = {-1.0 / 560.0, 8.0 / 315.0, -1.0 / 5.0, 8.0 / 5.0, -205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0};
+        const double bd[8][12] = {
+            {4870382994799.0 / 1358976868290.0, -893640087518.0 / 75498714905.0, 926594825119.0 / 60398971924.0, -1315109406200.0 / 135897686829.0, 39126983272.0 / 15099742981.0, 12344491342.0 / 75498714905.0, -451560522577.0 / 2717953736580.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+            {333806012194.0 / 390619153855.0, -154646272029.0 / 111605472530.0, 1168338040.0 / 33481641759.0, 82699112501.0 / 133926567036.0, -171562838.0 / 11160547253.0, -28244698346.0 / 167408208795.0, 11904122576.0 / 167408208795.0, -2598164715.0 / 312495323084.0, 0.0, 0.0, 0.0, 0.0},
+            {7838984095.0 / 52731029988.0, 1168338040.0 / 5649753213.0, -88747895.0 / 144865467.0, 423587231.0 / 627750357.0, -43205598281.0 / 22599012852.0, 4876378562.0 / 1883251071.0, -5124426509.0 / 3766502142.0, 10496900965.0 / 39548272491.0, 0.0, 0.0, 0.0, 0.0},
+            {-94978241528.0 / 828644350023.0, 82699112501.0 / 157837019052.0, 1270761693.0 / 13153084921.0, -167389605005.0 / 118377764289.0, 48242560214.0 / 39459254763.0, -31673996013.0 / 52612339684.0, 43556319241.0 / 118377764289.0, -44430275135.0 / 552429566682.0, 0.0, 0.0, 0.0, 0.0},
+            {1455067816.0 / 21132528431.0, -171562838.0 / 3018932633.0, -43205598281.0 / 36227191596.0, 48242560214.0 / 9056797899.0, -52276055645.0 / 6037865266.0, 57521587238.0 / 9056797899.0, -80321706377.0 / 36227191596.0, 8078087158.0 / 21132528431.0, -1296.0 / 299527.0, 0.0, 0.0, 0.0},
+            {10881504334.0 / 327321118845.0, -28244698346.0 / 140280479505.0, 4876378562.0 / 9352031967.0, -10557998671.0 / 12469375956.0, 57521587238.0 / 28056095901.0, -278531401019.0 / 93520319670.0, 73790130002.0 / 46760159835.0, -137529995233.0 / 785570685228.0, 2048.0 / 103097.0, -144.0 / 103097.0, 0.0, 0.0},
+            {-135555328849.0 / 8509847458140.0, 11904122576.0 / 101307707835.0, -5124426509.0 / 13507694378.0, 43556319241.0 / 60784624701.0, -80321706377.0 / 81046166268.0, 73790130002.0 / 33769235945.0, -950494905688.0 / 303923123505.0, 239073018673.0 / 141830790969.0, -145152.0 / 670091.0, 18432.0 / 670091.0, -1296.0 / 670091.0, 0.0},
+            {0.0, -2598164715.0 / 206729925524.0, 10496900965.0 / 155047444143.0, -44430275135.0 / 310094888286.0, 425162482.0 / 2720130599.0, -137529995233.0 / 620189776572.0, 239073018673.0 / 155047444143.0, -144648000000.0 / 51682481381.0, 8128512.0 / 5127739.0, -1016064.0 / 5127739.0, 129024.0 / 5127739.0, -9072.0 / 5127739.0}};
+        const double bs[7] = {4723.0 / 2100.0, -839.0 / 175.0, 157.0 / 35.0, -278.0 / 105.0, 103.0 / 140.0, 1.0 / 175.0, -6.0 / 175.0};
+        fill(8, 12, 7, bh, d, &bd[0][0], bs);
         return true;
     }
     return false;
@@ -97,14 +137,23 @@ inline void dense_factors(const Sbp1D& s, int n, std::vector<double>& hw, std::v
 inline bool build_optab(int p, int N, OpTab* t, RhsTab* rt) {
     Sbp1D s;
     if (!sbp_1d(p, &s)) return false;
-    const int R = p == 2 ? 3 : 4;
-    const int C = p == 2 ? 4 : 6;
+    // closure rows / columns of the factors (rows below R are the interior stencil)
+    const int R = p == 2 ? 3 : p;
+    const int C = p == 2 ? 4 : p == 4 ? 6 : p == 6 ? 9 : 12;
     if (N < 2 * C || N < 2 * s.bm || N < s.bn) return false;
     const int n0 = 2 * C + 4;
     std::vector<double> hw, m, k;
     dense_factors(s, n0, hw, m, k);
     std::memset(t, 0, sizeof(*t));
-    t->N = N; t->R = R; t->C = C; t->RH = s.bm;
+    t->N = N; t->R = R; t->C = C; t->RH = s.bm; t->H = p / 2;
+    for (int q = 0; q <= p / 2; ++q) t->cint[q] = -s.d[p / 2 + q];
+    // every row from R on is the interior stencil (checked on the dense build)
+    for (int r = R; r < n0 - R; ++r)
+        for (int c = 0; c < n0; ++c) {
+            const int o = c - r < 0 ? r - c : c - r;
+            const double want = o <= p / 2 ? t->cint[o] : 0.0;
+            if (std::fabs(m[r * n0 + c] - want) > 1e-12 || std::fabs(k[r * n0 + c] - want) > 1e-12) return false;
+        }
     for (int r = 0; r < s.bm; ++r) t->hw[r] = hw[r];
     for (int r = 0; r < R; ++r) {
         for (int c = 0; c < C; ++c) {
@@ -146,18 +195,15 @@ inline void dense_operator(const OpTab& t, std::vector<double>& A) {
         if (r >= N - t.RH) return t.hw[N - 1 - r];
         return 1.0;
     };
-    const int H = t.C == 6 ? 2 : 1;
+    const int H = t.H;
     auto coef = [&](bool is_k, int r, int c) -> double {
         if (r < t.R) return (c < t.C) ? (is_k ? t.kb[r][c] : t.mb[r][c]) : 0.0;
         if (r >= N - t.R) {
             const int rr = N - 1 - r, cc = N - 1 - c;
             return (cc >= 0 && cc < t.C) ? (is_k ? t.kb[rr][cc] : t.mb[rr][cc]) : 0.0;
         }
-        const int s = c - r;
-        if (s < -H || s > H) return 0.0;
-        if (s == 0) return t.mid;
-        if (H == 1) return -1.0;
-        return (s == 1 || s == -1) ? -4.0 / 3.0 : 1.0 / 12.0;
+        const int s = c - r < 0 ? r - c : c - r;
+        return s <= H ? t.cint[s] : 0.0;
     };
     for (int j = 0; j < N; ++j)
         for (int i = 0; i < N; ++i) {
@@ -186,19 +232,30 @@ inline bool spd_inverse(std::vector<double>& A, int n) {
             L[(size_t)i * n + j] = v / ljj;
         }
     }
-    // Solve L L^T X = I column by column.
-    std::vector<double> X((size_t)n * n, 0.0), y(n);
-    for (int c = 0; c < n; ++c) {
-        for (int i = 0; i < n; ++i) {
-            double v = (i == c) ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) v -= L[(size_t)i * n + k] * y[k];
-            y[i] = v / L[(size_t)i * n + i];
+    // Solve L L^T X = I for all columns at once, a row at a time (the inner
+    // loops run along contiguous rows): Y = L^-1 (lower triangular, so row i
+    // of Y has entries in columns <= i only), then X = L^-T Y.
+    std::vector<double> X((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double* yi = &X[(size_t)i * n];
+        yi[i] = 1.0;
+        for (int k = 0; k < i; ++k) {
+            const double l = L[(size_t)i * n + k];
+            const double* yk = &X[(size_t)k * n];
+            for (int c = 0; c <= k; ++c) yi[c] -= l * yk[c];
         }
-        for (int i = n - 1; i >= 0; --i) {
-            double v = y[i];
-            for (int k = i + 1; k < n; ++k) v -= L[(size_t)k * n + i] * X[(size_t)k * n + c];
-            X[(size_t)i * n + c] = v / L[(size_t)i * n + i];
+        const double inv = 1.0 / L[(size_t)i * n + i];
+        for (int c = 0; c <= i; ++c) yi[c] *= inv;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double* xi = &X[(size_t)i * n];
+        for (int k = i + 1; k < n; ++k) {
+            const double l = L[(size_t)k * n + i];
+            const double* xk = &X[(size_t)k * n];
+            for (int c = 0; c < n; ++c) xi[c] -= l * xk[c];
         }
+        const double inv = 1.0 / L[(size_t)i * n + i];
+        for (int c = 0; c < n; ++c) xi[c] *= inv;
     }
     // Symmetrise (the inverse of a symmetric matrix).
     for (int i = 0; i < n; ++i)

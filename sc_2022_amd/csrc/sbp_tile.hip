@@ -1,5 +1,6 @@
 // sbp_tile.hip -- instantiates the stencil pass for one tile shape
-// (compile with -DSBP_TI=<threads along i> -DSBP_TJ=<rows per thread>).
+// (compile with -DSBP_TI=<threads along i> -DSBP_TJ=<rows per thread>; with
+// -DSBP_HI=1 also for the orders p = 6 and 8).
 #include "sbp_launch.h"
 
 #ifndef SBP_TI
@@ -13,10 +14,15 @@ int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, PassCtl c,
     c.nbx = (tb.N + TI - 1) / TI;
     c.nby = (tb.N + J - 1) / J;
     const dim3 g(c.nbx * c.nby * c.bt.nprob);
-    if (p == 2)
-        hipLaunchKernelGGL((stencil_pass<2, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c);
-    else
-        hipLaunchKernelGGL((stencil_pass<4, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c);
+    switch (p) {
+    case 2: hipLaunchKernelGGL((stencil_pass<2, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
+    case 4: hipLaunchKernelGGL((stencil_pass<4, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
+#if SBP_HI
+    case 6: hipLaunchKernelGGL((stencil_pass<6, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
+    case 8: hipLaunchKernelGGL((stencil_pass<8, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
+#endif
+    default: return 3;   // order not compiled for this tile shape
+    }
     return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
@@ -31,10 +37,15 @@ int launch_resid_restrict(int p, const OpTab& tf, const T* u, const T* b, T* rc,
     c.nby = (Nc + J / 2 - 1) / (J / 2);
     c.bt = bt;
     const dim3 g(c.nbx * c.nby * bt.nprob);
-    if (p == 2)
-        hipLaunchKernelGGL((resid_restrict<2, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c);
-    else
-        hipLaunchKernelGGL((resid_restrict<4, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c);
+    switch (p) {
+    case 2: hipLaunchKernelGGL((resid_restrict<2, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c); break;
+    case 4: hipLaunchKernelGGL((resid_restrict<4, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c); break;
+#if SBP_HI
+    case 6: hipLaunchKernelGGL((resid_restrict<6, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c); break;
+    case 8: hipLaunchKernelGGL((resid_restrict<8, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c); break;
+#endif
+    default: return 3;
+    }
     return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 

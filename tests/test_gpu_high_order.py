@@ -1,0 +1,188 @@
+"""GPU parity of the p = 6 and p = 8 operators (diagonal_sbp.jl:210-311)
+against the CPU oracle: the apply (both signs, FP64 and FP32, every tile
+shape compiled for these orders, ragged sides), diag, the right-hand side,
+the V-cycle and the solvers for p = 6 (SPD), and the defined behaviour of
+the solvers on the indefinite p = 8 operator."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+import torch
+
+from oracle import sbp_oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def T(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def relmax(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+_A = {}
+
+
+def A_of(level, p):
+    if (level, p) not in _A:
+        _A[(level, p)] = O.assemble_operator(level, p)
+    return _A[(level, p)]
+
+
+@pytest.fixture(scope="module")
+def SP():
+    from sc_2022_amd.solver import SbpPoisson
+    return SbpPoisson
+
+
+@pytest.mark.parametrize("p", [6, 8])
+@pytest.mark.parametrize("level", [5, 6, 7, 8])
+def test_apply_matches_assembled(SP, level, p):
+    N = 2 ** level + 1
+    h = SP(N, p)
+    u = np.random.default_rng(level * 10 + p).standard_normal(N * N)
+    Au = A_of(level, p) @ u
+    assert relmax(h.apply(T(u)).cpu().numpy(), Au) < 1e-13
+    assert relmax(h.apply(T(u), sign=-1).cpu().numpy(), -Au) < 1e-13
+    assert relmax(h.diag().cpu().numpy(), A_of(level, p).diagonal()) < 1e-15
+    uf = u.astype(np.float32)
+    out = h.apply(T(uf, torch.float32)).cpu().double().numpy()
+    assert relmax(out, A_of(level, p) @ uf.astype(np.float64)) < 2e-5
+
+
+@pytest.mark.parametrize("p", [6, 8])
+@pytest.mark.parametrize("tiles", [(64, 4), (128, 8), (256, 8)])
+def test_apply_every_compiled_tile_shape(SP, p, tiles):
+    level = 7
+    N = 2 ** level + 1
+    h = SP(N, p, tile_i=tiles[0], tile_j=tiles[1])
+    u = np.random.default_rng(3).standard_normal(N * N)
+    assert relmax(h.apply(T(u)).cpu().numpy(), A_of(level, p) @ u) < 1e-13
+
+
+def test_uncompiled_tile_shape_rejected(SP):
+    with pytest.raises(Exception):
+        SP(129, 6, tile_i=64, tile_j=16)
+    h = SP(129, 8)
+    with pytest.raises(Exception):
+        h.set_tiles(0, 256, 16)
+
+
+@pytest.mark.parametrize("p", [6, 8])
+@pytest.mark.parametrize("N", [24, 25, 37, 100, 300])
+def test_apply_ragged_sides_against_closed_form(SP, N, p):
+    if N < (18 if p == 6 else 24):
+        pytest.skip("below the closure minimum 2C")
+    hw, m, k = O.one_d_factors(p, N)
+    A = sp.kron(sp.csr_matrix(k), sp.diags(hw)) + sp.kron(sp.diags(hw), sp.csr_matrix(m))
+    h = SP(N, p)
+    u = np.random.default_rng(N).standard_normal(N * N)
+    assert relmax(h.apply(T(u)).cpu().numpy(), A @ u) < 1e-13
+
+
+@pytest.mark.parametrize("p", [6, 8])
+def test_apply_1025_properties(SP, p):
+    """1025^2: <Au, v> = <u, Av>, and A 1 = hw_i (k 1)_j in closed form."""
+    N = 1025
+    h = SP(N, p)
+    g = torch.Generator(device=DEV).manual_seed(9)
+    u = torch.randn(N * N, dtype=torch.float64, device=DEV, generator=g)
+    v = torch.randn(N * N, dtype=torch.float64, device=DEV, generator=g)
+    a, b = float(h.apply(u) @ v), float(u @ h.apply(v))
+    assert abs(a - b) < 1e-13 * float(u.norm()) * float(v.norm()) * 10
+    A1 = h.apply(torch.ones(N * N, dtype=torch.float64, device=DEV)).reshape(N, N).cpu().numpy()
+    hw, m, k = O.one_d_factors(p, 64)
+    k1 = k.sum(axis=1)
+    hwN = np.ones(N); hwN[:p] = hw[:p]; hwN[-p:] = hw[-p:]
+    kN = np.zeros(N); kN[:16] = k1[:16]; kN[-16:] = k1[-16:]
+    assert np.abs(A1 - kN[:, None] * hwN[None, :]).max() < 1e-11
+
+
+@pytest.mark.parametrize("p", [6, 8])
+def test_rhs_matches_oracle(SP, p):
+    level = 6
+    N = 2 ** level + 1
+    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    h = SP(N, p)
+    b = h.assemble_rhs(T(f), T(gW), T(gE), T(gS), T(gN)).cpu().numpy()
+    assert relmax(b, O.assemble_rhs(level, p, f, gW, gE, gS, gN)) < 1e-13
+
+
+def test_vcycle_p6_matches_oracle(SP):
+    """The p = 6 V-cycle (separate passes, coarsest grid 33^2) against the
+    oracle's hierarchy with the same smoother and transfers."""
+    level = 7
+    N = 2 ** level + 1
+    h = SP(N, 6)
+    assert h.nlevels == 3                       # 129, 65, 33
+    hier = O.Hierarchy(level, 6, coarse_level=5)
+    r = np.random.default_rng(4).standard_normal(N * N)
+    z = h.vcycle(T(r)).cpu().numpy()
+    assert relmax(z, O.vcycle(hier, r)) < 1e-11
+
+
+@pytest.mark.parametrize("N", [65, 129])
+def test_cg_p6_against_oracle(SP, N):
+    level = int(math.log2(N - 1))
+    A = A_of(level, 6)
+    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    b = O.assemble_rhs(level, 6, f, gW, gE, gS, gN)
+    tol = 1e-8 * np.linalg.norm(b)
+    xo, ito, _ = O.cg_cpu(A, b, abstol=tol)
+    h = SP(N, 6)
+    x, res = h.cg(T(b), abstol=tol)
+    # hundreds of iterations amplify the different summation orders
+    # (difference form on the GPU, assembled rows in the oracle)
+    assert res.converged and abs(res.iters - ito) <= max(2, 0.01 * ito), (res.iters, ito)
+    assert np.linalg.norm(A @ x.cpu().numpy() - b) < 1.000001 * tol
+
+
+def test_mgpcg_p6_against_direct_and_rate(SP):
+    """MG-PCG (FP64 and mixed precision) for p = 6 to round-off against the
+    sparse direct solve, and the manufactured H-norm error rate on the GPU."""
+    errs = []
+    for level in (5, 6, 7):
+        N = 2 ** level + 1
+        A = A_of(level, 6)
+        f, gW, gE, gS, gN, u = O.baseline_data(level)
+        b = O.assemble_rhs(level, 6, f, gW, gE, gS, gN)
+        h = SP(N, 6)
+        x, res = h.mgpcg(T(b), abstol=1e-13 * np.linalg.norm(b))
+        assert res.converged and res.iters <= 40, res
+        xg = x.cpu().numpy()
+        if level == 6:
+            xd = spla.spsolve(A.tocsc(), b)
+            assert np.linalg.norm(xg - xd) < 1e-10 * np.linalg.norm(xd)
+            xm, rm = h.mgpcg_mixed(T(b), abstol=1e-11 * np.linalg.norm(b))
+            assert rm.converged
+            assert np.linalg.norm(A @ xm.cpu().numpy() - b) < 1.000001e-11 * np.linalg.norm(b)
+        errs.append(O.h_norm(xg - u, O.h_tilde_diag(level, 6)))
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(2)]
+    assert all(5.2 < r < 5.8 for r in rates), (errs, rates)
+
+
+def test_p8_solvers_report_a_defined_status(SP):
+    """p = 8 with the reference's penalty is indefinite: the solvers must end
+    with a defined status, and a reported convergence must be real."""
+    from sc_2022_amd import _capi as capi
+    level = 6
+    N = 2 ** level + 1
+    A = A_of(level, 8)
+    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    b = O.assemble_rhs(level, 8, f, gW, gE, gS, gN)
+    h = SP(N, 8)
+    ok = (capi.SBP_OK, capi.SBP_ERR_BREAKDOWN, capi.SBP_ERR_NOT_CONVERGED, capi.SBP_ERR_NONFINITE)
+    tol = 1e-8 * np.linalg.norm(b)
+    x, res = h.cg(T(b), abstol=tol, maxiter=2000, raise_on_fail=False)
+    assert res.status in ok, res
+    if res.status == capi.SBP_OK:
+        assert np.linalg.norm(A @ x.cpu().numpy() - b) < 1.000001 * tol
+    # the coarsest operator is not positive definite: no multigrid hierarchy
+    x, res = h.mgpcg(T(b), abstol=tol, maxiter=50, raise_on_fail=False)
+    assert res.status == capi.SBP_ERR_INVALID_SIZE

@@ -16,13 +16,13 @@ import scipy.sparse.linalg as spla
 from oracle import sbp_oracle as O
 
 
-@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("p", [2, 4, 6, 8])
 def test_d2_polynomial_exactness(p):
     N = 40
     D2, S0, SN, HI, H, r = O.diagonal_sbp_D2(p, N, xc=(0, 1))
     D2 = D2.toarray()
-    bm = 1 if p == 2 else 4
-    kb = 2 if p == 2 else 3          # boundary accuracy degree
+    bm = {2: 1, 4: 4, 6: 6, 8: 8}[p]
+    kb = p // 2 + 1                  # boundary accuracy degree
     for k in range(0, p + 2):
         v = r ** k
         exact = k * (k - 1) * r ** (k - 2) if k >= 2 else np.zeros_like(r)
@@ -32,7 +32,7 @@ def test_d2_polynomial_exactness(p):
             assert err.max() < 1e-7 * max(1, k ** 2), (p, k)
 
 
-@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("p", [2, 4, 6, 8])
 def test_d2_sbp_property(p):
     """H D2 = -M + BS with M symmetric positive semidefinite and M 1 = 0
     (diagonal_sbp.jl:159-160)."""
@@ -51,7 +51,7 @@ def test_d2_sbp_property(p):
     assert abs((BS @ r ** 2)[0] - 0.0) < 1e-9 and abs((BS @ r ** 2)[-1] - 2.0) < 1e-9
 
 
-@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("p", [2, 4, 6, 8])
 def test_d1_sbp_property(p):
     N = 30
     D, HI, H, r = O.diagonal_sbp_D1(p, N, xc=(0, 1))
@@ -86,6 +86,43 @@ def test_operator_spd_and_closed_form(level, p):
     # A commutes with the 180-degree reversal of the grid
     R = np.eye(A.shape[0])[::-1]
     assert np.abs(R @ Ad @ R - Ad).max() < 1e-13 * np.abs(Ad).max()
+
+
+@pytest.mark.parametrize("p", [6, 8])
+@pytest.mark.parametrize("level", [4, 5])
+def test_high_order_operator_closed_form_and_definiteness(level, p):
+    """p = 6, 8 (diagonal_sbp.jl:210-311): symmetric, equal to the closed
+    form, 180-degree symmetric; positive definite for p = 6, but indefinite
+    for p = 8 with the reference's Dirichlet penalty tau = 13/h (the 1-D
+    Dirichlet factor needs tau above ~800 for p = 8)."""
+    A = O.assemble_operator(level, p)
+    Ad = A.toarray()
+    assert np.abs(Ad - Ad.T).max() < 1e-13 * np.abs(Ad).max()
+    assert abs(A - O.closed_form_operator(level, p)).max() < 1e-13 * abs(A).max()
+    R = np.eye(A.shape[0])[::-1]
+    assert np.abs(R @ Ad @ R - Ad).max() < 1e-13 * np.abs(Ad).max()
+    ev = np.linalg.eigvalsh(Ad).min()
+    if p == 6:
+        assert ev > 0
+    else:
+        assert ev < -1.0
+        hw, m, k = O.one_d_factors(8, 33)
+        E = np.zeros((33, 33)); E[0, 0] = E[-1, -1] = 1
+        assert np.linalg.eigvalsh(k + (1000 - 13) * E).min() > 0
+
+
+def test_high_order_manufactured_convergence():
+    """p = 6: H-norm error of the direct solve falls at about 5.5 per
+    halving (boundary order 3 + 2)."""
+    errs = []
+    for level in (5, 6, 7, 8):
+        A = O.assemble_operator(level, 6)
+        f, gW, gE, gS, gN, u = O.baseline_data(level)
+        b = O.assemble_rhs(level, 6, f, gW, gE, gS, gN)
+        x = spla.spsolve(A.tocsc(), b)
+        errs.append(O.h_norm(x - u, O.h_tilde_diag(level, 6)))
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(3)]
+    assert all(5.2 < r < 5.8 for r in rates), rates
 
 
 @pytest.mark.parametrize("level", [3, 4, 5])
