@@ -48,7 +48,7 @@ def parse():
                     help="mixed: relative tolerance of each FP32 inner solve")
     ap.add_argument("--ir-rtol", type=float, default=0.0,
                     help="mixed: also stop when the correction-based error estimate <= ir_rtol ||x||")
-    ap.add_argument("--no-extra", action="store_true", help="skip the 8193^2, CSR comparison and initial-guess CG legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 8193^2, CSR comparison, initial-guess CG and reference-algorithm legs")
     ap.add_argument("--tol-sweep", action="store_true",
                     help="print iterations, error and time per stopping rule for both legs, then exit")
     ap.add_argument("--apply-reps", type=int, default=200)
@@ -255,6 +255,61 @@ def nested_leg(a, dev, torch, N=1025, p=2, rtol=1e-10):
     return out
 
 
+def reference_modes_leg(a, dev, torch, N=1025, p=2):
+    """The reference's own algorithms on this hardware, beside the build's:
+    MGCG with the reference's preconditioner (two-level, 3 + 3 Richardson
+    sweeps, exact coarse solve) against the multi-level MG-PCG, both to
+    ||r|| < 1e-10 ||b||; and the Poisson_CG.jl-style efficiency report of
+    plain CG (to 1e-8 ||b||), matrix-free and CSR: time in the operator
+    products (iterations x one product) over the whole solve, and the rest as
+    overhead."""
+    from sc_2022_amd import reference_api as R
+    from sc_2022_amd.data import manufactured_fields
+    from sc_2022_amd.solver import SbpPoisson
+    href = SbpPoisson(N, p, device=dev, **R.reference_mg_params(N, 2))
+    h = SbpPoisson(N, p, device=dev)
+    f, gW, gE, gS, gN, u = manufactured_fields(N, dev)
+    b = h.assemble_rhs(f, gW, gE, gS, gN)
+    bn = float(torch.linalg.vector_norm(b))
+    x = torch.zeros_like(b)
+
+    def timed(fn, reps=3):
+        r = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t0) / reps * 1e3
+
+    out = {"workload": f"N={N}, p={p}, u = sin(pi x) sin(pi y) + x y"}
+    for name, hh in (("mgcg_reference_two_level", href), ("mgpcg_multilevel", h)):
+        r, ms = timed(lambda: hh.mgpcg(b, x.zero_(), abstol=1e-10 * bn, maxiter=500)[1])
+        out[name] = {"ms": ms, "iters": r.iters, "converged": r.converged}
+    out["mgcg_speedup_multilevel_vs_reference"] = out["mgcg_reference_two_level"]["ms"] / out["mgpcg_multilevel"]["ms"]
+    # efficiency report
+    apply_ms = time_apply(h, torch, torch.float64, 200, 8)
+    r, cg_ms = timed(lambda: h.cg(b, x.zero_(), abstol=1e-8 * bn)[1], reps=2)
+    csr = h.csr_assemble()
+    y = torch.empty_like(b)
+    for _ in range(3):
+        h.csr_spmv(csr, b, y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(50):
+        h.csr_spmv(csr, b, y)
+    torch.cuda.synchronize()
+    spmv_ms = (time.perf_counter() - t0) / 50 * 1e3
+    rc, csr_ms = timed(lambda: h.csr_cg(csr, b, x.zero_(), abstol=1e-8 * bn)[1], reps=2)
+    for name, it, tot, op in (("cg_matrix_free", r.iters, cg_ms, apply_ms), ("cg_csr", rc.iters, csr_ms, spmv_ms)):
+        t_op = it * op
+        out[name] = {"iters": it, "ms": tot, "op_ms_each": op, "op_ms_in_cg": t_op,
+                     "overhead_ms": tot - t_op, "efficiency": t_op / tot}
+    href.close()
+    h.close()
+    return out
+
+
 def tol_sweep(a, h, b, x, bnorm, rel_err, torch):
     """Iterations, relative H-norm error and time of both legs over a grid of
     stopping rules (one JSON line each), to pick the loosest rule per leg
@@ -390,7 +445,8 @@ def main():
     if not a.no_extra:
         for name, fn in (("csr_spmv", lambda: csr_leg(h, torch, apply_ms64 * 1e3)),
                          ("n8193", lambda: leg_8193(a, dev, torch)),
-                         ("initial_guess_cg", lambda: nested_leg(a, dev, torch))):
+                         ("initial_guess_cg", lambda: nested_leg(a, dev, torch)),
+                         ("reference_algorithms", lambda: reference_modes_leg(a, dev, torch))):
             try:
                 extra[name] = fn()
             except Exception as e:  # reported, never silently replaced

@@ -23,6 +23,8 @@ operator, literally (same kron products, same SAT terms):
   levels to a hierarchy down to 17x17 with damped-Jacobi smoothing).
 * ``initial_guess_interpolation_cg`` -- initial_guess_interpolation_CG and its
   three-level form, initial_guess_cg.jl:394-399, 447-456.
+* ``two_level_richardson`` / ``matrix_free_mgcg`` -- the reference's own
+  preconditioner and MGCG, initial_guess_cg.jl:147-180, 303-346.
 
 Grid convention (the reference's): level k gives N = 2^k + 1 points per side
 on [0,1]^2, h = 1/2^k.  A grid function is a flat vector with element (i, j)
@@ -677,6 +679,52 @@ def bilinear_data(level, c=(0.5, -1.0, 2.0, 3.0)):
     gS = -(c[2] + c[3] * t)
     gN = c[2] + c[3] * t
     return f, gW, gE, gS, gN, u
+
+
+def two_level_richardson(A, A2h, b, nu=3, omega=0.15):
+    """matrix_free_Two_level_multigrid (initial_guess_cg.jl:147-180) on the
+    SPD A (the reference's -A kernel and sign flips cancel): nu Richardson
+    sweeps from zero, residual, full weighting (no scaling), exact coarse
+    solve with the rediscretised A_2h, bilinear prolongation, nu sweeps.
+    Returns (z, ||A z - b||)."""
+    N = int(round(math.sqrt(len(b))))
+    N2 = (N + 1) // 2
+    v = modified_richardson(A, np.zeros_like(b), b, nu, omega)
+    f = restriction_2d(N) @ (b - A @ v)
+    v = v + prolongation_2d(N2) @ spla.spsolve(sp.csc_matrix(A2h), f)
+    v = modified_richardson(A, v, b, nu, omega)
+    return v, float(np.linalg.norm(A @ v - b))
+
+
+def matrix_free_mgcg(A, b, precond, x=None, abstol=None, maxiter=None):
+    """matrix_free_MGCG (initial_guess_cg.jl:303-346) with a given
+    preconditioner z = precond(r); stops when ||r||_2 < abstol.
+    Returns (x, iterations, norms)."""
+    b = np.asarray(b, dtype=np.float64)
+    x = np.zeros_like(b) if x is None else np.array(x, dtype=np.float64)
+    abstol = math.sqrt(np.finfo(np.float64).eps) if abstol is None else abstol
+    maxiter = len(b) if maxiter is None else maxiter
+    r = b - A @ x
+    z = precond(r)
+    p = z.copy()
+    rz = r @ z
+    norms = [float(np.linalg.norm(r))]
+    it = 0
+    for _ in range(maxiter):
+        it += 1
+        Ap = A @ p
+        alpha = rz / (p @ Ap)
+        x += alpha * p
+        r -= alpha * Ap
+        rs = r @ r
+        norms.append(math.sqrt(rs))
+        if math.sqrt(rs) < abstol:
+            break
+        z = precond(r)
+        rznew = r @ z
+        p = z + (rznew / rz) * p
+        rz = rznew
+    return x, it, norms
 
 
 def h_norm(v, hdiag):

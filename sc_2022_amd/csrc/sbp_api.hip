@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/sbp.h"
@@ -84,6 +85,13 @@ struct sbp_handle_s {
     // both made on first use.
     sbp_handle child = nullptr;
     double* xc = nullptr;
+    // Coarsest grids above 33 x 33 (coarse_n > 33, e.g. the reference's
+    // two-level cycle): solved "exactly" by a nested FP64 MG-PCG on a handle
+    // of that size (default parameters), made on first use; cbuf stages
+    // FP32 right-hand sides and solutions as FP64.
+    bool coarse_nested = false;
+    sbp_handle cexact = nullptr;
+    double* cbuf = nullptr;
 };
 
 namespace {
@@ -251,6 +259,60 @@ int launch_bottom(sbp_handle h, const T* b, T* out, const T* dotv, const Scalars
     return SBP_OK;
 }
 
+constexpr double kExactRtol = 1e-13;   // "exact" solves: ||r|| < 1e-13 ||b||
+constexpr int kExactMaxiter = 100;
+
+// x = A^-1 b on handle c, for the reference's sparse LU solve A_2h \ b_2h:
+// MG-PCG from zero to ||r|| < 1e-13 ||b|| (round-off level for FP64).  An
+// iterate short of that after kExactMaxiter iterations is still returned as
+// the guess; HIP errors, NaN/Inf and breakdown are reported.
+int exact_solve(sbp_handle c, const double* b, double* x) {
+    HIPCHK(hipMemsetAsync(x, 0, c->L[0].n * sizeof(double), c->st));
+    double bb = 0.0;
+    STCHK(sbp_dot(c, b, b, &bb));
+    if (!std::isfinite(bb)) return SBP_ERR_NONFINITE;
+    if (bb == 0.0) return SBP_OK;
+    int it = 0;
+    double rn = 0.0;
+    const int rc = sbp_mgpcg(c, b, x, kExactRtol * std::sqrt(bb), kExactMaxiter, &it, &rn, nullptr);
+    return rc == SBP_ERR_NOT_CONVERGED ? SBP_OK : rc;
+}
+
+// Coarsest level above 33 x 33: out = A_c^-1 b by exact_solve on the nested
+// handle (one problem; FP32 data is staged through FP64), and with dotv the
+// single partial dotv' out.
+template <typename T>
+int coarse_exact(sbp_handle h, const T* b, T* out, const T* dotv, double* partial) {
+    const Level& L = h->L[h->nlev - 1];
+    if (h->nprob != 1) return SBP_ERR_INVALID_ARG;
+    if (!h->cexact) {
+        sbp_params cp;
+        sbp_default_params(&cp);
+        STCHK(sbp_create(L.N, h->p, &cp, &h->cexact));
+        if (hipMalloc(&h->cbuf, 2 * L.n * sizeof(double)) != hipSuccess) return SBP_ERR_HIP;
+    }
+    h->cexact->st = h->st;
+    if constexpr (std::is_same<T, double>::value) {
+        STCHK(exact_solve(h->cexact, b, out));
+    } else {
+        double* b64 = h->cbuf;
+        double* x64 = h->cbuf + L.n;
+        hipLaunchKernelGGL((cast_vec<T, double, kEwThreads>), dim3(ew_blocks(L.n)), dim3(kEwThreads), 0, h->st, b,
+                           b64, L.n);
+        HIPCHK(hipGetLastError());
+        STCHK(exact_solve(h->cexact, b64, x64));
+        hipLaunchKernelGGL((cast_vec<double, T, kEwThreads>), dim3(ew_blocks(L.n)), dim3(kEwThreads), 0, h->st,
+                           (const double*)x64, out, L.n);
+        HIPCHK(hipGetLastError());
+    }
+    if (dotv) {
+        hipLaunchKernelGGL((dot_partial<T, kEwThreads>), dim3(1, 1), dim3(kEwThreads), 0, h->st, dotv, (const T*)out,
+                           L.n, partial);
+        HIPCHK(hipGetLastError());
+    }
+    return SBP_OK;
+}
+
 // V-cycle on level l: x_out ~ A_l^-1 b (zero initial guess), every problem.
 // When dotv is non-null the last pass also writes block partials of dotv' x_out.
 template <typename T>
@@ -260,6 +322,7 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
     hipStream_t st = h->st;
     if (l == h->bottom_l0) return launch_bottom<T>(h, b, out, dotv, scb, partial);
     if (l == h->nlev - 1) {
+        if (h->coarse_nested) return coarse_exact<T>(h, b, out, dotv, partial);
         hipLaunchKernelGGL((coarse_solve<T>), dim3(((int)L.n + kCoarseRows - 1) / kCoarseRows, h->nprob), dim3(256), 0,
                            st, h->Ainv, b, out, (int)L.n, scb);
         HIPCHK(hipGetLastError());
@@ -509,7 +572,8 @@ int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use
                        (const Scalars*)sc);
     HIPCHK(hipGetLastError());
     hipGraphExec_t ex = nullptr;
-    if (h->prm.use_graph) STCHK(iteration_graph<T>(h, v, sc, use_mg, &ex));
+    // (a nested coarse solve synchronises with the host: no capture)
+    if (h->prm.use_graph && !(use_mg && h->coarse_nested)) STCHK(iteration_graph<T>(h, v, sc, use_mg, &ex));
     const int poll_every = std::max(2, h->prm.poll_every + (h->prm.poll_every & 1));
     for (int it = 0; it < maxiter; it += 2) {
         if (ex) {
@@ -892,7 +956,8 @@ int sbp_destroy(sbp_handle h) {
     drop_graphs(h);
     if (h->cap) (void)hipStreamDestroy(h->cap);
     if (h->child) (void)sbp_destroy(h->child);
-    void* ptrs[] = {h->xc, h->w, h->lam, h->Ainv, h->r, h->pv, h->pv2, h->Ap, h->z, h->x, h->r32, h->p32, h->p32b, h->Ap32,
+    if (h->cexact) (void)sbp_destroy(h->cexact);
+    void* ptrs[] = {h->cbuf, h->xc, h->w, h->lam, h->Ainv, h->r, h->pv, h->pv2, h->Ap, h->z, h->x, h->r32, h->p32, h->p32b, h->Ap32,
                     h->z32, h->d32, h->partial, h->sc, h->sc_in, h->dtmp, h->dabs};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -936,7 +1001,7 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
     int nlev = 1;
     const int cn = h->prm.coarse_n;
     auto pow2p1 = [](int v) { return v >= 3 && ((v - 1) & (v - 2)) == 0; };
-    if (pow2p1(N) && pow2p1(cn) && cn <= N && cn <= 33) {
+    if (pow2p1(N) && pow2p1(cn) && cn <= N && (cn <= 33 || nprob == 1)) {
         int n = N;
         while (n > cn && nlev < kMaxLevels) {
             n = (n + 1) / 2;
@@ -1009,6 +1074,11 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
             if ((rc = alloc_level_buffers(h, 0)) != SBP_OK) break;
             // Coarsest level: FP64 inverse of the rediscretised operator via Cholesky.
             const Level& Lc = h->L[nlev - 1];
+            if (Lc.N > 33) {   // nested exact solve instead of a dense inverse
+                h->coarse_nested = true;
+                h->nc = (int)Lc.n;
+                break;
+            }
             std::vector<double> A;
             dense_operator(Lc.tb, A);
             const int nc = (int)Lc.n;
@@ -1187,6 +1257,7 @@ int sbp_smooth(sbp_handle h, int level, const void* b, void* x, int nsweeps, int
 
 int sbp_coarse_solve(sbp_handle h, const double* b, double* x) {
     if (!h || !b || !x || !h->mg) return SBP_ERR_INVALID_ARG;
+    if (h->coarse_nested) return coarse_exact<double>(h, b, x, nullptr, nullptr);
     hipLaunchKernelGGL((coarse_solve<double>), dim3((h->nc + kCoarseRows - 1) / kCoarseRows, h->nprob), dim3(256), 0,
                        h->st, h->Ainv, b, x, h->nc, (const Scalars*)nullptr);
     HIPCHK(hipGetLastError());
@@ -1297,9 +1368,6 @@ int sbp_adjoint_grad(sbp_handle h, const double* u, const double* u_obs, double 
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr double kExactRtol = 1e-13;   // coarsest solve: ||r|| < 1e-13 ||b||
-constexpr int kExactMaxiter = 100;
-
 // Handle of the next coarser grid ((N+1)/2, same order and parameters, level
 // l keeping h's tile shapes for level l+1), made on first use; it runs on h's
 // stream.
@@ -1324,22 +1392,6 @@ int coarser(sbp_handle h, sbp_handle* out) {
     h->child->st = h->st;
     *out = h->child;
     return SBP_OK;
-}
-
-// x = A^-1 b on handle c, for the reference's sparse LU solve A_2h \ b_2h:
-// MG-PCG from zero to ||r|| < 1e-13 ||b|| (round-off level for FP64).  An
-// iterate short of that after kExactMaxiter iterations is still returned as
-// the guess; HIP errors, NaN/Inf and breakdown are reported.
-int exact_solve(sbp_handle c, const double* b, double* x) {
-    HIPCHK(hipMemsetAsync(x, 0, c->L[0].n * sizeof(double), c->st));
-    double bb = 0.0;
-    STCHK(sbp_dot(c, b, b, &bb));
-    if (!std::isfinite(bb)) return SBP_ERR_NONFINITE;
-    if (bb == 0.0) return SBP_OK;
-    int it = 0;
-    double rn = 0.0;
-    const int rc = sbp_mgpcg(c, b, x, kExactRtol * std::sqrt(bb), kExactMaxiter, &it, &rn, nullptr);
-    return rc == SBP_ERR_NOT_CONVERGED ? SBP_OK : rc;
 }
 
 // bs[0] is this grid's right-hand side, bs[1] the next coarser grid's, ...

@@ -372,6 +372,13 @@ __global__ __launch_bounds__(NT) void copy_vec(const T* __restrict__ x, T* __res
     for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) y[k] = x[k];
 }
 
+// y = (D) x, one problem
+template <typename S, typename D, int NT>
+__global__ __launch_bounds__(NT) void cast_vec(const S* __restrict__ x, D* __restrict__ y, size_t n) {
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) y[k] = (D)x[k];
+}
+
 // diag(A)
 __global__ __launch_bounds__(256) void diag_kernel(double* d, OpTab t) {
     const size_t n = (size_t)t.N * t.N;

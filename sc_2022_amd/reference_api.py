@@ -74,13 +74,56 @@ def matrix_free_richardson(idata, odata, b, maxiter=3, omega=0.15, p=2):
     return None
 
 
-def matrix_free_MGCG(b, x, maxiter=None, abstol=None, nu=2, p=2):
-    """MG-preconditioned CG (initial_guess_cg.jl:303-346) -> (iters, norms, x).
-    Differences: the preconditioner is the multi-level V-cycle with nu
-    damped-Jacobi sweeps down to 17x17 (the reference: a two-level cycle with
-    Richardson and a CPU LU); abstol defaults to sqrt(eps) ||b|| as at its
-    call site (matrix_free_mg_cg.jl:466-468)."""
-    h = handle(_side(b), p, device=b.device, nu_pre=nu, nu_post=nu)
+def reference_mg_params(N, levels=2, nu=3, omega=0.15):
+    """sbp_params of the reference's own preconditioner: nu Richardson sweeps
+    (weight omega) before and after, full weighting without scaling, and an
+    exact solve on the grid `levels - 1` halvings down
+    (initial_guess_cg.jl:134-180; levels = 3: :244-301)."""
+    Nc = N
+    for _ in range(levels - 1):
+        Nc = (Nc + 1) // 2
+    return dict(smoother=1, richardson_omega=float(omega), nu_pre=int(nu), nu_post=int(nu),
+                restrict_scale=1.0, coarse_n=Nc)
+
+
+def matrix_free_Two_level_multigrid(b, A_2h=None, nu=3, NUM_V_CYCLES=1, SBPp=2):
+    """One two-level cycle from zero (initial_guess_cg.jl:147-180) -> (z,
+    ||A z - b||).  A_2h is unused: the coarse operator is rediscretised by the
+    handle and solved exactly (dense inverse up to 33^2, nested MG-PCG to
+    round-off above)."""
+    if NUM_V_CYCLES != 1:
+        raise ValueError("one cycle per call, as the reference's callers use it")
+    N = _side(b)
+    h = handle(N, SBPp, device=b.device, **reference_mg_params(N, 2, nu))
+    bb = b.reshape(-1)
+    z = h.vcycle(bb)
+    r = h.apply(z) - bb
+    return z.reshape(b.shape), float(torch.linalg.vector_norm(r))
+
+
+def matrix_free_Three_level_multigrid(b, A_4h=None, nu=3, NUM_V_CYCLES=1, SBPp=2):
+    """Three-level cycle (initial_guess_cg.jl:244-301) -> (z, ||A z - b||).
+    Difference: the reference forms the middle grid's residual without that
+    grid's right-hand side (r_2h = -A_2h v, :282) and repeats the last
+    smoothing call on an aliased argument (:295-296); this is the standard
+    three-level cycle with the same smoother, transfers and exact solve."""
+    if NUM_V_CYCLES != 1:
+        raise ValueError("one cycle per call, as the reference's callers use it")
+    N = _side(b)
+    h = handle(N, SBPp, device=b.device, **reference_mg_params(N, 3, nu))
+    bb = b.reshape(-1)
+    z = h.vcycle(bb)
+    r = h.apply(z) - bb
+    return z.reshape(b.shape), float(torch.linalg.vector_norm(r))
+
+
+def matrix_free_MGCG(b, x, maxiter=None, abstol=None, nu=3, p=2, levels=2):
+    """MG-preconditioned CG (initial_guess_cg.jl:303-346) -> (iters, norms, x)
+    with the reference's preconditioner (reference_mg_params: two-level
+    Richardson, nu = 3, exact coarse solve); abstol defaults to
+    sqrt(eps) ||b|| as at its call site (matrix_free_mg_cg.jl:466-468).  The
+    build's own multi-level V-cycle is SbpPoisson.mgpcg."""
+    h = handle(_side(b), p, device=b.device, **reference_mg_params(_side(b), levels, nu))
     bb = b.reshape(-1)
     if abstol is None:
         abstol = math.sqrt(2.220446049250313e-16) * float(torch.linalg.vector_norm(bb))

@@ -232,6 +232,35 @@ def test_cg_interp_mirror_and_errors(SP):
     assert res.converged
 
 
+@pytest.mark.parametrize("N", [65, 129])
+def test_reference_two_level_mode(SP, N):
+    """The reference's own preconditioner (two-level, 3 + 3 Richardson sweeps
+    with weight 0.15, unscaled full weighting, exact coarse solve) against the
+    oracle's restatement, alone and inside MGCG (p = 2).  At 65^2 the coarse
+    33^2 grid takes the dense inverse; at 129^2 the coarse 65^2 grid takes the
+    nested MG-PCG solve."""
+    from sc_2022_amd import reference_api as R
+    level = int(math.log2(N - 1))
+    A, A2 = O.assemble_operator(level, 2), O.assemble_operator(level - 1, 2)
+    h = SP(N, 2, **R.reference_mg_params(N, 2))
+    assert h.nlevels == 2
+    r = np.random.default_rng(6).standard_normal(N * N)
+    zo, no = O.two_level_richardson(A, A2, r)
+    z = h.vcycle(T(r)).cpu().numpy()
+    assert np.abs(z - zo).max() < 1e-10 * np.abs(zo).max()
+    zt, nt = R.matrix_free_Two_level_multigrid(T(r))
+    assert abs(nt - no) < 1e-8 * no
+    _, _, _, (f, gW, gE, gS, gN) = problem(level, 2)
+    b = O.assemble_rhs(level, 2, f, gW, gE, gS, gN)
+    tol = 1e-10 * np.linalg.norm(b)
+    xo, ito, _ = O.matrix_free_mgcg(A, b, lambda q: O.two_level_richardson(A, A2, q)[0], abstol=tol)
+    x, res = h.mgpcg(T(b), abstol=tol, maxiter=200)
+    assert res.converged and abs(res.iters - ito) <= 1, (res.iters, ito)
+    assert np.linalg.norm(A @ x.cpu().numpy() - b) < 1.000001 * tol
+    it, norms, xr = R.matrix_free_MGCG(T(b), T(np.zeros(N * N)), abstol=tol)
+    assert it == res.iters and len(norms) == it + 1
+
+
 def test_status_codes(SP):
     from sc_2022_amd import _capi as capi
     level, p = 5, 2

@@ -286,6 +286,29 @@ def test_initial_guess_interpolation_cuts_cg_iterations(p):
     assert O.h_norm(x0 - xd, hd) < 1e-3 * O.h_norm(xd, hd)
 
 
+def test_two_level_richardson_restatement():
+    """The reference's two-level cycle: with nu = 0 it is P A_2h^-1 FW b
+    exactly; it is a symmetric operator (same smoother before and after,
+    FW = P^T / 4); as the MGCG preconditioner (p = 2) it converges to the
+    direct solution."""
+    level = 5
+    N = 2 ** level + 1
+    A, A2 = O.assemble_operator(level, 2), O.assemble_operator(level - 1, 2)
+    rng = np.random.default_rng(8)
+    b, c = rng.standard_normal(N * N), rng.standard_normal(N * N)
+    z0, _ = O.two_level_richardson(A, A2, b, nu=0)
+    ref = O.prolongation_2d((N + 1) // 2) @ spla.spsolve(A2.tocsc(), O.restriction_2d(N) @ b)
+    assert np.abs(z0 - ref).max() < 1e-12 * np.abs(ref).max()
+    M = lambda r: O.two_level_richardson(A, A2, r)[0]
+    assert abs(c @ M(b) - b @ M(c)) < 1e-10 * abs(c @ M(b))
+    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    rhs = O.assemble_rhs(level, 2, f, gW, gE, gS, gN)
+    x, it, norms = O.matrix_free_mgcg(A, rhs, M, abstol=1e-10 * np.linalg.norm(rhs))
+    assert norms[-1] < 1e-10 * np.linalg.norm(rhs) and it < 60
+    xd = spla.spsolve(A.tocsc(), rhs)
+    assert np.linalg.norm(x - xd) < 1e-8 * np.linalg.norm(xd)
+
+
 def test_fourier_modes_deterministic():
     s1, b1 = O.fourier_modes(123)
     s2, b2 = O.fourier_modes(123)
