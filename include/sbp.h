@@ -69,7 +69,9 @@ typedef struct {
                                 4 (default) gives R = P^T, 1 the reference's scale */
     int smoother;            /* 0 = damped Jacobi (default), 1 = Richardson */
     double richardson_omega; /* Richardson weight (reference: 0.15) */
-    int poll_every;          /* host convergence poll interval in iterations (default 4) */
+    int poll_every;          /* host convergence poll interval in iterations; 0 (default):
+                                after 2 iterations, then where the measured rate of
+                                ||r|| predicts convergence (at most 16 ahead) */
     double inner_rtol;       /* mixed precision: inner FP32 solve relative tolerance (default 1e-5) */
     int inner_maxiter;       /* mixed precision: inner iteration cap (default 50) */
     int max_outer;           /* mixed precision: refinement step cap (default 10) */

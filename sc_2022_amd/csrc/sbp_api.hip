@@ -7,8 +7,9 @@
 // of one size and order stacked in its buffers (problem q at offset q * n of
 // every grid function); every launch covers all of them.  Krylov scalars live
 // on the device, one block per problem; kernels test that problem's "done"
-// flag and fall through once it has converged, so the host only polls every
-// poll_every iterations while each problem's iteration count keeps the
+// flag and fall through once it has converged, so the host only polls now
+// and then (every poll_every iterations, or where the measured convergence
+// rate predicts convergence) while each problem's iteration count keeps the
 // reference's semantics (the first k with ||r_k||_2 < abstol).
 #include <hip/hip_runtime.h>
 
@@ -78,8 +79,9 @@ struct sbp_handle_s {
     double clock_khz = 0.0;      // wall_clock64 rate
     // Captured Krylov iterations: [precision][0 = CG, 1 = MG-PCG]
     hipStream_t cap = nullptr;   // capture stream (default priority)
-    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    // [precision][0 = CG, 1 = MG-PCG][parity of the iteration]
+    hipGraphExec_t gexec[2][2][2] = {};
+    hipGraph_t graph[2][2][2] = {};
     // Initial-guess interpolation: the handle of the next coarser grid
     // ((N+1)/2, same order and parameters) and the coarse solution on it,
     // both made on first use.
@@ -418,10 +420,12 @@ int vcycle_nparts(sbp_handle h) {
 void drop_graphs(sbp_handle h) {
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) {
-            if (h->gexec[a][b]) (void)hipGraphExecDestroy(h->gexec[a][b]);
-            if (h->graph[a][b]) (void)hipGraphDestroy(h->graph[a][b]);
-            h->gexec[a][b] = nullptr;
-            h->graph[a][b] = nullptr;
+            for (int c = 0; c < 2; ++c) {
+                if (h->gexec[a][b][c]) (void)hipGraphExecDestroy(h->gexec[a][b][c]);
+                if (h->graph[a][b][c]) (void)hipGraphDestroy(h->graph[a][b][c]);
+                h->gexec[a][b][c] = nullptr;
+                h->graph[a][b][c] = nullptr;
+            }
         }
 }
 
@@ -521,18 +525,18 @@ int issue_iteration(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, i
     return SBP_OK;
 }
 
-// Captures two iterations (both parities) once per (precision, solver) into a
-// hipGraph on the handle's own capture stream; replays go to the handle's stream.
+// Captures one iteration of each parity once per (precision, solver) into a
+// hipGraph on the handle's own capture stream; replays go to the handle's
+// stream, alternating the parities (the search directions ping-pong).
 template <typename T>
-int iteration_graph(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, hipGraphExec_t* out) {
+int iteration_graph(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, int par, hipGraphExec_t* out) {
     const int a = prec_index<T>(), b = use_mg ? 1 : 0;
-    if (!h->gexec[a][b]) {
+    if (!h->gexec[a][b][par]) {
         if (!h->cap) HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
         HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
         hipStream_t saved = h->st;
         h->st = h->cap;
-        int rc = issue_iteration<T>(h, v, sc, use_mg, 1);
-        if (rc == SBP_OK) rc = issue_iteration<T>(h, v, sc, use_mg, 0);
+        const int rc = issue_iteration<T>(h, v, sc, use_mg, par);
         h->st = saved;
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(h->cap, &g);
@@ -543,17 +547,38 @@ int iteration_graph(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, h
             (void)hipGraphDestroy(g);
             return SBP_ERR_HIP;
         }
-        h->graph[a][b] = g;
-        h->gexec[a][b] = ex;
+        h->graph[a][b][par] = g;
+        h->gexec[a][b][par] = ex;
     }
-    *out = h->gexec[a][b];
+    *out = h->gexec[a][b][par];
     return SBP_OK;
+}
+
+// Iterations until the next host poll (poll_every = 0): from each unfinished
+// problem's residual history, ||r_k|| = ||r_0|| rho^k, the number still needed
+// to reach its tolerance, at least 1 and at most kPollAhead; the largest
+// over the problems.
+constexpr int kPollAhead = 16;
+int poll_ahead(const sbp_handle h) {
+    int need = 1;
+    for (int q = 0; q < h->nprob; ++q) {
+        const Scalars& s = h->sc_host[q];
+        if (s.done) continue;
+        int k = kPollAhead;
+        if (s.iters > 0 && s.resnorm > 0.0 && s.resnorm < s.rnorm0 && s.abstol > 0.0) {
+            const double lrho = std::log(s.resnorm / s.rnorm0) / s.iters;   // < 0
+            const double est = std::ceil(std::log(s.abstol / s.resnorm) / lrho);
+            k = est < 1.0 ? 1 : (est > kPollAhead ? kPollAhead : (int)est);
+        }
+        need = std::max(need, k);
+    }
+    return need;
 }
 
 // Preconditioned CG body shared by FP64 and FP32: solves A x = b for every
 // problem given the initial residuals in v.r (scalar blocks already set by the
 // caller's init kernel); the preconditioner is the V-cycle when use_mg, else
-// the identity.  The host polls every poll_every iterations and stops issuing
+// the identity.  The host polls now and then (poll_ahead) and stops issuing
 // once every problem is done.  Leaves the scalar blocks in h->sc_host.
 template <typename T>
 int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use_mg) {
@@ -571,21 +596,30 @@ int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use
     hipLaunchKernelGGL((copy_vec<T, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, z0, v.p[0], n,
                        (const Scalars*)sc);
     HIPCHK(hipGetLastError());
-    hipGraphExec_t ex = nullptr;
+    hipGraphExec_t ex[2] = {nullptr, nullptr};
     // (a nested coarse solve synchronises with the host: no capture)
-    if (h->prm.use_graph && !(use_mg && h->coarse_nested)) STCHK(iteration_graph<T>(h, v, sc, use_mg, &ex));
-    const int poll_every = std::max(2, h->prm.poll_every + (h->prm.poll_every & 1));
-    for (int it = 0; it < maxiter; it += 2) {
-        if (ex) {
-            HIPCHK(hipGraphLaunch(ex, st));
+    if (h->prm.use_graph && !(use_mg && h->coarse_nested)) {
+        STCHK(iteration_graph<T>(h, v, sc, use_mg, 1, &ex[1]));
+        STCHK(iteration_graph<T>(h, v, sc, use_mg, 0, &ex[0]));
+    }
+    // Host polls: every poll_every iterations, or (poll_every = 0) after two
+    // iterations and then where the measured convergence rate predicts the
+    // last problem converges, so that few launches run for finished problems.
+    const bool adaptive = h->prm.poll_every == 0;
+    int next = adaptive ? 2 : h->prm.poll_every;
+    for (int it = 0; it < maxiter;) {
+        const int par = (it & 1) ? 0 : 1;   // iteration 0 has parity 1
+        if (ex[par]) {
+            HIPCHK(hipGraphLaunch(ex[par], st));
         } else {
-            STCHK(issue_iteration<T>(h, v, sc, use_mg, 1));
-            STCHK(issue_iteration<T>(h, v, sc, use_mg, 0));
+            STCHK(issue_iteration<T>(h, v, sc, use_mg, par));
         }
-        if ((it + 2) % poll_every == 0 && it + 2 < maxiter) {
+        ++it;
+        if (it >= next && it < maxiter) {
             bool all = false;
             STCHK(poll(h, sc, &all));
             if (all) break;
+            next = it + (adaptive ? poll_ahead(h) : h->prm.poll_every);
         }
     }
     hipLaunchKernelGGL((flush_x<T, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, v.x, (const T*)v.p[0],
@@ -657,6 +691,7 @@ __global__ __launch_bounds__(1024) void ir_check(const double* partial, int np, 
         so->rr = rs;
         si->rr = rs;
         si->resnorm = nr;
+        si->rnorm0 = nr;
         if (!isfinite(rs)) { so->status = 5; mark_done(so); si->done = 1; return; }
         if (nr < so->abstol) { mark_done(so); si->done = 1; return; }
         si->abstol = inner_rtol * nr;
@@ -928,7 +963,7 @@ void sbp_default_params(sbp_params* prm) {
     prm->restrict_scale = 4.0;
     prm->smoother = 0;
     prm->richardson_omega = 0.15;
-    prm->poll_every = 4;
+    prm->poll_every = 0;
     prm->inner_rtol = 1e-5;
     prm->inner_maxiter = 50;
     prm->max_outer = 10;
@@ -978,7 +1013,7 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
     }
     if (nprob < 1 || nprob > 65535) return SBP_ERR_INVALID_ARG;   // problems ride grid.y of the elementwise kernels
     if ((long long)N * N * nprob > (1ll << 40)) return SBP_ERR_INVALID_SIZE;
-    if (prm && (prm->nu_pre < 0 || prm->nu_post < 0 || prm->poll_every < 1 || prm->max_outer < 1 ||
+    if (prm && (prm->nu_pre < 0 || prm->nu_post < 0 || prm->poll_every < 0 || prm->max_outer < 1 ||
                 prm->inner_maxiter < 1 || !(prm->omega > 0) || !(prm->restrict_scale > 0) ||
                 (prm->smoother != 0 && prm->smoother != 1) || prm->bottom_n < 0 ||
                 prm->fuse_n < 0))
@@ -1490,7 +1525,8 @@ int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, const double
     hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n), 1), dim3(kEwThreads), 0, st,
                        (const double*)h->r, h->pv, (size_t)n, (const Scalars*)sc);
     HIPCHK(hipGetLastError());
-    const int poll_every = std::max(1, h->prm.poll_every);
+    const bool adaptive = h->prm.poll_every == 0;
+    int next = adaptive ? 2 : h->prm.poll_every;
     const dim3 g(ew_blocks(n), 1);
     const int nbv = std::min(kCsrMaxBlocks, (n + 256 / kCsrLanes - 1) / (256 / kCsrLanes));
     for (int it = 0; it < maxiter; ++it) {
@@ -1503,10 +1539,11 @@ int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, const double
         hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)h->r,
                            (const Scalars*)sc, (size_t)n);
         HIPCHK(hipGetLastError());
-        if ((it + 1) % poll_every == 0) {
+        if (it + 1 >= next && it + 1 < maxiter) {
             bool all = false;
             STCHK(poll(h, sc, &all));
             if (all) break;
+            next = it + 1 + (adaptive ? poll_ahead(h) : h->prm.poll_every);
         }
     }
     STCHK(poll(h, sc));

@@ -16,6 +16,7 @@ struct Scalars {
     double alpha, beta;
     double abstol;
     double resnorm;        // latest ||r||_2
+    double rnorm0;         // ||r_0||_2 of the current solve (host-side convergence prediction)
     double bnorm;
     double* hist;          // nullable residual-norm history, hist[k] = ||r_k||_2
     double dprev;          // refinement: norm of the previous correction
@@ -218,6 +219,7 @@ __global__ __launch_bounds__(1024) void krylov_init(const double* partial, int n
         sc->maxiter = maxiter;
         sc->rr = rs;
         sc->resnorm = sqrt(rs);
+        sc->rnorm0 = sqrt(rs);
         sc->hist = hist;
         sc->iters = 0;
         sc->status = 0;
