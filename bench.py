@@ -179,7 +179,8 @@ def csr_leg(h, torch, apply_us):
 
 def leg_8193(a, dev, torch):
     """Mixed precision and FP64 MG-PCG at 8193^2 (configs[2] size) with the
-    headline stopping rules; errors against the FP64 solve to 1e-14."""
+    headline stopping rules scaled by the 4x larger condition number of A
+    (residual / 4 for the same error); errors against the FP64 solve to 1e-14."""
     from sc_2022_amd.data import fourier_rhs
     from sc_2022_amd.solver import SbpPoisson
     N = 8193
@@ -194,8 +195,8 @@ def leg_8193(a, dev, torch):
         def run():
             x.zero_()
             if leg == "mixed":
-                return h.mgpcg_mixed(b, x, abstol=a.mixed_rtol * bn)[1]
-            return h.mgpcg(b, x, abstol=a.rtol * bn, maxiter=200)[1]
+                return h.mgpcg_mixed(b, x, abstol=0.25 * a.mixed_rtol * bn)[1]
+            return h.mgpcg(b, x, abstol=0.25 * a.rtol * bn, maxiter=200)[1]
         run()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -206,6 +207,8 @@ def leg_8193(a, dev, torch):
         out[leg] = {"ms": dt * 1e3, "solves_per_s": 1.0 / dt, "rel_err": h.hnorm(x - xref) / href,
                     "iters": r.iters, "inner": r.inner_iters}
     out["speedup_mixed_vs_f64"] = out["f64"]["ms"] / out["mixed"]["ms"]
+    out["rules"] = {"mixed_rtol": 0.25 * a.mixed_rtol, "f64_rtol": 0.25 * a.rtol}
+    out["meets_rel_err_target"] = bool(max(out["mixed"]["rel_err"], out["f64"]["rel_err"]) <= REL_ERR_TARGET)
     h.close()
     return out
 

@@ -151,6 +151,7 @@ template <typename T> struct InPlain {
     const T* u;
     __device__ void at(int q, const Batch& bt) { u += q * bt.sf; }
     static constexpr bool kCol = false;
+    static constexpr bool kRawRhs = false;
     using Raw = T;
     __device__ void init() {}
     template <bool INT = false, bool LT = false>
@@ -167,6 +168,7 @@ template <typename T> struct InSmooth0 {
     const T* b; T omega; int use_diag;
     __device__ void at(int q, const Batch& bt) { b += q * bt.sf; }
     static constexpr bool kCol = false;
+    static constexpr bool kRawRhs = true;   // the raw fetch at a point is b there
     using Raw = T;
     __device__ void init() {}
     template <bool INT = false, bool LT = false>
@@ -188,6 +190,7 @@ template <typename T> struct InProlong {
     const T* x; const T* ec; int Nc;
     __device__ void at(int q, const Batch& bt) { x += q * bt.sf; ec += q * bt.sc; }
     static constexpr bool kCol = true;
+    static constexpr bool kRawRhs = false;
     __device__ void init() {}
     // Column window w[k] = x(i, jbeg+k) + (P ec)(i, jbeg+k), k < W, with jbeg of
     // parity PAR: each coarse row is loaded once (W/2+2 rows) instead of four
@@ -241,6 +244,7 @@ template <typename T> struct InAxpy {
     const T* r; const T* p; const double* beta; T bt;
     __device__ void at(int q, const Batch& bt) { r += q * bt.sf; p += q * bt.sf; beta += q * bt.ss; }
     static constexpr bool kCol = false;
+    static constexpr bool kRawRhs = false;
     __device__ void init() { bt = (T)(*beta); }
     struct Raw { T r, p; };
     template <bool INT = false, bool LT = false>
@@ -316,6 +320,8 @@ template <typename T> struct OutSmooth {
     const T* b; T* xo; T omega; int use_diag;
     __device__ void at(int q, const Batch& bt) { b += q * bt.sf; xo += q * bt.sf; }
     static constexpr bool kDot = false;
+    static constexpr bool kPreRhs = true;   // pre() is b at the point: an input whose raw
+                                            // fetch is b (kRawRhs) supplies it without a load
     using Aux = T;
     __device__ Aux pre(const OpTab&, size_t q) const { return b[q]; }
     template <bool INT = false, bool LT = false>
@@ -332,7 +338,12 @@ template <typename T> struct OutSmoothDot {
     __device__ void at(int q, const Batch& bt) { b += q * bt.sf; xo += q * bt.sf; dotv += q * bt.sf; }
     static constexpr bool kDot = true;
     struct Aux { T b, d; };
-    __device__ Aux pre(const OpTab&, size_t q) const { Aux a; a.b = b[q]; a.d = dotv[q]; return a; }
+    __device__ Aux pre(const OpTab&, size_t q) const {
+        Aux a;
+        a.b = b[q];
+        a.d = dotv == b ? a.b : dotv[q];   // r'z on the level-0 right-hand side r itself
+        return a;
+    }
     template <bool INT = false, bool LT = false>
     __device__ __forceinline__ void emit(const OpTab& t, int i, int j, size_t q, T u, T Au, Aux a, T& acc) const {
         T c = omega * (a.b - Au);
@@ -451,6 +462,9 @@ __device__ __forceinline__ void stage_table(const OpTab& t, OpTab* l) {
     __syncthreads();
 }
 
+template <class X, class = void> struct pre_rhs { static constexpr bool value = false; };
+template <class X> struct pre_rhs<X, decltype(void(X::kPreRhs))> { static constexpr bool value = X::kPreRhs; };
+
 template <class X, class = void> struct has_init { static constexpr bool value = false; };
 template <class X> struct has_init<X, decltype(void(&X::init))> { static constexpr bool value = true; };
 
@@ -471,6 +485,10 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
     const int N = t.N;
     const bool act = !ICHK || i < N;
     T w[W];
+    // the smoothing right-hand side at the tile's points is already in the
+    // input's raw window: no second load of it
+    constexpr bool kRhsRaw = IN::kRawRhs && pre_rhs<OUT>::value;
+    typename IN::Raw rhs[kRhsRaw ? J : 1];
     if (act) {
         if constexpr (IN::kCol && !ICHK) {
             in.template load_col<W, (H & 1)>(t, i, j0 - H, w);   // j0 is a multiple of the even J
@@ -480,6 +498,10 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
             for (int k = 0; k < W; ++k) raw[k] = in.template fetch<INT, LT>(t, i, j0 - H + k);
 #pragma unroll
             for (int k = 0; k < W; ++k) w[k] = in.template make<INT, LT>(t, i, j0 - H + k, raw[k]);
+            if constexpr (kRhsRaw) {
+#pragma unroll
+                for (int r = 0; r < J; ++r) rhs[r] = raw[r + H];
+            }
         }
     } else {
 #pragma unroll
@@ -502,7 +524,11 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
 #pragma unroll
     for (int r = 0; r < J; ++r) {
         tile[r][tid + H] = w[r + H];
-        if (act) aux[r] = out.pre(t, gidx(i, j0 + r, N));
+        if constexpr (kRhsRaw) {
+            aux[r] = rhs[r];
+        } else {
+            if (act) aux[r] = out.pre(t, gidx(i, j0 + r, N));
+        }
     }
     __syncthreads();
     T acc = T(0);
