@@ -178,13 +178,16 @@ def csr_leg(h, torch, apply_us):
 
 
 def leg_8193(a, dev, torch):
-    """Mixed precision and FP64 MG-PCG at 8193^2 (configs[2] size) with the
-    headline stopping rules scaled by the 4x larger condition number of A
-    (residual / 4 for the same error); errors against the FP64 solve to 1e-14."""
+    """Mixed precision and FP64 MG-PCG at 8193^2 (configs[2] size), each leg
+    with the loosest rule that meets rel err <= 1e-10 at this size
+    (bench.py --n 8193 --tol-sweep): FP64 ||r|| < rtol ||b||, mixed
+    ||r|| < mixed_rtol ||b|| with inner FP32 solves to 3e-6; errors against
+    the FP64 solve to 1e-14."""
     from sc_2022_amd.data import fourier_rhs
     from sc_2022_amd.solver import SbpPoisson
     N = 8193
-    h = SbpPoisson(N, a.p, device=dev, ir_rtol=a.ir_rtol, inner_rtol=a.inner_rtol)
+    inner = 3e-6
+    h = SbpPoisson(N, a.p, device=dev, ir_rtol=a.ir_rtol, inner_rtol=inner)
     b = fourier_rhs(h, a.seed)
     bn = float(torch.linalg.vector_norm(b))
     xref, _ = h.mgpcg(b, abstol=1e-14 * bn, maxiter=200, raise_on_fail=False)
@@ -195,8 +198,8 @@ def leg_8193(a, dev, torch):
         def run():
             x.zero_()
             if leg == "mixed":
-                return h.mgpcg_mixed(b, x, abstol=0.25 * a.mixed_rtol * bn)[1]
-            return h.mgpcg(b, x, abstol=0.25 * a.rtol * bn, maxiter=200)[1]
+                return h.mgpcg_mixed(b, x, abstol=a.mixed_rtol * bn)[1]
+            return h.mgpcg(b, x, abstol=a.rtol * bn, maxiter=200)[1]
         run()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -207,7 +210,7 @@ def leg_8193(a, dev, torch):
         out[leg] = {"ms": dt * 1e3, "solves_per_s": 1.0 / dt, "rel_err": h.hnorm(x - xref) / href,
                     "iters": r.iters, "inner": r.inner_iters}
     out["speedup_mixed_vs_f64"] = out["f64"]["ms"] / out["mixed"]["ms"]
-    out["rules"] = {"mixed_rtol": 0.25 * a.mixed_rtol, "f64_rtol": 0.25 * a.rtol}
+    out["rules"] = {"mixed_rtol": a.mixed_rtol, "inner_rtol": inner, "f64_rtol": a.rtol}
     out["meets_rel_err_target"] = bool(max(out["mixed"]["rel_err"], out["f64"]["rel_err"]) <= REL_ERR_TARGET)
     h.close()
     return out
@@ -328,7 +331,7 @@ def tol_sweep(a, h, b, x, bnorm, rel_err, torch):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / reps, r
 
-    for rt in (1e-10, 3e-11, 1e-11, 3e-12, 1e-12):
+    for rt in (1e-10, 3e-11, 1e-11, 5e-12, 3e-12, 1e-12):
         def f():
             x.zero_()
             return h.mgpcg(b, x, abstol=rt * bnorm, maxiter=200, raise_on_fail=False)[1]
@@ -336,7 +339,7 @@ def tol_sweep(a, h, b, x, bnorm, rel_err, torch):
         print(json.dumps({"leg": "f64", "rtol": rt, "iters": r.iters, "rel_err": rel_err(x), "ms": dt * 1e3}),
               flush=True)
     for irt in (1e-5, 3e-6, 1e-6):
-        for rt in (1e-10, 1e-11, 1e-12):
+        for rt in (1e-10, 5e-11, 3e-11, 1e-11, 1e-12):
             hm = SbpPoisson(h.N, h.p, device=h.device, inner_rtol=irt, ir_rtol=a.ir_rtol)
             def f():
                 x.zero_()
