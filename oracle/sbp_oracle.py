@@ -10,6 +10,9 @@ operator, literally (same kron products, same SAT terms):
 * ``diagonal_sbp_D1`` -- diagonal_sbp.jl:56-147 (p = 2, 4, 6, 8)
 * ``diagonal_sbp_D2`` -- diagonal_sbp.jl:189-454 (p = 2 :191-197, p = 4
   :198-208, p = 6 :210-222, p = 8 :224-311, H/HI :421-428, S0/SN :446-448)
+* ``variable_diagonal_sbp_D2`` -- diagonal_sbp.jl:470-740 (p = 2, 4), and
+  ``assemble_operator_var`` / ``assemble_rhs_var`` -- its 2-D use with one
+  coefficient profile per axis (legacy_functions.jl:100-174, generalised)
 * ``operators_2d`` / ``assembling_matrix`` -- level_2_multigrid_new.jl:48-188
 * ``prolongation_matrix`` / ``restriction_matrix_normal`` /
   ``prolongation_2d`` / ``restriction_2d`` -- level_2_multigrid_new.jl:190-246
@@ -208,6 +211,117 @@ def diagonal_sbp_D2(p, N, xc=(-1.0, 1.0)):
 
 
 # ---------------------------------------------------------------------------
+# Variable-coefficient D2 (diagonal_sbp.jl:470-740), orders 2 and 4
+# ---------------------------------------------------------------------------
+
+# p = 4 boundary block of M(B): entry (r, c) (1-based, r <= c, symmetric) is
+# sum of num/den * b_k over the listed (num, den, k), b_k = B[k] (1-based)
+# (diagonal_sbp.jl:527-547).
+_VAR_P4_M0 = {
+    (1, 1): [(12, 17, 1), (59, 192, 2), (27010400129, 345067064608, 3), (69462376031, 2070402387648, 4)],
+    (1, 2): [(-59, 68, 1), (-6025413881, 21126554976, 3), (-537416663, 7042184992, 4)],
+    (1, 3): [(2, 17, 1), (-59, 192, 2), (213318005, 16049630912, 4), (2083938599, 8024815456, 3)],
+    (1, 4): [(3, 68, 1), (-1244724001, 21126554976, 3), (752806667, 21126554976, 4)],
+    (1, 5): [(49579087, 10149031312, 3), (-49579087, 10149031312, 4)],
+    (1, 6): [(-1, 784, 4), (1, 784, 3)],
+    (2, 2): [(3481, 3264, 1), (9258282831623875, 7669235228057664, 3), (236024329996203, 1278205871342944, 4)],
+    (2, 3): [(-59, 408, 1), (-29294615794607, 29725717938208, 3), (-2944673881023, 29725717938208, 4)],
+    (2, 4): [(-59, 1088, 1), (260297319232891, 2556411742685888, 3), (-60834186813841, 1278205871342944, 4)],
+    (2, 5): [(-1328188692663, 37594290333616, 3), (1328188692663, 37594290333616, 4)],
+    (2, 6): [(-8673, 2904112, 3), (8673, 2904112, 4)],
+    (3, 3): [(1, 51, 1), (59, 192, 2), (13777050223300597, 26218083221499456, 4), (564461, 13384296, 5), (378288882302546512209, 270764341349677687456, 3)],
+    (3, 4): [(1, 136, 1), (-125059, 743572, 5), (-4836340090442187227, 5525802884687299744, 3), (-17220493277981, 89177153814624, 4)],
+    (3, 5): [(-10532412077335, 42840005263888, 4), (1613976761032884305, 7963657098519931984, 3), (564461, 4461432, 5)],
+    (3, 6): [(-960119, 1280713392, 4), (-3391, 6692148, 5), (33235054191, 26452850508784, 3)],
+    (4, 4): [(3, 1088, 1), (507284006600757858213, 475219048083107777984, 3), (1869103, 2230716, 5), (1, 24, 6), (1950062198436997, 3834617614028832, 4)],
+    (4, 5): [(-4959271814984644613, 20965546238960637264, 3), (-1, 6, 6), (-15998714909649, 37594290333616, 4), (-375177, 743572, 5)],
+    (4, 6): [(-368395, 2230716, 5), (752806667, 539854092016, 3), (1063649, 8712336, 4), (1, 8, 6)],
+    (5, 5): [(8386761355510099813, 128413970713633903242, 3), (2224717261773437, 2763180339520776, 4), (5, 6, 6), (1, 24, 7), (280535, 371786, 5)],
+    (5, 6): [(-35039615, 213452232, 4), (-1, 6, 7), (-13091810925, 13226425254392, 3), (-1118749, 2230716, 5), (-1, 2, 6)],
+    (6, 6): [(3290636, 80044587, 4), (5580181, 6692148, 5), (5, 6, 7), (1, 24, 8), (660204843, 13226425254392, 3), (3, 4, 6)],
+}
+
+
+def _ratio(num, den):
+    # the reference divides the integer literals in Float64 (float(num) / float(den))
+    return float(num) / float(den)
+
+
+def variable_diagonal_sbp_D2(p, N, B, xc=(-1.0, 1.0)):
+    """(D, S0, SN, HI, H, M, r) -- diagonal_sbp.jl:470-740 for p = 2, 4:
+    D approximates d/dx (B d/dx) with D = HI (-M + SN - S0), M symmetric and
+    B-dependent, S0 / SN the B-weighted boundary derivatives."""
+    if p not in (2, 4):
+        raise ValueError(f"Operators for order {p} are not implemented")
+    B = np.asarray(B, dtype=np.float64)
+    assert len(B) == N + 1
+    Np = N + 1
+    b = lambda k: B[k - 1]                         # 1-based access, as the reference
+    rows, cols, vals = [], [], []
+
+    def put(i, j, v):                              # 1-based (row, column)
+        rows.append(i - 1); cols.append(j - 1); vals.append(v)
+
+    if p == 2:
+        bhinv = [2.0]
+        BS = np.array([3 / 2, -2.0, 1 / 2])
+        put(1, 1, (b(1) + b(2)) / 2)
+        put(N + 1, N + 1, (b(N) + b(N + 1)) / 2)
+        for i in range(1, N + 1):                  # I_M = 2:N+1, J_M = 1:N
+            put(i + 1, i, -(b(i) + b(i + 1)) / 2)
+        for i in range(2, N + 1):                  # I_M = J_M = 2:N
+            put(i, i, (b(i - 1) + 2 * b(i) + b(i + 1)) / 2)
+        for i in range(1, N + 1):                  # I_M = 1:N, J_M = 2:N+1
+            put(i, i + 1, -(b(i) + b(i + 1)) / 2)
+    else:
+        bhinv = [48 / 17, 48 / 59, 48 / 43, 48 / 49]
+        BS = np.array([11 / 6, -3.0, 3 / 2, -1 / 3])
+
+        def block(bk):
+            V = np.zeros((6, 6))
+            for (r, c), terms in _VAR_P4_M0.items():
+                v = sum(_ratio(n, d) * bk[k - 1] for n, d, k in terms)
+                V[r - 1, c - 1] = v
+                V[c - 1, r - 1] = v
+            return V
+        V0 = block(B[0:8])
+        VN = block(B[::-1][0:8])                   # b1..b8 = B[N+1:-1:N-6]
+        for r in range(6):
+            for c in range(6):
+                put(r + 1, c + 1, V0[r, c])
+                put(N + 1 - r, N + 1 - c, VN[r, c])   # V_MN[7-r', 7-c'] at rows N-4:N+1
+        for J in range(5, N - 4):                   # I = J+2, J = 5:N-5
+            put(J + 2, J, (1 / 8) * b(J + 2) - (1 / 6) * b(J + 1) + (1 / 8) * b(J))
+        for J in range(6, N - 4):                   # I = J+1, J = 6:N-5
+            put(J + 1, J, -(1 / 6) * b(J + 2) - (1 / 2) * b(J + 1) - (1 / 2) * b(J) - (1 / 6) * b(J - 1))
+        for J in range(7, N - 4):                   # I = J, J = 7:N-5
+            put(J, J, (1 / 24) * b(J + 2) + (5 / 6) * b(J + 1) + (3 / 4) * b(J) + (5 / 6) * b(J - 1)
+                + (1 / 24) * b(J - 2))
+        for J in range(7, N - 3):                   # I = J-1, J = 7:N-4
+            put(J - 1, J, -(1 / 6) * b(J + 1) - (1 / 2) * b(J) - (1 / 2) * b(J - 1) - (1 / 6) * b(J - 2))
+        for J in range(7, N - 2):                   # I = J-2, J = 7:N-3
+            put(J - 2, J, (1 / 8) * b(J) - (1 / 6) * b(J - 1) + (1 / 8) * b(J - 2))
+    bm = len(bhinv)
+    if Np < 2 * bm:
+        raise ValueError("Grid not big enough to support the operator")
+    h = (xc[1] - xc[0]) / N
+    assert h > 0
+    M = sp.csr_matrix((np.asarray(vals), (rows, cols)), shape=(Np, Np)) / h
+    HV = np.ones(Np)
+    bh = np.asarray(bhinv)
+    HV[:bm] = bh
+    HV[Np - bm:] = bh[::-1]
+    HI = sp.diags(HV / h).tocsr()
+    H = sp.diags(h / HV).tocsr()
+    nb = len(BS)
+    S0 = sp.csr_matrix((-B[0] * BS / h, (np.zeros(nb, int), np.arange(nb))), shape=(Np, Np))
+    SN = sp.csr_matrix((B[-1] * BS / h, (np.full(nb, Np - 1), Np - 1 - np.arange(nb))), shape=(Np, Np))
+    D = HI @ (-M + SN - S0)
+    r = np.linspace(xc[0], xc[1], Np)
+    return D.tocsr(), S0, SN, HI, H, M, r
+
+
+# ---------------------------------------------------------------------------
 # 2-D assembly (level_2_multigrid_new.jl:48-188)
 # ---------------------------------------------------------------------------
 
@@ -346,6 +460,112 @@ def baseline_data(level):
     gS = -(np.pi * np.sin(np.pi * t) + t)       # -du/dy at y = 0
     gN = -np.pi * np.sin(np.pi * t) + t         # +du/dy at y = 1
     return f, gW, gE, gS, gN, u
+
+
+def _var_1d(level, p, b):
+    N = 2 ** level
+    D, S0, SN, HI, H, M, r = variable_diagonal_sbp_D2(p, N, b, xc=(0, 1))
+    return D, (SN - S0).tocsr(), HI, H, M
+
+
+def assemble_operator_var(level, p, bx, by):
+    """SPD A = -H~ (D2x(bx) + D2y(by) + SAT) with one coefficient profile per
+    axis: bx along the slow (Dirichlet, x) index, by along the fast
+    (Neumann, y) index.  The 2-D construction follows get_operators
+    (legacy_functions.jl:100-174) with the variable D2 on both axes, the
+    B-weighted boundary derivatives in the SAT terms, and the Dirichlet
+    penalty scaled by the face coefficient (tau = -13 b_face / h, the
+    reference's -13/h at b = 1)."""
+    op, sat = _sat_var(level, p, bx, by)
+    A = op["D2"] + sat["SAT_W"] + sat["SAT_E"] + sat["SAT_S"] + sat["SAT_N"]
+    A = sp.csr_matrix(-(op["H_tilde"] @ A))
+    A.eliminate_zeros()
+    A.sort_indices()
+    return A
+
+
+def _sat_var(level, p, bx, by):
+    n1 = 2 ** level + 1
+    h = 1.0 / 2 ** level
+    Dx, BSx1, HIx1, Hx1, _ = _var_1d(level, p, bx)
+    Dy, BSy1, HIy1, Hy1, _ = _var_1d(level, p, by)
+    I = sp.identity(n1, format="csr")
+    k = lambda a, b: sp.kron(a, b, format="csr")
+    e1 = _e(0, n1); eN = _e(n1 - 1, n1)
+    E_W, E_E = k(_diag_of(e1), I), k(_diag_of(eN), I)
+    E_S, E_N = k(I, _diag_of(e1)), k(I, _diag_of(eN))
+    HI_x, HI_y = k(HIx1, I), k(I, HIy1)
+    BS_x, BS_y = k(BSx1, I), k(I, BSy1)
+    tau_W, tau_E = -13.0 * bx[0] / h, -13.0 * bx[-1] / h
+    tau_S = tau_N = -1.0
+    op = dict(D2=k(Dx, I) + k(I, Dy), H_tilde=k(Hx1, Hy1))
+    sat = dict(
+        SAT_W=tau_W * HI_x @ E_W + HI_x @ BS_x.T @ E_W,
+        SAT_E=tau_E * HI_x @ E_E + HI_x @ BS_x.T @ E_E,
+        SAT_S=tau_S * HI_y @ E_S @ BS_y,
+        SAT_N=tau_N * HI_y @ E_N @ BS_y,
+        SAT_W_r=tau_W * HI_x @ E_W @ k(e1, I) + HI_x @ BS_x.T @ E_W @ k(e1, I),
+        SAT_E_r=tau_E * HI_x @ E_E @ k(eN, I) + HI_x @ BS_x.T @ E_E @ k(eN, I),
+        SAT_S_r=tau_S * HI_y @ E_S @ k(I, e1),
+        SAT_N_r=tau_N * HI_y @ E_N @ k(I, eN),
+    )
+    return op, sat
+
+
+def assemble_rhs_var(level, p, bx, by, f, gW, gE, gS, gN):
+    """b = -H~ (f + SAT_W_r gW + SAT_E_r gE + SAT_S_r gS + SAT_N_r gN) of the
+    variable-coefficient operator; gS / gN are the outward fluxes by du/dn."""
+    op, s = _sat_var(level, p, bx, by)
+    b = (np.asarray(f, dtype=np.float64).ravel(order="C") + s["SAT_W_r"] @ gW + s["SAT_E_r"] @ gE
+         + s["SAT_S_r"] @ gS + s["SAT_N_r"] @ gN)
+    return -(op["H_tilde"] @ b)
+
+
+def var_coefficients(level):
+    """Smooth positive profiles for tests: bx(x) = 1 + x / 2 + sin(3x) / 4 on the
+    slow index, by(y) = 1 + y^2 on the fast index."""
+    t = grid(level)
+    return 1.0 + t / 2 + np.sin(3 * t) / 4, 1.0 + t ** 2
+
+
+def var_data(level):
+    """u = sin(pi x) sin(pi y) + x y with the profiles of var_coefficients:
+    f = d/dx (bx u_x) + d/dy (by u_y), Dirichlet gW / gE = u, Neumann fluxes
+    gS = -by u_y at y = 0, gN = by u_y at y = 1.  Returns (bx, by, f, gW, gE, gS, gN, u)."""
+    t = grid(level)
+    yi = t[:, None]
+    xj = t[None, :]
+    bx = 1.0 + xj / 2 + np.sin(3 * xj) / 4
+    dbx = 0.5 + 0.75 * np.cos(3 * xj)
+    by = 1.0 + yi ** 2
+    dby = 2 * yi
+    pi = np.pi
+    U = np.sin(pi * xj) * np.sin(pi * yi) + xj * yi
+    ux = pi * np.cos(pi * xj) * np.sin(pi * yi) + yi
+    uxx = -pi ** 2 * np.sin(pi * xj) * np.sin(pi * yi)
+    uy = pi * np.sin(pi * xj) * np.cos(pi * yi) + xj
+    uyy = -pi ** 2 * np.sin(pi * xj) * np.sin(pi * yi)
+    F = dbx * ux + bx * uxx + dby * uy + by * uyy
+    bxs, bys = var_coefficients(level)
+    gW = t * 0.0                                   # u(0, y) = 0
+    gE = t.copy()                                  # u(1, y) = y
+    gS = -(1.0 + 0.0) * (pi * np.sin(pi * t) + t)  # -by(0) u_y(x, 0)
+    gN = (1.0 + 1.0) * (-pi * np.sin(pi * t) + t)  #  by(1) u_y(x, 1)
+    return bxs, bys, F.ravel(order="F"), gW, gE, gS, gN, U.ravel(order="F")
+
+
+def one_d_factors_var(p, N, b):
+    """Dimensionless variable-coefficient factors (hw, m, k): m = h M(b),
+    k = m - (E bs_b + bs_b^T E) + 13 b_face E with bs_b the b-weighted
+    boundary rows, so that A = kron(k(bx), diag(hw)) + kron(diag(hw), m(by))."""
+    D, S0, SN, HI, H, M, r = variable_diagonal_sbp_D2(p, N - 1, b, xc=(0, 1))
+    h = 1.0 / (N - 1)
+    m = h * M.toarray()
+    BS = h * (SN - S0).toarray()
+    E = np.zeros((N, N)); E[0, 0] = b[0]; E[-1, -1] = b[-1]
+    k = m - (np.diag((np.arange(N) == 0) | (np.arange(N) == N - 1)).astype(float) @ BS + BS.T @ np.diag(
+        ((np.arange(N) == 0) | (np.arange(N) == N - 1)).astype(float))) + 13.0 * E
+    return H.diagonal() / h, m, k
 
 
 def assembling_matrix(level, p=2):

@@ -309,6 +309,66 @@ def test_two_level_richardson_restatement():
     assert np.linalg.norm(x - xd) < 1e-8 * np.linalg.norm(xd)
 
 
+@pytest.mark.parametrize("p", [2, 4])
+def test_variable_d2_pins(p):
+    """variable_diagonal_sbp_D2 (diagonal_sbp.jl:470-740): B = 1 gives the
+    constant-coefficient D2 exactly; for a varying B, M is symmetric positive
+    semidefinite with M 1 = 0 (every b_k's weights cancel along a row), and D
+    approximates (B u')' to order p inside and p/2 at the faces."""
+    N = 40
+    D2 = O.diagonal_sbp_D2(p, N, xc=(0, 1))[0]
+    Dv = O.variable_diagonal_sbp_D2(p, N, np.ones(N + 1), xc=(0, 1))[0]
+    assert abs(Dv - D2).max() < 1e-13 * abs(D2).max()
+    r = np.linspace(0, 1, N + 1)
+    B = 1 + 0.5 * np.sin(3 * r) + r ** 2
+    M = O.variable_diagonal_sbp_D2(p, N, B, xc=(0, 1))[5].toarray()
+    assert np.abs(M - M.T).max() == 0.0
+    assert np.abs(M.sum(axis=1)).max() < 1e-13 * np.abs(M).max()
+    assert np.linalg.eigvalsh(M).min() > -1e-12 * np.abs(M).max()
+    ei, eb = [], []
+    bm = 1 if p == 2 else 6
+    for n in (32, 64, 128):
+        x = np.linspace(0, 1, n + 1)
+        Bb = 1 + x ** 2
+        D = O.variable_diagonal_sbp_D2(p, n, Bb, xc=(0, 1))[0]
+        err = np.abs(D @ np.sin(2 * x) - (4 * x * np.cos(2 * x) - 4 * Bb * np.sin(2 * x)))
+        ei.append(err[2 * bm:-2 * bm].max())
+        eb.append(err.max())
+    ri = math.log2(ei[1] / ei[2]); rb = math.log2(eb[1] / eb[2])
+    assert abs(ri - p) < 0.25, ri
+    assert abs(rb - p / 2) < 0.25, rb
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_variable_2d_operator(p):
+    """The 2-D variable-coefficient operator: SPD, equal to the constant one
+    at b = 1, equal to the tensor form k(bx) (x) hw + hw (x) m(by), and the
+    manufactured solution converges at rate 2 (p = 2) and about 4.2 (p = 4)."""
+    level = 4
+    bx, by = O.var_coefficients(level)
+    A = O.assemble_operator_var(level, p, bx, by)
+    Ad = A.toarray()
+    assert np.abs(Ad - Ad.T).max() < 1e-13 * np.abs(Ad).max()
+    assert np.linalg.eigvalsh(Ad).min() > 0
+    one = np.ones(2 ** level + 1)
+    assert abs(O.assemble_operator_var(level, p, one, one) - O.assemble_operator(level, p)).max() < 1e-13 * abs(A).max()
+    N = 2 ** level + 1
+    hw, mx, kx = O.one_d_factors_var(p, N, bx)
+    _, my, ky = O.one_d_factors_var(p, N, by)
+    C = np.kron(kx, np.diag(hw)) + np.kron(np.diag(hw), my)
+    assert np.abs(C - Ad).max() < 1e-12 * np.abs(Ad).max()
+    errs = []
+    for lv in (5, 6, 7):
+        bx, by, f, gW, gE, gS, gN, u = O.var_data(lv)
+        Av = O.assemble_operator_var(lv, p, bx, by)
+        b = O.assemble_rhs_var(lv, p, bx, by, f, gW, gE, gS, gN)
+        x = spla.spsolve(Av.tocsc(), b)
+        errs.append(O.h_norm(x - u, O.h_tilde_diag(lv, p)))
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(2)]
+    want = 2.0 if p == 2 else 4.2
+    assert all(abs(r - want) < 0.2 for r in rates), rates
+
+
 def test_fourier_modes_deterministic():
     s1, b1 = O.fourier_modes(123)
     s2, b2 = O.fourier_modes(123)
