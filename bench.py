@@ -48,7 +48,7 @@ def parse():
                     help="mixed: relative tolerance of each FP32 inner solve")
     ap.add_argument("--ir-rtol", type=float, default=0.0,
                     help="mixed: also stop when the correction-based error estimate <= ir_rtol ||x||")
-    ap.add_argument("--no-extra", action="store_true", help="skip the 8193^2, CSR comparison, initial-guess CG and reference-algorithm legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 8193^2, CSR comparison, initial-guess CG, reference-algorithm and variable-coefficient legs")
     ap.add_argument("--tol-sweep", action="store_true",
                     help="print iterations, error and time per stopping rule for both legs, then exit")
     ap.add_argument("--apply-reps", type=int, default=200)
@@ -316,6 +316,23 @@ def reference_modes_leg(a, dev, torch, N=1025, p=2):
     return out
 
 
+def variable_coefficient_leg(a, dev, torch, N=4097, p=4):
+    """The variable-coefficient apply (sbp_create_var) at N^2 beside the
+    constant-coefficient one: time per launch over rotating buffers and the
+    algorithmic rate (2 n 8 bytes; the per-row coefficient bands are N (2H+1)
+    doubles per axis and stay cache-resident)."""
+    import numpy as np
+    from sc_2022_amd.solver import SbpPoisson
+    t = np.linspace(0.0, 1.0, N)
+    hv = SbpPoisson(N, p, device=dev, coef=(1.0 + t / 2 + np.sin(3 * t) / 4, 1.0 + t ** 2))
+    n = N * N
+    nbuf = max(2, int((2 * 1024 ** 3) // (2 * n * 8)) + 1)
+    ms = time_apply(hv, torch, torch.float64, 100, nbuf)
+    hv.close()
+    return {"workload": f"N={N}, p={p}, bx = 1 + x/2 + sin(3x)/4, by = 1 + y^2", "apply_us": ms * 1e3,
+            "apply_GBs": 2 * n * 8 / (ms * 1e-3) / 1e9}
+
+
 def tol_sweep(a, h, b, x, bnorm, rel_err, torch):
     """Iterations, relative H-norm error and time of both legs over a grid of
     stopping rules (one JSON line each), to pick the loosest rule per leg
@@ -452,7 +469,8 @@ def main():
         for name, fn in (("csr_spmv", lambda: csr_leg(h, torch, apply_ms64 * 1e3)),
                          ("n8193", lambda: leg_8193(a, dev, torch)),
                          ("initial_guess_cg", lambda: nested_leg(a, dev, torch)),
-                         ("reference_algorithms", lambda: reference_modes_leg(a, dev, torch))):
+                         ("reference_algorithms", lambda: reference_modes_leg(a, dev, torch)),
+                         ("variable_coefficient", lambda: variable_coefficient_leg(a, dev, torch))):
             try:
                 extra[name] = fn()
             except Exception as e:  # reported, never silently replaced

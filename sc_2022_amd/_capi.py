@@ -82,6 +82,7 @@ SIGNATURES = {
     "sbp_mgpcg": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd, _vp]),
     "sbp_mgpcg_mixed": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pi, _pd]),
     "sbp_cg_interp": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _d, _i, _pi, _pd, _vp]),
+    "sbp_create_var": (_i, [_i, _i, _pd, _pd, C.POINTER(SbpParams), C.POINTER(_vp)]),
     "sbp_mgpcg_batch": (_i, [_vp, _vp, _vp, _pd, _i, _pi, _pd, _pd, _pi]),
     "sbp_mgpcg_mixed_batch": (_i, [_vp, _vp, _vp, _pd, _i, _pi, _pi, _pd, _pd, _pi]),
     "sbp_adjoint_grad": (_i, [_vp, _vp, _vp, _d, _i, _pd, _vp, _vp, _vp, _vp, _vp, _vp, _pi]),

@@ -23,6 +23,7 @@
 
 #include "../../include/sbp.h"
 #include "sbp_bottom.h"
+#include "sbp_var.h"
 #include "sbp_kernels.h"
 #include "sbp_launch.h"
 #include "sbp_tables.h"
@@ -94,6 +95,10 @@ struct sbp_handle_s {
     bool coarse_nested = false;
     sbp_handle cexact = nullptr;
     double* cbuf = nullptr;
+    // Variable-coefficient operator (sbp_create_var): device table and bands.
+    bool var = false;
+    VarTab* vtab = nullptr;
+    double* vband = nullptr;
 };
 
 namespace {
@@ -992,6 +997,8 @@ int sbp_destroy(sbp_handle h) {
     if (h->cap) (void)hipStreamDestroy(h->cap);
     if (h->child) (void)sbp_destroy(h->child);
     if (h->cexact) (void)sbp_destroy(h->cexact);
+    if (h->vtab) (void)hipFree(h->vtab);
+    if (h->vband) (void)hipFree(h->vband);
     void* ptrs[] = {h->cbuf, h->xc, h->w, h->lam, h->Ainv, h->r, h->pv, h->pv2, h->Ap, h->z, h->x, h->r32, h->p32, h->p32b, h->Ap32,
                     h->z32, h->d32, h->partial, h->sc, h->sc_in, h->dtmp, h->dabs};
     for (void* q : ptrs)
@@ -1002,7 +1009,16 @@ int sbp_destroy(sbp_handle h) {
     return SBP_OK;
 }
 
+namespace {
+int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_handle* out);
+}
+
 int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle* out) {
+    return create_impl(N, p, nprob, prm, false, out);
+}
+
+namespace {
+int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_handle* out) {
     if (!out) return SBP_ERR_INVALID_ARG;
     *out = nullptr;
     if (p != 2 && p != 4 && p != 6 && p != 8) return SBP_ERR_INVALID_ORDER;
@@ -1036,7 +1052,7 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
     int nlev = 1;
     const int cn = h->prm.coarse_n;
     auto pow2p1 = [](int v) { return v >= 3 && ((v - 1) & (v - 2)) == 0; };
-    if (pow2p1(N) && pow2p1(cn) && cn <= N && (cn <= 33 || nprob == 1)) {
+    if (!no_mg && pow2p1(N) && pow2p1(cn) && cn <= N && (cn <= 33 || nprob == 1)) {
         int n = N;
         while (n > cn && nlev < kMaxLevels) {
             n = (n + 1) / 2;
@@ -1136,8 +1152,51 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
     *out = h;
     return SBP_OK;
 }
+}  // namespace
 
 int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) { return sbp_create_batch(N, p, 1, prm, out); }
+
+int sbp_create_var(int N, int p, const double* bx, const double* by, const sbp_params* prm, sbp_handle* out) {
+    if (!out) return SBP_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (p != 2 && p != 4) return SBP_ERR_INVALID_ORDER;
+    if (!bx || !by) return SBP_ERR_INVALID_ARG;
+    for (int k = 0; k < N; ++k)
+        if (!(bx[k] > 0.0) || !(by[k] > 0.0) || !std::isfinite(bx[k]) || !std::isfinite(by[k]))
+            return SBP_ERR_INVALID_ARG;
+    VarTab t{};
+    std::vector<double> bk, bm;
+    if (N < 2 || !build_var_axis(p, N, bx, true, &t.k, bk) || !build_var_axis(p, N, by, false, &t.m, bm))
+        return SBP_ERR_INVALID_SIZE;
+    sbp_handle h = nullptr;
+    STCHK(create_impl(N, p, 1, prm, true, &h));
+    const OpTab& tb = h->L[0].tb;
+    t.N = N;
+    t.H = p / 2;
+    t.R = p == 2 ? 3 : 6;
+    t.C = p == 2 ? 4 : 8;
+    t.RH = tb.RH;
+    for (int r = 0; r < kMaxR; ++r) t.hw[r] = tb.hw[r];
+    const size_t nb = bk.size();
+    int rc = SBP_OK;
+    if (hipMalloc(&h->vband, 2 * nb * sizeof(double)) != hipSuccess ||
+        hipMalloc(&h->vtab, sizeof(VarTab)) != hipSuccess) {
+        rc = SBP_ERR_HIP;
+    } else {
+        t.k.band = h->vband;
+        t.m.band = h->vband + nb;
+        if (hipMemcpy(h->vband, bk.data(), nb * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->vband + nb, bm.data(), nb * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->vtab, &t, sizeof(VarTab), hipMemcpyHostToDevice) != hipSuccess)
+            rc = SBP_ERR_HIP;
+    }
+    if (rc != SBP_OK) { sbp_destroy(h); return rc; }
+    h->var = true;
+    h->rt.bW = bx[0];
+    h->rt.bE = bx[N - 1];
+    *out = h;
+    return SBP_OK;
+}
 
 int sbp_set_stream(sbp_handle h, void* s) {
     if (!h) return SBP_ERR_INVALID_ARG;
@@ -1174,8 +1233,73 @@ int sbp_set_tiles(sbp_handle h, int level, int dtype, int ti, int tj) {
     return SBP_OK;
 }
 
+namespace {
+// One launch of the variable-coefficient operator (MODE as var_apply);
+// *nparts receives the number of block partials.
+int launch_var(sbp_handle h, int mode, const double* u, double* out, const double* aux, double sign,
+               const Scalars* sc, int* nparts) {
+    const int N = h->N;
+    const dim3 g((N + kVarThreads - 1) / kVarThreads, (N + kVarRows - 1) / kVarRows);
+    if ((size_t)g.x * g.y > h->npp) return SBP_ERR_INVALID_SIZE;
+#define SBP_VAR_LAUNCH(P_, M_) \
+    hipLaunchKernelGGL((var_apply<P_, M_>), g, dim3(kVarThreads), 0, h->st, u, out, aux, sign, h->partial, \
+                       (const VarTab*)h->vtab, sc)
+    if (h->p == 2) {
+        if (mode == 0) SBP_VAR_LAUNCH(2, 0); else if (mode == 1) SBP_VAR_LAUNCH(2, 1); else SBP_VAR_LAUNCH(2, 2);
+    } else {
+        if (mode == 0) SBP_VAR_LAUNCH(4, 0); else if (mode == 1) SBP_VAR_LAUNCH(4, 1); else SBP_VAR_LAUNCH(4, 2);
+    }
+#undef SBP_VAR_LAUNCH
+    HIPCHK(hipGetLastError());
+    if (nparts) *nparts = (int)(g.x * g.y);
+    return SBP_OK;
+}
+
+// CG on the variable-coefficient operator (unfused: product with p'Ap,
+// alpha, x and r updates with r'r, check, direction update), polled as the
+// other solvers.
+int var_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, double* hist) {
+    hipStream_t st = h->st;
+    const size_t n = h->L[0].n;
+    Scalars* sc = h->sc;
+    STCHK(put_abstol(h, &abstol));
+    hipLaunchKernelGGL(stamp_start, dim3(1), dim3(256), 0, st, sc, 1);
+    int np = 0;
+    STCHK(launch_var(h, 2, x, h->r, b, 1.0, nullptr, &np));
+    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, (const double*)h->dabs,
+                       hist, maxiter);
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n), 1), dim3(kEwThreads), 0, st,
+                       (const double*)h->r, h->pv, n, (const Scalars*)sc);
+    HIPCHK(hipGetLastError());
+    const bool adaptive = h->prm.poll_every == 0;
+    int next = adaptive ? 2 : h->prm.poll_every;
+    const dim3 g(ew_blocks(n), 1);
+    for (int it = 0; it < maxiter; ++it) {
+        STCHK(launch_var(h, 1, h->pv, h->Ap, h->pv, 1.0, sc, &np));
+        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, 0);
+        hipLaunchKernelGGL((update_xr<double, kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, (const double*)h->pv,
+                           (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
+        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 1);
+        hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)h->r,
+                           (const Scalars*)sc, n);
+        HIPCHK(hipGetLastError());
+        if (it + 1 >= next && it + 1 < maxiter) {
+            bool all = false;
+            STCHK(poll(h, sc, &all));
+            if (all) break;
+            next = it + 1 + (adaptive ? poll_ahead(h) : h->prm.poll_every);
+        }
+    }
+    return poll(h, sc);
+}
+}  // namespace
+
 int sbp_apply(sbp_handle h, int level, const void* u, void* Au, int sign, int dtype) {
     if (!h || !u || !Au || (sign != 1 && sign != -1) || level < 0 || level >= h->nlev) return SBP_ERR_INVALID_ARG;
+    if (h->var) {   // variable coefficients: FP64, level 0
+        if (dtype != SBP_F64 || level != 0) return SBP_ERR_INVALID_ARG;
+        return launch_var(h, 0, (const double*)u, (double*)Au, nullptr, (double)sign, nullptr, nullptr);
+    }
     if (dtype == SBP_F64)
         return launch_pass<double>(h, level, InPlain<double>{(const double*)u},
                                    OutStore<double>{(double*)Au, (double)sign}, nullptr, nullptr, nullptr);
@@ -1188,7 +1312,8 @@ int sbp_apply(sbp_handle h, int level, const void* u, void* Au, int sign, int dt
 int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot) {
     if (!h || !u || !Au || !dot) return SBP_ERR_INVALID_ARG;
     int np = 0;
-    STCHK(launch_pass<double>(h, 0, InPlain<double>{u}, OutStoreDot<double>{Au}, nullptr, h->partial, &np));
+    if (h->var) STCHK(launch_var(h, 1, u, Au, u, 1.0, nullptr, &np));
+    else STCHK(launch_pass<double>(h, 0, InPlain<double>{u}, OutStoreDot<double>{Au}, nullptr, h->partial, &np));
     hipLaunchKernelGGL(reduce_to, dim3(h->nprob), dim3(kRedThreads), 0, h->st, h->partial, np, h->dtmp, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(dot, h->dtmp, sizeof(double) * h->nprob, hipMemcpyDeviceToHost, h->st));
@@ -1197,7 +1322,7 @@ int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot) {
 }
 
 int sbp_diag(sbp_handle h, double* d) {
-    if (!h || !d) return SBP_ERR_INVALID_ARG;
+    if (!h || !d || h->var) return SBP_ERR_INVALID_ARG;
     hipLaunchKernelGGL(diag_kernel, dim3(ew_blocks(h->L[0].n)), dim3(256), 0, h->st, d, h->L[0].tb);
     HIPCHK(hipGetLastError());
     return SBP_OK;
@@ -1281,7 +1406,7 @@ int sbp_prolong_add(sbp_handle h, int level, const void* ec, void* xf, int dtype
 }
 
 int sbp_smooth(sbp_handle h, int level, const void* b, void* x, int nsweeps, int dtype) {
-    if (!h || !b || !x || level < 0 || level >= h->nlev || nsweeps < 0) return SBP_ERR_INVALID_ARG;
+    if (!h || !b || !x || level < 0 || level >= h->nlev || nsweeps < 0 || h->var) return SBP_ERR_INVALID_ARG;
     if (dtype == SBP_F64) return smooth_t<double>(h, level, (const double*)b, (double*)x, nsweeps);
     if (dtype == SBP_F32) {
         STCHK(ensure32(h));
@@ -1312,6 +1437,11 @@ int sbp_vcycle(sbp_handle h, const void* r, void* z, int dtype) {
 int sbp_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
            double* hist) {
     if (!h || !b || !x || maxiter < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
+    if (h->var) {
+        STCHK(var_cg(h, b, x, abstol, maxiter, hist));
+        get_results(h, iters, resnorm, nullptr);
+        return batch_status(h);
+    }
     STCHK(krylov64(h, b, x, &abstol, maxiter, hist, false));
     get_results(h, iters, resnorm, nullptr);
     return batch_status(h);
@@ -1468,7 +1598,7 @@ int sbp_cg_interp(sbp_handle h, int nlev, const double* b, const double* b2h, co
 // CSR comparison path (one problem)
 // ---------------------------------------------------------------------------
 int sbp_csr_assemble(sbp_handle h, int32_t* row_ptr, int32_t* col, double* val, int64_t* nnz) {
-    if (!h || !nnz) return SBP_ERR_INVALID_ARG;
+    if (!h || !nnz || h->var) return SBP_ERR_INVALID_ARG;
     const int n = (int)h->L[0].n;
     int32_t* counts = nullptr;
     int32_t* rp = row_ptr;

@@ -404,6 +404,7 @@ struct RhsTab {
     double bs[kMaxR];   // boundary-derivative row (dimensionless, h * BS)
     int nbs;
     double h;
+    double bW, bE;      // Dirichlet-face coefficients (variable-coefficient operator; 1 otherwise)
 };
 
 __device__ __forceinline__ double dir_w(const RhsTab& r, int k) {
@@ -421,8 +422,8 @@ __global__ __launch_bounds__(256) void assemble_rhs(const double* f, const doubl
         const int i = (int)(q % N), j = (int)(q / N);
         const double hi = hw_of<double>(t, i), hj = hw_of<double>(t, j);
         double v = f ? -rt.h * rt.h * hi * hj * f[q] : 0.0;
-        if (gW && j < rt.nbs) v += hi * dir_w(rt, j) * gW[i];
-        if (gE && N - 1 - j < rt.nbs) v += hi * dir_w(rt, N - 1 - j) * gE[i];
+        if (gW && j < rt.nbs) v += rt.bW * hi * dir_w(rt, j) * gW[i];
+        if (gE && N - 1 - j < rt.nbs) v += rt.bE * hi * dir_w(rt, N - 1 - j) * gE[i];
         if (gS && i == 0) v += rt.h * hj * gS[j];
         if (gN && i == N - 1) v += rt.h * hj * gN[j];
         b[q] = v;
@@ -467,12 +468,12 @@ __global__ __launch_bounds__(256) void rhs_t_g(const double* lam, double* dgW, d
     if (dgW) {
         double s = 0.0;
         for (int k = 0; k < rt.nbs; ++k) s += dir_w(rt, k) * lam[gidx(q, k, N)];
-        dgW[q] = hq * s;
+        dgW[q] = rt.bW * hq * s;
     }
     if (dgE) {
         double s = 0.0;
         for (int k = 0; k < rt.nbs; ++k) s += dir_w(rt, k) * lam[gidx(q, N - 1 - k, N)];
-        dgE[q] = hq * s;
+        dgE[q] = rt.bE * hq * s;
     }
     if (dgS) dgS[q] = rt.h * hq * lam[gidx(0, q, N)];
     if (dgN) dgN[q] = rt.h * hq * lam[gidx(N - 1, q, N)];

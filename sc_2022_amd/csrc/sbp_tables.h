@@ -181,6 +181,7 @@ inline bool build_optab(int p, int N, OpTab* t, RhsTab* rt) {
         rt->nbs = s.nbs;
         for (int c = 0; c < s.nbs; ++c) rt->bs[c] = s.bs[c];
         rt->h = 1.0 / (N - 1);
+        rt->bW = rt->bE = 1.0;
     }
     return true;
 }

@@ -43,7 +43,11 @@ class SbpPoisson:
     """Matrix-free SBP-SAT operator of order p on an N x N grid (N = 2^k+1 for
     multigrid).  Keyword arguments override sbp_params fields."""
 
-    def __init__(self, N: int, p: int = 2, device=None, tuned: bool = True, **params):
+    def __init__(self, N: int, p: int = 2, device=None, tuned: bool = True, coef=None, **params):
+        """coef = (bx, by): the variable-coefficient operator d/dx (bx d/dx) +
+        d/dy (by d/dy) with N-value profiles along the slow (Dirichlet) and
+        fast (Neumann) index (sbp_create_var; p = 2, 4; apply, CG and the
+        right-hand side only)."""
         self.lib = capi.load()
         self.N = int(N)
         self.p = int(p)
@@ -52,7 +56,17 @@ class SbpPoisson:
         self.params = capi.default_params(**params)
         h = C.c_void_p()
         with torch.cuda.device(self.device):
-            capi.check(self.lib.sbp_create(self.N, self.p, C.byref(self.params), C.byref(h)), "sbp_create")
+            if coef is None:
+                capi.check(self.lib.sbp_create(self.N, self.p, C.byref(self.params), C.byref(h)), "sbp_create")
+            else:
+                import numpy as np
+                bx, by = (np.ascontiguousarray(np.asarray(c, dtype=np.float64)) for c in coef)
+                if bx.shape != (self.N,) or by.shape != (self.N,):
+                    raise ValueError("coef: two arrays of N values")
+                dp = C.POINTER(C.c_double)
+                capi.check(self.lib.sbp_create_var(self.N, self.p, bx.ctypes.data_as(dp), by.ctypes.data_as(dp),
+                                                   C.byref(self.params), C.byref(h)), "sbp_create_var")
+                tuned = False
         self._h = h
         nl = C.c_int()
         self.lib.sbp_get_info(self._h, None, None, C.byref(nl))

@@ -1,0 +1,128 @@
+"""GPU parity of the variable-coefficient operator (sbp_create_var; the
+reference's variable_diagonal_sbp_D2 on both axes) against the oracle's
+restatement: apply (both signs, with the dot), right-hand side and its
+transpose, CG, the constant-coefficient limit, and the manufactured-solution
+convergence rates."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import sbp_oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def T(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def relmax(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def profiles(level, seed):
+    t = O.grid(level)
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.2, 1.0, 3)
+    bx = 1.0 + a[0] * np.sin(3 * t + a[1]) ** 2 + t
+    by = 0.5 + a[2] * t ** 2 + 0.3 * np.cos(2 * t) ** 2
+    return bx, by
+
+
+@pytest.fixture(scope="module")
+def SP():
+    from sc_2022_amd.solver import SbpPoisson
+    return SbpPoisson
+
+
+@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("level", [5, 6, 8])
+def test_var_apply_matches_oracle(SP, level, p):
+    N = 2 ** level + 1
+    bx, by = profiles(level, level * 10 + p)
+    A = O.assemble_operator_var(level, p, bx, by)
+    h = SP(N, p, coef=(bx, by))
+    u = np.random.default_rng(level + p).standard_normal(N * N)
+    Au = A @ u
+    assert relmax(h.apply(T(u)).cpu().numpy(), Au) < 1e-13
+    assert relmax(h.apply(T(u), sign=-1).cpu().numpy(), -Au) < 1e-13
+    out, d = h.apply_dot(T(u))
+    assert relmax(out.cpu().numpy(), Au) < 1e-13
+    assert abs(d - u @ Au) < 1e-12 * abs(u @ Au)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_var_constant_limit_and_rhs(SP, p):
+    """b = 1 gives the constant-coefficient handle's apply and right-hand
+    side; with varying b the right-hand side and its transpose match the
+    oracle (Dirichlet data scaled by the face coefficient, Neumann data as
+    fluxes)."""
+    level = 6
+    N = 2 ** level + 1
+    one = np.ones(N)
+    hv, hc = SP(N, p, coef=(one, one)), SP(N, p)
+    u = np.random.default_rng(1).standard_normal(N * N)
+    assert relmax(hv.apply(T(u)).cpu().numpy(), hc.apply(T(u)).cpu().numpy()) < 1e-13
+    bx, by, f, gW, gE, gS, gN, uu = O.var_data(level)
+    h = SP(N, p, coef=(bx, by))
+    b = h.assemble_rhs(T(f), T(gW), T(gE), T(gS), T(gN)).cpu().numpy()
+    assert relmax(b, O.assemble_rhs_var(level, p, bx, by, f, gW, gE, gS, gN)) < 1e-13
+    lam = np.random.default_rng(2).standard_normal(N * N)
+    df, dgW, dgE, dgS, dgN = (t.cpu().numpy() for t in h.rhs_transpose(T(lam)))
+    # <b(f, g), lam> = <f, df> + <gW, dgW> + ... (the map is linear in (f, g))
+    lhs = O.assemble_rhs_var(level, p, bx, by, f, gW, gE, gS, gN) @ lam
+    rhs = f @ df + gW @ dgW + gE @ dgE + gS @ dgS + gN @ dgN
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_var_cg_against_oracle(SP, p):
+    level = 6
+    N = 2 ** level + 1
+    bx, by, f, gW, gE, gS, gN, u = O.var_data(level)
+    A = O.assemble_operator_var(level, p, bx, by)
+    b = O.assemble_rhs_var(level, p, bx, by, f, gW, gE, gS, gN)
+    tol = 1e-10 * np.linalg.norm(b)
+    xo, ito, _ = O.cg_cpu(A, b, abstol=tol)
+    h = SP(N, p, coef=(bx, by))
+    x, res = h.cg(T(b), abstol=tol, history=True)
+    assert res.converged and abs(res.iters - ito) <= max(2, 0.01 * ito), (res.iters, ito)
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(A @ xg - b) < 1.000001 * tol
+    assert np.linalg.norm(xg - xo) < 1e-7 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("p,rate,levels", [(2, 2.0, (6, 7, 8)), (4, 4.2, (5, 6, 7))])
+def test_var_manufactured_rate_on_gpu(SP, p, rate, levels):
+    errs = []
+    for level in levels:
+        N = 2 ** level + 1
+        bx, by, f, gW, gE, gS, gN, u = O.var_data(level)
+        h = SP(N, p, coef=(bx, by))
+        b = h.assemble_rhs(T(f), T(gW), T(gE), T(gS), T(gN))
+        x, res = h.cg(b, abstol=1e-13 * float(torch.linalg.vector_norm(b)))
+        assert res.converged
+        errs.append(O.h_norm(x.cpu().numpy() - u, O.h_tilde_diag(level, p)))
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(2)]
+    assert all(abs(r - rate) < 0.25 for r in rates), (errs, rates)
+
+
+def test_var_handle_rejects_what_it_does_not_serve(SP):
+    from sc_2022_amd import _capi as capi
+    N = 65
+    bx, by = profiles(6, 0)
+    h = SP(N, 4, coef=(bx, by))
+    with pytest.raises(Exception):
+        h.diag()
+    with pytest.raises(Exception):
+        h.apply(T(np.zeros(N * N), torch.float32))
+    x, res = h.mgpcg(T(np.ones(N * N)), abstol=1e-8, raise_on_fail=False)
+    assert res.status == capi.SBP_ERR_INVALID_SIZE
+    with pytest.raises(Exception):
+        SP(N, 4, coef=(bx, -by))            # coefficients must be positive
+    with pytest.raises(Exception):
+        SP(N, 6, coef=(bx, by))             # orders 2 and 4 only
