@@ -202,57 +202,84 @@ __device__ __forceinline__ void var_emit(double* out, const double* aux, double 
     }
 }
 
-// Blocks whose rows and columns (with halo) avoid the closure rows take the
-// tiled path of the constant-coefficient pass: each thread keeps its column
-// window u(i, j0-H .. j0+kVarRows-1+H) in registers for the j-direction, an
-// LDS tile serves the i-direction, the i-band coefficients are loaded once
-// per thread and the j-band ones once per row (uniform); H weights are 1
-// there.  Face blocks evaluate per point through the table staged in LDS.
+// Tiled evaluation for blocks whose rows (with halo) avoid the closure rows:
+// each thread keeps its column window u(i, j0-H .. j0+kVarRows-1+H) in
+// registers for the j-direction, an LDS tile serves the i-direction, the
+// i-band coefficients are loaded once per thread and the j-band ones once
+// per row (uniform); H weights along j are 1 there.  With ICHK (the first
+// and last column of blocks) each thread tests its column: out-of-grid
+// threads idle, threads on the closure columns take the closure row of m
+// (direct loads, table t staged in LDS) and their H weight.
+template <int P, int MODE, bool ICHK>
+__device__ __forceinline__ double var_tile(const double* __restrict__ u, double* __restrict__ out,
+                                           const double* __restrict__ aux, double sign, const VarTab* __restrict__ tp,
+                                           const VarTab& t, int i0, int j0,
+                                           double (*tile)[kVarThreads + 2 * (P / 2)]) {
+    constexpr int H = P / 2;
+    constexpr int J = kVarRows;
+    constexpr int W = J + 2 * H;
+    const int N = tp->N, R = tp->R;
+    const int tid = threadIdx.x;
+    const int i = i0 + tid;
+    const bool act = !ICHK || i < N;
+    const bool iint = !ICHK || (i >= R && i < N - R);
+    const double* kb = tp->k.band;
+    const double* mb = tp->m.band;
+    double w[W];
+    double cm[2 * H + 1];
+#pragma unroll
+    for (int k = 0; k < W; ++k) w[k] = act ? u[gidx(i, j0 - H + k, N)] : 0.0;
+#pragma unroll
+    for (int q = 0; q <= 2 * H; ++q) cm[q] = iint ? mb[(size_t)q * N + i] : 0.0;
+    if (tid < 2 * H * J) {   // halo columns: H left, H right, J rows
+        const int side = tid / (H * J), rem = tid % (H * J), off = rem / J, r = rem % J;
+        const int ii = side == 0 ? i0 - H + off : i0 + kVarThreads + off;
+        tile[r][side == 0 ? off : kVarThreads + H + off] = (!ICHK || (ii >= 0 && ii < N)) ? u[gidx(ii, j0 + r, N)] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < J; ++r) tile[r][tid + H] = w[r + H];
+    __syncthreads();
+    double acc = 0.0;
+    if (!act) return acc;
+    double hwi = 1.0;
+    if (ICHK) {
+        const int ri = i < N - 1 - i ? i : N - 1 - i;
+        hwi = ri < t.RH ? t.hw[ri] : 1.0;
+    }
+#pragma unroll
+    for (int r = 0; r < J; ++r) {
+        const int j = j0 + r;
+        double kp = 0.0, mp = 0.0;
+#pragma unroll
+        for (int q = 0; q <= 2 * H; ++q) kp += kb[(size_t)q * N + j] * w[r + q];
+        if (iint) {
+#pragma unroll
+            for (int q = 0; q <= 2 * H; ++q) mp += cm[q] * tile[r][tid + q];
+        } else {
+            mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
+        }
+        var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + mp, acc);
+    }
+    return acc;
+}
+
 template <int P, int MODE>
 __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restrict__ u, double* __restrict__ out,
                                                         const double* __restrict__ aux, double sign, double* partial,
                                                         const VarTab* __restrict__ tp, const Scalars* sc) {
     constexpr int H = P / 2;
     constexpr int J = kVarRows;
-    constexpr int W = J + 2 * H;
     if (sc && sc->done) return;   // the whole launch: a converged solve
     __shared__ VarTab t;
     __shared__ double tile[J][kVarThreads + 2 * H];
     const int N = tp->N, R = tp->R;
     const int tid = threadIdx.x;
     const int i0 = blockIdx.x * kVarThreads, j0 = blockIdx.y * J;
-    const int i = i0 + tid;
     const bool jin = j0 - H >= R && j0 + J + H <= N - R;
     const bool iin = i0 - H >= R && i0 + kVarThreads + H <= N - R;
     double acc = 0.0;
     if (jin && iin) {
-        const double* kb = tp->k.band;
-        const double* mb = tp->m.band;
-        double w[W];
-#pragma unroll
-        for (int k = 0; k < W; ++k) w[k] = u[gidx(i, j0 - H + k, N)];
-        double cm[2 * H + 1];
-#pragma unroll
-        for (int q = 0; q <= 2 * H; ++q) cm[q] = mb[(size_t)q * N + i];
-        if (tid < 2 * H * J) {   // halo columns: H left, H right, J rows
-            const int side = tid / (H * J), rem = tid % (H * J), off = rem / J, r = rem % J;
-            const int ii = side == 0 ? i0 - H + off : i0 + kVarThreads + off;
-            tile[r][side == 0 ? off : kVarThreads + H + off] = u[gidx(ii, j0 + r, N)];
-        }
-#pragma unroll
-        for (int r = 0; r < J; ++r) tile[r][tid + H] = w[r + H];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < J; ++r) {
-            const int j = j0 + r;
-            double kp = 0.0, mp = 0.0;
-#pragma unroll
-            for (int q = 0; q <= 2 * H; ++q) {
-                kp += kb[(size_t)q * N + j] * w[r + q];
-                mp += cm[q] * tile[r][tid + q];
-            }
-            var_emit<MODE>(out, aux, sign, gidx(i, j, N), kp + mp, acc);
-        }
+        acc = var_tile<P, MODE, false>(u, out, aux, sign, tp, t, i0, j0, tile);
     } else {
         {
             const unsigned long long* src = reinterpret_cast<const unsigned long long*>(tp);
@@ -260,21 +287,27 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
             for (int k = tid; k < (int)(sizeof(VarTab) / 8); k += kVarThreads) dst[k] = src[k];
         }
         __syncthreads();
-        if (i < N) {
-            const int ri = i < N - 1 - i ? i : N - 1 - i;
-            const double hwi = ri < t.RH ? t.hw[ri] : 1.0;
-            for (int r = 0; r < J; ++r) {
-                const int j = j0 + r;
-                if (j >= N) break;
-                const int rj = j < N - 1 - j ? j : N - 1 - j;
-                const double hwj = rj < t.RH ? t.hw[rj] : 1.0;
-                const double kp = var_row<H>(t.k, t.R, t.C, N, j, [&](int c) { return u[gidx(i, c, N)]; });
-                const double mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
-                var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + hwj * mp, acc);
+        if (jin) {
+            acc = var_tile<P, MODE, true>(u, out, aux, sign, tp, t, i0, j0, tile);
+        } else {
+            const int i = i0 + tid;   // face rows: per point
+            if (i < N) {
+                const int ri = i < N - 1 - i ? i : N - 1 - i;
+                const double hwi = ri < t.RH ? t.hw[ri] : 1.0;
+                for (int r = 0; r < J; ++r) {
+                    const int j = j0 + r;
+                    if (j >= N) break;
+                    const int rj = j < N - 1 - j ? j : N - 1 - j;
+                    const double hwj = rj < t.RH ? t.hw[rj] : 1.0;
+                    const double kp = var_row<H>(t.k, t.R, t.C, N, j, [&](int c) { return u[gidx(i, c, N)]; });
+                    const double mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
+                    var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + hwj * mp, acc);
+                }
             }
         }
     }
     if constexpr (MODE != 0) {
+        __syncthreads();
         const double s = block_sum<double, kVarThreads>(acc);
         if (threadIdx.x == 0) partial[blockIdx.x + (size_t)blockIdx.y * gridDim.x] = s;
     }
