@@ -105,7 +105,7 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out);
  * nprob = 1).  No counterpart in the reference, which solves one problem per
  * call of matrix_free_MGCG (initial_guess_cg.jl:303-346). */
 int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle* out);
-/* Variable-coefficient operator d/dx (bx d/dx) + d/dy (by d/dy), p = 2 or 4,
+/* Variable-coefficient operator d/dx (bx d/dx) + d/dy (by d/dy), p = 2, 4 or 6,
  * from variable_diagonal_sbp_D2 (diagonal_sbp.jl:470-740) on both axes as
  * get_operators uses it (legacy_functions.jl:100-174): bx (host, N values)
  * along the slow, Dirichlet index j, by along the fast, Neumann index i, both
@@ -113,8 +113,8 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
  * b = 1) and the Neumann data are the fluxes b du/dn.  Such a handle serves
  * sbp_apply (FP64, level 0), sbp_apply_dot, sbp_cg, sbp_assemble_rhs,
  * sbp_rhs_transpose and sbp_hnorm; the multigrid, smoother, diag and CSR
- * entry points return SBP_ERR_INVALID_ARG / SBP_ERR_INVALID_SIZE.  N >= 10
- * (p = 2) or 18 (p = 4). */
+ * entry points return SBP_ERR_INVALID_ARG / SBP_ERR_INVALID_SIZE.  N >= 10,
+ * 18, 26 for p = 2, 4, 6. */
 int sbp_create_var(int N, int p, const double* bx, const double* by, const sbp_params* prm,
                    sbp_handle* out);
 int sbp_get_batch(sbp_handle h, int* nprob);

@@ -10,7 +10,7 @@ operator, literally (same kron products, same SAT terms):
 * ``diagonal_sbp_D1`` -- diagonal_sbp.jl:56-147 (p = 2, 4, 6, 8)
 * ``diagonal_sbp_D2`` -- diagonal_sbp.jl:189-454 (p = 2 :191-197, p = 4
   :198-208, p = 6 :210-222, p = 8 :224-311, H/HI :421-428, S0/SN :446-448)
-* ``variable_diagonal_sbp_D2`` -- diagonal_sbp.jl:470-740 (p = 2, 4), and
+* ``variable_diagonal_sbp_D2`` -- diagonal_sbp.jl:470-740 (p = 2, 4, 6), and
   ``assemble_operator_var`` / ``assemble_rhs_var`` -- its 2-D use with one
   coefficient profile per axis (legacy_functions.jl:100-174, generalised)
 * ``operators_2d`` / ``assembling_matrix`` -- level_2_multigrid_new.jl:48-188
@@ -211,7 +211,7 @@ def diagonal_sbp_D2(p, N, xc=(-1.0, 1.0)):
 
 
 # ---------------------------------------------------------------------------
-# Variable-coefficient D2 (diagonal_sbp.jl:470-740), orders 2 and 4
+# Variable-coefficient D2 (diagonal_sbp.jl:470-740), orders 2, 4 and 6
 # ---------------------------------------------------------------------------
 
 # p = 4 boundary block of M(B): entry (r, c) (1-based, r <= c, symmetric) is
@@ -242,16 +242,67 @@ _VAR_P4_M0 = {
 }
 
 
+# p = 6 boundary block of M(B): entry (r, c) (1-based, r <= c, symmetric) is
+# sum of w * b_k over the listed (w, k) (diagonal_sbp.jl:577-624).
+_VAR_P6_M0 = {
+    (1, 1): [(0.79126675946955820939, 1), (0.29684720906380007429, 2), (0.0031855190887964290152, 3), (0.016324040425909519534, 4), (0.031603022440944150877, 5), (0.031679647480161052996, 6), (0.031485777339472539205, 7)],
+    (1, 2): [(-1.0166893393503381444, 1), (-0.028456273704916113690, 3), (-0.041280298383492988198, 4), (-0.13922814516201405075, 5), (-0.11957773256112017666, 6), (-0.11942677565293334109, 7)],
+    (1, 3): [(0.070756429372437150463, 1), (-0.18454761060241510503, 2), (-0.043641631471118923470, 4), (0.24323679072077324609, 5), (0.15821270735372154440, 6), (0.16023485783647863076, 7)],
+    (1, 4): [(0.22519915328913532127, 1), (-0.16627487110970548953, 2), (0.027105309616486712977, 3), (-0.19166461859684399091, 5), (-0.076841171601990145944, 6), (-0.082195869498316975759, 7)],
+    (1, 5): [(-0.052244034642020563167, 1), (0.044400639485098762210, 2), (-0.0010239765473093878745, 3), (0.074034846453161740905, 4), (0.012416255689984968954, 6), (0.071886528478926012827, 5), (0.013793629971047355034, 7)],
+    (1, 6): [(-0.018288968138771973527, 1), (0.0095746331632217580607, 2), (-0.00081057845305764042779, 3), (-0.0073488455877755196984, 4), (0.010636019497239069970, 5), (-0.013159670383826183824, 6), (-0.021179364788387535246, 7)],
+    (1, 7): [(0.0019118885633161709274, 4), (-0.040681303555291499361, 5), (0.013196749810737491670, 6), (0.025572665181237836763, 7)],
+    (1, 8): [(0.015596528711367857640, 5), (-0.0064861841573315378995, 6), (-0.0091103445540363197401, 7)],
+    (1, 9): [(0.00055939836966298630593, 6), (-0.0013848225351007963723, 5), (0.00082542416543781006633, 7)],
+    (2, 2): [(1.3063321571116676286, 1), (0.25420017604573457435, 3), (0.10438978280925626095, 4), (0.66723280210321129509, 5), (0.46818193597227494411, 6), (0.46764154101958369201, 7)],
+    (2, 3): [(-0.090914102699924646049, 1), (0.11036113131714764253, 4), (-1.2903975449975188870, 5), (-0.66396052487350447871, 6), (-0.66159744640052061842, 7)],
+    (2, 4): [(-0.28935573956534316666, 1), (-0.24213200040645927216, 3), (1.1876702550280310277, 5), (0.39565981499041363328, 6), (0.38600489217558000007, 7)],
+    (2, 5): [(0.067127744758037639890, 1), (0.0091471926820756301800, 3), (-0.18721961430038080217, 4), (-0.13193585588531745301, 6), (-0.48715757368119118874, 5), (-0.10475163122754481381, 7)],
+    (2, 6): [(0.023499279745900688694, 1), (0.0072409053835651813164, 3), (0.018583789963916794487, 4), (-0.092896161339386761743, 5), (0.12235132704188076670, 6), (0.11135203204362950339, 7)],
+    (2, 7): [(-0.0048347914064469075906, 4), (0.23106838326878204031, 5), (-0.10807741421960079917, 6), (-0.11815617764273433354, 7)],
+    (2, 8): [(-0.083681414344034553537, 5), (0.040934994667670546616, 6), (0.042746419676364006921, 7)],
+    (2, 9): [(-0.0035765451326969831434, 6), (0.0073893991241210786821, 5), (-0.0038128539914240955387, 7)],
+    (3, 3): [(0.0063271611471368738078, 1), (0.11473182007158685275, 2), (0.11667405542796800075, 4), (2.7666108082854440372, 5), (1.0709206899608171042, 6), (1.0131613910329730572, 7)],
+    (3, 4): [(0.020137694138847972466, 1), (0.10337179946308864017, 2), (-2.9132216211517427243, 5), (-0.87558073434822622598, 6), (-0.69099571834888124265, 7)],
+    (3, 5): [(-0.0046717510915754628683, 1), (-0.027603533656377128278, 2), (-0.19792902986208699745, 4), (0.54029853383734330523, 6), (1.2391775930319110779, 5), (0.26280380502473582273, 7)],
+    (3, 6): [(-0.0016354308669218878195, 1), (-0.0059524752758832596197, 2), (0.019646827777442752194, 4), (0.32366400126390466006, 5), (-0.46595166932288709739, 6), (-0.22172727209417368594, 7)],
+    (3, 7): [(-0.0051113531893524745496, 4), (-0.53558781637747543460, 5), (0.33283351044897389336, 6), (0.20786565911785401579, 7)],
+    (3, 8): [(0.18243281741342895622, 5), (-0.10598160301968184459, 6), (-0.076451214393747111630, 7)],
+    (3, 9): [(0.0092090899634437994856, 6), (-0.015915028188724931671, 5), (0.0067059382252811321853, 7)],
+    (4, 4): [(0.064092997759871869867, 1), (0.093136576388046999489, 2), (0.23063676246347492291, 3), (3.6894403082837166203, 5), (1.1905503386876088738, 6), (0.59124795468888565194, 7)],
+    (4, 5): [(-0.014868958192656041286, 1), (-0.024870405993901607642, 2), (-0.0087129289077117541871, 3), (-1.2635078373718242057, 6), (-0.30583173978439973269, 7), (-1.4706919260458029548, 5)],
+    (4, 6): [(-0.0052051474298559556576, 1), (-0.0053630987475285424890, 2), (-0.0068971427657906095463, 3), (-0.78575245216674501017, 5), (0.22911480054237346001, 7), (0.99770643562927505292, 6)],
+    (4, 7): [(0.66972974880676622652, 5), (-0.50132473560721279390, 6), (-0.17951612431066454373, 7)],
+    (4, 8): [(-0.20229090601117515652, 5), (0.14534218580636584986, 6), (0.056948720204809306656, 7)],
+    (4, 9): [(-0.012004296184410038337, 6), (-0.0047769156693859238415, 7), (0.016781211853795962179, 5)],
+    (5, 5): [(0.0034494550959102336252, 1), (0.0066411834994278261016, 2), (0.00032915450832718628585, 3), (0.33577217075764772000, 4), (2.0964133295790264390, 6), (0.23173232041831268550, 7), (0.0061078257643682645765, 8), (0.71091258506833766956, 5)],
+    (5, 6): [(0.0012075440723041938061, 1), (0.0014321166657521476075, 2), (0.00026055826461832559573, 3), (-0.033329411132516353908, 4), (-0.28082416973855326836, 7), (-0.027209080835250836084, 8), (0.10458654356829219874, 5), (-1.3484369866671155432, 6)],
+    (5, 7): [(0.0086710380841746926251, 4), (0.17360734113554285637, 6), (0.053313621252876254126, 8), (-0.24249352624045263018, 5), (0.15690152576785882706, 7)],
+    (5, 8): [(-0.086316839802171222760, 6), (0.026988423604709992435, 7), (0.080981941477156510853, 5), (-0.032764636390806391639, 8)],
+    (5, 9): [(0.0074620594845308550733, 6), (-0.00081216403616686789496, 7), (0.00055227020881270902093, 8), (-0.0072021656571766961993, 5)],
+    (6, 6): [(0.00042272261734493450425, 1), (0.00030882419443789644048, 2), (0.00020625757066474306202, 3), (0.0033083434042009682567, 4), (0.58280470164050018158, 5), (0.80541742203662154736, 7), (0.13383632334100334433, 8), (0.0055555555555555555556, 9), (1.1903620718618930511, 6)],
+    (6, 7): [(-0.00086070442526864133026, 4), (-0.17480747086739049893, 5), (-0.31325448501150501650, 8), (-0.025000000000000000000, 9), (-0.31691663053104292713, 7), (-0.66916070916479291611, 6)],
+    (6, 8): [(0.033546617916933521087, 5), (-0.33436200223869714050, 7), (0.050000000000000000000, 9), (0.21697906098076027508, 6), (0.18383632334100334433, 8)],
+    (6, 9): [(0.029125184768230046430, 7), (0.022790919164749163916, 8), (-0.030689859975187405305, 6), (-0.0017817995133473605962, 5), (-0.030555555555555555556, 9)],
+    (7, 7): [(0.00022392237357715991790, 4), (0.12754377854309566738, 5), (1.0116994839296081646, 6), (0.96988172751725752475, 8), (0.12500000000000000000, 9), (0.0055555555555555555556, 10), (0.48231775430312815001, 7)],
+    (7, 8): [(-0.037841139730330129499, 5), (-0.29975568851348273616, 6), (-0.30000000000000000000, 9), (-0.025000000000000000000, 10), (-0.39914868674468211784, 7), (-0.43825448501150501650, 8)],
+    (7, 9): [(0.046981462180226839339, 6), (-0.29668637874712374587, 8), (0.050000000000000000000, 10), (0.17163557041460064817, 7), (0.0030693461522962583624, 5), (0.17500000000000000000, 9)],
+    (8, 8): [(0.012303289427168044554, 5), (0.11836475296458983325, 6), (0.94105118982279433342, 7), (0.95000000000000000000, 9), (0.12500000000000000000, 10), (0.0055555555555555555556, 11), (0.56994743445211445545, 8)],
+    (8, 9): [(-0.023080678926719163396, 6), (-0.29866250537751494972, 7), (-0.30000000000000000000, 10), (-0.025000000000000000000, 11), (-0.0010477348605150508026, 5), (-0.42720908083525083608, 8), (-0.42500000000000000000, 9)],
+    (9, 9): [(0.0051393702211491099770, 6), (0.12477232150094220014, 7), (0.95055227020881270902, 8), (0.95000000000000000000, 10), (0.12500000000000000000, 11), (0.0055555555555555555556, 12), (0.000091593624651536418269, 5), (0.56111111111111111111, 9)],
+}
+
+
 def _ratio(num, den):
     # the reference divides the integer literals in Float64 (float(num) / float(den))
     return float(num) / float(den)
 
 
 def variable_diagonal_sbp_D2(p, N, B, xc=(-1.0, 1.0)):
-    """(D, S0, SN, HI, H, M, r) -- diagonal_sbp.jl:470-740 for p = 2, 4:
+    """(D, S0, SN, HI, H, M, r) -- diagonal_sbp.jl:470-740 for p = 2, 4, 6:
     D approximates d/dx (B d/dx) with D = HI (-M + SN - S0), M symmetric and
     B-dependent, S0 / SN the B-weighted boundary derivatives."""
-    if p not in (2, 4):
+    if p not in (2, 4, 6):
         raise ValueError(f"Operators for order {p} are not implemented")
     B = np.asarray(B, dtype=np.float64)
     assert len(B) == N + 1
@@ -273,6 +324,44 @@ def variable_diagonal_sbp_D2(p, N, B, xc=(-1.0, 1.0)):
             put(i, i, (b(i - 1) + 2 * b(i) + b(i + 1)) / 2)
         for i in range(1, N + 1):                  # I_M = 1:N, J_M = 2:N+1
             put(i, i + 1, -(b(i) + b(i + 1)) / 2)
+    elif p == 6:
+        bhinv = [43200 / 13649, 8640 / 12013, 4320 / 2711, 4320 / 5359, 8640 / 7877, 43200 / 43801]
+        BS = np.array([25 / 12, -4.0, 3.0, -4 / 3, 1 / 4])
+
+        def block6(bk):
+            V = np.zeros((9, 9))
+            for (r, c), terms in _VAR_P6_M0.items():
+                v = 0.0
+                for w, k in terms:
+                    v += w * bk[k - 1]
+                V[r - 1, c - 1] = v
+                V[c - 1, r - 1] = v
+            return V
+        V0 = block6(B[0:12])
+        VN = block6(B[::-1][0:12])                 # b1..b12 = B[N+1:-1:N-10]
+        for r in range(9):
+            for c in range(9):
+                put(r + 1, c + 1, V0[r, c])
+                put(N + 1 - r, N + 1 - c, VN[r, c])
+        for I in range(10, N - 4):                  # J = I-3
+            put(I, I - 3, -(11 / 360) * b(I - 3) + (1 / 40) * b(I - 2) + (1 / 40) * b(I - 1) - (11 / 360) * b(I))
+        for I in range(10, N - 5):                  # J = I-2
+            put(I, I - 2, (1 / 20) * b(I - 3) + (7 / 40) * b(I - 2) - (3 / 10) * b(I - 1) + (7 / 40) * b(I)
+                + (1 / 20) * b(I + 1))
+        for I in range(10, N - 6):                  # J = I-1
+            put(I, I - 1, -(1 / 40) * b(I - 3) - (3 / 10) * b(I - 2) - (17 / 40) * b(I - 1) - (17 / 40) * b(I)
+                - (3 / 10) * b(I + 1) - (1 / 40) * b(I + 2))
+        for I in range(10, N - 7):                  # J = I
+            put(I, I, (1 / 180) * b(I - 3) + (1 / 8) * b(I - 2) + (19 / 20) * b(I - 1) + (101 / 180) * b(I)
+                + (19 / 20) * b(I + 1) + (1 / 8) * b(I + 2) + (1 / 180) * b(I + 3))
+        for I in range(9, N - 7):                   # J = I+1
+            put(I, I + 1, -(1 / 40) * b(I - 2) - (3 / 10) * b(I - 1) - (17 / 40) * b(I) - (17 / 40) * b(I + 1)
+                - (3 / 10) * b(I + 2) - (1 / 40) * b(I + 3))
+        for I in range(8, N - 7):                   # J = I+2
+            put(I, I + 2, (1 / 20) * b(I - 1) + (7 / 40) * b(I) - (3 / 10) * b(I + 1) + (7 / 40) * b(I + 2)
+                + (1 / 20) * b(I + 3))
+        for I in range(7, N - 7):                   # J = I+3
+            put(I, I + 3, -(11 / 360) * b(I) + (1 / 40) * b(I + 1) + (1 / 40) * b(I + 2) - (11 / 360) * b(I + 3))
     else:
         bhinv = [48 / 17, 48 / 59, 48 / 43, 48 / 49]
         BS = np.array([11 / 6, -3.0, 3 / 2, -1 / 3])

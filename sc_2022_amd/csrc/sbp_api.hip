@@ -1159,7 +1159,7 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) { return sb
 int sbp_create_var(int N, int p, const double* bx, const double* by, const sbp_params* prm, sbp_handle* out) {
     if (!out) return SBP_ERR_INVALID_ARG;
     *out = nullptr;
-    if (p != 2 && p != 4) return SBP_ERR_INVALID_ORDER;
+    if (p != 2 && p != 4 && p != 6) return SBP_ERR_INVALID_ORDER;
     if (!bx || !by) return SBP_ERR_INVALID_ARG;
     for (int k = 0; k < N; ++k)
         if (!(bx[k] > 0.0) || !(by[k] > 0.0) || !std::isfinite(bx[k]) || !std::isfinite(by[k]))
@@ -1173,8 +1173,8 @@ int sbp_create_var(int N, int p, const double* bx, const double* by, const sbp_p
     const OpTab& tb = h->L[0].tb;
     t.N = N;
     t.H = p / 2;
-    t.R = p == 2 ? 3 : 6;
-    t.C = p == 2 ? 4 : 8;
+    t.R = var_R(p);
+    t.C = var_C(p);
     t.RH = tb.RH;
     for (int r = 0; r < kMaxR; ++r) t.hw[r] = tb.hw[r];
     const size_t nb = bk.size();
@@ -1246,8 +1246,10 @@ int launch_var(sbp_handle h, int mode, const double* u, double* out, const doubl
                        (const VarTab*)h->vtab, sc)
     if (h->p == 2) {
         if (mode == 0) SBP_VAR_LAUNCH(2, 0); else if (mode == 1) SBP_VAR_LAUNCH(2, 1); else SBP_VAR_LAUNCH(2, 2);
-    } else {
+    } else if (h->p == 4) {
         if (mode == 0) SBP_VAR_LAUNCH(4, 0); else if (mode == 1) SBP_VAR_LAUNCH(4, 1); else SBP_VAR_LAUNCH(4, 2);
+    } else {
+        if (mode == 0) SBP_VAR_LAUNCH(6, 0); else if (mode == 1) SBP_VAR_LAUNCH(6, 1); else SBP_VAR_LAUNCH(6, 2);
     }
 #undef SBP_VAR_LAUNCH
     HIPCHK(hipGetLastError());

@@ -1,4 +1,4 @@
-// sbp_var.h -- the variable-coefficient operator, orders 2 and 4.
+// sbp_var.h -- the variable-coefficient operator, orders 2, 4 and 6.
 //
 // d/dx (bx d/dx) + d/dy (by d/dy) with one coefficient profile per axis
 // (bx along the slow, Dirichlet index j; by along the fast, Neumann index i),
@@ -22,12 +22,19 @@
 
 namespace sbp {
 
+// Closure rows / columns of the variable-coefficient factors:
+// p = 2: 3 / 4, p = 4: 6 / 8, p = 6: 9 / 12.
+constexpr int kVarMaxR = 9;
+constexpr int kVarMaxC = 12;
+inline int var_R(int p) { return p == 2 ? 3 : (p == 4 ? 6 : 9); }
+inline int var_C(int p) { return p == 2 ? 4 : (p == 4 ? 8 : 12); }
+
 // One axis' factor on the device: closure rows (first / last R) and the band
 // of the rows between.
 struct VarAxis {
-    double lo[kMaxR][kMaxC];   // rows r < R, columns c < C
-    double hi[kMaxR][kMaxC];   // rows N-1-r, columns N-1-c
-    const double* band;        // (2H+1) x N, offset s = -H..H at band[(s+H) N + r]
+    double lo[kVarMaxR][kVarMaxC];   // rows r < R, columns c < C
+    double hi[kVarMaxR][kVarMaxC];   // rows N-1-r, columns N-1-c
+    const double* band;              // (2H+1) x N, offset s = -H..H at band[(s+H) N + r]
 };
 
 struct VarTab {
@@ -42,8 +49,9 @@ struct VarTab {
 // ---------------------------------------------------------------------------
 // p = 4 boundary block: entry (r, c), r <= c, is sum num/den * b_k; the
 // integer literals are divided in double precision as the reference does.
+// The p = 6 block's weights are decimals (den = 1).
 struct VarTerm { double num, den; int k; };
-struct VarEntry { int r, c; VarTerm t[6]; int nt; };
+struct VarEntry { int r, c; VarTerm t[9]; int nt; };
 
 inline const std::vector<VarEntry>& var_p4_block() {
     static const std::vector<VarEntry> e = {
@@ -80,6 +88,58 @@ inline const std::vector<VarEntry>& var_p4_block() {
     return e;
 }
 
+// p = 6 boundary block (diagonal_sbp.jl:577-624), b1..b12.
+inline const std::vector<VarEntry>& var_p6_block() {
+    static const std::vector<VarEntry> e = {
+        {1, 1, {{0.79126675946955820939, 1, 1}, {0.29684720906380007429, 1, 2}, {0.0031855190887964290152, 1, 3}, {0.016324040425909519534, 1, 4}, {0.031603022440944150877, 1, 5}, {0.031679647480161052996, 1, 6}, {0.031485777339472539205, 1, 7}}, 7},
+        {1, 2, {{-1.0166893393503381444, 1, 1}, {-0.028456273704916113690, 1, 3}, {-0.041280298383492988198, 1, 4}, {-0.13922814516201405075, 1, 5}, {-0.11957773256112017666, 1, 6}, {-0.11942677565293334109, 1, 7}}, 6},
+        {1, 3, {{0.070756429372437150463, 1, 1}, {-0.18454761060241510503, 1, 2}, {-0.043641631471118923470, 1, 4}, {0.24323679072077324609, 1, 5}, {0.15821270735372154440, 1, 6}, {0.16023485783647863076, 1, 7}}, 6},
+        {1, 4, {{0.22519915328913532127, 1, 1}, {-0.16627487110970548953, 1, 2}, {0.027105309616486712977, 1, 3}, {-0.19166461859684399091, 1, 5}, {-0.076841171601990145944, 1, 6}, {-0.082195869498316975759, 1, 7}}, 6},
+        {1, 5, {{-0.052244034642020563167, 1, 1}, {0.044400639485098762210, 1, 2}, {-0.0010239765473093878745, 1, 3}, {0.074034846453161740905, 1, 4}, {0.012416255689984968954, 1, 6}, {0.071886528478926012827, 1, 5}, {0.013793629971047355034, 1, 7}}, 7},
+        {1, 6, {{-0.018288968138771973527, 1, 1}, {0.0095746331632217580607, 1, 2}, {-0.00081057845305764042779, 1, 3}, {-0.0073488455877755196984, 1, 4}, {0.010636019497239069970, 1, 5}, {-0.013159670383826183824, 1, 6}, {-0.021179364788387535246, 1, 7}}, 7},
+        {1, 7, {{0.0019118885633161709274, 1, 4}, {-0.040681303555291499361, 1, 5}, {0.013196749810737491670, 1, 6}, {0.025572665181237836763, 1, 7}}, 4},
+        {1, 8, {{0.015596528711367857640, 1, 5}, {-0.0064861841573315378995, 1, 6}, {-0.0091103445540363197401, 1, 7}}, 3},
+        {1, 9, {{0.00055939836966298630593, 1, 6}, {-0.0013848225351007963723, 1, 5}, {0.00082542416543781006633, 1, 7}}, 3},
+        {2, 2, {{1.3063321571116676286, 1, 1}, {0.25420017604573457435, 1, 3}, {0.10438978280925626095, 1, 4}, {0.66723280210321129509, 1, 5}, {0.46818193597227494411, 1, 6}, {0.46764154101958369201, 1, 7}}, 6},
+        {2, 3, {{-0.090914102699924646049, 1, 1}, {0.11036113131714764253, 1, 4}, {-1.2903975449975188870, 1, 5}, {-0.66396052487350447871, 1, 6}, {-0.66159744640052061842, 1, 7}}, 5},
+        {2, 4, {{-0.28935573956534316666, 1, 1}, {-0.24213200040645927216, 1, 3}, {1.1876702550280310277, 1, 5}, {0.39565981499041363328, 1, 6}, {0.38600489217558000007, 1, 7}}, 5},
+        {2, 5, {{0.067127744758037639890, 1, 1}, {0.0091471926820756301800, 1, 3}, {-0.18721961430038080217, 1, 4}, {-0.13193585588531745301, 1, 6}, {-0.48715757368119118874, 1, 5}, {-0.10475163122754481381, 1, 7}}, 6},
+        {2, 6, {{0.023499279745900688694, 1, 1}, {0.0072409053835651813164, 1, 3}, {0.018583789963916794487, 1, 4}, {-0.092896161339386761743, 1, 5}, {0.12235132704188076670, 1, 6}, {0.11135203204362950339, 1, 7}}, 6},
+        {2, 7, {{-0.0048347914064469075906, 1, 4}, {0.23106838326878204031, 1, 5}, {-0.10807741421960079917, 1, 6}, {-0.11815617764273433354, 1, 7}}, 4},
+        {2, 8, {{-0.083681414344034553537, 1, 5}, {0.040934994667670546616, 1, 6}, {0.042746419676364006921, 1, 7}}, 3},
+        {2, 9, {{-0.0035765451326969831434, 1, 6}, {0.0073893991241210786821, 1, 5}, {-0.0038128539914240955387, 1, 7}}, 3},
+        {3, 3, {{0.0063271611471368738078, 1, 1}, {0.11473182007158685275, 1, 2}, {0.11667405542796800075, 1, 4}, {2.7666108082854440372, 1, 5}, {1.0709206899608171042, 1, 6}, {1.0131613910329730572, 1, 7}}, 6},
+        {3, 4, {{0.020137694138847972466, 1, 1}, {0.10337179946308864017, 1, 2}, {-2.9132216211517427243, 1, 5}, {-0.87558073434822622598, 1, 6}, {-0.69099571834888124265, 1, 7}}, 5},
+        {3, 5, {{-0.0046717510915754628683, 1, 1}, {-0.027603533656377128278, 1, 2}, {-0.19792902986208699745, 1, 4}, {0.54029853383734330523, 1, 6}, {1.2391775930319110779, 1, 5}, {0.26280380502473582273, 1, 7}}, 6},
+        {3, 6, {{-0.0016354308669218878195, 1, 1}, {-0.0059524752758832596197, 1, 2}, {0.019646827777442752194, 1, 4}, {0.32366400126390466006, 1, 5}, {-0.46595166932288709739, 1, 6}, {-0.22172727209417368594, 1, 7}}, 6},
+        {3, 7, {{-0.0051113531893524745496, 1, 4}, {-0.53558781637747543460, 1, 5}, {0.33283351044897389336, 1, 6}, {0.20786565911785401579, 1, 7}}, 4},
+        {3, 8, {{0.18243281741342895622, 1, 5}, {-0.10598160301968184459, 1, 6}, {-0.076451214393747111630, 1, 7}}, 3},
+        {3, 9, {{0.0092090899634437994856, 1, 6}, {-0.015915028188724931671, 1, 5}, {0.0067059382252811321853, 1, 7}}, 3},
+        {4, 4, {{0.064092997759871869867, 1, 1}, {0.093136576388046999489, 1, 2}, {0.23063676246347492291, 1, 3}, {3.6894403082837166203, 1, 5}, {1.1905503386876088738, 1, 6}, {0.59124795468888565194, 1, 7}}, 6},
+        {4, 5, {{-0.014868958192656041286, 1, 1}, {-0.024870405993901607642, 1, 2}, {-0.0087129289077117541871, 1, 3}, {-1.2635078373718242057, 1, 6}, {-0.30583173978439973269, 1, 7}, {-1.4706919260458029548, 1, 5}}, 6},
+        {4, 6, {{-0.0052051474298559556576, 1, 1}, {-0.0053630987475285424890, 1, 2}, {-0.0068971427657906095463, 1, 3}, {-0.78575245216674501017, 1, 5}, {0.22911480054237346001, 1, 7}, {0.99770643562927505292, 1, 6}}, 6},
+        {4, 7, {{0.66972974880676622652, 1, 5}, {-0.50132473560721279390, 1, 6}, {-0.17951612431066454373, 1, 7}}, 3},
+        {4, 8, {{-0.20229090601117515652, 1, 5}, {0.14534218580636584986, 1, 6}, {0.056948720204809306656, 1, 7}}, 3},
+        {4, 9, {{-0.012004296184410038337, 1, 6}, {-0.0047769156693859238415, 1, 7}, {0.016781211853795962179, 1, 5}}, 3},
+        {5, 5, {{0.0034494550959102336252, 1, 1}, {0.0066411834994278261016, 1, 2}, {0.00032915450832718628585, 1, 3}, {0.33577217075764772000, 1, 4}, {2.0964133295790264390, 1, 6}, {0.23173232041831268550, 1, 7}, {0.0061078257643682645765, 1, 8}, {0.71091258506833766956, 1, 5}}, 8},
+        {5, 6, {{0.0012075440723041938061, 1, 1}, {0.0014321166657521476075, 1, 2}, {0.00026055826461832559573, 1, 3}, {-0.033329411132516353908, 1, 4}, {-0.28082416973855326836, 1, 7}, {-0.027209080835250836084, 1, 8}, {0.10458654356829219874, 1, 5}, {-1.3484369866671155432, 1, 6}}, 8},
+        {5, 7, {{0.0086710380841746926251, 1, 4}, {0.17360734113554285637, 1, 6}, {0.053313621252876254126, 1, 8}, {-0.24249352624045263018, 1, 5}, {0.15690152576785882706, 1, 7}}, 5},
+        {5, 8, {{-0.086316839802171222760, 1, 6}, {0.026988423604709992435, 1, 7}, {0.080981941477156510853, 1, 5}, {-0.032764636390806391639, 1, 8}}, 4},
+        {5, 9, {{0.0074620594845308550733, 1, 6}, {-0.00081216403616686789496, 1, 7}, {0.00055227020881270902093, 1, 8}, {-0.0072021656571766961993, 1, 5}}, 4},
+        {6, 6, {{0.00042272261734493450425, 1, 1}, {0.00030882419443789644048, 1, 2}, {0.00020625757066474306202, 1, 3}, {0.0033083434042009682567, 1, 4}, {0.58280470164050018158, 1, 5}, {0.80541742203662154736, 1, 7}, {0.13383632334100334433, 1, 8}, {0.0055555555555555555556, 1, 9}, {1.1903620718618930511, 1, 6}}, 9},
+        {6, 7, {{-0.00086070442526864133026, 1, 4}, {-0.17480747086739049893, 1, 5}, {-0.31325448501150501650, 1, 8}, {-0.025000000000000000000, 1, 9}, {-0.31691663053104292713, 1, 7}, {-0.66916070916479291611, 1, 6}}, 6},
+        {6, 8, {{0.033546617916933521087, 1, 5}, {-0.33436200223869714050, 1, 7}, {0.050000000000000000000, 1, 9}, {0.21697906098076027508, 1, 6}, {0.18383632334100334433, 1, 8}}, 5},
+        {6, 9, {{0.029125184768230046430, 1, 7}, {0.022790919164749163916, 1, 8}, {-0.030689859975187405305, 1, 6}, {-0.0017817995133473605962, 1, 5}, {-0.030555555555555555556, 1, 9}}, 5},
+        {7, 7, {{0.00022392237357715991790, 1, 4}, {0.12754377854309566738, 1, 5}, {1.0116994839296081646, 1, 6}, {0.96988172751725752475, 1, 8}, {0.12500000000000000000, 1, 9}, {0.0055555555555555555556, 1, 10}, {0.48231775430312815001, 1, 7}}, 7},
+        {7, 8, {{-0.037841139730330129499, 1, 5}, {-0.29975568851348273616, 1, 6}, {-0.30000000000000000000, 1, 9}, {-0.025000000000000000000, 1, 10}, {-0.39914868674468211784, 1, 7}, {-0.43825448501150501650, 1, 8}}, 6},
+        {7, 9, {{0.046981462180226839339, 1, 6}, {-0.29668637874712374587, 1, 8}, {0.050000000000000000000, 1, 10}, {0.17163557041460064817, 1, 7}, {0.0030693461522962583624, 1, 5}, {0.17500000000000000000, 1, 9}}, 6},
+        {8, 8, {{0.012303289427168044554, 1, 5}, {0.11836475296458983325, 1, 6}, {0.94105118982279433342, 1, 7}, {0.95000000000000000000, 1, 9}, {0.12500000000000000000, 1, 10}, {0.0055555555555555555556, 1, 11}, {0.56994743445211445545, 1, 8}}, 7},
+        {8, 9, {{-0.023080678926719163396, 1, 6}, {-0.29866250537751494972, 1, 7}, {-0.30000000000000000000, 1, 10}, {-0.025000000000000000000, 1, 11}, {-0.0010477348605150508026, 1, 5}, {-0.42720908083525083608, 1, 8}, {-0.42500000000000000000, 1, 9}}, 7},
+        {9, 9, {{0.0051393702211491099770, 1, 6}, {0.12477232150094220014, 1, 7}, {0.95055227020881270902, 1, 8}, {0.95000000000000000000, 1, 10}, {0.12500000000000000000, 1, 11}, {0.0055555555555555555556, 1, 12}, {0.000091593624651536418269, 1, 5}, {0.56111111111111111111, 1, 9}}, 8},
+    };
+    return e;
+}
+
 // Calls put(row, col, value) (1-based) for every stored entry of h M(b)
 // (duplicates add, as in the reference's sparse()).
 template <class PUT>
@@ -94,9 +154,10 @@ inline void var_m_entries(int p, int Np, const double* B, PUT&& put) {
         for (int i = 1; i <= N; ++i) put(i, i + 1, -(b(i) + b(i + 1)) / 2);
         return;
     }
-    double blo[8], bhi[8];
-    for (int k = 0; k < 8; ++k) { blo[k] = B[k]; bhi[k] = B[N - k]; }
-    for (const VarEntry& e : var_p4_block()) {
+    const int nb = p == 4 ? 8 : 12;
+    double blo[12], bhi[12];
+    for (int k = 0; k < nb; ++k) { blo[k] = B[k]; bhi[k] = B[N - k]; }
+    for (const VarEntry& e : (p == 4 ? var_p4_block() : var_p6_block())) {
         double vl = 0.0, vh = 0.0;
         for (int q = 0; q < e.nt; ++q) {
             const double w = e.t[q].num / e.t[q].den;
@@ -107,6 +168,28 @@ inline void var_m_entries(int p, int Np, const double* B, PUT&& put) {
         if (e.r != e.c) put(e.c, e.r, vl);
         put(N + 2 - e.r, N + 2 - e.c, vh);
         if (e.r != e.c) put(N + 2 - e.c, N + 2 - e.r, vh);
+    }
+    if (p == 6) {   // bands by row I (diagonal_sbp.jl:711-728)
+        for (int I = 10; I <= N - 5; ++I)
+            put(I, I - 3, -(11.0 / 360) * b(I - 3) + (1.0 / 40) * b(I - 2) + (1.0 / 40) * b(I - 1) - (11.0 / 360) * b(I));
+        for (int I = 10; I <= N - 6; ++I)
+            put(I, I - 2, (1.0 / 20) * b(I - 3) + (7.0 / 40) * b(I - 2) - (3.0 / 10) * b(I - 1) + (7.0 / 40) * b(I) +
+                              (1.0 / 20) * b(I + 1));
+        for (int I = 10; I <= N - 7; ++I)
+            put(I, I - 1, -(1.0 / 40) * b(I - 3) - (3.0 / 10) * b(I - 2) - (17.0 / 40) * b(I - 1) -
+                              (17.0 / 40) * b(I) - (3.0 / 10) * b(I + 1) - (1.0 / 40) * b(I + 2));
+        for (int I = 10; I <= N - 8; ++I)
+            put(I, I, (1.0 / 180) * b(I - 3) + (1.0 / 8) * b(I - 2) + (19.0 / 20) * b(I - 1) + (101.0 / 180) * b(I) +
+                          (19.0 / 20) * b(I + 1) + (1.0 / 8) * b(I + 2) + (1.0 / 180) * b(I + 3));
+        for (int I = 9; I <= N - 8; ++I)
+            put(I, I + 1, -(1.0 / 40) * b(I - 2) - (3.0 / 10) * b(I - 1) - (17.0 / 40) * b(I) -
+                              (17.0 / 40) * b(I + 1) - (3.0 / 10) * b(I + 2) - (1.0 / 40) * b(I + 3));
+        for (int I = 8; I <= N - 8; ++I)
+            put(I, I + 2, (1.0 / 20) * b(I - 1) + (7.0 / 40) * b(I) - (3.0 / 10) * b(I + 1) + (7.0 / 40) * b(I + 2) +
+                              (1.0 / 20) * b(I + 3));
+        for (int I = 7; I <= N - 8; ++I)
+            put(I, I + 3, -(11.0 / 360) * b(I) + (1.0 / 40) * b(I + 1) + (1.0 / 40) * b(I + 2) - (11.0 / 360) * b(I + 3));
+        return;
     }
     for (int J = 5; J <= N - 5; ++J) put(J + 2, J, (1.0 / 8) * b(J + 2) - (1.0 / 6) * b(J + 1) + (1.0 / 8) * b(J));
     for (int J = 6; J <= N - 5; ++J)
@@ -121,10 +204,10 @@ inline void var_m_entries(int p, int Np, const double* B, PUT&& put) {
 
 // Builds one axis' factor (m, or k when dirichlet) from the coefficient
 // profile b (N values): closure blocks into ax, the band (2H+1) x N into
-// band.  R / C: closure rows / columns (p = 2: 3 / 4, p = 4: 6 / 8).
+// band.  R / C: closure rows / columns (var_R, var_C).
 // Returns false when N is too small or an entry falls outside the layout.
 inline bool build_var_axis(int p, int N, const double* b, bool dirichlet, VarAxis* ax, std::vector<double>& band) {
-    const int H = p / 2, R = p == 2 ? 3 : 6, C = p == 2 ? 4 : 8;
+    const int H = p / 2, R = var_R(p), C = var_C(p);
     if (N < 2 * C + 2) return false;
     std::memset(ax->lo, 0, sizeof(ax->lo));
     std::memset(ax->hi, 0, sizeof(ax->hi));
@@ -149,8 +232,9 @@ inline bool build_var_axis(int p, int N, const double* b, bool dirichlet, VarAxi
         // k = m - (E bs + bs^T E) + 13 b_face E, bs = b_face * BS (dimensionless)
         const double bs2[3] = {3.0 / 2.0, -2.0, 1.0 / 2.0};
         const double bs4[4] = {11.0 / 6.0, -3.0, 3.0 / 2.0, -1.0 / 3.0};
-        const double* bs = p == 2 ? bs2 : bs4;
-        const int nbs = p == 2 ? 3 : 4;
+        const double bs6[5] = {25.0 / 12.0, -4.0, 3.0, -4.0 / 3.0, 1.0 / 4.0};
+        const double* bs = p == 2 ? bs2 : (p == 4 ? bs4 : bs6);
+        const int nbs = p == 2 ? 3 : (p == 4 ? 4 : 5);
         for (int c = 0; c < nbs; ++c) {
             put0(0, c, -b[0] * bs[c]);
             put0(c, 0, -b[0] * bs[c]);

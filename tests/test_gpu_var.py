@@ -39,7 +39,7 @@ def SP():
     return SbpPoisson
 
 
-@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("p", [2, 4, 6])
 @pytest.mark.parametrize("level", [5, 6, 8])
 def test_var_apply_matches_oracle(SP, level, p):
     N = 2 ** level + 1
@@ -55,7 +55,7 @@ def test_var_apply_matches_oracle(SP, level, p):
     assert abs(d - u @ Au) < 1e-12 * abs(u @ Au)
 
 
-@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("p", [2, 4, 6])
 def test_var_constant_limit_and_rhs(SP, p):
     """b = 1 gives the constant-coefficient handle's apply and right-hand
     side; with varying b the right-hand side and its transpose match the
@@ -79,7 +79,7 @@ def test_var_constant_limit_and_rhs(SP, p):
     assert abs(lhs - rhs) < 1e-12 * abs(lhs)
 
 
-@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("p", [2, 4, 6])
 def test_var_cg_against_oracle(SP, p):
     level = 6
     N = 2 ** level + 1
@@ -96,7 +96,7 @@ def test_var_cg_against_oracle(SP, p):
     assert np.linalg.norm(xg - xo) < 1e-7 * np.linalg.norm(xo)
 
 
-@pytest.mark.parametrize("p,rate,levels", [(2, 2.0, (6, 7, 8)), (4, 4.2, (5, 6, 7))])
+@pytest.mark.parametrize("p,rate,levels", [(2, 2.0, (6, 7, 8)), (4, 4.2, (5, 6, 7)), (6, 5.6, (5, 6, 7))])
 def test_var_manufactured_rate_on_gpu(SP, p, rate, levels):
     errs = []
     for level in levels:
@@ -125,4 +125,6 @@ def test_var_handle_rejects_what_it_does_not_serve(SP):
     with pytest.raises(Exception):
         SP(N, 4, coef=(bx, -by))            # coefficients must be positive
     with pytest.raises(Exception):
-        SP(N, 6, coef=(bx, by))             # orders 2 and 4 only
+        SP(N, 8, coef=(bx, by))             # orders 2, 4 and 6 only
+    with pytest.raises(Exception):
+        SP(25, 6, coef=(np.ones(25), np.ones(25)))   # p = 6 closures need N >= 26

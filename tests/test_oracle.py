@@ -309,12 +309,13 @@ def test_two_level_richardson_restatement():
     assert np.linalg.norm(x - xd) < 1e-8 * np.linalg.norm(xd)
 
 
-@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("p", [2, 4, 6])
 def test_variable_d2_pins(p):
     """variable_diagonal_sbp_D2 (diagonal_sbp.jl:470-740): B = 1 gives the
     constant-coefficient D2 exactly; for a varying B, M is symmetric positive
     semidefinite with M 1 = 0 (every b_k's weights cancel along a row), and D
-    approximates (B u')' to order p inside and p/2 at the faces."""
+    approximates (B u')' to order p inside and p/2 at the faces.  For p = 6 a
+    higher frequency and finer grids keep the interior error above round-off."""
     N = 40
     D2 = O.diagonal_sbp_D2(p, N, xc=(0, 1))[0]
     Dv = O.variable_diagonal_sbp_D2(p, N, np.ones(N + 1), xc=(0, 1))[0]
@@ -326,25 +327,27 @@ def test_variable_d2_pins(p):
     assert np.abs(M.sum(axis=1)).max() < 1e-13 * np.abs(M).max()
     assert np.linalg.eigvalsh(M).min() > -1e-12 * np.abs(M).max()
     ei, eb = [], []
-    bm = 1 if p == 2 else 6
-    for n in (32, 64, 128):
+    bm = {2: 1, 4: 6, 6: 9}[p]
+    kf, ns = (2, (32, 64, 128)) if p < 6 else (8, (64, 128))
+    for n in ns:
         x = np.linspace(0, 1, n + 1)
         Bb = 1 + x ** 2
         D = O.variable_diagonal_sbp_D2(p, n, Bb, xc=(0, 1))[0]
-        err = np.abs(D @ np.sin(2 * x) - (4 * x * np.cos(2 * x) - 4 * Bb * np.sin(2 * x)))
+        err = np.abs(D @ np.sin(kf * x) - (2 * kf * x * np.cos(kf * x) - kf * kf * Bb * np.sin(kf * x)))
         ei.append(err[2 * bm:-2 * bm].max())
         eb.append(err.max())
-    ri = math.log2(ei[1] / ei[2]); rb = math.log2(eb[1] / eb[2])
+    ri = math.log2(ei[-2] / ei[-1]); rb = math.log2(eb[-2] / eb[-1])
     assert abs(ri - p) < 0.25, ri
     assert abs(rb - p / 2) < 0.25, rb
 
 
-@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("p", [2, 4, 6])
 def test_variable_2d_operator(p):
     """The 2-D variable-coefficient operator: SPD, equal to the constant one
     at b = 1, equal to the tensor form k(bx) (x) hw + hw (x) m(by), and the
-    manufactured solution converges at rate 2 (p = 2) and about 4.2 (p = 4)."""
-    level = 4
+    manufactured solution converges at rate 2 (p = 2), about 4.2 (p = 4) and
+    about 5.6 (p = 6)."""
+    level = 4 if p < 6 else 5
     bx, by = O.var_coefficients(level)
     A = O.assemble_operator_var(level, p, bx, by)
     Ad = A.toarray()
@@ -365,7 +368,7 @@ def test_variable_2d_operator(p):
         x = spla.spsolve(Av.tocsc(), b)
         errs.append(O.h_norm(x - u, O.h_tilde_diag(lv, p)))
     rates = [math.log2(errs[k] / errs[k + 1]) for k in range(2)]
-    want = 2.0 if p == 2 else 4.2
+    want = {2: 2.0, 4: 4.2, 6: 5.6}[p]
     assert all(abs(r - want) < 0.2 for r in rates), rates
 
 
