@@ -3,7 +3,7 @@
  *
  * Grid and layout (the reference's): a level-k problem has N = 2^k + 1 points
  * per side on [0,1]^2 (the apply and CG accept any N >= 2*closure; the
- * multigrid entry points need N = 2^k + 1 >= 17, >= 33 for p = 6, 8).  Grid functions are dense
+ * multigrid entry points need N = 2^k + 1 >= 17, >= 33 for p >= 6).  Grid functions are dense
  * device arrays of N*N elements with element (i, j) at i + j*N, i fast
  * (Julia column-major (Nx, Ny)).  The fast index i carries the Neumann faces
  * (i = 0 and i = N-1), the slow index j the Dirichlet faces (j = 0, j = N-1).
@@ -11,12 +11,12 @@
  * The operator is the reference's matrix
  *     A = -H~ (D2 + SAT_W + SAT_E + SAT_S + SAT_N)      (tau_D = -13/h, tau_N = -1)
  * (preconditioned_CG/level_2_multigrid_new.jl:141-185) for SBP order p = 2, 4,
- * 6 or 8 (diagonal_sbp.jl:191-311).  It is symmetric positive definite for
- * p <= 6; for p = 8 the reference's tau_D is too small for its wider
- * boundary closure and A is symmetric indefinite, so the apply is exact but
- * the CG-type solvers may stop with SBP_ERR_BREAKDOWN.  The two-sweep fused
- * and single-workgroup multigrid kernels are compiled for p = 2, 4 only; p =
- * 6, 8 use the separate passes.
+ * 6, 8 or 10 (diagonal_sbp.jl:191-408).  It is symmetric positive definite
+ * for p <= 6; for p = 8, 10 the reference's tau_D is too small for their
+ * wider boundary closures and A is symmetric indefinite, so the apply is
+ * exact but the CG-type solvers may stop with SBP_ERR_BREAKDOWN.  The
+ * two-sweep fused and single-workgroup multigrid kernels are compiled for
+ * p = 2, 4 only; p >= 6 use the separate passes.
  *
  * Conventions: every pointer argument that names a grid function is a device
  * pointer owned by the caller; work is ordered on the handle's stream (the
@@ -47,7 +47,7 @@ extern "C" {
 typedef enum {
     SBP_OK = 0,
     SBP_ERR_INVALID_SIZE = 1,   /* N too small, or not 2^k+1 where multigrid needs it */
-    SBP_ERR_INVALID_ORDER = 2,  /* p not in {2, 4, 6, 8} */
+    SBP_ERR_INVALID_ORDER = 2,  /* p not in {2, 4, 6, 8, 10} ({2, 4, 6} for sbp_create_var) */
     SBP_ERR_INVALID_ARG = 3,    /* null pointer, bad dtype/sign/level, bad parameter */
     SBP_ERR_HIP = 4,            /* a HIP runtime call failed */
     SBP_ERR_NONFINITE = 5,      /* NaN/Inf met in a residual or inner product */
@@ -64,7 +64,7 @@ typedef struct {
     int nu_pre;              /* pre-smoothing sweeps per level (default 2) */
     int nu_post;             /* post-smoothing sweeps per level (default 2) */
     double omega;            /* damped-Jacobi weight (default 2/3) */
-    int coarse_n;            /* coarsest grid side, 2^k+1 (default 17; raised to 33 for p = 6, 8) */
+    int coarse_n;            /* coarsest grid side, 2^k+1 (default 17; raised to 33 for p >= 6) */
     double restrict_scale;   /* coarse rhs = restrict_scale * full_weighting(r);
                                 4 (default) gives R = P^T, 1 the reference's scale */
     int smoother;            /* 0 = damped Jacobi (default), 1 = Richardson */

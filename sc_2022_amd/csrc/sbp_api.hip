@@ -261,7 +261,7 @@ int launch_bottom(sbp_handle h, const T* b, T* out, const T* dotv, const Scalars
     else if (h->p == 4)
         hipLaunchKernelGGL((vcycle_bottom<4, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, b, out, dotv, c);
     else
-        return SBP_ERR_INVALID_ORDER;   // not compiled for p = 6, 8 (never selected for them)
+        return SBP_ERR_INVALID_ORDER;   // not compiled for p > 4 (never selected for them)
     HIPCHK(hipGetLastError());
     return SBP_OK;
 }
@@ -947,7 +947,7 @@ const char* sbp_status_string(int s) {
     switch (s) {
         case SBP_OK: return "ok";
         case SBP_ERR_INVALID_SIZE: return "invalid size";
-        case SBP_ERR_INVALID_ORDER: return "invalid SBP order (p must be 2 or 4)";
+        case SBP_ERR_INVALID_ORDER: return "invalid SBP order (p must be 2, 4, 6, 8 or 10; 2, 4 or 6 with variable coefficients)";
         case SBP_ERR_INVALID_ARG: return "invalid argument";
         case SBP_ERR_HIP: return "HIP runtime error";
         case SBP_ERR_NONFINITE: return "non-finite value in residual or inner product";
@@ -1021,7 +1021,7 @@ namespace {
 int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_handle* out) {
     if (!out) return SBP_ERR_INVALID_ARG;
     *out = nullptr;
-    if (p != 2 && p != 4 && p != 6 && p != 8) return SBP_ERR_INVALID_ORDER;
+    if (p != 2 && p != 4 && p != 6 && p != 8 && p != 10) return SBP_ERR_INVALID_ORDER;
     {
         OpTab t0;
         if (!build_optab(p, N, &t0, nullptr)) return SBP_ERR_INVALID_SIZE;

@@ -17,11 +17,11 @@ namespace sbp {
 
 constexpr int kBottomN = 65;          // largest side handled here
 constexpr int kBottomThreads = 1024;
-constexpr int kBottomMaxLevels = 6;   // 65, 33, 17, 9, 5, 3
+constexpr int kBottomMaxLevels = 5;   // 65, 33, 17, 9, 5 (the LDS budget: five tables and the pool)
 constexpr int kBottomMaxCoarse = 289; // coarsest system size handled here (17 x 17)
 constexpr int kCoarseChunks = 3;      // the coarse product is split three ways over k
-// Three grid functions per level (b and two iterates) for a 65 -> 3 chain.
-constexpr int kBottomPool = 3 * (65 * 65 + 33 * 33 + 17 * 17 + 9 * 9 + 5 * 5 + 3 * 3);
+// Three grid functions per level (b and two iterates) for a 65 -> 5 chain.
+constexpr int kBottomPool = 3 * (65 * 65 + 33 * 33 + 17 * 17 + 9 * 9 + 5 * 5);
 
 struct BottomCtl {
     const OpTab* tab[kBottomMaxLevels];   // device tables, finest first

@@ -234,7 +234,7 @@ __global__ __launch_bounds__(NT) void smooth2_restrict(const T* __restrict__ b, 
         sr_body<P, TI, J, NT, T, false, true>(b, x2g, rc, omega, use_diag, scale, t, bx, by, Nc, Bt, U, V);
     } else {
         __shared__ OpTab ltab;
-        stage_table(t, &ltab);
+        stage_table<P>(t, &ltab);
         sr_body<P, TI, J, NT, T, true, false>(b, x2g, rc, omega, use_diag, scale, ltab, bx, by, Nc, Bt, U, V);
     }
 }
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(NT) void prolong_smooth2(const T* __restrict__ x, c
         acc = ps_body<P, TI, J, NT, T, false, true>(x, ec, b, out, dotv, omega, use_diag, t, bx, by, U, E, Bt, V);
     } else {
         __shared__ OpTab ltab;
-        stage_table(t, &ltab);
+        stage_table<P>(t, &ltab);
         acc = ps_body<P, TI, J, NT, T, true, false>(x, ec, b, out, dotv, omega, use_diag, ltab, bx, by, U, E, Bt,
                                                      V);
     }

@@ -24,20 +24,21 @@
 // (sign * Au, residual, Jacobi update, fused inner products).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace sbp {
 
-constexpr int kMaxR = 8;   // closure rows
-constexpr int kMaxC = 12;  // closure columns
+constexpr int kMaxR = 11;  // closure rows
+constexpr int kMaxC = 16;  // closure columns
 
 struct OpTab {
     int N;      // points per side
-    int R;      // closure rows of k and m (p = 2, 4, 6, 8: 3, 4, 6, 8)
-    int C;      // closure columns (4, 6, 9, 12)
-    int RH;     // number of non-unit H weights (1, 4, 6, 8)
+    int R;      // closure rows of k and m (p = 2, 4, 6, 8, 10: 3, 4, 6, 8, 11)
+    int C;      // closure columns (4, 6, 9, 12, 16)
+    int RH;     // number of non-unit H weights (1, 4, 6, 8, 11)
     int H;      // half-width of the interior stencil (p / 2)
-    double cint[5];   // interior stencil of the 1-D factors: cint[s] at offsets +-s
+    double cint[6];   // interior stencil of the 1-D factors: cint[s] at offsets +-s
     double hw[kMaxR];
     double mb[kMaxR][kMaxC];
     double kb[kMaxR][kMaxC];
@@ -73,28 +74,28 @@ struct Batch {
     int ss;
 };
 
-// Interior 1-D stencil s[-H..H] of -d (diagonal_sbp.jl:193, 201, 213, 227),
-// and the closure block width C of the order (columns of the first R rows).
+// Interior 1-D stencil s[-H..H] of -d (diagonal_sbp.jl:193, 201, 213, 227, 394),
+// the closure block width C of the order (columns of the first R rows) and R.
 template <int P> struct Stencil;
 template <> struct Stencil<2> {
-    static constexpr int H = 1, C = 4;
+    static constexpr int H = 1, C = 4, R = 3;
     template <typename T> __device__ static T c(int t) { return t == 0 ? T(2) : T(-1); }
 };
 template <> struct Stencil<4> {
-    static constexpr int H = 2, C = 6;
+    static constexpr int H = 2, C = 6, R = 4;
     template <typename T> __device__ static T c(int t) {
         return t == 0 ? T(2.5) : ((t == 1 || t == -1) ? T(-4.0 / 3.0) : T(1.0 / 12.0));
     }
 };
 template <> struct Stencil<6> {
-    static constexpr int H = 3, C = 9;
+    static constexpr int H = 3, C = 9, R = 6;
     template <typename T> __device__ static T c(int t) {
         const int a = t < 0 ? -t : t;
         return a == 0 ? T(49.0 / 18.0) : a == 1 ? T(-3.0 / 2.0) : a == 2 ? T(3.0 / 20.0) : T(-1.0 / 90.0);
     }
 };
 template <> struct Stencil<8> {
-    static constexpr int H = 4, C = 12;
+    static constexpr int H = 4, C = 12, R = 8;
     template <typename T> __device__ static T c(int t) {
         const int a = t < 0 ? -t : t;
         return a == 0 ? T(205.0 / 72.0)
@@ -102,6 +103,18 @@ template <> struct Stencil<8> {
              : a == 2 ? T(1.0 / 5.0)
              : a == 3 ? T(-8.0 / 315.0)
                       : T(1.0 / 560.0);
+    }
+};
+template <> struct Stencil<10> {
+    static constexpr int H = 5, C = 16, R = 11;
+    template <typename T> __device__ static T c(int t) {
+        const int a = t < 0 ? -t : t;
+        return a == 0 ? T(5269.0 / 1800.0)
+             : a == 1 ? T(-5.0 / 3.0)
+             : a == 2 ? T(5.0 / 21.0)
+             : a == 3 ? T(-5.0 / 126.0)
+             : a == 4 ? T(5.0 / 1008.0)
+                      : T(-1.0 / 3150.0);
     }
 };
 
@@ -454,11 +467,25 @@ struct PassCtl {
 };
 
 // Copies the device table into LDS (blockDim.x threads, one word each).
+// Copies the parts of the table an order-P kernel reads into LDS (the
+// scalars, hw, the first R rows of mb and kb, and the per-row arrays), so
+// the copy does not grow with the largest order's closure.
+template <int P>
 __device__ __forceinline__ void stage_table(const OpTab& t, OpTab* l) {
-    static_assert(sizeof(OpTab) % 8 == 0, "table is copied in 8-byte words");
+    static_assert(sizeof(OpTab) % 8 == 0 && offsetof(OpTab, mb) % 8 == 0 && offsetof(OpTab, kb) % 8 == 0 &&
+                      offsetof(OpTab, md) % 8 == 0,
+                  "table is copied in 8-byte words");
+    constexpr int rows = Stencil<P>::R * kMaxC;
+    constexpr int a1 = (int)(offsetof(OpTab, mb) / 8) + rows;   // scalars, hw, mb rows < R
+    constexpr int b0 = (int)(offsetof(OpTab, kb) / 8);          // kb rows < R
+    constexpr int c0 = (int)(offsetof(OpTab, md) / 8);          // md .. dev
+    constexpr int n = a1 + rows + ((int)(sizeof(OpTab) / 8) - c0);
     const unsigned long long* src = reinterpret_cast<const unsigned long long*>(t.dev);
     unsigned long long* dst = reinterpret_cast<unsigned long long*>(l);
-    for (int k = threadIdx.x; k < (int)(sizeof(OpTab) / 8); k += blockDim.x) dst[k] = src[k];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int w = k < a1 ? k : (k < a1 + rows ? b0 + (k - a1) : c0 + (k - a1 - rows));
+        dst[w] = src[w];
+    }
     __syncthreads();
 }
 
@@ -661,7 +688,7 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
         // first (one load per thread) so those lookups cost LDS, not global,
         // latency -- on the small multigrid levels every tile is a face tile.
         __shared__ OpTab ltab;
-        stage_table(t, &ltab);
+        stage_table<P>(t, &ltab);
         if (jin) acc = pass_interior<P, TI, J, true, T, true>(in, out, ltab, i0, j0, tile);
         else acc = pass_general<P, TI, J, T, true>(in, out, ltab, i0, j0, tile);
     }
@@ -744,7 +771,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
     const bool jin = fj0 - H >= t.R && fj0 + JR + H <= N - t.R;
     const bool iin = fi0 - H >= t.R && fi0 + TI + H <= N - t.R;
     __shared__ OpTab ltab;
-    if (!(jin && iin)) stage_table(t, &ltab);   // per-thread closure lookups from LDS
+    if (!(jin && iin)) stage_table<P>(t, &ltab);   // per-thread closure lookups from LDS
     const OpTab& tl = ltab;
     if (jin) {
         // Rows interior: register window along j, LDS tile along i; with

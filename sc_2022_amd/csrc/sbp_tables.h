@@ -1,7 +1,7 @@
 // sbp_tables.h -- host-side construction of the dimensionless 1-D factors.
 //
 // From the reference's diagonal-norm SBP second-derivative tables
-// (diagonal_sbp.jl:191-311, orders p = 2, 4, 6, 8: bhinv, interior d,
+// (diagonal_sbp.jl:191-408, orders p = 2, 4, 6, 8, 10: bhinv, interior d,
 // closure bd, boundary row BS)
 // this builds, for a side of N points,
 //     hw = H / h                        (H = diag(h / bhinv) near the faces, h inside)
@@ -14,8 +14,9 @@
 // are extracted from a small dense build and checked for shape.  With the
 // reference's tau = 13 the Dirichlet factor k is positive definite for
 // p <= 6 but not for p = 8 (its smallest eigenvalue is about -2.7; the
-// penalty would have to exceed about 800): the p = 8 operator is symmetric
-// indefinite, so its apply is exact but CG-type solves may break down.
+// penalty would have to exceed about 800) nor p = 10 (about -7.8): those
+// operators are symmetric indefinite, so their apply is exact but CG-type
+// solves may break down.
 #pragma once
 #include <cmath>
 #include <cstring>
@@ -28,12 +29,13 @@ namespace sbp {
 struct Sbp1D {
     int p = 0, bm = 0, bn = 0, nbs = 0;
     double bhinv[kMaxR] = {};
-    double d[9] = {};                    // interior D2 stencil (times h^2), p + 1 entries
+    double d[11] = {};                   // interior D2 stencil (times h^2), p + 1 entries
     double bd[kMaxR][kMaxC] = {};        // closure of D2 (times h^2), bm x bn
     double bs[kMaxR] = {};               // boundary derivative row (times h), nbs entries
 };
 
-// The reference's rationals (diagonal_sbp.jl:191-197, 198-208, 210-222, 224-311).
+// The reference's rationals (diagonal_sbp.jl:191-197, 198-208, 210-222, 224-311);
+// for p = 10 its decimal M entries and the construction of bd (313-408).
 inline bool sbp_1d(int p, Sbp1D* s) {
     *s = Sbp1D{};
     s->p = p;
@@ -95,6 +97,61 @@ inline bool sbp_1d(int p, Sbp1D* s) {
         fill(8, 12, 7, bh, d, &bd[0][0], bs);
         return true;
     }
+    if (p == 10) {
+        const double bh[11] = {18289152000.0 / 5261271563.0, 1828915200.0 / 2881040311.0, 406425600.0 / 52175551.0, 6096384.0 / 11662993.0, 87091200.0 / 50124587.0, 72576000.0 / 50124587.0, 87091200.0 / 148333439.0, 152409600.0 / 63867949.0, 16257024.0 / 20608675.0, 1828915200.0 / 1704508063.0, 18289152000.0 / 18425967263.0};
+        const double d[11] = {1.0 / 3150.0, -5.0 / 1008.0, 5.0 / 126.0, -5.0 / 21.0, 5.0 / 3.0, -5269.0 / 1800.0, 5.0 / 3.0, -5.0 / 21.0, 5.0 / 126.0, -5.0 / 1008.0, 1.0 / 3150.0};
+        // upper triangle of M (r <= c, 1-based), diagonal_sbp.jl:319-369
+        const struct { int r, c; double v; } mu[66] = {
+            {1, 1, 1.2056593789671863908}, {1, 2, -1.3378814169347239658}, {1, 3, 0.0036847309286546532061},
+            {1, 4, 0.15698288365600946515}, {1, 5, -0.0037472461482539197952},
+            {1, 6, -0.0062491712449361657064}, {1, 7, -0.029164045872729581661},
+            {1, 8, 0.00054848184117832929161}, {1, 9, 0.013613461413384884448},
+            {1, 10, -0.0025059220258337808220}, {1, 11, -0.00094113457993630916498},
+            {2, 2, 2.1749807117105597139}, {2, 3, -0.12369059547124894597}, {2, 4, -0.83712574037924152603},
+            {2, 5, 0.050065127254670973258}, {2, 6, 0.0081045853127317536361},
+            {2, 7, 0.097405846039248226536}, {2, 8, -0.00068942461520402214720},
+            {2, 9, -0.041326971493379188475}, {2, 10, 0.0075778529605774119402},
+            {2, 11, 0.0025800256160095691057}, {3, 3, 0.18361596652499065332},
+            {3, 4, 0.048289690013342693109}, {3, 5, -0.19719621435164680412}, {3, 6, 0.11406859029505842791},
+            {3, 7, -0.029646295985488126964}, {3, 8, -0.0016038463172861201306},
+            {3, 9, 0.0032879841528337653050}, {3, 10, -0.00093242311589807387463},
+            {3, 11, 0.00012241332668787820533}, {4, 4, 1.2886524606662484673},
+            {4, 5, -0.14403037739488789185}, {4, 6, -0.44846291607489015475},
+            {4, 7, -0.10598334599408054277}, {4, 8, -0.015873275740355918053},
+            {4, 9, 0.073988493386459608166}, {4, 10, -0.012508848749152899785},
+            {4, 11, -0.0039290233894513005339}, {5, 5, 0.51482665719685186210},
+            {5, 6, 0.051199577887125103015}, {5, 7, -0.36233561810883077365},
+            {5, 8, 0.091356850268746392169}, {5, 9, 0.0024195916108052419451},
+            {5, 10, -0.0018564214413731389338}, {5, 11, -0.00070192677320704413827},
+            {6, 6, 0.68636003380365860083}, {6, 7, -0.28358848290867614908}, {6, 8, -0.13836006478253396528},
+            {6, 9, 0.0076158070663111995297}, {6, 10, 0.011447010307180005164},
+            {6, 11, -0.0021349696610286552676}, {7, 7, 1.5216081480839085990},
+            {7, 8, -0.42653865162216293237}, {7, 9, -0.42047484981879143123},
+            {7, 10, 0.019813359263872926304}, {7, 11, 0.019221397241190103344},
+            {8, 8, 1.0656733504627815335}, {8, 9, -0.66921872668484232217}, {8, 10, 0.12022033144141336599},
+            {8, 11, -0.030157881394591483631}, {9, 9, 2.4064247712949611684},
+            {9, 10, -1.5150200315922263367}, {9, 11, 0.17373015320416595052},
+            {10, 10, 2.7682502485427255096}, {10, 11, -1.5975407111468405444},
+            {11, 11, 2.9033627686681129471},
+        };
+        const double bsneg[7] = {-49.0 / 20.0, 6.0, -15.0 / 2.0, 20.0 / 3.0, -15.0 / 4.0, 6.0 / 5.0, -1.0 / 6.0};
+        // M is 11 x 16: the symmetric 11 x 11 block from its upper triangle, and
+        // rows 7..11 reaching columns 12..16 with the negated interior stencil;
+        // then bd = diag(bhinv) (-M + e0 BS) with BS = -bsneg.
+        double M[11][16] = {};
+        for (const auto& e : mu) {
+            M[e.r - 1][e.c - 1] = e.v;
+            M[e.c - 1][e.r - 1] = e.v;
+        }
+        for (int k = 1; k <= 5; ++k)
+            for (int q = 1; q <= k; ++q) M[5 + k][10 + q] = -d[k - q];
+        double bs[7], bd[11][16];
+        for (int c = 0; c < 7; ++c) bs[c] = -bsneg[c];
+        for (int r = 0; r < 11; ++r)
+            for (int c = 0; c < 16; ++c) bd[r][c] = bh[r] * (-M[r][c] + (r == 0 && c < 7 ? bs[c] : 0.0));
+        fill(11, 16, 7, bh, d, &bd[0][0], bs);
+        return true;
+    }
     return false;
 }
 
@@ -138,8 +195,10 @@ inline bool build_optab(int p, int N, OpTab* t, RhsTab* rt) {
     Sbp1D s;
     if (!sbp_1d(p, &s)) return false;
     // closure rows / columns of the factors (rows below R are the interior stencil)
-    const int R = p == 2 ? 3 : p;
-    const int C = p == 2 ? 4 : p == 4 ? 6 : p == 6 ? 9 : 12;
+    // R = max(bm, nbs) (the SAT column reaches nbs rows); C = max(bn, R + p/2)
+    const int R = s.bm > s.nbs ? s.bm : s.nbs;
+    const int C = s.bn > R + p / 2 ? s.bn : R + p / 2;
+    if (R > kMaxR || C > kMaxC) return false;
     if (N < 2 * C || N < 2 * s.bm || N < s.bn) return false;
     const int n0 = 2 * C + 4;
     std::vector<double> hw, m, k;

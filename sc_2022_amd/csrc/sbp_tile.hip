@@ -1,6 +1,6 @@
 // sbp_tile.hip -- instantiates the stencil pass for one tile shape
 // (compile with -DSBP_TI=<threads along i> -DSBP_TJ=<rows per thread>; with
-// -DSBP_HI=1 also for the orders p = 6 and 8).
+// -DSBP_HI=1 also for the orders p = 6, 8 and 10).
 #include "sbp_launch.h"
 
 #ifndef SBP_TI
@@ -20,6 +20,7 @@ int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, PassCtl c,
 #if SBP_HI
     case 6: hipLaunchKernelGGL((stencil_pass<6, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
     case 8: hipLaunchKernelGGL((stencil_pass<8, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
+    case 10: hipLaunchKernelGGL((stencil_pass<10, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
 #endif
     default: return 3;   // order not compiled for this tile shape
     }
@@ -43,6 +44,7 @@ int launch_resid_restrict(int p, const OpTab& tf, const T* u, const T* b, T* rc,
 #if SBP_HI
     case 6: hipLaunchKernelGGL((resid_restrict<6, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c); break;
     case 8: hipLaunchKernelGGL((resid_restrict<8, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c); break;
+    case 10: hipLaunchKernelGGL((resid_restrict<10, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c); break;
 #endif
     default: return 3;
     }

@@ -46,10 +46,11 @@ def test_create_rejects_bad_order_and_size_before_device_work():
     h = C.c_void_p()
     prm = capi.default_params()
     assert lib.sbp_create(17, 3, C.byref(prm), C.byref(h)) == capi.SBP_ERR_INVALID_ORDER
-    assert lib.sbp_create(17, 10, C.byref(prm), C.byref(h)) == capi.SBP_ERR_INVALID_ORDER
-    # p = 6 / 8 closures need N >= 2 C = 18 / 24
+    assert lib.sbp_create(17, 12, C.byref(prm), C.byref(h)) == capi.SBP_ERR_INVALID_ORDER
+    # p = 6 / 8 / 10 closures need N >= 2 C = 18 / 24 / 32
     assert lib.sbp_create(17, 6, C.byref(prm), C.byref(h)) == capi.SBP_ERR_INVALID_SIZE
     assert lib.sbp_create(23, 8, C.byref(prm), C.byref(h)) == capi.SBP_ERR_INVALID_SIZE
+    assert lib.sbp_create(31, 10, C.byref(prm), C.byref(h)) == capi.SBP_ERR_INVALID_SIZE
     assert lib.sbp_create(7, 2, C.byref(prm), C.byref(h)) == capi.SBP_ERR_INVALID_SIZE
     assert lib.sbp_create(11, 4, C.byref(prm), C.byref(h)) == capi.SBP_ERR_INVALID_SIZE
     bad = capi.default_params(omega=-1.0)

@@ -1,8 +1,8 @@
-"""GPU parity of the p = 6 and p = 8 operators (diagonal_sbp.jl:210-311)
+"""GPU parity of the p = 6, 8 and 10 operators (diagonal_sbp.jl:210-408)
 against the CPU oracle: the apply (both signs, FP64 and FP32, every tile
 shape compiled for these orders, ragged sides), diag, the right-hand side,
 the V-cycle and the solvers for p = 6 (SPD), and the defined behaviour of
-the solvers on the indefinite p = 8 operator."""
+the solvers on the indefinite p = 8 and p = 10 operators."""
 import math
 
 import numpy as np
@@ -41,7 +41,7 @@ def SP():
     return SbpPoisson
 
 
-@pytest.mark.parametrize("p", [6, 8])
+@pytest.mark.parametrize("p", [6, 8, 10])
 @pytest.mark.parametrize("level", [5, 6, 7, 8])
 def test_apply_matches_assembled(SP, level, p):
     N = 2 ** level + 1
@@ -56,7 +56,7 @@ def test_apply_matches_assembled(SP, level, p):
     assert relmax(out, A_of(level, p) @ uf.astype(np.float64)) < 2e-5
 
 
-@pytest.mark.parametrize("p", [6, 8])
+@pytest.mark.parametrize("p", [6, 8, 10])
 @pytest.mark.parametrize("tiles", [(64, 4), (128, 8), (256, 8)])
 def test_apply_every_compiled_tile_shape(SP, p, tiles):
     level = 7
@@ -74,10 +74,10 @@ def test_uncompiled_tile_shape_rejected(SP):
         h.set_tiles(0, 256, 16)
 
 
-@pytest.mark.parametrize("p", [6, 8])
-@pytest.mark.parametrize("N", [24, 25, 37, 100, 300])
+@pytest.mark.parametrize("p", [6, 8, 10])
+@pytest.mark.parametrize("N", [24, 25, 32, 37, 100, 300])
 def test_apply_ragged_sides_against_closed_form(SP, N, p):
-    if N < (18 if p == 6 else 24):
+    if N < {6: 18, 8: 24, 10: 32}[p]:
         pytest.skip("below the closure minimum 2C")
     hw, m, k = O.one_d_factors(p, N)
     A = sp.kron(sp.csr_matrix(k), sp.diags(hw)) + sp.kron(sp.diags(hw), sp.csr_matrix(m))
@@ -86,7 +86,7 @@ def test_apply_ragged_sides_against_closed_form(SP, N, p):
     assert relmax(h.apply(T(u)).cpu().numpy(), A @ u) < 1e-13
 
 
-@pytest.mark.parametrize("p", [6, 8])
+@pytest.mark.parametrize("p", [6, 8, 10])
 def test_apply_1025_properties(SP, p):
     """1025^2: <Au, v> = <u, Av>, and A 1 = hw_i (k 1)_j in closed form."""
     N = 1025
@@ -99,12 +99,12 @@ def test_apply_1025_properties(SP, p):
     A1 = h.apply(torch.ones(N * N, dtype=torch.float64, device=DEV)).reshape(N, N).cpu().numpy()
     hw, m, k = O.one_d_factors(p, 64)
     k1 = k.sum(axis=1)
-    hwN = np.ones(N); hwN[:p] = hw[:p]; hwN[-p:] = hw[-p:]
+    hwN = np.ones(N); hwN[:16] = hw[:16]; hwN[-16:] = hw[-16:]   # hw is 1 beyond the closure
     kN = np.zeros(N); kN[:16] = k1[:16]; kN[-16:] = k1[-16:]
     assert np.abs(A1 - kN[:, None] * hwN[None, :]).max() < 1e-11
 
 
-@pytest.mark.parametrize("p", [6, 8])
+@pytest.mark.parametrize("p", [6, 8, 10])
 def test_rhs_matches_oracle(SP, p):
     level = 6
     N = 2 ** level + 1
@@ -167,16 +167,17 @@ def test_mgpcg_p6_against_direct_and_rate(SP):
     assert all(5.2 < r < 5.8 for r in rates), (errs, rates)
 
 
-def test_p8_solvers_report_a_defined_status(SP):
-    """p = 8 with the reference's penalty is indefinite: the solvers must end
-    with a defined status, and a reported convergence must be real."""
+@pytest.mark.parametrize("p", [8, 10])
+def test_indefinite_orders_report_a_defined_status(SP, p):
+    """p = 8, 10 with the reference's penalty are indefinite: the solvers must
+    end with a defined status, and a reported convergence must be real."""
     from sc_2022_amd import _capi as capi
     level = 6
     N = 2 ** level + 1
-    A = A_of(level, 8)
+    A = A_of(level, p)
     f, gW, gE, gS, gN, u = O.baseline_data(level)
-    b = O.assemble_rhs(level, 8, f, gW, gE, gS, gN)
-    h = SP(N, 8)
+    b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
+    h = SP(N, p)
     ok = (capi.SBP_OK, capi.SBP_ERR_BREAKDOWN, capi.SBP_ERR_NOT_CONVERGED, capi.SBP_ERR_NONFINITE)
     tol = 1e-8 * np.linalg.norm(b)
     x, res = h.cg(T(b), abstol=tol, maxiter=2000, raise_on_fail=False)

@@ -16,12 +16,12 @@ import scipy.sparse.linalg as spla
 from oracle import sbp_oracle as O
 
 
-@pytest.mark.parametrize("p", [2, 4, 6, 8])
+@pytest.mark.parametrize("p", [2, 4, 6, 8, 10])
 def test_d2_polynomial_exactness(p):
     N = 40
     D2, S0, SN, HI, H, r = O.diagonal_sbp_D2(p, N, xc=(0, 1))
     D2 = D2.toarray()
-    bm = {2: 1, 4: 4, 6: 6, 8: 8}[p]
+    bm = {2: 1, 4: 4, 6: 6, 8: 8, 10: 11}[p]
     kb = p // 2 + 1                  # boundary accuracy degree
     for k in range(0, p + 2):
         v = r ** k
@@ -32,7 +32,7 @@ def test_d2_polynomial_exactness(p):
             assert err.max() < 1e-7 * max(1, k ** 2), (p, k)
 
 
-@pytest.mark.parametrize("p", [2, 4, 6, 8])
+@pytest.mark.parametrize("p", [2, 4, 6, 8, 10])
 def test_d2_sbp_property(p):
     """H D2 = -M + BS with M symmetric positive semidefinite and M 1 = 0
     (diagonal_sbp.jl:159-160)."""
@@ -51,7 +51,7 @@ def test_d2_sbp_property(p):
     assert abs((BS @ r ** 2)[0] - 0.0) < 1e-9 and abs((BS @ r ** 2)[-1] - 2.0) < 1e-9
 
 
-@pytest.mark.parametrize("p", [2, 4, 6, 8])
+@pytest.mark.parametrize("p", [2, 4, 6, 8, 10])
 def test_d1_sbp_property(p):
     N = 30
     D, HI, H, r = O.diagonal_sbp_D1(p, N, xc=(0, 1))
@@ -88,13 +88,12 @@ def test_operator_spd_and_closed_form(level, p):
     assert np.abs(R @ Ad @ R - Ad).max() < 1e-13 * np.abs(Ad).max()
 
 
-@pytest.mark.parametrize("p", [6, 8])
-@pytest.mark.parametrize("level", [4, 5])
+@pytest.mark.parametrize("level,p", [(4, 6), (5, 6), (4, 8), (5, 8), (5, 10)])
 def test_high_order_operator_closed_form_and_definiteness(level, p):
-    """p = 6, 8 (diagonal_sbp.jl:210-311): symmetric, equal to the closed
+    """p = 6, 8, 10 (diagonal_sbp.jl:210-408): symmetric, equal to the closed
     form, 180-degree symmetric; positive definite for p = 6, but indefinite
-    for p = 8 with the reference's Dirichlet penalty tau = 13/h (the 1-D
-    Dirichlet factor needs tau above ~800 for p = 8)."""
+    for p = 8, 10 with the reference's Dirichlet penalty tau = 13/h (the 1-D
+    Dirichlet factor needs tau above ~800 for p = 8, ~3000 for p = 10)."""
     A = O.assemble_operator(level, p)
     Ad = A.toarray()
     assert np.abs(Ad - Ad.T).max() < 1e-13 * np.abs(Ad).max()
@@ -106,23 +105,28 @@ def test_high_order_operator_closed_form_and_definiteness(level, p):
         assert ev > 0
     else:
         assert ev < -1.0
-        hw, m, k = O.one_d_factors(8, 33)
+        hw, m, k = O.one_d_factors(p, 33)
         E = np.zeros((33, 33)); E[0, 0] = E[-1, -1] = 1
-        assert np.linalg.eigvalsh(k + (1000 - 13) * E).min() > 0
+        tau = {8: 1000, 10: 3000}[p]
+        assert np.linalg.eigvalsh(k + (tau - 13) * E).min() > 0
+        assert np.linalg.eigvalsh(k + (tau / 10 - 13) * E).min() < 0
 
 
-def test_high_order_manufactured_convergence():
-    """p = 6: H-norm error of the direct solve falls at about 5.5 per
-    halving (boundary order 3 + 2)."""
+@pytest.mark.parametrize("p,levels,lo,hi", [(6, (5, 6, 7, 8), 5.2, 5.8), (10, (5, 6, 7), 6.8, 7.8)])
+def test_high_order_manufactured_convergence(p, levels, lo, hi):
+    """H-norm error of the direct solve per halving: about 5.5 for p = 6
+    (boundary order 3 + 2) and about 7.3 for p = 10 (boundary order 5 + 2;
+    the p = 10 operator is indefinite but nonsingular).  Finer p = 10 grids
+    reach round-off."""
     errs = []
-    for level in (5, 6, 7, 8):
-        A = O.assemble_operator(level, 6)
+    for level in levels:
+        A = O.assemble_operator(level, p)
         f, gW, gE, gS, gN, u = O.baseline_data(level)
-        b = O.assemble_rhs(level, 6, f, gW, gE, gS, gN)
+        b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
         x = spla.spsolve(A.tocsc(), b)
-        errs.append(O.h_norm(x - u, O.h_tilde_diag(level, 6)))
-    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(3)]
-    assert all(5.2 < r < 5.8 for r in rates), rates
+        errs.append(O.h_norm(x - u, O.h_tilde_diag(level, p)))
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(len(levels) - 1)]
+    assert all(lo < r < hi for r in rates), rates
 
 
 @pytest.mark.parametrize("level", [3, 4, 5])
