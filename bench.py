@@ -367,15 +367,47 @@ def tol_sweep(a, h, b, x, bnorm, rel_err, torch):
             hm.close()
 
 
-def cpu_baseline(N, p, seed, threads, rtol):
-    """The C oracle's FP64 MG-PCG (same V-cycle, OpenMP) on the host cores,
-    one solve of the same problem: a bounded sample of the workload."""
+def host_cpu():
+    """Model name and the CPUs this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "cpus_usable": usable, "cpus_online": os.cpu_count()}
+
+
+def cpu_baseline(N, p, seed, threads, rtol, runs=5, cg_iters=10):
+    """The C oracle's FP64 MG-PCG (same V-cycle, OpenMP) on the host cores:
+    one warm-up and `runs` timed solves of the same problem (median), plus
+    the reference's own CPU path -- CSR assembly and CG_CPU, one thread --
+    over a bounded number of iterations."""
     from oracle import c_oracle
     cores = threads or min(16, os.cpu_count() or 1)
-    res = c_oracle.time_mgpcg_fourier(N, p, seed, rtol, cores)
+    b = c_oracle.fourier_rhs(p, N, seed)
+    res = c_oracle.time_mgpcg_fourier(N, p, seed, rtol, cores, runs=runs, warmup=1, b=b)
+    cg = c_oracle.time_cg_csr_sample(N, p, b, cg_iters, threads=1)
+    cgm = c_oracle.time_cg_csr_sample(N, p, b, cg_iters, threads=cores)
+    c_oracle.set_threads(cores)
     return {"value": 1.0 / res["seconds"], "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": f"1 FP64 MG-PCG solve of the same {N}^2 p={p} problem by oracle/sbp_oracle.c "
-                      f"({res['iters']} iterations, {res['seconds']:.2f} s, OpenMP {cores} threads)"}
+            "sample": f"FP64 MG-PCG solves of the same {N}^2 p={p} problem by oracle/sbp_oracle.c "
+                      f"({res['iters']} iterations; median of {runs} after 1 warm-up, "
+                      f"{res['seconds']:.2f} s each, OpenMP {cores} threads)",
+            "all_seconds": [round(t, 4) for t in res["all_seconds"]],
+            "host": host_cpu(),
+            "csr_cg_one_thread": {
+                "what": f"CSR assembly + {cg['iters']} CG iterations (CG_CPU's loop) at {N}^2 p={p}, 1 thread",
+                "assembly_s": cg["assembly_s"], "ms_per_iter": cg["s_per_iter"] * 1e3, "nnz": cg["nnz"]},
+            "csr_cg_all_cores": {"threads": cores, "assembly_s": cgm["assembly_s"],
+                                 "ms_per_iter": cgm["s_per_iter"] * 1e3}}
 
 
 def traffic_from_profiles(dtype_tag):

@@ -117,13 +117,32 @@ def fourier_rhs(p, N, seed):
     return rhs(p, N, F.reshape(-1), *g)
 
 
-def time_mgpcg_fourier(N, p, seed, rtol, threads):
-    """One FP64 MG-PCG solve of the seeded Fourier problem on `threads` host
-    threads (the bench's CPU-baseline sample)."""
+def time_mgpcg_fourier(N, p, seed, rtol, threads, runs=1, warmup=0, b=None):
+    """FP64 MG-PCG solves of the seeded Fourier problem on `threads` host
+    threads (the bench's CPU-baseline sample): `warmup` untimed solves, then
+    `runs` timed ones; reports the median."""
     set_threads(threads)
-    b = fourier_rhs(p, N, seed)
+    b = fourier_rhs(p, N, seed) if b is None else b
     abstol = rtol * float(np.linalg.norm(b))
+    for _ in range(warmup):
+        mgpcg(p, N, b, abstol)
+    ts = []
+    for _ in range(max(1, runs)):
+        t0 = time.perf_counter()
+        x, it, rn = mgpcg(p, N, b, abstol)
+        ts.append(time.perf_counter() - t0)
+    return {"seconds": float(np.median(ts)), "all_seconds": ts, "iters": it, "resnorm": rn}
+
+
+def time_cg_csr_sample(N, p, b, iters, threads=1):
+    """The reference's CPU path (CSR assembly + CG_CPU) on `threads` host
+    threads: the assembly time and the time of `iters` CG iterations (a
+    bounded sample; the full solve needs thousands)."""
+    set_threads(threads)
     t0 = time.perf_counter()
-    x, it, rn = mgpcg(p, N, b, abstol)
-    dt = time.perf_counter() - t0
-    return {"seconds": dt, "iters": it, "resnorm": rn}
+    A = csr(p, N)
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, it, rn = cg_csr(A, b, 0.0, maxiter=iters)
+    t_cg = time.perf_counter() - t0
+    return {"assembly_s": t_asm, "iters": it, "s_per_iter": t_cg / max(1, it), "nnz": int(len(A[2]))}
