@@ -106,16 +106,20 @@ struct FusedCtl {
 // ---------------------------------------------------------------------------
 // x2 = S(S(0)) and rc = scale * FW(b - A x2)
 // ---------------------------------------------------------------------------
+// The done flag, the table words (face blocks) and region A's loads are all
+// issued before the first wait; a converged problem's block returns false
+// after that wait, before any store.  ta: the kernel-argument table (sizes);
+// t: the table lookups read (ta itself, or its LDS copy ltab once staged).
 template <int P, int TI, int J, int NT, typename T, bool LT, bool INNER>
-__device__ __forceinline__ void sr_body(const T* __restrict__ b, T* __restrict__ x2g, T* __restrict__ rc,
-                                        T omega, int use_diag, T scale, const OpTab& t, int bx, int by,
-                                        int Nc, T* Bt, T* U, T* V) {
+__device__ __forceinline__ bool sr_body(const T* __restrict__ b, T* __restrict__ x2g, T* __restrict__ rc,
+                                        T omega, int use_diag, T scale, const OpTab& ta, const OpTab& t,
+                                        OpTab* ltab, const int* done, int bx, int by, int Nc, T* Bt, T* U, T* V) {
     constexpr int HA = FusedHalo<P>::HA, HB = FusedHalo<P>::HB;
     constexpr int JR = J + 1;                     // fine residual rows (region C)
     constexpr int CI = TI / 2 - 1, CJ = J / 2;    // coarse points owned per block
     constexpr int AW = TI + 2 * HA, AH = JR + 2 * HA;
     constexpr int BW = TI + 2 * HB, BH = JR + 2 * HB;
-    const int N = t.N;
+    const int N = ta.N;
     const int tid = threadIdx.x;
     int Ic0 = bx * CI < Nc - CI ? bx * CI : Nc - CI;
     int Jc0 = by * CJ < Nc - CJ ? by * CJ : Nc - CJ;
@@ -126,6 +130,9 @@ __device__ __forceinline__ void sr_body(const T* __restrict__ b, T* __restrict__
     const int bi0 = ci0 - HB, bj0 = cj0 - HB;   // region B origin
     // A: b and x1 = w D^-1 b (all loads issued before the first use)
     {
+        const int dflag = done ? *done : 0;
+        TabStage<P, NT> tab;
+        if constexpr (LT) tab.fetch(ta.dev);
         constexpr int NA = (AW * AH + NT - 1) / NT;
         T v[NA];
 #pragma unroll
@@ -134,6 +141,11 @@ __device__ __forceinline__ void sr_body(const T* __restrict__ b, T* __restrict__
             const int r = k / AW, c = k - r * AW;
             const int i = ai0 + c, j = aj0 + r;
             v[m] = (k < AW * AH && i >= 0 && i < N && j >= 0 && j < N) ? b[gidx(i, j, N)] : T(0);
+        }
+        if (dflag) return false;
+        if constexpr (LT) {
+            tab.store(ltab);
+            __syncthreads();
         }
 #pragma unroll
         for (int m = 0; m < NA; ++m) {
@@ -200,6 +212,7 @@ __device__ __forceinline__ void sr_body(const T* __restrict__ b, T* __restrict__
             rc[gidx(I, Jc, Nc)] = scale * T(0.25) * (s0 + T(0.5) * (sm + sp));
         }
     }
+    return true;
 }
 
 template <int P, int TI, int J, int NT, typename T>
@@ -213,7 +226,7 @@ __global__ __launch_bounds__(NT) void smooth2_restrict(const T* __restrict__ b, 
     const int nb = ctl.nbx * ctl.nby;
     const int ball = xcd_remap((int)blockIdx.x, nb * ctl.bt.nprob);
     const int q = ball / nb, blk = ball - q * nb;
-    if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
+    const int* done = ctl.done ? ctl.done + (size_t)q * ctl.ds : nullptr;
     b += q * ctl.bt.sf;
     x2g += q * ctl.bt.sf;
     rc += q * ctl.bt.sc;
@@ -231,26 +244,30 @@ __global__ __launch_bounds__(NT) void smooth2_restrict(const T* __restrict__ b, 
     const int ai0 = 2 * Ic0 - 1 - HA, aj0 = 2 * Jc0 - 1 - HA;
     const bool inner = ai0 >= t.R && ai0 + AW <= t.N - t.R && aj0 >= t.R && aj0 + AH <= t.N - t.R;
     if (inner) {
-        sr_body<P, TI, J, NT, T, false, true>(b, x2g, rc, omega, use_diag, scale, t, bx, by, Nc, Bt, U, V);
+        sr_body<P, TI, J, NT, T, false, true>(b, x2g, rc, omega, use_diag, scale, t, t, nullptr, done, bx, by, Nc,
+                                              Bt, U, V);
     } else {
         __shared__ OpTab ltab;
-        stage_table<P>(t, &ltab);
-        sr_body<P, TI, J, NT, T, true, false>(b, x2g, rc, omega, use_diag, scale, ltab, bx, by, Nc, Bt, U, V);
+        sr_body<P, TI, J, NT, T, true, false>(b, x2g, rc, omega, use_diag, scale, t, ltab, &ltab, done, bx, by, Nc,
+                                              Bt, U, V);
     }
 }
 
 // ---------------------------------------------------------------------------
 // out = S(S(x + P e)), optionally with block partials of dotv' out
 // ---------------------------------------------------------------------------
+// As sr_body: flag, table and data loads in one round trip; false (nothing
+// stored) for a converged problem.
 template <int P, int TI, int J, int NT, typename T, bool LT, bool INNER>
-__device__ __forceinline__ T ps_body(const T* __restrict__ x, const T* __restrict__ ec, const T* __restrict__ b,
-                                     T* __restrict__ out, const T* __restrict__ dotv, T omega, int use_diag,
-                                     const OpTab& t, int bx, int by, T* U, T* E, T* Bt, T* V) {
+__device__ __forceinline__ bool ps_body(const T* __restrict__ x, const T* __restrict__ ec, const T* __restrict__ b,
+                                        T* __restrict__ out, const T* __restrict__ dotv, T omega, int use_diag,
+                                        const OpTab& ta, const OpTab& t, OpTab* ltab, const int* done, int bx, int by,
+                                        T* U, T* E, T* Bt, T* V, T& acc) {
     constexpr int HA = FusedHalo<P>::HA, HB = FusedHalo<P>::HB;
     constexpr int AW = TI + 2 * HA, AH = J + 2 * HA;
     constexpr int BW = TI + 2 * HB, BH = J + 2 * HB;
     constexpr int EW = AW / 2 + 2, EH = AH / 2 + 2;
-    const int N = t.N, Nc = (N + 1) / 2;
+    const int N = ta.N, Nc = (N + 1) / 2;
     const int tid = threadIdx.x;
     int i0 = bx * TI < N - TI ? bx * TI : N - TI;
     int j0 = by * J < N - J ? by * J : N - J;
@@ -264,6 +281,9 @@ __device__ __forceinline__ T ps_body(const T* __restrict__ x, const T* __restric
         constexpr int NE = (EW * EH + NT - 1) / NT;
         constexpr int NA = (AW * AH + NT - 1) / NT;
         constexpr int NB = (BW * BH + NT - 1) / NT;
+        const int dflag = done ? *done : 0;
+        TabStage<P, NT> tab;
+        if constexpr (LT) tab.fetch(ta.dev);
         T ve[NE], va[NA], vb[NB];
 #pragma unroll
         for (int m = 0; m < NE; ++m) {
@@ -286,6 +306,8 @@ __device__ __forceinline__ T ps_body(const T* __restrict__ x, const T* __restric
             const int i = bi0 + c, j = bj0 + r;
             vb[m] = (k < BW * BH && i >= 0 && i < N && j >= 0 && j < N) ? b[gidx(i, j, N)] : T(0);
         }
+        if (dflag) return false;
+        if constexpr (LT) tab.store(ltab);   // read after the barriers below
 #pragma unroll
         for (int m = 0; m < NE; ++m) {
             const int k = tid + m * NT;
@@ -333,7 +355,7 @@ __device__ __forceinline__ T ps_body(const T* __restrict__ x, const T* __restric
     }
     __syncthreads();
     // C: out = v1 + w D^-1 (b - A v1) on the owned points
-    T acc = T(0);
+    acc = T(0);
     {
         const int oi0 = bx * TI, oj0 = by * J;
         const int oi1 = oi0 + TI < N ? oi0 + TI : N, oj1 = oj0 + J < N ? oj0 + J : N;
@@ -349,7 +371,7 @@ __device__ __forceinline__ T ps_body(const T* __restrict__ x, const T* __restric
             if (dotv) acc += dotv[g] * o;
         }
     }
-    return acc;
+    return true;
 }
 
 template <int P, int TI, int J, int NT, typename T>
@@ -364,7 +386,7 @@ __global__ __launch_bounds__(NT) void prolong_smooth2(const T* __restrict__ x, c
     const int nb = ctl.nbx * ctl.nby;
     const int ball = xcd_remap((int)blockIdx.x, nb * ctl.bt.nprob);
     const int q = ball / nb, blk = ball - q * nb;
-    if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
+    const int* done = ctl.done ? ctl.done + (size_t)q * ctl.ds : nullptr;
     x += q * ctl.bt.sf;
     b += q * ctl.bt.sf;
     out += q * ctl.bt.sf;
@@ -382,15 +404,17 @@ __global__ __launch_bounds__(NT) void prolong_smooth2(const T* __restrict__ x, c
     if (j0 < 0) j0 = 0;
     const int ai0 = i0 - HA, aj0 = j0 - HA;
     const bool inner = ai0 >= t.R && ai0 + AW <= N - t.R && aj0 >= t.R && aj0 + AH <= N - t.R;
-    T acc;
+    T acc = T(0);
+    bool ran;
     if (inner) {
-        acc = ps_body<P, TI, J, NT, T, false, true>(x, ec, b, out, dotv, omega, use_diag, t, bx, by, U, E, Bt, V);
+        ran = ps_body<P, TI, J, NT, T, false, true>(x, ec, b, out, dotv, omega, use_diag, t, t, nullptr, done, bx, by,
+                                                    U, E, Bt, V, acc);
     } else {
         __shared__ OpTab ltab;
-        stage_table<P>(t, &ltab);
-        acc = ps_body<P, TI, J, NT, T, true, false>(x, ec, b, out, dotv, omega, use_diag, ltab, bx, by, U, E, Bt,
-                                                     V);
+        ran = ps_body<P, TI, J, NT, T, true, false>(x, ec, b, out, dotv, omega, use_diag, t, ltab, &ltab, done, bx,
+                                                    by, U, E, Bt, V, acc);
     }
+    if (!ran) return;   // converged: every thread of the block left before its first barrier
     if (dotv) {
         __syncthreads();
         const T s = block_sum<T, NT>(acc);

@@ -467,25 +467,48 @@ struct PassCtl {
 };
 
 // Copies the device table into LDS (blockDim.x threads, one word each).
-// Copies the parts of the table an order-P kernel reads into LDS (the
-// scalars, hw, the first R rows of mb and kb, and the per-row arrays), so
-// the copy does not grow with the largest order's closure.
-template <int P>
-__device__ __forceinline__ void stage_table(const OpTab& t, OpTab* l) {
+// The parts of the table an order-P kernel reads (the scalars, hw, the first
+// R rows of mb and kb, and the per-row arrays), copied into LDS in 8-byte
+// words by NT threads, so the copy does not grow with the largest order's
+// closure.  fetch() only issues the loads (into registers) and store() writes
+// LDS, so a kernel can overlap the table's round trip with its data loads.
+template <int P, int NT>
+struct TabStage {
     static_assert(sizeof(OpTab) % 8 == 0 && offsetof(OpTab, mb) % 8 == 0 && offsetof(OpTab, kb) % 8 == 0 &&
                       offsetof(OpTab, md) % 8 == 0,
                   "table is copied in 8-byte words");
-    constexpr int rows = Stencil<P>::R * kMaxC;
-    constexpr int a1 = (int)(offsetof(OpTab, mb) / 8) + rows;   // scalars, hw, mb rows < R
-    constexpr int b0 = (int)(offsetof(OpTab, kb) / 8);          // kb rows < R
-    constexpr int c0 = (int)(offsetof(OpTab, md) / 8);          // md .. dev
-    constexpr int n = a1 + rows + ((int)(sizeof(OpTab) / 8) - c0);
-    const unsigned long long* src = reinterpret_cast<const unsigned long long*>(t.dev);
-    unsigned long long* dst = reinterpret_cast<unsigned long long*>(l);
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const int w = k < a1 ? k : (k < a1 + rows ? b0 + (k - a1) : c0 + (k - a1 - rows));
-        dst[w] = src[w];
+    static constexpr int rows = Stencil<P>::R * kMaxC;
+    static constexpr int a1 = (int)(offsetof(OpTab, mb) / 8) + rows;   // scalars, hw, mb rows < R
+    static constexpr int b0 = (int)(offsetof(OpTab, kb) / 8);          // kb rows < R
+    static constexpr int c0 = (int)(offsetof(OpTab, md) / 8);          // md .. dev
+    static constexpr int n = a1 + rows + ((int)(sizeof(OpTab) / 8) - c0);
+    static constexpr int K = (n + NT - 1) / NT;
+    unsigned long long v[K];
+    __device__ static int word(int k) { return k < a1 ? k : (k < a1 + rows ? b0 + (k - a1) : c0 + (k - a1 - rows)); }
+    __device__ __forceinline__ void fetch(const OpTab* src) {
+        const unsigned long long* s = reinterpret_cast<const unsigned long long*>(src);
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            const int k = (int)threadIdx.x + m * NT;
+            if (k < n) v[m] = s[word(k)];
+        }
     }
+    __device__ __forceinline__ void store(OpTab* dst) const {
+        unsigned long long* d = reinterpret_cast<unsigned long long*>(dst);
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            const int k = (int)threadIdx.x + m * NT;
+            if (k < n) d[word(k)] = v[m];
+        }
+    }
+};
+
+// Fetch, store and barrier in one (callers with nothing to overlap).
+template <int P, int NT>
+__device__ __forceinline__ void stage_table(const OpTab& t, OpTab* l) {
+    TabStage<P, NT> st;
+    st.fetch(t.dev);
+    st.store(l);
     __syncthreads();
 }
 
@@ -688,7 +711,7 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
         // first (one load per thread) so those lookups cost LDS, not global,
         // latency -- on the small multigrid levels every tile is a face tile.
         __shared__ OpTab ltab;
-        stage_table<P>(t, &ltab);
+        stage_table<P, TI>(t, &ltab);
         if (jin) acc = pass_interior<P, TI, J, true, T, true>(in, out, ltab, i0, j0, tile);
         else acc = pass_general<P, TI, J, T, true>(in, out, ltab, i0, j0, tile);
     }
@@ -771,7 +794,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
     const bool jin = fj0 - H >= t.R && fj0 + JR + H <= N - t.R;
     const bool iin = fi0 - H >= t.R && fi0 + TI + H <= N - t.R;
     __shared__ OpTab ltab;
-    if (!(jin && iin)) stage_table<P>(t, &ltab);   // per-thread closure lookups from LDS
+    if (!(jin && iin)) stage_table<P, TI>(t, &ltab);   // per-thread closure lookups from LDS
     const OpTab& tl = ltab;
     if (jin) {
         // Rows interior: register window along j, LDS tile along i; with
