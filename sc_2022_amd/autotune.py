@@ -24,7 +24,7 @@ CACHE = HERE / "tiles_gfx950.json"
 
 
 def compiled_shapes(p=2):
-    """Tile shapes libsbp.so has for order p (p = 6, 8: a subset)."""
+    """Tile shapes libsbp.so has for order p (p = 6, 8, 10: a subset)."""
     text = (HERE / "csrc" / "sbp_launch.h").read_text()
     name = "SBP_TILE_SHAPES" if p <= 4 else "SBP_HI_TILE_SHAPES"
     line = re.search(r"#define " + name + r"\(X\)(.*)", text).group(1)

@@ -1042,8 +1042,8 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
     h->p = p;
     h->nprob = nprob;
     if (prm) h->prm = *prm; else sbp_default_params(&h->prm);
-    // p = 6, 8 closures need at least 2 C = 18 / 24 points per side on every
-    // level: their coarsest grid is 33 x 33 at the least.
+    // p = 6, 8, 10 closures need at least 2 C = 18 / 24 / 32 points per side
+    // on every level: their coarsest grid is 33 x 33 at the least.
     if (p >= 6 && h->prm.coarse_n < 33) h->prm.coarse_n = 33;
     if (!build_optab(p, N, &h->L[0].tb, &h->rt)) { delete h; return SBP_ERR_INVALID_SIZE; }
     h->L[0].N = N;

@@ -39,7 +39,7 @@ using ResidFF = OutResid<float, float>;
 
 // Compiled tile shapes (TI threads along i, J rows per thread).
 #define SBP_TILE_SHAPES(X) X(64, 4) X(64, 8) X(64, 16) X(128, 8) X(128, 16) X(256, 4) X(256, 8) X(256, 16)
-// The subset also compiled for p = 6 and 8 (every level's default shape).
+// The subset also compiled for p = 6, 8 and 10 (every level's default shape).
 #define SBP_HI_TILE_SHAPES(X) X(64, 4) X(128, 8) X(256, 8)
 
 inline int nparts_of(int N, int ti, int tj) { return ((N + ti - 1) / ti) * ((N + tj - 1) / tj); }
