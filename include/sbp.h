@@ -112,9 +112,15 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
  * > 0.  The Dirichlet penalty is -13 b_face / h (the reference's -13/h at
  * b = 1) and the Neumann data are the fluxes b du/dn.  Such a handle serves
  * sbp_apply (FP64, level 0), sbp_apply_dot, sbp_cg, sbp_assemble_rhs,
- * sbp_rhs_transpose and sbp_hnorm; the multigrid, smoother, diag and CSR
- * entry points return SBP_ERR_INVALID_ARG / SBP_ERR_INVALID_SIZE.  N >= 10,
- * 18, 26 for p = 2, 4, 6. */
+ * sbp_rhs_transpose and sbp_hnorm, and for N = 2^k + 1 also sbp_vcycle
+ * (FP64) and sbp_mgpcg: FP64 MG-PCG on the hierarchy rediscretised with the
+ * profiles injected on every level (b_l[i] = b[2^l i]; damped Jacobi with
+ * nu_pre / nu_post / omega, full weighting times restrict_scale, bilinear
+ * prolongation, dense coarsest solve at coarse_n <= 33, raised to 33 for
+ * p = 6).  This hierarchy has no counterpart in the reference, which builds
+ * no multigrid for this operator.  The mixed-precision, batched, smoother,
+ * diag and CSR entry points return SBP_ERR_INVALID_ARG /
+ * SBP_ERR_INVALID_SIZE.  N >= 8, 16, 24 for p = 2, 4, 6. */
 int sbp_create_var(int N, int p, const double* bx, const double* by, const sbp_params* prm,
                    sbp_handle* out);
 int sbp_get_batch(sbp_handle h, int* nprob);

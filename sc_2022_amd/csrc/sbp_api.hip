@@ -99,6 +99,20 @@ struct sbp_handle_s {
     bool var = false;
     VarTab* vtab = nullptr;
     double* vband = nullptr;
+    // Its multigrid hierarchy (N = 2^k + 1): per level the device table and
+    // bands (level 0: vtab / vband), 1 / diag(A), and the level's right-hand
+    // side b, iterate x and sweep buffer t (level 0: b is the caller's, x and
+    // t are z / pv2).  The coarsest level is solved with Ainv (nc x nc).
+    struct VarLevel {
+        int N = 0;
+        size_t n = 0;
+        VarTab* dtab = nullptr;
+        double* band = nullptr;
+        double* dinv = nullptr;
+        double *b = nullptr, *x = nullptr, *t = nullptr;
+    };
+    std::vector<VarLevel> vl;
+    bool vmg = false;
 };
 
 namespace {
@@ -999,6 +1013,15 @@ int sbp_destroy(sbp_handle h) {
     if (h->cexact) (void)sbp_destroy(h->cexact);
     if (h->vtab) (void)hipFree(h->vtab);
     if (h->vband) (void)hipFree(h->vband);
+    for (size_t l = 0; l < h->vl.size(); ++l) {
+        auto& V = h->vl[l];
+        if (V.dinv) (void)hipFree(V.dinv);
+        if (l > 0) {
+            void* own[] = {V.dtab, V.band, V.b, V.x, V.t};
+            for (void* q : own)
+                if (q) (void)hipFree(q);
+        }
+    }
     void* ptrs[] = {h->cbuf, h->xc, h->w, h->lam, h->Ainv, h->r, h->pv, h->pv2, h->Ap, h->z, h->x, h->r32, h->p32, h->p32b, h->Ap32,
                     h->z32, h->d32, h->partial, h->sc, h->sc_in, h->dtmp, h->dabs};
     for (void* q : ptrs)
@@ -1156,6 +1179,83 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
 
 int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) { return sbp_create_batch(N, p, 1, prm, out); }
 
+namespace {
+// Multigrid hierarchy of a variable-coefficient handle: levels of side
+// (N+1)/2, ... down to the coarse side (coarse_n, at least 33 for p = 6,
+// whose closures need 26 points), each the operator rediscretised with the
+// profiles injected (b_l[i] = b[2^l i]); the coarsest level's dense inverse.
+// Leaves vmg false (apply and CG only) when N is not 2^k + 1, the coarse side
+// is not reachable or above 33, or the coarsest operator is not SPD.
+int var_hierarchy(sbp_handle h, const VarTab& t0, const double* bx, const double* by) {
+    const int N = h->N, p = h->p;
+    auto pow2p1 = [](int v) { return v >= 3 && ((v - 1) & (v - 2)) == 0; };
+    int cn = h->prm.coarse_n;
+    if (p == 6 && cn < 33) cn = 33;
+    if (!pow2p1(N) || !pow2p1(cn) || cn > 33 || cn > N) return SBP_OK;
+    std::vector<int> sides{N};
+    while (sides.back() > cn) sides.push_back((sides.back() + 1) / 2);
+    if (sides.back() != cn) return SBP_OK;
+    const int nlev = (int)sides.size();
+    h->vl.resize(nlev);
+    std::vector<double> bxl(bx, bx + N), byl(by, by + N), kb, mb, A;
+    for (int l = 0; l < nlev; ++l) {
+        auto& V = h->vl[l];
+        V.N = sides[l];
+        V.n = (size_t)V.N * V.N;
+        VarTab t = t0;
+        if (l == 0) {
+            V.dtab = h->vtab;
+            V.band = h->vband;
+            V.x = h->z;
+            V.t = h->pv2;
+        } else {
+            std::vector<double> cx(V.N), cy(V.N);
+            for (int k = 0; k < V.N; ++k) { cx[k] = bxl[2 * k]; cy[k] = byl[2 * k]; }
+            bxl.swap(cx);
+            byl.swap(cy);
+            OpTab ot;
+            if (!build_optab(p, V.N, &ot, nullptr)) return SBP_OK;
+            t = VarTab{};
+            if (!build_var_axis(p, V.N, bxl.data(), true, &t.k, kb) || !build_var_axis(p, V.N, byl.data(), false, &t.m, mb))
+                return SBP_OK;
+            t.N = V.N;
+            t.H = p / 2;
+            t.R = var_R(p);
+            t.C = var_C(p);
+            t.RH = ot.RH;
+            for (int r = 0; r < kMaxR; ++r) t.hw[r] = ot.hw[r];
+            const size_t nb = kb.size();
+            HIPCHK(hipMalloc(&V.band, 2 * nb * sizeof(double)));
+            HIPCHK(hipMalloc(&V.dtab, sizeof(VarTab)));
+            t.k.band = V.band;
+            t.m.band = V.band + nb;
+            HIPCHK(hipMemcpy(V.band, kb.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(V.band + nb, mb.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(V.dtab, &t, sizeof(VarTab), hipMemcpyHostToDevice));
+            HIPCHK(hipMalloc(&V.b, V.n * sizeof(double)));
+            HIPCHK(hipMalloc(&V.x, V.n * sizeof(double)));
+            HIPCHK(hipMalloc(&V.t, V.n * sizeof(double)));
+        }
+        HIPCHK(hipMalloc(&V.dinv, V.n * sizeof(double)));
+        hipLaunchKernelGGL(var_diag, dim3((V.N + 255) / 256, V.N), dim3(256), 0, h->st, V.dinv, (const VarTab*)V.dtab);
+        HIPCHK(hipGetLastError());
+        if (l == nlev - 1) {
+            if (l == 0) return SBP_OK;   // N is the coarse side: nothing to precondition
+            VarTab th = t;                // host bands for the dense build
+            dense_operator_var(th, kb.data(), mb.data(), A);
+            const int nc = (int)V.n;
+            if (!spd_inverse(A, nc)) return SBP_OK;
+            HIPCHK(hipMalloc(&h->Ainv, (size_t)nc * nc * sizeof(double)));
+            HIPCHK(hipMemcpy(h->Ainv, A.data(), (size_t)nc * nc * sizeof(double), hipMemcpyHostToDevice));
+            h->nc = nc;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->vmg = true;
+    return SBP_OK;
+}
+}  // namespace
+
 int sbp_create_var(int N, int p, const double* bx, const double* by, const sbp_params* prm, sbp_handle* out) {
     if (!out) return SBP_ERR_INVALID_ARG;
     *out = nullptr;
@@ -1194,6 +1294,7 @@ int sbp_create_var(int N, int p, const double* bx, const double* by, const sbp_p
     h->var = true;
     h->rt.bW = bx[0];
     h->rt.bE = bx[N - 1];
+    if ((rc = var_hierarchy(h, t, bx, by)) != SBP_OK) { sbp_destroy(h); return rc; }
     *out = h;
     return SBP_OK;
 }
@@ -1208,7 +1309,7 @@ int sbp_get_info(sbp_handle h, int* N, int* p, int* nlevels) {
     if (!h) return SBP_ERR_INVALID_ARG;
     if (N) *N = h->N;
     if (p) *p = h->p;
-    if (nlevels) *nlevels = h->mg ? h->nlev : 0;
+    if (nlevels) *nlevels = h->var ? (h->vmg ? (int)h->vl.size() : 0) : (h->mg ? h->nlev : 0);
     return SBP_OK;
 }
 
@@ -1237,20 +1338,31 @@ namespace {
 // One launch of the variable-coefficient operator (MODE as var_apply);
 // *nparts receives the number of block partials.
 int launch_var(sbp_handle h, int mode, const double* u, double* out, const double* aux, double sign,
-               const Scalars* sc, int* nparts) {
-    const int N = h->N;
+               const Scalars* sc, int* nparts, int level = 0, double omega = 0.0) {
+    const int N = level == 0 ? h->N : h->vl[level].N;
+    const VarTab* tab = level == 0 ? h->vtab : h->vl[level].dtab;
+    const double* dinv = level < (int)h->vl.size() ? h->vl[level].dinv : nullptr;
+    if (mode == 3 && !dinv) return SBP_ERR_INVALID_ARG;
     const dim3 g((N + kVarThreads - 1) / kVarThreads, (N + kVarRows - 1) / kVarRows);
     if ((size_t)g.x * g.y > h->npp) return SBP_ERR_INVALID_SIZE;
 #define SBP_VAR_LAUNCH(P_, M_) \
-    hipLaunchKernelGGL((var_apply<P_, M_>), g, dim3(kVarThreads), 0, h->st, u, out, aux, sign, h->partial, \
-                       (const VarTab*)h->vtab, sc)
-    if (h->p == 2) {
-        if (mode == 0) SBP_VAR_LAUNCH(2, 0); else if (mode == 1) SBP_VAR_LAUNCH(2, 1); else SBP_VAR_LAUNCH(2, 2);
-    } else if (h->p == 4) {
-        if (mode == 0) SBP_VAR_LAUNCH(4, 0); else if (mode == 1) SBP_VAR_LAUNCH(4, 1); else SBP_VAR_LAUNCH(4, 2);
-    } else {
-        if (mode == 0) SBP_VAR_LAUNCH(6, 0); else if (mode == 1) SBP_VAR_LAUNCH(6, 1); else SBP_VAR_LAUNCH(6, 2);
+    hipLaunchKernelGGL((var_apply<P_, M_>), g, dim3(kVarThreads), 0, h->st, u, out, aux, sign, h->partial, tab, sc, \
+                       dinv, omega)
+#define SBP_VAR_MODES(P_)                       \
+    switch (mode) {                             \
+    case 0: SBP_VAR_LAUNCH(P_, 0); break;       \
+    case 1: SBP_VAR_LAUNCH(P_, 1); break;       \
+    case 2: SBP_VAR_LAUNCH(P_, 2); break;       \
+    default: SBP_VAR_LAUNCH(P_, 3); break;      \
     }
+    if (h->p == 2) {
+        SBP_VAR_MODES(2)
+    } else if (h->p == 4) {
+        SBP_VAR_MODES(4)
+    } else {
+        SBP_VAR_MODES(6)
+    }
+#undef SBP_VAR_MODES
 #undef SBP_VAR_LAUNCH
     HIPCHK(hipGetLastError());
     if (nparts) *nparts = (int)(g.x * g.y);
@@ -1283,6 +1395,113 @@ int var_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter,
                            (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
         hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 1);
         hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)h->r,
+                           (const Scalars*)sc, n);
+        HIPCHK(hipGetLastError());
+        if (it + 1 >= next && it + 1 < maxiter) {
+            bool all = false;
+            STCHK(poll(h, sc, &all));
+            if (all) break;
+            next = it + 1 + (adaptive ? poll_ahead(h) : h->prm.poll_every);
+        }
+    }
+    return poll(h, sc);
+}
+
+// One V-cycle x ~ A_l^-1 b on level l of the variable-coefficient hierarchy
+// from a zero initial guess: nu_pre damped-Jacobi sweeps (the first one
+// x = w D^-1 b), residual, full-weighting restriction (times restrict_scale),
+// the coarser cycle, bilinear prolongation, nu_post sweeps; the coarsest
+// level is the dense inverse.  Sweeps ping-pong between x and t, starting so
+// that the last one lands in x; the residual goes to whichever is free.
+int var_vcycle(sbp_handle h, int l, const double* b, double* x, const Scalars* sc) {
+    auto& V = h->vl[l];
+    const int nlev = (int)h->vl.size();
+    hipStream_t st = h->st;
+    if (l == nlev - 1) {
+        hipLaunchKernelGGL((coarse_solve<double>), dim3((h->nc + kCoarseRows - 1) / kCoarseRows, 1), dim3(256), 0, st,
+                           (const double*)h->Ainv, b, x, h->nc, sc);
+        HIPCHK(hipGetLastError());
+        return SBP_OK;
+    }
+    const int nu1 = h->prm.nu_pre, nu2 = h->prm.nu_post;
+    const double om = h->prm.omega;
+    const dim3 eg(ew_blocks(V.n));
+    double* bufs[2] = {x, V.t};
+    // sweeps after the first: (nu1 - 1) + nu2 (or nu2 - 1 after a zero start)
+    const int later = nu1 > 0 ? (nu1 - 1) + nu2 : nu2;
+    int cur = later % 2 == 0 ? 0 : 1;   // the buffer the first sweep writes
+    if (nu1 > 0) {
+        hipLaunchKernelGGL(var_jacobi0, eg, dim3(256), 0, st, bufs[cur], b, (const double*)V.dinv, om, V.n, sc);
+    } else {
+        hipLaunchKernelGGL((fill<double, kEwThreads>), dim3(ew_blocks(V.n), 1), dim3(kEwThreads), 0, st, bufs[cur], 0.0,
+                           V.n, sc);
+    }
+    HIPCHK(hipGetLastError());
+    for (int s = 1; s < nu1; ++s) {
+        STCHK(launch_var(h, 3, bufs[cur], bufs[1 - cur], b, 1.0, sc, nullptr, l, om));
+        cur = 1 - cur;
+    }
+    // residual into the free buffer, restricted to the coarser right-hand side
+    auto& C = h->vl[l + 1];
+    STCHK(launch_var(h, 2, bufs[cur], bufs[1 - cur], b, 1.0, sc, nullptr, l));
+    hipLaunchKernelGGL((restrict_fw<double>), dim3((C.N + 255) / 256, C.N, 1), dim3(256), 0, st,
+                       (const double*)bufs[1 - cur], C.b, V.N, C.N, h->prm.restrict_scale, sc);
+    HIPCHK(hipGetLastError());
+    STCHK(var_vcycle(h, l + 1, C.b, C.x, sc));
+    hipLaunchKernelGGL((prolong_add<double>), dim3((V.N + 255) / 256, V.N, 1), dim3(256), 0, st, (const double*)C.x,
+                       bufs[cur], V.N, C.N, sc);
+    HIPCHK(hipGetLastError());
+    for (int s = 0; s < nu2; ++s) {
+        STCHK(launch_var(h, 3, bufs[cur], bufs[1 - cur], b, 1.0, sc, nullptr, l, om));
+        cur = 1 - cur;
+    }
+    if (bufs[cur] != x) {
+        hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(V.n), 1), dim3(kEwThreads), 0, st,
+                           (const double*)bufs[cur], x, V.n, sc);
+        HIPCHK(hipGetLastError());
+    }
+    return SBP_OK;
+}
+
+// MG-preconditioned CG on the variable-coefficient operator (unfused, as
+// var_cg): the V-cycle above as preconditioner; stops on ||r||_2 < abstol.
+int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, double* hist) {
+    hipStream_t st = h->st;
+    const size_t n = h->L[0].n;
+    Scalars* sc = h->sc;
+    double* z = h->vl[0].x;
+    STCHK(put_abstol(h, &abstol));
+    hipLaunchKernelGGL(stamp_start, dim3(1), dim3(256), 0, st, sc, 1);
+    int np = 0;
+    STCHK(launch_var(h, 2, x, h->r, b, 1.0, nullptr, &np));
+    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, (const double*)h->dabs,
+                       hist, maxiter);
+    HIPCHK(hipGetLastError());
+    const dim3 g(ew_blocks(n), 1);
+    auto precondition = [&]() -> int {   // z = M r ; rz partials
+        STCHK(var_vcycle(h, 0, h->r, z, sc));
+        hipLaunchKernelGGL((dot_partial<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->r,
+                           (const double*)z, n, h->partial);
+        HIPCHK(hipGetLastError());
+        return SBP_OK;
+    };
+    STCHK(precondition());
+    hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc);
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)z, h->pv, n,
+                       (const Scalars*)sc);
+    HIPCHK(hipGetLastError());
+    const bool adaptive = h->prm.poll_every == 0;
+    int next = adaptive ? 2 : h->prm.poll_every;
+    for (int it = 0; it < maxiter; ++it) {
+        STCHK(launch_var(h, 1, h->pv, h->Ap, h->pv, 1.0, sc, &np));
+        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, 1);
+        hipLaunchKernelGGL((update_xr<double, kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, (const double*)h->pv,
+                           (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
+        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 0);
+        HIPCHK(hipGetLastError());
+        STCHK(precondition());
+        hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc);
+        hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)z,
                            (const Scalars*)sc, n);
         HIPCHK(hipGetLastError());
         if (it + 1 >= next && it + 1 < maxiter) {
@@ -1427,6 +1646,10 @@ int sbp_coarse_solve(sbp_handle h, const double* b, double* x) {
 }
 
 int sbp_vcycle(sbp_handle h, const void* r, void* z, int dtype) {
+    if (h && h->var) {
+        if (!r || !z || !h->vmg || dtype != SBP_F64) return SBP_ERR_INVALID_ARG;
+        return var_vcycle(h, 0, (const double*)r, (double*)z, nullptr);
+    }
     if (!h || !r || !z || !h->mg) return SBP_ERR_INVALID_ARG;
     if (dtype == SBP_F64) return vcycle_level<double>(h, 0, (const double*)r, (double*)z, nullptr, nullptr, nullptr);
     if (dtype == SBP_F32) {
@@ -1452,6 +1675,12 @@ int sbp_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter,
 int sbp_mgpcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
               double* hist) {
     if (!h || !b || !x || maxiter < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
+    if (h->var) {
+        if (!h->vmg) return SBP_ERR_INVALID_SIZE;
+        STCHK(var_pcg(h, b, x, abstol, maxiter, hist));
+        get_results(h, iters, resnorm, nullptr);
+        return batch_status(h);
+    }
     if (!h->mg) return SBP_ERR_INVALID_SIZE;
     STCHK(krylov64(h, b, x, &abstol, maxiter, hist, true));
     get_results(h, iters, resnorm, nullptr);

@@ -208,7 +208,7 @@ inline void var_m_entries(int p, int Np, const double* B, PUT&& put) {
 // Returns false when N is too small or an entry falls outside the layout.
 inline bool build_var_axis(int p, int N, const double* b, bool dirichlet, VarAxis* ax, std::vector<double>& band) {
     const int H = p / 2, R = var_R(p), C = var_C(p);
-    if (N < 2 * C + 2) return false;
+    if (N < 2 * C) return false;
     std::memset(ax->lo, 0, sizeof(ax->lo));
     std::memset(ax->hi, 0, sizeof(ax->hi));
     band.assign((size_t)(2 * H + 1) * N, 0.0);
@@ -266,15 +266,18 @@ __device__ __forceinline__ double var_row(const VarAxis& a, int R, int C, int N,
     return s;
 }
 
-// MODE 0: out = sign A u;  1: out = A u, partial += aux' out;  2: out = aux - A u, partial += out' out.
+// MODE 0: out = sign A u;  1: out = A u, partial += aux' out;  2: out = aux - A u, partial += out' out;
+// 3: damped-Jacobi sweep out = u + omega dinv (aux - A u) (no partial).
 // A block: kVarThreads threads along i, kVarRows consecutive rows j.
 constexpr int kVarThreads = 256;
 constexpr int kVarRows = 8;
 
 template <int MODE>
 __device__ __forceinline__ void var_emit(double* out, const double* aux, double sign, size_t q, double Au,
-                                         double& acc) {
-    if constexpr (MODE == 0) {
+                                         double& acc, double uc, const double* dinv, double omega) {
+    if constexpr (MODE == 3) {
+        out[q] = uc + omega * dinv[q] * (aux[q] - Au);
+    } else if constexpr (MODE == 0) {
         out[q] = sign * Au;
     } else if constexpr (MODE == 1) {
         out[q] = Au;
@@ -296,7 +299,8 @@ __device__ __forceinline__ void var_emit(double* out, const double* aux, double 
 // (direct loads, table t staged in LDS) and their H weight.
 template <int P, int MODE, bool ICHK>
 __device__ __forceinline__ double var_tile(const double* __restrict__ u, double* __restrict__ out,
-                                           const double* __restrict__ aux, double sign, const VarTab* __restrict__ tp,
+                                           const double* __restrict__ aux, double sign, const double* __restrict__ dinv,
+                                           double omega, const VarTab* __restrict__ tp,
                                            const VarTab& t, int i0, int j0,
                                            double (*tile)[kVarThreads + 2 * (P / 2)]) {
     constexpr int H = P / 2;
@@ -342,7 +346,7 @@ __device__ __forceinline__ double var_tile(const double* __restrict__ u, double*
         } else {
             mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
         }
-        var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + mp, acc);
+        var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + mp, acc, w[r + H], dinv, omega);
     }
     return acc;
 }
@@ -350,7 +354,8 @@ __device__ __forceinline__ double var_tile(const double* __restrict__ u, double*
 template <int P, int MODE>
 __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restrict__ u, double* __restrict__ out,
                                                         const double* __restrict__ aux, double sign, double* partial,
-                                                        const VarTab* __restrict__ tp, const Scalars* sc) {
+                                                        const VarTab* __restrict__ tp, const Scalars* sc,
+                                                        const double* __restrict__ dinv, double omega) {
     constexpr int H = P / 2;
     constexpr int J = kVarRows;
     if (sc && sc->done) return;   // the whole launch: a converged solve
@@ -363,7 +368,7 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
     const bool iin = i0 - H >= R && i0 + kVarThreads + H <= N - R;
     double acc = 0.0;
     if (jin && iin) {
-        acc = var_tile<P, MODE, false>(u, out, aux, sign, tp, t, i0, j0, tile);
+        acc = var_tile<P, MODE, false>(u, out, aux, sign, dinv, omega, tp, t, i0, j0, tile);
     } else {
         {
             const unsigned long long* src = reinterpret_cast<const unsigned long long*>(tp);
@@ -372,7 +377,7 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
         }
         __syncthreads();
         if (jin) {
-            acc = var_tile<P, MODE, true>(u, out, aux, sign, tp, t, i0, j0, tile);
+            acc = var_tile<P, MODE, true>(u, out, aux, sign, dinv, omega, tp, t, i0, j0, tile);
         } else {
             const int i = i0 + tid;   // face rows: per point
             if (i < N) {
@@ -385,16 +390,75 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
                     const double hwj = rj < t.RH ? t.hw[rj] : 1.0;
                     const double kp = var_row<H>(t.k, t.R, t.C, N, j, [&](int c) { return u[gidx(i, c, N)]; });
                     const double mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
-                    var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + hwj * mp, acc);
+                    var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + hwj * mp, acc, u[gidx(i, j, N)],
+                                   dinv, omega);
                 }
             }
         }
     }
-    if constexpr (MODE != 0) {
+    if constexpr (MODE == 1 || MODE == 2) {
         __syncthreads();
         const double s = block_sum<double, kVarThreads>(acc);
         if (threadIdx.x == 0) partial[blockIdx.x + (size_t)blockIdx.y * gridDim.x] = s;
     }
+}
+
+// Diagonal entry of one axis' factor at row r.
+__device__ __forceinline__ double var_diag1(const VarAxis& a, int R, int H, int N, int r) {
+    if (r < R) return a.lo[r][r];
+    if (r >= N - R) return a.hi[N - 1 - r][N - 1 - r];
+    return a.band[(size_t)H * N + r];
+}
+
+// dinv = 1 / diag(A): diag at (i, j) is hw_i k_jj + hw_j m_ii.
+__global__ __launch_bounds__(256) void var_diag(double* __restrict__ dinv, const VarTab* __restrict__ tp) {
+    const VarTab& t = *tp;
+    const int N = t.N;
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= N) return;
+    const int ri = i < N - 1 - i ? i : N - 1 - i, rj = j < N - 1 - j ? j : N - 1 - j;
+    const double hwi = ri < t.RH ? t.hw[ri] : 1.0, hwj = rj < t.RH ? t.hw[rj] : 1.0;
+    const double d = hwi * var_diag1(t.k, t.R, t.H, N, j) + hwj * var_diag1(t.m, t.R, t.H, N, i);
+    dinv[gidx(i, j, N)] = 1.0 / d;
+}
+
+// First damped-Jacobi sweep from zero: x = omega dinv b.
+__global__ __launch_bounds__(256) void var_jacobi0(double* __restrict__ x, const double* __restrict__ b,
+                                                  const double* __restrict__ dinv, double omega, size_t n,
+                                                  const Scalars* sc) {
+    if (sc && sc->done) return;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += stride) x[k] = omega * dinv[k] * b[k];
+}
+
+// Dense A (row-major, n = N*N) of the variable-coefficient operator from
+// the host tables and host bands (coarsest-level factorisation).
+inline void dense_operator_var(const VarTab& t, const double* kband, const double* mband, std::vector<double>& A) {
+    const int N = t.N, n = N * N, H = t.H;
+    A.assign((size_t)n * n, 0.0);
+    auto hwf = [&](int r) {
+        const int rr = r < N - 1 - r ? r : N - 1 - r;
+        return rr < t.RH ? t.hw[rr] : 1.0;
+    };
+    auto coef = [&](const VarAxis& a, const double* band, int r, int c) -> double {
+        if (r < t.R) return c < t.C ? a.lo[r][c] : 0.0;
+        if (r >= N - t.R) {
+            const int rr = N - 1 - r, cc = N - 1 - c;
+            return (cc >= 0 && cc < t.C) ? a.hi[rr][cc] : 0.0;
+        }
+        const int s = c - r;
+        return (s >= -H && s <= H) ? band[(size_t)(s + H) * N + r] : 0.0;
+    };
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) {
+            const size_t row = (size_t)(i + j * N) * n;
+            for (int c = 0; c < N; ++c) {
+                const double kv = coef(t.k, kband, j, c);
+                if (kv != 0.0) A[row + (i + c * N)] += hwf(i) * kv;
+                const double mv = coef(t.m, mband, i, c);
+                if (mv != 0.0) A[row + (c + j * N)] += hwf(j) * mv;
+            }
+        }
 }
 
 }  // namespace sbp

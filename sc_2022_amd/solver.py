@@ -46,7 +46,7 @@ class SbpPoisson:
     def __init__(self, N: int, p: int = 2, device=None, tuned: bool = True, coef=None, **params):
         """coef = (bx, by): the variable-coefficient operator d/dx (bx d/dx) +
         d/dy (by d/dy) with N-value profiles along the slow (Dirichlet) and
-        fast (Neumann) index (sbp_create_var; p = 2, 4; apply, CG and the
+        fast (Neumann) index (sbp_create_var; p = 2, 4, 6; apply, CG, MG-PCG and the
         right-hand side only)."""
         self.lib = capi.load()
         self.N = int(N)

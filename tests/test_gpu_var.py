@@ -1,12 +1,13 @@
 """GPU parity of the variable-coefficient operator (sbp_create_var; the
 reference's variable_diagonal_sbp_D2 on both axes) against the oracle's
 restatement: apply (both signs, with the dot), right-hand side and its
-transpose, CG, the constant-coefficient limit, and the manufactured-solution
-convergence rates."""
+transpose, CG, the V-cycle and MG-PCG on the rediscretised hierarchy, the
+constant-coefficient limit, and the manufactured-solution convergence rates."""
 import math
 
 import numpy as np
 import pytest
+import scipy.sparse.linalg as spla
 import torch
 
 from oracle import sbp_oracle as O
@@ -111,6 +112,62 @@ def test_var_manufactured_rate_on_gpu(SP, p, rate, levels):
     assert all(abs(r - rate) < 0.25 for r in rates), (errs, rates)
 
 
+def coarse_level(p):
+    return 5 if p == 6 else 4               # p = 6 closures need 24 points: coarsest 33^2
+
+
+@pytest.mark.parametrize("p", [2, 4, 6])
+@pytest.mark.parametrize("level", [6, 7])
+def test_var_vcycle_matches_oracle(SP, level, p):
+    """One V-cycle (2 + 2 damped-Jacobi sweeps, full weighting, bilinear
+    prolongation, dense coarsest solve) on the hierarchy rediscretised with
+    the injected profiles, against the oracle's VarHierarchy."""
+    N = 2 ** level + 1
+    bx, by = profiles(level, 3 * level + p)
+    h = SP(N, p, coef=(bx, by))
+    assert h.nlevels == level - coarse_level(p) + 1
+    hier = O.VarHierarchy(level, p, bx, by, coarse_level=coarse_level(p))
+    r = np.random.default_rng(level + p).standard_normal(N * N)
+    z = h.vcycle(T(r)).cpu().numpy()
+    assert relmax(z, O.vcycle(hier, r)) < 1e-12
+
+
+@pytest.mark.parametrize("p", [2, 4, 6])
+def test_var_mgpcg_against_oracle_and_direct(SP, p):
+    level = 7
+    N = 2 ** level + 1
+    bx, by, f, gW, gE, gS, gN, u = O.var_data(level)
+    hier = O.VarHierarchy(level, p, bx, by, coarse_level=coarse_level(p))
+    b = O.assemble_rhs_var(level, p, bx, by, f, gW, gE, gS, gN)
+    xo, ito, _ = O.mgpcg(hier, b, rtol=1e-10)
+    h = SP(N, p, coef=(bx, by))
+    x, res = h.mgpcg(T(b), abstol=1e-10 * np.linalg.norm(b))
+    assert res.converged and abs(res.iters - ito) <= 1 and res.iters <= 16, (res.iters, ito)
+    xd = spla.spsolve(hier.A[0].tocsc(), b)
+    assert np.linalg.norm(x.cpu().numpy() - xd) < 1e-8 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("p,rate,levels", [(2, 2.0, (9, 10, 11)), (4, 4.2, (6, 7, 8))])
+def test_var_mgpcg_manufactured_rate_large(SP, p, rate, levels):
+    """MG-PCG to round-off (p = 2 up to 2049^2; p = 4 on coarser grids, its
+    error reaching round-off near 513^2), the H-norm error rate of the
+    variable-coefficient discretisation, and a level-independent iteration
+    count."""
+    errs, its = [], []
+    for level in levels:
+        N = 2 ** level + 1
+        bx, by, f, gW, gE, gS, gN, u = O.var_data(level)
+        h = SP(N, p, coef=(bx, by))
+        b = h.assemble_rhs(T(f), T(gW), T(gE), T(gS), T(gN))
+        x, res = h.mgpcg(b, abstol=1e-12 * float(torch.linalg.vector_norm(b)))
+        assert res.converged
+        its.append(res.iters)
+        errs.append(O.h_norm(x.cpu().numpy() - u, O.h_tilde_diag(level, p)))
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(2)]
+    assert all(abs(r - rate) < 0.25 for r in rates), (errs, rates)
+    assert max(its) - min(its) <= 2 and max(its) <= 20, its
+
+
 def test_var_handle_rejects_what_it_does_not_serve(SP):
     from sc_2022_amd import _capi as capi
     N = 65
@@ -120,11 +177,17 @@ def test_var_handle_rejects_what_it_does_not_serve(SP):
         h.diag()
     with pytest.raises(Exception):
         h.apply(T(np.zeros(N * N), torch.float32))
-    x, res = h.mgpcg(T(np.ones(N * N)), abstol=1e-8, raise_on_fail=False)
+    with pytest.raises(Exception):
+        h.vcycle(T(np.zeros(N * N), torch.float32))
+    x, res = h.mgpcg_mixed(T(np.ones(N * N)), abstol=1e-8, raise_on_fail=False)
+    assert res.status == capi.SBP_ERR_INVALID_SIZE
+    hr = SP(60, 4, coef=(np.ones(60), np.ones(60)))   # not 2^k + 1: no hierarchy
+    assert hr.nlevels == 0
+    x, res = hr.mgpcg(T(np.ones(60 * 60)), abstol=1e-8, raise_on_fail=False)
     assert res.status == capi.SBP_ERR_INVALID_SIZE
     with pytest.raises(Exception):
         SP(N, 4, coef=(bx, -by))            # coefficients must be positive
     with pytest.raises(Exception):
         SP(N, 8, coef=(bx, by))             # orders 2, 4 and 6 only
     with pytest.raises(Exception):
-        SP(25, 6, coef=(np.ones(25), np.ones(25)))   # p = 6 closures need N >= 26
+        SP(23, 6, coef=(np.ones(23), np.ones(23)))   # p = 6 closures need N >= 24
