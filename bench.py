@@ -68,6 +68,36 @@ def fourier_problem(h, seed):
     return fourier_rhs(h, seed)
 
 
+def vcycle_bytes(sides, es, fuse_n=1025):
+    """Algorithmic bytes of one 2 + 2 V-cycle from the kernels it launches
+    (distinct arrays read once plus arrays written once, coarse arrays at
+    their own size, es bytes per value): on a level above fuse_n^2 points the
+    separate passes -- first two sweeps (b in, x out: 2 n), residual and
+    restriction (x, b in, coarse out: 2.25 n), prolongation and sweep (x,
+    coarse, b in, x out: 3.25 n), last sweep (x, b in, x out: 3 n); on a fused
+    level smooth2_restrict (2.25 n) and prolong_smooth2 (3.25 n); the coarsest
+    level reads the FP64 inverse (n_c^2 x 8) and b and writes x."""
+    tot = 0
+    for l, N in enumerate(sides[:-1]):
+        n = N * N
+        tot += (5.5 if n <= fuse_n * fuse_n else 10.5) * n * es
+    nc = sides[-1] ** 2
+    return tot + nc * nc * 8 + 2 * nc * es
+
+
+def mgpcg_bytes(sides, es, iters, residual=True):
+    """Algorithmic bytes of an MG-PCG solve of `iters` iterations: per
+    iteration the search-direction pass (z, p, x in; p, Ap, x out: 6 n), the
+    residual update (r, Ap in; r out: 3 n) and a V-cycle; at the start the
+    residual (x, b in, r out: 3 n; not for an inner solve from zero, whose
+    residual is its right-hand side), a V-cycle and the copy p = z (2 n); at
+    the end the pending x update (3 n)."""
+    n = sides[0] ** 2
+    vc = vcycle_bytes(sides, es)
+    start = (3 * n * es if residual else 0) + vc + 2 * n * es
+    return iters * (9 * n * es + vc) + start + 3 * n * es
+
+
 def time_apply(h, torch, dtype, reps, nbuf):
     """Average duration of the stencil apply kernel over reps launches that
     rotate over nbuf input/output pairs (HIP events on the launch stream)."""
@@ -440,6 +470,24 @@ def traffic_from_profiles(dtype_tag):
         return None
 
 
+def solve_rates(h, N, rmx, r64, tmx, t64):
+    """Solve-level algorithmic GB/s: the FP64 solve's bytes (mgpcg_bytes at
+    8 B), and the mixed solve's FP32 inner solves (mgpcg_bytes at 4 B, one
+    start per correction) plus per correction the FP64 residual cast to FP32
+    (x, b in, r out: 20 B a point) and the FP64 update (x, d in, x out: 20 B),
+    and the final residual check (20 B)."""
+    sides = [h.level_size(l) for l in range(h.nlevels)]
+    n = N * N
+    b64 = mgpcg_bytes(sides, 8, r64.iters)
+    outer = rmx.iters
+    inner = rmx.inner_iters or 0
+    b32 = (mgpcg_bytes(sides, 4, 0, residual=False) * outer + inner * (9 * n * 4 + vcycle_bytes(sides, 4))
+           + outer * 40 * n + 20 * n)
+    return {"f64": b64 / t64 / 1e9, "mixed": b32 / tmx / 1e9, "f64_bytes": b64, "mixed_bytes": b32,
+            "f64_bytes_per_iter": 9 * n * 8 + vcycle_bytes(sides, 8),
+            "f32_bytes_per_iter": 9 * n * 4 + vcycle_bytes(sides, 4)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -568,6 +616,7 @@ def main():
                   "ms": tmx * 1e3},
         "f64": {"solves_per_s": 1.0 / t64, "iters": r64.iters, "ms": t64 * 1e3},
         "speedup_mixed_vs_f64": t64 / tmx,
+        "solve_GBs": solve_rates(h, N, rmx, r64, tmx, t64),
         "apply": {"f64_us": apply_ms64 * 1e3, "f64_GBs": gbs64, "f32_us": apply_ms32 * 1e3, "f32_GBs": gbs32,
                   "copy_GBs": copy_gbs, "rotating_buffers": nbuf64},
         "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
