@@ -383,6 +383,53 @@ def variable_coefficient_leg(a, dev, torch, N=4097, p=4):
                       "levels": hv.nlevels}}
 
 
+def adjoint_leg(a, dev, torch, N=2049, p=4):
+    """The differentiation workload: L = 1/2 (u - u_obs)' H~ (u - u_obs) at
+    N^2, u = A^-1 b(f, g) with seeded Fourier-mode f and g, u_obs the solution
+    of another seeded Fourier problem plus N(0, 1e-3) noise.  Times the
+    forward FP64 MG-PCG and the gradient (adjoint MG-PCG and the transposed
+    right-hand-side map), both to 1e-13 relative residual."""
+    from sc_2022_amd.data import fourier_fields
+    from sc_2022_amd.solver import SbpPoisson
+    h = SbpPoisson(N, p, device=dev)
+    f, gW, gE, gS, gN = fourier_fields(N, a.seed, dev)
+    b = h.assemble_rhs(f, gW, gE, gS, gN)
+    bo = h.assemble_rhs(*fourier_fields(N, a.seed + 1, dev))
+    uo, _ = h.mgpcg(bo, abstol=1e-13 * float(torch.linalg.vector_norm(bo)), maxiter=200)
+    g = torch.Generator(device=dev).manual_seed(a.seed)
+    u_obs = uo + 1e-3 * torch.randn(uo.shape, dtype=torch.float64, device=dev, generator=g)
+    bn = float(torch.linalg.vector_norm(b))
+    u = torch.zeros_like(b)
+
+    def forward():
+        u.zero_()
+        return h.mgpcg(b, u, abstol=1e-13 * bn, maxiter=200)[1]
+
+    def gradient():
+        return h.adjoint_grad(u, u_obs)
+
+    reps = 3
+    forward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rf = forward()
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) / reps
+    gradient()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L, lam, df, dW, dE, dS, dN, it = gradient()
+    torch.cuda.synchronize()
+    t_g = (time.perf_counter() - t0) / reps
+    h.close()
+    return {"workload": f"N={N}, p={p}, L = 1/2 |u - u_obs|_H^2, seeded Fourier f, g and u_obs (+ N(0, 1e-3))",
+            "forward_ms": t_f * 1e3, "forward_iters": rf.iters, "gradient_ms": t_g * 1e3, "adjoint_iters": it,
+            "loss": L, "grad_f_norm": float(torch.linalg.vector_norm(df)),
+            "grad_g_norm": float(sum(torch.linalg.vector_norm(v) ** 2 for v in (dW, dE, dS, dN)) ** 0.5)}
+
+
 def tol_sweep(a, h, b, x, bnorm, rel_err, torch):
     """Iterations, relative H-norm error and time of both legs over a grid of
     stopping rules (one JSON line each), to pick the loosest rule per leg
@@ -570,7 +617,8 @@ def main():
                          ("n8193", lambda: leg_8193(a, dev, torch)),
                          ("initial_guess_cg", lambda: nested_leg(a, dev, torch)),
                          ("reference_algorithms", lambda: reference_modes_leg(a, dev, torch)),
-                         ("variable_coefficient", lambda: variable_coefficient_leg(a, dev, torch))):
+                         ("variable_coefficient", lambda: variable_coefficient_leg(a, dev, torch)),
+                         ("adjoint", lambda: adjoint_leg(a, dev, torch))):
             try:
                 extra[name] = fn()
             except Exception as e:  # reported, never silently replaced
