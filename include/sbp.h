@@ -154,6 +154,11 @@ int sbp_rhs_transpose(sbp_handle h, const double* lam, double* df, double* dgW,
                       double* dgE, double* dgS, double* dgN);
 /* *out = sqrt(v' H~ v) (matrix_free_mg_cg.jl:464). */
 int sbp_hnorm(sbp_handle h, const double* v, double* out);
+/* The 1-D norm weights of the finest grid, hw[k] = H_kk / h for k < N (host
+ * array): the diagonal of H from diagonal_sbp_D1/D2 (diagonal_sbp.jl:421-428),
+ * so that the reference's H~ (level_2_multigrid_new.jl:120, returned by
+ * Assembling_matrix) is h^2 hw[i] hw[j] at (i, j).  Host-only, no device work. */
+int sbp_h_weights(sbp_handle h, double* hw);
 /* *out = u'v (FP64). */
 int sbp_dot(sbp_handle h, const double* u, const double* v, double* out);
 

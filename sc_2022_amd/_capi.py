@@ -72,6 +72,7 @@ SIGNATURES = {
     "sbp_assemble_rhs": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sbp_rhs_transpose": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sbp_hnorm": (_i, [_vp, _vp, _pd]),
+    "sbp_h_weights": (_i, [_vp, _pd]),
     "sbp_dot": (_i, [_vp, _vp, _vp, _pd]),
     "sbp_restrict": (_i, [_vp, _i, _vp, _vp, _i]),
     "sbp_prolong_add": (_i, [_vp, _i, _vp, _vp, _i]),

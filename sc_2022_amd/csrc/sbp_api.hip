@@ -1585,6 +1585,17 @@ int sbp_hnorm(sbp_handle h, const double* v, double* out) {
     return SBP_OK;
 }
 
+int sbp_h_weights(sbp_handle h, double* hw) {
+    if (!h || !hw) return SBP_ERR_INVALID_ARG;
+    const OpTab& t = h->L[0].tb;
+    const int N = h->N;
+    for (int k = 0; k < N; ++k) {
+        const int r = k < N - 1 - k ? k : N - 1 - k;
+        hw[k] = r < t.RH ? t.hw[r] : 1.0;
+    }
+    return SBP_OK;
+}
+
 int sbp_dot(sbp_handle h, const double* u, const double* v, double* out) {
     if (!h || !u || !v || !out) return SBP_ERR_INVALID_ARG;
     const int nb = ew_blocks(h->L[0].n);

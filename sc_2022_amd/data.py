@@ -114,6 +114,22 @@ def manufactured_fields(N: int, device=None):
     return (Fm.reshape(-1).contiguous(), gW, gE, gS, gN, U.reshape(-1).contiguous())
 
 
+def reference_fields(N: int, device=None):
+    """(f, gW, gE, gS, gN, u) of the reference's own manufactured solution
+    u = sin(pi (x + y)) (level_2_multigrid_new.jl:36-38, 171-177): f = Laplace(u),
+    Dirichlet data on the west / east faces, Neumann data on the south /
+    north faces, in the flat layout (element (i, j) at i + j N)."""
+    t = grid(N, device)
+    U = torch.sin(math.pi * t[:, None] + math.pi * t[None, :])   # symmetric in (i, j)
+    u = U.reshape(-1).contiguous()
+    f = -2.0 * math.pi ** 2 * u
+    gW = torch.sin(math.pi * t)
+    gE = -torch.sin(math.pi * t)
+    gS = -math.pi * torch.cos(math.pi * t)
+    gN = math.pi * torch.cos(math.pi * t + math.pi)
+    return (f, gW, gE, gS, gN, u)
+
+
 def fourier_rhs(op, seed: int):
     """b of the seeded Fourier-mode problem for an SbpPoisson operator."""
     return op.assemble_rhs(*fourier_fields(op.N, seed, op.device))

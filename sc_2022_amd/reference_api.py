@@ -36,6 +36,21 @@ def _side(t):
     return N
 
 
+def Assembling_matrix(level, p=2, device=None):
+    """(A, b, H_tilde, Nx, Ny, analy_sol) -- level_2_multigrid_new.jl:125-188,
+    assembled on the device: A the SPD operator as device CSR (row pointers,
+    columns, values; int32 / FP64), b its right-hand side for the reference's
+    u = sin(pi (x + y)), H_tilde the diagonal of H~ (N*N values), and the
+    analytic solution.  The reference returns Julia SparseMatrixCSC and dense
+    vectors on the host."""
+    from .data import reference_fields
+    N = 2 ** level + 1
+    h = handle(N, p, device)
+    f, gW, gE, gS, gN, u = reference_fields(N, h.device)
+    b = h.assemble_rhs(f, gW, gE, gS, gN)
+    return h.csr_assemble(), b, h.h_tilde(), N, N, u
+
+
 def matrix_free_A_full_GPU(idata, odata, p=2):
     """odata = -A idata (split_matrix_free.jl:444-486).  The reference exists for
     p = 2 only; p = 4 is the same operator family at fourth order."""

@@ -168,6 +168,19 @@ class SbpPoisson:
                    "sbp_rhs_transpose")
         return df, gW, gE, gS, gN
 
+    def h_weights(self):
+        """1-D norm weights hw (N values, host numpy): H~ = h^2 hw_i hw_j."""
+        import numpy as np
+        hw = np.empty(self.N)
+        capi.check(self.lib.sbp_h_weights(self._h, hw.ctypes.data_as(C.POINTER(C.c_double))), "sbp_h_weights")
+        return hw
+
+    def h_tilde(self):
+        """The diagonal of H~ (device, N*N values, element (i, j) at i + j N)."""
+        hw = torch.as_tensor(self.h_weights(), dtype=torch.float64, device=self.device)
+        h = 1.0 / (self.N - 1)
+        return (h * h * hw[None, :] * hw[:, None]).reshape(-1).contiguous()
+
     def hnorm(self, v):
         self._chk(v, dtype=torch.float64, name="v")
         d = C.c_double()

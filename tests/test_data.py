@@ -38,6 +38,16 @@ def test_manufactured_fields_match_oracle():
         np.testing.assert_allclose(g.numpy(), r, rtol=0, atol=1e-13)
 
 
+def test_reference_fields_match_oracle():
+    """u = sin(pi (x + y)) and its data (level_2_multigrid_new.jl:36-38,
+    171-177) against the oracle's reference_data."""
+    level = 4
+    got = D.reference_fields(2 ** level + 1, device="cpu")
+    ref = O.reference_data(level)
+    for g, r in zip(got, ref):
+        np.testing.assert_allclose(g.numpy(), r, rtol=0, atol=1e-13)
+
+
 def test_batch_specs():
     s1 = D.batch_specs(2022)
     s2 = D.batch_specs(2022)

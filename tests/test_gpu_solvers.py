@@ -19,6 +19,11 @@ def T(a, dtype=torch.float64):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
 
 
+def relmax(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
 @pytest.fixture(scope="module")
 def SP():
     from sc_2022_amd.solver import SbpPoisson
@@ -117,6 +122,33 @@ def test_csr_cg(SP, p):
     x, res = h.csr_cg(csr, T(b), abstol=tol)
     assert res.converged and abs(res.iters - ito) <= 1
     assert np.linalg.norm(x.cpu().numpy() - xo) < 1e-6 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_assembling_matrix_on_device(p):
+    """Assembling_matrix (level_2_multigrid_new.jl:125-188) assembled on the
+    device: CSR A, b for u = sin(pi (x + y)), H~ and u against the oracle's
+    restatement."""
+    import scipy.sparse as sps
+    from sc_2022_amd import reference_api as R
+    level = 5
+    (rp, col, val), b, Ht, Nx, Ny, u = R.Assembling_matrix(level, p)
+    A0, b0, H0, Nx0, Ny0, u0 = O.assembling_matrix(level, p)
+    n = Nx * Ny
+    A = sps.csr_matrix((val.cpu().numpy(), col.cpu().numpy(), rp.cpu().numpy()), shape=(n, n))
+    assert (Nx, Ny) == (Nx0, Ny0)
+    assert abs(A - A0).max() < 1e-13 * abs(A0).max()
+    assert relmax(b.cpu().numpy(), b0) < 1e-13
+    assert relmax(Ht.cpu().numpy(), H0.diagonal()) < 1e-15
+    assert relmax(u.cpu().numpy(), u0) < 1e-15
+
+
+@pytest.mark.parametrize("p", [2, 4, 6, 8, 10])
+def test_h_weights_match_oracle(SP, p):
+    N = 41
+    hw = SP(N, p).h_weights()
+    _, _, H, _ = O.diagonal_sbp_D1(p, N - 1, xc=(0, 1))
+    assert np.abs(hw - H.diagonal() * (N - 1)).max() < 1e-14
 
 
 def test_reference_api_mirror():
