@@ -100,15 +100,14 @@ struct sbp_handle_s {
     VarTab* vtab = nullptr;
     double* vband = nullptr;
     // Its multigrid hierarchy (N = 2^k + 1): per level the device table and
-    // bands (level 0: vtab / vband), 1 / diag(A), and the level's right-hand
-    // side b, iterate x and sweep buffer t (level 0: b is the caller's, x and
-    // t are z / pv2).  The coarsest level is solved with Ainv (nc x nc).
+    // bands (level 0: vtab / vband) and the level's right-hand side b,
+    // iterate x and sweep buffer t (level 0: b is the caller's, x and t are
+    // z / pv2).  The coarsest level is solved with Ainv (nc x nc).
     struct VarLevel {
         int N = 0;
         size_t n = 0;
         VarTab* dtab = nullptr;
         double* band = nullptr;
-        double* dinv = nullptr;
         double *b = nullptr, *x = nullptr, *t = nullptr;
     };
     std::vector<VarLevel> vl;
@@ -1015,7 +1014,6 @@ int sbp_destroy(sbp_handle h) {
     if (h->vband) (void)hipFree(h->vband);
     for (size_t l = 0; l < h->vl.size(); ++l) {
         auto& V = h->vl[l];
-        if (V.dinv) (void)hipFree(V.dinv);
         if (l > 0) {
             void* own[] = {V.dtab, V.band, V.b, V.x, V.t};
             for (void* q : own)
@@ -1236,9 +1234,6 @@ int var_hierarchy(sbp_handle h, const VarTab& t0, const double* bx, const double
             HIPCHK(hipMalloc(&V.x, V.n * sizeof(double)));
             HIPCHK(hipMalloc(&V.t, V.n * sizeof(double)));
         }
-        HIPCHK(hipMalloc(&V.dinv, V.n * sizeof(double)));
-        hipLaunchKernelGGL(var_diag, dim3((V.N + 255) / 256, V.N), dim3(256), 0, h->st, V.dinv, (const VarTab*)V.dtab);
-        HIPCHK(hipGetLastError());
         if (l == nlev - 1) {
             if (l == 0) return SBP_OK;   // N is the coarse side: nothing to precondition
             VarTab th = t;                // host bands for the dense build
@@ -1341,13 +1336,11 @@ int launch_var(sbp_handle h, int mode, const double* u, double* out, const doubl
                const Scalars* sc, int* nparts, int level = 0, double omega = 0.0) {
     const int N = level == 0 ? h->N : h->vl[level].N;
     const VarTab* tab = level == 0 ? h->vtab : h->vl[level].dtab;
-    const double* dinv = level < (int)h->vl.size() ? h->vl[level].dinv : nullptr;
-    if (mode == 3 && !dinv) return SBP_ERR_INVALID_ARG;
     const dim3 g((N + kVarThreads - 1) / kVarThreads, (N + kVarRows - 1) / kVarRows);
     if ((size_t)g.x * g.y > h->npp) return SBP_ERR_INVALID_SIZE;
 #define SBP_VAR_LAUNCH(P_, M_) \
     hipLaunchKernelGGL((var_apply<P_, M_>), g, dim3(kVarThreads), 0, h->st, u, out, aux, sign, h->partial, tab, sc, \
-                       dinv, omega)
+                       omega)
 #define SBP_VAR_MODES(P_)                       \
     switch (mode) {                             \
     case 0: SBP_VAR_LAUNCH(P_, 0); break;       \
@@ -1425,13 +1418,13 @@ int var_vcycle(sbp_handle h, int l, const double* b, double* x, const Scalars* s
     }
     const int nu1 = h->prm.nu_pre, nu2 = h->prm.nu_post;
     const double om = h->prm.omega;
-    const dim3 eg(ew_blocks(V.n));
     double* bufs[2] = {x, V.t};
     // sweeps after the first: (nu1 - 1) + nu2 (or nu2 - 1 after a zero start)
     const int later = nu1 > 0 ? (nu1 - 1) + nu2 : nu2;
     int cur = later % 2 == 0 ? 0 : 1;   // the buffer the first sweep writes
     if (nu1 > 0) {
-        hipLaunchKernelGGL(var_jacobi0, eg, dim3(256), 0, st, bufs[cur], b, (const double*)V.dinv, om, V.n, sc);
+        hipLaunchKernelGGL(var_jacobi0, dim3((V.N + 255) / 256, V.N), dim3(256), 0, st, bufs[cur], b,
+                           (const VarTab*)V.dtab, om, sc);
     } else {
         hipLaunchKernelGGL((fill<double, kEwThreads>), dim3(ew_blocks(V.n), 1), dim3(kEwThreads), 0, st, bufs[cur], 0.0,
                            V.n, sc);

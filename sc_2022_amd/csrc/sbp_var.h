@@ -267,16 +267,23 @@ __device__ __forceinline__ double var_row(const VarAxis& a, int R, int C, int N,
 }
 
 // MODE 0: out = sign A u;  1: out = A u, partial += aux' out;  2: out = aux - A u, partial += out' out;
-// 3: damped-Jacobi sweep out = u + omega dinv (aux - A u) (no partial).
+// 3: damped-Jacobi sweep out = u + omega (aux - A u) / diag(A) (no partial; dg the diagonal).
 // A block: kVarThreads threads along i, kVarRows consecutive rows j.
 constexpr int kVarThreads = 256;
 constexpr int kVarRows = 8;
 
+// Diagonal entry of one axis' factor at row r.
+__device__ __forceinline__ double var_diag1(const VarAxis& a, int R, int H, int N, int r) {
+    if (r < R) return a.lo[r][r];
+    if (r >= N - R) return a.hi[N - 1 - r][N - 1 - r];
+    return a.band[(size_t)H * N + r];
+}
+
 template <int MODE>
 __device__ __forceinline__ void var_emit(double* out, const double* aux, double sign, size_t q, double Au,
-                                         double& acc, double uc, const double* dinv, double omega) {
+                                         double& acc, double uc, double dg, double omega) {
     if constexpr (MODE == 3) {
-        out[q] = uc + omega * dinv[q] * (aux[q] - Au);
+        out[q] = uc + omega * (aux[q] - Au) / dg;
     } else if constexpr (MODE == 0) {
         out[q] = sign * Au;
     } else if constexpr (MODE == 1) {
@@ -299,8 +306,8 @@ __device__ __forceinline__ void var_emit(double* out, const double* aux, double 
 // (direct loads, table t staged in LDS) and their H weight.
 template <int P, int MODE, bool ICHK>
 __device__ __forceinline__ double var_tile(const double* __restrict__ u, double* __restrict__ out,
-                                           const double* __restrict__ aux, double sign, const double* __restrict__ dinv,
-                                           double omega, const VarTab* __restrict__ tp,
+                                           const double* __restrict__ aux, double sign, double omega,
+                                           const VarTab* __restrict__ tp,
                                            const VarTab& t, int i0, int j0,
                                            double (*tile)[kVarThreads + 2 * (P / 2)]) {
     constexpr int H = P / 2;
@@ -334,6 +341,7 @@ __device__ __forceinline__ double var_tile(const double* __restrict__ u, double*
         const int ri = i < N - 1 - i ? i : N - 1 - i;
         hwi = ri < t.RH ? t.hw[ri] : 1.0;
     }
+    const double mdi = iint ? cm[H] : var_diag1(t.m, t.R, H, N, i);   // m_ii (sweep mode only)
 #pragma unroll
     for (int r = 0; r < J; ++r) {
         const int j = j0 + r;
@@ -346,7 +354,8 @@ __device__ __forceinline__ double var_tile(const double* __restrict__ u, double*
         } else {
             mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
         }
-        var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + mp, acc, w[r + H], dinv, omega);
+        var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + mp, acc, w[r + H],
+                       hwi * kb[(size_t)H * N + j] + mdi, omega);
     }
     return acc;
 }
@@ -355,7 +364,7 @@ template <int P, int MODE>
 __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restrict__ u, double* __restrict__ out,
                                                         const double* __restrict__ aux, double sign, double* partial,
                                                         const VarTab* __restrict__ tp, const Scalars* sc,
-                                                        const double* __restrict__ dinv, double omega) {
+                                                        double omega) {
     constexpr int H = P / 2;
     constexpr int J = kVarRows;
     if (sc && sc->done) return;   // the whole launch: a converged solve
@@ -368,7 +377,7 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
     const bool iin = i0 - H >= R && i0 + kVarThreads + H <= N - R;
     double acc = 0.0;
     if (jin && iin) {
-        acc = var_tile<P, MODE, false>(u, out, aux, sign, dinv, omega, tp, t, i0, j0, tile);
+        acc = var_tile<P, MODE, false>(u, out, aux, sign, omega, tp, t, i0, j0, tile);
     } else {
         {
             const unsigned long long* src = reinterpret_cast<const unsigned long long*>(tp);
@@ -377,7 +386,7 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
         }
         __syncthreads();
         if (jin) {
-            acc = var_tile<P, MODE, true>(u, out, aux, sign, dinv, omega, tp, t, i0, j0, tile);
+            acc = var_tile<P, MODE, true>(u, out, aux, sign, omega, tp, t, i0, j0, tile);
         } else {
             const int i = i0 + tid;   // face rows: per point
             if (i < N) {
@@ -390,8 +399,9 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
                     const double hwj = rj < t.RH ? t.hw[rj] : 1.0;
                     const double kp = var_row<H>(t.k, t.R, t.C, N, j, [&](int c) { return u[gidx(i, c, N)]; });
                     const double mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
-                    var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + hwj * mp, acc, u[gidx(i, j, N)],
-                                   dinv, omega);
+                    const double dg = hwi * var_diag1(t.k, t.R, H, N, j) + hwj * var_diag1(t.m, t.R, H, N, i);
+                    var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + hwj * mp, acc, u[gidx(i, j, N)], dg,
+                                   omega);
                 }
             }
         }
@@ -403,15 +413,11 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
     }
 }
 
-// Diagonal entry of one axis' factor at row r.
-__device__ __forceinline__ double var_diag1(const VarAxis& a, int R, int H, int N, int r) {
-    if (r < R) return a.lo[r][r];
-    if (r >= N - R) return a.hi[N - 1 - r][N - 1 - r];
-    return a.band[(size_t)H * N + r];
-}
-
-// dinv = 1 / diag(A): diag at (i, j) is hw_i k_jj + hw_j m_ii.
-__global__ __launch_bounds__(256) void var_diag(double* __restrict__ dinv, const VarTab* __restrict__ tp) {
+// First damped-Jacobi sweep from zero: x = omega b / diag(A), the diagonal
+// hw_i k_jj + hw_j m_ii formed from the tables (blockIdx.y = row j).
+__global__ __launch_bounds__(256) void var_jacobi0(double* __restrict__ x, const double* __restrict__ b,
+                                                  const VarTab* __restrict__ tp, double omega, const Scalars* sc) {
+    if (sc && sc->done) return;
     const VarTab& t = *tp;
     const int N = t.N;
     const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
@@ -419,16 +425,8 @@ __global__ __launch_bounds__(256) void var_diag(double* __restrict__ dinv, const
     const int ri = i < N - 1 - i ? i : N - 1 - i, rj = j < N - 1 - j ? j : N - 1 - j;
     const double hwi = ri < t.RH ? t.hw[ri] : 1.0, hwj = rj < t.RH ? t.hw[rj] : 1.0;
     const double d = hwi * var_diag1(t.k, t.R, t.H, N, j) + hwj * var_diag1(t.m, t.R, t.H, N, i);
-    dinv[gidx(i, j, N)] = 1.0 / d;
-}
-
-// First damped-Jacobi sweep from zero: x = omega dinv b.
-__global__ __launch_bounds__(256) void var_jacobi0(double* __restrict__ x, const double* __restrict__ b,
-                                                  const double* __restrict__ dinv, double omega, size_t n,
-                                                  const Scalars* sc) {
-    if (sc && sc->done) return;
-    const size_t stride = (size_t)gridDim.x * 256;
-    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += stride) x[k] = omega * dinv[k] * b[k];
+    const size_t q = gidx(i, j, N);
+    x[q] = omega * b[q] / d;
 }
 
 // Dense A (row-major, n = N*N) of the variable-coefficient operator from
