@@ -1346,7 +1346,8 @@ int launch_var(sbp_handle h, int mode, const double* u, double* out, const doubl
     case 0: SBP_VAR_LAUNCH(P_, 0); break;       \
     case 1: SBP_VAR_LAUNCH(P_, 1); break;       \
     case 2: SBP_VAR_LAUNCH(P_, 2); break;       \
-    default: SBP_VAR_LAUNCH(P_, 3); break;      \
+    case 3: SBP_VAR_LAUNCH(P_, 3); break;       \
+    default: SBP_VAR_LAUNCH(P_, 4); break;      \
     }
     if (h->p == 2) {
         SBP_VAR_MODES(2)
@@ -1406,7 +1407,7 @@ int var_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter,
 // the coarser cycle, bilinear prolongation, nu_post sweeps; the coarsest
 // level is the dense inverse.  Sweeps ping-pong between x and t, starting so
 // that the last one lands in x; the residual goes to whichever is free.
-int var_vcycle(sbp_handle h, int l, const double* b, double* x, const Scalars* sc) {
+int var_vcycle(sbp_handle h, int l, const double* b, double* x, const Scalars* sc, int* np_rz = nullptr) {
     auto& V = h->vl[l];
     const int nlev = (int)h->vl.size();
     hipStream_t st = h->st;
@@ -1444,8 +1445,11 @@ int var_vcycle(sbp_handle h, int l, const double* b, double* x, const Scalars* s
     hipLaunchKernelGGL((prolong_add<double>), dim3((V.N + 255) / 256, V.N, 1), dim3(256), 0, st, (const double*)C.x,
                        bufs[cur], V.N, C.N, sc);
     HIPCHK(hipGetLastError());
+    // with np_rz the last sweep also leaves the block partials of b'x (its
+    // result lands in x by the parity choice above)
     for (int s = 0; s < nu2; ++s) {
-        STCHK(launch_var(h, 3, bufs[cur], bufs[1 - cur], b, 1.0, sc, nullptr, l, om));
+        const bool last = s == nu2 - 1 && np_rz;
+        STCHK(launch_var(h, last ? 4 : 3, bufs[cur], bufs[1 - cur], b, 1.0, sc, last ? np_rz : nullptr, l, om));
         cur = 1 - cur;
     }
     if (bufs[cur] != x) {
@@ -1471,15 +1475,20 @@ int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter
                        hist, maxiter);
     HIPCHK(hipGetLastError());
     const dim3 g(ew_blocks(n), 1);
-    auto precondition = [&]() -> int {   // z = M r ; rz partials
-        STCHK(var_vcycle(h, 0, h->r, z, sc));
-        hipLaunchKernelGGL((dot_partial<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->r,
-                           (const double*)z, n, h->partial);
-        HIPCHK(hipGetLastError());
+    int nrz = 0;
+    auto precondition = [&]() -> int {   // z = M r ; r'z partials (nrz of them)
+        nrz = 0;
+        STCHK(var_vcycle(h, 0, h->r, z, sc, h->prm.nu_post > 0 ? &nrz : nullptr));
+        if (nrz == 0) {
+            hipLaunchKernelGGL((dot_partial<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->r,
+                               (const double*)z, n, h->partial);
+            HIPCHK(hipGetLastError());
+            nrz = (int)g.x;
+        }
         return SBP_OK;
     };
     STCHK(precondition());
-    hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc);
+    hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(kRedThreads), 0, st, h->partial, nrz, sc);
     hipLaunchKernelGGL((copy_vec<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)z, h->pv, n,
                        (const Scalars*)sc);
     HIPCHK(hipGetLastError());
@@ -1493,7 +1502,7 @@ int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter
         hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 0);
         HIPCHK(hipGetLastError());
         STCHK(precondition());
-        hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc);
+        hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(kRedThreads), 0, st, h->partial, nrz, sc);
         hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)z,
                            (const Scalars*)sc, n);
         HIPCHK(hipGetLastError());

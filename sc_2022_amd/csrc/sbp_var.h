@@ -267,7 +267,8 @@ __device__ __forceinline__ double var_row(const VarAxis& a, int R, int C, int N,
 }
 
 // MODE 0: out = sign A u;  1: out = A u, partial += aux' out;  2: out = aux - A u, partial += out' out;
-// 3: damped-Jacobi sweep out = u + omega (aux - A u) / diag(A) (no partial; dg the diagonal).
+// 3: damped-Jacobi sweep out = u + omega (aux - A u) / diag(A) (no partial; dg the diagonal);
+// 4: the same sweep with partial += aux' out (r'z after a V-cycle's last sweep).
 // A block: kVarThreads threads along i, kVarRows consecutive rows j.
 constexpr int kVarThreads = 256;
 constexpr int kVarRows = 8;
@@ -284,6 +285,11 @@ __device__ __forceinline__ void var_emit(double* out, const double* aux, double 
                                          double& acc, double uc, double dg, double omega) {
     if constexpr (MODE == 3) {
         out[q] = uc + omega * (aux[q] - Au) / dg;
+    } else if constexpr (MODE == 4) {
+        const double a = aux[q];
+        const double o = uc + omega * (a - Au) / dg;
+        out[q] = o;
+        acc += a * o;
     } else if constexpr (MODE == 0) {
         out[q] = sign * Au;
     } else if constexpr (MODE == 1) {
@@ -406,7 +412,7 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
             }
         }
     }
-    if constexpr (MODE == 1 || MODE == 2) {
+    if constexpr (MODE == 1 || MODE == 2 || MODE == 4) {
         __syncthreads();
         const double s = block_sum<double, kVarThreads>(acc);
         if (threadIdx.x == 0) partial[blockIdx.x + (size_t)blockIdx.y * gridDim.x] = s;
