@@ -95,6 +95,32 @@ def test_apply_ragged_sizes_against_closed_form(SP, N, p):
     assert relmax(h.apply(T(u)).cpu().numpy(), A @ u) < 1e-13
 
 
+def test_apply_seeded_random_cases(SP):
+    """30 seeded random cases -- order 2..10, side 2C..400 (mostly not
+    2^k + 1), a compiled tile shape for the order, sign +-1, FP64 and FP32 --
+    against the closed form kron(k, hw) + kron(hw, m)."""
+    from sc_2022_amd.autotune import compiled_shapes
+    rng = np.random.default_rng(20261203)
+    cmin = {2: 8, 4: 12, 6: 18, 8: 24, 10: 32}
+    for case in range(30):
+        p = int(rng.choice([2, 4, 6, 8, 10]))
+        N = int(rng.integers(cmin[p], 401))
+        shapes = compiled_shapes(p)
+        ti, tj = shapes[int(rng.integers(len(shapes)))]
+        sign = int(rng.choice([1, -1]))
+        hw, m, k = O.one_d_factors(p, N)
+        A = sp.kron(sp.csr_matrix(k), sp.diags(hw)) + sp.kron(sp.diags(hw), sp.csr_matrix(m))
+        h = SP(N, p, tile_i=ti, tile_j=tj)
+        u = rng.standard_normal(N * N)
+        ref = sign * (A @ u)
+        e64 = relmax(h.apply(T(u), sign=sign).cpu().numpy(), ref)
+        uf = u.astype(np.float32)
+        e32 = relmax(h.apply(T(uf, torch.float32), sign=sign).cpu().double().numpy(),
+                     sign * (A @ uf.astype(np.float64)))
+        assert e64 < 1e-13 and e32 < 2e-5, (case, p, N, ti, tj, sign, e64, e32)
+        h.close()
+
+
 @pytest.mark.parametrize("p", [2, 4])
 def test_apply_f32(SP, p):
     level = 6
