@@ -246,7 +246,9 @@ int smooth_pass(sbp_handle h, int l, const IN& in, const T* b, T* dst, const T* 
 // levels; above that the redundant halo work of the fused blocks costs more
 // than the passes it saves).
 inline bool fuse_level(const sbp_handle h, int l) {
-    if (h->p > 4) return false;   // the two-sweep kernels are compiled for p = 2, 4
+    if (h->p > 6) return false;   // the two-sweep kernels are compiled for p = 2, 4 (and 6, 64-wide blocks)
+    if (!fused_shape_ok(h->L[l].fti[0], h->L[l].ftj[0], h->p) || !fused_shape_ok(h->L[l].fti[1], h->L[l].ftj[1], h->p))
+        return false;
     const long long pts = (long long)h->L[l].n * h->nprob;
     return pts <= (long long)h->prm.fuse_n * h->prm.fuse_n;
 }
@@ -1318,7 +1320,7 @@ int sbp_set_tiles(sbp_handle h, int level, int dtype, int ti, int tj) {
     if (!h || level < 0 || level >= h->nlev) return SBP_ERR_INVALID_ARG;
     const bool fused = (dtype & SBP_FUSED) != 0;
     const int d = dtype & ~SBP_FUSED;
-    if ((d != SBP_F64 && d != SBP_F32) || (fused ? !fused_shape_ok(ti, tj) : !tile_ok(h->p, ti, tj)))
+    if ((d != SBP_F64 && d != SBP_F32) || (fused ? !fused_shape_ok(ti, tj, h->p) : !tile_ok(h->p, ti, tj)))
         return SBP_ERR_INVALID_ARG;
     Level& L = h->L[level];
     int* pti = fused ? &L.fti[d] : &L.ti[d];

@@ -80,7 +80,7 @@ __device__ __forceinline__ T au_grid(const OpTab& t, const T* a, int i0, int j0,
 // around the output region C and the operand's region A another R-1 (both
 // >= the stencil radius H, which interior points need).
 template <int P> struct FusedHalo {
-    static constexpr int R = P == 2 ? 3 : 4;
+    static constexpr int R = Stencil<P>::R;
     static constexpr int HB = R - 1;
     static constexpr int HA = 2 * (R - 1);
 };

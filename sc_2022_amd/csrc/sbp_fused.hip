@@ -10,6 +10,10 @@ namespace sbp {
 namespace {
 constexpr int kFusedNT = 256;   // threads per block
 
+// p = 6 is compiled for the 64-wide blocks only: its wider halos (5 and 10
+// points) put the wider blocks' tiles above the 64 KiB workgroup LDS limit.
+template <int P, int TI> constexpr bool fused_compiled() { return P <= 4 || (P == 6 && TI == 64); }
+
 template <int P, int TI, int J, typename T>
 void sr_launch(const OpTab& tf, const T* b, T* x2, T* rc, T omega, int use_diag, T scale, FusedCtl c,
                hipStream_t st) {
@@ -30,8 +34,9 @@ void ps_launch(const OpTab& tf, const T* x, const T* ec, const T* b, T* out, con
 
 int fused_post_blocks(int N, int ti, int tj) { return ((N + ti - 1) / ti) * ((N + tj - 1) / tj); }
 
-bool fused_shape_ok(int ti, int tj) {
-#define SBP_FUSED_OK(TI_, J_) if (ti == TI_ && tj == J_) return true;
+bool fused_shape_ok(int ti, int tj, int p) {
+#define SBP_FUSED_OK(TI_, J_) \
+    if (ti == TI_ && tj == J_) return p <= 4 || (p == 6 && fused_compiled<6, TI_>());
     SBP_FUSED_SHAPES(SBP_FUSED_OK)
 #undef SBP_FUSED_OK
     return false;
@@ -52,7 +57,11 @@ int launch_smooth2_restrict(int p, int ti, int tj, const OpTab& tf, const T* b, 
     if (ti == TI_ && tj == J_) {                                                              \
         if (p == 2) sr_launch<2, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);  \
         else if (p == 4) sr_launch<4, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st); \
-        else return 3;                                                                        \
+        else if (p == 6) {                                                                    \
+            if constexpr (fused_compiled<6, TI_>())                                           \
+                sr_launch<6, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);       \
+            else return 3;                                                                    \
+        } else return 3;                                                                      \
         return hipGetLastError() == hipSuccess ? 0 : 4;                                       \
     }
     SBP_FUSED_SHAPES(SBP_SR_CASE)
@@ -75,7 +84,11 @@ int launch_prolong_smooth2(int p, int ti, int tj, const OpTab& tf, const T* x, c
     if (ti == TI_ && tj == J_) {                                                               \
         if (p == 2) ps_launch<2, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st); \
         else if (p == 4) ps_launch<4, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st); \
-        else return 3;                                                                         \
+        else if (p == 6) {                                                                     \
+            if constexpr (fused_compiled<6, TI_>())                                            \
+                ps_launch<6, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st);     \
+            else return 3;                                                                     \
+        } else return 3;                                                                       \
         return hipGetLastError() == hipSuccess ? 0 : 4;                                        \
     }
     SBP_FUSED_SHAPES(SBP_PS_CASE)

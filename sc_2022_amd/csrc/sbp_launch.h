@@ -60,7 +60,7 @@ int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, PassCtl c,
 // Temporally blocked V-cycle passes (two sweeps per kernel); block shape ti
 // output columns x tj fine rows, one of the compiled shapes (fused_shape_ok).
 int fused_post_blocks(int N, int ti, int tj);
-bool fused_shape_ok(int ti, int tj);
+bool fused_shape_ok(int ti, int tj, int p);
 // x2 = S(S(0)) on the fine level and rc = scale * FW(b - A x2).
 template <typename T>
 int launch_smooth2_restrict(int p, int ti, int tj, const OpTab& tf, const T* b, T* x2, T* rc, T omega,

@@ -72,6 +72,10 @@ def test_uncompiled_tile_shape_rejected(SP):
     h = SP(129, 8)
     with pytest.raises(Exception):
         h.set_tiles(0, 256, 16)
+    h6 = SP(129, 6)
+    with pytest.raises(Exception):
+        h6.set_tiles(0, 128, 8, fused=True)   # p = 6 fused blocks: 64 wide only
+    h6.set_tiles(0, 64, 8, fused=True)
 
 
 @pytest.mark.parametrize("p", [6, 8, 10])

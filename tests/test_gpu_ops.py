@@ -306,9 +306,9 @@ def test_vcycle_large_against_oracle(SP, p, bottom):
     assert relmax(z32, O.vcycle(hier, r)) < 1e-4
 
 
-@pytest.mark.parametrize("shape", [(64, 4), (64, 8), (128, 8), (256, 8)])
-@pytest.mark.parametrize("level", [5, 7, 10])
-@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("p,level,shape", [(p, lv, sh) for p in (2, 4) for lv in (5, 7, 10)
+                                            for sh in ((64, 4), (64, 8), (128, 8), (256, 8))]
+                         + [(6, lv, sh) for lv in (7, 10) for sh in ((64, 4), (64, 8))])
 def test_vcycle_fused_passes_match_separate_passes(SP, p, level, shape):
     """The temporally blocked two-sweep kernels (pre-smoothing + residual +
     restriction; prolongation + post-smoothing) against the pass-per-sweep
@@ -326,7 +326,7 @@ def test_vcycle_fused_passes_match_separate_passes(SP, p, level, shape):
     zs = hs.vcycle(T(r)).cpu().numpy()
     assert relmax(zf, zs) < 1e-13
     if level <= 7:
-        assert relmax(zf, O.vcycle(O.Hierarchy(level, p), r)) < 1e-12
+        assert relmax(zf, O.vcycle(O.Hierarchy(level, p, coarse_level=5 if p == 6 else 4), r)) < 1e-12
     zf32 = hf.vcycle(T(r, torch.float32)).cpu().double().numpy()
     zs32 = hs.vcycle(T(r, torch.float32)).cpu().double().numpy()
     assert relmax(zf32, zs32) < 1e-5
