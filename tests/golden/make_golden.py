@@ -25,7 +25,7 @@ def make(level, p):
     rng = np.random.default_rng(1000 * level + p)
     A = O.assemble_operator(level, p)
     u = rng.standard_normal(N * N)
-    f, gW, gE, gS, gN, uex = O.baseline_data(level)
+    f, gW, gE, gS, gN, uex = O.manufactured_data(level)
     b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
     x = spla.spsolve(A.tocsc(), b)
     # Fourier-mode data (seed 7)

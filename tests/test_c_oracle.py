@@ -53,7 +53,7 @@ def test_cg_and_mgpcg(p):
     level = 6
     N = 2 ** level + 1
     A = O.assemble_operator(level, p)
-    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    f, gW, gE, gS, gN, u = O.manufactured_data(level)
     b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
     tol = 1e-8 * np.linalg.norm(b)
     xo, ito, _ = O.cg_cpu(A, b, abstol=tol)

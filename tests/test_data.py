@@ -33,7 +33,7 @@ def test_fourier_fields_match_oracle(level):
 def test_manufactured_fields_match_oracle():
     level = 4
     got = D.manufactured_fields(2 ** level + 1, device="cpu")
-    ref = O.baseline_data(level)
+    ref = O.manufactured_data(level)
     for g, r in zip(got, ref):
         np.testing.assert_allclose(g.numpy(), r, rtol=0, atol=1e-13)
 

@@ -112,7 +112,7 @@ def test_apply_1025_properties(SP, p):
 def test_rhs_matches_oracle(SP, p):
     level = 6
     N = 2 ** level + 1
-    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    f, gW, gE, gS, gN, u = O.manufactured_data(level)
     h = SP(N, p)
     b = h.assemble_rhs(T(f), T(gW), T(gE), T(gS), T(gN)).cpu().numpy()
     assert relmax(b, O.assemble_rhs(level, p, f, gW, gE, gS, gN)) < 1e-13
@@ -135,7 +135,7 @@ def test_vcycle_p6_matches_oracle(SP):
 def test_cg_p6_against_oracle(SP, N):
     level = int(math.log2(N - 1))
     A = A_of(level, 6)
-    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    f, gW, gE, gS, gN, u = O.manufactured_data(level)
     b = O.assemble_rhs(level, 6, f, gW, gE, gS, gN)
     tol = 1e-8 * np.linalg.norm(b)
     xo, ito, _ = O.cg_cpu(A, b, abstol=tol)
@@ -154,7 +154,7 @@ def test_mgpcg_p6_against_direct_and_rate(SP):
     for level in (5, 6, 7):
         N = 2 ** level + 1
         A = A_of(level, 6)
-        f, gW, gE, gS, gN, u = O.baseline_data(level)
+        f, gW, gE, gS, gN, u = O.manufactured_data(level)
         b = O.assemble_rhs(level, 6, f, gW, gE, gS, gN)
         h = SP(N, 6)
         x, res = h.mgpcg(T(b), abstol=1e-13 * np.linalg.norm(b))
@@ -179,7 +179,7 @@ def test_indefinite_orders_report_a_defined_status(SP, p):
     level = 6
     N = 2 ** level + 1
     A = A_of(level, p)
-    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    f, gW, gE, gS, gN, u = O.manufactured_data(level)
     b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
     h = SP(N, p)
     ok = (capi.SBP_OK, capi.SBP_ERR_BREAKDOWN, capi.SBP_ERR_NOT_CONVERGED, capi.SBP_ERR_NONFINITE)

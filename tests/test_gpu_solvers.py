@@ -33,7 +33,7 @@ def SP():
 def problem(level, p, data="baseline"):
     A = O.assemble_operator(level, p)
     if data == "baseline":
-        f, gW, gE, gS, gN, u = O.baseline_data(level)
+        f, gW, gE, gS, gN, u = O.manufactured_data(level)
     else:
         f, gW, gE, gS, gN = O.fourier_data(level, 7)
         u = None
@@ -209,7 +209,7 @@ def test_cg_interp_against_oracle(SP, p, three):
     problem against the oracle's restatement: per-grid CG iteration counts,
     the oracle-recomputed residual, the solution."""
     level = 7
-    A, rhs, u = nested_problem(level, p, O.baseline_data)
+    A, rhs, u = nested_problem(level, p, O.manufactured_data)
     tol = 1e-9 * np.linalg.norm(rhs[0])
     kw = dict(A4h=A[2], b4h=rhs[2]) if three else {}
     xo, ito, nro, x0o = O.initial_guess_interpolation_cg(A[0], rhs[0], A[1], rhs[1], tol, **kw)
@@ -233,7 +233,7 @@ def test_cg_interp_mirror_and_errors(SP):
     from sc_2022_amd import _capi as capi
     from sc_2022_amd import reference_api as R
     level, p = 6, 2
-    A, rhs, u = nested_problem(level, p, O.baseline_data)
+    A, rhs, u = nested_problem(level, p, O.manufactured_data)
     N = 2 ** level + 1
     tol = 1e-9 * np.linalg.norm(rhs[0])
     x = T(np.zeros(N * N))
@@ -246,8 +246,8 @@ def test_cg_interp_mirror_and_errors(SP):
     assert np.linalg.norm(A[0] @ x.cpu().numpy() - rhs[0]) < 1.0001 * tol
     # three levels need (N+3)/4 >= 17: not on 33^2
     h33 = SP(33, p)
-    b33 = T(O.assemble_rhs(5, p, *O.baseline_data(5)[:5]))
-    b17 = T(O.assemble_rhs(4, p, *O.baseline_data(4)[:5]))
+    b33 = T(O.assemble_rhs(5, p, *O.manufactured_data(5)[:5]))
+    b17 = T(O.assemble_rhs(4, p, *O.manufactured_data(4)[:5]))
     b9 = T(np.zeros(81))
     lib = h33.lib
     import ctypes as C

@@ -121,7 +121,7 @@ def test_high_order_manufactured_convergence(p, levels, lo, hi):
     errs = []
     for level in levels:
         A = O.assemble_operator(level, p)
-        f, gW, gE, gS, gN, u = O.baseline_data(level)
+        f, gW, gE, gS, gN, u = O.manufactured_data(level)
         b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
         x = spla.spsolve(A.tocsc(), b)
         errs.append(O.h_norm(x - u, O.h_tilde_diag(level, p)))
@@ -145,7 +145,7 @@ def test_manufactured_convergence(p, rate_lo, rate_hi, e129):
     errs = []
     for level in (4, 5, 6, 7):
         A = O.assemble_operator(level, p)
-        f, gW, gE, gS, gN, u = O.baseline_data(level)
+        f, gW, gE, gS, gN, u = O.manufactured_data(level)
         b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
         x = spla.spsolve(A.tocsc(), b)
         errs.append(O.h_norm(x - u, O.h_tilde_diag(level, p)))
@@ -165,7 +165,7 @@ def test_rhs_matches_manufactured_limit(p):
     SAT data maps carry the right signs and scalings."""
     for level, tol in ((6, 5e-2), (7, 2.5e-2)):
         A = O.assemble_operator(level, p)
-        f, gW, gE, gS, gN, u = O.baseline_data(level)
+        f, gW, gE, gS, gN, u = O.manufactured_data(level)
         b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
         hd = O.h_tilde_diag(level, p)
         rel = np.linalg.norm((A @ u - b) / np.sqrt(hd)) / np.linalg.norm(b / np.sqrt(hd))
@@ -208,7 +208,7 @@ def test_transfers():
 def test_cg_cpu_against_direct(p):
     level = 4
     A = O.assemble_operator(level, p)
-    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    f, gW, gE, gS, gN, u = O.manufactured_data(level)
     b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
     xd = spla.spsolve(A.tocsc(), b)
     x, it, norms = O.cg_cpu(A, b, abstol=1e-12 * np.linalg.norm(b))
@@ -222,7 +222,7 @@ def test_mgpcg_oracle_against_direct(p):
     level = 6
     hier = O.Hierarchy(level, p)
     A = hier.A[0]
-    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    f, gW, gE, gS, gN, u = O.manufactured_data(level)
     b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
     xd = spla.spsolve(A.tocsc(), b)
     x, it, norms = O.mgpcg(hier, b, rtol=1e-12)
@@ -238,7 +238,7 @@ def test_mgpcg_oracle_against_direct(p):
 def test_cg_iteration_count_129_p2():
     """Expected count for the 129^2 CG case (relative residual 1e-8)."""
     A = O.assemble_operator(7, 2)
-    f, gW, gE, gS, gN, u = O.baseline_data(7)
+    f, gW, gE, gS, gN, u = O.manufactured_data(7)
     b = O.assemble_rhs(7, 2, f, gW, gE, gS, gN)
     x, it, norms = O.cg_cpu(A, b, abstol=1e-8 * np.linalg.norm(b))
     assert 540 <= it <= 580
@@ -274,9 +274,9 @@ def test_initial_guess_interpolation_cuts_cg_iterations(p):
     solution in fewer fine-grid CG iterations than CG from zero."""
     level = 6
     A, A2 = O.assemble_operator(level, p), O.assemble_operator(level - 1, p)
-    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    f, gW, gE, gS, gN, u = O.manufactured_data(level)
     b = O.assemble_rhs(level, p, f, gW, gE, gS, gN)
-    f2, gW2, gE2, gS2, gN2, _ = O.baseline_data(level - 1)
+    f2, gW2, gE2, gS2, gN2, _ = O.manufactured_data(level - 1)
     b2 = O.assemble_rhs(level - 1, p, f2, gW2, gE2, gS2, gN2)
     tol = 1e-10 * np.linalg.norm(b)
     x, iters, norms, x0 = O.initial_guess_interpolation_cg(A, b, A2, b2, tol)
@@ -305,7 +305,7 @@ def test_two_level_richardson_restatement():
     assert np.abs(z0 - ref).max() < 1e-12 * np.abs(ref).max()
     M = lambda r: O.two_level_richardson(A, A2, r)[0]
     assert abs(c @ M(b) - b @ M(c)) < 1e-10 * abs(c @ M(b))
-    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    f, gW, gE, gS, gN, u = O.manufactured_data(level)
     rhs = O.assemble_rhs(level, 2, f, gW, gE, gS, gN)
     x, it, norms = O.matrix_free_mgcg(A, rhs, M, abstol=1e-10 * np.linalg.norm(rhs))
     assert norms[-1] < 1e-10 * np.linalg.norm(rhs) and it < 60
@@ -419,6 +419,6 @@ def test_golden_fixtures(golden_dir, level, p):
     A0 = sp.csr_matrix((d["A_data"], d["A_indices"], d["A_indptr"]), shape=A.shape)
     assert abs(A - A0).max() < 1e-14
     assert np.abs(A @ d["u"] - d["Au"]).max() < 1e-12
-    f, gW, gE, gS, gN, u = O.baseline_data(level)
+    f, gW, gE, gS, gN, u = O.manufactured_data(level)
     assert np.abs(O.assemble_rhs(level, p, f, gW, gE, gS, gN) - d["b"]).max() < 1e-14
     assert np.abs(A @ d["x"] - d["b"]).max() < 1e-10
