@@ -197,7 +197,11 @@ int sbp_mgpcg(sbp_handle h, const double* b, double* x, double abstol, int maxit
  * estimate from successive corrections drops below ir_rtol ||x||.
  * outer_iters receives the number of corrections applied, inner_iters
  * (nullable) the total inner iterations, resnorm the last FP64 residual norm
- * evaluated. */
+ * evaluated.  On a variable-coefficient handle it is instead FP64 MG-PCG
+ * whose V-cycle runs in FP32 (r cast down, z cast up; the operator products,
+ * the Krylov vectors and the inner products stay FP64), for at most
+ * inner_maxiter * max(max_outer, 1) iterations; outer_iters and inner_iters
+ * both receive the iteration count. */
 int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double abstol, int max_outer,
                     int* outer_iters, int* inner_iters, double* resnorm);
 /* Batched forms of sbp_mgpcg and sbp_mgpcg_mixed over the handle's nprob

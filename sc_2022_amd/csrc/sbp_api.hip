@@ -109,6 +109,7 @@ struct sbp_handle_s {
         VarTab* dtab = nullptr;
         double* band = nullptr;
         double *b = nullptr, *x = nullptr, *t = nullptr;
+        float *b32 = nullptr, *x32 = nullptr, *t32 = nullptr;   // FP32 V-cycle (level 0: r32 / z32 / Ap32)
     };
     std::vector<VarLevel> vl;
     bool vmg = false;
@@ -1017,7 +1018,7 @@ int sbp_destroy(sbp_handle h) {
     for (size_t l = 0; l < h->vl.size(); ++l) {
         auto& V = h->vl[l];
         if (l > 0) {
-            void* own[] = {V.dtab, V.band, V.b, V.x, V.t};
+            void* own[] = {V.dtab, V.band, V.b, V.x, V.t, V.b32, V.x32, V.t32};
             for (void* q : own)
                 if (q) (void)hipFree(q);
         }
@@ -1332,17 +1333,19 @@ int sbp_set_tiles(sbp_handle h, int level, int dtype, int ti, int tj) {
 }
 
 namespace {
+extern "C++" {   // templates inside the C-linkage block
 // One launch of the variable-coefficient operator (MODE as var_apply);
 // *nparts receives the number of block partials.
-int launch_var(sbp_handle h, int mode, const double* u, double* out, const double* aux, double sign,
+template <typename T = double>
+int launch_var(sbp_handle h, int mode, const T* u, T* out, const T* aux, double sign,
                const Scalars* sc, int* nparts, int level = 0, double omega = 0.0) {
     const int N = level == 0 ? h->N : h->vl[level].N;
     const VarTab* tab = level == 0 ? h->vtab : h->vl[level].dtab;
     const dim3 g((N + kVarThreads - 1) / kVarThreads, (N + kVarRows - 1) / kVarRows);
     if ((size_t)g.x * g.y > h->npp) return SBP_ERR_INVALID_SIZE;
 #define SBP_VAR_LAUNCH(P_, M_) \
-    hipLaunchKernelGGL((var_apply<P_, M_>), g, dim3(kVarThreads), 0, h->st, u, out, aux, sign, h->partial, tab, sc, \
-                       omega)
+    hipLaunchKernelGGL((var_apply<P_, M_, T>), g, dim3(kVarThreads), 0, h->st, u, out, aux, sign, h->partial, tab, \
+                       sc, omega)
 #define SBP_VAR_MODES(P_)                       \
     switch (mode) {                             \
     case 0: SBP_VAR_LAUNCH(P_, 0); break;       \
@@ -1409,27 +1412,32 @@ int var_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter,
 // the coarser cycle, bilinear prolongation, nu_post sweeps; the coarsest
 // level is the dense inverse.  Sweeps ping-pong between x and t, starting so
 // that the last one lands in x; the residual goes to whichever is free.
-int var_vcycle(sbp_handle h, int l, const double* b, double* x, const Scalars* sc, int* np_rz = nullptr) {
+template <typename T>
+int var_vcycle(sbp_handle h, int l, const T* b, T* x, const Scalars* sc, int* np_rz = nullptr) {
+    constexpr bool f32 = std::is_same<T, float>::value;
     auto& V = h->vl[l];
     const int nlev = (int)h->vl.size();
     hipStream_t st = h->st;
     if (l == nlev - 1) {
-        hipLaunchKernelGGL((coarse_solve<double>), dim3((h->nc + kCoarseRows - 1) / kCoarseRows, 1), dim3(256), 0, st,
+        hipLaunchKernelGGL((coarse_solve<T>), dim3((h->nc + kCoarseRows - 1) / kCoarseRows, 1), dim3(256), 0, st,
                            (const double*)h->Ainv, b, x, h->nc, sc);
         HIPCHK(hipGetLastError());
         return SBP_OK;
     }
     const int nu1 = h->prm.nu_pre, nu2 = h->prm.nu_post;
     const double om = h->prm.omega;
-    double* bufs[2] = {x, V.t};
+    T* tl;   // this level's sweep buffer
+    if constexpr (f32) tl = l == 0 ? h->Ap32 : V.t32;
+    else tl = V.t;
+    T* bufs[2] = {x, tl};
     // sweeps after the first: (nu1 - 1) + nu2 (or nu2 - 1 after a zero start)
     const int later = nu1 > 0 ? (nu1 - 1) + nu2 : nu2;
     int cur = later % 2 == 0 ? 0 : 1;   // the buffer the first sweep writes
     if (nu1 > 0) {
-        hipLaunchKernelGGL(var_jacobi0, dim3((V.N + 255) / 256, V.N), dim3(256), 0, st, bufs[cur], b,
+        hipLaunchKernelGGL((var_jacobi0<T>), dim3((V.N + 255) / 256, V.N), dim3(256), 0, st, bufs[cur], b,
                            (const VarTab*)V.dtab, om, sc);
     } else {
-        hipLaunchKernelGGL((fill<double, kEwThreads>), dim3(ew_blocks(V.n), 1), dim3(kEwThreads), 0, st, bufs[cur], 0.0,
+        hipLaunchKernelGGL((fill<T, kEwThreads>), dim3(ew_blocks(V.n), 1), dim3(kEwThreads), 0, st, bufs[cur], T(0),
                            V.n, sc);
     }
     HIPCHK(hipGetLastError());
@@ -1439,12 +1447,16 @@ int var_vcycle(sbp_handle h, int l, const double* b, double* x, const Scalars* s
     }
     // residual into the free buffer, restricted to the coarser right-hand side
     auto& C = h->vl[l + 1];
+    T* cb;
+    T* cx;
+    if constexpr (f32) { cb = C.b32; cx = C.x32; }
+    else { cb = C.b; cx = C.x; }
     STCHK(launch_var(h, 2, bufs[cur], bufs[1 - cur], b, 1.0, sc, nullptr, l));
-    hipLaunchKernelGGL((restrict_fw<double>), dim3((C.N + 255) / 256, C.N, 1), dim3(256), 0, st,
-                       (const double*)bufs[1 - cur], C.b, V.N, C.N, h->prm.restrict_scale, sc);
+    hipLaunchKernelGGL((restrict_fw<T>), dim3((C.N + 255) / 256, C.N, 1), dim3(256), 0, st,
+                       (const T*)bufs[1 - cur], cb, V.N, C.N, (T)h->prm.restrict_scale, sc);
     HIPCHK(hipGetLastError());
-    STCHK(var_vcycle(h, l + 1, C.b, C.x, sc));
-    hipLaunchKernelGGL((prolong_add<double>), dim3((V.N + 255) / 256, V.N, 1), dim3(256), 0, st, (const double*)C.x,
+    STCHK(var_vcycle<T>(h, l + 1, (const T*)cb, cx, sc));
+    hipLaunchKernelGGL((prolong_add<T>), dim3((V.N + 255) / 256, V.N, 1), dim3(256), 0, st, (const T*)cx,
                        bufs[cur], V.N, C.N, sc);
     HIPCHK(hipGetLastError());
     // with np_rz the last sweep also leaves the block partials of b'x (its
@@ -1455,16 +1467,32 @@ int var_vcycle(sbp_handle h, int l, const double* b, double* x, const Scalars* s
         cur = 1 - cur;
     }
     if (bufs[cur] != x) {
-        hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(V.n), 1), dim3(kEwThreads), 0, st,
-                           (const double*)bufs[cur], x, V.n, sc);
+        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(V.n), 1), dim3(kEwThreads), 0, st,
+                           (const T*)bufs[cur], x, V.n, sc);
         HIPCHK(hipGetLastError());
+    }
+    return SBP_OK;
+}
+
+// FP32 copies of the variable-coefficient V-cycle's level buffers (made on
+// first use; level 0 borrows r32 / z32 / Ap32).
+int ensure_var32(sbp_handle h) {
+    STCHK(ensure32(h));
+    for (size_t l = 1; l < h->vl.size(); ++l) {
+        auto& V = h->vl[l];
+        if (V.b32) continue;
+        HIPCHK(hipMalloc(&V.b32, V.n * sizeof(float)));
+        HIPCHK(hipMalloc(&V.x32, V.n * sizeof(float)));
+        HIPCHK(hipMalloc(&V.t32, V.n * sizeof(float)));
     }
     return SBP_OK;
 }
 
 // MG-preconditioned CG on the variable-coefficient operator (unfused, as
 // var_cg): the V-cycle above as preconditioner; stops on ||r||_2 < abstol.
-int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, double* hist) {
+// With f32 the V-cycle runs on FP32 copies (r cast down, z cast up; the
+// Krylov vectors, the operator products and the inner products stay FP64).
+int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, double* hist, bool f32 = false) {
     hipStream_t st = h->st;
     const size_t n = h->L[0].n;
     Scalars* sc = h->sc;
@@ -1480,7 +1508,16 @@ int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter
     int nrz = 0;
     auto precondition = [&]() -> int {   // z = M r ; r'z partials (nrz of them)
         nrz = 0;
-        STCHK(var_vcycle(h, 0, h->r, z, sc, h->prm.nu_post > 0 ? &nrz : nullptr));
+        if (f32) {
+            hipLaunchKernelGGL((cast_vec<double, float, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->r,
+                               h->r32, n);
+            STCHK(var_vcycle<float>(h, 0, (const float*)h->r32, h->z32, sc));
+            hipLaunchKernelGGL((cast_vec<float, double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const float*)h->z32,
+                               z, n);
+            HIPCHK(hipGetLastError());
+        } else {
+            STCHK(var_vcycle<double>(h, 0, (const double*)h->r, z, sc, h->prm.nu_post > 0 ? &nrz : nullptr));
+        }
         if (nrz == 0) {
             hipLaunchKernelGGL((dot_partial<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->r,
                                (const double*)z, n, h->partial);
@@ -1517,13 +1554,14 @@ int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter
     }
     return poll(h, sc);
 }
+}  // extern "C++"
 }  // namespace
 
 int sbp_apply(sbp_handle h, int level, const void* u, void* Au, int sign, int dtype) {
     if (!h || !u || !Au || (sign != 1 && sign != -1) || level < 0 || level >= h->nlev) return SBP_ERR_INVALID_ARG;
     if (h->var) {   // variable coefficients: FP64, level 0
         if (dtype != SBP_F64 || level != 0) return SBP_ERR_INVALID_ARG;
-        return launch_var(h, 0, (const double*)u, (double*)Au, nullptr, (double)sign, nullptr, nullptr);
+        return launch_var<double>(h, 0, (const double*)u, (double*)Au, nullptr, (double)sign, nullptr, nullptr);
     }
     if (dtype == SBP_F64)
         return launch_pass<double>(h, level, InPlain<double>{(const double*)u},
@@ -1663,7 +1701,7 @@ int sbp_coarse_solve(sbp_handle h, const double* b, double* x) {
 int sbp_vcycle(sbp_handle h, const void* r, void* z, int dtype) {
     if (h && h->var) {
         if (!r || !z || !h->vmg || dtype != SBP_F64) return SBP_ERR_INVALID_ARG;
-        return var_vcycle(h, 0, (const double*)r, (double*)z, nullptr);
+        return var_vcycle<double>(h, 0, (const double*)r, (double*)z, nullptr);
     }
     if (!h || !r || !z || !h->mg) return SBP_ERR_INVALID_ARG;
     if (dtype == SBP_F64) return vcycle_level<double>(h, 0, (const double*)r, (double*)z, nullptr, nullptr, nullptr);
@@ -1718,6 +1756,15 @@ int sbp_mgpcg_batch(sbp_handle h, const double* b, double* x, const double* abst
 int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double abstol, int max_outer, int* outer_iters,
                     int* inner_iters, double* resnorm) {
     if (!h || !b || !x || max_outer < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
+    if (h->var) {   // FP64 MG-PCG with the V-cycle in FP32 (no refinement loop)
+        if (!h->vmg) return SBP_ERR_INVALID_SIZE;
+        STCHK(ensure_var32(h));
+        const int cap = h->prm.inner_maxiter * std::max(max_outer, 1);
+        STCHK(var_pcg(h, b, x, abstol, cap, nullptr, true));
+        get_results(h, outer_iters, resnorm, nullptr);
+        if (inner_iters && outer_iters) *inner_iters = *outer_iters;
+        return batch_status(h);
+    }
     if (!h->mg) return SBP_ERR_INVALID_SIZE;
     int inner = 0;
     STCHK(mixed_solve(h, b, x, &abstol, max_outer, &inner));

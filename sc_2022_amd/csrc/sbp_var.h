@@ -280,24 +280,26 @@ __device__ __forceinline__ double var_diag1(const VarAxis& a, int R, int H, int 
     return a.band[(size_t)H * N + r];
 }
 
-template <int MODE>
-__device__ __forceinline__ void var_emit(double* out, const double* aux, double sign, size_t q, double Au,
+// T is the storage type of u, out and aux (double or float); the arithmetic
+// is FP64 either way.
+template <int MODE, typename T>
+__device__ __forceinline__ void var_emit(T* out, const T* aux, double sign, size_t q, double Au,
                                          double& acc, double uc, double dg, double omega) {
     if constexpr (MODE == 3) {
-        out[q] = uc + omega * (aux[q] - Au) / dg;
+        out[q] = (T)(uc + omega * ((double)aux[q] - Au) / dg);
     } else if constexpr (MODE == 4) {
         const double a = aux[q];
         const double o = uc + omega * (a - Au) / dg;
-        out[q] = o;
+        out[q] = (T)o;
         acc += a * o;
     } else if constexpr (MODE == 0) {
-        out[q] = sign * Au;
+        out[q] = (T)(sign * Au);
     } else if constexpr (MODE == 1) {
-        out[q] = Au;
-        acc += aux[q] * Au;
+        out[q] = (T)Au;
+        acc += (double)aux[q] * Au;
     } else {
-        const double rv = aux[q] - Au;
-        out[q] = rv;
+        const double rv = (double)aux[q] - Au;
+        out[q] = (T)rv;
         acc += rv * rv;
     }
 }
@@ -310,9 +312,9 @@ __device__ __forceinline__ void var_emit(double* out, const double* aux, double 
 // and last column of blocks) each thread tests its column: out-of-grid
 // threads idle, threads on the closure columns take the closure row of m
 // (direct loads, table t staged in LDS) and their H weight.
-template <int P, int MODE, bool ICHK>
-__device__ __forceinline__ double var_tile(const double* __restrict__ u, double* __restrict__ out,
-                                           const double* __restrict__ aux, double sign, double omega,
+template <int P, int MODE, bool ICHK, typename T>
+__device__ __forceinline__ double var_tile(const T* __restrict__ u, T* __restrict__ out,
+                                           const T* __restrict__ aux, double sign, double omega,
                                            const VarTab* __restrict__ tp,
                                            const VarTab& t, int i0, int j0,
                                            double (*tile)[kVarThreads + 2 * (P / 2)]) {
@@ -329,13 +331,13 @@ __device__ __forceinline__ double var_tile(const double* __restrict__ u, double*
     double w[W];
     double cm[2 * H + 1];
 #pragma unroll
-    for (int k = 0; k < W; ++k) w[k] = act ? u[gidx(i, j0 - H + k, N)] : 0.0;
+    for (int k = 0; k < W; ++k) w[k] = act ? (double)u[gidx(i, j0 - H + k, N)] : 0.0;
 #pragma unroll
     for (int q = 0; q <= 2 * H; ++q) cm[q] = iint ? mb[(size_t)q * N + i] : 0.0;
     if (tid < 2 * H * J) {   // halo columns: H left, H right, J rows
         const int side = tid / (H * J), rem = tid % (H * J), off = rem / J, r = rem % J;
         const int ii = side == 0 ? i0 - H + off : i0 + kVarThreads + off;
-        tile[r][side == 0 ? off : kVarThreads + H + off] = (!ICHK || (ii >= 0 && ii < N)) ? u[gidx(ii, j0 + r, N)] : 0.0;
+        tile[r][side == 0 ? off : kVarThreads + H + off] = (!ICHK || (ii >= 0 && ii < N)) ? (double)u[gidx(ii, j0 + r, N)] : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < J; ++r) tile[r][tid + H] = w[r + H];
@@ -360,15 +362,15 @@ __device__ __forceinline__ double var_tile(const double* __restrict__ u, double*
         } else {
             mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
         }
-        var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + mp, acc, w[r + H],
+        var_emit<MODE, T>(out, aux, sign, gidx(i, j, N), hwi * kp + mp, acc, w[r + H],
                        hwi * kb[(size_t)H * N + j] + mdi, omega);
     }
     return acc;
 }
 
-template <int P, int MODE>
-__global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restrict__ u, double* __restrict__ out,
-                                                        const double* __restrict__ aux, double sign, double* partial,
+template <int P, int MODE, typename T = double>
+__global__ __launch_bounds__(kVarThreads) void var_apply(const T* __restrict__ u, T* __restrict__ out,
+                                                        const T* __restrict__ aux, double sign, double* partial,
                                                         const VarTab* __restrict__ tp, const Scalars* sc,
                                                         double omega) {
     constexpr int H = P / 2;
@@ -383,7 +385,7 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
     const bool iin = i0 - H >= R && i0 + kVarThreads + H <= N - R;
     double acc = 0.0;
     if (jin && iin) {
-        acc = var_tile<P, MODE, false>(u, out, aux, sign, omega, tp, t, i0, j0, tile);
+        acc = var_tile<P, MODE, false, T>(u, out, aux, sign, omega, tp, t, i0, j0, tile);
     } else {
         {
             const unsigned long long* src = reinterpret_cast<const unsigned long long*>(tp);
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
         }
         __syncthreads();
         if (jin) {
-            acc = var_tile<P, MODE, true>(u, out, aux, sign, omega, tp, t, i0, j0, tile);
+            acc = var_tile<P, MODE, true, T>(u, out, aux, sign, omega, tp, t, i0, j0, tile);
         } else {
             const int i = i0 + tid;   // face rows: per point
             if (i < N) {
@@ -406,8 +408,8 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
                     const double kp = var_row<H>(t.k, t.R, t.C, N, j, [&](int c) { return u[gidx(i, c, N)]; });
                     const double mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
                     const double dg = hwi * var_diag1(t.k, t.R, H, N, j) + hwj * var_diag1(t.m, t.R, H, N, i);
-                    var_emit<MODE>(out, aux, sign, gidx(i, j, N), hwi * kp + hwj * mp, acc, u[gidx(i, j, N)], dg,
-                                   omega);
+                    var_emit<MODE, T>(out, aux, sign, gidx(i, j, N), hwi * kp + hwj * mp, acc,
+                                      (double)u[gidx(i, j, N)], dg, omega);
                 }
             }
         }
@@ -421,7 +423,8 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const double* __restric
 
 // First damped-Jacobi sweep from zero: x = omega b / diag(A), the diagonal
 // hw_i k_jj + hw_j m_ii formed from the tables (blockIdx.y = row j).
-__global__ __launch_bounds__(256) void var_jacobi0(double* __restrict__ x, const double* __restrict__ b,
+template <typename T>
+__global__ __launch_bounds__(256) void var_jacobi0(T* __restrict__ x, const T* __restrict__ b,
                                                   const VarTab* __restrict__ tp, double omega, const Scalars* sc) {
     if (sc && sc->done) return;
     const VarTab& t = *tp;
@@ -432,7 +435,7 @@ __global__ __launch_bounds__(256) void var_jacobi0(double* __restrict__ x, const
     const double hwi = ri < t.RH ? t.hw[ri] : 1.0, hwj = rj < t.RH ? t.hw[rj] : 1.0;
     const double d = hwi * var_diag1(t.k, t.R, t.H, N, j) + hwj * var_diag1(t.m, t.R, t.H, N, i);
     const size_t q = gidx(i, j, N);
-    x[q] = omega * b[q] / d;
+    x[q] = (T)(omega * (double)b[q] / d);
 }
 
 // Dense A (row-major, n = N*N) of the variable-coefficient operator from

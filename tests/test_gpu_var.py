@@ -168,6 +168,32 @@ def test_var_mgpcg_manufactured_rate_large(SP, p, rate, levels):
     assert max(its) - min(its) <= 2 and max(its) <= 20, its
 
 
+@pytest.mark.parametrize("p,level", [(2, 7), (4, 7), (6, 7), (4, 10)])
+def test_var_mgpcg_fp32_vcycle(SP, p, level):
+    """FP64 MG-PCG with the V-cycle in FP32 (sbp_mgpcg_mixed on a
+    variable-coefficient handle): the FP64 residual reaches the same tolerance
+    and the solution agrees with the FP64 V-cycle's, in about as many
+    iterations (the FP32 preconditioner is a perturbation of order 1e-7 of the
+    FP64 one)."""
+    N = 2 ** level + 1
+    bx, by, f, gW, gE, gS, gN, u = O.var_data(level)
+    h = SP(N, p, coef=(bx, by))
+    b = h.assemble_rhs(T(f), T(gW), T(gE), T(gS), T(gN))
+    tol = 1e-11 * float(torch.linalg.vector_norm(b))
+    x64, r64 = h.mgpcg(b, abstol=tol)
+    x32, r32 = h.mgpcg_mixed(b, abstol=tol)
+    assert r64.converged and r32.converged
+    assert r32.iters <= r64.iters + 2, (r32.iters, r64.iters)
+    assert r32.inner_iters == r32.iters
+    assert relmax(x32.cpu().numpy(), x64.cpu().numpy()) < 1e-9
+    Ax = torch.empty_like(b)
+    h.apply(x32, Ax)
+    assert float(torch.linalg.vector_norm(b - Ax)) < 1.01 * tol
+    # a second call on the same handle reuses the FP32 level buffers
+    x2, r2 = h.mgpcg_mixed(b, abstol=tol)
+    assert r2.iters == r32.iters and relmax(x2.cpu().numpy(), x32.cpu().numpy()) < 1e-13
+
+
 def test_var_handle_rejects_what_it_does_not_serve(SP):
     from sc_2022_amd import _capi as capi
     N = 65
@@ -179,11 +205,11 @@ def test_var_handle_rejects_what_it_does_not_serve(SP):
         h.apply(T(np.zeros(N * N), torch.float32))
     with pytest.raises(Exception):
         h.vcycle(T(np.zeros(N * N), torch.float32))
-    x, res = h.mgpcg_mixed(T(np.ones(N * N)), abstol=1e-8, raise_on_fail=False)
-    assert res.status == capi.SBP_ERR_INVALID_SIZE
     hr = SP(60, 4, coef=(np.ones(60), np.ones(60)))   # not 2^k + 1: no hierarchy
     assert hr.nlevels == 0
     x, res = hr.mgpcg(T(np.ones(60 * 60)), abstol=1e-8, raise_on_fail=False)
+    assert res.status == capi.SBP_ERR_INVALID_SIZE
+    x, res = hr.mgpcg_mixed(T(np.ones(60 * 60)), abstol=1e-8, raise_on_fail=False)
     assert res.status == capi.SBP_ERR_INVALID_SIZE
     with pytest.raises(Exception):
         SP(N, 4, coef=(bx, -by))            # coefficients must be positive
