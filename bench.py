@@ -12,7 +12,7 @@ rotating buffers larger than the 256 MB Infinity Cache for the roofline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4097] [--p 4]
 
-Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+Prints one JSON line.  --gpus above 1 is refused: the library runs on one GPU.
 """
 from __future__ import annotations
 
@@ -234,7 +234,7 @@ def csr_leg(h, torch, apply_us):
 
 
 def leg_8193(a, dev, torch):
-    """Mixed precision and FP64 MG-PCG at 8193^2 (configs[2] size), each leg
+    """Mixed precision and FP64 MG-PCG at 8193^2 (p = 4, the largest single-GPU grid), each leg
     with the loosest rule that meets rel err <= 1e-10 at this size
     (bench.py --n 8193 --tol-sweep): FP64 ||r|| < rtol ||b||, mixed
     ||r|| < mixed_rtol ||b|| with inner FP32 solves to 3e-6; errors against
@@ -573,7 +573,7 @@ def solve_rates(h, N, rmx, r64, tmx, t64):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+        print(json.dumps({"error": "the solver runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}), flush=True)
         sys.exit(2)
     import torch

@@ -52,7 +52,7 @@ def test_manufactured_convergence_rate(p, levels, rate):
 
 @pytest.mark.parametrize("p", [2, 4])
 def test_mgpcg_4097_residual_by_oracle(p):
-    """configs[1] at its largest size: FP64 MG-PCG on the Fourier-mode problem
+    """4097^2 (the MG-PCG workload at its largest size): FP64 MG-PCG on the Fourier-mode problem
     stops at ||r|| < 1e-12 ||b||; the host oracle's apply confirms it."""
     from sc_2022_amd.data import fourier_rhs
     from sc_2022_amd.solver import SbpPoisson
@@ -67,7 +67,7 @@ def test_mgpcg_4097_residual_by_oracle(p):
 
 
 def test_mixed_precision_8193():
-    """FP32 inner MG-PCG inside FP64 refinement at 8193^2, p = 4,
+    """8193^2, p = 4: FP32 inner MG-PCG inside FP64 refinement at 8193^2, p = 4,
     to rel H-norm error <= 1e-10 against the FP64 solve converged to
     ||r|| < 1e-14 ||b||; the host oracle recomputes the mixed solution's residual.
     Rules as the 8193^2 bench leg: ||r|| < 1e-10 ||b|| with inner FP32 solves

@@ -30,9 +30,9 @@ def SP():
     return SbpPoisson
 
 
-def problem(level, p, data="baseline"):
+def problem(level, p, data="manufactured"):
     A = O.assemble_operator(level, p)
-    if data == "baseline":
+    if data == "manufactured":
         f, gW, gE, gS, gN, u = O.manufactured_data(level)
     else:
         f, gW, gE, gS, gN = O.fourier_data(level, 7)
@@ -43,7 +43,7 @@ def problem(level, p, data="baseline"):
 
 @pytest.mark.parametrize("p", [2, 4])
 def test_cg_129(SP, p):
-    """FP64 matrix-free CG, 129^2, x0 = 0, ||r|| < 1e-8 ||b||."""
+    """The reference's CPU-runnable case: FP64 matrix-free CG, 129^2, x0 = 0, ||r|| < 1e-8 ||b||."""
     level = 7
     A, b, u, _ = problem(level, p)
     tol = 1e-8 * np.linalg.norm(b)
