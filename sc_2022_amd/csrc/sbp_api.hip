@@ -1412,6 +1412,8 @@ int var_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter,
 // the coarser cycle, bilinear prolongation, nu_post sweeps; the coarsest
 // level is the dense inverse.  Sweeps ping-pong between x and t, starting so
 // that the last one lands in x; the residual goes to whichever is free.
+// T is the storage type of the level vectors (float: the b32 / x32 / t32
+// copies, level 0 sweeping through Ap32); the kernels compute in FP64.
 template <typename T>
 int var_vcycle(sbp_handle h, int l, const T* b, T* x, const Scalars* sc, int* np_rz = nullptr) {
     constexpr bool f32 = std::is_same<T, float>::value;
