@@ -48,7 +48,8 @@ def parse():
                     help="mixed: relative tolerance of each FP32 inner solve")
     ap.add_argument("--ir-rtol", type=float, default=0.0,
                     help="mixed: also stop when the correction-based error estimate <= ir_rtol ||x||")
-    ap.add_argument("--no-extra", action="store_true", help="skip the 8193^2, CSR comparison, initial-guess CG, reference-algorithm and variable-coefficient legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 8193^2, CSR comparison, initial-guess CG, reference-algorithm, variable-coefficient and adjoint legs")
+    ap.add_argument("--extra", default="", help="comma list of the extra legs to run (default: all of them)")
     ap.add_argument("--tol-sweep", action="store_true",
                     help="print iterations, error and time per stopping rule for both legs, then exit")
     ap.add_argument("--apply-reps", type=int, default=200)
@@ -654,10 +655,14 @@ def main():
                          ("reference_algorithms", lambda: reference_modes_leg(a, dev, torch)),
                          ("variable_coefficient", lambda: variable_coefficient_leg(a, dev, torch)),
                          ("adjoint", lambda: adjoint_leg(a, dev, torch))):
+            if a.extra and name not in a.extra.split(","):
+                continue
+            t_leg = time.perf_counter()
             try:
                 extra[name] = fn()
             except Exception as e:  # reported, never silently replaced
                 extra[name] = {"error": f"{type(e).__name__}: {e}"}
+            print(f"# {name}: {time.perf_counter() - t_leg:.1f} s", file=sys.stderr, flush=True)
 
     batched = None
     if a.batch > 0:
