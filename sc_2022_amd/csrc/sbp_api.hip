@@ -1763,8 +1763,10 @@ int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double abstol, int
         STCHK(ensure_var32(h));
         const int cap = h->prm.inner_maxiter * std::max(max_outer, 1);
         STCHK(var_pcg(h, b, x, abstol, cap, nullptr, true));
-        get_results(h, outer_iters, resnorm, nullptr);
-        if (inner_iters && outer_iters) *inner_iters = *outer_iters;
+        int it = 0;
+        get_results(h, &it, resnorm, nullptr);
+        if (outer_iters) *outer_iters = it;
+        if (inner_iters) *inner_iters = it;
         return batch_status(h);
     }
     if (!h->mg) return SBP_ERR_INVALID_SIZE;
