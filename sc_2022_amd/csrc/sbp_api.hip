@@ -1492,8 +1492,9 @@ int ensure_var32(sbp_handle h) {
 
 // MG-preconditioned CG on the variable-coefficient operator (unfused, as
 // var_cg): the V-cycle above as preconditioner; stops on ||r||_2 < abstol.
-// With f32 the V-cycle runs on FP32 copies (r cast down, z cast up; the
-// Krylov vectors, the operator products and the inner products stay FP64).
+// With f32 the V-cycle runs on FP32 copies (r rounded down by the residual
+// update, z read in FP32 by r'z and the search-direction update; the Krylov
+// vectors, the operator products and the inner products stay FP64).
 int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, double* hist, bool f32 = false) {
     hipStream_t st = h->st;
     const size_t n = h->L[0].n;
@@ -1508,15 +1509,21 @@ int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter
     HIPCHK(hipGetLastError());
     const dim3 g(ew_blocks(n), 1);
     int nrz = 0;
+    // f32: r32 holds r rounded to FP32 (the residual update writes it after
+    // the first precondition) and z stays in z32 (read as such by r'z and
+    // the search-direction update)
+    bool r32_ready = false;
     auto precondition = [&]() -> int {   // z = M r ; r'z partials (nrz of them)
         nrz = 0;
         if (f32) {
-            hipLaunchKernelGGL((cast_vec<double, float, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->r,
-                               h->r32, n);
+            if (!r32_ready)
+                hipLaunchKernelGGL((cast_vec<double, float, kEwThreads>), g, dim3(kEwThreads), 0, st,
+                                   (const double*)h->r, h->r32, n);
             STCHK(var_vcycle<float>(h, 0, (const float*)h->r32, h->z32, sc));
-            hipLaunchKernelGGL((cast_vec<float, double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const float*)h->z32,
-                               z, n);
+            hipLaunchKernelGGL((dot_partial<double, kEwThreads, float>), g, dim3(kEwThreads), 0, st,
+                               (const double*)h->r, (const float*)h->z32, n, h->partial);
             HIPCHK(hipGetLastError());
+            nrz = (int)g.x;
         } else {
             STCHK(var_vcycle<double>(h, 0, (const double*)h->r, z, sc, h->prm.nu_post > 0 ? &nrz : nullptr));
         }
@@ -1530,22 +1537,35 @@ int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter
     };
     STCHK(precondition());
     hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(kRedThreads), 0, st, h->partial, nrz, sc);
-    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)z, h->pv, n,
-                       (const Scalars*)sc);
+    if (f32)
+        hipLaunchKernelGGL((cast_vec<float, double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const float*)h->z32,
+                           h->pv, n);
+    else
+        hipLaunchKernelGGL((copy_vec<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)z, h->pv, n,
+                           (const Scalars*)sc);
     HIPCHK(hipGetLastError());
+    r32_ready = f32;
     const bool adaptive = h->prm.poll_every == 0;
     int next = adaptive ? 2 : h->prm.poll_every;
     for (int it = 0; it < maxiter; ++it) {
         STCHK(launch_var(h, 1, h->pv, h->Ap, h->pv, 1.0, sc, &np));
         hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, 1);
-        hipLaunchKernelGGL((update_xr<double, kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, (const double*)h->pv,
-                           (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
+        if (f32)
+            hipLaunchKernelGGL((update_xr_r32<kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, h->r32,
+                               (const double*)h->pv, (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
+        else
+            hipLaunchKernelGGL((update_xr<double, kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r,
+                               (const double*)h->pv, (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
         hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 0);
         HIPCHK(hipGetLastError());
         STCHK(precondition());
         hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(kRedThreads), 0, st, h->partial, nrz, sc);
-        hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)z,
-                           (const Scalars*)sc, n);
+        if (f32)
+            hipLaunchKernelGGL((update_p<double, kEwThreads, float>), g, dim3(kEwThreads), 0, st, h->pv,
+                               (const float*)h->z32, (const Scalars*)sc, n);
+        else
+            hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)z,
+                               (const Scalars*)sc, n);
         HIPCHK(hipGetLastError());
         if (it + 1 >= next && it + 1 < maxiter) {
             bool all = false;

@@ -294,8 +294,31 @@ __global__ __launch_bounds__(NT) void update_xr(T* __restrict__ x, T* __restrict
     if (threadIdx.x == 0) partial[(size_t)q * gridDim.x + blockIdx.x] = (double)s;
 }
 
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void update_p(T* __restrict__ p, const T* __restrict__ r, const Scalars* scb,
+// update_xr in FP64 that also leaves the new residual rounded to FP32 in r32
+// (the input of a single-precision preconditioner; single problem).
+template <int NT>
+__global__ __launch_bounds__(NT) void update_xr_r32(double* __restrict__ x, double* __restrict__ r,
+                                                   float* __restrict__ r32, const double* __restrict__ p,
+                                                   const double* __restrict__ Ap, const Scalars* sc, size_t n,
+                                                   double* partial) {
+    if (sc->done) return;
+    const double a = sc->alpha;
+    double acc = 0.0;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) {
+        x[k] += a * p[k];
+        const double rv = r[k] - a * Ap[k];
+        r[k] = rv;
+        r32[k] = (float)rv;
+        acc += rv * rv;
+    }
+    const double s = block_sum<double, NT>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// p = z + beta p; z may be stored in a narrower type Z than p.
+template <typename T, int NT, typename Z = T>
+__global__ __launch_bounds__(NT) void update_p(T* __restrict__ p, const Z* __restrict__ r, const Scalars* scb,
                                               size_t n) {
     const int q = blockIdx.y;
     const Scalars* sc = scb + q;
@@ -304,7 +327,7 @@ __global__ __launch_bounds__(NT) void update_p(T* __restrict__ p, const T* __res
     r += q * n;
     const T b = (T)sc->beta;
     const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) p[k] = r[k] + b * p[k];
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) p[k] = (T)r[k] + b * p[k];
 }
 
 // The update x += alpha_K p_K still pending after the last iteration K (the
@@ -326,15 +349,16 @@ __global__ __launch_bounds__(NT) void flush_x(T* __restrict__ x, const T* __rest
 }
 
 // block partials of u'v (v may equal u).
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void dot_partial(const T* __restrict__ u, const T* __restrict__ v, size_t n,
+// v may be stored in a narrower type V than u (the sum is in T).
+template <typename T, int NT, typename V = T>
+__global__ __launch_bounds__(NT) void dot_partial(const T* __restrict__ u, const V* __restrict__ v, size_t n,
                                                  double* partial) {
     const int q = blockIdx.y;
     u += q * n;
     v += q * n;
     T acc = T(0);
     const size_t stride = (size_t)gridDim.x * NT;
-    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) acc += u[k] * v[k];
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) acc += u[k] * (T)v[k];
     T s = block_sum<T, NT>(acc);
     if (threadIdx.x == 0) partial[(size_t)q * gridDim.x + blockIdx.x] = (double)s;
 }
