@@ -194,6 +194,23 @@ def test_var_mgpcg_fp32_vcycle(SP, p, level):
     assert r2.iters == r32.iters and relmax(x2.cpu().numpy(), x32.cpu().numpy()) < 1e-13
 
 
+def test_var_mgpcg_fp32_vcycle_iteration_cap(SP):
+    """With the V-cycle in FP32 the solve runs at most
+    inner_maxiter * max(max_outer, 1) iterations and reports not-converged
+    when the tolerance is out of reach."""
+    from sc_2022_amd import _capi as capi
+    N = 129
+    bx, by = profiles(7, 1)
+    h = SP(N, 4, coef=(bx, by), inner_maxiter=2)
+    b = T(np.random.default_rng(3).standard_normal(N * N))
+    tol = 1e-15 * float(torch.linalg.vector_norm(b))
+    x, res = h.mgpcg_mixed(b, abstol=tol, max_outer=3, raise_on_fail=False)
+    assert res.status == capi.SBP_ERR_NOT_CONVERGED
+    assert res.iters == 6 and res.inner_iters == 6
+    x, res = h.mgpcg_mixed(b, abstol=tol, max_outer=0, raise_on_fail=False)
+    assert res.status == capi.SBP_ERR_NOT_CONVERGED and res.iters == 2
+
+
 def test_var_handle_rejects_what_it_does_not_serve(SP):
     from sc_2022_amd import _capi as capi
     N = 65
