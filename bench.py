@@ -5,8 +5,8 @@ Workload (one "step"): one mixed-precision MG-PCG solve (FP32 inner
 MG-preconditioned CG inside FP64 iterative refinement) of the 4097 x 4097,
 SBP order 4 problem with a seeded random Fourier-mode source (64 modes) and
 boundary data (8 modes per face), from x0 = 0 to a relative residual that
-gives a relative H-norm error <= 1e-10 against a tightly converged FP64
-reference solution.  The same problem is also solved with the plain FP64
+gives a relative H-norm error <= 1e-10 against the FP64 oracle's solution
+(the C restatement in oracle/, MG-PCG to 1e-14 on the host cores).  The same problem is also solved with the plain FP64
 MG-PCG (reported beside it), and the matrix-free apply is timed alone with
 rotating buffers larger than the 256 MB Infinity Cache for the roofline.
 
@@ -67,6 +67,33 @@ def fourier_problem(h, seed):
     """b of the seeded Fourier-mode problem, generated on the device."""
     from sc_2022_amd.data import fourier_rhs
     return fourier_rhs(h, seed)
+
+
+def oracle_reference(h, b, p, seed, torch, keep_b=None):
+    """The FP64 oracle's solution of the seeded Fourier problem, against
+    which every headline error is measured: the C oracle
+    (oracle/sbp_oracle.c, an independent CPU restatement) builds the
+    right-hand side on the host from the same mode list and solves it by
+    MG-PCG to ||r|| < 1e-14 ||b|| on the host cores.  Returns the solution on
+    the device, its H-norm, and what was checked (host vs device right-hand
+    side, oracle iterations and residual, host seconds)."""
+    from oracle import c_oracle
+    N = h.N
+    c_oracle.set_threads(min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    bh = c_oracle.fourier_rhs(p, N, seed)
+    x, it, rr = c_oracle.solve_reference(p, N, bh)
+    secs = time.perf_counter() - t0
+    db = float(np.linalg.norm(b.cpu().numpy() - bh) / np.linalg.norm(bh))
+    if db > 1e-13:
+        raise RuntimeError(f"device and oracle right-hand sides differ: {db:.3g}")
+    xd = torch.as_tensor(x, device=h.device)
+    out = {"x": xd, "hnorm": h.hnorm(xd),
+           "what": f"C oracle FP64 MG-PCG to ||r|| < 1e-14 ||b|| on the host ({N}^2, p={p})",
+           "oracle_iters": it, "oracle_rel_resid": rr, "rhs_rel_diff": db, "host_s": secs}
+    if keep_b is not None:
+        keep_b.append(bh)
+    return out
 
 
 def vcycle_bytes(sides, es, fuse_n=1025):
@@ -239,7 +266,7 @@ def leg_8193(a, dev, torch):
     with the loosest rule that meets rel err <= 1e-10 at this size
     (bench.py --n 8193 --tol-sweep): FP64 ||r|| < rtol ||b||, mixed
     ||r|| < mixed_rtol ||b|| with inner FP32 solves to 3e-6; errors against
-    the FP64 solve to 1e-14."""
+    the FP64 oracle's solution (oracle_reference)."""
     from sc_2022_amd.data import fourier_rhs
     from sc_2022_amd.solver import SbpPoisson
     N = 8193
@@ -247,8 +274,8 @@ def leg_8193(a, dev, torch):
     h = SbpPoisson(N, a.p, device=dev, ir_rtol=a.ir_rtol, inner_rtol=inner)
     b = fourier_rhs(h, a.seed)
     bn = float(torch.linalg.vector_norm(b))
-    xref, _ = h.mgpcg(b, abstol=1e-14 * bn, maxiter=200, raise_on_fail=False)
-    href = h.hnorm(xref)
+    ref = oracle_reference(h, b, a.p, a.seed, torch)
+    xref, href = ref.pop("x"), ref.pop("hnorm")
     x = torch.zeros_like(b)
     out = {}
     for leg in ("mixed", "f64"):
@@ -268,6 +295,7 @@ def leg_8193(a, dev, torch):
                     "iters": r.iters, "inner": r.inner_iters}
     out["speedup_mixed_vs_f64"] = out["f64"]["ms"] / out["mixed"]["ms"]
     out["rules"] = {"mixed_rtol": a.mixed_rtol, "inner_rtol": inner, "f64_rtol": a.rtol}
+    out["rel_err_reference"] = ref
     out["meets_rel_err_target"] = bool(max(out["mixed"]["rel_err"], out["f64"]["rel_err"]) <= REL_ERR_TARGET)
     h.close()
     return out
@@ -419,21 +447,25 @@ def variable_coefficient_leg(a, dev, torch, N=4097, p=4):
                                   "speedup": solve_ms / mixed_ms}}
 
 
-def adjoint_leg(a, dev, torch, N=2049, p=4):
+def adjoint_leg(a, dev, torch, N=2049, p=4, ndir=32, eps=1e-6):
     """The differentiation workload: L = 1/2 (u - u_obs)' H~ (u - u_obs) at
-    N^2, u = A^-1 b(f, g) with seeded Fourier-mode f and g, u_obs the solution
-    of another seeded Fourier problem plus N(0, 1e-3) noise.  Times the
-    forward FP64 MG-PCG and the gradient (adjoint MG-PCG and the transposed
-    right-hand-side map), both to 1e-13 relative residual."""
+    N^2, u = A^-1 b(f, g) with seeded Fourier-mode f and g, u_obs a seeded
+    Fourier field (the source-type field of seed + 1) plus N(0, 1e-3) noise.
+    Times the forward FP64 MG-PCG and the gradient (adjoint MG-PCG and the
+    transposed right-hand-side map), both to 1e-13 relative residual; then
+    checks the gradient against the C oracle's FP64 gradient (its own
+    right-hand side, forward and adjoint MG-PCG on the host) and against
+    central differences (step eps, solves to 1e-14) along ndir seeded
+    directions in (f, g)."""
+    from oracle import c_oracle
     from sc_2022_amd.data import fourier_fields
     from sc_2022_amd.solver import SbpPoisson
     h = SbpPoisson(N, p, device=dev)
     f, gW, gE, gS, gN = fourier_fields(N, a.seed, dev)
     b = h.assemble_rhs(f, gW, gE, gS, gN)
-    bo = h.assemble_rhs(*fourier_fields(N, a.seed + 1, dev))
-    uo, _ = h.mgpcg(bo, abstol=1e-13 * float(torch.linalg.vector_norm(bo)), maxiter=200)
-    g = torch.Generator(device=dev).manual_seed(a.seed)
-    u_obs = uo + 1e-3 * torch.randn(uo.shape, dtype=torch.float64, device=dev, generator=g)
+    c_oracle.set_threads(min(16, os.cpu_count() or 1))
+    uo_h = c_oracle.fourier_fields(N, a.seed + 1)[0] + 1e-3 * np.random.default_rng(a.seed).standard_normal(N * N)
+    u_obs = torch.as_tensor(uo_h, device=dev)
     bn = float(torch.linalg.vector_norm(b))
     u = torch.zeros_like(b)
 
@@ -459,11 +491,40 @@ def adjoint_leg(a, dev, torch, N=2049, p=4):
         L, lam, df, dW, dE, dS, dN, it = gradient()
     torch.cuda.synchronize()
     t_g = (time.perf_counter() - t0) / reps
+    grads = (df, dW, dE, dS, dN)
+    # the C oracle's gradient, end to end on the host
+    t0 = time.perf_counter()
+    uh, _, _ = c_oracle.solve_reference(p, N, c_oracle.fourier_rhs(p, N, a.seed))
+    Lo, _, *go, it_o = c_oracle.adjoint_grad(p, N, uh, uo_h, rtol=1e-14)
+    t_o = time.perf_counter() - t0
+    orc_err = max(float(np.abs(g.cpu().numpy() - r).max() / np.abs(r).max()) for g, r in zip(grads, go))
+    # central differences along seeded directions
+    def loss(fv, gv):
+        bb = h.assemble_rhs(fv, *gv)
+        uu, _ = h.mgpcg(bb, abstol=1e-14 * float(torch.linalg.vector_norm(bb)), maxiter=200)
+        return 0.5 * h.hnorm(uu - u_obs) ** 2
+
+    gen = torch.Generator(device=dev).manual_seed(123)
+    fg = (f, gW, gE, gS, gN)
+    fd_err = 0.0
+    for _ in range(ndir):
+        d = [torch.randn(t.numel(), dtype=torch.float64, device=dev, generator=gen) for t in fg]
+        d = [di * float(torch.linalg.vector_norm(t)) / float(torch.linalg.vector_norm(di)) for di, t in zip(d, fg)]
+        Lp = loss(f + eps * d[0], [g + eps * dg for g, dg in zip(fg[1:], d[1:])])
+        Lm = loss(f - eps * d[0], [g - eps * dg for g, dg in zip(fg[1:], d[1:])])
+        fd = (Lp - Lm) / (2 * eps)
+        ad = sum(float(gr @ di) for gr, di in zip(grads, d))
+        fd_err = max(fd_err, abs(fd - ad) / abs(ad))
     h.close()
-    return {"workload": f"N={N}, p={p}, L = 1/2 |u - u_obs|_H^2, seeded Fourier f, g and u_obs (+ N(0, 1e-3))",
+    return {"workload": f"N={N}, p={p}, L = 1/2 |u - u_obs|_H^2, seeded Fourier f and g; u_obs a seeded Fourier "
+                        f"field + N(0, 1e-3)",
             "forward_ms": t_f * 1e3, "forward_iters": rf.iters, "gradient_ms": t_g * 1e3, "adjoint_iters": it,
-            "loss": L, "grad_f_norm": float(torch.linalg.vector_norm(df)),
-            "grad_g_norm": float(sum(torch.linalg.vector_norm(v) ** 2 for v in (dW, dE, dS, dN)) ** 0.5)}
+            "loss": L, "loss_oracle": Lo, "grad_f_norm": float(torch.linalg.vector_norm(df)),
+            "grad_g_norm": float(sum(torch.linalg.vector_norm(v) ** 2 for v in (dW, dE, dS, dN)) ** 0.5),
+            "oracle_rel_err": orc_err,
+            "oracle": f"C oracle FP64 gradient (forward and adjoint MG-PCG to 1e-14, {it_o} adjoint iterations, "
+                      f"{t_o:.1f} s on the host)",
+            "fd_max_rel_err": fd_err, "fd_directions": ndir, "fd_step": eps}
 
 
 def tol_sweep(a, h, b, x, bnorm, rel_err, torch):
@@ -518,14 +579,14 @@ def host_cpu():
     return {"model": model, "cpus_usable": usable, "cpus_online": os.cpu_count()}
 
 
-def cpu_baseline(N, p, seed, threads, rtol, runs=5, cg_iters=10):
+def cpu_baseline(N, p, seed, threads, rtol, runs=5, cg_iters=10, b=None):
     """The C oracle's FP64 MG-PCG (same V-cycle, OpenMP) on the host cores:
     one warm-up and `runs` timed solves of the same problem (median), plus
     the reference's own CPU path -- CSR assembly and CG_CPU, one thread --
     over a bounded number of iterations."""
     from oracle import c_oracle
     cores = threads or min(16, os.cpu_count() or 1)
-    b = c_oracle.fourier_rhs(p, N, seed)
+    b = c_oracle.fourier_rhs(p, N, seed) if b is None else b
     res = c_oracle.time_mgpcg_fourier(N, p, seed, rtol, cores, runs=runs, warmup=1, b=b)
     cg = c_oracle.time_cg_csr_sample(N, p, b, cg_iters, threads=1)
     cgm = c_oracle.time_cg_csr_sample(N, p, b, cg_iters, threads=cores)
@@ -603,14 +664,20 @@ def main():
                           "copy_GBs": copy_gbs}), flush=True)
         return
 
-    # ---- problem and FP64 reference solution (untimed) ----
+    # ---- problem and the FP64 oracle's reference solution (untimed) ----
     b = fourier_problem(h, a.seed)
     bnorm = float(torch.linalg.vector_norm(b))
-    xref, rref = h.mgpcg(b, abstol=1e-14 * bnorm, maxiter=200, raise_on_fail=False)
-    href = h.hnorm(xref)
+    ref_b_host = []
+    ref = oracle_reference(h, b, p, a.seed, torch, keep_b=ref_b_host)
+    xref, href = ref.pop("x"), ref.pop("hnorm")
 
     def rel_err(x):
         return h.hnorm(x - xref) / href
+
+    # the GPU's own tightly converged FP64 solve, against the oracle's
+    xg, rg = h.mgpcg(b, abstol=1e-14 * bnorm, maxiter=200, raise_on_fail=False)
+    ref["gpu_f64_1e-14"] = {"status": rg.status, "rel_resid": rg.resnorm / bnorm, "rel_err_vs_oracle": rel_err(xg)}
+    del xg
 
     x = torch.zeros_like(b)
     if a.tol_sweep:
@@ -674,7 +741,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         try:
-            cpu = cpu_baseline(N, p, a.seed, a.cpu_threads, a.rtol)
+            cpu = cpu_baseline(N, p, a.seed, a.cpu_threads, a.rtol, b=ref_b_host[0])
         except Exception as e:  # reported, never silently replaced
             cpu = {"error": f"{type(e).__name__}: {e}"}
 
@@ -699,6 +766,7 @@ def main():
                    "seed": a.seed},
         "rel_err_mixed": errmx,
         "rel_err_f64": err64,
+        "rel_err_reference": ref,
         "meets_rel_err_target": bool(errmx <= REL_ERR_TARGET and err64 <= REL_ERR_TARGET),
         "mixed": {"solves_per_s": 1.0 / tmx, "outer": rmx.iters, "inner": rmx.inner_iters,
                   "ms": tmx * 1e3},

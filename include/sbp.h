@@ -89,6 +89,12 @@ typedef struct {
                                 handle's problems number at most fuse_n^2 run their pre-smoothing
                                 + residual + restriction, and their prolongation + post-smoothing,
                                 as one temporally blocked kernel each (default 1025, 0 = off) */
+    int drop_coarse_rhs;     /* 1: on every level below the finest, the residual restricted to
+                                the next coarser level is -A v, without that level's right-hand
+                                side -- the reference's three-level cycle
+                                (matrix_free_Three_level_multigrid, initial_guess_cg.jl:281-283);
+                                such handles run the separate passes (no fused or bottom
+                                kernels).  0 (default): the standard residual rhs - A v. */
 } sbp_params;
 
 typedef struct sbp_handle_s* sbp_handle;

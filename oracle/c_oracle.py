@@ -32,6 +32,7 @@ def lib():
         ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_apply.argtypes = [C.c_int, C.c_int, dp, dp]
+        L.orc_apply_plain.argtypes = [C.c_int, C.c_int, dp, dp]
         L.orc_diag.argtypes = [C.c_int, C.c_int, dp]
         L.orc_csr.restype = C.c_long
         L.orc_csr.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -39,6 +40,13 @@ def lib():
         L.orc_cg_csr.argtypes = [C.c_long, ip, ip, dp, dp, dp, C.c_double, C.c_int, C.POINTER(C.c_double)]
         L.orc_mgpcg.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
                                 C.c_double, C.POINTER(C.c_double)]
+        L.orc_hnorm.restype = C.c_double
+        L.orc_hnorm.argtypes = [C.c_int, C.c_int, dp]
+        L.orc_rhs_transpose.argtypes = [C.c_int, C.c_int, dp] + [C.c_void_p] * 5
+        L.orc_adjoint_grad.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_int, C.POINTER(C.c_double)] + \
+            [C.c_void_p] * 6
+        L.orc_fourier_field.argtypes = [C.c_int, C.c_int, dp, dp]
+        L.orc_fourier_edge.argtypes = [C.c_int, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -50,6 +58,14 @@ def set_threads(t):
 def apply(p, N, u):
     out = np.empty(N * N)
     assert lib().orc_apply(p, N, np.ascontiguousarray(u, dtype=np.float64), out) == 0
+    return out
+
+
+def apply_plain(p, N, u):
+    """A u with each row a plain sum of the rounded FP64 entries (the
+    evaluation the oracle's apply avoids; kept to show the difference)."""
+    out = np.empty(N * N)
+    assert lib().orc_apply_plain(p, N, np.ascontiguousarray(u, dtype=np.float64), out) == 0
     return out
 
 
@@ -98,23 +114,71 @@ def mgpcg(p, N, b, abstol, maxiter=200, nu_pre=2, nu_post=2, omega=2.0 / 3.0, rs
     return x, it, rn.value
 
 
+def hnorm(p, N, v):
+    """sqrt(v' H~ v) on the N x N grid of order p."""
+    r = lib().orc_hnorm(p, N, np.ascontiguousarray(v, dtype=np.float64))
+    assert r >= 0.0
+    return r
+
+
+def rhs_transpose(p, N, lam):
+    """Adjoint of ``rhs`` with respect to (f, gW, gE, gS, gN) at lam."""
+    df = np.empty(N * N)
+    g = [np.empty(N) for _ in range(4)]
+    assert lib().orc_rhs_transpose(p, N, np.ascontiguousarray(lam, dtype=np.float64), df.ctypes.data,
+                                   *[v.ctypes.data for v in g]) == 0
+    return (df, *g)
+
+
+def adjoint_grad(p, N, u, u_obs, abstol=None, maxiter=200, rtol=1e-14):
+    """FP64 gradient of L = 1/2 (u - u_obs)' H~ (u - u_obs), u = A^-1 b(f, g):
+    (L, lam, df, dgW, dgE, dgS, dgN, iterations of the adjoint MG-PCG).  The
+    adjoint solve stops at ||r|| < abstol, or (abstol None) at
+    rtol ||H~ (u - u_obs)||."""
+    if abstol is None:
+        abstol = -rtol
+    lam = np.empty(N * N)
+    df = np.empty(N * N)
+    g = [np.empty(N) for _ in range(4)]
+    loss = C.c_double()
+    it = lib().orc_adjoint_grad(p, N, np.ascontiguousarray(u, dtype=np.float64),
+                                np.ascontiguousarray(u_obs, dtype=np.float64), abstol, maxiter, C.byref(loss),
+                                lam.ctypes.data, df.ctypes.data, *[v.ctypes.data for v in g])
+    if it < 0:
+        raise ValueError("orc_adjoint_grad: unsupported size")
+    return (loss.value, lam, df, *g, it)
+
+
+def fourier_fields(N, seed):
+    """(f, gW, gE, gS, gN) of the seeded Fourier-mode problem (mode list of
+    sbp_oracle.fourier_modes), evaluated mode by mode in C."""
+    from .sbp_oracle import fourier_modes
+    src, bnd = fourier_modes(seed)
+    F = np.empty(N * N)
+    lib().orc_fourier_field(N, len(src), np.ascontiguousarray(np.array(src, dtype=np.float64).ravel()), F)
+    g = []
+    for modes in bnd:
+        v = np.empty(N)
+        lib().orc_fourier_edge(N, len(modes), np.ascontiguousarray(np.array(modes, dtype=np.float64).ravel()), v)
+        g.append(v)
+    return (F, *g)
+
+
 def fourier_rhs(p, N, seed):
     """Right-hand side of the seeded Fourier-mode problem (same mode list as
     sbp_oracle.fourier_modes), built on the host."""
-    from .sbp_oracle import fourier_modes
-    src, bnd = fourier_modes(seed)
-    t = np.linspace(0.0, 1.0, N)
-    F = np.zeros((N, N))            # F[j, i]
-    xj = t[:, None]; yi = t[None, :]
-    for kx, ky, a, ph in src:
-        F += a * np.cos(np.pi * (kx * xj + ky * yi) + ph)
-    g = []
-    for modes in bnd:
-        v = np.zeros(N)
-        for kk, a, ph in modes:
-            v += a * np.cos(np.pi * kk * t + ph)
-        g.append(v)
-    return rhs(p, N, F.reshape(-1), *g)
+    return rhs(p, N, *fourier_fields(N, seed))
+
+
+def solve_reference(p, N, b, rtol=1e-14, maxiter=100):
+    """FP64 reference solution x = A^-1 b by the C oracle's MG-PCG to
+    ||r|| < rtol ||b|| (the headline errors are measured against it);
+    raises if it does not get there."""
+    bn = float(np.linalg.norm(b))
+    x, it, rn = mgpcg(p, N, b, rtol * bn, maxiter=maxiter)
+    if not rn < rtol * bn:
+        raise RuntimeError(f"oracle MG-PCG stopped at ||r||/||b|| = {rn / bn:.3g} after {it} iterations")
+    return x, it, rn / bn
 
 
 def time_mgpcg_fourier(N, p, seed, rtol, threads, runs=1, warmup=0, b=None):

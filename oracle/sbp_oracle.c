@@ -15,7 +15,13 @@
  *     of damped-Jacobi sweeps (jacobi_brittany! :335-344), full weighting
  *     times a scale (restriction_matrix_normal :220-233), bilinear
  *     prolongation (prolongation_matrix :190-203) and an exact coarse solve
- *     (dense Cholesky), as the GPU library builds it.
+ *     (dense Cholesky), as the GPU library builds it;
+ *   - the transpose of the right-hand-side map (the adjoint of :164-167,181,185
+ *     with respect to f and the four boundary data vectors) and, from it, the
+ *     FP64 gradient of L = 1/2 (u - u_obs)' H~ (u - u_obs) with u = A^-1 b(f, g)
+ *     (adjoint MG-PCG with the same V-cycle; A is symmetric);
+ *   - the seeded Fourier-mode data fields (sum of cosine modes; the mode list
+ *     is the caller's) by direct evaluation, and the H~ norm.
  *
  * Pinned in tests/test_c_oracle.py against the numpy/scipy oracle
  * (oracle/sbp_oracle.py), itself pinned in tests/test_oracle.py.
@@ -35,6 +41,7 @@ typedef struct {
     int *mc0, *kc0;  /* first column of row r's band  (N)      */
     int *mw, *kw;    /* band width of row r           (N)      */
     double *mv, *kv; /* band values, 8 slots per row  (8 N)    */
+    double *mrs, *krs; /* exact row sums of m and k (N): 0 except the Dirichlet SAT rows of k */
 } op1d;
 
 static int tables(int p, int *bm, int *bn, int *nbs, double *bhinv, double *d, double bd[4][6], double *bs) {
@@ -65,6 +72,7 @@ static int tables(int p, int *bm, int *bn, int *nbs, double *bhinv, double *d, d
 
 static void op1d_free(op1d *o) {
     free(o->hw); free(o->mc0); free(o->kc0); free(o->mw); free(o->kw); free(o->mv); free(o->kv);
+    free(o->mrs); free(o->krs);
 }
 
 /* Dense row r of the dimensionless D2 (times h^2) and BS (times h), placed in
@@ -79,6 +87,7 @@ static int op1d_build(int p, int N, op1d *o) {
     o->mc0 = malloc(sizeof(int) * N); o->kc0 = malloc(sizeof(int) * N);
     o->mw = malloc(sizeof(int) * N); o->kw = malloc(sizeof(int) * N);
     o->mv = calloc(8 * (size_t)N, sizeof(double)); o->kv = calloc(8 * (size_t)N, sizeof(double));
+    o->mrs = calloc(N, sizeof(double)); o->krs = calloc(N, sizeof(double));
     for (int r = 0; r < N; ++r) o->hw[r] = 1.0;
     for (int r = 0; r < bm; ++r) { o->hw[r] = 1.0 / bhinv[r]; o->hw[N - 1 - r] = 1.0 / bhinv[r]; }
     const int half = p / 2;
@@ -107,6 +116,14 @@ static int op1d_build(int p, int N, op1d *o) {
         if (r == N - 1) { for (int c = 0; c < nbs; ++c) kwin[(N - 1 - c) - c0] -= bs[c]; kwin[(N - 1) - c0] += 13.0; }
         if (r < nbs) kwin[0 - c0] -= bs[r];
         if (N - 1 - r < nbs) kwin[(N - 1) - c0] -= bs[N - 1 - r];
+        /* Exact row sums (from the rational tables, not from the rounded
+         * entries): D2 1 = 0 and BS 1 = 0 give m 1 = 0; k adds the Dirichlet
+         * SAT, whose rows sum to 13 delta_r0 - bs[r] (r < nbs, mirrored). */
+        o->mrs[r] = 0.0;
+        o->krs[r] = 0.0;
+        if (r == 0 || r == N - 1) o->krs[r] += 13.0;
+        if (r < nbs) o->krs[r] -= bs[r];
+        if (N - 1 - r < nbs) o->krs[r] -= bs[N - 1 - r];
         o->mc0[r] = o->kc0[r] = c0;
         o->mw[r] = o->kw[r] = 8;
         for (int c = 0; c < 8; ++c) { o->mv[8 * (size_t)r + c] = win[c]; o->kv[8 * (size_t)r + c] = kwin[c]; }
@@ -121,8 +138,37 @@ static double dot_(const double *a, const double *b, size_t n) {
     return s;
 }
 
-/* out = A u (matrix-free, tensor form). */
+/* out = A u (matrix-free, tensor form).  Each 1-D row is evaluated as
+ *     rowsum_r u_r + sum_{c != r} a_rc (u_c - u_r)
+ * with the exact row sum of the rational operator.  This is the same
+ * operator, but it keeps the zero row sums of diagonal_sbp.jl's D2 and BS
+ * exactly: the rounded FP64 entries themselves do not sum to zero (p = 4
+ * interior: 1/12 - 4/3 + 5/2 - 4/3 + 1/12 = 1.4e-16 in FP64), and a plain
+ * sum of them acts as a diagonal shift of that size, which moves the
+ * solution of A x = b by about 1.4e-16 / lambda_min(A) -- 2e-10..4e-10
+ * relative at 4097^2.  apply_plain_ keeps the plain sum for the test that
+ * shows the shift. */
 static void apply_(const op1d *o, const double *u, double *out) {
+    const int N = o->N;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < N; ++j) {
+        const double *kv = o->kv + 8 * (size_t)j;
+        const int kc0 = o->kc0[j];
+        for (int i = 0; i < N; ++i) {
+            const double *mv = o->mv + 8 * (size_t)i;
+            const int mc0 = o->mc0[i];
+            const double uc = u[i + (size_t)j * N];
+            double ks = o->krs[j] * uc, ms = o->mrs[i] * uc;
+            for (int c = 0; c < 8; ++c) {
+                if (kv[c] != 0.0 && kc0 + c != j) ks += kv[c] * (u[i + (size_t)(kc0 + c) * N] - uc);
+                if (mv[c] != 0.0 && mc0 + c != i) ms += mv[c] * (u[(mc0 + c) + (size_t)j * N] - uc);
+            }
+            out[i + (size_t)j * N] = o->hw[i] * ks + o->hw[j] * ms;
+        }
+    }
+}
+
+static void apply_plain_(const op1d *o, const double *u, double *out) {
     const int N = o->N;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < N; ++j) {
@@ -160,6 +206,16 @@ int orc_apply(int p, int N, const double *u, double *out) {
     int rc = op1d_build(p, N, &o);
     if (rc) return rc;
     apply_(&o, u, out);
+    op1d_free(&o);
+    return 0;
+}
+
+/* The plain-sum evaluation of the same entries (see apply_). */
+int orc_apply_plain(int p, int N, const double *u, double *out) {
+    op1d o;
+    int rc = op1d_build(p, N, &o);
+    if (rc) return rc;
+    apply_plain_(&o, u, out);
     op1d_free(&o);
     return 0;
 }
@@ -460,4 +516,132 @@ int orc_mgpcg(int p, int N, const double *b, double *x, double abstol, int maxit
     free(r); free(z); free(pv); free(Ap);
     mg_free(&g);
     return it;
+}
+
+/* ---- H~ norm and the transposed right-hand-side map ---------------------- */
+/* sqrt(v' H~ v), H~ = h^2 hw_i hw_j (matrix_free_mg_cg.jl:464). */
+double orc_hnorm(int p, int N, const double *v) {
+    op1d o;
+    if (op1d_build(p, N, &o)) return -1.0;
+    const double h = 1.0 / (N - 1);
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int j = 0; j < N; ++j) {
+        double sj = 0.0;
+        for (int i = 0; i < N; ++i) {
+            const double x = v[i + (size_t)j * N];
+            sj += o.hw[i] * x * x;
+        }
+        s += o.hw[j] * sj;
+    }
+    op1d_free(&o);
+    return h * sqrt(s);
+}
+
+/* Adjoint of orc_rhs: with b = -H~ (f + sum_X SAT_X_r g_X),
+ *   df = -H~ lam,   dg_X = -(H~ SAT_X_r)' lam.
+ * Row by row from the entries orc_rhs adds (Dirichlet W/E: the columns
+ * j < nbs from each face carry HI_x (tau_D E + beta BS^T E) e; Neumann S/N:
+ * the rows i = 0 / N-1 carry HI_y tau_N).  Any output may be NULL. */
+int orc_rhs_transpose(int p, int N, const double *lam, double *df, double *dgW, double *dgE, double *dgS,
+                      double *dgN) {
+    int bm, bn, nbs;
+    double bhinv[4], d[5], bd[4][6], bs[4];
+    if (tables(p, &bm, &bn, &nbs, bhinv, d, bd, bs)) return -1;
+    op1d o;
+    if (op1d_build(p, N, &o)) return -2;
+    const double h = 1.0 / (N - 1), tauD = -13.0 / h, tauN = -1.0;
+    if (df) {
+#pragma omp parallel for schedule(static)
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i)
+                df[i + (size_t)j * N] = -(h * o.hw[i]) * (h * o.hw[j]) * lam[i + (size_t)j * N];
+    }
+    /* b(i,j) gets -(h hw_i)(h hw_j) (1/(h hw_j)) c_j g_W(i) = -(h hw_i) c_j g_W(i) */
+    for (int i = 0; i < N; ++i) {
+        double sW = 0.0, sE = 0.0;
+        for (int k = 0; k < nbs; ++k) {
+            const double c = (k == 0 ? tauD : 0.0) + bs[k] / h;
+            sW += c * lam[i + (size_t)k * N];
+            sE += c * lam[i + (size_t)(N - 1 - k) * N];
+        }
+        if (dgW) dgW[i] = -(h * o.hw[i]) * sW;
+        if (dgE) dgE[i] = -(h * o.hw[i]) * sE;
+    }
+    /* b(0,j) gets -(h hw_0)(h hw_j) (1/(h hw_0)) tau_N g_S(j) = -(h hw_j) tau_N g_S(j) */
+    for (int j = 0; j < N; ++j) {
+        if (dgS) dgS[j] = -(h * o.hw[j]) * tauN * lam[0 + (size_t)j * N];
+        if (dgN) dgN[j] = -(h * o.hw[j]) * tauN * lam[(N - 1) + (size_t)j * N];
+    }
+    op1d_free(&o);
+    return 0;
+}
+
+/* Gradient of L = 1/2 (u - u_obs)' H~ (u - u_obs) for u = A^-1 b(f, g):
+ * lam = A^-1 H~ (u - u_obs) by MG-PCG from zero to ||r||_2 < abstol (a
+ * negative abstol: -abstol times ||H~ (u - u_obs)||_2) (2 + 2
+ * damped Jacobi, full weighting times 4, 17^2 Cholesky), then the transposed
+ * right-hand-side map.  loss / lam / gradients nullable.  Returns the MG-PCG
+ * iterations (< 0 on error). */
+int orc_adjoint_grad(int p, int N, const double *u, const double *u_obs, double abstol, int maxiter, double *loss,
+                     double *lam, double *df, double *dgW, double *dgE, double *dgS, double *dgN) {
+    op1d o;
+    if (op1d_build(p, N, &o)) return -2;
+    const size_t n = (size_t)N * N;
+    const double h = 1.0 / (N - 1);
+    double *w = malloc(n * sizeof(double));
+    double *l = lam ? lam : malloc(n * sizeof(double));
+    double L = 0.0;
+#pragma omp parallel for reduction(+ : L) schedule(static)
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) {
+            const size_t q = i + (size_t)j * N;
+            const double e = u[q] - u_obs[q];
+            const double hq = (h * o.hw[i]) * (h * o.hw[j]);
+            w[q] = hq * e;
+            L += 0.5 * hq * e * e;
+        }
+    op1d_free(&o);
+    if (abstol < 0.0) abstol = -abstol * sqrt(dot_(w, w, n));   /* relative to ||H~ (u - u_obs)|| */
+    memset(l, 0, n * sizeof(double));
+    double rn = 0.0;
+    const int it = orc_mgpcg(p, N, w, l, abstol, maxiter, 2, 2, 2.0 / 3.0, 4.0, &rn);
+    if (it >= 0) orc_rhs_transpose(p, N, l, df, dgW, dgE, dgS, dgN);
+    if (loss) *loss = L;
+    free(w);
+    if (!lam) free(l);
+    return it;
+}
+
+/* ---- seeded Fourier-mode fields ------------------------------------------- */
+static const double kPi = 3.14159265358979323846;
+
+/* F(i, j) = sum_m a_m cos(pi (kx_m x_j + ky_m y_i) + ph_m) with x_j = j/(N-1)
+ * (slow index) and y_i = i/(N-1) (fast index); modes = nm rows (kx, ky, a, ph).
+ * g(k) = sum_m a_m cos(pi k_m t_k + ph_m); bmodes = nb rows (k, a, ph). */
+void orc_fourier_field(int N, int nm, const double *modes, double *F) {
+    const double h = 1.0 / (N - 1);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < N; ++j) {
+        const double x = j == N - 1 ? 1.0 : j * h;
+        for (int i = 0; i < N; ++i) {
+            const double y = i == N - 1 ? 1.0 : i * h;
+            double s = 0.0;
+            for (int m = 0; m < nm; ++m) {
+                const double *md = modes + 4 * (size_t)m;
+                s += md[2] * cos(kPi * (md[0] * x + md[1] * y) + md[3]);
+            }
+            F[i + (size_t)j * N] = s;
+        }
+    }
+}
+
+void orc_fourier_edge(int N, int nb, const double *bmodes, double *g) {
+    const double h = 1.0 / (N - 1);
+    for (int k = 0; k < N; ++k) {
+        const double t = k == N - 1 ? 1.0 : k * h;
+        double s = 0.0;
+        for (int m = 0; m < nb; ++m) s += bmodes[3 * m + 1] * cos(kPi * bmodes[3 * m] * t + bmodes[3 * m + 2]);
+        g[k] = s;
+    }
 }

@@ -52,6 +52,7 @@ class SbpParams(C.Structure):
         ("ir_rtol", C.c_double),
         ("bottom_n", C.c_int),
         ("fuse_n", C.c_int),
+        ("drop_coarse_rhs", C.c_int),
     ]
 
 

@@ -247,7 +247,7 @@ int smooth_pass(sbp_handle h, int l, const IN& in, const T* b, T* dst, const T* 
 // levels; above that the redundant halo work of the fused blocks costs more
 // than the passes it saves).
 inline bool fuse_level(const sbp_handle h, int l) {
-    if (h->p > 6) return false;   // the two-sweep kernels are compiled for p = 2, 4 (and 6, 64-wide blocks)
+    if (h->p > 6 || h->prm.drop_coarse_rhs) return false;   // the two-sweep kernels are compiled for p = 2, 4 (and 6, 64-wide blocks)
     if (!fused_shape_ok(h->L[l].fti[0], h->L[l].ftj[0], h->p) || !fused_shape_ok(h->L[l].fti[1], h->L[l].ftj[1], h->p))
         return false;
     const long long pts = (long long)h->L[l].n * h->nprob;
@@ -390,7 +390,10 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
     }
     // Residual and full-weighting restriction, fused (the fine residual stays on chip).
     Level& Lc = h->L[l + 1];
-    if (!fused_pre) STCHK(launch_rr<T>(h, l, (const T*)cur, b, (T*)Lc.b[pi], scb));
+    // (drop_coarse_rhs: below the finest level the restricted residual is -A x,
+    // the reference's three-level cycle)
+    if (!fused_pre)
+        STCHK(launch_rr<T>(h, l, (const T*)cur, (l > 0 && h->prm.drop_coarse_rhs) ? nullptr : b, (T*)Lc.b[pi], scb));
     T* ec = (T*)Lc.xo[pi];
     STCHK(vcycle_level<T>(h, l + 1, (const T*)Lc.b[pi], ec, nullptr, scb, nullptr));
     // Prolongation folded into the first post-smoothing pass.
@@ -1056,7 +1059,7 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
     if (prm && (prm->nu_pre < 0 || prm->nu_post < 0 || prm->poll_every < 0 || prm->max_outer < 1 ||
                 prm->inner_maxiter < 1 || !(prm->omega > 0) || !(prm->restrict_scale > 0) ||
                 (prm->smoother != 0 && prm->smoother != 1) || prm->bottom_n < 0 ||
-                prm->fuse_n < 0))
+                prm->fuse_n < 0 || (prm->drop_coarse_rhs != 0 && prm->drop_coarse_rhs != 1)))
         return SBP_ERR_INVALID_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SBP_ERR_NO_DEVICE;
@@ -1092,7 +1095,7 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
     h->nlev = nlev;
     // Bottom kernel: from the first level of side <= bottom_n down, when the
     // coarsest system is small enough for one workgroup's direct solve.
-    if (h->mg && p <= 4 && h->L[nlev - 1].n <= (size_t)kBottomMaxCoarse) {
+    if (h->mg && p <= 4 && !h->prm.drop_coarse_rhs && h->L[nlev - 1].n <= (size_t)kBottomMaxCoarse) {
         const int bn = std::min(h->prm.bottom_n, kBottomN);
         for (int l = 0; l < nlev; ++l)
             if (h->L[l].N <= bn) { h->bottom_l0 = l; break; }

@@ -761,7 +761,8 @@ struct RestrictCtl {
     Batch bt;          // sf: fine elements, sc: coarse elements per problem
 };
 
-// rc = scale * FW(b - A u) on the coarse grid of side Nc = (N+1)/2.  A block
+// rc = scale * FW(b - A u) on the coarse grid of side Nc = (N+1)/2 (b null:
+// rc = scale * FW(-A u)).  A block
 // owns TI/2-1 coarse columns and J/2 coarse rows; its TI threads form the fine
 // residual on the TI x (J+1) fine points those coarse points weigh (one
 // column/row overlap with the neighbouring blocks), keep it in LDS and
@@ -779,7 +780,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
     const int q = ball / nb, blk = ball - q * nb;
     if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
     u += q * ctl.bt.sf;
-    b += q * ctl.bt.sf;
+    if (b) b += q * ctl.bt.sf;
     rc += q * ctl.bt.sc;
     const int bx = blk % ctl.nbx, by = blk / ctl.nbx;
     const int N = t.N, Nc = ctl.Nc;
@@ -817,7 +818,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
 #pragma unroll
         for (int r = 0; r < JR; ++r) {
             ut[r][tid + H] = w[r + H];
-            bq[r] = act ? b[gidx(fi, fj0 + r, N)] : T(0);
+            bq[r] = (act && b) ? b[gidx(fi, fj0 + r, N)] : T(0);
         }
         __syncthreads();
         const bool iint = iin || ((fi >= t.R) && (fi < N - t.R));
@@ -844,7 +845,8 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
         for (int r = 0; r < JR; ++r) {
             const int fj = fj0 + r;
             T v = T(0);
-            if (fi >= 0 && fi < N && fj >= 0 && fj < N) v = b[gidx(fi, fj, N)] - au_point<P, T, true>(tl, in, fi, fj);
+            if (fi >= 0 && fi < N && fj >= 0 && fj < N)
+                v = (b ? b[gidx(fi, fj, N)] : T(0)) - au_point<P, T, true>(tl, in, fi, fj);
             res[r] = v;
         }
     }

@@ -63,7 +63,9 @@ def matrix_free_prolongation_2d_GPU(idata, odata):
     """odata = P idata, bilinear, coarse (n, n) -> fine (2n-1, 2n-1)
     (initial_guess_cg.jl:41-77)."""
     Nc = _side(idata)
-    h = handle(2 * Nc - 1, 2, device=idata.device)
+    # a two-level handle whose coarse grid is Nc: any fine side whose coarse
+    # operator table builds (17 -> 9 and up), not only the default 17^2 bottom
+    h = handle(2 * Nc - 1, 2, device=idata.device, coarse_n=Nc)
     odata.zero_()
     h.prolong_add(idata.reshape(-1), odata.reshape(-1), level=0)
     return None
@@ -72,7 +74,8 @@ def matrix_free_prolongation_2d_GPU(idata, odata):
 def matrix_free_restriction_2d_GPU(idata, odata):
     """odata = full weighting [1 2 1;2 4 2;1 2 1]/16 of idata with zero padding
     (initial_guess_cg.jl:98-131), without the padded host copy."""
-    h = handle(_side(idata), 2, device=idata.device, restrict_scale=1.0)
+    Nf = _side(idata)
+    h = handle(Nf, 2, device=idata.device, restrict_scale=1.0, coarse_n=(Nf + 1) // 2)
     h.restrict(idata.reshape(-1), level=0, out=odata.reshape(-1))
     return None
 
@@ -93,12 +96,16 @@ def reference_mg_params(N, levels=2, nu=3, omega=0.15):
     """sbp_params of the reference's own preconditioner: nu Richardson sweeps
     (weight omega) before and after, full weighting without scaling, and an
     exact solve on the grid `levels - 1` halvings down
-    (initial_guess_cg.jl:134-180; levels = 3: :244-301)."""
+    (initial_guess_cg.jl:134-180).  levels = 3 is the three-level cycle as
+    the reference writes it (:244-301): the middle grid's residual is
+    restricted without that grid's right-hand side (:281-283,
+    drop_coarse_rhs); its aliased smoothing call at :296 is dead (the call
+    at :297 overwrites its output) and has no counterpart."""
     Nc = N
     for _ in range(levels - 1):
         Nc = (Nc + 1) // 2
     return dict(smoother=1, richardson_omega=float(omega), nu_pre=int(nu), nu_post=int(nu),
-                restrict_scale=1.0, coarse_n=Nc)
+                restrict_scale=1.0, coarse_n=Nc, drop_coarse_rhs=int(levels >= 3))
 
 
 def matrix_free_Two_level_multigrid(b, A_2h=None, nu=3, NUM_V_CYCLES=1, SBPp=2):
@@ -117,11 +124,11 @@ def matrix_free_Two_level_multigrid(b, A_2h=None, nu=3, NUM_V_CYCLES=1, SBPp=2):
 
 
 def matrix_free_Three_level_multigrid(b, A_4h=None, nu=3, NUM_V_CYCLES=1, SBPp=2):
-    """Three-level cycle (initial_guess_cg.jl:244-301) -> (z, ||A z - b||).
-    Difference: the reference forms the middle grid's residual without that
-    grid's right-hand side (r_2h = -A_2h v, :282) and repeats the last
-    smoothing call on an aliased argument (:295-296); this is the standard
-    three-level cycle with the same smoother, transfers and exact solve."""
+    """Three-level cycle as the reference writes it (initial_guess_cg.jl:244-301)
+    -> (z, ||A z - b||): Richardson smoothing, the middle grid's residual
+    restricted without its right-hand side (r_2h = -A_2h v, :282), exact
+    solve on the 4h grid.  A_4h is unused: the handle rediscretises and
+    inverts it.  The dead aliased smoothing call of :296 is not repeated."""
     if NUM_V_CYCLES != 1:
         raise ValueError("one cycle per call, as the reference's callers use it")
     N = _side(b)
@@ -135,16 +142,32 @@ def matrix_free_Three_level_multigrid(b, A_4h=None, nu=3, NUM_V_CYCLES=1, SBPp=2
 def matrix_free_MGCG(b, x, maxiter=None, abstol=None, nu=3, p=2, levels=2):
     """MG-preconditioned CG (initial_guess_cg.jl:303-346) -> (iters, norms, x)
     with the reference's preconditioner (reference_mg_params: two-level
-    Richardson, nu = 3, exact coarse solve); abstol defaults to
-    sqrt(eps) ||b|| as at its call site (matrix_free_mg_cg.jl:466-468).  The
-    build's own multi-level V-cycle is SbpPoisson.mgpcg."""
+    Richardson, nu = 3, exact coarse solve); abstol defaults to the
+    reference's absolute sqrt(eps) (:303; some drivers pass sqrt(eps) ||b||,
+    matrix_free_mg_cg.jl:466-468).  The build's own multi-level V-cycle is
+    SbpPoisson.mgpcg."""
     h = handle(_side(b), p, device=b.device, **reference_mg_params(_side(b), levels, nu))
     bb = b.reshape(-1)
     if abstol is None:
-        abstol = math.sqrt(2.220446049250313e-16) * float(torch.linalg.vector_norm(bb))
+        abstol = math.sqrt(2.220446049250313e-16)
     maxiter = bb.numel() if maxiter is None else maxiter
     xs, res = h.mgpcg(bb, x.reshape(-1), abstol=abstol, maxiter=maxiter, history=True)
     return res.iters, res.history, x
+
+
+def matrix_free_MGCG_Three_level(b, x, maxiter=None, abstol=None, nu=3, p=2):
+    """matrix_free_MGCG_Three_level (initial_guess_cg.jl:348-391) -> (iters,
+    norms): MG-preconditioned CG with the reference's three-level cycle
+    (reference_mg_params(N, 3)); abstol defaults to the reference's absolute
+    sqrt(eps).  x receives the solution (the reference returns only the
+    iteration count and the residual norms)."""
+    N = _side(b)
+    h = handle(N, p, device=b.device, **reference_mg_params(N, 3, nu))
+    bb = b.reshape(-1)
+    abstol = math.sqrt(2.220446049250313e-16) if abstol is None else abstol
+    maxiter = bb.numel() if maxiter is None else maxiter
+    _, res = h.mgpcg(bb, x.reshape(-1), abstol=abstol, maxiter=maxiter, history=True, raise_on_fail=False)
+    return res.iters, res.history
 
 
 def CG_Matrix_Free_GPU_v2(x, Ap, b, Nx, Ny, abstol=None, p=2):

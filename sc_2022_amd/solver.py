@@ -361,8 +361,12 @@ class PoissonSolve(torch.autograd.Function):
         b = op.assemble_rhs(f, gW, gE, gS, gN)
         bn = float(torch.linalg.vector_norm(b))
         u, res = op.mgpcg(b, abstol=rtol * bn)
+        if not res.converged:
+            raise RuntimeError(f"PoissonSolve forward: MG-PCG did not converge ({res})")
         ctx.op = op
         ctx.rtol = rtol
+        # data passed as None (zero) gets no gradient
+        ctx.present = tuple(v is not None for v in (f, gW, gE, gS, gN))
         return u
 
     @staticmethod
@@ -372,9 +376,13 @@ class PoissonSolve(torch.autograd.Function):
         gn = float(torch.linalg.vector_norm(gu))
         lam = torch.zeros_like(gu)
         if gn > 0:
-            lam, _ = op.mgpcg(gu, abstol=ctx.rtol * gn)
-        df, gW, gE, gS, gN = op.rhs_transpose(lam)
-        return None, df, gW, gE, gS, gN, None
+            lam, res = op.mgpcg(gu, abstol=ctx.rtol * gn)
+            if not res.converged:
+                raise RuntimeError(f"PoissonSolve backward: adjoint MG-PCG did not converge ({res})")
+        grads = op.rhs_transpose(lam)
+        grads = [g if (have and ctx.needs_input_grad[k + 1]) else None
+                 for k, (g, have) in enumerate(zip(grads, ctx.present))]
+        return (None, *grads, None)
 
 
 def poisson_solve(op, f, gW, gE, gS, gN, rtol=1e-13):

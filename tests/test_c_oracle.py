@@ -66,3 +66,102 @@ def test_cg_and_mgpcg(p):
     assert abs(it - itm) <= 1
     xd = spla.spsolve(A.tocsc(), b)
     assert np.linalg.norm(x - xd) < 1e-10 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("level", [5, 6])
+def test_rhs_transpose_and_hnorm(level, p):
+    """The C transposed data map equals the kron-assembled one (SAT_X_r^T of
+    level_2_multigrid_new.jl:164-167) and satisfies <b(f,g), lam> = <(f,g), b^T lam>."""
+    N = 2 ** level + 1
+    rng = np.random.default_rng(level + p)
+    lam = rng.standard_normal(N * N)
+    ref = O.rhs_transpose(level, p, lam)
+    got = CO.rhs_transpose(p, N, lam)
+    for g, r in zip(got, ref):
+        assert np.abs(g - r).max() <= 1e-13 * np.abs(r).max()
+    f, gW, gE, gS, gN = (rng.standard_normal(N * N), *rng.standard_normal((4, N)))
+    b = CO.rhs(p, N, f, gW, gE, gS, gN)
+    lhs = b @ lam
+    rhs = sum(a @ c for a, c in zip((f, gW, gE, gS, gN), got))
+    assert abs(lhs - rhs) < 1e-12 * (abs(lhs) + 1)
+    hd = O.h_tilde_diag(level, p)
+    assert abs(CO.hnorm(p, N, lam) - O.h_norm(lam, hd)) < 1e-13 * O.h_norm(lam, hd)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("level", [5, 6])
+def test_adjoint_grad_against_direct(level, p):
+    """The C FP64 gradient (adjoint MG-PCG + transposed data map) equals the
+    gradient from sparse direct solves: lam = A^-1 H~ (u - u_obs),
+    dL/d(f, g) = b^T lam."""
+    N = 2 ** level + 1
+    A = O.assemble_operator(level, p).tocsc()
+    fg = O.fourier_data(level, 11)
+    b = O.assemble_rhs(level, p, *fg)
+    u = spla.spsolve(A, b)
+    uo = spla.spsolve(A, O.assemble_rhs(level, p, *O.fourier_data(level, 12)))
+    u_obs = uo + 1e-3 * np.random.default_rng(1).standard_normal(N * N)
+    hd = O.h_tilde_diag(level, p)
+    Lo = 0.5 * float((u - u_obs) @ (hd * (u - u_obs)))
+    lam_o = spla.spsolve(A, hd * (u - u_obs))
+    go = O.rhs_transpose(level, p, lam_o)
+    w = hd * (u - u_obs)
+    L, lam, *g, it = CO.adjoint_grad(p, N, u, u_obs, 1e-14 * np.linalg.norm(w))
+    L2, lam2, *_ = CO.adjoint_grad(p, N, u, u_obs, rtol=1e-14)
+    assert np.array_equal(lam, lam2) and L2 == L
+    assert it <= 30
+    assert abs(L - Lo) < 1e-12 * Lo
+    assert np.linalg.norm(lam - lam_o) < 1e-11 * np.linalg.norm(lam_o)
+    for a, r in zip(g, go):
+        assert np.abs(a - r).max() <= 1e-10 * np.abs(r).max()
+
+
+def test_fourier_fields_match_numpy():
+    """The C evaluation of the seeded Fourier fields equals the numpy one."""
+    for level in (4, 6):
+        N = 2 ** level + 1
+        for a, c in zip(O.fourier_data(level, 5), CO.fourier_fields(N, 5)):
+            assert np.abs(a - c).max() <= 1e-14
+
+
+def test_solve_reference_meets_tolerance():
+    """The reference solution used for the headline errors: converged to
+    1e-14 ||b|| and equal to the sparse direct solve."""
+    level, p = 6, 4
+    N = 2 ** level + 1
+    b = CO.fourier_rhs(p, N, 2022)
+    x, it, rr = CO.solve_reference(p, N, b)
+    assert rr < 1e-14
+    xd = spla.spsolve(O.assemble_operator(level, p).tocsc(), b)
+    assert np.linalg.norm(x - xd) < 1e-12 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_apply_keeps_exact_row_sums(p):
+    """A 1 equals the rational row sums exactly: 0 on every row away from the
+    Dirichlet faces (D2 1 = 0, BS 1 = 0, diagonal_sbp.jl:189-454) and
+    hw_i (13 delta_j0 - bs_j) on the Dirichlet closure rows.  The plain sum of
+    the rounded FP64 entries leaves ~1e-16 there for p = 4 (a diagonal shift
+    that moves the 4097^2 solution by ~1e-10 relative), which the oracle's
+    difference-form evaluation does not."""
+    N = 65
+    one = np.ones(N * N)
+    hw = O.one_d_factors(p, N)[0]
+    bs = {2: [1.5, -2.0, 0.5], 4: [11 / 6, -3.0, 1.5, -1 / 3]}[p]
+    krs = np.zeros(N)
+    krs[0] += 13.0
+    krs[-1] += 13.0
+    for r, v in enumerate(bs):
+        krs[r] -= v
+        krs[N - 1 - r] -= v
+    want = (hw[:, None] * krs[None, :]).reshape(-1, order="F")   # i fast
+    got = CO.apply(p, N, one)
+    assert np.abs(got - want).max() <= 1e-15 * 13
+    inner = np.zeros((N, N), dtype=bool)
+    inner[8:-8, 8:-8] = True
+    inner = inner.reshape(-1, order="F")
+    assert not np.any(got[inner])
+    plain = CO.apply_plain(p, N, one)
+    if p == 4:
+        assert np.abs(plain[inner]).max() > 1e-17

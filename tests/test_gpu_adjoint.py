@@ -54,15 +54,54 @@ def test_adjoint_grad_matches_oracle(p):
         assert np.abs(g - r).max() < 1e-9 * np.abs(r).max()
 
 
-@pytest.mark.parametrize("level,ndir", [(7, 8), (11, 4)])
+def observed(N, seed):
+    """u_obs of the differentiation workload: a seeded random Fourier field
+    (the source-type field of seed + 1) plus N(0, 1e-3) noise, made on the
+    host so that the GPU and the C oracle see the same values."""
+    from oracle import c_oracle as CO
+    field = CO.fourier_fields(N, seed + 1)[0]
+    return field + 1e-3 * np.random.default_rng(seed).standard_normal(N * N)
+
+
+def test_adjoint_grad_2049_against_c_oracle():
+    """The differentiation workload at its size (2049^2, p = 4): the GPU's
+    gradient (GPU forward solve, adjoint MG-PCG, transposed data map) against
+    the C oracle's FP64 gradient computed end to end on the host (its own
+    right-hand side, forward MG-PCG and adjoint MG-PCG to 1e-14)."""
+    from oracle import c_oracle as CO
+    from sc_2022_amd.data import fourier_fields
+    from sc_2022_amd.solver import SbpPoisson
+    level, p, seed = 11, 4, 2022
+    N = 2 ** level + 1
+    CO.set_threads(16)
+    u_obs = observed(N, seed)
+    h = SbpPoisson(N, p)
+    b = h.assemble_rhs(*fourier_fields(N, seed, DEV))
+    u, _ = h.mgpcg(b, abstol=1e-14 * float(b.norm()), maxiter=100)
+    L, lam, *grads, it = h.adjoint_grad(u, T(u_obs))
+    bh = CO.fourier_rhs(p, N, seed)
+    uh, _, _ = CO.solve_reference(p, N, bh)
+    Lo, lam_o, *go, it_o = CO.adjoint_grad(p, N, uh, u_obs, rtol=1e-14)
+    # both sides solve to ~1e-14 relative residual; L and the gradients are
+    # then equal to ~1e-10 (the differences of two independent FP64 solves
+    # at kappa ~ 1e6, amplified by L's cancellation u - u_obs)
+    assert abs(L - Lo) <= 1e-9 * abs(Lo), (L, Lo)
+    for g, r in zip(grads, go):
+        g = g.cpu().numpy()
+        assert np.abs(g - r).max() <= 1e-9 * np.abs(r).max()
+
+
+@pytest.mark.parametrize("level,ndir", [(7, 8), (11, 32)])
 def test_adjoint_grad_finite_differences(level, ndir):
-    """Central differences (step 1e-6) along seeded random directions in (f, g)."""
+    """Central differences (step 1e-6) along seeded random directions in (f, g),
+    u_obs a seeded Fourier field plus N(0, 1e-3) noise; at 2049^2 (p = 4) along
+    32 directions as the differentiation workload asks."""
     from sc_2022_amd.solver import SbpPoisson
     p = 4
     N = 2 ** level + 1
     h = SbpPoisson(N, p)
     f, gW, gE, gS, gN = [T(v) for v in O.fourier_data(level, 5)]
-    u_obs = torch.randn(N * N, dtype=torch.float64, device=DEV, generator=torch.Generator(DEV).manual_seed(9)) * 1e-3
+    u_obs = T(observed(N, 5))
     def loss(fv, gv):
         b = h.assemble_rhs(fv, *gv)
         u, _ = h.mgpcg(b, abstol=1e-14 * float(b.norm()))
@@ -73,6 +112,7 @@ def test_adjoint_grad_finite_differences(level, ndir):
     assert abs(L - L0) < 1e-12 * abs(L0)
     gen = torch.Generator(DEV).manual_seed(123)
     eps = 1e-6
+    worst = 0.0
     for k in range(ndir):
         d = [torch.randn(t.numel(), dtype=torch.float64, device=DEV, generator=gen) for t in (f, gW, gE, gS, gN)]
         # scale the perturbation to the data's size
@@ -84,6 +124,8 @@ def test_adjoint_grad_finite_differences(level, ndir):
         # L is quadratic in (f, g): central differences are exact up to the
         # solves' 1e-14 residual and rounding, amplified by 1/eps
         assert abs(fd - ad) <= 1e-5 * abs(ad) + 1e-9 * abs(L0), (k, fd, ad)
+        worst = max(worst, abs(fd - ad) / abs(ad))
+    print(f"FD level {level}: {ndir} directions, worst relative difference {worst:.2e}")
 
 
 def test_autograd_function_matches_fused_entry():
@@ -101,3 +143,23 @@ def test_autograd_function_matches_fused_entry():
     assert abs(float(L.detach()) - Lc) < 1e-12 * abs(Lc)
     for t, g in zip(fg, grads):
         assert torch.allclose(t.grad, g, rtol=1e-9, atol=1e-9 * float(g.abs().max()))
+
+
+def test_autograd_function_with_absent_data():
+    """Data passed as None (zero) get no gradient and the present ones the
+    same gradient as with explicit zeros."""
+    from sc_2022_amd.solver import SbpPoisson, poisson_solve
+    level, p = 6, 4
+    N = 2 ** level + 1
+    h = SbpPoisson(N, p)
+    f, gW, gE, gS, gN = (T(v) for v in O.fourier_data(level, 4))
+    hd = T(O.h_tilde_diag(level, p))
+    f1, gS1, gN1 = (t.clone().requires_grad_(True) for t in (f, gS, gN))
+    u = poisson_solve(h, f1, None, None, gS1, gN1)
+    (0.5 * (u * hd * u).sum()).backward()
+    full = [t.clone().requires_grad_(True) for t in (f, torch.zeros_like(gW), torch.zeros_like(gE), gS, gN)]
+    u2 = poisson_solve(h, *full)
+    (0.5 * (u2 * hd * u2).sum()).backward()
+    assert torch.equal(u, u2)
+    for a, b in ((f1, full[0]), (gS1, full[3]), (gN1, full[4])):
+        assert torch.allclose(a.grad, b.grad, rtol=1e-12, atol=0)

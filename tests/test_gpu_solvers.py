@@ -171,7 +171,10 @@ def test_reference_api_mirror():
     R.matrix_free_richardson(xi, xo_, T(b), maxiter=3, omega=0.15)
     assert np.abs(xo_.cpu().numpy() - O.modified_richardson(A, np.zeros(N * N), b)).max() < 1e-12
     x = T(np.zeros(N * N))
-    iters, norms, x = R.matrix_free_MGCG(T(b), x)
+    iters, norms, x = R.matrix_free_MGCG(T(b), x)           # the reference's absolute sqrt(eps)
+    assert np.linalg.norm(A @ x.cpu().numpy() - b) < 1.0000001 * math.sqrt(2.220446049250313e-16)
+    x = T(np.zeros(N * N))
+    iters, norms, x = R.matrix_free_MGCG(T(b), x, abstol=math.sqrt(2.2e-16) * np.linalg.norm(b))
     assert np.linalg.norm(A @ x.cpu().numpy() - b) < math.sqrt(2.2e-16) * np.linalg.norm(b)
     assert len(norms) == iters + 1
     x = T(np.zeros(N * N))
@@ -291,6 +294,50 @@ def test_reference_two_level_mode(SP, N):
     assert np.linalg.norm(A @ x.cpu().numpy() - b) < 1.000001 * tol
     it, norms, xr = R.matrix_free_MGCG(T(b), T(np.zeros(N * N)), abstol=tol)
     assert it == res.iters and len(norms) == it + 1
+
+
+@pytest.mark.parametrize("N", [65, 129])
+def test_reference_three_level_mode(SP, N):
+    """The reference's three-level cycle as written (Richardson 3 + 3, weight
+    0.15, unscaled full weighting, the middle grid's residual without its
+    right-hand side, exact 4h solve; initial_guess_cg.jl:244-301) against the
+    oracle's restatement, alone and inside matrix_free_MGCG_Three_level
+    (:348-391), p = 2: same iteration count within one."""
+    from sc_2022_amd import reference_api as R
+    level = int(math.log2(N - 1))
+    A, A2, A4 = (O.assemble_operator(level - k, 2) for k in range(3))
+    h = SP(N, 2, **R.reference_mg_params(N, 3))
+    assert h.nlevels == 3
+    r = np.random.default_rng(7).standard_normal(N * N)
+    zo, no = O.three_level_richardson(A, A2, A4, r)
+    z = h.vcycle(T(r)).cpu().numpy()
+    assert np.abs(z - zo).max() < 1e-10 * np.abs(zo).max()
+    zt, nt = R.matrix_free_Three_level_multigrid(T(r))
+    assert abs(nt - no) < 1e-8 * no
+    _, _, _, (f, gW, gE, gS, gN) = problem(level, 2)
+    b = O.assemble_rhs(level, 2, f, gW, gE, gS, gN)
+    tol = 1e-10 * np.linalg.norm(b)
+    xo, ito, _ = O.matrix_free_mgcg(A, b, lambda q: O.three_level_richardson(A, A2, A4, q)[0], abstol=tol)
+    x = T(np.zeros(N * N))
+    it, norms = R.matrix_free_MGCG_Three_level(T(b), x, abstol=tol)
+    assert abs(it - ito) <= 1, (it, ito)
+    assert len(norms) == it + 1 and norms[-1] < tol
+    assert np.linalg.norm(A @ x.cpu().numpy() - b) < 1.000001 * tol
+
+
+def test_reference_transfers_small_grids():
+    """matrix_free_restriction_2d_GPU / matrix_free_prolongation_2d_GPU between
+    17^2 and 9^2 (below the default 17^2 multigrid bottom)."""
+    from sc_2022_amd import reference_api as R
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(17 * 17)
+    rc = T(np.zeros(9 * 9))
+    R.matrix_free_restriction_2d_GPU(T(u), rc)
+    assert np.abs(rc.cpu().numpy() - O.restriction_2d(17) @ u).max() < 1e-14
+    ec = rng.standard_normal(9 * 9)
+    fo = T(np.ones(17 * 17))
+    R.matrix_free_prolongation_2d_GPU(T(ec), fo)
+    assert np.abs(fo.cpu().numpy() - O.prolongation_2d(9) @ ec).max() < 1e-14
 
 
 def test_status_codes(SP):

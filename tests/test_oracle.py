@@ -290,6 +290,70 @@ def test_initial_guess_interpolation_cuts_cg_iterations(p):
     assert O.h_norm(x0 - xd, hd) < 1e-3 * O.h_norm(xd, hd)
 
 
+@pytest.mark.parametrize("p", [2, 4])
+def test_smoothers_literal_and_fixed_point(p):
+    """jacobi_sweeps equals the reference's literal update
+    x <- omega P^-1 (b - Q x) + (1 - omega) x with P = diag(A), Q = A - P
+    (jacobi_brittany!, level_2_multigrid_new.jl:335-344), sweep by sweep;
+    modified_richardson equals x <- x + omega (b - A x) (:346-350); both leave
+    the exact solution A^-1 b unchanged, and a wrong sign of the update (a
+    plausible slip) moves it."""
+    level = 4
+    N = 2 ** level + 1
+    A = O.assemble_operator(level, p).tocsr()
+    rng = np.random.default_rng(4)
+    b, x0 = rng.standard_normal(N * N), rng.standard_normal(N * N)
+    Pd = A.diagonal()
+    Q = A - sp.diags(Pd)
+    om = 2.0 / 3.0
+    x = x0.copy()
+    for nu in range(1, 4):
+        x = om * (b - Q @ x) / Pd + (1 - om) * x
+        got = O.jacobi_sweeps(A, x0, b, nu, om)
+        assert np.abs(got - x).max() <= 1e-13 * np.abs(x).max()
+    xr = x0.copy()
+    for _ in range(3):
+        xr = xr + 0.15 * (b - A @ xr)
+    assert np.abs(O.modified_richardson(A, x0, b) - xr).max() <= 1e-13 * np.abs(xr).max()
+    xs = spla.spsolve(A.tocsc(), b)
+    for sm in (O.jacobi_sweeps(A, xs, b, 3), O.modified_richardson(A, xs, b, 3)):
+        assert np.abs(sm - xs).max() <= 1e-10 * np.abs(xs).max()
+    wrong = xs - om * (b - A @ (xs + 1e-3)) / Pd
+    assert np.abs(O.jacobi_sweeps(A, xs + 1e-3, b, 1, om) - wrong).max() > 1e-6
+
+
+def test_three_level_richardson_restatement():
+    """The reference's three-level cycle as written (initial_guess_cg.jl:244-301):
+    with nu = 0 it returns zero exactly (the middle grid never receives its
+    right-hand side, so its correction -- built from -A_2h v = 0 -- is zero,
+    where a standard cycle would return P (P A_4h^-1 R) R b); it equals its
+    algebraic statement as one matrix; it is linear but not symmetric, and
+    as the MGCG preconditioner (p = 2) it still reaches the direct solution,
+    in several times the two-level cycle's iterations."""
+    level = 5
+    N = 2 ** level + 1
+    A, A2, A4 = (O.assemble_operator(level - k, 2) for k in range(3))
+    rng = np.random.default_rng(9)
+    b, c = rng.standard_normal(N * N), rng.standard_normal(N * N)
+    z0, _ = O.three_level_richardson(A, A2, A4, b, nu=0)
+    assert not np.any(z0)
+    for nu in (1, 3):
+        M = O.three_level_richardson_matrix(A, A2, A4, nu=nu)
+        z, _ = O.three_level_richardson(A, A2, A4, b, nu=nu)
+        assert np.abs(z - M @ b).max() <= 1e-11 * np.abs(z).max()
+    M3 = lambda r: O.three_level_richardson(A, A2, A4, r)[0]
+    assert np.abs(M3(b + 2 * c) - M3(b) - 2 * M3(c)).max() <= 1e-11 * np.abs(M3(c)).max()
+    assert abs(c @ M3(b) - b @ M3(c)) > 1e-6 * abs(c @ M3(b))
+    f, gW, gE, gS, gN, u = O.manufactured_data(level)
+    rhs = O.assemble_rhs(level, 2, f, gW, gE, gS, gN)
+    tol = 1e-10 * np.linalg.norm(rhs)
+    x, it, norms = O.matrix_free_mgcg(A, rhs, M3, abstol=tol)
+    _, it2, _ = O.matrix_free_mgcg(A, rhs, lambda r: O.two_level_richardson(A, A2, r)[0], abstol=tol)
+    assert norms[-1] < tol and it > 2 * it2
+    xd = spla.spsolve(A.tocsc(), rhs)
+    assert np.linalg.norm(x - xd) < 1e-8 * np.linalg.norm(xd)
+
+
 def test_two_level_richardson_restatement():
     """The reference's two-level cycle: with nu = 0 it is P A_2h^-1 FW b
     exactly; it is a symmetric operator (same smoother before and after,
