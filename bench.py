@@ -236,27 +236,42 @@ def batch_leg(a, dev, torch):
 
 def csr_leg(h, torch, apply_us):
     """The assembled CSR operator (the comparison path of the reference's
-    CG_GPU_sparse) through the hand-written CSR SpMV, against the matrix-free
-    apply: time per product and algorithmic bytes (values 8 B and column
-    indices 4 B per nonzero, row pointers, x read once, y written)."""
+    CG_GPU_sparse, which multiplies with CUSPARSE) against the matrix-free
+    apply: time per product of the hand-written CSR SpMV and of rocSPARSE's
+    (csr_adaptive and csr_rowsplit, analysis done once, outside the timing),
+    with algorithmic bytes (values 8 B and column indices 4 B per nonzero,
+    row pointers, x read once, y written)."""
     rp, col, val = h.csr_assemble()
     n, nnz = h.n, col.numel()
     x = torch.randn(n, dtype=torch.float64, device=h.device)
     y = torch.empty_like(x)
-    for _ in range(3):
-        h.csr_spmv((rp, col, val), x, y)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 50
-    e0.record(torch.cuda.current_stream())
-    for _ in range(reps):
-        h.csr_spmv((rp, col, val), x, y)
-    e1.record(torch.cuda.current_stream())
-    torch.cuda.synchronize()
-    us = e0.elapsed_time(e1) / reps * 1e3
     nbytes = 12 * nnz + 4 * (n + 1) + 16 * n
-    res = {"nnz": nnz, "spmv_us": us, "spmv_GBs": nbytes / (us * 1e-6) / 1e9, "bytes": nbytes,
+
+    def timed(fn, reps=50):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(torch.cuda.current_stream())
+        for _ in range(reps):
+            fn()
+        e1.record(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    us = timed(lambda: h.csr_spmv((rp, col, val), x, y))
+    ref = y.clone()
+    res = {"nnz": nnz, "bytes": nbytes, "spmv_us": us, "spmv_GBs": nbytes / (us * 1e-6) / 1e9,
            "matrix_free_apply_us": apply_us, "matrix_free_speedup": us / apply_us}
+    for alg, name in ((0, "rocsparse_adaptive"), (1, "rocsparse_rowsplit")):
+        try:
+            h.csr_vendor_attach((rp, col, val), alg)
+            vus = timed(lambda: h.csr_vendor_spmv(x, y))
+            diff = float((y - ref).abs().max() / ref.abs().max())
+            res[name] = {"spmv_us": vus, "spmv_GBs": nbytes / (vus * 1e-6) / 1e9,
+                         "matrix_free_speedup": vus / apply_us, "max_rel_diff_vs_hand_csr": diff}
+        except Exception as e:  # reported, never silently replaced
+            res[name] = {"error": f"{type(e).__name__}: {e}"}
     del rp, col, val
     return res
 
@@ -392,7 +407,18 @@ def reference_modes_leg(a, dev, torch, N=1025, p=2):
     torch.cuda.synchronize()
     spmv_ms = (time.perf_counter() - t0) / 50 * 1e3
     rc, csr_ms = timed(lambda: h.csr_cg(csr, b, x.zero_(), abstol=1e-8 * bn)[1], reps=2)
-    for name, it, tot, op in (("cg_matrix_free", r.iters, cg_ms, apply_ms), ("cg_csr", rc.iters, csr_ms, spmv_ms)):
+    h.csr_vendor_attach(csr, 0)
+    for _ in range(3):
+        h.csr_vendor_spmv(b, y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(50):
+        h.csr_vendor_spmv(b, y)
+    torch.cuda.synchronize()
+    vspmv_ms = (time.perf_counter() - t0) / 50 * 1e3
+    rv, vcsr_ms = timed(lambda: h.csr_vendor_cg(b, x.zero_(), abstol=1e-8 * bn)[1], reps=2)
+    for name, it, tot, op in (("cg_matrix_free", r.iters, cg_ms, apply_ms), ("cg_csr", rc.iters, csr_ms, spmv_ms),
+                              ("cg_csr_rocsparse", rv.iters, vcsr_ms, vspmv_ms)):
         t_op = it * op
         out[name] = {"iters": it, "ms": tot, "op_ms_each": op, "op_ms_in_cg": t_op,
                      "overhead_ms": tot - t_op, "efficiency": t_op / tot}

@@ -54,7 +54,8 @@ typedef enum {
     SBP_ERR_NONFINITE = 5,      /* NaN/Inf met in a residual or inner product */
     SBP_ERR_BREAKDOWN = 6,      /* p'Ap <= 0 (operator or preconditioner not SPD) */
     SBP_ERR_NOT_CONVERGED = 7,  /* maxiter reached; x holds the last iterate */
-    SBP_ERR_NO_DEVICE = 8       /* no HIP device visible */
+    SBP_ERR_NO_DEVICE = 8,      /* no HIP device visible */
+    SBP_ERR_VENDOR = 9          /* a rocSPARSE call failed (sbp_csr_vendor_*) */
 } sbp_status;
 
 typedef enum { SBP_F64 = 0, SBP_F32 = 1 } sbp_dtype;
@@ -266,6 +267,20 @@ int sbp_csr_spmv(sbp_handle h, const int32_t* row_ptr, const int32_t* col, const
 int sbp_csr_cg(sbp_handle h, const int32_t* row_ptr, const int32_t* col, const double* val,
                const double* b, double* x, double abstol, int maxiter, int* iters,
                double* resnorm);
+
+/* The same CSR matrix through rocSPARSE (the vendor SpMV: the reference's
+ * comparison path multiplies with CUSPARSE, Poisson_CG.jl:217-271,
+ * kernel_comparisons/SPMV_CSR.jl:197-202, CG_GPU_sparse split_matrix_free.jl:744-769).
+ * attach: keeps the caller's CSR arrays by reference (they must stay alive
+ * until sbp_destroy or the next attach) and runs rocSPARSE's analysis once
+ * (alg 0: csr_adaptive, 1: csr_rowsplit); single-problem handles. */
+int sbp_csr_vendor_attach(sbp_handle h, const int32_t* row_ptr, const int32_t* col, const double* val, int64_t nnz,
+                          int alg);
+/* y = A x (rocsparse_spmv on the handle's stream). */
+int sbp_csr_vendor_spmv(sbp_handle h, const double* x, double* y);
+/* CG as sbp_csr_cg with the rocSPARSE SpMV. */
+int sbp_csr_vendor_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters,
+                      double* resnorm);
 
 /* Streaming-copy calibration kernel (copy_naive!, split_matrix_free.jl:51-77). */
 int sbp_copy(sbp_handle h, const void* src, void* dst, size_t bytes);

@@ -20,6 +20,7 @@ SBP_ERR_NONFINITE = 5
 SBP_ERR_BREAKDOWN = 6
 SBP_ERR_NOT_CONVERGED = 7
 SBP_ERR_NO_DEVICE = 8
+SBP_ERR_VENDOR = 9
 
 SBP_F64 = 0
 SBP_F32 = 1
@@ -91,6 +92,9 @@ SIGNATURES = {
     "sbp_csr_assemble": (_i, [_vp, _vp, _vp, _vp, C.POINTER(C.c_int64)]),
     "sbp_csr_spmv": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sbp_csr_cg": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _pi, _pd]),
+    "sbp_csr_vendor_attach": (_i, [_vp, _vp, _vp, _vp, C.c_int64, _i]),
+    "sbp_csr_vendor_spmv": (_i, [_vp, _vp, _vp]),
+    "sbp_csr_vendor_cg": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd]),
     "sbp_copy": (_i, [_vp, _vp, _vp, C.c_size_t]),
 }
 

@@ -65,7 +65,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
                 print(out)
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-          *[str(o) for _, o, _ in units]])
+          *[str(o) for _, o, _ in units], "-L/opt/rocm/lib", "-lrocsparse", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -13,6 +13,8 @@
 // reference's semantics (the first k with ||r_k||_2 < abstol).
 #include <hip/hip_runtime.h>
 
+#include <rocsparse/rocsparse.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -113,6 +115,15 @@ struct sbp_handle_s {
     };
     std::vector<VarLevel> vl;
     bool vmg = false;
+    int dev = 0;                 // device current at creation
+    // The CSR operator through rocSPARSE (sbp_csr_vendor_*): library context,
+    // matrix and vector descriptors, the SpMV work buffer from its analysis.
+    rocsparse_handle vs = nullptr;
+    rocsparse_spmat_descr vs_mat = nullptr;
+    rocsparse_dnvec_descr vs_x = nullptr, vs_y = nullptr;
+    rocsparse_spmv_alg vs_alg = rocsparse_spmv_alg_csr_adaptive;
+    void* vs_buf = nullptr;
+    size_t vs_bufsz = 0;
 };
 
 namespace {
@@ -943,6 +954,21 @@ __global__ __launch_bounds__(256) void scan_add(int32_t* rp, const int64_t* bsum
     if (idx == 0) rp[n] = (int32_t)*total;
 }
 
+// r = b - y with block partials of r'r (one problem).
+template <int NT>
+__global__ __launch_bounds__(NT) void axpby_dot(const double* __restrict__ b, const double* __restrict__ y,
+                                               double* __restrict__ r, size_t n, double* partial) {
+    double acc = 0.0;
+    const size_t stride = (size_t)gridDim.x * NT;
+    for (size_t k = (size_t)blockIdx.x * NT + threadIdx.x; k < n; k += stride) {
+        const double v = b[k] - y[k];
+        r[k] = v;
+        acc += v * v;
+    }
+    const double s = block_sum<double, NT>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 __global__ __launch_bounds__(256) void copy_vec16(const uint4* __restrict__ s, uint4* __restrict__ d, size_t n16) {
     const size_t stride = (size_t)gridDim.x * 256;
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n16; q += stride) d[q] = s[q];
@@ -973,6 +999,7 @@ const char* sbp_status_string(int s) {
         case SBP_ERR_BREAKDOWN: return "breakdown: p'Ap <= 0";
         case SBP_ERR_NOT_CONVERGED: return "not converged within maxiter";
         case SBP_ERR_NO_DEVICE: return "no HIP device";
+        case SBP_ERR_VENDOR: return "rocSPARSE call failed";
         default: return "unknown status";
     }
 }
@@ -1013,6 +1040,11 @@ int sbp_destroy(sbp_handle h) {
         }
     }
     drop_graphs(h);
+    if (h->vs_x) (void)rocsparse_destroy_dnvec_descr(h->vs_x);
+    if (h->vs_y) (void)rocsparse_destroy_dnvec_descr(h->vs_y);
+    if (h->vs_mat) (void)rocsparse_destroy_spmat_descr(h->vs_mat);
+    if (h->vs) (void)rocsparse_destroy_handle(h->vs);
+    if (h->vs_buf) (void)hipFree(h->vs_buf);
     if (h->cap) (void)hipStreamDestroy(h->cap);
     if (h->child) (void)sbp_destroy(h->child);
     if (h->cexact) (void)sbp_destroy(h->cexact);
@@ -1120,6 +1152,7 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess)
         h->clock_khz = (double)khz;
+    h->dev = dev;
     int rc = SBP_OK;
     do {
         for (int l = 0; l < nlev && rc == SBP_OK; ++l) {
@@ -1988,6 +2021,110 @@ int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, const double
         hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 1);
         hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)h->r,
                            (const Scalars*)sc, (size_t)n);
+        HIPCHK(hipGetLastError());
+        if (it + 1 >= next && it + 1 < maxiter) {
+            bool all = false;
+            STCHK(poll(h, sc, &all));
+            if (all) break;
+            next = it + 1 + (adaptive ? poll_ahead(h) : h->prm.poll_every);
+        }
+    }
+    STCHK(poll(h, sc));
+    get_results(h, iters, resnorm, nullptr);
+    return batch_status(h);
+}
+
+// ---- the CSR operator through rocSPARSE -----------------------------------
+namespace {
+#define VSCHK(x)                                             \
+    do {                                                     \
+        if ((x) != rocsparse_status_success) return SBP_ERR_VENDOR; \
+    } while (0)
+
+// y = A x with the attached matrix (descriptors repointed, compute stage).
+int vendor_spmv(sbp_handle h, const double* x, double* y) {
+    const double one = 1.0, zero = 0.0;
+    VSCHK(rocsparse_set_stream(h->vs, h->st));
+    VSCHK(rocsparse_dnvec_set_values(h->vs_x, (void*)x));
+    VSCHK(rocsparse_dnvec_set_values(h->vs_y, (void*)y));
+    size_t bs = h->vs_bufsz;
+    VSCHK(rocsparse_spmv(h->vs, rocsparse_operation_none, &one, h->vs_mat, h->vs_x, &zero, h->vs_y,
+                         rocsparse_datatype_f64_r, h->vs_alg, rocsparse_spmv_stage_compute, &bs, h->vs_buf));
+    return SBP_OK;
+}
+}  // namespace
+
+int sbp_csr_vendor_attach(sbp_handle h, const int32_t* rp, const int32_t* col, const double* val, int64_t nnz,
+                          int alg) {
+    if (!h || !rp || !col || !val || nnz <= 0 || h->nprob != 1 || (alg != 0 && alg != 1)) return SBP_ERR_INVALID_ARG;
+    const int64_t n = (int64_t)h->L[0].n;
+    if (h->vs_mat) {   // re-attach: drop the previous matrix and its analysis
+        (void)rocsparse_destroy_spmat_descr(h->vs_mat);
+        h->vs_mat = nullptr;
+        if (h->vs_buf) (void)hipFree(h->vs_buf);
+        h->vs_buf = nullptr;
+        h->vs_bufsz = 0;
+    }
+    if (!h->vs) {
+        VSCHK(rocsparse_create_handle(&h->vs));
+        VSCHK(rocsparse_create_dnvec_descr(&h->vs_x, n, (void*)h->pv, rocsparse_datatype_f64_r));
+        VSCHK(rocsparse_create_dnvec_descr(&h->vs_y, n, (void*)h->Ap, rocsparse_datatype_f64_r));
+    }
+    VSCHK(rocsparse_set_stream(h->vs, h->st));
+    h->vs_alg = alg == 0 ? rocsparse_spmv_alg_csr_adaptive : rocsparse_spmv_alg_csr_rowsplit;
+    VSCHK(rocsparse_create_csr_descr(&h->vs_mat, n, n, nnz, (void*)rp, (void*)col, (void*)val,
+                                     rocsparse_indextype_i32, rocsparse_indextype_i32, rocsparse_index_base_zero,
+                                     rocsparse_datatype_f64_r));
+    const double one = 1.0, zero = 0.0;
+    VSCHK(rocsparse_dnvec_set_values(h->vs_x, (void*)h->pv));
+    VSCHK(rocsparse_dnvec_set_values(h->vs_y, (void*)h->Ap));
+    size_t bs = 0;
+    VSCHK(rocsparse_spmv(h->vs, rocsparse_operation_none, &one, h->vs_mat, h->vs_x, &zero, h->vs_y,
+                         rocsparse_datatype_f64_r, h->vs_alg, rocsparse_spmv_stage_buffer_size, &bs, nullptr));
+    if (bs > 0) HIPCHK(hipMalloc(&h->vs_buf, bs));
+    h->vs_bufsz = bs;
+    VSCHK(rocsparse_spmv(h->vs, rocsparse_operation_none, &one, h->vs_mat, h->vs_x, &zero, h->vs_y,
+                         rocsparse_datatype_f64_r, h->vs_alg, rocsparse_spmv_stage_preprocess, &bs, h->vs_buf));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SBP_OK;
+}
+
+int sbp_csr_vendor_spmv(sbp_handle h, const double* x, double* y) {
+    if (!h || !x || !y || !h->vs_mat) return SBP_ERR_INVALID_ARG;
+    return vendor_spmv(h, x, y);
+}
+
+int sbp_csr_vendor_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters,
+                      double* resnorm) {
+    if (!h || !b || !x || !h->vs_mat || maxiter < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
+    hipStream_t st = h->st;
+    const size_t n = h->L[0].n;
+    const dim3 g(ew_blocks(n), 1);
+    if ((size_t)g.x > h->npp) return SBP_ERR_INVALID_SIZE;
+    Scalars* sc = h->sc;
+    STCHK(put_abstol(h, &abstol));
+    hipLaunchKernelGGL(stamp_start, dim3(1), dim3(256), 0, st, sc, 1);
+    // r = b - A x: Ap = A x, then r = b - Ap with partials of r'r
+    STCHK(vendor_spmv(h, x, h->Ap));
+    hipLaunchKernelGGL((axpby_dot<kEwThreads>), g, dim3(kEwThreads), 0, st, b, (const double*)h->Ap, h->r, n,
+                       h->partial);
+    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc,
+                       (const double*)h->dabs, (double*)nullptr, maxiter);
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->r, h->pv, n,
+                       (const Scalars*)sc);
+    HIPCHK(hipGetLastError());
+    const bool adaptive = h->prm.poll_every == 0;
+    int next = adaptive ? 2 : h->prm.poll_every;
+    for (int it = 0; it < maxiter; ++it) {
+        STCHK(vendor_spmv(h, h->pv, h->Ap));
+        hipLaunchKernelGGL((dot_partial<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->pv,
+                           (const double*)h->Ap, n, h->partial);
+        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 0);
+        hipLaunchKernelGGL((update_xr<double, kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, (const double*)h->pv,
+                           (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
+        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 1);
+        hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)h->r,
+                           (const Scalars*)sc, n);
         HIPCHK(hipGetLastError());
         if (it + 1 >= next && it + 1 < maxiter) {
             bool all = false;

@@ -342,6 +342,34 @@ class SbpPoisson:
             capi.check(st, "sbp_csr_cg")
         return x, SolveResult(it.value, rn.value, st)
 
+    # -- the same CSR matrix through rocSPARSE (the vendor SpMV) ------------
+    def csr_vendor_attach(self, csr, alg=0):
+        """Hands the CSR arrays to rocSPARSE (kept by reference: the handle
+        holds on to the tensors) and runs its analysis (alg 0: csr_adaptive,
+        1: csr_rowsplit)."""
+        rp, col, val = csr
+        self._sync_stream()
+        capi.check(self.lib.sbp_csr_vendor_attach(self._h, _ptr(rp), _ptr(col), _ptr(val), int(col.numel()), int(alg)),
+                   "sbp_csr_vendor_attach")
+        self._vendor_csr = csr
+
+    def csr_vendor_spmv(self, x, out=None):
+        out = torch.empty_like(x) if out is None else out
+        self._sync_stream()
+        capi.check(self.lib.sbp_csr_vendor_spmv(self._h, _ptr(x), _ptr(out)), "sbp_csr_vendor_spmv")
+        return out
+
+    def csr_vendor_cg(self, b, x=None, abstol=1e-8, maxiter=None, raise_on_fail=True):
+        maxiter = self.n if maxiter is None else maxiter
+        x = torch.zeros_like(b) if x is None else x
+        it, rn = C.c_int(), C.c_double()
+        self._sync_stream()
+        st = self.lib.sbp_csr_vendor_cg(self._h, _ptr(b), _ptr(x), float(abstol), int(maxiter), C.byref(it),
+                                        C.byref(rn))
+        if raise_on_fail and st not in (capi.SBP_OK, capi.SBP_ERR_NOT_CONVERGED):
+            capi.check(st, "sbp_csr_vendor_cg")
+        return x, SolveResult(it.value, rn.value, st)
+
     def copy(self, src, dst):
         self._sync_stream()
         capi.check(self.lib.sbp_copy(self._h, _ptr(src), _ptr(dst), src.numel() * src.element_size()), "sbp_copy")

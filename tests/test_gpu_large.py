@@ -158,6 +158,16 @@ def test_csr_cg_1025_against_oracle(p):
     e_gpu = np.linalg.norm(x.cpu().numpy() - xt)
     e_orc = np.linalg.norm(xo - xt)
     assert e_gpu <= 3 * e_orc + 1e-12 * np.linalg.norm(xt), (e_gpu, e_orc)
+    # the same CG with rocSPARSE's SpMV (the vendor comparison path)
+    for alg in (0, 1):
+        h.csr_vendor_attach(csr, alg)
+        y = h.csr_vendor_spmv(b)
+        ref = c_oracle.apply_plain(p, N, bh)
+        assert np.abs(y.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+        xv, rv = h.csr_vendor_cg(b, abstol=1e-8 * bn)
+        assert rv.converged and abs(rv.iters - it_o) <= max(2, 0.01 * it_o), (alg, rv.iters, it_o)
+        rr = bh - c_oracle.apply(p, N, xv.cpu().numpy())
+        assert np.linalg.norm(rr) < 1.01e-8 * bn
 
 
 def test_mixed_precision_8193():
