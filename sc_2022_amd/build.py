@@ -49,7 +49,17 @@ def _run(cmd):
     return err
 
 
-def build(jobs: int | None = None, verbose: bool = False) -> Path:
+def _up_to_date(obj: Path) -> bool:
+    """Whether obj is newer than every file its dependency list names."""
+    dep = obj.with_suffix(".d")
+    if not obj.exists() or not dep.exists():
+        return False
+    files = dep.read_text().replace("\\\n", " ").split(":", 1)[1].split()
+    t = obj.stat().st_mtime
+    return all(Path(f).exists() and Path(f).stat().st_mtime <= t for f in files)
+
+
+def build(jobs: int | None = None, verbose: bool = False, incremental: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
     units = [(CSRC / "sbp_api.hip", OBJ / "sbp_api.o", []),
              (CSRC / "sbp_fused.hip", OBJ / "sbp_fused.o", [])]
@@ -58,7 +68,8 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
         units.append((CSRC / "sbp_tile.hip", OBJ / f"sbp_tile_{ti}x{tj}.o",
                       [f"-DSBP_TI={ti}", f"-DSBP_TJ={tj}", f"-DSBP_HI={int((ti, tj) in hi)}"]))
     jobs = jobs or min(len(units), os.cpu_count() or 4)
-    cmds = [[HIPCC, *FLAGS, *defs, "-c", str(src), "-o", str(obj)] for src, obj, defs in units]
+    cmds = [[HIPCC, *FLAGS, *defs, "-MMD", "-MF", str(obj.with_suffix(".d")), "-c", str(src), "-o", str(obj)]
+            for src, obj, defs in units if not (incremental and _up_to_date(obj))]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for out in ex.map(_run, cmds):
             if verbose and out.strip():
@@ -74,8 +85,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only the units whose sources or headers changed since their last build")
     a = ap.parse_args(argv)
-    print(build(a.jobs, a.verbose))
+    print(build(a.jobs, a.verbose, a.incremental))
 
 
 if __name__ == "__main__":
