@@ -55,7 +55,9 @@ typedef enum {
     SBP_ERR_BREAKDOWN = 6,      /* p'Ap <= 0 (operator or preconditioner not SPD) */
     SBP_ERR_NOT_CONVERGED = 7,  /* maxiter reached; x holds the last iterate */
     SBP_ERR_NO_DEVICE = 8,      /* no HIP device visible */
-    SBP_ERR_VENDOR = 9          /* a rocSPARSE call failed (sbp_csr_vendor_*) */
+    SBP_ERR_VENDOR = 9,         /* a rocSPARSE call failed (sbp_csr_vendor_*) */
+    SBP_ERR_SCRATCH = 10        /* sbp_apply_dot_noalloc: the handle's block partials exceed the
+                                   library's static scratch array */
 } sbp_status;
 
 typedef enum { SBP_F64 = 0, SBP_F32 = 1 } sbp_dtype;
@@ -150,6 +152,20 @@ int sbp_set_tiles(sbp_handle h, int level, int dtype, int tile_i, int tile_j);
 int sbp_apply(sbp_handle h, int level, const void* u, void* Au, int sign, int dtype);
 /* Au = A u and dot[q] = u_q'A u_q in one pass (FP64; dot: nprob host doubles). */
 int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot);
+/* sbp_apply_dot without device memory of its own: Au = A u and dot[q] =
+ * u_q'A u_q (FP64), with the block partials and the per-problem results in
+ * the library's file-scope static device array (1 MiB per GPU, held until the
+ * process ends) and the results written to `dot`, a DEVICE array of nprob
+ * doubles; nothing is allocated and nothing waits for the stream, so the call
+ * may be captured into a hipGraph.  The handle (made beforehand) supplies the
+ * operator tables and the stream; its device must be current.  Handles whose
+ * finest pass has more than 2^17 - nprob block partials over all problems
+ * return SBP_ERR_SCRATCH (4097^2 with the default 256 x 8 tiles uses 8721 per
+ * problem).  Every call on one GPU shares the array: calls that overlap on one
+ * GPU (two streams, two host threads) race.  Variable-coefficient handles:
+ * SBP_ERR_INVALID_ARG.  No counterpart in the reference; for callers that
+ * cannot allocate (code under stream capture, tight serving loops). */
+int sbp_apply_dot_noalloc(sbp_handle h, const double* u, double* Au, double* dot);
 /* d = diag(A) on the finest level (FP64). */
 int sbp_diag(sbp_handle h, double* d);
 /* b = -H~ (f + SAT_W_r gW + SAT_E_r gE + SAT_S_r gS + SAT_N_r gN)

@@ -21,6 +21,7 @@ SBP_ERR_BREAKDOWN = 6
 SBP_ERR_NOT_CONVERGED = 7
 SBP_ERR_NO_DEVICE = 8
 SBP_ERR_VENDOR = 9
+SBP_ERR_SCRATCH = 10
 
 SBP_F64 = 0
 SBP_F32 = 1
@@ -70,6 +71,7 @@ SIGNATURES = {
     "sbp_set_tiles": (_i, [_vp, _i, _i, _i, _i]),
     "sbp_apply": (_i, [_vp, _i, _vp, _vp, _i, _i]),
     "sbp_apply_dot": (_i, [_vp, _vp, _vp, _pd]),
+    "sbp_apply_dot_noalloc": (_i, [_vp, _vp, _vp, _vp]),
     "sbp_diag": (_i, [_vp, _vp]),
     "sbp_assemble_rhs": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sbp_rhs_transpose": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -981,6 +982,39 @@ __global__ __launch_bounds__(256) void copy_u8(const unsigned char* __restrict__
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Static scratch of the no-allocation entry point (sbp_apply_dot_noalloc):
+// the stencil pass's block partials and the per-problem results live in a
+// file-scope device array, so that call needs no device memory of its own.
+// Each GPU holds its own copy until the process ends.  Host code never uses
+// the array's name as an address: its device address is resolved once per
+// device with hipGetSymbolAddress (scratch_ptr below) and only that pointer
+// is passed to kernels.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchDoubles = size_t(1) << 17;   // 1 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (resolved on first use).
+int scratch_ptr(double** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return SBP_ERR_HIP;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch[dev];
+    return SBP_OK;
+}
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -999,6 +1033,7 @@ const char* sbp_status_string(int s) {
         case SBP_ERR_BREAKDOWN: return "breakdown: p'Ap <= 0";
         case SBP_ERR_NOT_CONVERGED: return "not converged within maxiter";
         case SBP_ERR_NO_DEVICE: return "no HIP device";
+        case SBP_ERR_SCRATCH: return "problem too large for the library's static scratch array (no-allocation entry point)";
         case SBP_ERR_VENDOR: return "rocSPARSE call failed";
         default: return "unknown status";
     }
@@ -1639,6 +1674,26 @@ int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(dot, h->dtmp, sizeof(double) * h->nprob, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
+    return SBP_OK;
+}
+
+int sbp_apply_dot_noalloc(sbp_handle h, const double* u, double* Au, double* dot) {
+    if (!h || !u || !Au || !dot || h->var) return SBP_ERR_INVALID_ARG;
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != h->dev) return SBP_ERR_INVALID_ARG;   // the handle's device must be current
+    const Level& L = h->L[0];
+    const size_t np = pass_blocks(L.N, L.ti[0], L.tj[0]);
+    // block partials of every problem, then one result per problem
+    if (np * (size_t)h->nprob + (size_t)h->nprob > kScratchDoubles) return SBP_ERR_SCRATCH;
+    double* s = nullptr;
+    STCHK(scratch_ptr(&s));
+    int npi = 0;
+    STCHK(launch_pass<double>(h, 0, InPlain<double>{u}, OutStoreDot<double>{Au}, nullptr, s, &npi));
+    double* res = s + np * (size_t)h->nprob;
+    hipLaunchKernelGGL(reduce_to, dim3(h->nprob), dim3(kRedThreads), 0, h->st, (const double*)s, npi, res, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dot, res, sizeof(double) * h->nprob, hipMemcpyDeviceToDevice, h->st));
     return SBP_OK;
 }
 

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_status_strings():
     lib = capi.load()
     assert b"gfx950" in lib.sbp_version()
-    for s in range(0, 10):
+    for s in range(0, 11):
         assert lib.sbp_status_string(s)
 
 
