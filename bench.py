@@ -193,43 +193,60 @@ def time_copy(h, torch, reps, nbytes, nbuf):
 def batch_leg(a, dev, torch):
     """Uneven batch: a.batch problems, N = 2^k+1 with k uniform in 6..10 and p
     in {2, 4} drawn from the seed, each with its own Fourier-mode data; mixed
-    MG-PCG with per-problem masks.  Reports batch solves/s, p50/p99 of the
-    per-problem device times and the worst relative H-norm error against a
-    tightly converged FP64 solve of the same problems."""
+    MG-PCG with per-problem masks, one ragged handle per order (shared
+    multigrid levels, one launch per pass).  Reports batch solves/s, p50/p99
+    of the per-problem device times and the worst relative H-norm error
+    against the C oracle's FP64 solution of every problem (host right-hand
+    side from the same modes, MG-PCG to 1e-14) -- at the batch rule
+    (batch_rtol) and at the headline's rule (mixed_rtol)."""
+    from oracle import c_oracle
     from sc_2022_amd.batch import UnevenBatch
     from sc_2022_amd.data import batch_specs
     specs = batch_specs(a.seed, count=a.batch)
     ub = UnevenBatch([(N, p) for N, p, _ in specs], device=dev, ir_rtol=a.ir_rtol, inner_rtol=a.inner_rtol)
-    tol_ref, tol = [], []
     for q, (N, p, s) in enumerate(specs):
         ub.set_fourier(q, s)
     torch.cuda.synchronize()
-    for q in range(len(specs)):
-        bn = float(torch.linalg.vector_norm(ub.rhs(q)))
-        tol_ref.append(1e-14 * bn)
-        tol.append(a.batch_rtol * bn)
-    ub.solve(tol_ref, mixed=False, maxiter=200)
-    xref = [ub.solution(q).clone() for q in range(len(specs))]
-    for _ in range(max(1, a.warmup)):
-        out = ub.solve(tol, mixed=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.batch_steps):
-        out = ub.solve(tol, mixed=True)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / a.batch_steps
-    errs = []
-    for q in range(len(specs)):
-        op = ub.op(q)
-        errs.append(op.hnorm(ub.solution(q) - xref[q]) / op.hnorm(xref[q]))
-    sec = sorted(o["seconds"] for o in out)
+    bnorm = [float(torch.linalg.vector_norm(ub.rhs(q))) for q in range(len(specs))]
+    c_oracle.set_threads(min(16, os.cpu_count() or 1))
+    t1 = time.perf_counter()
+    xo, rhs_diff = [], 0.0
+    for q, (N, p, s) in enumerate(specs):
+        bh = c_oracle.fourier_rhs(p, N, s)
+        rhs_diff = max(rhs_diff, float(np.linalg.norm(ub.rhs(q).cpu().numpy() - bh) / np.linalg.norm(bh)))
+        xo.append(c_oracle.solve_reference(p, N, bh)[0])
+    t_orc = time.perf_counter() - t1
+
+    def run(rtol):
+        tol = [rtol * bn for bn in bnorm]
+        for _ in range(max(1, a.warmup)):
+            out = ub.solve(tol, mixed=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.batch_steps):
+            out = ub.solve(tol, mixed=True)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / a.batch_steps
+        errs = []
+        for q in range(len(specs)):
+            x = torch.as_tensor(xo[q], device=dev)
+            errs.append(ub.hnorm(q, ub.solution(q) - x) / ub.hnorm(q, x))
+        sec = sorted(o["seconds"] for o in out)
+        return {"rule": f"per problem FP64 ||r|| < {rtol:g} ||b||, FP32 inner solves to {a.inner_rtol:g}",
+                "solves_per_s": len(specs) / dt, "ms_per_batch": dt * 1e3,
+                "p50_ms": 1e3 * float(np.percentile(sec, 50)), "p99_ms": 1e3 * float(np.percentile(sec, 99)),
+                "max_outer": max(o["iters"] for o in out), "max_inner": max(o["inner_iters"] for o in out),
+                "max_rel_err_vs_oracle": max(errs), "all_converged": all(o["status"] == 0 for o in out)}
+
+    res = run(a.batch_rtol)
+    res["at_headline_rule"] = run(a.mixed_rtol)
     pts = sum(N * N for N, _, _ in specs)
-    res = {"problems": len(specs), "groups": len(ub.groups), "points": pts,
-           "solves_per_s": len(specs) / dt, "ms_per_batch": dt * 1e3,
-           "p50_ms": 1e3 * float(np.percentile(sec, 50)), "p99_ms": 1e3 * float(np.percentile(sec, 99)),
-           "max_rel_err": max(errs), "all_converged": all(o["status"] == 0 for o in out),
-           "sizes": {f"{N}^2 p{p}": sum(1 for n, pp, _ in specs if (n, pp) == (N, p))
-                     for N, p in sorted({(n, pp) for n, pp, _ in specs})}}
+    res.update({"problems": len(specs), "handles": len(ub.groups), "points": pts,
+                "design": "one ragged handle per order (shared multigrid levels, one launch per pass), "
+                          "the two orders on two streams",
+                "oracle_rhs_rel_diff": rhs_diff, "oracle_host_s": t_orc,
+                "sizes": {f"{N}^2 p{p}": sum(1 for n, pp, _ in specs if (n, pp) == (N, p))
+                          for N, p in sorted({(n, pp) for n, pp, _ in specs})}})
     ub.close()
     return res
 

@@ -115,6 +115,21 @@ int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out);
  * nprob = 1).  No counterpart in the reference, which solves one problem per
  * call of matrix_free_MGCG (initial_guess_cg.jl:303-346). */
 int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle* out);
+/* A handle for nprob problems of one order p and UNEVEN sides (the ragged
+ * batch): sides[q] = 2^k + 1, sorted largest first, all on the multigrid
+ * hierarchy of sides[0] down to the coarse side (<= 33).  Grid functions are
+ * ragged arrays: problem q at element offset sum_{q' < q} sides[q']^2.  The
+ * multigrid levels are shared across sizes: level l holds every problem whose
+ * side is at least the level's (a 513^2 problem's finest level and a 1025^2
+ * problem's first coarse level sit in the same stack), so each V-cycle pass
+ * is ONE launch over all the problems present on its level, and the Krylov
+ * passes are one launch per level over the problems whose finest level it
+ * is.  Serves sbp_mgpcg_batch and sbp_mgpcg_mixed_batch (per-problem
+ * tolerances and masks, problems in this order), sbp_set_tiles, sbp_get_info,
+ * sbp_get_batch, sbp_set_stream and sbp_destroy; every other entry point
+ * returns SBP_ERR_INVALID_ARG.  No counterpart in the reference (one problem
+ * per matrix_free_MGCG call, initial_guess_cg.jl:303-346). */
+int sbp_create_ragged(int p, int nprob, const int* sides, const sbp_params* prm, sbp_handle* out);
 /* Variable-coefficient operator d/dx (bx d/dx) + d/dy (by d/dy), p = 2, 4 or 6,
  * from variable_diagonal_sbp_D2 (diagonal_sbp.jl:470-740) on both axes as
  * get_operators uses it (legacy_functions.jl:100-174): bx (host, N values)

@@ -64,6 +64,7 @@ SIGNATURES = {
     "sbp_default_params": (None, [C.POINTER(SbpParams)]),
     "sbp_create": (_i, [_i, _i, C.POINTER(SbpParams), C.POINTER(_vp)]),
     "sbp_create_batch": (_i, [_i, _i, _i, C.POINTER(SbpParams), C.POINTER(_vp)]),
+    "sbp_create_ragged": (_i, [_i, _i, _pi, C.POINTER(SbpParams), C.POINTER(_vp)]),
     "sbp_get_batch": (_i, [_vp, _pi]),
     "sbp_destroy": (_i, [_vp]),
     "sbp_set_stream": (_i, [_vp, _vp]),

@@ -6,12 +6,13 @@ device array, problem q at element offset q*N*N.  Every kernel launch covers
 all of them; each problem has its own tolerance and device-side convergence
 mask, so a converged problem's tiles fall through while the others iterate.
 
-``UnevenBatch`` solves problems of mixed (N, p): it groups them by (N, p),
-keeps one stacked handle per group, and drives the groups concurrently, each
-from its own host thread on its own HIP stream (default priority), so the
-small groups' launches overlap the large groups' on the GPU.  Per-problem
-times are device wall-clock stamps from the group's solve start to that
-problem's convergence.
+``RaggedBatch`` wraps one ragged handle (sbp_create_ragged): problems of one
+order and uneven sides whose multigrid levels are shared, so each V-cycle
+pass is one launch over every problem present on that level.
+``UnevenBatch`` solves problems of mixed (N, p) with one ragged handle per
+order, the orders running concurrently on their own streams.  Per-problem
+times are device wall-clock stamps from the solve start to that problem's
+convergence.
 
 No counterpart in the reference, which solves one problem per call of
 matrix_free_MGCG (initial_guess_cg.jl:303-346).
@@ -199,12 +200,106 @@ class SbpBatch:
         return x, BatchResult(list(oi), list(rn), list(sec), st, list(sts), list(ii))
 
 
+class RaggedBatch:
+    """Problems of one order p and uneven sides in one ragged handle
+    (sbp_create_ragged): the multigrid levels are shared across sizes, so
+    every V-cycle pass is one launch over all the problems present on its
+    level.  Problems are held largest first (``rank``); grid functions are
+    ragged tensors, problem of rank k at element offset ``offset[k]``."""
+
+    def __init__(self, p: int, sides, device=None, tuned: bool = True, **params):
+        self.lib = capi.load()
+        self.p = int(p)
+        sides = [int(N) for N in sides]
+        # rank -> index in `sides` (stable, largest first)
+        self.order = sorted(range(len(sides)), key=lambda q: -sides[q])
+        self.rank = [0] * len(sides)
+        for k, q in enumerate(self.order):
+            self.rank[q] = k
+        self.sides = [sides[q] for q in self.order]
+        self.nprob = len(sides)
+        self.offset = [0]
+        for N in self.sides:
+            self.offset.append(self.offset[-1] + N * N)
+        self.total = self.offset[-1]
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.params = capi.default_params(**params)
+        h = C.c_void_p()
+        arr = (C.c_int * self.nprob)(*self.sides)
+        with torch.cuda.device(self.device):
+            capi.check(self.lib.sbp_create_ragged(self.p, self.nprob, arr, C.byref(self.params), C.byref(h)),
+                       "sbp_create_ragged")
+        self._h = h
+        nl = C.c_int()
+        self.lib.sbp_get_info(self._h, None, None, C.byref(nl))
+        self.nlevels = nl.value
+        self.stream = None
+        if tuned and not (params.get("tile_i") or params.get("tile_j")):
+            self.apply_tuned_tiles()
+
+    close = SbpBatch.close
+    __del__ = SbpBatch.__del__
+    _bind_stream = SbpBatch._bind_stream
+    _join = SbpBatch._join
+    _results = SbpBatch._results
+
+    def level_size(self, level):
+        N = self.sides[0]
+        for _ in range(level):
+            N = (N + 1) // 2
+        return N
+
+    apply_tuned_tiles = SbpBatch.apply_tuned_tiles
+
+    def view(self, t, k):
+        """The slice of ragged tensor t that holds the problem of rank k."""
+        return t[self.offset[k]:self.offset[k + 1]]
+
+    def _chk(self, t, name):
+        if t.device != self.device or not t.is_contiguous() or t.numel() != self.total or t.dtype != torch.float64:
+            raise ValueError(f"{name}: need a contiguous float64 tensor of {self.total} elements on {self.device}")
+        return t
+
+    def _abstol(self, abstol):
+        if isinstance(abstol, (int, float)):
+            abstol = [float(abstol)] * self.nprob
+        if len(abstol) != self.nprob:
+            raise ValueError(f"abstol: need {self.nprob} values (by rank)")
+        return (C.c_double * self.nprob)(*[float(a) for a in abstol])
+
+    def mgpcg(self, b, x=None, abstol=1e-8, maxiter=500, raise_on_fail=True):
+        """FP64 MG-PCG of every problem (b, x ragged, abstol by rank)."""
+        self._chk(b, "b")
+        x = torch.zeros_like(b) if x is None else self._chk(x, "x")
+        it, _, rn, sec, sts = self._results()
+        tol = self._abstol(abstol)
+        self._bind_stream()
+        st = self.lib.sbp_mgpcg_batch(self._h, _ptr(b), _ptr(x), tol, int(maxiter), it, rn, sec, sts)
+        self._join()
+        if raise_on_fail and st not in (capi.SBP_OK, capi.SBP_ERR_NOT_CONVERGED):
+            capi.check(st, "sbp_mgpcg_batch")
+        return x, BatchResult(list(it), list(rn), list(sec), st, list(sts))
+
+    def mgpcg_mixed(self, b, x=None, abstol=1e-8, max_outer=None, raise_on_fail=True):
+        """FP32 inner MG-PCG inside FP64 refinement, every problem."""
+        self._chk(b, "b")
+        x = torch.zeros_like(b) if x is None else self._chk(x, "x")
+        max_outer = self.params.max_outer if max_outer is None else max_outer
+        oi, ii, rn, sec, sts = self._results()
+        tol = self._abstol(abstol)
+        self._bind_stream()
+        st = self.lib.sbp_mgpcg_mixed_batch(self._h, _ptr(b), _ptr(x), tol, int(max_outer), oi, ii, rn, sec, sts)
+        self._join()
+        if raise_on_fail and st not in (capi.SBP_OK, capi.SBP_ERR_NOT_CONVERGED):
+            capi.check(st, "sbp_mgpcg_mixed_batch")
+        return x, BatchResult(list(oi), list(rn), list(sec), st, list(sts), list(ii))
+
+
 @dataclass
 class _Group:
-    N: int
     p: int
-    members: list                  # problem indices in submission order
-    op: SbpBatch | None = None
+    members: list                  # problem indices (submission order)
+    op: RaggedBatch | None = None
     stream: torch.cuda.Stream | None = None
     b: torch.Tensor | None = None
     x: torch.Tensor | None = None
@@ -216,64 +311,70 @@ class _Group:
 class UnevenBatch:
     """Problems of mixed side and order, solved together.
 
-    specs: sequence of (N, p).  The groups' handles, right-hand-side and
-    solution stacks are allocated once here; ``set_rhs`` fills problem q's
-    right-hand side and ``solve`` runs every group concurrently."""
+    specs: sequence of (N, p).  One ragged handle per order holds all of that
+    order's problems (shared multigrid levels: one launch per pass over every
+    problem present on a level); the orders run concurrently, each from its
+    own host thread on its own HIP stream (default priority), replaying its
+    own captured iteration graph.  ``set_rhs`` fills problem q's right-hand
+    side and ``solve`` runs every problem."""
 
     def __init__(self, specs, device=None, tuned=True, **params):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.specs = [(int(N), int(p)) for N, p in specs]
-        groups = {}
-        for q, key in enumerate(self.specs):
-            groups.setdefault(key, []).append(q)
+        self.params = params
+        orders = {}
+        for q, (N, p) in enumerate(self.specs):
+            orders.setdefault(p, []).append(q)
         self.groups = []
-        self.where = [None] * len(self.specs)    # problem -> (group, slot)
-        # largest groups first: their handles are launched first
-        for (N, p), members in sorted(groups.items(), key=lambda kv: -kv[0][0] ** 2 * len(kv[1])):
-            g = _Group(N, p, members)
-            g.op = SbpBatch(N, p, len(members), device=self.device, tuned=tuned, **params)
+        self.where = [None] * len(self.specs)    # problem -> (group, rank)
+        # the order with the most points first: its launches are issued first
+        for p, members in sorted(orders.items(), key=lambda kv: -sum(self.specs[q][0] ** 2 for q in kv[1])):
+            g = _Group(p, members)
+            g.op = RaggedBatch(p, [self.specs[q][0] for q in members], device=self.device, tuned=tuned, **params)
             g.stream = torch.cuda.Stream(device=self.device)
             g.op.stream = g.stream
-            n = N * N * len(members)
-            g.b = torch.zeros(n, dtype=torch.float64, device=self.device)
-            g.x = torch.zeros(n, dtype=torch.float64, device=self.device)
-            for slot, q in enumerate(members):
-                self.where[q] = (len(self.groups), slot)
+            g.b = torch.zeros(g.op.total, dtype=torch.float64, device=self.device)
+            g.x = torch.zeros(g.op.total, dtype=torch.float64, device=self.device)
+            for i, q in enumerate(members):
+                self.where[q] = (len(self.groups), g.op.rank[i])
             self.groups.append(g)
+        self._single = {}
 
     def __len__(self):
         return len(self.specs)
 
-    def _view(self, t, q):
-        gi, slot = self.where[q]
+    def _view(self, t_of_group, q):
+        gi, k = self.where[q]
         g = self.groups[gi]
-        n = g.N * g.N
-        return t[slot * n:(slot + 1) * n]
+        return g.op.view(t_of_group(g), k)
 
     def rhs(self, q):
-        """Problem q's right-hand-side slot (a view into its group's stack)."""
-        return self._view(self.groups[self.where[q][0]].b, q)
+        """Problem q's right-hand-side slot (a view into its order's ragged tensor)."""
+        return self._view(lambda g: g.b, q)
 
     def solution(self, q):
-        return self._view(self.groups[self.where[q][0]].x, q)
+        return self._view(lambda g: g.x, q)
 
     def set_rhs(self, q, b):
         self.rhs(q).copy_(b)
 
-    def op(self, q):
-        """The stacked handle that holds problem q."""
-        return self.groups[self.where[q][0]].op
+    def single(self, q):
+        """A single-problem handle of problem q's side and order (for its
+        right-hand side and norms; cached per (N, p))."""
+        from .solver import SbpPoisson
+        key = self.specs[q]
+        if key not in self._single:
+            self._single[key] = SbpPoisson(key[0], key[1], device=self.device, tuned=False)
+        return self._single[key]
+
+    def hnorm(self, q, v):
+        return self.single(q).hnorm(v)
 
     def set_fourier(self, q, seed):
         """Problem q's right-hand side from the seeded Fourier-mode data."""
         from .data import fourier_fields
         N, _ = self.specs[q]
-        op = self.op(q)
-        saved, op.stream = op.stream, None   # torch's current stream: ordered after the field generation
-        try:
-            op.assemble_rhs(*fourier_fields(N, seed, self.device), out=self.rhs(q))
-        finally:
-            op.stream = saved
+        self.single(q).assemble_rhs(*fourier_fields(N, seed, self.device), out=self.rhs(q))
 
     def solve(self, abstol, mixed=True, maxiter=500, max_outer=None, zero_x=True):
         """Solves every problem; abstol[q] per problem.  Returns a list of
@@ -285,7 +386,9 @@ class UnevenBatch:
                 with torch.cuda.stream(g.stream):
                     if zero_x:
                         g.x.zero_()
-                    tol = [abstol[q] for q in g.members]
+                    tol = [0.0] * len(g.members)
+                    for i, q in enumerate(g.members):
+                        tol[g.op.rank[i]] = abstol[q]
                     if mixed:
                         _, g.result = g.op.mgpcg_mixed(g.b, g.x, tol, max_outer, raise_on_fail=False)
                     else:
@@ -301,15 +404,18 @@ class UnevenBatch:
         torch.cuda.synchronize(self.device)
         for g in self.groups:
             if g.error is not None:
-                raise g.error
+                e, g.error = g.error, None
+                raise e
         out = [None] * len(self.specs)
         for g in self.groups:
             r = g.result
-            for slot, q in enumerate(g.members):
-                d = {"N": g.N, "p": g.p, "iters": r.iters[slot], "resnorm": r.resnorm[slot],
-                     "seconds": r.seconds[slot], "status": r.statuses[slot]}
+            for i, q in enumerate(g.members):
+                k = g.op.rank[i]
+                N, p = self.specs[q]
+                d = {"N": N, "p": p, "iters": r.iters[k], "resnorm": r.resnorm[k],
+                     "seconds": r.seconds[k], "status": r.statuses[k]}
                 if r.inner_iters is not None:
-                    d["inner_iters"] = r.inner_iters[slot]
+                    d["inner_iters"] = r.inner_iters[k]
                 out[q] = d
         return out
 
@@ -317,3 +423,6 @@ class UnevenBatch:
         for g in self.groups:
             if g.op is not None:
                 g.op.close()
+        for h in self._single.values():
+            h.close()
+        self._single = {}

@@ -59,6 +59,20 @@ struct Level {
 struct sbp_handle_s {
     int N = 0, p = 0, nlev = 0, nprob = 1;
     bool mg = false;
+    // Problems present on each level: all nprob for a stacked handle; for a
+    // ragged handle (sbp_create_ragged: problems of several sides, sorted by
+    // side, largest first) the problems whose side is at least the level's.
+    // Problem rank q sits in slot q of every level's stacks.  j0[l] is the
+    // first rank whose finest level is l (np[l] - j0[l] of them join there);
+    // koff[l] is the element offset of rank j0[l] in the ragged layout of the
+    // caller's arrays (problem q after all problems of rank < q, N_q^2 each).
+    bool ragged = false;
+    int np[kMaxLevels] = {};
+    int j0[kMaxLevels] = {};
+    size_t koff[kMaxLevels] = {};
+    size_t rtot = 0;             // ragged: total points over the problems
+    size_t roff[kMaxLevels] = {};   // ragged: partial-slot offset of level l (np[l] * npl[l] slots)
+    size_t npl[kMaxLevels] = {};    // ragged: partial slots per problem on level l
     int bottom_l0 = -1;          // first level run by the single-workgroup bottom kernel (-1: none)
     sbp_params prm{};
     hipStream_t st = nullptr;
@@ -174,7 +188,7 @@ template <> inline int prec_index<float>() { return 1; }
 
 inline Batch batch_of(const sbp_handle h, int l) {
     Batch bt;
-    bt.nprob = h->nprob;
+    bt.nprob = h->np[l];
     bt.sf = (long long)h->L[l].n;
     bt.sc = l + 1 < h->nlev ? (long long)h->L[l + 1].n : 0;
     bt.ss = kScalarDoubles;
@@ -182,12 +196,14 @@ inline Batch batch_of(const sbp_handle h, int l) {
 }
 
 inline dim3 ew_grid(const sbp_handle h, size_t n) { return dim3(ew_blocks(n), h->nprob); }
+// The same over the problems present on level l.
+inline dim3 ew_grid_l(const sbp_handle h, int l) { return dim3(ew_blocks(h->L[l].n), h->np[l]); }
 
 // One stencil pass on level l over every problem; *nparts receives the
 // number of block partials per problem.
 template <typename T, class IN, class OUT>
 int launch_pass(sbp_handle h, int l, const IN& in, const OUT& out, const Scalars* scb, double* partial,
-                int* nparts) {
+                int* nparts, int nprob = -1) {
     const Level& L = h->L[l];
     const int ti = L.ti[prec_index<T>()], tj = L.tj[prec_index<T>()];
     if (nparts) *nparts = (int)pass_blocks(L.N, ti, tj);
@@ -197,6 +213,7 @@ int launch_pass(sbp_handle h, int l, const IN& in, const OUT& out, const Scalars
     c.partial = partial;
     c.nbx = c.nby = 0;
     c.bt = batch_of(h, l);
+    if (nprob >= 0) c.bt.nprob = nprob;   // a subset of the level's problems (pointers pre-offset)
 #define SBP_TILE_CASE(TI_, J_) \
     if (ti == TI_ && tj == J_) return launch_tile<TI_, J_, T, IN, OUT>(h->p, L.tb, in, out, c, h->st);
     SBP_TILE_SHAPES(SBP_TILE_CASE)
@@ -225,13 +242,12 @@ int alloc_level_buffers(sbp_handle h, int prec) {
     const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
     for (int l = 0; l < h->nlev; ++l) {
         Level& L = h->L[l];
-        const size_t bytes = L.n * es * h->nprob;
-        if (l > 0) HIPCHK(hipMalloc(&L.b[prec], bytes));
+        const size_t bytes = L.n * es * h->np[l];
+        if (l > 0 || h->ragged) HIPCHK(hipMalloc(&L.b[prec], bytes));
         HIPCHK(hipMalloc(&L.xo[prec], bytes));
         if (l < h->nlev - 1) {
             HIPCHK(hipMalloc(&L.xa[prec], bytes));
             HIPCHK(hipMalloc(&L.xb[prec], bytes));
-            HIPCHK(hipMalloc(&L.r[prec], bytes));
         }
     }
     return SBP_OK;
@@ -262,7 +278,7 @@ inline bool fuse_level(const sbp_handle h, int l) {
     if (h->p > 6 || h->prm.drop_coarse_rhs) return false;   // the two-sweep kernels are compiled for p = 2, 4 (and 6, 64-wide blocks)
     if (!fused_shape_ok(h->L[l].fti[0], h->L[l].ftj[0], h->p) || !fused_shape_ok(h->L[l].fti[1], h->L[l].ftj[1], h->p))
         return false;
-    const long long pts = (long long)h->L[l].n * h->nprob;
+    const long long pts = (long long)h->L[l].n * h->np[l];
     return pts <= (long long)h->prm.fuse_n * h->prm.fuse_n;
 }
 
@@ -358,11 +374,11 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
     if (l == h->bottom_l0) return launch_bottom<T>(h, b, out, dotv, scb, partial);
     if (l == h->nlev - 1) {
         if (h->coarse_nested) return coarse_exact<T>(h, b, out, dotv, partial);
-        hipLaunchKernelGGL((coarse_solve<T>), dim3(((int)L.n + kCoarseRows - 1) / kCoarseRows, h->nprob), dim3(256), 0,
+        hipLaunchKernelGGL((coarse_solve<T>), dim3(((int)L.n + kCoarseRows - 1) / kCoarseRows, h->np[l]), dim3(256), 0,
                            st, h->Ainv, b, out, (int)L.n, scb);
         HIPCHK(hipGetLastError());
         if (dotv) {
-            hipLaunchKernelGGL((dot_partial<T, kEwThreads>), dim3(1, h->nprob), dim3(kEwThreads), 0, st, dotv,
+            hipLaunchKernelGGL((dot_partial<T, kEwThreads>), dim3(1, h->np[l]), dim3(kEwThreads), 0, st, dotv,
                                (const T*)out, L.n, partial);
             HIPCHK(hipGetLastError());
         }
@@ -385,7 +401,7 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
                                                   (T)h->prm.restrict_scale, done, kScalarInts, batch_of(h, l), st);
         if (rc != SBP_OK) return rc;
     } else if (nu1 == 0) {
-        hipLaunchKernelGGL((fill<T, kEwThreads>), ew_grid(h, L.n), dim3(kEwThreads), 0, st, xa, T(0), L.n, scb);
+        hipLaunchKernelGGL((fill<T, kEwThreads>), ew_grid_l(h, l), dim3(kEwThreads), 0, st, xa, T(0), L.n, scb);
         HIPCHK(hipGetLastError());
     } else {
         const InSmooth0<T> in0{b, om, h->prm.smoother == 1 ? 0 : 1};
@@ -407,20 +423,23 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
     if (!fused_pre)
         STCHK(launch_rr<T>(h, l, (const T*)cur, (l > 0 && h->prm.drop_coarse_rhs) ? nullptr : b, (T*)Lc.b[pi], scb));
     T* ec = (T*)Lc.xo[pi];
-    STCHK(vcycle_level<T>(h, l + 1, (const T*)Lc.b[pi], ec, nullptr, scb, nullptr));
+    // ragged: problems whose finest level is l + 1 take r'z from that level's last pass
+    const bool rz_c = h->ragged && h->np[l + 1] > h->j0[l + 1];
+    STCHK(vcycle_level<T>(h, l + 1, (const T*)Lc.b[pi], ec, rz_c ? (const T*)Lc.b[pi] : nullptr, scb,
+                          rz_c ? h->partial + h->roff[l + 1] : nullptr));
     // Prolongation folded into the first post-smoothing pass.
     if (fused_post)
         return launch_prolong_smooth2<T>(h->p, L.fti[pi], L.ftj[pi], L.tb, (const T*)cur, (const T*)ec, b, out, dotv, om, use_diag,
                                          done, kScalarInts, partial, batch_of(h, l), st);
     if (nu2 == 0) {
-        hipLaunchKernelGGL((prolong_add<T>), dim3((L.N + 255) / 256, L.N, h->nprob), dim3(256), 0, st, (const T*)ec,
+        hipLaunchKernelGGL((prolong_add<T>), dim3((L.N + 255) / 256, L.N, h->np[l]), dim3(256), 0, st, (const T*)ec,
                            cur, L.N, Lc.N, scb);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), ew_grid(h, L.n), dim3(kEwThreads), 0, st, (const T*)cur, out,
+        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), ew_grid_l(h, l), dim3(kEwThreads), 0, st, (const T*)cur, out,
                            L.n, scb);
         HIPCHK(hipGetLastError());
         if (dotv) {
-            hipLaunchKernelGGL((dot_partial<T, kEwThreads>), ew_grid(h, L.n), dim3(kEwThreads), 0, st, dotv,
+            hipLaunchKernelGGL((dot_partial<T, kEwThreads>), ew_grid_l(h, l), dim3(kEwThreads), 0, st, dotv,
                                (const T*)out, L.n, partial);
             HIPCHK(hipGetLastError());
         }
@@ -442,12 +461,12 @@ int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const S
     return SBP_OK;
 }
 
-// Number of partials per problem the V-cycle's final pass writes on level 0.
+// Number of partials per problem the V-cycle's final pass writes on level l.
 template <typename T>
-int vcycle_nparts(sbp_handle h) {
-    const Level& L = h->L[0];
-    if (h->nlev == 1 || h->bottom_l0 == 0) return 1;
-    if (h->prm.nu_post == 2 && fuse_level(h, 0))
+int vcycle_nparts(sbp_handle h, int l = 0) {
+    const Level& L = h->L[l];
+    if (l == h->nlev - 1 || h->bottom_l0 == l) return 1;
+    if (h->prm.nu_post == 2 && fuse_level(h, l))
         return fused_post_blocks(L.N, L.fti[prec_index<T>()], L.ftj[prec_index<T>()]);
     if (h->prm.nu_post == 0) return ew_blocks(L.n);
     return (int)pass_blocks(L.N, L.ti[prec_index<T>()], L.tj[prec_index<T>()]);
@@ -498,7 +517,7 @@ int batch_status(sbp_handle h) {
 
 int ensure32(sbp_handle h) {
     if (h->have32) return SBP_OK;
-    const size_t n = h->L[0].n * h->nprob;
+    const size_t n = h->ragged ? h->rtot : h->L[0].n * h->nprob;
     HIPCHK(hipMalloc(&h->r32, n * sizeof(float)));
     HIPCHK(hipMalloc(&h->p32, n * sizeof(float)));
     HIPCHK(hipMalloc(&h->p32b, n * sizeof(float)));
@@ -530,6 +549,84 @@ template <> KVecs<float> kvecs<float>(sbp_handle h) {
     return v;
 }
 
+// Ragged handles: the Krylov vectors of the problems whose finest level is l
+// (ranks j0[l] .. np[l]-1).  x, p[2] and Ap are the handle's ragged arrays
+// (those problems from element koff[l]); r and z are their slots of level
+// l's right-hand-side and output stacks, which the V-cycle reads and writes
+// (r is the V-cycle's right-hand side on that level, z its result).
+template <typename T>
+KVecs<T> rkvecs(sbp_handle h, int l) {
+    const int pi = prec_index<T>();
+    const Level& L = h->L[l];
+    const size_t off = h->koff[l];
+    const size_t slot = (size_t)h->j0[l] * L.n;
+    const KVecs<T> v0 = kvecs<T>(h);
+    KVecs<T> v;
+    v.x = v0.x + off;
+    v.p[0] = v0.p[0] + off;
+    v.p[1] = v0.p[1] + off;
+    v.Ap = v0.Ap + off;
+    v.r = (T*)L.b[pi] + slot;
+    v.z = (T*)L.xo[pi] + slot;
+    return v;
+}
+
+inline int joiners(const sbp_handle h, int l) { return h->np[l] - h->j0[l]; }
+inline double* rpart(sbp_handle h, int l) { return h->partial + h->roff[l]; }
+
+// The V-cycle of every problem of a ragged handle (all start on the level of
+// their own side; each level's last pass leaves r'z partials of the problems
+// that joined there).
+template <typename T>
+int vcycle_ragged(sbp_handle h, Scalars* sc) {
+    const int pi = prec_index<T>();
+    Level& L0 = h->L[0];
+    const bool rz0 = joiners(h, 0) > 0;
+    return vcycle_level<T>(h, 0, (const T*)L0.b[pi], (T*)L0.xo[pi], rz0 ? (const T*)L0.b[pi] : nullptr, sc,
+                           rz0 ? rpart(h, 0) : nullptr);
+}
+
+// One Krylov iteration of a ragged handle: per level with joiners the fused
+// search-direction pass, alpha, the residual update and the convergence test
+// (one launch each over that level's joiners), then one V-cycle over every
+// problem and beta per level.
+template <typename T>
+int issue_iteration_rag(sbp_handle h, Scalars* sc, bool use_mg, int par) {
+    hipStream_t st = h->st;
+    for (int l = 0; l < h->nlev; ++l) {
+        const int nj = joiners(h, l);
+        if (nj <= 0) continue;
+        const KVecs<T> v = rkvecs<T>(h, l);
+        Scalars* s = sc + h->j0[l];
+        double* part = rpart(h, l);
+        const size_t n = h->L[l].n;
+        T* pold = v.p[par ^ 1];
+        T* pnew = v.p[par];
+        const T* zz = use_mg ? v.z : v.r;
+        int np = 0;
+        STCHK(launch_pass<T>(h, l, InAxpy<T>{zz, pold, &s->beta, T(0)},
+                             OutPUpd<T>{pnew, v.Ap, v.x, pold, &s->alpha, T(0)}, s, part, &np, nj));
+        hipLaunchKernelGGL(krylov_alpha, dim3(nj), dim3(kRedThreads), 0, st, part, np, s, use_mg ? 1 : 0);
+        const dim3 g(ew_blocks(n), nj);
+        hipLaunchKernelGGL((update_r<T, kEwThreads>), g, dim3(kEwThreads), 0, st, v.r, (const T*)v.Ap,
+                           (const Scalars*)s, n, part);
+        hipLaunchKernelGGL(krylov_check, dim3(nj), dim3(kRedThreads), 0, st, part, (int)g.x, s, use_mg ? 0 : 1);
+        HIPCHK(hipGetLastError());
+    }
+    if (use_mg) {
+        STCHK(vcycle_ragged<T>(h, sc));
+        for (int l = 0; l < h->nlev; ++l) {
+            const int nj = joiners(h, l);
+            if (nj <= 0) continue;
+            const int nvp = vcycle_nparts<T>(h, l);
+            hipLaunchKernelGGL(krylov_beta, dim3(nj), dim3(kRedThreads), 0, st, rpart(h, l) + (size_t)h->j0[l] * nvp,
+                               nvp, sc + h->j0[l]);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    return SBP_OK;
+}
+
 // Iteration k (1-based, k % 2 == par): p_k = z_{k-1} + beta p_{k-1} formed on
 // the fly inside the apply, which also writes A p_k, accumulates p_k'A p_k and
 // applies the pending x += alpha_{k-1} p_{k-1}; then alpha, r -= alpha A p_k
@@ -537,6 +634,7 @@ template <> KVecs<float> kvecs<float>(sbp_handle h) {
 // Every kernel tests its problem's "done" flag.
 template <typename T>
 int issue_iteration(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, int par) {
+    if (h->ragged) return issue_iteration_rag<T>(h, sc, use_mg, par);
     hipStream_t st = h->st;
     const size_t n = h->L[0].n;
     T* pold = v.p[par ^ 1];
@@ -621,17 +719,38 @@ int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use
     hipStream_t st = h->st;
     const size_t n = h->L[0].n;
     const T* z0 = v.r;
-    if (use_mg) {
-        STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, sc, h->partial));
-        hipLaunchKernelGGL(krylov_set_rz, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, vcycle_nparts<T>(h),
-                           sc);
+    if (h->ragged) {
+        if (use_mg) {
+            STCHK(vcycle_ragged<T>(h, sc));
+            for (int l = 0; l < h->nlev; ++l) {
+                const int nj = joiners(h, l);
+                if (nj <= 0) continue;
+                const int nvp = vcycle_nparts<T>(h, l);
+                hipLaunchKernelGGL(krylov_set_rz, dim3(nj), dim3(kRedThreads), 0, st,
+                                   rpart(h, l) + (size_t)h->j0[l] * nvp, nvp, sc + h->j0[l]);
+            }
+        }
+        for (int l = 0; l < h->nlev; ++l) {
+            const int nj = joiners(h, l);
+            if (nj <= 0) continue;
+            const KVecs<T> vl = rkvecs<T>(h, l);
+            hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(h->L[l].n), nj), dim3(kEwThreads), 0, st,
+                               (const T*)(use_mg ? vl.z : vl.r), vl.p[0], h->L[l].n, (const Scalars*)(sc + h->j0[l]));
+        }
         HIPCHK(hipGetLastError());
-        z0 = v.z;
+    } else {
+        if (use_mg) {
+            STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, sc, h->partial));
+            hipLaunchKernelGGL(krylov_set_rz, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial,
+                               vcycle_nparts<T>(h), sc);
+            HIPCHK(hipGetLastError());
+            z0 = v.z;
+        }
+        // p_0 := z_0 (any finite values serve: alpha = beta = 0 in the first pass)
+        hipLaunchKernelGGL((copy_vec<T, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, z0, v.p[0], n,
+                           (const Scalars*)sc);
+        HIPCHK(hipGetLastError());
     }
-    // p_0 := z_0 (any finite values serve: alpha = beta = 0 in the first pass)
-    hipLaunchKernelGGL((copy_vec<T, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, z0, v.p[0], n,
-                       (const Scalars*)sc);
-    HIPCHK(hipGetLastError());
     hipGraphExec_t ex[2] = {nullptr, nullptr};
     // (a nested coarse solve synchronises with the host: no capture)
     if (h->prm.use_graph && !(use_mg && h->coarse_nested)) {
@@ -658,8 +777,19 @@ int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use
             next = it + (adaptive ? poll_ahead(h) : h->prm.poll_every);
         }
     }
-    hipLaunchKernelGGL((flush_x<T, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, v.x, (const T*)v.p[0],
-                       (const T*)v.p[1], (const Scalars*)sc, n);
+    if (h->ragged) {
+        for (int l = 0; l < h->nlev; ++l) {
+            const int nj = joiners(h, l);
+            if (nj <= 0) continue;
+            const KVecs<T> vl = rkvecs<T>(h, l);
+            hipLaunchKernelGGL((flush_x<T, kEwThreads>), dim3(ew_blocks(h->L[l].n), nj), dim3(kEwThreads), 0, st,
+                               vl.x, (const T*)vl.p[0], (const T*)vl.p[1], (const Scalars*)(sc + h->j0[l]),
+                               h->L[l].n);
+        }
+    } else {
+        hipLaunchKernelGGL((flush_x<T, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, v.x, (const T*)v.p[0],
+                           (const T*)v.p[1], (const Scalars*)sc, n);
+    }
     HIPCHK(hipGetLastError());
     return poll(h, sc);
 }
@@ -691,18 +821,34 @@ int krylov64(sbp_handle h, const double* b, double* x, const double* abstol, int
     hipStream_t st = h->st;
     STCHK(put_abstol(h, abstol));
     hipLaunchKernelGGL(stamp_start, dim3(1), dim3(256), 0, st, h->sc, h->nprob);
-    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, x, h->x, n,
+    // (ragged: x and h->x are ragged arrays of rtot elements)
+    const dim3 gx = h->ragged ? dim3(ew_blocks(h->rtot), 1) : ew_grid(h, n);
+    const size_t nx = h->ragged ? h->rtot : n;
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), gx, dim3(kEwThreads), 0, st, x, h->x, nx,
                        (const Scalars*)nullptr);
     HIPCHK(hipGetLastError());
-    int np = 0;
-    STCHK(launch_pass<double>(h, 0, InPlain<double>{h->x}, OutResid<double, double>{b, h->r}, nullptr, h->partial,
-                              &np));
-    hipLaunchKernelGGL(krylov_init, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, np, h->sc,
-                       (const double*)h->dabs, hist, maxiter);
+    if (h->ragged) {
+        for (int l = 0; l < h->nlev; ++l) {
+            const int nj = joiners(h, l);
+            if (nj <= 0) continue;
+            const KVecs<double> vl = rkvecs<double>(h, l);
+            int np = 0;
+            STCHK(launch_pass<double>(h, l, InPlain<double>{vl.x}, OutResid<double, double>{b + h->koff[l], vl.r},
+                                      nullptr, rpart(h, l), &np, nj));
+            hipLaunchKernelGGL(krylov_init, dim3(nj), dim3(kRedThreads), 0, st, rpart(h, l), np, h->sc + h->j0[l],
+                               (const double*)h->dabs + h->j0[l], (double*)nullptr, maxiter);
+        }
+    } else {
+        int np = 0;
+        STCHK(launch_pass<double>(h, 0, InPlain<double>{h->x}, OutResid<double, double>{b, h->r}, nullptr,
+                                  h->partial, &np));
+        hipLaunchKernelGGL(krylov_init, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, np, h->sc,
+                           (const double*)h->dabs, hist, maxiter);
+    }
     HIPCHK(hipGetLastError());
     STCHK(pcg_loop<double>(h, kvecs<double>(h), h->sc, maxiter, use_mg));
-    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, (const double*)h->x,
-                       x, n, (const Scalars*)nullptr);
+    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), gx, dim3(kEwThreads), 0, st, (const double*)h->x,
+                       x, nx, (const Scalars*)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return SBP_OK;
@@ -827,27 +973,62 @@ int mixed_solve(sbp_handle h, const double* b, double* x, const double* abstol, 
     HIPCHK(hipGetLastError());
     for (int q = 0; q < h->nprob; ++q) inner_tot[q] = 0;
     for (int o = 0; o <= max_outer; ++o) {
-        int np = 0;
         // FP64 residual r = b - A x, cast to FP32 for the inner solve.
-        STCHK(launch_pass<double>(h, 0, InPlain<double>{x}, OutResid<double, float>{b, h->r32}, h->sc, h->partial,
-                                  &np));
-        hipLaunchKernelGGL(ir_check, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, np, h->sc, h->sc_in,
-                           h->prm.inner_rtol, h->prm.inner_maxiter);
+        if (h->ragged) {
+            for (int l = 0; l < h->nlev; ++l) {
+                const int nj = joiners(h, l);
+                if (nj <= 0) continue;
+                int np = 0;
+                STCHK(launch_pass<double>(h, l, InPlain<double>{x + h->koff[l]},
+                                          OutResid<double, float>{b + h->koff[l], rkvecs<float>(h, l).r},
+                                          h->sc + h->j0[l], rpart(h, l), &np, nj));
+                hipLaunchKernelGGL(ir_check, dim3(nj), dim3(kRedThreads), 0, st, rpart(h, l), np, h->sc + h->j0[l],
+                                   h->sc_in + h->j0[l], h->prm.inner_rtol, h->prm.inner_maxiter);
+            }
+        } else {
+            int np = 0;
+            STCHK(launch_pass<double>(h, 0, InPlain<double>{x}, OutResid<double, float>{b, h->r32}, h->sc,
+                                      h->partial, &np));
+            hipLaunchKernelGGL(ir_check, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, np, h->sc, h->sc_in,
+                               h->prm.inner_rtol, h->prm.inner_maxiter);
+        }
         HIPCHK(hipGetLastError());
         bool all = false;
         STCHK(poll(h, h->sc, &all));
         if (all || o == max_outer) break;
         // Inner FP32 MG-PCG on A d = r from d = 0.
-        hipLaunchKernelGGL((fill<float, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, h->d32, 0.0f, n,
-                           (const Scalars*)h->sc_in);
+        if (h->ragged) {
+            for (int l = 0; l < h->nlev; ++l) {
+                const int nj = joiners(h, l);
+                if (nj <= 0) continue;
+                hipLaunchKernelGGL((fill<float, kEwThreads>), dim3(ew_blocks(h->L[l].n), nj), dim3(kEwThreads), 0, st,
+                                   h->d32 + h->koff[l], 0.0f, h->L[l].n, (const Scalars*)(h->sc_in + h->j0[l]));
+            }
+        } else {
+            hipLaunchKernelGGL((fill<float, kEwThreads>), ew_grid(h, n), dim3(kEwThreads), 0, st, h->d32, 0.0f, n,
+                               (const Scalars*)h->sc_in);
+        }
         HIPCHK(hipGetLastError());
         STCHK(pcg_loop<float>(h, kvecs<float>(h), h->sc_in, h->prm.inner_maxiter, true));
         for (int q = 0; q < h->nprob; ++q) inner_tot[q] += h->sc_host[q].iters;
         // FP64 update and the error estimate from the correction norms.
-        hipLaunchKernelGGL((ir_update<kEwThreads>), dim3(ub, h->nprob), dim3(kEwThreads), 0, st, x,
-                           (const float*)h->d32, n, h->sc, (const Scalars*)h->sc_in, h->partial);
-        hipLaunchKernelGGL(ir_estimate, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, ub, h->sc,
-                           h->prm.ir_rtol);
+        if (h->ragged) {
+            for (int l = 0; l < h->nlev; ++l) {
+                const int nj = joiners(h, l);
+                if (nj <= 0) continue;
+                const int ubl = std::min<int>(ew_blocks(h->L[l].n), (int)(h->npl[l] / 2));
+                hipLaunchKernelGGL((ir_update<kEwThreads>), dim3(ubl, nj), dim3(kEwThreads), 0, st, x + h->koff[l],
+                                   (const float*)h->d32 + h->koff[l], h->L[l].n, h->sc + h->j0[l],
+                                   (const Scalars*)(h->sc_in + h->j0[l]), rpart(h, l));
+                hipLaunchKernelGGL(ir_estimate, dim3(nj), dim3(kRedThreads), 0, st, rpart(h, l), ubl, h->sc + h->j0[l],
+                                   h->prm.ir_rtol);
+            }
+        } else {
+            hipLaunchKernelGGL((ir_update<kEwThreads>), dim3(ub, h->nprob), dim3(kEwThreads), 0, st, x,
+                               (const float*)h->d32, n, h->sc, (const Scalars*)h->sc_in, h->partial);
+            hipLaunchKernelGGL(ir_estimate, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, ub, h->sc,
+                               h->prm.ir_rtol);
+        }
         HIPCHK(hipGetLastError());
     }
     return poll(h, h->sc);
@@ -1067,7 +1248,7 @@ int sbp_destroy(sbp_handle h) {
         Level& L = h->L[l];
         if (L.dtab) (void)hipFree(L.dtab);
         for (int q = 0; q < 2; ++q) {
-            if (L.b[q] && l > 0) (void)hipFree(L.b[q]);
+            if (L.b[q] && (l > 0 || h->ragged)) (void)hipFree(L.b[q]);
             if (L.xa[q]) (void)hipFree(L.xa[q]);
             if (L.xb[q]) (void)hipFree(L.xb[q]);
             if (L.xo[q]) (void)hipFree(L.xo[q]);
@@ -1104,7 +1285,8 @@ int sbp_destroy(sbp_handle h) {
 }
 
 namespace {
-int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_handle* out);
+int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_handle* out,
+                const int* sides = nullptr);
 }
 
 int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle* out) {
@@ -1112,7 +1294,9 @@ int sbp_create_batch(int N, int p, int nprob, const sbp_params* prm, sbp_handle*
 }
 
 namespace {
-int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_handle* out) {
+// sides (nullable): a ragged handle, problem q of side sides[q] (non-increasing,
+// sides[0] = N, each a level side of N's hierarchy).
+int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_handle* out, const int* sides) {
     if (!out) return SBP_ERR_INVALID_ARG;
     *out = nullptr;
     if (p != 2 && p != 4 && p != 6 && p != 8 && p != 10) return SBP_ERR_INVALID_ORDER;
@@ -1135,6 +1319,7 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
     h->N = N;
     h->p = p;
     h->nprob = nprob;
+    h->ragged = sides != nullptr;
     if (prm) h->prm = *prm; else sbp_default_params(&h->prm);
     // p = 6, 8, 10 closures need at least 2 C = 18 / 24 / 32 points per side
     // on every level: their coarsest grid is 33 x 33 at the least.
@@ -1160,9 +1345,28 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
         else nlev = 1;
     }
     h->nlev = nlev;
+    if (h->ragged) {
+        // every side must be one of the hierarchy's levels
+        if (!h->mg || h->L[nlev - 1].N > 33) { delete h; return SBP_ERR_INVALID_SIZE; }
+        size_t tot = 0;
+        int q = 0;
+        for (int l = 0; l < nlev; ++l) {
+            h->j0[l] = q;
+            h->koff[l] = tot;
+            while (q < nprob && sides[q] == h->L[l].N) { tot += h->L[l].n; ++q; }
+            h->np[l] = q;
+        }
+        if (q != nprob) { delete h; return SBP_ERR_INVALID_SIZE; }   // a side off the hierarchy
+        h->rtot = tot;
+    } else {
+        for (int l = 0; l < nlev; ++l) {
+            h->np[l] = nprob;
+            h->j0[l] = l == 0 ? 0 : nprob;
+        }
+    }
     // Bottom kernel: from the first level of side <= bottom_n down, when the
     // coarsest system is small enough for one workgroup's direct solve.
-    if (h->mg && p <= 4 && !h->prm.drop_coarse_rhs && h->L[nlev - 1].n <= (size_t)kBottomMaxCoarse) {
+    if (h->mg && p <= 4 && !h->ragged && !h->prm.drop_coarse_rhs && h->L[nlev - 1].n <= (size_t)kBottomMaxCoarse) {
         const int bn = std::min(h->prm.bottom_n, kBottomN);
         for (int l = 0; l < nlev; ++l)
             if (h->L[l].N <= bn) { h->bottom_l0 = l; break; }
@@ -1180,7 +1384,7 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
             }
             // fused two-sweep blocks: small blocks keep the per-block chain short
             L.fti[q] = 64;
-            L.ftj[q] = (long long)L.n * nprob >= 1025ll * 1025 ? 8 : 4;
+            L.ftj[q] = (long long)L.n * h->np[l] >= 1025ll * 1025 ? 8 : 4;
         }
     }
     int dev = 0, khz = 0;
@@ -1197,14 +1401,23 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
             if (hipMemcpy(L.dtab, &L.tb, sizeof(OpTab), hipMemcpyHostToDevice) != hipSuccess) rc = SBP_ERR_HIP;
         }
         if (rc != SBP_OK) break;
-        const size_t n = h->L[0].n * (size_t)nprob;
+        const size_t n = h->ragged ? h->rtot : h->L[0].n * (size_t)nprob;
         if (hipMalloc(&h->r, n * 8) != hipSuccess || hipMalloc(&h->pv, n * 8) != hipSuccess ||
             hipMalloc(&h->Ap, n * 8) != hipSuccess || hipMalloc(&h->z, n * 8) != hipSuccess ||
             hipMalloc(&h->x, n * 8) != hipSuccess || hipMalloc(&h->pv2, n * 8) != hipSuccess) { rc = SBP_ERR_HIP; break; }
         // Partial slots per problem: the finest pass at any compiled tile shape,
         // and the elementwise kernels (two per block in the refinement update).
         h->npp = std::max<size_t>(pass_blocks(N, 64, 4), 2 * 4096) + 64;
-        if (hipMalloc(&h->partial, h->npp * nprob * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
+        size_t nslots = h->npp * nprob;
+        if (h->ragged) {   // one region per level: np[l] problems x npl[l] slots
+            nslots = 0;
+            for (int l = 0; l < nlev; ++l) {
+                h->npl[l] = std::max<size_t>(pass_blocks(h->L[l].N, 64, 4), 2 * 4096) + 64;
+                h->roff[l] = nslots;
+                nslots += h->npl[l] * h->np[l];
+            }
+        }
+        if (hipMalloc(&h->partial, nslots * sizeof(double)) != hipSuccess) { rc = SBP_ERR_HIP; break; }
         if (hipMalloc(&h->sc, sizeof(Scalars) * nprob) != hipSuccess ||
             hipMalloc(&h->sc_in, sizeof(Scalars) * nprob) != hipSuccess ||
             hipMalloc(&h->dtmp, nprob * sizeof(double)) != hipSuccess ||
@@ -1250,6 +1463,19 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
 }  // namespace
 
 int sbp_create(int N, int p, const sbp_params* prm, sbp_handle* out) { return sbp_create_batch(N, p, 1, prm, out); }
+
+int sbp_create_ragged(int p, int nprob, const int* sides, const sbp_params* prm, sbp_handle* out) {
+    if (!out || !sides) return SBP_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (nprob < 1 || nprob > 65535) return SBP_ERR_INVALID_ARG;
+    long long tot = 0;
+    for (int q = 0; q < nprob; ++q) {
+        if (sides[q] < 3 || (q > 0 && sides[q] > sides[q - 1])) return SBP_ERR_INVALID_ARG;   // sorted, largest first
+        tot += (long long)sides[q] * sides[q];
+    }
+    if (tot > (1ll << 40)) return SBP_ERR_INVALID_SIZE;
+    return create_impl(sides[0], p, nprob, prm, false, out, sides);
+}
 
 namespace {
 // Multigrid hierarchy of a variable-coefficient handle: levels of side
@@ -1651,6 +1877,7 @@ int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter
 }  // namespace
 
 int sbp_apply(sbp_handle h, int level, const void* u, void* Au, int sign, int dtype) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !u || !Au || (sign != 1 && sign != -1) || level < 0 || level >= h->nlev) return SBP_ERR_INVALID_ARG;
     if (h->var) {   // variable coefficients: FP64, level 0
         if (dtype != SBP_F64 || level != 0) return SBP_ERR_INVALID_ARG;
@@ -1666,6 +1893,7 @@ int sbp_apply(sbp_handle h, int level, const void* u, void* Au, int sign, int dt
 }
 
 int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !u || !Au || !dot) return SBP_ERR_INVALID_ARG;
     int np = 0;
     if (h->var) STCHK(launch_var(h, 1, u, Au, u, 1.0, nullptr, &np));
@@ -1678,6 +1906,7 @@ int sbp_apply_dot(sbp_handle h, const double* u, double* Au, double* dot) {
 }
 
 int sbp_apply_dot_noalloc(sbp_handle h, const double* u, double* Au, double* dot) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !u || !Au || !dot || h->var) return SBP_ERR_INVALID_ARG;
     int dev = -1;
     HIPCHK(hipGetDevice(&dev));
@@ -1698,6 +1927,7 @@ int sbp_apply_dot_noalloc(sbp_handle h, const double* u, double* Au, double* dot
 }
 
 int sbp_diag(sbp_handle h, double* d) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !d || h->var) return SBP_ERR_INVALID_ARG;
     hipLaunchKernelGGL(diag_kernel, dim3(ew_blocks(h->L[0].n)), dim3(256), 0, h->st, d, h->L[0].tb);
     HIPCHK(hipGetLastError());
@@ -1706,6 +1936,7 @@ int sbp_diag(sbp_handle h, double* d) {
 
 int sbp_assemble_rhs(sbp_handle h, const double* f, const double* gW, const double* gE, const double* gS,
                      const double* gN, double* b) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !b) return SBP_ERR_INVALID_ARG;
     hipLaunchKernelGGL(assemble_rhs, dim3(ew_blocks(h->L[0].n)), dim3(256), 0, h->st, f, gW, gE, gS, gN, b,
                        h->L[0].tb, h->rt);
@@ -1715,6 +1946,7 @@ int sbp_assemble_rhs(sbp_handle h, const double* f, const double* gW, const doub
 
 int sbp_rhs_transpose(sbp_handle h, const double* lam, double* df, double* dgW, double* dgE, double* dgS,
                       double* dgN) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !lam) return SBP_ERR_INVALID_ARG;
     if (df) {
         hipLaunchKernelGGL(rhs_t_f, dim3(ew_blocks(h->L[0].n)), dim3(256), 0, h->st, lam, df, h->L[0].tb, h->rt);
@@ -1729,6 +1961,7 @@ int sbp_rhs_transpose(sbp_handle h, const double* lam, double* df, double* dgW, 
 }
 
 int sbp_hnorm(sbp_handle h, const double* v, double* out) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !v || !out) return SBP_ERR_INVALID_ARG;
     const int nb = ew_blocks(h->L[0].n);
     const double hh = 1.0 / (h->N - 1);
@@ -1741,6 +1974,7 @@ int sbp_hnorm(sbp_handle h, const double* v, double* out) {
 }
 
 int sbp_h_weights(sbp_handle h, double* hw) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !hw) return SBP_ERR_INVALID_ARG;
     const OpTab& t = h->L[0].tb;
     const int N = h->N;
@@ -1752,6 +1986,7 @@ int sbp_h_weights(sbp_handle h, double* hw) {
 }
 
 int sbp_dot(sbp_handle h, const double* u, const double* v, double* out) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !u || !v || !out) return SBP_ERR_INVALID_ARG;
     const int nb = ew_blocks(h->L[0].n);
     hipLaunchKernelGGL((dot_partial<double, 256>), dim3(nb, 1), dim3(256), 0, h->st, u, v, h->L[0].n, h->partial);
@@ -1763,6 +1998,7 @@ int sbp_dot(sbp_handle h, const double* u, const double* v, double* out) {
 }
 
 int sbp_restrict(sbp_handle h, int level, const void* rf, void* rc, int dtype) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !rf || !rc || level < 0 || level + 1 >= h->nlev) return SBP_ERR_INVALID_ARG;
     const int Nf = h->L[level].N, Nc = h->L[level + 1].N;
     const dim3 g((Nc + 255) / 256, Nc, h->nprob);
@@ -1778,6 +2014,7 @@ int sbp_restrict(sbp_handle h, int level, const void* rf, void* rc, int dtype) {
 }
 
 int sbp_prolong_add(sbp_handle h, int level, const void* ec, void* xf, int dtype) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !ec || !xf || level < 0 || level + 1 >= h->nlev) return SBP_ERR_INVALID_ARG;
     const int Nf = h->L[level].N, Nc = h->L[level + 1].N;
     const dim3 g((Nf + 255) / 256, Nf, h->nprob);
@@ -1793,6 +2030,7 @@ int sbp_prolong_add(sbp_handle h, int level, const void* ec, void* xf, int dtype
 }
 
 int sbp_smooth(sbp_handle h, int level, const void* b, void* x, int nsweeps, int dtype) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !b || !x || level < 0 || level >= h->nlev || nsweeps < 0 || h->var) return SBP_ERR_INVALID_ARG;
     if (dtype == SBP_F64) return smooth_t<double>(h, level, (const double*)b, (double*)x, nsweeps);
     if (dtype == SBP_F32) {
@@ -1803,6 +2041,7 @@ int sbp_smooth(sbp_handle h, int level, const void* b, void* x, int nsweeps, int
 }
 
 int sbp_coarse_solve(sbp_handle h, const double* b, double* x) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !b || !x || !h->mg) return SBP_ERR_INVALID_ARG;
     if (h->coarse_nested) return coarse_exact<double>(h, b, x, nullptr, nullptr);
     hipLaunchKernelGGL((coarse_solve<double>), dim3((h->nc + kCoarseRows - 1) / kCoarseRows, h->nprob), dim3(256), 0,
@@ -1812,6 +2051,7 @@ int sbp_coarse_solve(sbp_handle h, const double* b, double* x) {
 }
 
 int sbp_vcycle(sbp_handle h, const void* r, void* z, int dtype) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (h && h->var) {
         if (!r || !z || !h->vmg || dtype != SBP_F64) return SBP_ERR_INVALID_ARG;
         return var_vcycle<double>(h, 0, (const double*)r, (double*)z, nullptr);
@@ -1907,6 +2147,7 @@ int sbp_mgpcg_mixed_batch(sbp_handle h, const double* b, double* x, const double
 // Adjoint gradient: L = 1/2 (u - u_obs)' H~ (u - u_obs), u = A^-1 b(f, g).
 int sbp_adjoint_grad(sbp_handle h, const double* u, const double* u_obs, double abstol, int maxiter, double* loss,
                      double* lam, double* df, double* dgW, double* dgE, double* dgS, double* dgN, int* iters) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !u || !u_obs || maxiter < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
     if (!h->mg) return SBP_ERR_INVALID_SIZE;
     const size_t n = h->L[0].n;
@@ -1995,6 +2236,7 @@ int nested_cg(sbp_handle h, int nlev, const double* const* bs, double* x, double
 
 int sbp_cg_interp(sbp_handle h, int nlev, const double* b, const double* b2h, const double* b4h, double* x,
                   double abstol, int maxiter, int* iters, double* resnorm, double* hist) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !b || !b2h || !x || (nlev != 2 && nlev != 3) || (nlev == 3 && !b4h) || maxiter < 0 ||
         !(abstol >= 0) || h->nprob != 1)
         return SBP_ERR_INVALID_ARG;
@@ -2006,6 +2248,7 @@ int sbp_cg_interp(sbp_handle h, int nlev, const double* b, const double* b2h, co
 // CSR comparison path (one problem)
 // ---------------------------------------------------------------------------
 int sbp_csr_assemble(sbp_handle h, int32_t* row_ptr, int32_t* col, double* val, int64_t* nnz) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !nnz || h->var) return SBP_ERR_INVALID_ARG;
     const int n = (int)h->L[0].n;
     int32_t* counts = nullptr;
@@ -2036,6 +2279,7 @@ int sbp_csr_assemble(sbp_handle h, int32_t* row_ptr, int32_t* col, double* val, 
 
 int sbp_csr_spmv(sbp_handle h, const int32_t* rp, const int32_t* col, const double* val, const double* x,
                  double* y) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !rp || !col || !val || !x || !y) return SBP_ERR_INVALID_ARG;
     const int n = (int)h->L[0].n;
     const int nb = std::min(kCsrMaxBlocks, (n + 256 / kCsrLanes - 1) / (256 / kCsrLanes));
@@ -2047,6 +2291,7 @@ int sbp_csr_spmv(sbp_handle h, const int32_t* rp, const int32_t* col, const doub
 
 int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, const double* val, const double* b, double* x,
                double abstol, int maxiter, int* iters, double* resnorm) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !rp || !col || !val || !b || !x || maxiter < 0 || !(abstol >= 0) || h->nprob != 1)
         return SBP_ERR_INVALID_ARG;
     hipStream_t st = h->st;
@@ -2111,6 +2356,7 @@ int vendor_spmv(sbp_handle h, const double* x, double* y) {
 
 int sbp_csr_vendor_attach(sbp_handle h, const int32_t* rp, const int32_t* col, const double* val, int64_t nnz,
                           int alg) {
+    if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (!h || !rp || !col || !val || nnz <= 0 || h->nprob != 1 || (alg != 0 && alg != 1)) return SBP_ERR_INVALID_ARG;
     const int64_t n = (int64_t)h->L[0].n;
     if (h->vs_mat) {   // re-attach: drop the previous matrix and its analysis

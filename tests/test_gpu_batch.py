@@ -170,3 +170,91 @@ def test_uneven_batch_matches_single_and_oracle():
     out64 = ub.solve(tol, mixed=False)
     assert all(o["status"] == 0 for o in out64)
     ub.close()
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_ragged_matches_single_bitwise(p):
+    """A ragged handle (sides 129, 129, 65, 33, 33 sharing their multigrid
+    levels) gives every problem the iterates of its single-problem solve, bit
+    for bit, FP64 and mixed, with the same kernels (fuse_n = 0 on both: the
+    fused-kernel choice depends on the points per launch)."""
+    from sc_2022_amd.batch import RaggedBatch
+    from sc_2022_amd.solver import SbpPoisson
+    sides = [65, 129, 33, 129, 33]
+    seeds = [11, 12, 13, 14, 15]
+    rb = RaggedBatch(p, sides, device=DEV, fuse_n=0)
+    assert rb.sides == [129, 129, 65, 33, 33] and rb.nlevels == 4
+    b = torch.zeros(rb.total, dtype=torch.float64, device=DEV)
+    bs = {}
+    for q, (N, s) in enumerate(zip(sides, seeds)):
+        bs[q] = rhs_for(N, p, s)
+        rb.view(b, rb.rank[q]).copy_(bs[q])
+    tol = [0.0] * len(sides)
+    for q in range(len(sides)):
+        tol[rb.rank[q]] = 1e-11 * float(torch.linalg.vector_norm(bs[q]))
+    x64, r64 = rb.mgpcg(b, abstol=tol)
+    xmx, rmx = rb.mgpcg_mixed(b, abstol=tol)
+    assert r64.status == 0 and rmx.status == 0
+    for q, N in enumerate(sides):
+        k = rb.rank[q]
+        one = SbpPoisson(N, p, device=DEV, fuse_n=0)
+        x1, res1 = one.mgpcg(bs[q], abstol=tol[k])
+        assert r64.iters[k] == res1.iters and r64.resnorm[k] == res1.resnorm
+        assert torch.equal(rb.view(x64, k), x1), (q, N)
+        x2, res2 = one.mgpcg_mixed(bs[q], abstol=tol[k])
+        assert (rmx.iters[k], rmx.inner_iters[k]) == (res2.iters, res2.inner_iters)
+        assert torch.equal(rb.view(xmx, k), x2), (q, N)
+        one.close()
+    rb.close()
+
+
+def test_ragged_256_problem_batch_against_single_and_oracle():
+    """The 256-problem uneven workload (k in 6..10, p in {2, 4}, seeded
+    Fourier data) through one ragged handle per order: every problem
+    converges; a seeded sample of 12 problems matches its single-problem
+    solve (iterations within one, solutions within 1e-10) and the C oracle's
+    FP64 solution (within 1e-10)."""
+    from oracle import c_oracle
+    from sc_2022_amd.batch import UnevenBatch
+    from sc_2022_amd.data import batch_specs
+    from sc_2022_amd.solver import SbpPoisson
+    specs = batch_specs(2022, count=256)
+    ub = UnevenBatch([(N, p) for N, p, _ in specs], device=DEV)
+    assert len(ub.groups) == 2
+    for q, (N, p, s) in enumerate(specs):
+        ub.set_fourier(q, s)
+    tol = [1e-12 * float(torch.linalg.vector_norm(ub.rhs(q))) for q in range(len(specs))]
+    out = ub.solve(tol, mixed=True)
+    assert all(o["status"] == 0 and o["seconds"] > 0 for o in out)
+    c_oracle.set_threads(16)
+    rng = np.random.default_rng(7)
+    for q in sorted(rng.choice(len(specs), size=12, replace=False)):
+        N, p, s = specs[q]
+        one = SbpPoisson(N, p, device=DEV)
+        x1, r1 = one.mgpcg_mixed(ub.rhs(q).clone(), abstol=tol[q])
+        assert abs(out[q]["inner_iters"] - r1.inner_iters) <= 1, (q, out[q], r1)
+        xq = ub.solution(q)
+        assert one.hnorm(xq - x1) <= 1e-10 * one.hnorm(x1), (q, N, p)
+        bh = c_oracle.fourier_rhs(p, N, s)
+        xo = torch.as_tensor(c_oracle.solve_reference(p, N, bh)[0], device=DEV)
+        assert one.hnorm(xq - xo) <= 1e-10 * one.hnorm(xo), (q, N, p)
+        one.close()
+    ub.close()
+
+
+def test_ragged_handle_validation_and_rejections():
+    from sc_2022_amd import _capi as capi
+    lib = capi.load()
+    h = C.c_void_p()
+    arr = lambda *v: (C.c_int * len(v))(*v)
+    assert lib.sbp_create_ragged(2, 2, arr(33, 65), None, C.byref(h)) == capi.SBP_ERR_INVALID_ARG   # not sorted
+    assert lib.sbp_create_ragged(2, 2, arr(65, 40), None, C.byref(h)) == capi.SBP_ERR_INVALID_SIZE  # off the hierarchy
+    assert lib.sbp_create_ragged(2, 0, arr(65), None, C.byref(h)) == capi.SBP_ERR_INVALID_ARG
+    assert lib.sbp_create_ragged(4, 3, arr(65, 33, 33), None, C.byref(h)) == capi.SBP_OK
+    n = C.c_int()
+    assert lib.sbp_get_batch(h, C.byref(n)) == 0 and n.value == 3
+    u = torch.zeros(65 * 65 + 2 * 33 * 33, dtype=torch.float64, device=DEV)
+    pu = C.c_void_p(u.data_ptr())
+    assert lib.sbp_apply(h, 0, pu, pu, 1, 0) == capi.SBP_ERR_INVALID_ARG
+    assert lib.sbp_vcycle(h, pu, pu, 0) == capi.SBP_ERR_INVALID_ARG
+    assert lib.sbp_destroy(h) == capi.SBP_OK
