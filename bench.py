@@ -820,6 +820,9 @@ def main():
                   "copy_GBs": copy_gbs, "rotating_buffers": nbuf64},
         "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs64 / HBM_PEAK_GBS, "traffic": traffic_from_profiles("f64"),
+                     "traffic_source": "profiles/apply_traffic.json: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this "
+                                       "kernel at this size (a profiled run, not this run), scaled by a plain copy; "
+                                       "also profiles/solve_counters_4097_p4.md",
                      "kernel": "stencil_pass FP64 apply (OutStore), algorithmic bytes 2*n*8 per launch"},
         "batched": batched,
         **extra,
