@@ -89,10 +89,9 @@ template <int P> struct FusedHalo {
 // inverse diagonal is the constant idint away from the faces) or w (Richardson).
 template <typename T, bool LT, bool INNER>
 __device__ __forceinline__ T smooth_corr(const OpTab& t, int i, int j, T r, T omega, int use_diag) {
-    T c = omega * r;
-    if (!use_diag) return c;
-    if (INNER || is_interior(t, i, j)) return c * (T)t.idint;
-    return c / diag_of<T, LT>(t, i, j);
+    if (INNER || is_interior(t, i, j)) return r * smooth_scale_int(t, omega, use_diag);
+    const T c = omega * r;
+    return use_diag ? c / diag_of<T, LT>(t, i, j) : c;
 }
 
 struct FusedCtl {

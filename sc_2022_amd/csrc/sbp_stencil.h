@@ -120,6 +120,24 @@ template <> struct Stencil<10> {
 
 __device__ __forceinline__ size_t gidx(int i, int j, int N) { return (size_t)j * (size_t)N + (size_t)i; }
 
+// Storage location of grid point (i, j): the offset of its row and its column
+// (0 <= i < N where it is accessed).  The policies read and write grid
+// functions only through ld/st, so the addressing form is chosen here.
+// (Measured: buffer-resource loads with the row in the scalar base and a
+// shared 32-bit column offset cut vector registers but doubled the scalar
+// instructions per wave and ran the smoothing passes 2-17 % slower.)
+struct Loc {
+    size_t row;
+    int i;
+};
+__device__ __forceinline__ Loc loc(int i, int j, int N) { return Loc{(size_t)j * (size_t)N, i}; }
+template <typename T>
+__device__ __forceinline__ T* elem(T* a, Loc q) { return a + q.row + (size_t)q.i; }
+template <typename T>
+__device__ __forceinline__ T ld(const T* a, Loc q) { return *elem(a, q); }
+template <typename T>
+__device__ __forceinline__ void st(T* a, Loc q, T v) { *elem(a, q) = v; }
+
 // Where per-thread table lookups go: LT = false, the table's device-memory
 // copy (t.dev); LT = true, the table object itself, which the caller has
 // placed in LDS (its fields are then plain LDS reads).
@@ -153,6 +171,23 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
+// Scale of a smoothing correction away from the faces, where the inverse
+// diagonal is the constant idint: w idint (damped Jacobi) or w (Richardson).
+// Uniform over a launch, so the compiler hoists it and each point costs one
+// multiply.
+template <typename T>
+__device__ __forceinline__ T smooth_scale_int(const OpTab& t, T omega, int use_diag) {
+    return use_diag ? omega * (T)t.idint : omega;
+}
+
+// The correction w D^-1 r (or w r) at (i, j); INT: (i, j) is off the closure rows.
+template <typename T, bool INT, bool LT>
+__device__ __forceinline__ T smooth_corr_at(const OpTab& t, int i, int j, T r, T omega, int use_diag) {
+    if constexpr (INT) return r * smooth_scale_int(t, omega, use_diag);
+    const T c = omega * r;
+    return use_diag ? c / diag_of<T, LT>(t, i, j) : c;
+}
+
 // ---------------------------------------------------------------------------
 // Input policies: value of the operand at (i, j)
 // ---------------------------------------------------------------------------
@@ -168,11 +203,11 @@ template <typename T> struct InPlain {
     using Raw = T;
     __device__ void init() {}
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const { return u[gidx(i, j, t.N)]; }
+    __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const { return ld(u, loc(i, j, t.N)); }
     template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T make(const OpTab&, int, int, const Raw& r) const { return r; }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ T load(const OpTab& t, int i, int j) const { return u[gidx(i, j, t.N)]; }
+    __device__ __forceinline__ T load(const OpTab& t, int i, int j) const { return ld(u, loc(i, j, t.N)); }
 };
 
 // First smoothing sweep from x = 0: x1 = w D^-1 b (damped Jacobi) or
@@ -185,12 +220,12 @@ template <typename T> struct InSmooth0 {
     using Raw = T;
     __device__ void init() {}
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const { return b[gidx(i, j, t.N)]; }
+    __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const { return ld(b, loc(i, j, t.N)); }
     template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T make(const OpTab& t, int i, int j, const Raw& r) const {
+        if constexpr (INT) return r * smooth_scale_int(t, omega, use_diag);
         const T v = omega * r;
-        if (!use_diag) return v;
-        return INT ? v * (T)t.idint : v / diag_of<T, LT>(t, i, j);
+        return use_diag ? v / diag_of<T, LT>(t, i, j) : v;
     }
     template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
@@ -207,7 +242,8 @@ template <typename T> struct InProlong {
     __device__ void init() {}
     // Column window w[k] = x(i, jbeg+k) + (P ec)(i, jbeg+k), k < W, with jbeg of
     // parity PAR: each coarse row is loaded once (W/2+2 rows) instead of four
-    // coarse loads per fine point.  Interior tiles only (no bounds tests).
+    // coarse loads per fine point.  No bounds tests: rows jbeg .. jbeg+W-1 and
+    // the column i lie inside the grid (an odd i < N has i/2 + 1 < Nc).
     template <int W, int PAR>
     __device__ __forceinline__ void load_col(const OpTab& t, int i, int jbeg, T* w) const {
         constexpr int NC = ((PAR + W - 1) >> 1) + 2;
@@ -216,14 +252,14 @@ template <typename T> struct InProlong {
         T c[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
-            const T* e = ec + gidx(ic, jc0 + m, Nc);
-            c[m] = odd ? (e[0] + e[1]) * T(0.5) : e[0];
+            const T e0 = ld(ec, loc(ic, jc0 + m, Nc));
+            c[m] = odd ? (e0 + ld(ec, loc(ic + 1, jc0 + m, Nc))) * T(0.5) : e0;
         }
 #pragma unroll
         for (int k = 0; k < W; ++k) {
             const int s = PAR + k;
             const T v = (s & 1) ? (c[s >> 1] + c[(s >> 1) + 1]) * T(0.5) : c[s >> 1];
-            w[k] = x[gidx(i, jbeg + k, t.N)] + v;
+            w[k] = ld(x, loc(i, jbeg + k, t.N)) + v;
         }
     }
     struct Raw { T x, e00, e10, e01, e11; };
@@ -232,11 +268,11 @@ template <typename T> struct InProlong {
         const int ic = i >> 1, jc = j >> 1;
         const int ic1 = ic + 1 < Nc ? ic + 1 : ic, jc1 = jc + 1 < Nc ? jc + 1 : jc;
         Raw r;
-        r.x = x[gidx(i, j, t.N)];
-        r.e00 = ec[gidx(ic, jc, Nc)];
-        r.e10 = ec[gidx(ic1, jc, Nc)];
-        r.e01 = ec[gidx(ic, jc1, Nc)];
-        r.e11 = ec[gidx(ic1, jc1, Nc)];
+        r.x = ld(x, loc(i, j, t.N));
+        r.e00 = ld(ec, loc(ic, jc, Nc));
+        r.e10 = ld(ec, loc(ic1, jc, Nc));
+        r.e01 = ld(ec, loc(ic, jc1, Nc));
+        r.e11 = ld(ec, loc(ic1, jc1, Nc));
         return r;
     }
     template <bool INT = false, bool LT = false>
@@ -262,18 +298,18 @@ template <typename T> struct InAxpy {
     struct Raw { T r, p; };
     template <bool INT = false, bool LT = false>
     __device__ __forceinline__ Raw fetch(const OpTab& t, int i, int j) const {
-        const size_t q = gidx(i, j, t.N);
+        const Loc q = loc(i, j, t.N);
         Raw v;
-        v.r = r[q];
-        v.p = p[q];
+        v.r = ld(r, q);
+        v.p = ld(p, q);
         return v;
     }
     template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T make(const OpTab&, int, int, const Raw& v) const { return v.r + bt * v.p; }
     template <bool INT = false, bool LT = false>
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const {
-        const size_t q = gidx(i, j, t.N);
-        return r[q] + bt * p[q];
+        const Loc q = loc(i, j, t.N);
+        return ld(r, q) + bt * ld(p, q);
     }
 };
 
@@ -286,9 +322,9 @@ template <typename T> struct OutStore {            // o = s * Au
     __device__ void at(int q, const Batch& bt) { o += q * bt.sf; }
     static constexpr bool kDot = false;
     using Aux = T;
-    __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    __device__ Aux pre(const OpTab&, Loc) const { return T(0); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T, T Au, Aux, T&) const { o[q] = s * Au; }
+    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T, T Au, Aux, T&) const { st(o, q, s * Au); }
 };
 
 template <typename T> struct OutStoreDot {         // o = Au ; acc += u Au
@@ -296,10 +332,10 @@ template <typename T> struct OutStoreDot {         // o = Au ; acc += u Au
     __device__ void at(int q, const Batch& bt) { o += q * bt.sf; }
     static constexpr bool kDot = true;
     using Aux = T;
-    __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    __device__ Aux pre(const OpTab&, Loc) const { return T(0); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux, T& acc) const {
-        o[q] = Au; acc += u * Au;
+    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T u, T Au, Aux, T& acc) const {
+        st(o, q, Au); acc += u * Au;
     }
 };
 
@@ -308,10 +344,10 @@ template <typename T> struct OutWriteDot {         // w = u (materialise the for
     __device__ void at(int q, const Batch& bt) { o += q * bt.sf; w += q * bt.sf; }
     static constexpr bool kDot = true;
     using Aux = T;
-    __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    __device__ Aux pre(const OpTab&, Loc) const { return T(0); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux, T& acc) const {
-        w[q] = u; o[q] = Au; acc += u * Au;
+    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T u, T Au, Aux, T& acc) const {
+        st(w, q, u); st(o, q, Au); acc += u * Au;
     }
 };
 
@@ -320,10 +356,10 @@ template <typename T, typename TO> struct OutResid {  // r = b - Au (optionally 
     __device__ void at(int q, const Batch& bt) { b += q * bt.sf; r += q * bt.sf; }
     static constexpr bool kDot = true;
     using Aux = T;
-    __device__ Aux pre(const OpTab&, size_t q) const { return b[q]; }
+    __device__ Aux pre(const OpTab&, Loc q) const { return ld(b, q); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T, T Au, Aux bq, T& acc) const {
-        T rv = bq - Au; r[q] = (TO)rv; acc += rv * rv;
+    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T, T Au, Aux bq, T& acc) const {
+        T rv = bq - Au; st(r, q, (TO)rv); acc += rv * rv;
     }
 };
 
@@ -336,12 +372,10 @@ template <typename T> struct OutSmooth {
     static constexpr bool kPreRhs = true;   // pre() is b at the point: an input whose raw
                                             // fetch is b (kRawRhs) supplies it without a load
     using Aux = T;
-    __device__ Aux pre(const OpTab&, size_t q) const { return b[q]; }
+    __device__ Aux pre(const OpTab&, Loc q) const { return ld(b, q); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab& t, int i, int j, size_t q, T u, T Au, Aux bq, T&) const {
-        T c = omega * (bq - Au);
-        if (use_diag) c = INT ? c * (T)t.idint : c / diag_of<T, LT>(t, i, j);
-        xo[q] = u + c;
+    __device__ __forceinline__ void emit(const OpTab& t, int i, int j, Loc q, T u, T Au, Aux bq, T&) const {
+        st(xo, q, u + smooth_corr_at<T, INT, LT>(t, i, j, bq - Au, omega, use_diag));
     }
 };
 
@@ -351,18 +385,16 @@ template <typename T> struct OutSmoothDot {
     __device__ void at(int q, const Batch& bt) { b += q * bt.sf; xo += q * bt.sf; dotv += q * bt.sf; }
     static constexpr bool kDot = true;
     struct Aux { T b, d; };
-    __device__ Aux pre(const OpTab&, size_t q) const {
+    __device__ Aux pre(const OpTab&, Loc q) const {
         Aux a;
-        a.b = b[q];
-        a.d = dotv == b ? a.b : dotv[q];   // r'z on the level-0 right-hand side r itself
+        a.b = ld(b, q);
+        a.d = dotv == b ? a.b : ld(dotv, q);   // r'z on the level-0 right-hand side r itself
         return a;
     }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab& t, int i, int j, size_t q, T u, T Au, Aux a, T& acc) const {
-        T c = omega * (a.b - Au);
-        if (use_diag) c = INT ? c * (T)t.idint : c / diag_of<T, LT>(t, i, j);
-        const T xn = u + c;
-        xo[q] = xn; acc += a.d * xn;
+    __device__ __forceinline__ void emit(const OpTab& t, int i, int j, Loc q, T u, T Au, Aux a, T& acc) const {
+        const T xn = u + smooth_corr_at<T, INT, LT>(t, i, j, a.b - Au, omega, use_diag);
+        st(xo, q, xn); acc += a.d * xn;
     }
 };
 
@@ -375,10 +407,10 @@ template <typename T> struct OutPUpd {
     static constexpr bool kDot = true;
     struct Aux { T x, p; };
     __device__ void init() { al = (T)(*alpha); }
-    __device__ Aux pre(const OpTab&, size_t q) const { Aux a; a.x = x[q]; a.p = pold[q]; return a; }
+    __device__ Aux pre(const OpTab&, Loc q) const { Aux a; a.x = ld(x, q); a.p = ld(pold, q); return a; }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T Au, Aux a, T& acc) const {
-        pnew[q] = u; Ap[q] = Au; x[q] = a.x + al * a.p; acc += u * Au;
+    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T u, T Au, Aux a, T& acc) const {
+        st(pnew, q, u); st(Ap, q, Au); st(x, q, a.x + al * a.p); acc += u * Au;
     }
 };
 
@@ -388,9 +420,9 @@ template <typename T> struct OutCopy {
     __device__ void at(int q, const Batch& bt) { o += q * bt.sf; }
     static constexpr bool kDot = false;
     using Aux = T;
-    __device__ Aux pre(const OpTab&, size_t) const { return T(0); }
+    __device__ Aux pre(const OpTab&, Loc) const { return T(0); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, size_t q, T u, T, Aux, T&) const { o[q] = u; }
+    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T u, T, Aux, T&) const { st(o, q, u); }
 };
 
 // ---------------------------------------------------------------------------
@@ -426,15 +458,29 @@ __device__ __forceinline__ T block_sum(T v) {
 // columns; uc is the operand at (i, j).
 // (The closure width is the order's compile-time C, so the unrolled loops
 // carry no dead iterations for the narrower orders.)
+// Inputs whose raw fetch is more than two values (x + P e: five) form each
+// closure operand as it is loaded instead of holding every raw fetch at once:
+// the face tiles are few, but the kernel's register allocation is the largest
+// over all its paths, so a face path holding C x 5 values would cap the
+// occupancy of every tile.
+template <class IN, typename T>
+constexpr bool big_raw() { return sizeof(typename IN::Raw) > 2 * sizeof(T); }
+
 template <int P, typename T, bool LT = false, class IN>
 __device__ __forceinline__ T k_closure(const OpTab& t, const IN& in, int i, int j, T uc) {
     constexpr int CW = Stencil<P>::C;
     const bool lo = j < t.R;
     const int r = lo ? j : t.N - 1 - j;
+    T s = (T)lut<LT>(t).krs[r] * uc;
+    if constexpr (big_raw<IN, T>()) {
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+            if (c != r) s += (T)lut<LT>(t).kb[r][c] * (in.template load<false, LT>(t, i, lo ? c : t.N - 1 - c) - uc);
+        return s;
+    }
     typename IN::Raw raw[CW];
 #pragma unroll
     for (int c = 0; c < CW; ++c) raw[c] = in.template fetch<false, LT>(t, i, lo ? c : t.N - 1 - c);
-    T s = (T)lut<LT>(t).krs[r] * uc;
 #pragma unroll
     for (int c = 0; c < CW; ++c)
         if (c != r)
@@ -447,10 +493,16 @@ __device__ __forceinline__ T m_closure(const OpTab& t, const IN& in, int i, int 
     constexpr int CW = Stencil<P>::C;
     const bool lo = i < t.R;
     const int r = lo ? i : t.N - 1 - i;
+    T s = (T)lut<LT>(t).mrs[r] * uc;
+    if constexpr (big_raw<IN, T>()) {
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+            if (c != r) s += (T)lut<LT>(t).mb[r][c] * (in.template load<false, LT>(t, lo ? c : t.N - 1 - c, j) - uc);
+        return s;
+    }
     typename IN::Raw raw[CW];
 #pragma unroll
     for (int c = 0; c < CW; ++c) raw[c] = in.template fetch<false, LT>(t, lo ? c : t.N - 1 - c, j);
-    T s = (T)lut<LT>(t).mrs[r] * uc;
 #pragma unroll
     for (int c = 0; c < CW; ++c)
         if (c != r)
@@ -540,8 +592,11 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
     constexpr bool kRhsRaw = IN::kRawRhs && pre_rhs<OUT>::value;
     typename IN::Raw rhs[kRhsRaw ? J : 1];
     if (act) {
-        if constexpr (IN::kCol && !ICHK) {
-            in.template load_col<W, (H & 1)>(t, i, j0 - H, w);   // j0 is a multiple of the even J
+        if constexpr (IN::kCol) {
+            // j0 is a multiple of the even J; the window's rows lie inside the
+            // grid (the tile is off the j closure rows), and so do the coarse
+            // rows and columns an active column reads
+            in.template load_col<W, (H & 1)>(t, i, j0 - H, w);
         } else {
             typename IN::Raw raw[W];
 #pragma unroll
@@ -577,7 +632,7 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
         if constexpr (kRhsRaw) {
             aux[r] = rhs[r];
         } else {
-            if (act) aux[r] = out.pre(t, gidx(i, j0 + r, N));
+            if (act) aux[r] = out.pre(t, loc(i, j0 + r, N));
         }
     }
     __syncthreads();
@@ -601,7 +656,7 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
             mp = m_closure<P, T, LT>(t, in, i, j0 + r, uc);
         }
         const T Au = ICHK ? hwi * kp + mp : kp + mp;   // hw_j = 1 on interior rows
-        out.template emit<INT, LT>(t, i, j0 + r, gidx(i, j0 + r, N), uc, Au, aux[r], acc);
+        out.template emit<INT, LT>(t, i, j0 + r, loc(i, j0 + r, N), uc, Au, aux[r], acc);
     }
     return acc;
 }
@@ -617,7 +672,13 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
     const int i = i0 + tid;
     const bool act = i < N;
     T w[W];
-    {
+    if constexpr (big_raw<IN, T>()) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const int jj = j0 - H + k;
+            w[k] = (act && jj >= 0 && jj < N) ? in.template load<false, LT>(t, i, jj) : T(0);
+        }
+    } else {
         typename IN::Raw raw[W];
 #pragma unroll
         for (int k = 0; k < W; ++k) {
@@ -634,14 +695,22 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
         const int side = tid / H, off = tid % H;
         const int ii = side == 0 ? i0 - H + off : i0 + TI + off;
         const int col = side == 0 ? off : TI + H + off;
-        typename IN::Raw raw[J];
+        if constexpr (big_raw<IN, T>()) {
 #pragma unroll
-        for (int r = 0; r < J; ++r)
-            if (ii >= 0 && ii < N && j0 + r < N) raw[r] = in.template fetch<false, LT>(t, ii, j0 + r);
+            for (int r = 0; r < J; ++r) {
+                const int jj = j0 + r;
+                tile[r][col] = (ii >= 0 && ii < N && jj < N) ? in.template load<false, LT>(t, ii, jj) : T(0);
+            }
+        } else {
+            typename IN::Raw raw[J];
 #pragma unroll
-        for (int r = 0; r < J; ++r) {
-            const int jj = j0 + r;
-            tile[r][col] = (ii >= 0 && ii < N && jj < N) ? in.template make<false, LT>(t, ii, jj, raw[r]) : T(0);
+            for (int r = 0; r < J; ++r)
+                if (ii >= 0 && ii < N && j0 + r < N) raw[r] = in.template fetch<false, LT>(t, ii, j0 + r);
+#pragma unroll
+            for (int r = 0; r < J; ++r) {
+                const int jj = j0 + r;
+                tile[r][col] = (ii >= 0 && ii < N && jj < N) ? in.template make<false, LT>(t, ii, jj, raw[r]) : T(0);
+            }
         }
     }
     typename OUT::Aux aux[J];
@@ -649,7 +718,7 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
     for (int r = 0; r < J; ++r) {
         tile[r][tid + H] = w[r + H];
         const int jj = j0 + r;
-        if (act && jj < N) aux[r] = out.pre(t, gidx(i, jj, N));
+        if (act && jj < N) aux[r] = out.pre(t, loc(i, jj, N));
     }
     __syncthreads();
     T acc = T(0);
@@ -679,7 +748,7 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
                 mp = m_closure<P, T, LT>(t, in, i, j, uc);
             }
             const T Au = (iint ? kp : hwi * kp) + (jint ? mp : hw_of<T, LT>(t, j) * mp);
-            out.template emit<false, LT>(t, i, j, gidx(i, j, N), uc, Au, aux[r], acc);
+            out.template emit<false, LT>(t, i, j, loc(i, j, N), uc, Au, aux[r], acc);
         }
     }
     return acc;
@@ -803,7 +872,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
         const bool act = iin || (fi >= 0 && fi < N);
         T w[W];
 #pragma unroll
-        for (int k = 0; k < W; ++k) w[k] = act ? u[gidx(fi, fj0 - H + k, N)] : T(0);
+        for (int k = 0; k < W; ++k) w[k] = act ? ld(u, loc(fi, fj0 - H + k, N)) : T(0);
 #pragma unroll
         for (int e0 = 0; e0 < 2 * H * JR; e0 += TI) {
             const int e = e0 + tid;
@@ -811,14 +880,14 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
                 const int side = e / (H * JR), rem = e % (H * JR);
                 const int off = rem / JR, r = rem % JR;
                 const int ii = side == 0 ? fi0 - H + off : fi0 + TI + off;
-                ut[r][side == 0 ? off : TI + H + off] = (iin || (ii >= 0 && ii < N)) ? u[gidx(ii, fj0 + r, N)] : T(0);
+                ut[r][side == 0 ? off : TI + H + off] = (iin || (ii >= 0 && ii < N)) ? *elem(u, loc(ii, fj0 + r, N)) : T(0);
             }
         }
         T bq[JR];
 #pragma unroll
         for (int r = 0; r < JR; ++r) {
             ut[r][tid + H] = w[r + H];
-            bq[r] = (act && b) ? b[gidx(fi, fj0 + r, N)] : T(0);
+            bq[r] = (act && b) ? ld(b, loc(fi, fj0 + r, N)) : T(0);
         }
         __syncthreads();
         const bool iint = iin || ((fi >= t.R) && (fi < N - t.R));
@@ -846,7 +915,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
             const int fj = fj0 + r;
             T v = T(0);
             if (fi >= 0 && fi < N && fj >= 0 && fj < N)
-                v = (b ? b[gidx(fi, fj, N)] : T(0)) - au_point<P, T, true>(tl, in, fi, fj);
+                v = (b ? ld(b, loc(fi, fj, N)) : T(0)) - au_point<P, T, true>(tl, in, fi, fj);
             res[r] = v;
         }
     }
@@ -863,7 +932,7 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
         const T s0 = rt[lr][lc] + T(0.5) * (rt[lr][lc - 1] + rt[lr][lc + 1]);
         const T sm = rt[lr - 1][lc] + T(0.5) * (rt[lr - 1][lc - 1] + rt[lr - 1][lc + 1]);
         const T sp = rt[lr + 1][lc] + T(0.5) * (rt[lr + 1][lc - 1] + rt[lr + 1][lc + 1]);
-        rc[gidx(I, Jc, Nc)] = scale * T(0.25) * (s0 + T(0.5) * (sm + sp));
+        *elem(rc, loc(I, Jc, Nc)) = scale * T(0.25) * (s0 + T(0.5) * (sm + sp));
     }
 }
 
