@@ -15,8 +15,8 @@
  * for p <= 6; for p = 8, 10 the reference's tau_D is too small for their
  * wider boundary closures and A is symmetric indefinite, so the apply is
  * exact but the CG-type solvers may stop with SBP_ERR_BREAKDOWN.  The
- * two-sweep fused multigrid kernels are compiled for p = 2, 4 and (64-wide
- * blocks only) p = 6, the single-workgroup bottom kernel for p = 2, 4; other
+ * two-sweep fused multigrid kernels are compiled for p = 2, 4 and (64- and
+ * 128-wide blocks) p = 6, the single-workgroup bottom kernel for p = 2, 4; other
  * orders use the separate passes.
  *
  * Conventions: every pointer argument that names a grid function is a device
@@ -156,7 +156,8 @@ int sbp_get_info(sbp_handle h, int* N, int* p, int* nlevels);
  * on `level` for element type `dtype`; shapes not compiled in are rejected.
  * With SBP_FUSED or-ed into dtype: the block shape (tile_i output columns x
  * tile_j fine rows) of the level's fused two-sweep kernels (64 x 4, 64 x 8,
- * 128 x 8, 256 x 8; for p = 6 the 64-wide ones). */
+ * 128 x 8, 256 x 8; for p = 6 all but 256 x 8, whose FP64 tiles exceed the
+ * 160 KiB of workgroup LDS). */
 int sbp_set_tiles(sbp_handle h, int level, int dtype, int tile_i, int tile_j);
 
 /* Operator --------------------------------------------------------------- */

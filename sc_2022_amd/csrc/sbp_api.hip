@@ -275,7 +275,7 @@ int smooth_pass(sbp_handle h, int l, const IN& in, const T* b, T* dst, const T* 
 // levels; above that the redundant halo work of the fused blocks costs more
 // than the passes it saves).
 inline bool fuse_level(const sbp_handle h, int l) {
-    if (h->p > 6 || h->prm.drop_coarse_rhs) return false;   // the two-sweep kernels are compiled for p = 2, 4 (and 6, 64-wide blocks)
+    if (h->p > 6 || h->prm.drop_coarse_rhs) return false;   // the two-sweep kernels are compiled for p = 2, 4, 6 (fused_shape_ok)
     if (!fused_shape_ok(h->L[l].fti[0], h->L[l].ftj[0], h->p) || !fused_shape_ok(h->L[l].fti[1], h->L[l].ftj[1], h->p))
         return false;
     const long long pts = (long long)h->L[l].n * h->np[l];

@@ -10,9 +10,28 @@ namespace sbp {
 namespace {
 constexpr int kFusedNT = 256;   // threads per block
 
-// p = 6 is compiled for the 64-wide blocks only: its wider halos (5 and 10
-// points) put the wider blocks' tiles above the 64 KiB workgroup LDS limit.
-template <int P, int TI> constexpr bool fused_compiled() { return P <= 4 || (P == 6 && TI == 64); }
+// LDS of the two kernels' tiles (sbp_fused.h: smooth2_restrict holds b and
+// x1 on region A and x2 on region B; prolong_smooth2 the operand on A, the
+// coarse patch, and b and the first sweep on B), plus the staged table.
+template <int P, int TI, int J, typename T> constexpr int sr_lds_bytes() {
+    constexpr int HA = FusedHalo<P>::HA, HB = FusedHalo<P>::HB;
+    return (int)sizeof(T) * (2 * (TI + 2 * HA) * (J + 1 + 2 * HA) + (TI + 2 * HB) * (J + 1 + 2 * HB)) +
+           (int)sizeof(OpTab);
+}
+template <int P, int TI, int J, typename T> constexpr int ps_lds_bytes() {
+    constexpr int HA = FusedHalo<P>::HA, HB = FusedHalo<P>::HB;
+    constexpr int AW = TI + 2 * HA, AH = J + 2 * HA;
+    return (int)sizeof(T) * (AW * AH + (AW / 2 + 2) * (AH / 2 + 2) + 2 * (TI + 2 * HB) * (J + 2 * HB)) +
+           (int)sizeof(OpTab);
+}
+// A shape is compiled for order P when both kernels fit the 160 KiB of LDS a
+// gfx950 workgroup can hold, in both precisions (the V-cycle of a handle runs
+// either).  Orders 2 and 4 fit every shape; p = 6 (halos of 5 and 10 points)
+// fits the 64- and 128-wide blocks, not the 256-wide FP64 ones (169 KB).
+constexpr int kLdsBytes = 160 * 1024;
+template <int P, int TI, int J> constexpr bool fused_compiled() {
+    return P <= 6 && sr_lds_bytes<P, TI, J, double>() <= kLdsBytes && ps_lds_bytes<P, TI, J, double>() <= kLdsBytes;
+}
 
 template <int P, int TI, int J, typename T>
 void sr_launch(const OpTab& tf, const T* b, T* x2, T* rc, T omega, int use_diag, T scale, FusedCtl c,
@@ -36,7 +55,7 @@ int fused_post_blocks(int N, int ti, int tj) { return ((N + ti - 1) / ti) * ((N 
 
 bool fused_shape_ok(int ti, int tj, int p) {
 #define SBP_FUSED_OK(TI_, J_) \
-    if (ti == TI_ && tj == J_) return p <= 4 || (p == 6 && fused_compiled<6, TI_>());
+    if (ti == TI_ && tj == J_) return p <= 4 || (p == 6 && fused_compiled<6, TI_, J_>());
     SBP_FUSED_SHAPES(SBP_FUSED_OK)
 #undef SBP_FUSED_OK
     return false;
@@ -58,7 +77,7 @@ int launch_smooth2_restrict(int p, int ti, int tj, const OpTab& tf, const T* b, 
         if (p == 2) sr_launch<2, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);  \
         else if (p == 4) sr_launch<4, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st); \
         else if (p == 6) {                                                                    \
-            if constexpr (fused_compiled<6, TI_>())                                           \
+            if constexpr (fused_compiled<6, TI_, J_>())                                       \
                 sr_launch<6, TI_, J_, T>(tf, b, x2, rc, omega, use_diag, scale, c, st);       \
             else return 3;                                                                    \
         } else return 3;                                                                      \
@@ -85,7 +104,7 @@ int launch_prolong_smooth2(int p, int ti, int tj, const OpTab& tf, const T* x, c
         if (p == 2) ps_launch<2, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st); \
         else if (p == 4) ps_launch<4, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st); \
         else if (p == 6) {                                                                     \
-            if constexpr (fused_compiled<6, TI_>())                                            \
+            if constexpr (fused_compiled<6, TI_, J_>())                                        \
                 ps_launch<6, TI_, J_, T>(tf, x, ec, b, out, dotv, omega, use_diag, c, st);     \
             else return 3;                                                                     \
         } else return 3;                                                                       \

@@ -74,7 +74,8 @@ def test_uncompiled_tile_shape_rejected(SP):
         h.set_tiles(0, 256, 16)
     h6 = SP(129, 6)
     with pytest.raises(Exception):
-        h6.set_tiles(0, 128, 8, fused=True)   # p = 6 fused blocks: 64 wide only
+        h6.set_tiles(0, 256, 8, fused=True)   # p = 6 FP64 256-wide fused tiles: 169 KB > 160 KB of LDS
+    h6.set_tiles(0, 128, 8, fused=True)
     h6.set_tiles(0, 64, 8, fused=True)
 
 

@@ -308,7 +308,7 @@ def test_vcycle_large_against_oracle(SP, p, bottom):
 
 @pytest.mark.parametrize("p,level,shape", [(p, lv, sh) for p in (2, 4) for lv in (5, 7, 10)
                                             for sh in ((64, 4), (64, 8), (128, 8), (256, 8))]
-                         + [(6, lv, sh) for lv in (7, 10) for sh in ((64, 4), (64, 8))])
+                         + [(6, lv, sh) for lv in (7, 10) for sh in ((64, 4), (64, 8), (128, 8))])
 def test_vcycle_fused_passes_match_separate_passes(SP, p, level, shape):
     """The temporally blocked two-sweep kernels (pre-smoothing + residual +
     restriction; prolongation + post-smoothing) against the pass-per-sweep
