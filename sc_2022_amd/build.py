@@ -62,7 +62,8 @@ def _up_to_date(obj: Path) -> bool:
 def build(jobs: int | None = None, verbose: bool = False, incremental: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
     units = [(CSRC / "sbp_api.hip", OBJ / "sbp_api.o", []),
-             (CSRC / "sbp_fused.hip", OBJ / "sbp_fused.o", [])]
+             (CSRC / "sbp_fused.hip", OBJ / "sbp_fused.o", []),
+             (CSRC / "sbp_var_fused.hip", OBJ / "sbp_var_fused.o", [])]
     hi = set(tile_shapes("SBP_HI_TILE_SHAPES"))
     for ti, tj in tile_shapes():
         units.append((CSRC / "sbp_tile.hip", OBJ / f"sbp_tile_{ti}x{tj}.o",

@@ -1631,37 +1631,64 @@ int sbp_set_tiles(sbp_handle h, int level, int dtype, int ti, int tj) {
 
 namespace {
 extern "C++" {   // templates inside the C-linkage block
-// One launch of the variable-coefficient operator (MODE as var_apply);
-// *nparts receives the number of block partials.
+// One pass of the variable-coefficient operator on level `level` with
+// operand policy IN and output policy OUT (sbp_var.h); *nparts receives the
+// number of block partials.
+template <typename T, class IN, class OUT>
+int launch_var_pass(sbp_handle h, int level, const IN& in, const OUT& out, const Scalars* sc, int* nparts) {
+    const int N = level == 0 ? h->N : h->vl[level].N;
+    const VarTab* tab = level == 0 ? h->vtab : h->vl[level].dtab;
+    const double* kb = level == 0 ? h->vband : h->vl[level].band;   // k band, then m band
+    const double* mb = kb + (size_t)(h->p + 1) * N;
+    const int nbx = (N + kVarThreads - 1) / kVarThreads, nby = (N + kVarRows - 1) / kVarRows;
+    if ((size_t)nbx * nby > h->npp) return SBP_ERR_INVALID_SIZE;
+    const dim3 g(nbx * nby);
+    if (h->p == 2)
+        hipLaunchKernelGGL((var_pass<2, T, IN, OUT>), g, dim3(kVarThreads), 0, h->st, in, out, tab, kb, mb, sc, h->partial, nbx, nby);
+    else if (h->p == 4)
+        hipLaunchKernelGGL((var_pass<4, T, IN, OUT>), g, dim3(kVarThreads), 0, h->st, in, out, tab, kb, mb, sc, h->partial, nbx, nby);
+    else
+        hipLaunchKernelGGL((var_pass<6, T, IN, OUT>), g, dim3(kVarThreads), 0, h->st, in, out, tab, kb, mb, sc, h->partial, nbx, nby);
+    HIPCHK(hipGetLastError());
+    if (nparts) *nparts = nbx * nby;
+    return SBP_OK;
+}
+
+// The plain-operand passes by mode: 0 out = sign A u; 1 out = A u with
+// partials of aux' out; 2 out = aux - A u with partials of out' out; 3 the
+// damped-Jacobi sweep out = u + w (aux - A u) / diag(A); 4 the same with
+// partials of aux' out.
 template <typename T = double>
 int launch_var(sbp_handle h, int mode, const T* u, T* out, const T* aux, double sign,
                const Scalars* sc, int* nparts, int level = 0, double omega = 0.0) {
-    const int N = level == 0 ? h->N : h->vl[level].N;
-    const VarTab* tab = level == 0 ? h->vtab : h->vl[level].dtab;
-    const dim3 g((N + kVarThreads - 1) / kVarThreads, (N + kVarRows - 1) / kVarRows);
-    if ((size_t)g.x * g.y > h->npp) return SBP_ERR_INVALID_SIZE;
-#define SBP_VAR_LAUNCH(P_, M_) \
-    hipLaunchKernelGGL((var_apply<P_, M_, T>), g, dim3(kVarThreads), 0, h->st, u, out, aux, sign, h->partial, tab, \
-                       sc, omega)
-#define SBP_VAR_MODES(P_)                       \
-    switch (mode) {                             \
-    case 0: SBP_VAR_LAUNCH(P_, 0); break;       \
-    case 1: SBP_VAR_LAUNCH(P_, 1); break;       \
-    case 2: SBP_VAR_LAUNCH(P_, 2); break;       \
-    case 3: SBP_VAR_LAUNCH(P_, 3); break;       \
-    default: SBP_VAR_LAUNCH(P_, 4); break;      \
+    const VarIn<T> in{u};
+    switch (mode) {
+    case 0: return launch_var_pass<T>(h, level, in, VarOutStore<T>{out, sign}, sc, nparts);
+    case 1: return launch_var_pass<T>(h, level, in, VarOutStoreDot<T>{out, aux}, sc, nparts);
+    case 2: return launch_var_pass<T>(h, level, in, VarOutResid<T>{aux, out}, sc, nparts);
+    case 3: return launch_var_pass<T>(h, level, in, VarOutSmooth<T>{aux, out, omega}, sc, nparts);
+    default: return launch_var_pass<T>(h, level, in, VarOutSmoothDot<T>{aux, out, omega}, sc, nparts);
     }
-    if (h->p == 2) {
-        SBP_VAR_MODES(2)
-    } else if (h->p == 4) {
-        SBP_VAR_MODES(4)
-    } else {
-        SBP_VAR_MODES(6)
-    }
-#undef SBP_VAR_MODES
-#undef SBP_VAR_LAUNCH
+}
+
+// rc = restrict_scale * FW(b - A u) from level l to l + 1, the fine residual on chip.
+template <typename T>
+int launch_var_rr(sbp_handle h, int l, const T* u, const T* b, T* rc, const Scalars* sc) {
+    const auto& V = h->vl[l];
+    const int Nc = h->vl[l + 1].N;
+    constexpr int CI = kVarThreads / 2 - 1, CJ = kVarRows / 2;
+    const int nbx = (Nc + CI - 1) / CI, nby = (Nc + CJ - 1) / CJ;
+    const dim3 g(nbx * nby);
+    const double sc_r = h->prm.restrict_scale;
+    const double* kb = V.band;
+    const double* mb = kb + (size_t)(h->p + 1) * V.N;
+    if (h->p == 2)
+        hipLaunchKernelGGL((var_resid_restrict<2, T>), g, dim3(kVarThreads), 0, h->st, u, b, rc, sc_r, V.dtab, kb, mb, Nc, nbx, nby, sc);
+    else if (h->p == 4)
+        hipLaunchKernelGGL((var_resid_restrict<4, T>), g, dim3(kVarThreads), 0, h->st, u, b, rc, sc_r, V.dtab, kb, mb, Nc, nbx, nby, sc);
+    else
+        hipLaunchKernelGGL((var_resid_restrict<6, T>), g, dim3(kVarThreads), 0, h->st, u, b, rc, sc_r, V.dtab, kb, mb, Nc, nbx, nby, sc);
     HIPCHK(hipGetLastError());
-    if (nparts) *nparts = (int)(g.x * g.y);
     return SBP_OK;
 }
 
@@ -1728,42 +1755,70 @@ int var_vcycle(sbp_handle h, int l, const T* b, T* x, const Scalars* sc, int* np
     T* tl;   // this level's sweep buffer
     if constexpr (f32) tl = l == 0 ? h->Ap32 : V.t32;
     else tl = V.t;
-    T* bufs[2] = {x, tl};
-    // sweeps after the first: (nu1 - 1) + nu2 (or nu2 - 1 after a zero start)
-    const int later = nu1 > 0 ? (nu1 - 1) + nu2 : nu2;
-    int cur = later % 2 == 0 ? 0 : 1;   // the buffer the first sweep writes
-    if (nu1 > 0) {
-        hipLaunchKernelGGL((var_jacobi0<T>), dim3((V.N + 255) / 256, V.N), dim3(256), 0, st, bufs[cur], b,
-                           (const VarTab*)V.dtab, om, sc);
-    } else {
-        hipLaunchKernelGGL((fill<T, kEwThreads>), dim3(ew_blocks(V.n), 1), dim3(kEwThreads), 0, st, bufs[cur], T(0),
-                           V.n, sc);
-    }
-    HIPCHK(hipGetLastError());
-    for (int s = 1; s < nu1; ++s) {
-        STCHK(launch_var(h, 3, bufs[cur], bufs[1 - cur], b, 1.0, sc, nullptr, l, om));
-        cur = 1 - cur;
-    }
-    // residual into the free buffer, restricted to the coarser right-hand side
     auto& C = h->vl[l + 1];
     T* cb;
     T* cx;
     if constexpr (f32) { cb = C.b32; cx = C.x32; }
     else { cb = C.b; cx = C.x; }
-    STCHK(launch_var(h, 2, bufs[cur], bufs[1 - cur], b, 1.0, sc, nullptr, l));
-    hipLaunchKernelGGL((restrict_fw<T>), dim3((C.N + 255) / 256, C.N, 1), dim3(256), 0, st,
-                       (const T*)bufs[1 - cur], cb, V.N, C.N, (T)h->prm.restrict_scale, sc);
-    HIPCHK(hipGetLastError());
+    // Small levels (at most fuse_n^2 points) with 2 + 2 sweeps: the two
+    // temporally blocked kernels, x2 = S(S(0)) with the restricted residual,
+    // then x = S(S(x2 + P e)) (and the b'x partials).
+    if (nu1 == 2 && nu2 == 2 && (long long)V.n <= (long long)h->prm.fuse_n * h->prm.fuse_n) {
+        const int ti = 64, tj = V.n >= (size_t)1025 * 1025 ? 8 : 4;
+        const size_t nb = (size_t)(h->p + 1) * V.N;
+        const VarFusedArgs a{V.N, V.dtab, V.band, V.band + nb, om, h->prm.restrict_scale, sc ? &sc->done : nullptr};
+        if (!var_fused_shape_ok(ti, tj)) return SBP_ERR_INVALID_ARG;
+        STCHK(launch_var_smooth2_restrict<T>(h->p, ti, tj, a, b, tl, cb, st));
+        STCHK(var_vcycle<T>(h, l + 1, (const T*)cb, cx, sc));
+        STCHK(launch_var_prolong_smooth2<T>(h->p, ti, tj, a, (const T*)tl, (const T*)cx, b, x, np_rz ? b : nullptr,
+                                            h->partial, st));
+        if (np_rz) *np_rz = fused_post_blocks(V.N, ti, tj);
+        return SBP_OK;
+    }
+    T* bufs[2] = {x, tl};
+    // Buffer writes: nu1 - 1 for the pre-smoothing (the first two sweeps are
+    // one pass; one write for nu1 = 0 or 1) and nu2 for the post-smoothing
+    // (the prolongation is folded into its first sweep).  The first write goes
+    // to the buffer that makes the last one land in x.
+    const int writes = (nu1 >= 2 ? nu1 - 1 : 1) + nu2;
+    int cur = (writes - 1) % 2;
+    if (nu1 >= 2) {
+        // x2 = S(S(0)): the first sweep x1 = w D^-1 b formed on the fly
+        STCHK(launch_var_pass<T>(h, l, VarInSmooth0<T>{b, om}, VarOutSmooth<T>{b, bufs[cur], om}, sc, nullptr));
+        for (int s = 2; s < nu1; ++s) {
+            STCHK(launch_var(h, 3, bufs[cur], bufs[1 - cur], b, 1.0, sc, nullptr, l, om));
+            cur = 1 - cur;
+        }
+    } else if (nu1 == 1) {
+        hipLaunchKernelGGL((var_jacobi0<T>), dim3((V.N + 255) / 256, V.N), dim3(256), 0, st, bufs[cur], b,
+                           (const VarTab*)V.dtab, om, sc);
+        HIPCHK(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL((fill<T, kEwThreads>), dim3(ew_blocks(V.n), 1), dim3(kEwThreads), 0, st, bufs[cur], T(0),
+                           V.n, sc);
+        HIPCHK(hipGetLastError());
+    }
+    // residual and full-weighting restriction in one pass (the fine residual stays on chip)
+    STCHK(launch_var_rr<T>(h, l, (const T*)bufs[cur], b, cb, sc));
     STCHK(var_vcycle<T>(h, l + 1, (const T*)cb, cx, sc));
-    hipLaunchKernelGGL((prolong_add<T>), dim3((V.N + 255) / 256, V.N, 1), dim3(256), 0, st, (const T*)cx,
-                       bufs[cur], V.N, C.N, sc);
-    HIPCHK(hipGetLastError());
-    // with np_rz the last sweep also leaves the block partials of b'x (its
-    // result lands in x by the parity choice above)
-    for (int s = 0; s < nu2; ++s) {
-        const bool last = s == nu2 - 1 && np_rz;
-        STCHK(launch_var(h, last ? 4 : 3, bufs[cur], bufs[1 - cur], b, 1.0, sc, last ? np_rz : nullptr, l, om));
+    if (nu2 == 0) {
+        hipLaunchKernelGGL((prolong_add<T>), dim3((V.N + 255) / 256, V.N, 1), dim3(256), 0, st, (const T*)cx,
+                           bufs[cur], V.N, C.N, sc);
+        HIPCHK(hipGetLastError());
+    } else {
+        // the first post-smoothing sweep on x + P e; with np_rz the last sweep
+        // also leaves the block partials of b'x
+        const VarInProlong<T> inp{bufs[cur], cx, C.N};
+        if (nu2 == 1 && np_rz)
+            STCHK(launch_var_pass<T>(h, l, inp, VarOutSmoothDot<T>{b, bufs[1 - cur], om}, sc, np_rz));
+        else
+            STCHK(launch_var_pass<T>(h, l, inp, VarOutSmooth<T>{b, bufs[1 - cur], om}, sc, nullptr));
         cur = 1 - cur;
+        for (int s = 1; s < nu2; ++s) {
+            const bool last = s == nu2 - 1 && np_rz;
+            STCHK(launch_var(h, last ? 4 : 3, bufs[cur], bufs[1 - cur], b, 1.0, sc, last ? np_rz : nullptr, l, om));
+            cur = 1 - cur;
+        }
     }
     if (bufs[cur] != x) {
         hipLaunchKernelGGL((copy_vec<T, kEwThreads>), dim3(ew_blocks(V.n), 1), dim3(kEwThreads), 0, st,

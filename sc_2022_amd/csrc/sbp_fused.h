@@ -79,11 +79,12 @@ __device__ __forceinline__ T au_grid(const OpTab& t, const T* a, int i0, int j0,
 // with R closure rows the first sweep's region B needs R-1 points of halo
 // around the output region C and the operand's region A another R-1 (both
 // >= the stencil radius H, which interior points need).
-template <int P> struct FusedHalo {
-    static constexpr int R = Stencil<P>::R;
+template <int R_> struct FusedHaloR {
+    static constexpr int R = R_;
     static constexpr int HB = R - 1;
     static constexpr int HA = 2 * (R - 1);
 };
+template <int P> struct FusedHalo : FusedHaloR<Stencil<P>::R> {};
 
 // Scale of the smoothing correction w (b - Au): w / diag (damped Jacobi; the
 // inverse diagonal is the constant idint away from the faces) or w (Richardson).
@@ -93,6 +94,38 @@ __device__ __forceinline__ T smooth_corr(const OpTab& t, int i, int j, T r, T om
     const T c = omega * r;
     return use_diag ? c / diag_of<T, LT>(t, i, j) : c;
 }
+
+// The operator as the fused bodies see it (constant coefficients): its closure
+// rows R (the halos), the table staging of face blocks (fetch issues the
+// loads, store writes LDS; kStage: the body then waits before the first
+// lookup), the smoothing correction and A u on an LDS array.
+template <int P, typename T, int NT, bool LT, bool INNER>
+struct FusedOpC {
+    static constexpr int R = Stencil<P>::R;
+    static constexpr bool kStage = LT;
+    static constexpr bool kInner = INNER;   // every point the block evaluates is off the closures
+    const OpTab& ta;   // kernel-argument table
+    OpTab* lt;         // its LDS copy (LT)
+    TabStage<P, NT> st;
+    __device__ FusedOpC(const OpTab& a, OpTab* l) : ta(a), lt(l) {}
+    __device__ __forceinline__ const OpTab& t() const {
+        if constexpr (LT) return *lt;
+        else return ta;
+    }
+    __device__ __forceinline__ void fetch(int, int) {
+        if constexpr (LT) st.fetch(ta.dev);
+    }
+    __device__ __forceinline__ void store() {
+        if constexpr (LT) st.store(lt);
+    }
+    __device__ __forceinline__ T corr(int i, int j, T r, T omega, int use_diag) const {
+        return smooth_corr<T, LT, INNER>(t(), i, j, r, omega, use_diag);
+    }
+    template <int W>
+    __device__ __forceinline__ T au(const T* a, int i0, int j0, int i, int j) const {
+        return au_grid<P, T, LT, INNER, W>(t(), a, i0, j0, i, j);
+    }
+};
 
 struct FusedCtl {
     const int* done;    // nullable: per-problem flags, stride ds ints
@@ -107,18 +140,18 @@ struct FusedCtl {
 // ---------------------------------------------------------------------------
 // The done flag, the table words (face blocks) and region A's loads are all
 // issued before the first wait; a converged problem's block returns false
-// after that wait, before any store.  ta: the kernel-argument table (sizes);
-// t: the table lookups read (ta itself, or its LDS copy ltab once staged).
-template <int P, int TI, int J, int NT, typename T, bool LT, bool INNER>
-__device__ __forceinline__ bool sr_body(const T* __restrict__ b, T* __restrict__ x2g, T* __restrict__ rc,
-                                        T omega, int use_diag, T scale, const OpTab& ta, const OpTab& t,
-                                        OpTab* ltab, const int* done, int bx, int by, int Nc, T* Bt, T* U, T* V) {
-    constexpr int HA = FusedHalo<P>::HA, HB = FusedHalo<P>::HB;
+// after that wait, before any store.  op: the operator (FusedOpC, or the
+// variable-coefficient one); T the arithmetic and LDS type, TG the element
+// type of the grid functions in global memory.
+template <int TI, int J, int NT, typename T, typename TG, class OP>
+__device__ __forceinline__ bool sr_body(OP& op, int N, const TG* __restrict__ b, TG* __restrict__ x2g,
+                                        TG* __restrict__ rc, T omega, int use_diag, T scale, const int* done, int bx,
+                                        int by, int Nc, T* Bt, T* U, T* V) {
+    constexpr int HA = FusedHaloR<OP::R>::HA, HB = FusedHaloR<OP::R>::HB;
     constexpr int JR = J + 1;                     // fine residual rows (region C)
     constexpr int CI = TI / 2 - 1, CJ = J / 2;    // coarse points owned per block
     constexpr int AW = TI + 2 * HA, AH = JR + 2 * HA;
     constexpr int BW = TI + 2 * HB, BH = JR + 2 * HB;
-    const int N = ta.N;
     const int tid = threadIdx.x;
     int Ic0 = bx * CI < Nc - CI ? bx * CI : Nc - CI;
     int Jc0 = by * CJ < Nc - CJ ? by * CJ : Nc - CJ;
@@ -130,8 +163,7 @@ __device__ __forceinline__ bool sr_body(const T* __restrict__ b, T* __restrict__
     // A: b and x1 = w D^-1 b (all loads issued before the first use)
     {
         const int dflag = done ? *done : 0;
-        TabStage<P, NT> tab;
-        if constexpr (LT) tab.fetch(ta.dev);
+        op.fetch(ai0, aj0);
         constexpr int NA = (AW * AH + NT - 1) / NT;
         T v[NA];
 #pragma unroll
@@ -139,13 +171,11 @@ __device__ __forceinline__ bool sr_body(const T* __restrict__ b, T* __restrict__
             const int k = tid + m * NT;
             const int r = k / AW, c = k - r * AW;
             const int i = ai0 + c, j = aj0 + r;
-            v[m] = (k < AW * AH && i >= 0 && i < N && j >= 0 && j < N) ? b[gidx(i, j, N)] : T(0);
+            v[m] = (k < AW * AH && i >= 0 && i < N && j >= 0 && j < N) ? (T)b[gidx(i, j, N)] : T(0);
         }
         if (dflag) return false;
-        if constexpr (LT) {
-            tab.store(ltab);
-            __syncthreads();
-        }
+        op.store();
+        if constexpr (OP::kStage) __syncthreads();
 #pragma unroll
         for (int m = 0; m < NA; ++m) {
             const int k = tid + m * NT;
@@ -153,8 +183,7 @@ __device__ __forceinline__ bool sr_body(const T* __restrict__ b, T* __restrict__
                 const int r = k / AW, c = k - r * AW;
                 const int i = ai0 + c, j = aj0 + r;
                 Bt[k] = v[m];
-                U[k] = (i >= 0 && i < N && j >= 0 && j < N) ? smooth_corr<T, LT, INNER>(t, i, j, v[m], omega, use_diag)
-                                                            : T(0);
+                U[k] = (i >= 0 && i < N && j >= 0 && j < N) ? op.corr(i, j, v[m], omega, use_diag) : T(0);
             }
         }
     }
@@ -164,10 +193,10 @@ __device__ __forceinline__ bool sr_body(const T* __restrict__ b, T* __restrict__
         const int r = k / BW, c = k - r * BW;
         const int i = bi0 + c, j = bj0 + r;
         T x2 = T(0);
-        if (INNER || (i >= 0 && i < N && j >= 0 && j < N)) {
+        if (OP::kInner || (i >= 0 && i < N && j >= 0 && j < N)) {
             const int ka = (r + HA - HB) * AW + (c + HA - HB);
-            const T Au = au_grid<P, T, LT, INNER, AW>(t, U, ai0, aj0, i, j);
-            x2 = U[ka] + smooth_corr<T, LT, INNER>(t, i, j, Bt[ka] - Au, omega, use_diag);
+            const T Au = op.template au<AW>(U, ai0, aj0, i, j);
+            x2 = U[ka] + op.corr(i, j, Bt[ka] - Au, omega, use_diag);
         }
         V[k] = x2;
     }
@@ -181,7 +210,7 @@ __device__ __forceinline__ bool sr_body(const T* __restrict__ b, T* __restrict__
         for (int k = tid; k < ow * (oj1 - oj0); k += NT) {
             const int r = k / ow, c = k - r * ow;
             const int i = oi0 + c, j = oj0 + r;
-            x2g[gidx(i, j, N)] = V[(j - bj0) * BW + (i - bi0)];
+            x2g[gidx(i, j, N)] = (TG)V[(j - bj0) * BW + (i - bi0)];
         }
     }
     // C: residual r = b - A x2 into U's storage (x1 is no longer read)
@@ -190,8 +219,8 @@ __device__ __forceinline__ bool sr_body(const T* __restrict__ b, T* __restrict__
         const int r = k / TI, c = k - r * TI;
         const int i = ci0 + c, j = cj0 + r;
         T rv = T(0);
-        if (INNER || (i >= 0 && i < N && j >= 0 && j < N))
-            rv = Bt[(r + HA) * AW + (c + HA)] - au_grid<P, T, LT, INNER, BW>(t, V, bi0, bj0, i, j);
+        if (OP::kInner || (i >= 0 && i < N && j >= 0 && j < N))
+            rv = Bt[(r + HA) * AW + (c + HA)] - op.template au<BW>(V, bi0, bj0, i, j);
         R[k] = rv;
     }
     __syncthreads();
@@ -208,7 +237,7 @@ __device__ __forceinline__ bool sr_body(const T* __restrict__ b, T* __restrict__
             const T s0 = r0[0] + T(0.5) * (r0[-1] + r0[1]);
             const T sm = r0[-TI] + T(0.5) * (r0[-TI - 1] + r0[-TI + 1]);
             const T sp = r0[TI] + T(0.5) * (r0[TI - 1] + r0[TI + 1]);
-            rc[gidx(I, Jc, Nc)] = scale * T(0.25) * (s0 + T(0.5) * (sm + sp));
+            rc[gidx(I, Jc, Nc)] = (TG)(scale * T(0.25) * (s0 + T(0.5) * (sm + sp)));
         }
     }
     return true;
@@ -243,12 +272,12 @@ __global__ __launch_bounds__(NT) void smooth2_restrict(const T* __restrict__ b, 
     const int ai0 = 2 * Ic0 - 1 - HA, aj0 = 2 * Jc0 - 1 - HA;
     const bool inner = ai0 >= t.R && ai0 + AW <= t.N - t.R && aj0 >= t.R && aj0 + AH <= t.N - t.R;
     if (inner) {
-        sr_body<P, TI, J, NT, T, false, true>(b, x2g, rc, omega, use_diag, scale, t, t, nullptr, done, bx, by, Nc,
-                                              Bt, U, V);
+        FusedOpC<P, T, NT, false, true> op(t, nullptr);
+        sr_body<TI, J, NT, T, T>(op, t.N, b, x2g, rc, omega, use_diag, scale, done, bx, by, Nc, Bt, U, V);
     } else {
         __shared__ OpTab ltab;
-        sr_body<P, TI, J, NT, T, true, false>(b, x2g, rc, omega, use_diag, scale, t, ltab, &ltab, done, bx, by, Nc,
-                                              Bt, U, V);
+        FusedOpC<P, T, NT, true, false> op(t, &ltab);
+        sr_body<TI, J, NT, T, T>(op, t.N, b, x2g, rc, omega, use_diag, scale, done, bx, by, Nc, Bt, U, V);
     }
 }
 
@@ -257,16 +286,16 @@ __global__ __launch_bounds__(NT) void smooth2_restrict(const T* __restrict__ b, 
 // ---------------------------------------------------------------------------
 // As sr_body: flag, table and data loads in one round trip; false (nothing
 // stored) for a converged problem.
-template <int P, int TI, int J, int NT, typename T, bool LT, bool INNER>
-__device__ __forceinline__ bool ps_body(const T* __restrict__ x, const T* __restrict__ ec, const T* __restrict__ b,
-                                        T* __restrict__ out, const T* __restrict__ dotv, T omega, int use_diag,
-                                        const OpTab& ta, const OpTab& t, OpTab* ltab, const int* done, int bx, int by,
-                                        T* U, T* E, T* Bt, T* V, T& acc) {
-    constexpr int HA = FusedHalo<P>::HA, HB = FusedHalo<P>::HB;
+template <int TI, int J, int NT, typename T, typename TG, class OP>
+__device__ __forceinline__ bool ps_body(OP& op, int N, const TG* __restrict__ x, const TG* __restrict__ ec,
+                                        const TG* __restrict__ b, TG* __restrict__ out, const TG* __restrict__ dotv,
+                                        T omega, int use_diag, const int* done, int bx, int by, T* U, T* E, T* Bt,
+                                        T* V, T& acc) {
+    constexpr int HA = FusedHaloR<OP::R>::HA, HB = FusedHaloR<OP::R>::HB;
     constexpr int AW = TI + 2 * HA, AH = J + 2 * HA;
     constexpr int BW = TI + 2 * HB, BH = J + 2 * HB;
     constexpr int EW = AW / 2 + 2, EH = AH / 2 + 2;
-    const int N = ta.N, Nc = (N + 1) / 2;
+    const int Nc = (N + 1) / 2;
     const int tid = threadIdx.x;
     int i0 = bx * TI < N - TI ? bx * TI : N - TI;
     int j0 = by * J < N - J ? by * J : N - J;
@@ -281,32 +310,31 @@ __device__ __forceinline__ bool ps_body(const T* __restrict__ x, const T* __rest
         constexpr int NA = (AW * AH + NT - 1) / NT;
         constexpr int NB = (BW * BH + NT - 1) / NT;
         const int dflag = done ? *done : 0;
-        TabStage<P, NT> tab;
-        if constexpr (LT) tab.fetch(ta.dev);
+        op.fetch(ai0, aj0);
         T ve[NE], va[NA], vb[NB];
 #pragma unroll
         for (int m = 0; m < NE; ++m) {
             const int k = tid + m * NT;
             const int r = k / EW, c = k - r * EW;
             const int I = ei0 + c, Jc = ej0 + r;
-            ve[m] = (k < EW * EH && I >= 0 && I < Nc && Jc >= 0 && Jc < Nc) ? ec[gidx(I, Jc, Nc)] : T(0);
+            ve[m] = (k < EW * EH && I >= 0 && I < Nc && Jc >= 0 && Jc < Nc) ? (T)ec[gidx(I, Jc, Nc)] : T(0);
         }
 #pragma unroll
         for (int m = 0; m < NA; ++m) {
             const int k = tid + m * NT;
             const int r = k / AW, c = k - r * AW;
             const int i = ai0 + c, j = aj0 + r;
-            va[m] = (k < AW * AH && i >= 0 && i < N && j >= 0 && j < N) ? x[gidx(i, j, N)] : T(0);
+            va[m] = (k < AW * AH && i >= 0 && i < N && j >= 0 && j < N) ? (T)x[gidx(i, j, N)] : T(0);
         }
 #pragma unroll
         for (int m = 0; m < NB; ++m) {
             const int k = tid + m * NT;
             const int r = k / BW, c = k - r * BW;
             const int i = bi0 + c, j = bj0 + r;
-            vb[m] = (k < BW * BH && i >= 0 && i < N && j >= 0 && j < N) ? b[gidx(i, j, N)] : T(0);
+            vb[m] = (k < BW * BH && i >= 0 && i < N && j >= 0 && j < N) ? (T)b[gidx(i, j, N)] : T(0);
         }
         if (dflag) return false;
-        if constexpr (LT) tab.store(ltab);   // read after the barriers below
+        op.store();   // read after the barriers below
 #pragma unroll
         for (int m = 0; m < NE; ++m) {
             const int k = tid + m * NT;
@@ -346,9 +374,9 @@ __device__ __forceinline__ bool ps_body(const T* __restrict__ x, const T* __rest
         const int r = k / BW, c = k - r * BW;
         const int i = bi0 + c, j = bj0 + r;
         T v1 = T(0);
-        if (INNER || (i >= 0 && i < N && j >= 0 && j < N)) {
-            const T Au = au_grid<P, T, LT, INNER, AW>(t, U, ai0, aj0, i, j);
-            v1 = U[(r + HA - HB) * AW + (c + HA - HB)] + smooth_corr<T, LT, INNER>(t, i, j, Bt[k] - Au, omega, use_diag);
+        if (OP::kInner || (i >= 0 && i < N && j >= 0 && j < N)) {
+            const T Au = op.template au<AW>(U, ai0, aj0, i, j);
+            v1 = U[(r + HA - HB) * AW + (c + HA - HB)] + op.corr(i, j, Bt[k] - Au, omega, use_diag);
         }
         V[k] = v1;
     }
@@ -363,11 +391,11 @@ __device__ __forceinline__ bool ps_body(const T* __restrict__ x, const T* __rest
             const int r = k / ow, c = k - r * ow;
             const int i = oi0 + c, j = oj0 + r;
             const int kb = (j - bj0) * BW + (i - bi0);
-            const T Av = au_grid<P, T, LT, INNER, BW>(t, V, bi0, bj0, i, j);
-            const T o = V[kb] + smooth_corr<T, LT, INNER>(t, i, j, Bt[kb] - Av, omega, use_diag);
+            const T Av = op.template au<BW>(V, bi0, bj0, i, j);
+            const T o = V[kb] + op.corr(i, j, Bt[kb] - Av, omega, use_diag);
             const size_t g = gidx(i, j, N);
-            out[g] = o;
-            if (dotv) acc += dotv[g] * o;
+            out[g] = (TG)o;
+            if (dotv) acc += (T)dotv[g] * o;
         }
     }
     return true;
@@ -406,12 +434,12 @@ __global__ __launch_bounds__(NT) void prolong_smooth2(const T* __restrict__ x, c
     T acc = T(0);
     bool ran;
     if (inner) {
-        ran = ps_body<P, TI, J, NT, T, false, true>(x, ec, b, out, dotv, omega, use_diag, t, t, nullptr, done, bx, by,
-                                                    U, E, Bt, V, acc);
+        FusedOpC<P, T, NT, false, true> op(t, nullptr);
+        ran = ps_body<TI, J, NT, T, T>(op, N, x, ec, b, out, dotv, omega, use_diag, done, bx, by, U, E, Bt, V, acc);
     } else {
         __shared__ OpTab ltab;
-        ran = ps_body<P, TI, J, NT, T, true, false>(x, ec, b, out, dotv, omega, use_diag, t, ltab, &ltab, done, bx,
-                                                    by, U, E, Bt, V, acc);
+        FusedOpC<P, T, NT, true, false> op(t, &ltab);
+        ran = ps_body<TI, J, NT, T, T>(op, N, x, ec, b, out, dotv, omega, use_diag, done, bx, by, U, E, Bt, V, acc);
     }
     if (!ran) return;   // converged: every thread of the block left before its first barrier
     if (dotv) {

@@ -71,4 +71,25 @@ int launch_prolong_smooth2(int p, int ti, int tj, const OpTab& tf, const T* x, c
                            const T* dotv, T omega, int use_diag, const int* done, int ds, double* partial,
                            const Batch& bt, hipStream_t st);
 
+// The same two kernels for the variable-coefficient operator (one problem;
+// FP64 arithmetic, grid functions of element type TG; block shapes 64 x 4 and
+// 64 x 8, var_fused_shape_ok).  tab: the level's device table; kband /
+// mband: its (2H+1) x N coefficient bands; done: nullable converged flag.
+struct VarTab;
+struct VarFusedArgs {
+    int N;
+    const VarTab* tab;
+    const double* kband;
+    const double* mband;
+    double omega, scale;
+    const int* done;
+};
+bool var_fused_shape_ok(int ti, int tj);
+template <typename TG>
+int launch_var_smooth2_restrict(int p, int ti, int tj, const VarFusedArgs& a, const TG* b, TG* x2, TG* rc,
+                                hipStream_t st);
+template <typename TG>
+int launch_var_prolong_smooth2(int p, int ti, int tj, const VarFusedArgs& a, const TG* x, const TG* ec, const TG* b,
+                               TG* out, const TG* dotv, double* partial, hipStream_t st);
+
 }  // namespace sbp

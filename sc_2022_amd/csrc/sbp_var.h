@@ -19,29 +19,9 @@
 #include <vector>
 
 #include "sbp_kernels.h"
+#include "sbp_var_tab.h"
 
 namespace sbp {
-
-// Closure rows / columns of the variable-coefficient factors:
-// p = 2: 3 / 4, p = 4: 6 / 8, p = 6: 9 / 12.
-constexpr int kVarMaxR = 9;
-constexpr int kVarMaxC = 12;
-inline int var_R(int p) { return p == 2 ? 3 : (p == 4 ? 6 : 9); }
-inline int var_C(int p) { return p == 2 ? 4 : (p == 4 ? 8 : 12); }
-
-// One axis' factor on the device: closure rows (first / last R) and the band
-// of the rows between.
-struct VarAxis {
-    double lo[kVarMaxR][kVarMaxC];   // rows r < R, columns c < C
-    double hi[kVarMaxR][kVarMaxC];   // rows N-1-r, columns N-1-c
-    const double* band;              // (2H+1) x N, offset s = -H..H at band[(s+H) N + r]
-};
-
-struct VarTab {
-    int N, R, C, H, RH;
-    double hw[kMaxR];
-    VarAxis k, m;
-};
 
 // ---------------------------------------------------------------------------
 // Host: the entries of M(b) (dimensionless, i.e. h M), restated from
@@ -250,9 +230,12 @@ inline bool build_var_axis(int p, int N, const double* b, bool dirichlet, VarAxi
 // ---------------------------------------------------------------------------
 // Device
 // ---------------------------------------------------------------------------
-// sum_c a[r][c] u(c) along one axis for row r (load(c) gives u at index c).
+// sum_c a[r][c] u(c) along one axis for row r (load(c) gives u at index c;
+// band: the axis' coefficient band, passed where the kernel has it as an
+// argument so that its loads are known to address global memory).
 template <int H, class LOAD>
-__device__ __forceinline__ double var_row(const VarAxis& a, int R, int C, int N, int r, LOAD&& load) {
+__device__ __forceinline__ double var_row(const VarAxis& a, const double* band, int R, int C, int N, int r,
+                                          LOAD&& load) {
     double s = 0.0;
     if (r < R) {
         for (int c = 0; c < C; ++c) s += a.lo[r][c] * load(c);
@@ -261,131 +244,328 @@ __device__ __forceinline__ double var_row(const VarAxis& a, int R, int C, int N,
         for (int c = 0; c < C; ++c) s += a.hi[rr][c] * load(N - 1 - c);
     } else {
 #pragma unroll
-        for (int q = -H; q <= H; ++q) s += a.band[(size_t)(q + H) * N + r] * load(r + q);
+        for (int q = -H; q <= H; ++q) s += band[(size_t)(q + H) * N + r] * load(r + q);
     }
     return s;
 }
 
-// MODE 0: out = sign A u;  1: out = A u, partial += aux' out;  2: out = aux - A u, partial += out' out;
-// 3: damped-Jacobi sweep out = u + omega (aux - A u) / diag(A) (no partial; dg the diagonal);
-// 4: the same sweep with partial += aux' out (r'z after a V-cycle's last sweep).
 // A block: kVarThreads threads along i, kVarRows consecutive rows j.
 constexpr int kVarThreads = 256;
 constexpr int kVarRows = 8;
 
 // Diagonal entry of one axis' factor at row r.
-__device__ __forceinline__ double var_diag1(const VarAxis& a, int R, int H, int N, int r) {
+__device__ __forceinline__ double var_diag1(const VarAxis& a, const double* band, int R, int H, int N, int r) {
     if (r < R) return a.lo[r][r];
     if (r >= N - R) return a.hi[N - 1 - r][N - 1 - r];
-    return a.band[(size_t)H * N + r];
+    return band[(size_t)H * N + r];
 }
 
-// T is the storage type of u, out and aux (double or float); the arithmetic
-// is FP64 either way.
-template <int MODE, typename T>
-__device__ __forceinline__ void var_emit(T* out, const T* aux, double sign, size_t q, double Au,
-                                         double& acc, double uc, double dg, double omega) {
-    if constexpr (MODE == 3) {
-        out[q] = (T)(uc + omega * ((double)aux[q] - Au) / dg);
-    } else if constexpr (MODE == 4) {
-        const double a = aux[q];
-        const double o = uc + omega * (a - Au) / dg;
-        out[q] = (T)o;
-        acc += a * o;
-    } else if constexpr (MODE == 0) {
-        out[q] = (T)(sign * Au);
-    } else if constexpr (MODE == 1) {
-        out[q] = (T)Au;
-        acc += (double)aux[q] * Au;
-    } else {
-        const double rv = (double)aux[q] - Au;
-        out[q] = (T)rv;
+__device__ __forceinline__ double var_hw(const VarTab& t, int N, int r) {
+    const int rr = r < N - 1 - r ? r : N - 1 - r;
+    return rr < t.RH ? t.hw[rr] : 1.0;
+}
+
+// diag(A) at (i, j): hw_i k_jj + hw_j m_ii.
+__device__ __forceinline__ double var_diag(const VarTab& t, const double* kb, const double* mb, int N, int i, int j) {
+    return var_hw(t, N, i) * var_diag1(t.k, kb, t.R, t.H, N, j) + var_hw(t, N, j) * var_diag1(t.m, mb, t.R, t.H, N, i);
+}
+
+// ---------------------------------------------------------------------------
+// Operand (IN) and output (OUT) policies of the variable-coefficient pass.
+// The arithmetic is FP64 whatever the element type T of the grid functions.
+// IN: the operand at (i, j) as a raw fetch (loads only) and a transform
+// make(raw, i, j, d), d = diag(A)(i, j) when kDiag (else unused); kCol: a
+// column loader for the interior tiles.  OUT: pre(q) loads what emit needs
+// at point q (issued before the block's barrier); emit(q, u, Au, dinv, aux,
+// acc) writes; kDiag: emit needs dinv = 1 / diag(A)(i, j); kDot: emit
+// accumulates the block partial.  (make likewise receives dinv: a pass
+// forms the reciprocal once per point and both sweeps of the first
+// smoothing pass multiply by it.)
+// ---------------------------------------------------------------------------
+template <typename T> struct VarIn {            // u
+    const T* u;
+    static constexpr bool kDiag = false, kCol = false;
+    using Raw = double;
+    __device__ __forceinline__ Raw fetch(int i, int j, int N) const { return (double)u[gidx(i, j, N)]; }
+    __device__ __forceinline__ double make(const Raw& r, int, int, double) const { return r; }
+};
+
+// The first damped-Jacobi sweep from zero, x1 = w b / diag(A).
+template <typename T> struct VarInSmooth0 {
+    const T* b;
+    double omega;
+    static constexpr bool kDiag = true, kCol = false;
+    using Raw = double;
+    __device__ __forceinline__ Raw fetch(int i, int j, int N) const { return (double)b[gidx(i, j, N)]; }
+    __device__ __forceinline__ double make(const Raw& r, int, int, double dinv) const { return omega * r * dinv; }
+};
+
+// x + P e_c, P the bilinear prolongation from the coarse side Nc = (N+1)/2.
+template <typename T> struct VarInProlong {
+    const T* x;
+    const T* ec;
+    int Nc;
+    static constexpr bool kDiag = false, kCol = true;
+    struct Raw { double x, e00, e10, e01, e11; };
+    __device__ __forceinline__ Raw fetch(int i, int j, int N) const {
+        const int ic = i >> 1, jc = j >> 1;
+        const int ic1 = ic + 1 < Nc ? ic + 1 : ic, jc1 = jc + 1 < Nc ? jc + 1 : jc;
+        Raw r;
+        r.x = (double)x[gidx(i, j, N)];
+        r.e00 = (double)ec[gidx(ic, jc, Nc)];
+        r.e10 = (double)ec[gidx(ic1, jc, Nc)];
+        r.e01 = (double)ec[gidx(ic, jc1, Nc)];
+        r.e11 = (double)ec[gidx(ic1, jc1, Nc)];
+        return r;
+    }
+    __device__ __forceinline__ double make(const Raw& r, int i, int j, double) const {
+        double v;
+        if (!(i & 1)) v = (!(j & 1)) ? r.e00 : (r.e00 + r.e01) * 0.5;
+        else v = (!(j & 1)) ? (r.e00 + r.e10) * 0.5 : (r.e00 + r.e10 + r.e01 + r.e11) * 0.25;
+        return r.x + v;
+    }
+    // w[k] = (x + P e_c)(i, jbeg + k), k < W, jbeg of parity PAR: each coarse
+    // row loaded once.  Rows jbeg .. jbeg+W-1 lie inside the grid and i < N
+    // (an odd i has i/2 + 1 < Nc).
+    template <int W, int PAR>
+    __device__ __forceinline__ void load_col(int i, int jbeg, int N, double* w) const {
+        constexpr int NC = ((PAR + W - 1) >> 1) + 2;
+        const int ic = i >> 1, jc0 = jbeg >> 1;
+        const bool odd = i & 1;
+        double c[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            const T* e = ec + gidx(ic, jc0 + m, Nc);
+            c[m] = odd ? ((double)e[0] + (double)e[1]) * 0.5 : (double)e[0];
+        }
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const int s = PAR + k;
+            const double v = (s & 1) ? (c[s >> 1] + c[(s >> 1) + 1]) * 0.5 : c[s >> 1];
+            w[k] = (double)x[gidx(i, jbeg + k, N)] + v;
+        }
+    }
+};
+
+// out = s A u
+template <typename T> struct VarOutStore {
+    T* o;
+    double s;
+    static constexpr bool kDiag = false, kDot = false;
+    __device__ __forceinline__ double pre(size_t) const { return 0.0; }
+    __device__ __forceinline__ void emit(size_t q, double, double Au, double, double, double&) const {
+        o[q] = (T)(s * Au);
+    }
+};
+
+// out = A u, partial += a' out
+template <typename T> struct VarOutStoreDot {
+    T* o;
+    const T* a;
+    static constexpr bool kDiag = false, kDot = true;
+    __device__ __forceinline__ double pre(size_t q) const { return (double)a[q]; }
+    __device__ __forceinline__ void emit(size_t q, double, double Au, double, double av, double& acc) const {
+        o[q] = (T)Au;
+        acc += av * Au;
+    }
+};
+
+// r = b - A u, partial += r'r
+template <typename T> struct VarOutResid {
+    const T* b;
+    T* r;
+    static constexpr bool kDiag = false, kDot = true;
+    __device__ __forceinline__ double pre(size_t q) const { return (double)b[q]; }
+    __device__ __forceinline__ void emit(size_t q, double, double Au, double, double bq, double& acc) const {
+        const double rv = bq - Au;
+        r[q] = (T)rv;
         acc += rv * rv;
     }
+};
+
+// Damped-Jacobi sweep xo = u + w (b - A u) / diag(A).
+template <typename T> struct VarOutSmooth {
+    const T* b;
+    T* xo;
+    double omega;
+    static constexpr bool kDiag = true, kDot = false;
+    __device__ __forceinline__ double pre(size_t q) const { return (double)b[q]; }
+    __device__ __forceinline__ void emit(size_t q, double u, double Au, double dinv, double bq, double&) const {
+        xo[q] = (T)(u + omega * (bq - Au) * dinv);
+    }
+};
+
+// The same sweep with partial += b' xo (r'z after a V-cycle's last sweep).
+template <typename T> struct VarOutSmoothDot {
+    const T* b;
+    T* xo;
+    double omega;
+    static constexpr bool kDiag = true, kDot = true;
+    __device__ __forceinline__ double pre(size_t q) const { return (double)b[q]; }
+    __device__ __forceinline__ void emit(size_t q, double u, double Au, double dinv, double bq, double& acc) const {
+        const double o = u + omega * (bq - Au) * dinv;
+        xo[q] = (T)o;
+        acc += bq * o;
+    }
+};
+
+// The operand at any point (face rows and closure columns): d from the table.
+template <class IN>
+__device__ __forceinline__ double var_operand(const IN& in, const VarTab& t, const double* kb, const double* mb,
+                                              int N, int i, int j) {
+    return in.make(in.fetch(i, j, N), i, j, IN::kDiag ? 1.0 / var_diag(t, kb, mb, N, i, j) : 0.0);
 }
 
 // Tiled evaluation for blocks whose rows (with halo) avoid the closure rows:
-// each thread keeps its column window u(i, j0-H .. j0+kVarRows-1+H) in
-// registers for the j-direction, an LDS tile serves the i-direction, the
-// i-band coefficients are loaded once per thread and the j-band ones once
-// per row (uniform); H weights along j are 1 there.  With ICHK (the first
-// and last column of blocks) each thread tests its column: out-of-grid
-// threads idle, threads on the closure columns take the closure row of m
-// (direct loads, table t staged in LDS) and their H weight.
-template <int P, int MODE, bool ICHK, typename T>
-__device__ __forceinline__ double var_tile(const T* __restrict__ u, T* __restrict__ out,
-                                           const T* __restrict__ aux, double sign, double omega,
-                                           const VarTab* __restrict__ tp,
-                                           const VarTab& t, int i0, int j0,
-                                           double (*tile)[kVarThreads + 2 * (P / 2)]) {
+// each thread keeps its column window of the operand, rows j0-H ..
+// j0+kVarRows-1+H, in registers for the j-direction; an LDS tile of the
+// operand on the block's rows serves the i-direction (centre column of
+// thread tid at tid + HL); the i-band coefficients are loaded once per thread
+// and the j-band ones of the block's rows go to LDS (kr) with the operand
+// (read after the output stores have begun they could not come through the
+// scalar cache: the compiler cannot rule out that the stores reach them);
+// H weights along j are 1 there.  With ICHK (the first and last columns of
+// blocks) each thread tests its column: out-of-grid threads idle, and
+// threads on the closure columns take the closure row of m (table t staged
+// in LDS) and their H weight.  The tile's left halo is then HL = C columns
+// wide, so every closure column a thread of the block reads is in the tile:
+// at the left face they are the first block's own columns, at the right face
+// the last C columns, which a last block narrower than C reaches through
+// that halo.
+template <int P> constexpr int var_tile_hl() { return VarShape<P>::C; }
+template <int P> constexpr int var_tile_w() { return kVarThreads + var_tile_hl<P>() + P / 2; }
+
+template <int P, bool ICHK, typename T, class IN, class OUT>
+__device__ __forceinline__ double var_tile(const IN& in, const OUT& out, const VarTab* __restrict__ tp,
+                                           const double* __restrict__ kb, const double* __restrict__ mb,
+                                           const VarTab& t, int i0, int j0, double (*tile)[var_tile_w<P>()],
+                                           double (*kr)[kVarRows]) {
     constexpr int H = P / 2;
     constexpr int J = kVarRows;
     constexpr int W = J + 2 * H;
+    constexpr int HL = var_tile_hl<P>();
+    constexpr int C = VarShape<P>::C;
+    constexpr int NL = ICHK ? HL : H;          // left halo columns loaded
+    constexpr bool kD = IN::kDiag || OUT::kDiag;
     const int N = tp->N, R = tp->R;
     const int tid = threadIdx.x;
     const int i = i0 + tid;
     const bool act = !ICHK || i < N;
     const bool iint = !ICHK || (i >= R && i < N - R);
-    const double* kb = tp->k.band;
-    const double* mb = tp->m.band;
     double w[W];
+    // loads first: the operand window, the output's pre-loads, the i-band
+    // coefficients of this column
+    typename IN::Raw raw[IN::kCol ? 1 : W];
+    if (act) {
+        if constexpr (IN::kCol) {
+            in.template load_col<W, (H & 1)>(i, j0 - H, N, w);   // j0 is a multiple of the even J
+        } else {
+#pragma unroll
+            for (int k = 0; k < W; ++k) raw[k] = in.fetch(i, j0 - H + k, N);
+        }
+    }
+    double aux[J];
+#pragma unroll
+    for (int r = 0; r < J; ++r) aux[r] = act ? out.pre(gidx(i, j0 + r, N)) : 0.0;
     double cm[2 * H + 1];
 #pragma unroll
-    for (int k = 0; k < W; ++k) w[k] = act ? (double)u[gidx(i, j0 - H + k, N)] : 0.0;
-#pragma unroll
     for (int q = 0; q <= 2 * H; ++q) cm[q] = iint ? mb[(size_t)q * N + i] : 0.0;
-    if (tid < 2 * H * J) {   // halo columns: H left, H right, J rows
-        const int side = tid / (H * J), rem = tid % (H * J), off = rem / J, r = rem % J;
-        const int ii = side == 0 ? i0 - H + off : i0 + kVarThreads + off;
-        tile[r][side == 0 ? off : kVarThreads + H + off] = (!ICHK || (ii >= 0 && ii < N)) ? (double)u[gidx(ii, j0 + r, N)] : 0.0;
+    if (tid < (2 * H + 1) * J) {
+        const int q = tid / J, r = tid - q * J;
+        kr[q][r] = kb[(size_t)q * N + j0 + r];
+    }
+    // halo columns: NL left, H right, J rows
+    for (int e = tid; e < (NL + H) * J; e += kVarThreads) {
+        const int r = e / (NL + H), c = e - r * (NL + H);
+        const int ii = c < NL ? i0 - NL + c : i0 + kVarThreads + (c - NL);
+        double v = 0.0;
+        if (!ICHK || (ii >= 0 && ii < N)) {
+            if constexpr (ICHK) v = var_operand(in, t, kb, mb, N, ii, j0 + r);
+            else v = in.make(in.fetch(ii, j0 + r, N), ii, j0 + r,
+                             IN::kDiag ? 1.0 / (kb[(size_t)H * N + j0 + r] + mb[(size_t)H * N + ii]) : 0.0);
+        }
+        tile[r][c < NL ? HL - NL + c : HL + kVarThreads + (c - NL)] = v;
+    }
+    double hwi = 1.0, mdi = 0.0;
+    if (ICHK && act) {
+        hwi = var_hw(t, N, i);
+        if (kD && !iint) mdi = var_diag1(t.m, mb, R, H, N, i);
+    }
+    // 1 / diag(A) on the window's rows for an operand that needs it (the
+    // first sweep from zero; the output sweep reuses the centre rows); an
+    // output alone forms it per row after the barrier.  hw_j = 1 on these rows.
+    constexpr int WD = IN::kDiag ? W : 1;
+    double dinv[WD];
+    if constexpr (IN::kDiag) {
+#pragma unroll
+        for (int k = 0; k < W; ++k)
+            dinv[k] = act ? 1.0 / (hwi * kb[(size_t)H * N + j0 - H + k] + (iint ? cm[H] : mdi)) : 0.0;
+    }
+    if constexpr (!IN::kCol) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) w[k] = act ? in.make(raw[k], i, j0 - H + k, IN::kDiag ? dinv[k] : 0.0) : 0.0;
+    } else if (!act) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) w[k] = 0.0;
     }
 #pragma unroll
-    for (int r = 0; r < J; ++r) tile[r][tid + H] = w[r + H];
+    for (int r = 0; r < J; ++r) tile[r][tid + HL] = w[r + H];
     __syncthreads();
     double acc = 0.0;
     if (!act) return acc;
-    double hwi = 1.0;
-    if (ICHK) {
-        const int ri = i < N - 1 - i ? i : N - 1 - i;
-        hwi = ri < t.RH ? t.hw[ri] : 1.0;
-    }
-    const double mdi = iint ? cm[H] : var_diag1(t.m, t.R, H, N, i);   // m_ii (sweep mode only)
 #pragma unroll
     for (int r = 0; r < J; ++r) {
         const int j = j0 + r;
         double kp = 0.0, mp = 0.0;
 #pragma unroll
-        for (int q = 0; q <= 2 * H; ++q) kp += kb[(size_t)q * N + j] * w[r + q];
+        for (int q = 0; q <= 2 * H; ++q) kp += kr[q][r] * w[r + q];
         if (iint) {
 #pragma unroll
-            for (int q = 0; q <= 2 * H; ++q) mp += cm[q] * tile[r][tid + q];
+            for (int q = 0; q <= 2 * H; ++q) mp += cm[q] * tile[r][tid + HL - H + q];
         } else {
-            mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
+            // closure row of m from the tile (the order of var_row); a column
+            // outside it (the block before a last block wider than H, only
+            // for N not of the form 2^k + 1) from memory
+            const bool lo = i < R;
+            const int rr = lo ? i : N - 1 - i;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int cc = lo ? c : N - 1 - c;
+                const double v = cc < i0 + kVarThreads + H ? tile[r][cc - i0 + HL] : var_operand(in, t, kb, mb, N, cc, j);
+                mp += (lo ? t.m.lo[rr][c] : t.m.hi[rr][c]) * v;
+            }
         }
-        var_emit<MODE, T>(out, aux, sign, gidx(i, j, N), hwi * kp + mp, acc, w[r + H],
-                       hwi * kb[(size_t)H * N + j] + mdi, omega);
+        double di = 0.0;
+        if constexpr (OUT::kDiag) di = IN::kDiag ? dinv[r + H] : 1.0 / (hwi * kr[H][r] + (iint ? cm[H] : mdi));
+        out.emit(gidx(i, j, N), w[r + H], hwi * kp + mp, di, aux[r], acc);
     }
     return acc;
 }
 
-template <int P, int MODE, typename T = double>
-__global__ __launch_bounds__(kVarThreads) void var_apply(const T* __restrict__ u, T* __restrict__ out,
-                                                        const T* __restrict__ aux, double sign, double* partial,
-                                                        const VarTab* __restrict__ tp, const Scalars* sc,
-                                                        double omega) {
+// One pass of the variable-coefficient operator over the N x N grid with
+// operand IN and output OUT (one problem; a launch on a converged solve,
+// sc->done, returns at once).  kb / mb: the table's coefficient bands (as
+// kernel arguments their loads are known to be global-memory loads).  Blocks: kVarThreads x kVarRows tiles, nbx
+// along i, the block index remapped so each XCD walks a contiguous band.
+template <int P, typename T, class IN, class OUT>
+__global__ __launch_bounds__(kVarThreads) void var_pass(IN in, OUT out, const VarTab* __restrict__ tp,
+                                                       const double* __restrict__ kb, const double* __restrict__ mb,
+                                                       const Scalars* sc, double* partial, int nbx, int nby) {
     constexpr int H = P / 2;
     constexpr int J = kVarRows;
-    if (sc && sc->done) return;   // the whole launch: a converged solve
+    if (sc && sc->done) return;
     __shared__ VarTab t;
-    __shared__ double tile[J][kVarThreads + 2 * H];
+    __shared__ double tile[J][var_tile_w<P>()];
+    __shared__ double kr[2 * H + 1][J];
     const int N = tp->N, R = tp->R;
     const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * kVarThreads, j0 = blockIdx.y * J;
+    const int blk = xcd_remap((int)blockIdx.x, nbx * nby);
+    const int i0 = (blk % nbx) * kVarThreads, j0 = (blk / nbx) * J;
     const bool jin = j0 - H >= R && j0 + J + H <= N - R;
     const bool iin = i0 - H >= R && i0 + kVarThreads + H <= N - R;
     double acc = 0.0;
     if (jin && iin) {
-        acc = var_tile<P, MODE, false, T>(u, out, aux, sign, omega, tp, t, i0, j0, tile);
+        acc = var_tile<P, false, T>(in, out, tp, kb, mb, t, i0, j0, tile, kr);
     } else {
         {
             const unsigned long long* src = reinterpret_cast<const unsigned long long*>(tp);
@@ -394,30 +574,140 @@ __global__ __launch_bounds__(kVarThreads) void var_apply(const T* __restrict__ u
         }
         __syncthreads();
         if (jin) {
-            acc = var_tile<P, MODE, true, T>(u, out, aux, sign, omega, tp, t, i0, j0, tile);
+            acc = var_tile<P, true, T>(in, out, tp, kb, mb, t, i0, j0, tile, kr);
         } else {
             const int i = i0 + tid;   // face rows: per point
             if (i < N) {
-                const int ri = i < N - 1 - i ? i : N - 1 - i;
-                const double hwi = ri < t.RH ? t.hw[ri] : 1.0;
+                const double hwi = var_hw(t, N, i);
                 for (int r = 0; r < J; ++r) {
                     const int j = j0 + r;
                     if (j >= N) break;
-                    const int rj = j < N - 1 - j ? j : N - 1 - j;
-                    const double hwj = rj < t.RH ? t.hw[rj] : 1.0;
-                    const double kp = var_row<H>(t.k, t.R, t.C, N, j, [&](int c) { return u[gidx(i, c, N)]; });
-                    const double mp = var_row<H>(t.m, t.R, t.C, N, i, [&](int c) { return u[gidx(c, j, N)]; });
-                    const double dg = hwi * var_diag1(t.k, t.R, H, N, j) + hwj * var_diag1(t.m, t.R, H, N, i);
-                    var_emit<MODE, T>(out, aux, sign, gidx(i, j, N), hwi * kp + hwj * mp, acc,
-                                      (double)u[gidx(i, j, N)], dg, omega);
+                    const double hwj = var_hw(t, N, j);
+                    const double kp = var_row<H>(t.k, kb, t.R, t.C, N, j, [&](int c) { return var_operand(in, t, kb, mb, N, i, c); });
+                    const double mp = var_row<H>(t.m, mb, t.R, t.C, N, i, [&](int c) { return var_operand(in, t, kb, mb, N, c, j); });
+                    const double dinv = OUT::kDiag ? 1.0 / var_diag(t, kb, mb, N, i, j) : 0.0;
+                    const size_t q = gidx(i, j, N);
+                    out.emit(q, var_operand(in, t, kb, mb, N, i, j), hwi * kp + hwj * mp, dinv, out.pre(q), acc);
                 }
             }
         }
     }
-    if constexpr (MODE == 1 || MODE == 2 || MODE == 4) {
+    if constexpr (OUT::kDot) {
         __syncthreads();
         const double s = block_sum<double, kVarThreads>(acc);
-        if (threadIdx.x == 0) partial[blockIdx.x + (size_t)blockIdx.y * gridDim.x] = s;
+        if (threadIdx.x == 0) partial[blk] = s;
+    }
+}
+
+// rc = scale * FW(b - A u) on the coarse grid of side Nc = (N+1)/2, the fine
+// residual kept on chip: a block owns kVarThreads/2 - 1 coarse columns and
+// kVarRows/2 coarse rows and forms the residual on the kVarThreads x
+// (kVarRows + 1) fine points they weigh (the geometry of resid_restrict);
+// interior rows through the register window and LDS tile, face rows per
+// point.  Full weighting as resid_restrict.
+template <int P, typename T>
+__global__ __launch_bounds__(kVarThreads) void var_resid_restrict(const T* __restrict__ u, const T* __restrict__ b,
+                                                                 T* __restrict__ rc, double scale,
+                                                                 const VarTab* __restrict__ tp,
+                                                                 const double* __restrict__ kb,
+                                                                 const double* __restrict__ mb, int Nc, int nbx,
+                                                                 int nby, const Scalars* sc) {
+    constexpr int H = P / 2;
+    constexpr int TI = kVarThreads;
+    constexpr int JR = kVarRows + 1;      // fine residual rows per block
+    constexpr int W = JR + 2 * H;         // column window
+    constexpr int CI = TI / 2 - 1;        // coarse columns per block
+    constexpr int CJ = kVarRows / 2;      // coarse rows per block
+    if (sc && sc->done) return;
+    __shared__ VarTab t;
+    __shared__ double ut[JR][TI + 2 * H];
+    __shared__ double kr[2 * H + 1][JR];  // the j-band coefficients of the block's rows
+    double (*rt)[TI + 2 * H] = ut;        // the residual reuses the operand tile after a barrier
+    const int N = tp->N, R = tp->R;
+    const int tid = threadIdx.x;
+    const int blk = xcd_remap((int)blockIdx.x, nbx * nby);
+    const int Ic0 = (blk % nbx) * CI, Jc0 = (blk / nbx) * CJ;
+    const int fi0 = 2 * Ic0 - 1, fj0 = 2 * Jc0 - 1;   // fine origin (may be -1)
+    const int fi = fi0 + tid;
+    const bool jin = fj0 - H >= R && fj0 + JR + H <= N - R;
+    const bool iin = fi0 - H >= R && fi0 + TI + H <= N - R;
+    if (!(jin && iin)) {
+        const unsigned long long* src = reinterpret_cast<const unsigned long long*>(tp);
+        unsigned long long* dst = reinterpret_cast<unsigned long long*>(&t);
+        for (int k = tid; k < (int)(sizeof(VarTab) / 8); k += TI) dst[k] = src[k];
+        __syncthreads();
+    }
+    double res[JR];
+    if (jin) {
+        const bool act = iin || (fi >= 0 && fi < N);
+        const bool iint = iin || (fi >= R && fi < N - R);
+        double w[W], cm[2 * H + 1], bq[JR];
+#pragma unroll
+        for (int k = 0; k < W; ++k) w[k] = act ? (double)u[gidx(fi, fj0 - H + k, N)] : 0.0;
+#pragma unroll
+        for (int q = 0; q <= 2 * H; ++q) cm[q] = (act && iint) ? mb[(size_t)q * N + fi] : 0.0;
+#pragma unroll
+        for (int e0 = 0; e0 < 2 * H * JR; e0 += TI) {
+            const int e = e0 + tid;
+            if (e < 2 * H * JR) {
+                const int side = e / (H * JR), rem = e % (H * JR);
+                const int off = rem / JR, r = rem % JR;
+                const int ii = side == 0 ? fi0 - H + off : fi0 + TI + off;
+                ut[r][side == 0 ? off : TI + H + off] =
+                    (iin || (ii >= 0 && ii < N)) ? (double)u[gidx(ii, fj0 + r, N)] : 0.0;
+            }
+        }
+        if (tid < (2 * H + 1) * JR) {
+            const int q = tid / JR, r = tid - q * JR;
+            kr[q][r] = kb[(size_t)q * N + fj0 + r];
+        }
+#pragma unroll
+        for (int r = 0; r < JR; ++r) {
+            ut[r][tid + H] = w[r + H];
+            bq[r] = act ? (double)b[gidx(fi, fj0 + r, N)] : 0.0;
+        }
+        __syncthreads();
+        const double hwi = iin ? 1.0 : (act ? var_hw(t, N, fi) : 0.0);
+#pragma unroll
+        for (int r = 0; r < JR; ++r) {
+            const int j = fj0 + r;
+            double kp = 0.0, mp = 0.0;
+#pragma unroll
+            for (int q = 0; q <= 2 * H; ++q) kp += kr[q][r] * w[r + q];
+            if (iint) {
+#pragma unroll
+                for (int q = 0; q <= 2 * H; ++q) mp += cm[q] * ut[r][tid + q];
+            } else if (act) {
+                mp = var_row<H>(t.m, mb, R, t.C, N, fi, [&](int c) { return (double)u[gidx(c, j, N)]; });
+            }
+            res[r] = act ? bq[r] - (hwi * kp + mp) : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < JR; ++r) {
+            const int fj = fj0 + r;
+            double v = 0.0;
+            if (fi >= 0 && fi < N && fj >= 0 && fj < N) {
+                const double kp = var_row<H>(t.k, kb, R, t.C, N, fj, [&](int c) { return (double)u[gidx(fi, c, N)]; });
+                const double mp = var_row<H>(t.m, mb, R, t.C, N, fi, [&](int c) { return (double)u[gidx(c, fj, N)]; });
+                v = (double)b[gidx(fi, fj, N)] - (var_hw(t, N, fi) * kp + var_hw(t, N, fj) * mp);
+            }
+            res[r] = v;
+        }
+    }
+    __syncthreads();                      // every thread is done reading the operand tile
+#pragma unroll
+    for (int r = 0; r < JR; ++r) rt[r][tid] = res[r];
+    __syncthreads();
+    for (int o = tid; o < CI * CJ; o += TI) {
+        const int cI = o % CI, cJ = o / CI;
+        const int I = Ic0 + cI, Jc = Jc0 + cJ;
+        if (I >= Nc || Jc >= Nc) continue;
+        const int lc = 2 * cI + 1, lr = 2 * cJ + 1;
+        const double s0 = rt[lr][lc] + 0.5 * (rt[lr][lc - 1] + rt[lr][lc + 1]);
+        const double sm = rt[lr - 1][lc] + 0.5 * (rt[lr - 1][lc - 1] + rt[lr - 1][lc + 1]);
+        const double sp = rt[lr + 1][lc] + 0.5 * (rt[lr + 1][lc - 1] + rt[lr + 1][lc + 1]);
+        rc[gidx(I, Jc, Nc)] = (T)(scale * 0.25 * (s0 + 0.5 * (sm + sp)));
     }
 }
 
@@ -433,7 +723,7 @@ __global__ __launch_bounds__(256) void var_jacobi0(T* __restrict__ x, const T* _
     if (i >= N) return;
     const int ri = i < N - 1 - i ? i : N - 1 - i, rj = j < N - 1 - j ? j : N - 1 - j;
     const double hwi = ri < t.RH ? t.hw[ri] : 1.0, hwj = rj < t.RH ? t.hw[rj] : 1.0;
-    const double d = hwi * var_diag1(t.k, t.R, t.H, N, j) + hwj * var_diag1(t.m, t.R, t.H, N, i);
+    const double d = hwi * var_diag1(t.k, t.k.band, t.R, t.H, N, j) + hwj * var_diag1(t.m, t.m.band, t.R, t.H, N, i);
     const size_t q = gidx(i, j, N);
     x[q] = (T)(omega * (double)b[q] / d);
 }

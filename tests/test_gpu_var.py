@@ -133,6 +133,33 @@ def test_var_vcycle_matches_oracle(SP, level, p):
 
 
 @pytest.mark.parametrize("p", [2, 4, 6])
+@pytest.mark.parametrize("level", [7, 10])
+def test_var_vcycle_fused_matches_passes(SP, level, p):
+    """The two-sweep kernels (levels of at most fuse_n^2 points: at 1025^2
+    every level, the finest with 64 x 8 blocks, the others 64 x 4; interior,
+    face and shifted last blocks) against the pass-per-sweep V-cycle
+    (fuse_n = 0), in FP64 and inside the FP32-V-cycle MG-PCG."""
+    N = 2 ** level + 1
+    bx, by = profiles(level, 5 * level + p)
+    hf = SP(N, p, coef=(bx, by))
+    hs = SP(N, p, coef=(bx, by), fuse_n=0)
+    r = np.random.default_rng(7 * level + p).standard_normal(N * N)
+    zf = hf.vcycle(T(r)).cpu().numpy()
+    zs = hs.vcycle(T(r)).cpu().numpy()
+    assert relmax(zf, zs) < 1e-13
+    b = T(r)
+    tol = 1e-10 * float(torch.linalg.vector_norm(b))
+    xf, rf = hf.mgpcg(b, abstol=tol)
+    xs, rs = hs.mgpcg(b, abstol=tol)
+    assert rf.converged and rs.converged and rf.iters == rs.iters
+    assert relmax(xf.cpu().numpy(), xs.cpu().numpy()) < 1e-10
+    xf32, rf32 = hf.mgpcg_mixed(b, abstol=tol)
+    xs32, rs32 = hs.mgpcg_mixed(b, abstol=tol)
+    assert rf32.converged and rs32.converged and abs(rf32.iters - rs32.iters) <= 1
+    assert relmax(xf32.cpu().numpy(), xs32.cpu().numpy()) < 1e-8
+
+
+@pytest.mark.parametrize("p", [2, 4, 6])
 def test_var_mgpcg_against_oracle_and_direct(SP, p):
     level = 7
     N = 2 ** level + 1
