@@ -586,6 +586,37 @@ int vcycle_ragged(sbp_handle h, Scalars* sc) {
                            rz0 ? rpart(h, 0) : nullptr);
 }
 
+// The variable-coefficient operator's pass and V-cycle (defined with the
+// rest of that operator below); Krylov loops of such a handle call them in
+// place of the constant-coefficient ones.
+template <typename T, class IN, class OUT>
+int launch_var_pass(sbp_handle h, int level, const IN& in, const OUT& out, const Scalars* sc, int* nparts);
+template <typename T>
+int var_vcycle(sbp_handle h, int l, const T* b, T* x, const Scalars* sc, int* np_rz);
+
+// z = V(r) with the r'z partials in h->partial (*nparts of them per problem).
+template <typename T>
+int krylov_vcycle(sbp_handle h, const KVecs<T>& v, Scalars* sc, int* nparts) {
+    if constexpr (std::is_same<T, double>::value) {
+        if (h->var) {
+            int nrz = 0;
+            STCHK(var_vcycle<double>(h, 0, v.r, v.z, sc, &nrz));
+            if (nrz == 0) {   // no post-smoothing: r'z by its own kernel
+                const size_t n = h->L[0].n;
+                hipLaunchKernelGGL((dot_partial<double, kEwThreads>), dim3(ew_blocks(n), 1), dim3(kEwThreads), 0, h->st,
+                                   (const double*)v.r, (const double*)v.z, n, h->partial);
+                HIPCHK(hipGetLastError());
+                nrz = (int)ew_blocks(n);
+            }
+            *nparts = nrz;
+            return SBP_OK;
+        }
+    }
+    STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, sc, h->partial));
+    *nparts = vcycle_nparts<T>(h);
+    return SBP_OK;
+}
+
 // One Krylov iteration of a ragged handle: per level with joiners the fused
 // search-direction pass, alpha, the residual update and the convergence test
 // (one launch each over that level's joiners), then one V-cycle over every
@@ -641,8 +672,14 @@ int issue_iteration(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, i
     T* pnew = v.p[par];
     const T* zz = use_mg ? v.z : v.r;
     int np = 0;
-    STCHK(launch_pass<T>(h, 0, InAxpy<T>{zz, pold, &sc->beta, T(0)},
-                         OutPUpd<T>{pnew, v.Ap, v.x, pold, &sc->alpha, T(0)}, sc, h->partial, &np));
+    if constexpr (std::is_same<T, double>::value) {
+        if (h->var)
+            STCHK(launch_var_pass<double>(h, 0, VarInAxpy{zz, pold, &sc->beta, 0.0},
+                                          VarOutPUpd{pnew, v.Ap, v.x, pold, &sc->alpha, 0.0}, sc, &np));
+    }
+    if (!h->var)
+        STCHK(launch_pass<T>(h, 0, InAxpy<T>{zz, pold, &sc->beta, T(0)},
+                             OutPUpd<T>{pnew, v.Ap, v.x, pold, &sc->alpha, T(0)}, sc, h->partial, &np));
     hipLaunchKernelGGL(krylov_alpha, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, np, sc, use_mg ? 1 : 0);
     const dim3 g = ew_grid(h, n);
     hipLaunchKernelGGL((update_r<T, kEwThreads>), g, dim3(kEwThreads), 0, st, v.r, (const T*)v.Ap,
@@ -651,9 +688,9 @@ int issue_iteration(sbp_handle h, const KVecs<T>& v, Scalars* sc, bool use_mg, i
                        use_mg ? 0 : 1);
     HIPCHK(hipGetLastError());
     if (use_mg) {
-        STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, sc, h->partial));
-        hipLaunchKernelGGL(krylov_beta, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, vcycle_nparts<T>(h),
-                           sc);
+        int nvp = 0;
+        STCHK(krylov_vcycle<T>(h, v, sc, &nvp));
+        hipLaunchKernelGGL(krylov_beta, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, nvp, sc);
         HIPCHK(hipGetLastError());
     }
     return SBP_OK;
@@ -740,9 +777,9 @@ int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use
         HIPCHK(hipGetLastError());
     } else {
         if (use_mg) {
-            STCHK(vcycle_level<T>(h, 0, v.r, v.z, v.r, sc, h->partial));
-            hipLaunchKernelGGL(krylov_set_rz, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial,
-                               vcycle_nparts<T>(h), sc);
+            int nvp = 0;
+            STCHK(krylov_vcycle<T>(h, v, sc, &nvp));
+            hipLaunchKernelGGL(krylov_set_rz, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, nvp, sc);
             HIPCHK(hipGetLastError());
             z0 = v.z;
         }
@@ -840,8 +877,11 @@ int krylov64(sbp_handle h, const double* b, double* x, const double* abstol, int
         }
     } else {
         int np = 0;
-        STCHK(launch_pass<double>(h, 0, InPlain<double>{h->x}, OutResid<double, double>{b, h->r}, nullptr,
-                                  h->partial, &np));
+        if (h->var)
+            STCHK(launch_var_pass<double>(h, 0, VarIn<double>{h->x}, VarOutResid<double>{b, h->r}, nullptr, &np));
+        else
+            STCHK(launch_pass<double>(h, 0, InPlain<double>{h->x}, OutResid<double, double>{b, h->r}, nullptr,
+                                      h->partial, &np));
         hipLaunchKernelGGL(krylov_init, dim3(h->nprob), dim3(kRedThreads), 0, st, h->partial, np, h->sc,
                            (const double*)h->dabs, hist, maxiter);
     }
@@ -1505,7 +1545,7 @@ int var_hierarchy(sbp_handle h, const VarTab& t0, const double* bx, const double
             V.dtab = h->vtab;
             V.band = h->vband;
             V.x = h->z;
-            V.t = h->pv2;
+            V.t = h->Ap;   // free while a Krylov iteration runs its V-cycle (p ping-pongs over pv / pv2)
         } else {
             std::vector<double> cx(V.N), cy(V.N);
             for (int k = 0; k < V.N; ++k) { cx[k] = bxl[2 * k]; cy[k] = byl[2 * k]; }
@@ -1692,44 +1732,6 @@ int launch_var_rr(sbp_handle h, int l, const T* u, const T* b, T* rc, const Scal
     return SBP_OK;
 }
 
-// CG on the variable-coefficient operator (unfused: product with p'Ap,
-// alpha, x and r updates with r'r, check, direction update), polled as the
-// other solvers.
-int var_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, double* hist) {
-    hipStream_t st = h->st;
-    const size_t n = h->L[0].n;
-    Scalars* sc = h->sc;
-    STCHK(put_abstol(h, &abstol));
-    hipLaunchKernelGGL(stamp_start, dim3(1), dim3(256), 0, st, sc, 1);
-    int np = 0;
-    STCHK(launch_var(h, 2, x, h->r, b, 1.0, nullptr, &np));
-    hipLaunchKernelGGL(krylov_init, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, (const double*)h->dabs,
-                       hist, maxiter);
-    hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n), 1), dim3(kEwThreads), 0, st,
-                       (const double*)h->r, h->pv, n, (const Scalars*)sc);
-    HIPCHK(hipGetLastError());
-    const bool adaptive = h->prm.poll_every == 0;
-    int next = adaptive ? 2 : h->prm.poll_every;
-    const dim3 g(ew_blocks(n), 1);
-    for (int it = 0; it < maxiter; ++it) {
-        STCHK(launch_var(h, 1, h->pv, h->Ap, h->pv, 1.0, sc, &np));
-        hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, 0);
-        hipLaunchKernelGGL((update_xr<double, kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, (const double*)h->pv,
-                           (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
-        hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 1);
-        hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)h->r,
-                           (const Scalars*)sc, n);
-        HIPCHK(hipGetLastError());
-        if (it + 1 >= next && it + 1 < maxiter) {
-            bool all = false;
-            STCHK(poll(h, sc, &all));
-            if (all) break;
-            next = it + 1 + (adaptive ? poll_ahead(h) : h->prm.poll_every);
-        }
-    }
-    return poll(h, sc);
-}
-
 // One V-cycle x ~ A_l^-1 b on level l of the variable-coefficient hierarchy
 // from a zero initial guess: nu_pre damped-Jacobi sweeps (the first one
 // x = w D^-1 b), residual, full-weighting restriction (times restrict_scale),
@@ -1739,7 +1741,7 @@ int var_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter,
 // T is the storage type of the level vectors (float: the b32 / x32 / t32
 // copies, level 0 sweeping through Ap32); the kernels compute in FP64.
 template <typename T>
-int var_vcycle(sbp_handle h, int l, const T* b, T* x, const Scalars* sc, int* np_rz = nullptr) {
+int var_vcycle(sbp_handle h, int l, const T* b, T* x, const Scalars* sc, int* np_rz) {
     constexpr bool f32 = std::is_same<T, float>::value;
     auto& V = h->vl[l];
     const int nlev = (int)h->vl.size();
@@ -1769,7 +1771,7 @@ int var_vcycle(sbp_handle h, int l, const T* b, T* x, const Scalars* sc, int* np
         const VarFusedArgs a{V.N, V.dtab, V.band, V.band + nb, om, h->prm.restrict_scale, sc ? &sc->done : nullptr};
         if (!var_fused_shape_ok(ti, tj)) return SBP_ERR_INVALID_ARG;
         STCHK(launch_var_smooth2_restrict<T>(h->p, ti, tj, a, b, tl, cb, st));
-        STCHK(var_vcycle<T>(h, l + 1, (const T*)cb, cx, sc));
+        STCHK(var_vcycle<T>(h, l + 1, (const T*)cb, cx, sc, nullptr));
         STCHK(launch_var_prolong_smooth2<T>(h->p, ti, tj, a, (const T*)tl, (const T*)cx, b, x, np_rz ? b : nullptr,
                                             h->partial, st));
         if (np_rz) *np_rz = fused_post_blocks(V.N, ti, tj);
@@ -1800,7 +1802,7 @@ int var_vcycle(sbp_handle h, int l, const T* b, T* x, const Scalars* sc, int* np
     }
     // residual and full-weighting restriction in one pass (the fine residual stays on chip)
     STCHK(launch_var_rr<T>(h, l, (const T*)bufs[cur], b, cb, sc));
-    STCHK(var_vcycle<T>(h, l + 1, (const T*)cb, cx, sc));
+    STCHK(var_vcycle<T>(h, l + 1, (const T*)cb, cx, sc, nullptr));
     if (nu2 == 0) {
         hipLaunchKernelGGL((prolong_add<T>), dim3((V.N + 255) / 256, V.N, 1), dim3(256), 0, st, (const T*)cx,
                            bufs[cur], V.N, C.N, sc);
@@ -1842,82 +1844,56 @@ int ensure_var32(sbp_handle h) {
     return SBP_OK;
 }
 
-// MG-preconditioned CG on the variable-coefficient operator (unfused, as
-// var_cg): the V-cycle above as preconditioner; stops on ||r||_2 < abstol.
-// With f32 the V-cycle runs on FP32 copies (r rounded down by the residual
-// update, z read in FP32 by r'z and the search-direction update; the Krylov
-// vectors, the operator products and the inner products stay FP64).
-int var_pcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, double* hist, bool f32 = false) {
+// FP64 MG-PCG on the variable-coefficient operator with the V-cycle on FP32
+// copies of the level vectors (sbp_mgpcg_mixed on such a handle): r rounded
+// to FP32 by the residual update, z read in FP32 by r'z and the
+// search-direction update; the Krylov vectors, the operator products and the
+// inner products stay FP64.  Stops on ||r||_2 < abstol.  (The FP64 V-cycle
+// runs in the shared Krylov loop, krylov64.)
+int var_pcg_f32vc(sbp_handle h, const double* b, double* x, double abstol, int maxiter) {
     hipStream_t st = h->st;
     const size_t n = h->L[0].n;
     Scalars* sc = h->sc;
-    double* z = h->vl[0].x;
     STCHK(put_abstol(h, &abstol));
     hipLaunchKernelGGL(stamp_start, dim3(1), dim3(256), 0, st, sc, 1);
     int np = 0;
     STCHK(launch_var(h, 2, x, h->r, b, 1.0, nullptr, &np));
     hipLaunchKernelGGL(krylov_init, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, (const double*)h->dabs,
-                       hist, maxiter);
+                       (double*)nullptr, maxiter);
     HIPCHK(hipGetLastError());
     const dim3 g(ew_blocks(n), 1);
-    int nrz = 0;
-    // f32: r32 holds r rounded to FP32 (the residual update writes it after
-    // the first precondition) and z stays in z32 (read as such by r'z and
-    // the search-direction update)
+    // z = M r with the r'z partials (g.x of them); r32 holds r rounded to
+    // FP32 once the residual update has written it
     bool r32_ready = false;
-    auto precondition = [&]() -> int {   // z = M r ; r'z partials (nrz of them)
-        nrz = 0;
-        if (f32) {
-            if (!r32_ready)
-                hipLaunchKernelGGL((cast_vec<double, float, kEwThreads>), g, dim3(kEwThreads), 0, st,
-                                   (const double*)h->r, h->r32, n);
-            STCHK(var_vcycle<float>(h, 0, (const float*)h->r32, h->z32, sc));
-            hipLaunchKernelGGL((dot_partial<double, kEwThreads, float>), g, dim3(kEwThreads), 0, st,
-                               (const double*)h->r, (const float*)h->z32, n, h->partial);
-            HIPCHK(hipGetLastError());
-            nrz = (int)g.x;
-        } else {
-            STCHK(var_vcycle<double>(h, 0, (const double*)h->r, z, sc, h->prm.nu_post > 0 ? &nrz : nullptr));
-        }
-        if (nrz == 0) {
-            hipLaunchKernelGGL((dot_partial<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->r,
-                               (const double*)z, n, h->partial);
-            HIPCHK(hipGetLastError());
-            nrz = (int)g.x;
-        }
+    auto precondition = [&]() -> int {
+        if (!r32_ready)
+            hipLaunchKernelGGL((cast_vec<double, float, kEwThreads>), g, dim3(kEwThreads), 0, st,
+                               (const double*)h->r, h->r32, n);
+        STCHK(var_vcycle<float>(h, 0, (const float*)h->r32, h->z32, sc, nullptr));
+        hipLaunchKernelGGL((dot_partial<double, kEwThreads, float>), g, dim3(kEwThreads), 0, st,
+                           (const double*)h->r, (const float*)h->z32, n, h->partial);
+        HIPCHK(hipGetLastError());
         return SBP_OK;
     };
     STCHK(precondition());
-    hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(kRedThreads), 0, st, h->partial, nrz, sc);
-    if (f32)
-        hipLaunchKernelGGL((cast_vec<float, double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const float*)h->z32,
-                           h->pv, n);
-    else
-        hipLaunchKernelGGL((copy_vec<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)z, h->pv, n,
-                           (const Scalars*)sc);
+    hipLaunchKernelGGL(krylov_set_rz, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc);
+    hipLaunchKernelGGL((cast_vec<float, double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const float*)h->z32,
+                       h->pv, n);
     HIPCHK(hipGetLastError());
-    r32_ready = f32;
+    r32_ready = true;
     const bool adaptive = h->prm.poll_every == 0;
     int next = adaptive ? 2 : h->prm.poll_every;
     for (int it = 0; it < maxiter; ++it) {
         STCHK(launch_var(h, 1, h->pv, h->Ap, h->pv, 1.0, sc, &np));
         hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, 1);
-        if (f32)
-            hipLaunchKernelGGL((update_xr_r32<kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, h->r32,
-                               (const double*)h->pv, (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
-        else
-            hipLaunchKernelGGL((update_xr<double, kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r,
-                               (const double*)h->pv, (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
+        hipLaunchKernelGGL((update_xr_r32<kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, h->r32,
+                           (const double*)h->pv, (const double*)h->Ap, (const Scalars*)sc, n, h->partial);
         hipLaunchKernelGGL(krylov_check, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc, 0);
         HIPCHK(hipGetLastError());
         STCHK(precondition());
-        hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(kRedThreads), 0, st, h->partial, nrz, sc);
-        if (f32)
-            hipLaunchKernelGGL((update_p<double, kEwThreads, float>), g, dim3(kEwThreads), 0, st, h->pv,
-                               (const float*)h->z32, (const Scalars*)sc, n);
-        else
-            hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)z,
-                               (const Scalars*)sc, n);
+        hipLaunchKernelGGL(krylov_beta, dim3(1), dim3(kRedThreads), 0, st, h->partial, (int)g.x, sc);
+        hipLaunchKernelGGL((update_p<double, kEwThreads, float>), g, dim3(kEwThreads), 0, st, h->pv,
+                           (const float*)h->z32, (const Scalars*)sc, n);
         HIPCHK(hipGetLastError());
         if (it + 1 >= next && it + 1 < maxiter) {
             bool all = false;
@@ -2109,7 +2085,7 @@ int sbp_vcycle(sbp_handle h, const void* r, void* z, int dtype) {
     if (h && h->ragged) return SBP_ERR_INVALID_ARG;   // ragged handles serve the batched solvers only
     if (h && h->var) {
         if (!r || !z || !h->vmg || dtype != SBP_F64) return SBP_ERR_INVALID_ARG;
-        return var_vcycle<double>(h, 0, (const double*)r, (double*)z, nullptr);
+        return var_vcycle<double>(h, 0, (const double*)r, (double*)z, nullptr, nullptr);
     }
     if (!h || !r || !z || !h->mg) return SBP_ERR_INVALID_ARG;
     if (dtype == SBP_F64) return vcycle_level<double>(h, 0, (const double*)r, (double*)z, nullptr, nullptr, nullptr);
@@ -2123,12 +2099,7 @@ int sbp_vcycle(sbp_handle h, const void* r, void* z, int dtype) {
 int sbp_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
            double* hist) {
     if (!h || !b || !x || maxiter < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
-    if (h->var) {
-        STCHK(var_cg(h, b, x, abstol, maxiter, hist));
-        get_results(h, iters, resnorm, nullptr);
-        return batch_status(h);
-    }
-    STCHK(krylov64(h, b, x, &abstol, maxiter, hist, false));
+    STCHK(krylov64(h, b, x, &abstol, maxiter, hist, false));   // (either operator)
     get_results(h, iters, resnorm, nullptr);
     return batch_status(h);
 }
@@ -2136,13 +2107,7 @@ int sbp_cg(sbp_handle h, const double* b, double* x, double abstol, int maxiter,
 int sbp_mgpcg(sbp_handle h, const double* b, double* x, double abstol, int maxiter, int* iters, double* resnorm,
               double* hist) {
     if (!h || !b || !x || maxiter < 0 || !(abstol >= 0) || h->nprob != 1) return SBP_ERR_INVALID_ARG;
-    if (h->var) {
-        if (!h->vmg) return SBP_ERR_INVALID_SIZE;
-        STCHK(var_pcg(h, b, x, abstol, maxiter, hist));
-        get_results(h, iters, resnorm, nullptr);
-        return batch_status(h);
-    }
-    if (!h->mg) return SBP_ERR_INVALID_SIZE;
+    if (h->var ? !h->vmg : !h->mg) return SBP_ERR_INVALID_SIZE;
     STCHK(krylov64(h, b, x, &abstol, maxiter, hist, true));
     get_results(h, iters, resnorm, nullptr);
     return batch_status(h);
@@ -2168,7 +2133,7 @@ int sbp_mgpcg_mixed(sbp_handle h, const double* b, double* x, double abstol, int
         if (!h->vmg) return SBP_ERR_INVALID_SIZE;
         STCHK(ensure_var32(h));
         const int cap = h->prm.inner_maxiter * std::max(max_outer, 1);
-        STCHK(var_pcg(h, b, x, abstol, cap, nullptr, true));
+        STCHK(var_pcg_f32vc(h, b, x, abstol, cap));
         int it = 0;
         get_results(h, &it, resnorm, nullptr);
         if (outer_iters) *outer_iters = it;

@@ -352,7 +352,8 @@ template <typename T> struct VarOutStore {
     T* o;
     double s;
     static constexpr bool kDiag = false, kDot = false;
-    __device__ __forceinline__ double pre(size_t) const { return 0.0; }
+    using Aux = double;
+    __device__ __forceinline__ Aux pre(size_t) const { return 0.0; }
     __device__ __forceinline__ void emit(size_t q, double, double Au, double, double, double&) const {
         o[q] = (T)(s * Au);
     }
@@ -363,7 +364,8 @@ template <typename T> struct VarOutStoreDot {
     T* o;
     const T* a;
     static constexpr bool kDiag = false, kDot = true;
-    __device__ __forceinline__ double pre(size_t q) const { return (double)a[q]; }
+    using Aux = double;
+    __device__ __forceinline__ Aux pre(size_t q) const { return (double)a[q]; }
     __device__ __forceinline__ void emit(size_t q, double, double Au, double, double av, double& acc) const {
         o[q] = (T)Au;
         acc += av * Au;
@@ -375,7 +377,8 @@ template <typename T> struct VarOutResid {
     const T* b;
     T* r;
     static constexpr bool kDiag = false, kDot = true;
-    __device__ __forceinline__ double pre(size_t q) const { return (double)b[q]; }
+    using Aux = double;
+    __device__ __forceinline__ Aux pre(size_t q) const { return (double)b[q]; }
     __device__ __forceinline__ void emit(size_t q, double, double Au, double, double bq, double& acc) const {
         const double rv = bq - Au;
         r[q] = (T)rv;
@@ -389,7 +392,8 @@ template <typename T> struct VarOutSmooth {
     T* xo;
     double omega;
     static constexpr bool kDiag = true, kDot = false;
-    __device__ __forceinline__ double pre(size_t q) const { return (double)b[q]; }
+    using Aux = double;
+    __device__ __forceinline__ Aux pre(size_t q) const { return (double)b[q]; }
     __device__ __forceinline__ void emit(size_t q, double u, double Au, double dinv, double bq, double&) const {
         xo[q] = (T)(u + omega * (bq - Au) * dinv);
     }
@@ -401,11 +405,57 @@ template <typename T> struct VarOutSmoothDot {
     T* xo;
     double omega;
     static constexpr bool kDiag = true, kDot = true;
-    __device__ __forceinline__ double pre(size_t q) const { return (double)b[q]; }
+    using Aux = double;
+    __device__ __forceinline__ Aux pre(size_t q) const { return (double)b[q]; }
     __device__ __forceinline__ void emit(size_t q, double u, double Au, double dinv, double bq, double& acc) const {
         const double o = u + omega * (bq - Au) * dinv;
         xo[q] = (T)o;
         acc += bq * o;
+    }
+};
+
+// The fused Krylov step on the search direction (FP64): the operand is
+// p_new = z + beta p_old (beta from the device scalar block); the output writes
+// p_new and A p_new, applies the pending x += alpha p_old of the previous
+// iteration and accumulates p_new' A p_new.
+struct VarInAxpy {
+    const double* r;
+    const double* p;
+    const double* beta;
+    double bt;
+    static constexpr bool kDiag = false, kCol = false;
+    struct Raw { double r, p; };
+    __device__ __forceinline__ void init() { bt = *beta; }
+    __device__ __forceinline__ Raw fetch(int i, int j, int N) const {
+        const size_t q = gidx(i, j, N);
+        Raw v;
+        v.r = r[q];
+        v.p = p[q];
+        return v;
+    }
+    __device__ __forceinline__ double make(const Raw& v, int, int, double) const { return v.r + bt * v.p; }
+};
+struct VarOutPUpd {
+    double* pnew;
+    double* Ap;
+    double* x;
+    const double* pold;
+    const double* alpha;
+    double al;
+    static constexpr bool kDiag = false, kDot = true;
+    struct Aux { double x, p; };
+    __device__ __forceinline__ void init() { al = *alpha; }
+    __device__ __forceinline__ Aux pre(size_t q) const {
+        Aux a;
+        a.x = x[q];
+        a.p = pold[q];
+        return a;
+    }
+    __device__ __forceinline__ void emit(size_t q, double u, double Au, double, const Aux& a, double& acc) const {
+        pnew[q] = u;
+        Ap[q] = Au;
+        x[q] = a.x + al * a.p;
+        acc += u * Au;
     }
 };
 
@@ -464,9 +514,9 @@ __device__ __forceinline__ double var_tile(const IN& in, const OUT& out, const V
             for (int k = 0; k < W; ++k) raw[k] = in.fetch(i, j0 - H + k, N);
         }
     }
-    double aux[J];
+    typename OUT::Aux aux[J];
 #pragma unroll
-    for (int r = 0; r < J; ++r) aux[r] = act ? out.pre(gidx(i, j0 + r, N)) : 0.0;
+    for (int r = 0; r < J; ++r) aux[r] = act ? out.pre(gidx(i, j0 + r, N)) : typename OUT::Aux{};
     double cm[2 * H + 1];
 #pragma unroll
     for (int q = 0; q <= 2 * H; ++q) cm[q] = iint ? mb[(size_t)q * N + i] : 0.0;
@@ -561,6 +611,8 @@ __global__ __launch_bounds__(kVarThreads) void var_pass(IN in, OUT out, const Va
     const int tid = threadIdx.x;
     const int blk = xcd_remap((int)blockIdx.x, nbx * nby);
     const int i0 = (blk % nbx) * kVarThreads, j0 = (blk / nbx) * J;
+    if constexpr (has_init<IN>::value) in.init();
+    if constexpr (has_init<OUT>::value) out.init();
     const bool jin = j0 - H >= R && j0 + J + H <= N - R;
     const bool iin = i0 - H >= R && i0 + kVarThreads + H <= N - R;
     double acc = 0.0;
