@@ -458,13 +458,14 @@ __device__ __forceinline__ T block_sum(T v) {
 // columns; uc is the operand at (i, j).
 // (The closure width is the order's compile-time C, so the unrolled loops
 // carry no dead iterations for the narrower orders.)
-// Inputs whose raw fetch is more than two values (x + P e: five) form each
-// closure operand as it is loaded instead of holding every raw fetch at once:
-// the face tiles are few, but the kernel's register allocation is the largest
-// over all its paths, so a face path holding C x 5 values would cap the
-// occupancy of every tile.
+// Inputs whose raw fetch is more than one value (x + P e: five, r + beta p:
+// two) form each operand of the closures and of the face-row tile as it is
+// loaded instead of holding every raw fetch at once: few blocks take those
+// paths, but the kernel's register allocation is the largest over all its
+// paths, so a face path holding C x 5 values would cap the occupancy of
+// every tile.
 template <class IN, typename T>
-constexpr bool big_raw() { return sizeof(typename IN::Raw) > 2 * sizeof(T); }
+constexpr bool multi_raw() { return sizeof(typename IN::Raw) > sizeof(T); }
 
 template <int P, typename T, bool LT = false, class IN>
 __device__ __forceinline__ T k_closure(const OpTab& t, const IN& in, int i, int j, T uc) {
@@ -472,7 +473,7 @@ __device__ __forceinline__ T k_closure(const OpTab& t, const IN& in, int i, int 
     const bool lo = j < t.R;
     const int r = lo ? j : t.N - 1 - j;
     T s = (T)lut<LT>(t).krs[r] * uc;
-    if constexpr (big_raw<IN, T>()) {
+    if constexpr (multi_raw<IN, T>()) {
 #pragma unroll
         for (int c = 0; c < CW; ++c)
             if (c != r) s += (T)lut<LT>(t).kb[r][c] * (in.template load<false, LT>(t, i, lo ? c : t.N - 1 - c) - uc);
@@ -494,7 +495,7 @@ __device__ __forceinline__ T m_closure(const OpTab& t, const IN& in, int i, int 
     const bool lo = i < t.R;
     const int r = lo ? i : t.N - 1 - i;
     T s = (T)lut<LT>(t).mrs[r] * uc;
-    if constexpr (big_raw<IN, T>()) {
+    if constexpr (multi_raw<IN, T>()) {
 #pragma unroll
         for (int c = 0; c < CW; ++c)
             if (c != r) s += (T)lut<LT>(t).mb[r][c] * (in.template load<false, LT>(t, lo ? c : t.N - 1 - c, j) - uc);
@@ -570,6 +571,37 @@ template <class X> struct pre_rhs<X, decltype(void(X::kPreRhs))> { static conste
 template <class X, class = void> struct has_init { static constexpr bool value = false; };
 template <class X> struct has_init<X, decltype(void(&X::init))> { static constexpr bool value = true; };
 
+// The pass's LDS tile: J rows of the operand on the block's TI columns, a
+// right halo of H columns and a left halo of HL = C columns (the closure
+// width), so that the closure rows of m on a face-column block read their C
+// operand values from the tile: at the left face they are the first block's
+// own columns, at the right face the last C columns, which a last block
+// narrower than C reaches through that halo.  Interior blocks load H of the
+// left halo.
+template <int P, int TI> constexpr int tile_w() { return TI + Stencil<P>::C + Stencil<P>::H; }
+
+// m_closure with the operand read from the tile row (columns [i0 - HL,
+// i0 + TI + H)); a column outside it (the block before a last block wider
+// than H, only for N not of the form 2^k + 1) from memory.  Same order of
+// operations as m_closure.
+template <int P, int TI, typename T, bool LT, class IN>
+__device__ __forceinline__ T m_closure_tile(const OpTab& t, const IN& in, int i, int j, int i0, const T* trow, T uc) {
+    constexpr int CW = Stencil<P>::C;
+    constexpr int H = Stencil<P>::H;
+    constexpr int HL = Stencil<P>::C;
+    const bool lo = i < t.R;
+    const int r = lo ? i : t.N - 1 - i;
+    T s = (T)lut<LT>(t).mrs[r] * uc;
+#pragma unroll
+    for (int c = 0; c < CW; ++c)
+        if (c != r) {
+            const int col = lo ? c : t.N - 1 - c;
+            const T v = col < i0 + TI + H ? trow[col - i0 + HL] : in.template load<false, LT>(t, col, j);
+            s += (T)lut<LT>(t).mb[r][c] * (v - uc);
+        }
+    return s;
+}
+
 // Tile whose rows (and their halo) lie inside the grid, away from the closure
 // rows in j.  With ICHK = false the columns are interior too: straight-line
 // code with no bounds tests and a constant inverse diagonal (the common
@@ -578,9 +610,11 @@ template <class X> struct has_init<X, decltype(void(&X::init))> { static constex
 // the closure sum and the H weight; everything else stays on the fast path.
 template <int P, int TI, int J, bool ICHK, typename T, bool LT = false, class IN, class OUT>
 __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const OpTab& t, int i0, int j0,
-                                           T (*tile)[TI + 2 * Stencil<P>::H]) {
+                                           T (*tile)[tile_w<P, TI>()]) {
     constexpr int H = Stencil<P>::H;
     constexpr int W = J + 2 * H;
+    constexpr int HL = Stencil<P>::C;          // centre column of thread tid: tid + HL
+    constexpr int NL = ICHK ? HL : H;           // left halo columns loaded
     constexpr bool INT = !ICHK;
     const int tid = threadIdx.x;
     const int i = i0 + tid;
@@ -612,23 +646,22 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
 #pragma unroll
         for (int k = 0; k < W; ++k) w[k] = T(0);
     }
-    // 2H halo columns x J rows, spread over the first threads.
+    // NL + H halo columns x J rows, spread over the first threads.
 #pragma unroll
-    for (int e0 = 0; e0 < 2 * H * J; e0 += TI) {
+    for (int e0 = 0; e0 < (NL + H) * J; e0 += TI) {
         const int e = e0 + tid;
-        if (e < 2 * H * J) {
-            const int side = e / (H * J), rem = e % (H * J);
-            const int off = rem / J, r = rem % J;
-            const int ii = side == 0 ? i0 - H + off : i0 + TI + off;
+        if (e < (NL + H) * J) {
+            const int c = e / J, r = e % J;
+            const int ii = c < NL ? i0 - NL + c : i0 + TI + (c - NL);
             T v = T(0);
             if (!ICHK || (ii >= 0 && ii < N)) v = in.template load<INT, LT>(t, ii, j0 + r);
-            tile[r][side == 0 ? off : TI + H + off] = v;
+            tile[r][c < NL ? HL - NL + c : HL + TI + (c - NL)] = v;
         }
     }
     typename OUT::Aux aux[J];
 #pragma unroll
     for (int r = 0; r < J; ++r) {
-        tile[r][tid + H] = w[r + H];
+        tile[r][tid + HL] = w[r + H];
         if constexpr (kRhsRaw) {
             aux[r] = rhs[r];
         } else {
@@ -648,12 +681,12 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
         for (int s = 2; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
         T mp;
         if (iint) {
-            const T* row = &tile[r][tid + H];
+            const T* row = &tile[r][tid + HL];
             mp = Stencil<P>::template c<T>(1) * ((row[-1] - uc) + (row[1] - uc));
 #pragma unroll
             for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
         } else {
-            mp = m_closure<P, T, LT>(t, in, i, j0 + r, uc);
+            mp = m_closure_tile<P, TI, T, LT>(t, in, i, j0 + r, i0, tile[r], uc);
         }
         const T Au = ICHK ? hwi * kp + mp : kp + mp;   // hw_j = 1 on interior rows
         out.template emit<INT, LT>(t, i, j0 + r, loc(i, j0 + r, N), uc, Au, aux[r], acc);
@@ -664,7 +697,7 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
 // General tile: bounds tests, face closures and H weights.
 template <int P, int TI, int J, typename T, bool LT = false, class IN, class OUT>
 __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const OpTab& t, int i0, int j0,
-                                          T (*tile)[TI + 2 * Stencil<P>::H]) {
+                                          T (*tile)[tile_w<P, TI>()]) {
     constexpr int H = Stencil<P>::H;
     constexpr int W = J + 2 * H;
     const int tid = threadIdx.x;
@@ -672,7 +705,7 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
     const int i = i0 + tid;
     const bool act = i < N;
     T w[W];
-    if constexpr (big_raw<IN, T>()) {
+    if constexpr (multi_raw<IN, T>()) {
 #pragma unroll
         for (int k = 0; k < W; ++k) {
             const int jj = j0 - H + k;
@@ -695,7 +728,7 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
         const int side = tid / H, off = tid % H;
         const int ii = side == 0 ? i0 - H + off : i0 + TI + off;
         const int col = side == 0 ? off : TI + H + off;
-        if constexpr (big_raw<IN, T>()) {
+        if constexpr (multi_raw<IN, T>()) {
 #pragma unroll
             for (int r = 0; r < J; ++r) {
                 const int jj = j0 + r;
@@ -768,7 +801,7 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
     out.at(q, ctl.bt);
     in.init();
     if constexpr (has_init<OUT>::value) out.init();
-    __shared__ T tile[J][TI + 2 * H];
+    __shared__ T tile[J][tile_w<P, TI>()];
     // every point the tile loads (halo included) lies outside the closure rows
     const bool jin = j0 - H >= t.R && j0 + J + H <= N - t.R;
     const bool iin = i0 - H >= t.R && i0 + TI + H <= N - t.R;
