@@ -63,6 +63,10 @@ typedef enum {
 typedef enum { SBP_F64 = 0, SBP_F32 = 1 } sbp_dtype;
 /* Or-ed into sbp_set_tiles' dtype: the shape of the fused two-sweep kernels. */
 #define SBP_FUSED 2
+/* Or-ed into sbp_set_tiles' dtype: the line-aligned variant of a stencil tile
+ * shape (256 x 8, 256 x 16 and 512 x 8; p = 2 and 4): each block stores every grid row
+ * in whole 128-byte lines with non-temporal stores. */
+#define SBP_ALIGNED 4
 
 typedef struct {
     int nu_pre;              /* pre-smoothing sweeps per level (default 2) */
@@ -154,6 +158,9 @@ int sbp_set_stream(sbp_handle h, void* hip_stream);
 int sbp_get_info(sbp_handle h, int* N, int* p, int* nlevels);
 /* Stencil tile shape (tile_i threads along i x tile_j rows per thread) used
  * on `level` for element type `dtype`; shapes not compiled in are rejected.
+ * With SBP_ALIGNED or-ed into dtype: the shape's line-aligned variant (each
+ * block computes tile_i columns and stores tile_i - 128 / element-size of
+ * them per row, starting on a 128-byte line).
  * With SBP_FUSED or-ed into dtype: the block shape (tile_i output columns x
  * tile_j fine rows) of the level's fused two-sweep kernels (64 x 4, 64 x 8,
  * 128 x 8, 256 x 8; for p = 6 all but 256 x 8, whose FP64 tiles exceed the

@@ -26,6 +26,7 @@ SBP_ERR_SCRATCH = 10
 SBP_F64 = 0
 SBP_F32 = 1
 SBP_FUSED = 2   # or-ed into sbp_set_tiles' dtype: shape of the fused two-sweep kernels
+SBP_ALIGNED = 4   # or-ed into sbp_set_tiles' dtype: the line-aligned variant of a stencil tile shape
 
 # Every exported entry point: name -> (restype, argtypes)
 _vp = C.c_void_p

@@ -24,11 +24,16 @@ CACHE = HERE / "tiles_gfx950.json"
 
 
 def compiled_shapes(p=2):
-    """Tile shapes libsbp.so has for order p (p = 6, 8, 10: a subset)."""
+    """Tile shapes (ti, tj, aligned) libsbp.so has for order p (p = 6, 8, 10: a subset)."""
     text = (HERE / "csrc" / "sbp_launch.h").read_text()
     name = "SBP_TILE_SHAPES" if p <= 4 else "SBP_HI_TILE_SHAPES"
     line = re.search(r"#define " + name + r"\(X\)(.*)", text).group(1)
-    return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+),\s*(\d+)\)", line)]
+    return [(int(a), int(b), int(c)) for a, b, c in re.findall(r"X\((\d+),\s*(\d+),\s*(\d+)\)", line)]
+
+
+def shape_name(shape):
+    """'256x8' for an unaligned shape, '256x8a' for its line-aligned variant."""
+    return f"{shape[0]}x{shape[1]}" + ("a" if len(shape) > 2 and shape[2] else "")
 
 
 def load_cache(path: os.PathLike | None = None) -> dict:
@@ -59,8 +64,9 @@ def time_level(h, level, dtype, shapes, reps=None):
     ins = [torch.randn(n, dtype=dtype, device=h.device) for _ in range(nbuf)]
     outs = [torch.empty_like(ins[0]) for _ in range(nbuf)]
     res = {}
-    for ti, tj in shapes:
-        h.set_tiles(level, ti, tj)
+    for shape in shapes:
+        ti, tj = shape[0], shape[1]
+        h.set_tiles(level, ti, tj, aligned=len(shape) > 2 and bool(shape[2]))
         for k in range(nbuf):
             h.apply(ins[k], outs[k], level=level)
         torch.cuda.synchronize()
@@ -71,7 +77,7 @@ def time_level(h, level, dtype, shapes, reps=None):
             h.apply(ins[r % nbuf], outs[r % nbuf], level=level)
         e1.record(s)
         torch.cuda.synchronize()
-        res[(ti, tj)] = e0.elapsed_time(e1) / reps * 1e3   # microseconds
+        res[tuple(shape)] = e0.elapsed_time(e1) / reps * 1e3   # microseconds
     return res
 
 
@@ -85,13 +91,15 @@ def tune(N, orders=(2, 4), out=None, verbose=True):
         nlev = max(1, h.nlevels)
         for level in range(nlev):
             Nl = h.level_size(level)
+            # line-aligned shapes only where a level streams far more than launch latency
+            lshapes = [sh for sh in shapes if Nl >= 2049 or not sh[2]]
             for dtype, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
-                res = time_level(h, level, dtype, shapes)
+                res = time_level(h, level, dtype, lshapes)
                 best = min(res, key=res.get)
                 es = 8 if tag == "f64" else 4
                 cache[key(p, tag, Nl)] = {"tile": list(best), "us": res[best],
                                           "GBs": 2 * Nl * Nl * es / (res[best] * 1e-6) / 1e9,
-                                          "all_us": {f"{a}x{b}": round(v, 3) for (a, b), v in res.items()}}
+                                          "all_us": {shape_name(k): round(v, 3) for k, v in res.items()}}
                 if verbose:
                     print(f"p={p} {tag} N={Nl}: best {best} {res[best]:.2f} us "
                           f"({cache[key(p, tag, Nl)]['GBs']:.0f} GB/s)", flush=True)

@@ -106,7 +106,8 @@ class SbpBatch:
             for dt, tag in ((capi.SBP_F64, "f64"), (capi.SBP_F32, "f32")):
                 t = autotune.lookup(self.p, tag, Nl, c)
                 if t:
-                    capi.check(self.lib.sbp_set_tiles(self._h, level, dt, t[0], t[1]), "sbp_set_tiles")
+                    fl = capi.SBP_ALIGNED if len(t) > 2 and t[2] else 0
+                    capi.check(self.lib.sbp_set_tiles(self._h, level, dt | fl, t[0], t[1]), "sbp_set_tiles")
 
     def _abstol(self, abstol):
         if isinstance(abstol, (int, float)):

@@ -36,9 +36,10 @@ FLAGS = [
 
 
 def tile_shapes(name="SBP_TILE_SHAPES"):
+    """(ti, tj, aligned) of every compiled stencil tile shape."""
     text = (CSRC / "sbp_launch.h").read_text()
     line = re.search(r"#define " + name + r"\(X\)(.*)", text).group(1)
-    return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+),\s*(\d+)\)", line)]
+    return [(int(a), int(b), int(c)) for a, b, c in re.findall(r"X\((\d+),\s*(\d+),\s*(\d+)\)", line)]
 
 
 def _run(cmd):
@@ -65,9 +66,9 @@ def build(jobs: int | None = None, verbose: bool = False, incremental: bool = Fa
              (CSRC / "sbp_fused.hip", OBJ / "sbp_fused.o", []),
              (CSRC / "sbp_var_fused.hip", OBJ / "sbp_var_fused.o", [])]
     hi = set(tile_shapes("SBP_HI_TILE_SHAPES"))
-    for ti, tj in tile_shapes():
-        units.append((CSRC / "sbp_tile.hip", OBJ / f"sbp_tile_{ti}x{tj}.o",
-                      [f"-DSBP_TI={ti}", f"-DSBP_TJ={tj}", f"-DSBP_HI={int((ti, tj) in hi)}"]))
+    for ti, tj, ta in tile_shapes():
+        units.append((CSRC / "sbp_tile.hip", OBJ / f"sbp_tile_{ti}x{tj}{'a' if ta else ''}.o",
+                      [f"-DSBP_TI={ti}", f"-DSBP_TJ={tj}", f"-DSBP_TA={ta}", f"-DSBP_HI={int((ti, tj, ta) in hi)}"]))
     jobs = jobs or min(len(units), os.cpu_count() or 4)
     cmds = [[HIPCC, *FLAGS, *defs, "-MMD", "-MF", str(obj.with_suffix(".d")), "-c", str(src), "-o", str(obj)]
             for src, obj, defs in units if not (incremental and _up_to_date(obj))]

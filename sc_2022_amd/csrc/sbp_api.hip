@@ -44,6 +44,7 @@ struct Level {
     OpTab tb{};
     OpTab* dtab = nullptr;                // device copy of tb (tb.dev points here)
     int ti[2] = {0, 0}, tj[2] = {0, 0};   // tile shape per precision (0 = f64, 1 = f32)
+    int ta[2] = {0, 0};                   // 1: the shape's line-aligned variant
     int fti[2] = {128, 128}, ftj[2] = {8, 8};   // fused two-sweep block shape per precision
     // per precision (0 = f64, 1 = f32), nprob problems each: rhs, two sweep
     // buffers, output, residual
@@ -161,8 +162,8 @@ inline int ew_blocks(size_t n) {
 }
 
 // Tile shapes that are compiled in.
-inline bool tile_ok(int p, int ti, int tj) {
-#define SBP_TILE_OK(TI_, J_) if (ti == TI_ && tj == J_) return true;
+inline bool tile_ok(int p, int ti, int tj, int ta = 0) {
+#define SBP_TILE_OK(TI_, J_, A_) if (ti == TI_ && tj == J_ && ta == A_) return true;
     if (p > 4) {
         SBP_HI_TILE_SHAPES(SBP_TILE_OK)
         return false;
@@ -172,19 +173,48 @@ inline bool tile_ok(int p, int ti, int tj) {
     return false;
 }
 
-inline void default_tile(int N, int* ti, int* tj) {
-    if (N >= 1025) { *ti = 256; *tj = 8; }
+// Default stencil tile of a level of side N in precision q (0 = FP64, 1 = FP32)
+// of an order-p handle; the autotuner's table (tiles_gfx950.json) refines it.
+inline void default_tile(int N, int p, int q, int* ti, int* tj, int* ta) {
+    *ta = 0;
+    if (p <= 4 && N >= 4097 && (q == 0 || N < 8193)) { *ti = q == 0 ? 512 : 256; *tj = 8; *ta = 1; }   // line-aligned apply
+    else if (N >= 1025) { *ti = 256; *tj = 8; }
     else if (N >= 129) { *ti = 128; *tj = 8; }
     else { *ti = 64; *tj = 4; }
 }
 
-inline size_t pass_blocks(int N, int ti, int tj) {
-    return (size_t)((N + ti - 1) / ti) * (size_t)((N + tj - 1) / tj);
-}
+inline size_t pass_blocks(int N, int ti, int tj, int aln = 0) { return (size_t)nparts_of(N, ti, tj, aln); }
+
+// The tile shape a pass with output policy OUT runs on level L in precision
+// T: the level's shape, except that policies that do not take line-aligned
+// stores (OUT::kAlign false) run an aligned level on the unaligned shape of
+// the same size (256 x 8 where that size has none).
+struct Shape { int ti, tj, ta; };
+template <typename T, class OUT> inline Shape pass_shape(int p, const Level& L);
+
+// Blocks (and block partials per problem) of that pass.
+template <typename T, class OUT> inline size_t pass_nblocks(int p, const Level& L);
 
 template <typename T> inline int prec_index();
 template <> inline int prec_index<double>() { return 0; }
 template <> inline int prec_index<float>() { return 1; }
+
+template <typename T, class OUT> inline Shape pass_shape(int p, const Level& L) {
+    const int q = prec_index<T>();
+    Shape s{L.ti[q], L.tj[q], L.ta[q]};
+    if (s.ta && !OUT::kAlign) {
+        s.ta = 0;
+        if (!tile_ok(p, s.ti, s.tj, 0)) { s.ti = 256; s.tj = 8; }
+    }
+    return s;
+}
+
+template <typename T, class OUT> inline size_t pass_nblocks(int p, const Level& L) {
+    const Shape s = pass_shape<T, OUT>(p, L);
+    return pass_blocks(L.N, s.ti, s.tj, aln_of(s.ta, (int)sizeof(T)));
+}
+
+
 
 inline Batch batch_of(const sbp_handle h, int l) {
     Batch bt;
@@ -205,8 +235,9 @@ template <typename T, class IN, class OUT>
 int launch_pass(sbp_handle h, int l, const IN& in, const OUT& out, const Scalars* scb, double* partial,
                 int* nparts, int nprob = -1) {
     const Level& L = h->L[l];
-    const int ti = L.ti[prec_index<T>()], tj = L.tj[prec_index<T>()];
-    if (nparts) *nparts = (int)pass_blocks(L.N, ti, tj);
+    const Shape sh = pass_shape<T, OUT>(h->p, L);
+    const int ti = sh.ti, tj = sh.tj, ta = sh.ta;
+    if (nparts) *nparts = (int)pass_nblocks<T, OUT>(h->p, L);
     PassCtl c;
     c.done = scb ? &scb->done : nullptr;
     c.ds = kScalarInts;
@@ -214,8 +245,13 @@ int launch_pass(sbp_handle h, int l, const IN& in, const OUT& out, const Scalars
     c.nbx = c.nby = 0;
     c.bt = batch_of(h, l);
     if (nprob >= 0) c.bt.nprob = nprob;   // a subset of the level's problems (pointers pre-offset)
-#define SBP_TILE_CASE(TI_, J_) \
-    if (ti == TI_ && tj == J_) return launch_tile<TI_, J_, T, IN, OUT>(h->p, L.tb, in, out, c, h->st);
+#define SBP_TILE_CASE(TI_, J_, A_)                                                              \
+    if (ti == TI_ && tj == J_ && ta == A_) {                                                    \
+        if constexpr (A_ == 0 || OUT::kAlign)                                                   \
+            return launch_tile<TI_, J_, A_, T, IN, OUT>(h->p, L.tb, in, out, c, h->st);        \
+        else                                                                                    \
+            return SBP_ERR_INVALID_ARG;   /* pass_shape never selects this */                   \
+    }
     SBP_TILE_SHAPES(SBP_TILE_CASE)
 #undef SBP_TILE_CASE
     return SBP_ERR_INVALID_ARG;
@@ -225,13 +261,15 @@ int launch_pass(sbp_handle h, int l, const IN& in, const OUT& out, const Scalars
 template <typename T>
 int launch_rr(sbp_handle h, int l, const T* u, const T* b, T* rc, const Scalars* scb) {
     const Level& L = h->L[l];
-    const int ti = L.ti[prec_index<T>()], tj = L.tj[prec_index<T>()];
+    // the residual + restriction kernel is compiled for the unaligned shapes
+    const bool al = L.ta[prec_index<T>()] != 0;
+    const int ti = al ? 256 : L.ti[prec_index<T>()], tj = al ? 8 : L.tj[prec_index<T>()];
     const Batch bt = batch_of(h, l);
     const int Nc = h->L[l + 1].N;
     const T scale = (T)h->prm.restrict_scale;
     const int* done = scb ? &scb->done : nullptr;
-#define SBP_RR_CASE(TI_, J_) \
-    if (ti == TI_ && tj == J_)   \
+#define SBP_RR_CASE(TI_, J_, A_) \
+    if (ti == TI_ && tj == J_ && A_ == 0)   \
         return launch_resid_restrict<TI_, J_, T>(h->p, L.tb, u, b, rc, Nc, scale, done, kScalarInts, bt, h->st);
     SBP_TILE_SHAPES(SBP_RR_CASE)
 #undef SBP_RR_CASE
@@ -469,7 +507,7 @@ int vcycle_nparts(sbp_handle h, int l = 0) {
     if (h->prm.nu_post == 2 && fuse_level(h, l))
         return fused_post_blocks(L.N, L.fti[prec_index<T>()], L.ftj[prec_index<T>()]);
     if (h->prm.nu_post == 0) return ew_blocks(L.n);
-    return (int)pass_blocks(L.N, L.ti[prec_index<T>()], L.tj[prec_index<T>()]);
+    return (int)pass_nblocks<T, OutSmoothDot<T>>(h->p, L);
 }
 
 void drop_graphs(sbp_handle h) {
@@ -1419,8 +1457,9 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
                 if (!tile_ok(p, h->prm.tile_i, h->prm.tile_j)) { delete h; return SBP_ERR_INVALID_ARG; }
                 L.ti[q] = h->prm.tile_i;
                 L.tj[q] = h->prm.tile_j;
+                L.ta[q] = 0;
             } else {
-                default_tile(L.N, &L.ti[q], &L.tj[q]);
+                default_tile(L.N, p, q, &L.ti[q], &L.tj[q], &L.ta[q]);
             }
             // fused two-sweep blocks: small blocks keep the per-block chain short
             L.fti[q] = 64;
@@ -1657,15 +1696,18 @@ int sbp_get_batch(sbp_handle h, int* nprob) {
 int sbp_set_tiles(sbp_handle h, int level, int dtype, int ti, int tj) {
     if (!h || level < 0 || level >= h->nlev) return SBP_ERR_INVALID_ARG;
     const bool fused = (dtype & SBP_FUSED) != 0;
-    const int d = dtype & ~SBP_FUSED;
-    if ((d != SBP_F64 && d != SBP_F32) || (fused ? !fused_shape_ok(ti, tj, h->p) : !tile_ok(h->p, ti, tj)))
+    const int ta = (dtype & SBP_ALIGNED) ? 1 : 0;
+    const int d = dtype & ~(SBP_FUSED | SBP_ALIGNED);
+    if ((d != SBP_F64 && d != SBP_F32) || (fused && ta) ||
+        (fused ? !fused_shape_ok(ti, tj, h->p) : !tile_ok(h->p, ti, tj, ta)))
         return SBP_ERR_INVALID_ARG;
     Level& L = h->L[level];
     int* pti = fused ? &L.fti[d] : &L.ti[d];
     int* ptj = fused ? &L.ftj[d] : &L.tj[d];
-    if (*pti != ti || *ptj != tj) drop_graphs(h);   // launch shapes change
+    if (*pti != ti || *ptj != tj || (!fused && L.ta[d] != ta)) drop_graphs(h);   // launch shapes change
     *pti = ti;
     *ptj = tj;
+    if (!fused) L.ta[d] = ta;
     return SBP_OK;
 }
 
@@ -1943,7 +1985,7 @@ int sbp_apply_dot_noalloc(sbp_handle h, const double* u, double* Au, double* dot
     HIPCHK(hipGetDevice(&dev));
     if (dev != h->dev) return SBP_ERR_INVALID_ARG;   // the handle's device must be current
     const Level& L = h->L[0];
-    const size_t np = pass_blocks(L.N, L.ti[0], L.tj[0]);
+    const size_t np = pass_nblocks<double, OutStoreDot<double>>(h->p, L);
     // block partials of every problem, then one result per problem
     if (np * (size_t)h->nprob + (size_t)h->nprob > kScratchDoubles) return SBP_ERR_SCRATCH;
     double* s = nullptr;
@@ -2214,6 +2256,7 @@ int coarser(sbp_handle h, sbp_handle* out) {
             for (int q = 0; q < 2; ++q) {
                 c->L[l].ti[q] = h->L[l + 1].ti[q];
                 c->L[l].tj[q] = h->L[l + 1].tj[q];
+                c->L[l].ta[q] = h->L[l + 1].ta[q];
                 c->L[l].fti[q] = h->L[l + 1].fti[q];
                 c->L[l].ftj[q] = h->L[l + 1].ftj[q];
             }

@@ -37,12 +37,29 @@ using ResidFF = OutResid<float, float>;
     X(float, InAxpy<float>, OutPUpd<float>)         \
     X(float, InSmooth0<float>, OutCopy<float>)
 
-// Compiled tile shapes (TI threads along i, J rows per thread).
-#define SBP_TILE_SHAPES(X) X(64, 4) X(64, 8) X(64, 16) X(128, 8) X(128, 16) X(256, 4) X(256, 8) X(256, 16)
-// The subset also compiled for p = 6, 8 and 10 (every level's default shape).
-#define SBP_HI_TILE_SHAPES(X) X(64, 4) X(128, 8) X(256, 8)
+// The combinations whose output policy takes line-aligned stores (kAlign):
+// the ones the aligned tile shapes are compiled for.
+#define SBP_ALIGNED_COMBOS(X)                     \
+    X(double, InPlain<double>, OutStore<double>)    \
+    X(double, InPlain<double>, OutStoreDot<double>) \
+    X(float, InPlain<float>, OutStore<float>)       \
+    X(float, InPlain<float>, OutStoreDot<float>)    \
+    X(float, InAxpy<float>, OutPUpd<float>)
 
-inline int nparts_of(int N, int ti, int tj) { return ((N + ti - 1) / ti) * ((N + tj - 1) / tj); }
+// Compiled tile shapes (TI threads along i, J rows per thread, A = 1:
+// line-aligned stores, the Cols mapping with ALN = 128 bytes / element size).
+#define SBP_TILE_SHAPES(X) X(64, 4, 0) X(64, 8, 0) X(128, 8, 0) X(256, 4, 0) X(256, 8, 0) X(256, 16, 0) X(256, 8, 1) X(256, 16, 1) X(512, 8, 1)
+// The subset also compiled for p = 6, 8 and 10 (every level's default shape).
+#define SBP_HI_TILE_SHAPES(X) X(64, 4, 0) X(128, 8, 0) X(256, 8, 0)
+
+// Elements per 128-byte line for an aligned shape of element size es (0: unaligned).
+inline int aln_of(int ta, int es) { return ta ? 128 / es : 0; }
+
+// Tile columns of a pass over an N-wide grid (Cols::nbx).
+inline int pass_cols(int N, int ti, int aln) { return aln ? (N + aln - 1 + (ti - aln) - 1) / (ti - aln) : (N + ti - 1) / ti; }
+
+// Blocks (and block partials) of a pass over one N x N problem.
+inline int nparts_of(int N, int ti, int tj, int aln = 0) { return pass_cols(N, ti, aln) * ((N + tj - 1) / tj); }
 
 inline int rr_blocks(int Nc, int ti, int tj) { return ((Nc + ti / 2 - 2) / (ti / 2 - 1)) * ((Nc + tj / 2 - 1) / (tj / 2)); }
 
@@ -52,9 +69,10 @@ template <int TI, int J, typename T>
 int launch_resid_restrict(int p, const OpTab& tf, const T* u, const T* b, T* rc, int Nc, T scale, const int* done,
                           int ds, const Batch& bt, hipStream_t st);
 
-// Launches stencil_pass<p, TI, J, T, IN, OUT> over the whole N x N grid of tb
-// for every stacked problem (c.done/ds/partial/bt given; the tile grid is set here).
-template <int TI, int J, typename T, class IN, class OUT>
+// Launches stencil_pass<p, TI, J, ALN, T, IN, OUT> (ALN = aln_of(TA, sizeof(T)))
+// over the whole N x N grid of tb for every stacked problem (c.done/ds/partial/bt
+// given; the tile grid is set here).
+template <int TI, int J, int TA, typename T, class IN, class OUT>
 int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, PassCtl c, hipStream_t st);
 
 // Temporally blocked V-cycle passes (two sweeps per kernel); block shape ti

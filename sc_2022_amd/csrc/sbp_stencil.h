@@ -27,6 +27,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace sbp {
 
 constexpr int kMaxR = 11;  // closure rows
@@ -316,38 +318,59 @@ template <typename T> struct InAxpy {
 // ---------------------------------------------------------------------------
 // Output policies
 // ---------------------------------------------------------------------------
-// acc_t: accumulator type of fused inner products (the element type).
+// A policy stores kNS grid functions per point: vals() gives their values at
+// (i, j) from the operand u, (A u)(i, j) and what pre() loaded there, dst<K>()
+// the K-th destination array (its element type may differ from T: the value
+// is converted on store), and with kDot, dterm() the point's term of the fused
+// inner product (accumulated in T).  Values are pure functions of those
+// inputs, so a pass may form them at the computing thread and store them from
+// another lane (the line-aligned stores below).  kAlign: the policy runs on
+// the line-aligned tiles when its level has them (measured per pass at
+// 4097^2: the apply and the FP32 Krylov pass gain; the smoothing passes, whose
+// held values raise the register count, lose).
 template <typename T> struct OutStore {            // o = s * Au
     T* o; T s;
     __device__ void at(int q, const Batch& bt) { o += q * bt.sf; }
     static constexpr bool kDot = false;
+    static constexpr bool kAlign = true;
+    static constexpr int kNS = 1;
     using Aux = T;
     __device__ Aux pre(const OpTab&, Loc) const { return T(0); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T, T Au, Aux, T&) const { st(o, q, s * Au); }
+    __device__ __forceinline__ void vals(const OpTab&, int, int, T, T Au, const Aux&, T* v) const { v[0] = s * Au; }
+    template <int K> __device__ __forceinline__ T* dst() const { return o; }
 };
 
 template <typename T> struct OutStoreDot {         // o = Au ; acc += u Au
     T* o;
     __device__ void at(int q, const Batch& bt) { o += q * bt.sf; }
     static constexpr bool kDot = true;
+    static constexpr bool kAlign = true;
+    static constexpr int kNS = 1;
     using Aux = T;
     __device__ Aux pre(const OpTab&, Loc) const { return T(0); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T u, T Au, Aux, T& acc) const {
-        st(o, q, Au); acc += u * Au;
-    }
+    __device__ __forceinline__ void vals(const OpTab&, int, int, T, T Au, const Aux&, T* v) const { v[0] = Au; }
+    __device__ __forceinline__ T dterm(T u, T Au, const Aux&) const { return u * Au; }
+    template <int K> __device__ __forceinline__ T* dst() const { return o; }
 };
 
 template <typename T> struct OutWriteDot {         // w = u (materialise the formed operand); o = Au ; acc += u Au
     T* o; T* w;
     __device__ void at(int q, const Batch& bt) { o += q * bt.sf; w += q * bt.sf; }
     static constexpr bool kDot = true;
+    static constexpr bool kAlign = false;
+    static constexpr int kNS = 2;
     using Aux = T;
     __device__ Aux pre(const OpTab&, Loc) const { return T(0); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T u, T Au, Aux, T& acc) const {
-        st(w, q, u); st(o, q, Au); acc += u * Au;
+    __device__ __forceinline__ void vals(const OpTab&, int, int, T u, T Au, const Aux&, T* v) const {
+        v[0] = u; v[1] = Au;
+    }
+    __device__ __forceinline__ T dterm(T u, T Au, const Aux&) const { return u * Au; }
+    template <int K> __device__ __forceinline__ T* dst() const {
+        if constexpr (K == 0) return w;
+        else return o;
     }
 };
 
@@ -355,12 +378,14 @@ template <typename T, typename TO> struct OutResid {  // r = b - Au (optionally 
     const T* b; TO* r;
     __device__ void at(int q, const Batch& bt) { b += q * bt.sf; r += q * bt.sf; }
     static constexpr bool kDot = true;
+    static constexpr bool kAlign = false;
+    static constexpr int kNS = 1;
     using Aux = T;
     __device__ Aux pre(const OpTab&, Loc q) const { return ld(b, q); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T, T Au, Aux bq, T& acc) const {
-        T rv = bq - Au; st(r, q, (TO)rv); acc += rv * rv;
-    }
+    __device__ __forceinline__ void vals(const OpTab&, int, int, T, T Au, const Aux& bq, T* v) const { v[0] = bq - Au; }
+    __device__ __forceinline__ T dterm(T, T Au, const Aux& bq) const { const T rv = bq - Au; return rv * rv; }
+    template <int K> __device__ __forceinline__ TO* dst() const { return r; }
 };
 
 // Smoothing sweep x_new = u + w (b - Au) / diag (damped Jacobi,
@@ -369,14 +394,17 @@ template <typename T> struct OutSmooth {
     const T* b; T* xo; T omega; int use_diag;
     __device__ void at(int q, const Batch& bt) { b += q * bt.sf; xo += q * bt.sf; }
     static constexpr bool kDot = false;
+    static constexpr bool kAlign = false;
+    static constexpr int kNS = 1;
     static constexpr bool kPreRhs = true;   // pre() is b at the point: an input whose raw
                                             // fetch is b (kRawRhs) supplies it without a load
     using Aux = T;
     __device__ Aux pre(const OpTab&, Loc q) const { return ld(b, q); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab& t, int i, int j, Loc q, T u, T Au, Aux bq, T&) const {
-        st(xo, q, u + smooth_corr_at<T, INT, LT>(t, i, j, bq - Au, omega, use_diag));
+    __device__ __forceinline__ void vals(const OpTab& t, int i, int j, T u, T Au, const Aux& bq, T* v) const {
+        v[0] = u + smooth_corr_at<T, INT, LT>(t, i, j, bq - Au, omega, use_diag);
     }
+    template <int K> __device__ __forceinline__ T* dst() const { return xo; }
 };
 
 // The same sweep, also accumulating dotv' x_new (r'z of the last pass of a V-cycle).
@@ -384,6 +412,8 @@ template <typename T> struct OutSmoothDot {
     const T* b; T* xo; T omega; const T* dotv; int use_diag;
     __device__ void at(int q, const Batch& bt) { b += q * bt.sf; xo += q * bt.sf; dotv += q * bt.sf; }
     static constexpr bool kDot = true;
+    static constexpr bool kAlign = false;
+    static constexpr int kNS = 1;
     struct Aux { T b, d; };
     __device__ Aux pre(const OpTab&, Loc q) const {
         Aux a;
@@ -392,10 +422,13 @@ template <typename T> struct OutSmoothDot {
         return a;
     }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab& t, int i, int j, Loc q, T u, T Au, Aux a, T& acc) const {
-        const T xn = u + smooth_corr_at<T, INT, LT>(t, i, j, a.b - Au, omega, use_diag);
-        st(xo, q, xn); acc += a.d * xn;
+    __device__ __forceinline__ void vals(const OpTab& t, int i, int j, T u, T Au, const Aux& a, T* v) const {
+        v[0] = u + smooth_corr_at<T, INT, LT>(t, i, j, a.b - Au, omega, use_diag);
     }
+    // kDotOfValue: the pass hands dterm the stored value x_new in place of Au
+    static constexpr bool kDotOfValue = true;
+    __device__ __forceinline__ T dterm(T, T xn, const Aux& a) const { return a.d * xn; }
+    template <int K> __device__ __forceinline__ T* dst() const { return xo; }
 };
 
 // Fused Krylov step on the search direction: the operand is p_new = z + beta p_old
@@ -405,12 +438,20 @@ template <typename T> struct OutPUpd {
     T* pnew; T* Ap; T* x; const T* pold; const double* alpha; T al;
     __device__ void at(int q, const Batch& bt) { pnew += q * bt.sf; Ap += q * bt.sf; x += q * bt.sf; pold += q * bt.sf; alpha += q * bt.ss; }
     static constexpr bool kDot = true;
+    static constexpr bool kAlign = sizeof(T) == 4;
+    static constexpr int kNS = 3;
     struct Aux { T x, p; };
     __device__ void init() { al = (T)(*alpha); }
     __device__ Aux pre(const OpTab&, Loc q) const { Aux a; a.x = ld(x, q); a.p = ld(pold, q); return a; }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T u, T Au, Aux a, T& acc) const {
-        st(pnew, q, u); st(Ap, q, Au); st(x, q, a.x + al * a.p); acc += u * Au;
+    __device__ __forceinline__ void vals(const OpTab&, int, int, T u, T Au, const Aux& a, T* v) const {
+        v[0] = u; v[1] = Au; v[2] = a.x + al * a.p;
+    }
+    __device__ __forceinline__ T dterm(T u, T Au, const Aux&) const { return u * Au; }
+    template <int K> __device__ __forceinline__ T* dst() const {
+        if constexpr (K == 0) return pnew;
+        else if constexpr (K == 1) return Ap;
+        else return x;
     }
 };
 
@@ -419,11 +460,35 @@ template <typename T> struct OutCopy {
     T* o;
     __device__ void at(int q, const Batch& bt) { o += q * bt.sf; }
     static constexpr bool kDot = false;
+    static constexpr bool kAlign = false;
+    static constexpr int kNS = 1;
     using Aux = T;
     __device__ Aux pre(const OpTab&, Loc) const { return T(0); }
     template <bool INT = false, bool LT = false>
-    __device__ __forceinline__ void emit(const OpTab&, int, int, Loc q, T u, T, Aux, T&) const { st(o, q, u); }
+    __device__ __forceinline__ void vals(const OpTab&, int, int, T u, T, const Aux&, T* v) const { v[0] = u; }
+    template <int K> __device__ __forceinline__ T* dst() const { return o; }
 };
+
+template <class X, class = void> struct dot_of_value { static constexpr bool value = false; };
+template <class X> struct dot_of_value<X, decltype(void(X::kDotOfValue))> { static constexpr bool value = X::kDotOfValue; };
+
+template <int K, class OUT, typename T>
+__device__ __forceinline__ void store_vals(const OUT& out, Loc q, const T* v) {
+    auto* d = out.template dst<K>();
+    using E = std::remove_pointer_t<decltype(d)>;
+    st(d, q, (E)v[K]);
+    if constexpr (K + 1 < OUT::kNS) store_vals<K + 1>(out, q, v);
+}
+
+// Stores every output at its own point and accumulates the inner product.
+template <bool INT, bool LT, class OUT, typename T>
+__device__ __forceinline__ void emit(const OUT& out, const OpTab& t, int i, int j, Loc q, T u, T Au,
+                                     const typename OUT::Aux& a, T& acc) {
+    T v[OUT::kNS];
+    out.template vals<INT, LT>(t, i, j, u, Au, a, v);
+    store_vals<0>(out, q, v);
+    if constexpr (OUT::kDot) acc += out.dterm(u, dot_of_value<OUT>::value ? v[0] : Au, a);
+}
 
 // ---------------------------------------------------------------------------
 // Block reduction helpers (wave64 shuffles, then LDS across waves)
@@ -602,13 +667,112 @@ __device__ __forceinline__ T m_closure_tile(const OpTab& t, const IN& in, int i,
     return s;
 }
 
+// ---------------------------------------------------------------------------
+// Line-aligned stores (ALN > 0)
+// ---------------------------------------------------------------------------
+// ALN is the number of elements in a 128-byte line.  A block of TI threads
+// computes the TI columns cbase .. cbase + TI - 1, cbase = bx TO - ALN with
+// TO = TI - ALN, but on row j stores only the TO columns that start at
+// bx TO - s_j, s_j being the offset of the destination's element (0, j)
+// within its line.  TO is a multiple of ALN, so every block's stored range
+// starts on a line boundary, the ranges of one row's blocks tile it exactly,
+// and each line is written whole by one wave.  (With N = 2^k + 1 every row
+// starts mid-line; a line written in two parts from blocks on different XCDs
+// is merged by a read-modify-write in memory, which costs an unaligned pass
+// about a tenth of its bandwidth.)  A value travels from the thread that
+// computed it to the storing lane through the LDS tile; the stores are
+// non-temporal.  ALN = 0: each thread stores its own column (cbase = bx TI).
+template <int TI, int ALN> struct Cols {
+    static constexpr int TO = ALN ? TI - ALN : TI;   // columns stored per row
+    __host__ __device__ static int base(int bx) { return ALN ? bx * TO - ALN : bx * TI; }
+    __host__ __device__ static int nbx(int N) { return ALN ? (N + ALN - 1 + TO - 1) / TO : (N + TI - 1) / TI; }
+};
+
+template <int ALN, typename E>
+__device__ __forceinline__ int line_shift(const E* row) {
+    return (int)(((uintptr_t)row / sizeof(E)) & (uintptr_t)(ALN - 1));
+}
+
+// Whether the calling thread's column belongs to its block's stored range on
+// row j (by the first destination's lines): the fused inner products count
+// each point once, at the block that stores it.
+template <int TI, int ALN, class OUT>
+__device__ __forceinline__ bool owns_col(const OUT& out, int j, int N) {
+    if constexpr (ALN == 0) {
+        return true;
+    } else {
+        const int s = line_shift<ALN>(out.template dst<0>() + (size_t)j * N);
+        return (unsigned)((int)threadIdx.x - (ALN - s)) < (unsigned)(TI - ALN);
+    }
+}
+
+// Stores destination K (and, recursively, the ones after it) of rows j0 ..
+// j0 + J - 1: every thread puts its own column's value vv[r][K] in the tile,
+// then the block stores each row's TO columns from cbase + ALN - s_j as
+// 16-byte vectors (non-temporal; single elements at the grid's edges).
+template <typename E> struct Vec16 {
+    static constexpr int V = 16 / (int)sizeof(E);
+    typedef E type __attribute__((ext_vector_type(16 / sizeof(E))));
+};
+
+template <int K, int TI, int J, int ALN, int NS, int TW, typename T, class OUT>
+__device__ __forceinline__ void aligned_store(const OUT& out, int N, T (*tile)[TW], int cbase, int j0,
+                                              const T (*vv)[NS]) {
+    constexpr int TO = TI - ALN;
+    const int tid = threadIdx.x;
+    __syncthreads();   // the tile's previous contents have been read
+#pragma unroll
+    for (int r = 0; r < J; ++r) tile[r][tid] = vv[r][K];
+    __syncthreads();
+    auto* d = out.template dst<K>();
+    using E = std::remove_pointer_t<decltype(d)>;
+    constexpr int V = Vec16<E>::V;
+    static_assert(TO % V == 0, "stored range is whole vectors");
+    constexpr int NV = TO / V;        // vectors per row
+#pragma unroll
+    for (int k0 = 0; k0 < J * NV; k0 += TI) {
+        const int k = k0 + tid;
+        if (k < J * NV) {
+            const int r = k / NV, e = k - r * NV;
+            const int j = j0 + r;
+            if (j < N) {
+                E* row = d + (size_t)j * N;
+                const int s = line_shift<ALN>(row);
+                const int col = cbase + ALN - s + e * V;
+                const T* src = &tile[r][ALN - s + e * V];
+                if (col >= 0 && col + V <= N) {
+                    typename Vec16<E>::type v;
+#pragma unroll
+                    for (int m = 0; m < V; ++m) v[m] = (E)src[m];
+                    __builtin_nontemporal_store(v, reinterpret_cast<typename Vec16<E>::type*>(row + col));
+                } else {
+#pragma unroll
+                    for (int m = 0; m < V; ++m)
+                        if (col + m >= 0 && col + m < N) row[col + m] = (E)src[m];
+                }
+            }
+        }
+    }
+    if constexpr (K + 1 < NS) aligned_store<K + 1, TI, J, ALN>(out, N, tile, cbase, j0, vv);
+}
+
+// The values of one point for the aligned path: vals() into vv, and with
+// kDot the point's inner-product term when the block stores the point.
+template <int TI, int ALN, bool INT, bool LT, class OUT, typename T>
+__device__ __forceinline__ void point_vals(const OUT& out, const OpTab& t, int i, int j, T u, T Au,
+                                           const typename OUT::Aux& a, T* vv, T& acc) {
+    out.template vals<INT, LT>(t, i, j, u, Au, a, vv);
+    if constexpr (OUT::kDot)
+        if (owns_col<TI, ALN>(out, j, t.N)) acc += out.dterm(u, dot_of_value<OUT>::value ? vv[0] : Au, a);
+}
+
 // Tile whose rows (and their halo) lie inside the grid, away from the closure
 // rows in j.  With ICHK = false the columns are interior too: straight-line
 // code with no bounds tests and a constant inverse diagonal (the common
 // case).  With ICHK = true (the first/last column of tiles) each thread tests
 // its own column: out-of-grid threads idle, threads on the i-closure rows take
 // the closure sum and the H weight; everything else stays on the fast path.
-template <int P, int TI, int J, bool ICHK, typename T, bool LT = false, class IN, class OUT>
+template <int P, int TI, int J, int ALN, bool ICHK, typename T, bool LT = false, class IN, class OUT>
 __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const OpTab& t, int i0, int j0,
                                            T (*tile)[tile_w<P, TI>()]) {
     constexpr int H = Stencil<P>::H;
@@ -619,7 +783,7 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
     const int tid = threadIdx.x;
     const int i = i0 + tid;
     const int N = t.N;
-    const bool act = !ICHK || i < N;
+    const bool act = !ICHK || (i >= 0 && i < N);
     T w[W];
     // the smoothing right-hand side at the tile's points is already in the
     // input's raw window: no second load of it
@@ -670,32 +834,36 @@ __device__ __forceinline__ T pass_interior(const IN& in, const OUT& out, const O
     }
     __syncthreads();
     T acc = T(0);
-    if (!act) return acc;
-    const bool iint = !ICHK || ((i >= t.R) && (i < N - t.R));
-    const T hwi = ICHK ? hw_of<T, LT>(t, i) : T(1);
+    T vv[ALN ? J : 1][OUT::kNS];
+    if (act) {
+        const bool iint = !ICHK || ((i >= t.R) && (i < N - t.R));
+        const T hwi = ICHK ? hw_of<T, LT>(t, i) : T(1);
 #pragma unroll
-    for (int r = 0; r < J; ++r) {
-        const T uc = w[r + H];
-        T kp = Stencil<P>::template c<T>(1) * ((w[r + H - 1] - uc) + (w[r + H + 1] - uc));
+        for (int r = 0; r < J; ++r) {
+            const T uc = w[r + H];
+            T kp = Stencil<P>::template c<T>(1) * ((w[r + H - 1] - uc) + (w[r + H + 1] - uc));
 #pragma unroll
-        for (int s = 2; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
-        T mp;
-        if (iint) {
-            const T* row = &tile[r][tid + HL];
-            mp = Stencil<P>::template c<T>(1) * ((row[-1] - uc) + (row[1] - uc));
+            for (int s = 2; s <= H; ++s) kp += Stencil<P>::template c<T>(s) * ((w[r + H - s] - uc) + (w[r + H + s] - uc));
+            T mp;
+            if (iint) {
+                const T* row = &tile[r][tid + HL];
+                mp = Stencil<P>::template c<T>(1) * ((row[-1] - uc) + (row[1] - uc));
 #pragma unroll
-            for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
-        } else {
-            mp = m_closure_tile<P, TI, T, LT>(t, in, i, j0 + r, i0, tile[r], uc);
+                for (int s = 2; s <= H; ++s) mp += Stencil<P>::template c<T>(s) * ((row[-s] - uc) + (row[s] - uc));
+            } else {
+                mp = m_closure_tile<P, TI, T, LT>(t, in, i, j0 + r, i0, tile[r], uc);
+            }
+            const T Au = ICHK ? hwi * kp + mp : kp + mp;   // hw_j = 1 on interior rows
+            if constexpr (ALN == 0) emit<INT, LT>(out, t, i, j0 + r, loc(i, j0 + r, N), uc, Au, aux[r], acc);
+            else point_vals<TI, ALN, INT, LT>(out, t, i, j0 + r, uc, Au, aux[r], vv[r], acc);
         }
-        const T Au = ICHK ? hwi * kp + mp : kp + mp;   // hw_j = 1 on interior rows
-        out.template emit<INT, LT>(t, i, j0 + r, loc(i, j0 + r, N), uc, Au, aux[r], acc);
     }
+    if constexpr (ALN > 0) aligned_store<0, TI, J, ALN>(out, N, tile, i0, j0, vv);
     return acc;
 }
 
 // General tile: bounds tests, face closures and H weights.
-template <int P, int TI, int J, typename T, bool LT = false, class IN, class OUT>
+template <int P, int TI, int J, int ALN, typename T, bool LT = false, class IN, class OUT>
 __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const OpTab& t, int i0, int j0,
                                           T (*tile)[tile_w<P, TI>()]) {
     constexpr int H = Stencil<P>::H;
@@ -703,7 +871,7 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
     const int tid = threadIdx.x;
     const int N = t.N;
     const int i = i0 + tid;
-    const bool act = i < N;
+    const bool act = i >= 0 && i < N;
     T w[W];
     if constexpr (multi_raw<IN, T>()) {
 #pragma unroll
@@ -755,6 +923,7 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
     }
     __syncthreads();
     T acc = T(0);
+    T vv[ALN ? J : 1][OUT::kNS];
     if (act) {
         const bool iint = (i >= t.R) && (i < N - t.R);
         const T hwi = hw_of<T, LT>(t, i);
@@ -781,22 +950,27 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
                 mp = m_closure<P, T, LT>(t, in, i, j, uc);
             }
             const T Au = (iint ? kp : hwi * kp) + (jint ? mp : hw_of<T, LT>(t, j) * mp);
-            out.template emit<false, LT>(t, i, j, loc(i, j, N), uc, Au, aux[r], acc);
+            if constexpr (ALN == 0) emit<false, LT>(out, t, i, j, loc(i, j, N), uc, Au, aux[r], acc);
+            else point_vals<TI, ALN, false, LT>(out, t, i, j, uc, Au, aux[r], vv[r], acc);
         }
     }
+    if constexpr (ALN > 0) aligned_store<0, TI, J, ALN>(out, N, tile, i0, j0, vv);
     return acc;
 }
 
-template <int P, int TI, int J, typename T, class IN, class OUT>
+// ALN > 0: line-aligned stores (Cols); the blocks are then taken in launch
+// order, so the whole chip sweeps one band of rows at a time.  ALN = 0: the
+// block index is remapped so that each XCD walks its own band of tiles.
+template <int P, int TI, int J, int ALN, typename T, class IN, class OUT>
 __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, PassCtl ctl) {
     constexpr int H = Stencil<P>::H;
     const int nbt = ctl.nbx * ctl.nby;
-    const int ball = xcd_remap((int)blockIdx.x, nbt * ctl.bt.nprob);
+    const int ball = ALN ? (int)blockIdx.x : xcd_remap((int)blockIdx.x, nbt * ctl.bt.nprob);
     const int q = ball / nbt, b = ball - q * nbt;
     if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
     const int bx = b % ctl.nbx, by = b / ctl.nbx;
     const int N = t.N;
-    const int i0 = bx * TI, j0 = by * J;
+    const int i0 = Cols<TI, ALN>::base(bx), j0 = by * J;
     in.at(q, ctl.bt);
     out.at(q, ctl.bt);
     in.init();
@@ -807,15 +981,15 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
     const bool iin = i0 - H >= t.R && i0 + TI + H <= N - t.R;
     T acc;
     if (jin && iin) {
-        acc = pass_interior<P, TI, J, false, T>(in, out, t, i0, j0, tile);
+        acc = pass_interior<P, TI, J, ALN, false, T>(in, out, t, i0, j0, tile);
     } else {
         // Face tiles look the closure tables up per thread: stage them in LDS
         // first (one load per thread) so those lookups cost LDS, not global,
         // latency -- on the small multigrid levels every tile is a face tile.
         __shared__ OpTab ltab;
         stage_table<P, TI>(t, &ltab);
-        if (jin) acc = pass_interior<P, TI, J, true, T, true>(in, out, ltab, i0, j0, tile);
-        else acc = pass_general<P, TI, J, T, true>(in, out, ltab, i0, j0, tile);
+        if (jin) acc = pass_interior<P, TI, J, ALN, true, T, true>(in, out, ltab, i0, j0, tile);
+        else acc = pass_general<P, TI, J, ALN, T, true>(in, out, ltab, i0, j0, tile);
     }
     if constexpr (OUT::kDot) {
         if (ctl.partial) {

@@ -1,26 +1,32 @@
 // sbp_tile.hip -- instantiates the stencil pass for one tile shape
-// (compile with -DSBP_TI=<threads along i> -DSBP_TJ=<rows per thread>; with
-// -DSBP_HI=1 also for the orders p = 6, 8 and 10).
+// (compile with -DSBP_TI=<threads along i> -DSBP_TJ=<rows per thread>
+// -DSBP_TA=<1: line-aligned stores>; with -DSBP_HI=1 also for the orders
+// p = 6, 8 and 10).  The residual + restriction kernel is instantiated for
+// the unaligned shapes only.
 #include "sbp_launch.h"
 
 #ifndef SBP_TI
-#error "define SBP_TI and SBP_TJ"
+#error "define SBP_TI, SBP_TJ and SBP_TA"
+#endif
+#ifndef SBP_TA
+#define SBP_TA 0
 #endif
 
 namespace sbp {
 
-template <int TI, int J, typename T, class IN, class OUT>
+template <int TI, int J, int TA, typename T, class IN, class OUT>
 int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, PassCtl c, hipStream_t st) {
-    c.nbx = (tb.N + TI - 1) / TI;
+    constexpr int ALN = TA ? 128 / (int)sizeof(T) : 0;
+    c.nbx = Cols<TI, ALN>::nbx(tb.N);
     c.nby = (tb.N + J - 1) / J;
     const dim3 g(c.nbx * c.nby * c.bt.nprob);
     switch (p) {
-    case 2: hipLaunchKernelGGL((stencil_pass<2, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
-    case 4: hipLaunchKernelGGL((stencil_pass<4, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
+    case 2: hipLaunchKernelGGL((stencil_pass<2, TI, J, ALN, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
+    case 4: hipLaunchKernelGGL((stencil_pass<4, TI, J, ALN, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
 #if SBP_HI
-    case 6: hipLaunchKernelGGL((stencil_pass<6, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
-    case 8: hipLaunchKernelGGL((stencil_pass<8, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
-    case 10: hipLaunchKernelGGL((stencil_pass<10, TI, J, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
+    case 6: hipLaunchKernelGGL((stencil_pass<6, TI, J, ALN, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
+    case 8: hipLaunchKernelGGL((stencil_pass<8, TI, J, ALN, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
+    case 10: hipLaunchKernelGGL((stencil_pass<10, TI, J, ALN, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
 #endif
     default: return 3;   // order not compiled for this tile shape
     }
@@ -51,13 +57,20 @@ int launch_resid_restrict(int p, const OpTab& tf, const T* u, const T* b, T* rc,
     return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
+#if !SBP_TA
 template int launch_resid_restrict<SBP_TI, SBP_TJ, double>(int, const OpTab&, const double*, const double*, double*,
                                                            int, double, const int*, int, const Batch&, hipStream_t);
 template int launch_resid_restrict<SBP_TI, SBP_TJ, float>(int, const OpTab&, const float*, const float*, float*, int,
                                                           float, const int*, int, const Batch&, hipStream_t);
+#endif
 
 #define SBP_INST(T, IN, OUT) \
-    template int launch_tile<SBP_TI, SBP_TJ, T, IN, OUT>(int, const OpTab&, const IN&, const OUT&, PassCtl, hipStream_t);
+    template int launch_tile<SBP_TI, SBP_TJ, SBP_TA, T, IN, OUT>(int, const OpTab&, const IN&, const OUT&, PassCtl, \
+                                                                 hipStream_t);
+#if SBP_TA
+SBP_ALIGNED_COMBOS(SBP_INST)
+#else
 SBP_STENCIL_COMBOS(SBP_INST)
+#endif
 
 }  // namespace sbp

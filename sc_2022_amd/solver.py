@@ -104,13 +104,14 @@ class SbpPoisson:
             N = (N + 1) // 2
         return N
 
-    def set_tiles(self, level, ti, tj, dtype=None, fused=False):
+    def set_tiles(self, level, ti, tj, dtype=None, fused=False, aligned=False):
         """Tile shape for `level` (fused: the block shape of the level's
-        two-sweep kernels); both precisions unless dtype is given."""
+        two-sweep kernels; aligned: the line-aligned variant of a stencil
+        shape); both precisions unless dtype is given."""
         dts = [capi.SBP_F64, capi.SBP_F32] if dtype is None else [_DT[dtype]]
+        flags = (capi.SBP_FUSED if fused else 0) | (capi.SBP_ALIGNED if aligned else 0)
         for d in dts:
-            capi.check(self.lib.sbp_set_tiles(self._h, level, d | (capi.SBP_FUSED if fused else 0), ti, tj),
-                       "sbp_set_tiles")
+            capi.check(self.lib.sbp_set_tiles(self._h, level, d | flags, ti, tj), "sbp_set_tiles")
 
     def apply_tuned_tiles(self, cache=None):
         """Applies the autotuner's choices (tiles_gfx950.json) to every level."""
@@ -121,7 +122,7 @@ class SbpPoisson:
             for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
                 t = autotune.lookup(self.p, tag, Nl, c)
                 if t:
-                    self.set_tiles(level, t[0], t[1], dt)
+                    self.set_tiles(level, t[0], t[1], dt, aligned=len(t) > 2 and bool(t[2]))
 
     # -- operator ----------------------------------------------------------
     def apply(self, u, out=None, sign=1, level=0):

@@ -73,13 +73,18 @@ def test_apply_golden(SP, golden_dir, p):
 
 
 @pytest.mark.parametrize("p", [2, 4])
-@pytest.mark.parametrize("tiles", [(64, 4), (64, 8), (64, 16), (128, 8), (128, 16), (256, 4), (256, 8), (256, 16)])
+@pytest.mark.parametrize("tiles", [(64, 4, 0), (64, 8, 0), (128, 8, 0), (256, 4, 0), (256, 8, 0), (256, 16, 0),
+                                   (256, 8, 1), (256, 16, 1), (512, 8, 1)])
 def test_apply_every_tile_shape(SP, p, tiles):
     level = 7
     N = 2 ** level + 1
-    h = SP(N, p, tile_i=tiles[0], tile_j=tiles[1])
+    h = SP(N, p)
+    h.set_tiles(0, tiles[0], tiles[1], aligned=bool(tiles[2]))
     u = np.random.default_rng(11).standard_normal(N * N)
     assert relmax(h.apply(T(u)).cpu().numpy(), A_of(level, p) @ u) < 1e-13
+    uf = u.astype(np.float32)
+    e32 = relmax(h.apply(T(uf, torch.float32)).cpu().double().numpy(), A_of(level, p) @ uf.astype(np.float64))
+    assert e32 < 2e-5
 
 
 @pytest.mark.parametrize("p", [2, 4])
@@ -106,18 +111,20 @@ def test_apply_seeded_random_cases(SP):
         p = int(rng.choice([2, 4, 6, 8, 10]))
         N = int(rng.integers(cmin[p], 401))
         shapes = compiled_shapes(p)
-        ti, tj = shapes[int(rng.integers(len(shapes)))]
+        ti, tj, ta = shapes[int(rng.integers(len(shapes)))]
         sign = int(rng.choice([1, -1]))
         hw, m, k = O.one_d_factors(p, N)
         A = sp.kron(sp.csr_matrix(k), sp.diags(hw)) + sp.kron(sp.diags(hw), sp.csr_matrix(m))
-        h = SP(N, p, tile_i=ti, tile_j=tj)
+        h = SP(N, p, tile_i=ti, tile_j=tj) if not ta else SP(N, p)
+        if ta:
+            h.set_tiles(0, ti, tj, aligned=True)
         u = rng.standard_normal(N * N)
         ref = sign * (A @ u)
         e64 = relmax(h.apply(T(u), sign=sign).cpu().numpy(), ref)
         uf = u.astype(np.float32)
         e32 = relmax(h.apply(T(uf, torch.float32), sign=sign).cpu().double().numpy(),
                      sign * (A @ uf.astype(np.float64)))
-        assert e64 < 1e-13 and e32 < 2e-5, (case, p, N, ti, tj, sign, e64, e32)
+        assert e64 < 1e-13 and e32 < 2e-5, (case, p, N, ti, tj, ta, sign, e64, e32)
         h.close()
 
 
