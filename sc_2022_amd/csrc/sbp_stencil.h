@@ -958,14 +958,16 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
     return acc;
 }
 
-// ALN > 0: line-aligned stores (Cols); the blocks are then taken in launch
-// order, so the whole chip sweeps one band of rows at a time.  ALN = 0: the
-// block index is remapped so that each XCD walks its own band of tiles.
+// The block index is remapped so that each XCD walks its own band of tiles
+// (workgroups go to the 8 XCDs round-robin), where the tiles above and below
+// a tile find the halo rows they share in the XCD's L2 -- except for FP32
+// line-aligned tiles, which measured faster taken in launch order (the whole
+// chip sweeping one band of rows at a time).
 template <int P, int TI, int J, int ALN, typename T, class IN, class OUT>
 __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, PassCtl ctl) {
     constexpr int H = Stencil<P>::H;
     const int nbt = ctl.nbx * ctl.nby;
-    const int ball = ALN ? (int)blockIdx.x : xcd_remap((int)blockIdx.x, nbt * ctl.bt.nprob);
+    const int ball = (ALN && sizeof(T) < 8) ? (int)blockIdx.x : xcd_remap((int)blockIdx.x, nbt * ctl.bt.nprob);
     const int q = ball / nbt, b = ball - q * nbt;
     if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
     const int bx = b % ctl.nbx, by = b / ctl.nbx;
