@@ -776,7 +776,9 @@ int poll_ahead(const sbp_handle h) {
         int k = kPollAhead;
         if (s.iters > 0 && s.resnorm > 0.0 && s.resnorm < s.rnorm0 && s.abstol > 0.0) {
             const double lrho = std::log(s.resnorm / s.rnorm0) / s.iters;   // < 0
-            const double est = std::ceil(std::log(s.abstol / s.resnorm) / lrho);
+            // nearest iteration: the rate of a multigrid-preconditioned solve
+            // improves as it runs, so rounding up would overshoot
+            const double est = std::ceil(std::log(s.abstol / s.resnorm) / lrho - 0.5);
             k = est < 1.0 ? 1 : (est > kPollAhead ? kPollAhead : (int)est);
         }
         need = std::max(need, k);
@@ -790,7 +792,7 @@ int poll_ahead(const sbp_handle h) {
 // the identity.  The host polls now and then (poll_ahead) and stops issuing
 // once every problem is done.  Leaves the scalar blocks in h->sc_host.
 template <typename T>
-int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use_mg) {
+int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use_mg, bool final_poll = true) {
     hipStream_t st = h->st;
     const size_t n = h->L[0].n;
     const T* z0 = v.r;
@@ -866,7 +868,7 @@ int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use
                            (const T*)v.p[1], (const Scalars*)sc, n);
     }
     HIPCHK(hipGetLastError());
-    return poll(h, sc);
+    return final_poll ? poll(h, sc) : SBP_OK;
 }
 
 // Per-problem tolerances to the device (through pinned staging, stream-ordered).
@@ -968,6 +970,7 @@ __global__ void ir_init(Scalars* so, const double* abstol, int nprob) {
         so[q].iters = 0;
         so[q].dprev = 0.0;
         so[q].est = 0.0;
+        so[q].inner = 0;
         so[q].t0 = t;
         so[q].t1 = 0;
     }
@@ -982,6 +985,7 @@ __global__ __launch_bounds__(NT) void ir_update(double* __restrict__ x, const fl
                                                Scalars* sob, const Scalars* sib, double* partial) {
     const int q = blockIdx.y;
     if (sob[q].done) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) sob[q].inner += sib[q].iters;
     const int ist = sib[q].status;
     if (ist == 5 || ist == 6) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1071,8 +1075,10 @@ int mixed_solve(sbp_handle h, const double* b, double* x, const double* abstol, 
                                h->prm.inner_rtol, h->prm.inner_maxiter);
         }
         HIPCHK(hipGetLastError());
+        // The first residual is not checked from the host: a problem that is
+        // already converged makes the inner solve's kernels fall through.
         bool all = false;
-        STCHK(poll(h, h->sc, &all));
+        if (o > 0 || o == max_outer) STCHK(poll(h, h->sc, &all));
         if (all || o == max_outer) break;
         // Inner FP32 MG-PCG on A d = r from d = 0.
         if (h->ragged) {
@@ -1087,8 +1093,9 @@ int mixed_solve(sbp_handle h, const double* b, double* x, const double* abstol, 
                                (const Scalars*)h->sc_in);
         }
         HIPCHK(hipGetLastError());
-        STCHK(pcg_loop<float>(h, kvecs<float>(h), h->sc_in, h->prm.inner_maxiter, true));
-        for (int q = 0; q < h->nprob; ++q) inner_tot[q] += h->sc_host[q].iters;
+        // (no host wait after the inner solve: its iterations are added up on
+        // the device by ir_update)
+        STCHK(pcg_loop<float>(h, kvecs<float>(h), h->sc_in, h->prm.inner_maxiter, true, false));
         // FP64 update and the error estimate from the correction norms.
         if (h->ragged) {
             for (int l = 0; l < h->nlev; ++l) {
@@ -1109,7 +1116,9 @@ int mixed_solve(sbp_handle h, const double* b, double* x, const double* abstol, 
         }
         HIPCHK(hipGetLastError());
     }
-    return poll(h, h->sc);
+    STCHK(poll(h, h->sc));
+    for (int q = 0; q < h->nprob; ++q) inner_tot[q] = h->sc_host[q].inner;
+    return SBP_OK;
 }
 
 // Status of one finished refinement.

@@ -27,6 +27,7 @@ struct Scalars {
     int iters;             // iterations completed (advanced on the device)
     int done;              // convergence mask: kernels of a done problem fall through
     int status;
+    int inner;             // refinement: inner iterations over the corrections so far
 };
 constexpr int kScalarInts = (int)(sizeof(Scalars) / sizeof(int));
 constexpr int kScalarDoubles = (int)(sizeof(Scalars) / sizeof(double));
