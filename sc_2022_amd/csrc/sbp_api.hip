@@ -177,7 +177,7 @@ inline bool tile_ok(int p, int ti, int tj, int ta = 0) {
 // of an order-p handle; the autotuner's table (tiles_gfx950.json) refines it.
 inline void default_tile(int N, int p, int q, int* ti, int* tj, int* ta) {
     *ta = 0;
-    if (p <= 4 && N >= 4097 && (q == 0 || N < 8193)) { *ti = q == 0 ? 512 : 256; *tj = 8; *ta = 1; }   // line-aligned apply
+    if (p <= 4 && N >= 4097 && (q == 0 || N < 8193)) { *ti = 256; *tj = 8; *ta = 1; }   // line-aligned apply
     else if (N >= 1025) { *ti = 256; *tj = 8; }
     else if (N >= 129) { *ti = 128; *tj = 8; }
     else { *ti = 64; *tj = 4; }

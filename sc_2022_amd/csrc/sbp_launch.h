@@ -44,7 +44,8 @@ using ResidFF = OutResid<float, float>;
     X(double, InPlain<double>, OutStoreDot<double>) \
     X(float, InPlain<float>, OutStore<float>)       \
     X(float, InPlain<float>, OutStoreDot<float>)    \
-    X(float, InAxpy<float>, OutPUpd<float>)
+    X(float, InAxpy<float>, OutPUpd<float>)       \
+    X(double, InAxpy<double>, OutPUpd<double>)
 
 // Compiled tile shapes (TI threads along i, J rows per thread, A = 1:
 // line-aligned stores, the Cols mapping with ALN = 128 bytes / element size).

@@ -326,7 +326,7 @@ template <typename T> struct InAxpy {
 // inputs, so a pass may form them at the computing thread and store them from
 // another lane (the line-aligned stores below).  kAlign: the policy runs on
 // the line-aligned tiles when its level has them (measured per pass at
-// 4097^2: the apply and the FP32 Krylov pass gain; the smoothing passes, whose
+// 4097^2: the apply and the Krylov pass gain; the smoothing passes, whose
 // held values raise the register count, lose).
 template <typename T> struct OutStore {            // o = s * Au
     T* o; T s;
@@ -438,7 +438,7 @@ template <typename T> struct OutPUpd {
     T* pnew; T* Ap; T* x; const T* pold; const double* alpha; T al;
     __device__ void at(int q, const Batch& bt) { pnew += q * bt.sf; Ap += q * bt.sf; x += q * bt.sf; pold += q * bt.sf; alpha += q * bt.ss; }
     static constexpr bool kDot = true;
-    static constexpr bool kAlign = sizeof(T) == 4;
+    static constexpr bool kAlign = true;
     static constexpr int kNS = 3;
     struct Aux { T x, p; };
     __device__ void init() { al = (T)(*alpha); }
