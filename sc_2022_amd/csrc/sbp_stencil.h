@@ -581,8 +581,34 @@ struct PassCtl {
     int ds;              // ints between problems' done flags
     double* partial;     // nullable: problem q's block partials at partial[q * nbx * nby + tile]
     int nbx, nby;        // tile grid of one problem
+    int bx0, bx1, by0, by1;   // tiles [bx0, bx1) x [by0, by1) are interior (face_first)
     Batch bt;
 };
+
+// Order of a problem's tiles: the face tiles first, then the interior ones.
+// A face tile (closure rows or columns, bounds tests, staged tables) lives
+// longer than an interior one; dispatched last, the face rows at the end of
+// the grid set the kernel's tail.  Dispatched first, their latency hides
+// behind the interior tiles.  b: the block's index within its problem; k:
+// the index among the interior tiles (the caller remaps it for XCD bands).
+// Returns true for a face tile with (bx, by) set, false for interior tile k.
+__device__ __forceinline__ bool face_first(const PassCtl& c, int b, int& bx, int& by, int& k) {
+    const int nix = c.bx1 - c.bx0, niy = c.by1 - c.by0;
+    const int nface = c.nbx * c.nby - nix * niy;
+    if (b >= nface) { k = b - nface; return false; }
+    const int fr = (c.nby - niy) * c.nbx;   // tiles of the face rows
+    if (b < fr) {
+        const int rb = b / c.nbx;
+        bx = b - rb * c.nbx;
+        by = rb < c.by0 ? rb : c.by1 + (rb - c.by0);
+    } else {
+        const int f = b - fr, ncol = c.nbx - nix;
+        const int r = f / ncol, col = f - r * ncol;
+        by = c.by0 + r;
+        bx = col < c.bx0 ? col : c.bx1 + (col - c.bx0);
+    }
+    return true;
+}
 
 // Copies the device table into LDS (blockDim.x threads, one word each).
 // The parts of the table an order-P kernel reads (the scalars, hw, the first
@@ -958,19 +984,26 @@ __device__ __forceinline__ T pass_general(const IN& in, const OUT& out, const Op
     return acc;
 }
 
-// The block index is remapped so that each XCD walks its own band of tiles
-// (workgroups go to the 8 XCDs round-robin), where the tiles above and below
-// a tile find the halo rows they share in the XCD's L2 -- except for FP32
-// line-aligned tiles, which measured faster taken in launch order (the whole
-// chip sweeping one band of rows at a time).
+// Face tiles go first (face_first); the interior tiles are remapped so that
+// each XCD walks its own band of them (workgroups go to the 8 XCDs
+// round-robin), where the tiles above and below a tile find the halo rows
+// they share in the XCD's L2 -- except for FP32 line-aligned tiles, which
+// measured faster taken in launch order (the whole chip sweeping one band of
+// rows at a time).
 template <int P, int TI, int J, int ALN, typename T, class IN, class OUT>
 __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, PassCtl ctl) {
     constexpr int H = Stencil<P>::H;
     const int nbt = ctl.nbx * ctl.nby;
-    const int ball = (ALN && sizeof(T) < 8) ? (int)blockIdx.x : xcd_remap((int)blockIdx.x, nbt * ctl.bt.nprob);
-    const int q = ball / nbt, b = ball - q * nbt;
+    const int q = (int)blockIdx.x / nbt;
     if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
-    const int bx = b % ctl.nbx, by = b / ctl.nbx;
+    int bx, by, k;
+    if (!face_first(ctl, (int)blockIdx.x - q * nbt, bx, by, k)) {
+        const int nix = ctl.bx1 - ctl.bx0;
+        if (!(ALN && sizeof(T) < 8)) k = xcd_remap(k, nix * (ctl.by1 - ctl.by0));
+        by = ctl.by0 + k / nix;
+        bx = ctl.bx0 + (k - (by - ctl.by0) * nix);
+    }
+    const int b = by * ctl.nbx + bx;
     const int N = t.N;
     const int i0 = Cols<TI, ALN>::base(bx), j0 = by * J;
     in.at(q, ctl.bt);

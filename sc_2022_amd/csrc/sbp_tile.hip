@@ -3,6 +3,8 @@
 // -DSBP_TA=<1: line-aligned stores>; with -DSBP_HI=1 also for the orders
 // p = 6, 8 and 10).  The residual + restriction kernel is instantiated for
 // the unaligned shapes only.
+#include <algorithm>
+
 #include "sbp_launch.h"
 
 #ifndef SBP_TI
@@ -19,6 +21,18 @@ int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, PassCtl c,
     constexpr int ALN = TA ? 128 / (int)sizeof(T) : 0;
     c.nbx = Cols<TI, ALN>::nbx(tb.N);
     c.nby = (tb.N + J - 1) / J;
+    {   // interior tiles: every point they load (halo included) lies off the closure rows
+        constexpr int TO = Cols<TI, ALN>::TO;
+        const int N = tb.N, R = tb.R, H = tb.H;
+        const int ylast = N - R - H - J, xlast = N - R - H - TI + ALN;
+        c.by0 = (R + H + J - 1) / J;
+        c.by1 = ylast >= 0 ? std::min(c.nby, ylast / J + 1) : 0;
+        c.bx0 = (R + H + ALN + TO - 1) / TO;
+        c.bx1 = xlast >= 0 ? std::min(c.nbx, xlast / TO + 1) : 0;
+        if (c.by1 < c.by0) c.by1 = c.by0 = 0;
+        if (c.bx1 < c.bx0) c.bx1 = c.bx0 = 0;
+        if (c.by1 == c.by0 || c.bx1 == c.bx0) c.bx0 = c.bx1 = c.by0 = c.by1 = 0;
+    }
     const dim3 g(c.nbx * c.nby * c.bt.nprob);
     switch (p) {
     case 2: hipLaunchKernelGGL((stencil_pass<2, TI, J, ALN, T, IN, OUT>), g, dim3(TI), 0, st, in, out, tb, c); break;
