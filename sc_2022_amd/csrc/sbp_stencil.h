@@ -581,7 +581,7 @@ struct PassCtl {
     int ds;              // ints between problems' done flags
     double* partial;     // nullable: problem q's block partials at partial[q * nbx * nby + tile]
     int nbx, nby;        // tile grid of one problem
-    int bx0, bx1, by0, by1;   // tiles [bx0, bx1) x [by0, by1) are interior (face_first)
+    int bx0, bx1, by0, by1;   // tiles [bx0, bx1) x [by0, by1) are interior (tile_order)
     Batch bt;
 };
 
@@ -589,25 +589,32 @@ struct PassCtl {
 // A face tile (closure rows or columns, bounds tests, staged tables) lives
 // longer than an interior one; dispatched last, the face rows at the end of
 // the grid set the kernel's tail.  Dispatched first, their latency hides
-// behind the interior tiles.  b: the block's index within its problem; k:
-// the index among the interior tiles (the caller remaps it for XCD bands).
-// Returns true for a face tile with (bx, by) set, false for interior tile k.
-__device__ __forceinline__ bool face_first(const PassCtl& c, int b, int& bx, int& by, int& k) {
-    const int nix = c.bx1 - c.bx0, niy = c.by1 - c.by0;
-    const int nface = c.nbx * c.nby - nix * niy;
-    if (b >= nface) { k = b - nface; return false; }
-    const int fr = (c.nby - niy) * c.nbx;   // tiles of the face rows
-    if (b < fr) {
-        const int rb = b / c.nbx;
-        bx = b - rb * c.nbx;
-        by = rb < c.by0 ? rb : c.by1 + (rb - c.by0);
-    } else {
-        const int f = b - fr, ncol = c.nbx - nix;
-        const int r = f / ncol, col = f - r * ncol;
-        by = c.by0 + r;
-        bx = col < c.bx0 ? col : c.bx1 + (col - c.bx0);
+// behind the interior tiles.  b: the block's index within its problem on an
+// nbx x nby tile grid whose tiles [bx0, bx1) x [by0, by1) are interior; the
+// interior tiles follow in row-major order, remapped (remap) so that each
+// XCD walks its own band of them.
+__device__ __forceinline__ void tile_order(int nbx, int nby, int bx0, int bx1, int by0, int by1, int b, bool remap,
+                                           int& bx, int& by) {
+    const int nix = bx1 - bx0, niy = by1 - by0;
+    const int nface = nbx * nby - nix * niy;
+    if (b >= nface) {
+        int k = b - nface;
+        if (remap) k = xcd_remap(k, nix * niy);
+        by = by0 + k / nix;
+        bx = bx0 + (k - (by - by0) * nix);
+        return;
     }
-    return true;
+    const int fr = (nby - niy) * nbx;   // tiles of the face rows
+    if (b < fr) {
+        const int rb = b / nbx;
+        bx = b - rb * nbx;
+        by = rb < by0 ? rb : by1 + (rb - by0);
+    } else {
+        const int f = b - fr, ncol = nbx - nix;
+        const int r = f / ncol, col = f - r * ncol;
+        by = by0 + r;
+        bx = col < bx0 ? col : bx1 + (col - bx0);
+    }
 }
 
 // Copies the device table into LDS (blockDim.x threads, one word each).
@@ -996,13 +1003,9 @@ __global__ __launch_bounds__(TI) void stencil_pass(IN in, OUT out, OpTab t, Pass
     const int nbt = ctl.nbx * ctl.nby;
     const int q = (int)blockIdx.x / nbt;
     if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
-    int bx, by, k;
-    if (!face_first(ctl, (int)blockIdx.x - q * nbt, bx, by, k)) {
-        const int nix = ctl.bx1 - ctl.bx0;
-        if (!(ALN && sizeof(T) < 8)) k = xcd_remap(k, nix * (ctl.by1 - ctl.by0));
-        by = ctl.by0 + k / nix;
-        bx = ctl.bx0 + (k - (by - ctl.by0) * nix);
-    }
+    int bx, by;
+    tile_order(ctl.nbx, ctl.nby, ctl.bx0, ctl.bx1, ctl.by0, ctl.by1, (int)blockIdx.x - q * nbt,
+               !(ALN && sizeof(T) < 8), bx, by);
     const int b = by * ctl.nbx + bx;
     const int N = t.N;
     const int i0 = Cols<TI, ALN>::base(bx), j0 = by * J;
@@ -1069,6 +1072,7 @@ struct RestrictCtl {
     int ds;
     int Nc;
     int nbx, nby;      // block grid over coarse points: TI/2-1 columns x J/2 rows per block
+    int bx0, bx1, by0, by1;   // blocks [bx0, bx1) x [by0, by1) are interior (tile_order)
     Batch bt;          // sf: fine elements, sc: coarse elements per problem
 };
 
@@ -1087,13 +1091,13 @@ __global__ __launch_bounds__(TI) void resid_restrict(const T* __restrict__ u, co
     constexpr int CI = TI / 2 - 1;       // coarse columns per block
     constexpr int CJ = J / 2;            // coarse rows per block
     const int nb = ctl.nbx * ctl.nby;
-    const int ball = xcd_remap((int)blockIdx.x, nb * ctl.bt.nprob);
-    const int q = ball / nb, blk = ball - q * nb;
+    const int q = (int)blockIdx.x / nb;
     if (ctl.done && ctl.done[(size_t)q * ctl.ds]) return;
     u += q * ctl.bt.sf;
     if (b) b += q * ctl.bt.sf;
     rc += q * ctl.bt.sc;
-    const int bx = blk % ctl.nbx, by = blk / ctl.nbx;
+    int bx, by;
+    tile_order(ctl.nbx, ctl.nby, ctl.bx0, ctl.bx1, ctl.by0, ctl.by1, (int)blockIdx.x - q * nb, true, bx, by);
     const int N = t.N, Nc = ctl.Nc;
     const int Ic0 = bx * CI, Jc0 = by * CJ;
     const int tid = threadIdx.x;

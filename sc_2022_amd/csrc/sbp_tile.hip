@@ -29,9 +29,7 @@ int launch_tile(int p, const OpTab& tb, const IN& in, const OUT& out, PassCtl c,
         c.by1 = ylast >= 0 ? std::min(c.nby, ylast / J + 1) : 0;
         c.bx0 = (R + H + ALN + TO - 1) / TO;
         c.bx1 = xlast >= 0 ? std::min(c.nbx, xlast / TO + 1) : 0;
-        if (c.by1 < c.by0) c.by1 = c.by0 = 0;
-        if (c.bx1 < c.bx0) c.bx1 = c.bx0 = 0;
-        if (c.by1 == c.by0 || c.bx1 == c.bx0) c.bx0 = c.bx1 = c.by0 = c.by1 = 0;
+        if (c.by1 <= c.by0 || c.bx1 <= c.bx0) c.bx0 = c.bx1 = c.by0 = c.by1 = 0;
     }
     const dim3 g(c.nbx * c.nby * c.bt.nprob);
     switch (p) {
@@ -57,6 +55,16 @@ int launch_resid_restrict(int p, const OpTab& tf, const T* u, const T* b, T* rc,
     c.nbx = (Nc + TI / 2 - 2) / (TI / 2 - 1);
     c.nby = (Nc + J / 2 - 1) / (J / 2);
     c.bt = bt;
+    {   // interior blocks: the fine points they load (halo included) lie off the closure rows
+        constexpr int CI = TI / 2 - 1;
+        const int N = tf.N, R = tf.R, H = tf.H;
+        const int ylast = N - R - H - J, xlast = N - R - H - TI + 1;
+        c.by0 = (R + H + 1 + J - 1) / J;
+        c.by1 = ylast >= 0 ? std::min(c.nby, ylast / J + 1) : 0;
+        c.bx0 = (R + H + 1 + 2 * CI - 1) / (2 * CI);
+        c.bx1 = xlast >= 0 ? std::min(c.nbx, xlast / (2 * CI) + 1) : 0;
+        if (c.by1 <= c.by0 || c.bx1 <= c.bx0) c.bx0 = c.bx1 = c.by0 = c.by1 = 0;
+    }
     const dim3 g(c.nbx * c.nby * bt.nprob);
     switch (p) {
     case 2: hipLaunchKernelGGL((resid_restrict<2, TI, J, T>), g, dim3(TI), 0, st, u, b, rc, scale, tf, c); break;
