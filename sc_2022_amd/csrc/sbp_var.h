@@ -596,7 +596,8 @@ __device__ __forceinline__ double var_tile(const IN& in, const OUT& out, const V
 // operand IN and output OUT (one problem; a launch on a converged solve,
 // sc->done, returns at once).  kb / mb: the table's coefficient bands (as
 // kernel arguments their loads are known to be global-memory loads).  Blocks: kVarThreads x kVarRows tiles, nbx
-// along i, the block index remapped so each XCD walks a contiguous band.
+// along i, the face tiles first and the interior ones remapped so each XCD
+// walks a contiguous band (tile_order).
 template <int P, typename T, class IN, class OUT>
 __global__ __launch_bounds__(kVarThreads) void var_pass(IN in, OUT out, const VarTab* __restrict__ tp,
                                                        const double* __restrict__ kb, const double* __restrict__ mb,
@@ -609,8 +610,16 @@ __global__ __launch_bounds__(kVarThreads) void var_pass(IN in, OUT out, const Va
     __shared__ double kr[2 * H + 1][J];
     const int N = tp->N, R = tp->R;
     const int tid = threadIdx.x;
-    const int blk = xcd_remap((int)blockIdx.x, nbx * nby);
-    const int i0 = (blk % nbx) * kVarThreads, j0 = (blk / nbx) * J;
+    int bx, by;
+    {   // face tiles first, then the interior ones in per-XCD bands (tile_order)
+        const int ylast = N - R - H - J, xlast = N - R - H - kVarThreads;
+        int by0 = (R + H + J - 1) / J, by1 = ylast >= 0 ? min(nby, ylast / J + 1) : 0;
+        int bx0 = (R + H + kVarThreads - 1) / kVarThreads, bx1 = xlast >= 0 ? min(nbx, xlast / kVarThreads + 1) : 0;
+        if (by1 <= by0 || bx1 <= bx0) bx0 = bx1 = by0 = by1 = 0;
+        tile_order(nbx, nby, bx0, bx1, by0, by1, (int)blockIdx.x, true, bx, by);
+    }
+    const int blk = by * nbx + bx;
+    const int i0 = bx * kVarThreads, j0 = by * J;
     if constexpr (has_init<IN>::value) in.init();
     if constexpr (has_init<OUT>::value) out.init();
     const bool jin = j0 - H >= R && j0 + J + H <= N - R;
@@ -677,8 +686,15 @@ __global__ __launch_bounds__(kVarThreads) void var_resid_restrict(const T* __res
     double (*rt)[TI + 2 * H] = ut;        // the residual reuses the operand tile after a barrier
     const int N = tp->N, R = tp->R;
     const int tid = threadIdx.x;
-    const int blk = xcd_remap((int)blockIdx.x, nbx * nby);
-    const int Ic0 = (blk % nbx) * CI, Jc0 = (blk / nbx) * CJ;
+    int bx, by;
+    {   // face blocks first (tile_order); interior blocks: the fine points they load lie off the closure rows
+        const int ylast = N - R - H - kVarRows, xlast = N - R - H - TI + 1;
+        int by0 = (R + H + 1 + kVarRows - 1) / kVarRows, by1 = ylast >= 0 ? min(nby, ylast / kVarRows + 1) : 0;
+        int bx0 = (R + H + 1 + 2 * CI - 1) / (2 * CI), bx1 = xlast >= 0 ? min(nbx, xlast / (2 * CI) + 1) : 0;
+        if (by1 <= by0 || bx1 <= bx0) bx0 = bx1 = by0 = by1 = 0;
+        tile_order(nbx, nby, bx0, bx1, by0, by1, (int)blockIdx.x, true, bx, by);
+    }
+    const int Ic0 = bx * CI, Jc0 = by * CJ;
     const int fi0 = 2 * Ic0 - 1, fj0 = 2 * Jc0 - 1;   // fine origin (may be -1)
     const int fi = fi0 + tid;
     const bool jin = fj0 - H >= R && fj0 + JR + H <= N - R;
