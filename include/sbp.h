@@ -16,7 +16,7 @@
  * wider boundary closures and A is symmetric indefinite, so the apply is
  * exact but the CG-type solvers may stop with SBP_ERR_BREAKDOWN.  The
  * two-sweep fused multigrid kernels are compiled for p = 2, 4 and (64- and
- * 128-wide blocks) p = 6, the single-workgroup bottom kernel for p = 2, 4; other
+ * 128-wide blocks) p = 6, the single-workgroup bottom kernels for p = 2, 4; other
  * orders use the separate passes.
  *
  * Conventions: every pointer argument that names a grid function is a device
@@ -89,9 +89,10 @@ typedef struct {
     double ir_rtol;          /* mixed precision: also stop when the correction-based error
                                 estimate ||d_k|| min(1, ||d_k||/||d_k-1||) <= ir_rtol ||x||
                                 (0 = residual test only, the default) */
-    int bottom_n;            /* multigrid levels of side <= bottom_n (at most 65) run as one
-                                single-workgroup kernel per problem with the level data in
-                                LDS, when the coarsest side is <= 17 (default 0 = off) */
+    int bottom_n;            /* multigrid levels of side <= bottom_n (at most 65) above the
+                                coarsest run as two single-workgroup kernels per problem
+                                (down and up leg, level data in LDS) around the coarse
+                                solve, for at most three such levels (default 0 = off) */
     int fuse_n;              /* with nu_pre = 2 / nu_post = 2, levels whose points over all the
                                 handle's problems number at most fuse_n^2 run their pre-smoothing
                                 + residual + restriction, and their prolongation + post-smoothing,

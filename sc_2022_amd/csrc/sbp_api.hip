@@ -320,14 +320,30 @@ inline bool fuse_level(const sbp_handle h, int l) {
     return pts <= (long long)h->prm.fuse_n * h->prm.fuse_n;
 }
 
-// Levels bottom_l0 .. nlev-1 of the V-cycle in one launch (one workgroup per problem).
+template <typename T>
+int vcycle_level(sbp_handle h, int l, const T* b, T* out, const T* dotv, const Scalars* scb, double* partial);
+
+// Levels bottom_l0 .. nlev-2 of the V-cycle in two single-workgroup launches
+// per problem, the down and the up leg, around the coarsest level's solve.
 template <typename T>
 int launch_bottom(sbp_handle h, const T* b, T* out, const T* dotv, const Scalars* scb, double* partial) {
-    const int l0 = h->bottom_l0;
+    const int l0 = h->bottom_l0, lc = h->nlev - 1;
+    const int pi = prec_index<T>();
     BottomCtl c{};
-    c.nlev = h->nlev - l0;
-    for (int k = 0; k < c.nlev; ++k) c.tab[k] = h->L[l0 + k].dtab;
-    c.Ainv = h->Ainv;
+    c.nlev = lc - l0;
+    for (int k = 0; k < c.nlev; ++k) {
+        const Level& L = h->L[l0 + k];
+        c.tab[k] = L.dtab;
+        c.side[k] = L.N;
+        c.x2[k] = L.xa[pi];
+        c.b[k] = k == 0 ? (void*)b : L.b[pi];
+        c.sz[k] = (long long)L.n;
+    }
+    const Level& Lc = h->L[lc];
+    c.side[c.nlev] = Lc.N;
+    c.b[c.nlev] = Lc.b[pi];
+    c.sz[c.nlev] = (long long)Lc.n;
+    c.ec = Lc.xo[pi];
     c.nu1 = h->prm.nu_pre;
     c.nu2 = h->prm.nu_post;
     c.use_diag = h->prm.smoother == 1 ? 0 : 1;
@@ -335,15 +351,20 @@ int launch_bottom(sbp_handle h, const T* b, T* out, const T* dotv, const Scalars
     c.scale = h->prm.restrict_scale;
     c.done = scb ? &scb->done : nullptr;
     c.ds = kScalarInts;
-    c.sb = (long long)h->L[l0].n;
     c.ss = 1;
     c.partial = partial;
     if (h->p == 2)
-        hipLaunchKernelGGL((vcycle_bottom<2, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, b, out, dotv, c);
+        hipLaunchKernelGGL((vcycle_bottom_down<2, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, c);
     else if (h->p == 4)
-        hipLaunchKernelGGL((vcycle_bottom<4, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, b, out, dotv, c);
+        hipLaunchKernelGGL((vcycle_bottom_down<4, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, c);
     else
         return SBP_ERR_INVALID_ORDER;   // not compiled for p > 4 (never selected for them)
+    HIPCHK(hipGetLastError());
+    STCHK(vcycle_level<T>(h, lc, (const T*)Lc.b[pi], (T*)Lc.xo[pi], nullptr, scb, nullptr));
+    if (h->p == 2)
+        hipLaunchKernelGGL((vcycle_bottom_up<2, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, out, dotv, c);
+    else
+        hipLaunchKernelGGL((vcycle_bottom_up<4, T>), dim3(h->nprob), dim3(kBottomThreads), 0, h->st, out, dotv, c);
     HIPCHK(hipGetLastError());
     return SBP_OK;
 }
@@ -1451,13 +1472,27 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
             h->j0[l] = l == 0 ? 0 : nprob;
         }
     }
-    // Bottom kernel: from the first level of side <= bottom_n down, when the
-    // coarsest system is small enough for one workgroup's direct solve.
-    if (h->mg && p <= 4 && !h->ragged && !h->prm.drop_coarse_rhs && h->L[nlev - 1].n <= (size_t)kBottomMaxCoarse) {
+    // Bottom kernels: the levels from the first of side <= bottom_n down to
+    // the coarsest, when they fit the kernels' LDS pool and prefetch registers.
+    if (h->mg && p <= 4 && !h->ragged && !h->prm.drop_coarse_rhs) {
         const int bn = std::min(h->prm.bottom_n, kBottomN);
-        for (int l = 0; l < nlev; ++l)
-            if (h->L[l].N <= bn) { h->bottom_l0 = l; break; }
-        if (h->bottom_l0 >= 0 && nlev - h->bottom_l0 > kBottomMaxLevels) h->bottom_l0 = -1;
+        int l0 = -1;
+        for (int l = 0; l < nlev - 1; ++l)
+            if (h->L[l].N <= bn) { l0 = l; break; }
+        if (l0 >= 0 && nlev - 1 - l0 <= kBottomMaxLevels) {
+            const long long nc = (long long)h->L[nlev - 1].n;
+            long long down = 0, up = nc, pre = nc;
+            bool ok = (long long)h->L[l0].n <= (long long)kBottomDownPre * kBottomThreads;
+            for (int l = l0; l < nlev - 1; ++l) {
+                const long long n = (long long)h->L[l].n;
+                ok = ok && h->L[l].N > 2 * h->L[l].tb.R;
+                down += 3 * n;
+                up += (l == l0 ? 3 : 4) * n;
+                pre += 2 * n;
+            }
+            if (ok && down <= kBottomPool && up <= kBottomPool && pre <= (long long)kBottomUpPre * kBottomThreads)
+                h->bottom_l0 = l0;
+        }
     }
     for (int l = 0; l < nlev; ++l) {
         Level& L = h->L[l];

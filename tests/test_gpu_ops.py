@@ -272,6 +272,35 @@ def test_vcycle_sweep_counts(SP, nu, bottom):
     assert relmax(z, O.vcycle(hier, r, nu[0], nu[1])) < 1e-12
 
 
+@pytest.mark.parametrize("level", [6, 9])
+@pytest.mark.parametrize("p", [2, 4])
+def test_bottom_kernels_match_fused_levels(SP, p, level):
+    """The bottom kernels (the levels of side <= 65 in one workgroup per
+    problem, down and up leg around the coarse solve) evaluate every point as
+    the fused two-sweep kernels do: V-cycles with bottom_n = 65 and 0 agree to
+    rounding, FP64 and FP32, also over a stacked batch of three problems."""
+    N = 2 ** level + 1
+    hf = SP(N, p, bottom_n=0)
+    hb = SP(N, p, bottom_n=65)
+    g = torch.Generator(device=DEV).manual_seed(level * 10 + p)
+    r = torch.randn(N * N, dtype=torch.float64, device=DEV, generator=g)
+    for dt, tol in ((torch.float64, 1e-14), (torch.float32, 1e-6)):
+        zf = hf.vcycle(r.to(dt))
+        zb = hb.vcycle(r.to(dt))
+        assert float((zb - zf).abs().max()) <= tol * float(zf.abs().max())
+    hf.close(); hb.close()
+    from sc_2022_amd.batch import SbpBatch
+    bb = SbpBatch(N, p, 3, device=DEV, bottom_n=65)
+    rs = torch.randn(3 * N * N, dtype=torch.float64, device=DEV, generator=g)
+    hf = SP(N, p, bottom_n=0)
+    for dt, tol in ((torch.float64, 1e-14), (torch.float32, 1e-6)):
+        zs = bb.vcycle(rs.to(dt))
+        for q in range(3):
+            zf = hf.vcycle(rs[q * N * N:(q + 1) * N * N].to(dt))
+            assert float((zs[q * N * N:(q + 1) * N * N] - zf).abs().max()) <= tol * float(zf.abs().max())
+    bb.close(); hf.close()
+
+
 @pytest.mark.parametrize("p", [2, 4])
 def test_csr_assembly_and_spmv(SP, p):
     level = 5

@@ -401,7 +401,7 @@ def test_mgpcg_bottom_kernel_matches_tiled_levels(SP, p):
     A, b, u, _ = problem(level, p, data="fourier")
     tol = 1e-11 * np.linalg.norm(b)
     h0 = SP(N, p, bottom_n=0)
-    h1 = SP(N, p)
+    h1 = SP(N, p, bottom_n=65)
     x0, r0 = h0.mgpcg(T(b), abstol=tol)
     x1, r1 = h1.mgpcg(T(b), abstol=tol)
     assert r0.converged and r1.converged
