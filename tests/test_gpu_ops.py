@@ -242,8 +242,9 @@ def test_smoother_matches_oracle(SP, p):
 @pytest.mark.parametrize("bottom", [0, 65])
 @pytest.mark.parametrize("p", [2, 4])
 def test_coarse_solve_and_vcycle(SP, p, bottom):
-    """65^2: with bottom_n = 65 the whole V-cycle is the single-workgroup
-    bottom kernel; with 0 every level runs the tiled passes."""
+    """65^2: with bottom_n = 65 the levels 65 and 33 run in the
+    single-workgroup bottom kernels around the 17^2 coarse solve; with 0 they
+    run the fused two-sweep kernels."""
     level = 6
     N = 2 ** level + 1
     h = SP(N, p, bottom_n=bottom)
@@ -330,7 +331,7 @@ def test_copy_kernel(SP):
 def test_vcycle_large_against_oracle(SP, p, bottom):
     """Level 9 (513^2): every V-cycle kernel runs its interior-tile path
     (including the fused residual + restriction); the levels at or below
-    bottom_n run in the single-workgroup bottom kernel."""
+    bottom_n run in the single-workgroup bottom kernels."""
     level = 9
     N = 2 ** level + 1
     h = SP(N, p, bottom_n=bottom)
