@@ -68,14 +68,6 @@ template <typename T> struct InLds {
     __device__ __forceinline__ T load(const OpTab& t, int i, int j) const { return u[gidx(i, j, t.N)]; }
 };
 
-// m / d for 0 <= m < 2^20, 1 <= d <= 2^10 (a float estimate, corrected by one).
-__device__ __forceinline__ int div_small(int m, int d, float rd) {
-    int r = (int)((float)m * rd);
-    if (r * d > m) --r;
-    else if ((r + 1) * d <= m) ++r;
-    return r;
-}
-
 // Calls f(std::true_type, i, j) for the interior points of an N x N level
 // (i, j in [R, N - R)), then f(std::false_type, i, j) for the 4 R (N - R)
 // points of the closure frame, each set spread over the workgroup.

@@ -86,6 +86,71 @@ template <int R_> struct FusedHaloR {
 };
 template <int P> struct FusedHalo : FusedHaloR<Stencil<P>::R> {};
 
+// Calls f(std::true_type, i, j, k) for the points of the region
+// [x0, x0 + W) x [y0, y0 + H) (k = (j - y0) W + (i - x0)) that lie off the
+// closure rows of both axes, f(std::false_type, i, j, k) for its other grid
+// points and z(k) for its points outside the grid, each set spread over the
+// NT threads in turn.  A wave then runs either the plain cross or the
+// closure form: the closure points of a face block (a few columns or rows of
+// its region) fill a wave or two, instead of one lane of every wave making
+// all its lanes wait for the closure form.
+template <int NT, int W, int H, class F, class Z>
+__device__ __forceinline__ void region_points(int N, int R, int x0, int y0, F&& f, Z&& z) {
+    const int tid = threadIdx.x;
+    const int gx0 = x0 > 0 ? x0 : 0, gx1 = x0 + W < N ? x0 + W : N;
+    const int gy0 = y0 > 0 ? y0 : 0, gy1 = y0 + H < N ? y0 + H : N;
+    const int gw = gx1 > gx0 ? gx1 - gx0 : 0, gh = gy1 > gy0 ? gy1 - gy0 : 0;
+    // the interior part [cx0, cx1) x [cy0, cy1), clamped into the grid part
+    auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+    const int cx0 = clampi(R, gx0, gx0 + gw), cx1 = clampi(N - R, cx0, gx0 + gw);
+    const int cy0 = clampi(R, gy0, gy0 + gh), cy1 = clampi(N - R, cy0, gy0 + gh);
+    const int wi = cx1 - cx0, hi = cy1 - cy0;
+    if (wi > 0 && hi > 0) {
+        const float rw = 1.0f / (float)wi;
+        for (int m = tid; m < wi * hi; m += NT) {
+            const int r = div_small(m, wi, rw);
+            const int i = cx0 + (m - r * wi), j = cy0 + r;
+            f(std::true_type{}, i, j, (j - y0) * W + (i - x0));
+        }
+    }
+    // the frame: rows [gy0, cy0) and [cy1, gy1) over the grid part's columns,
+    // then columns [gx0, cx0) and [cx1, gx1) of the rows [cy0, cy1)
+    const int left = cx0 - gx0, right = gx0 + gw - cx1;
+    const int n1 = (cy0 - gy0) * gw, n2 = n1 + (gy0 + gh - cy1) * gw;
+    const int n3 = n2 + hi * left, n4 = n3 + hi * right;
+    if (n4 > 0) {
+        const float rgw = gw > 0 ? 1.0f / (float)gw : 0.0f;
+        const float rl = left > 0 ? 1.0f / (float)left : 0.0f, rr = right > 0 ? 1.0f / (float)right : 0.0f;
+        for (int m = tid; m < n4; m += NT) {
+            int i, j;
+            if (m < n2) {
+                const int mm = m < n1 ? m : m - n1;
+                const int r = div_small(mm, gw, rgw);
+                i = gx0 + (mm - r * gw);
+                j = (m < n1 ? gy0 : cy1) + r;
+            } else if (m < n3) {
+                const int mm = m - n2;
+                const int r = div_small(mm, left, rl);
+                i = gx0 + (mm - r * left);
+                j = cy0 + r;
+            } else {
+                const int mm = m - n3;
+                const int r = div_small(mm, right, rr);
+                i = cx1 + (mm - r * right);
+                j = cy0 + r;
+            }
+            f(std::false_type{}, i, j, (j - y0) * W + (i - x0));
+        }
+    }
+    if (gw < W || gh < H) {
+        for (int k = tid; k < W * H; k += NT) {
+            const int r = k / W, c = k - r * W;
+            const int i = x0 + c, j = y0 + r;
+            if (i < 0 || i >= N || j < 0 || j >= N) z(k);
+        }
+    }
+}
+
 // Scale of the smoothing correction w (b - Au): w / diag (damped Jacobi; the
 // inverse diagonal is the constant idint away from the faces) or w (Richardson).
 template <typename T, bool LT, bool INNER>
@@ -118,12 +183,14 @@ struct FusedOpC {
     __device__ __forceinline__ void store() {
         if constexpr (LT) st.store(lt);
     }
+    // INT: the caller knows (i, j) is off the closure rows of both axes.
+    template <bool INT = false>
     __device__ __forceinline__ T corr(int i, int j, T r, T omega, int use_diag) const {
-        return smooth_corr<T, LT, INNER>(t(), i, j, r, omega, use_diag);
+        return smooth_corr<T, LT, INNER || INT>(t(), i, j, r, omega, use_diag);
     }
-    template <int W>
+    template <int W, bool INT = false>
     __device__ __forceinline__ T au(const T* a, int i0, int j0, int i, int j) const {
-        return au_grid<P, T, LT, INNER, W>(t(), a, i0, j0, i, j);
+        return au_grid<P, T, LT, INNER || INT, W>(t(), a, i0, j0, i, j);
     }
 };
 
@@ -189,16 +256,21 @@ __device__ __forceinline__ bool sr_body(OP& op, int N, const TG* __restrict__ b,
     }
     __syncthreads();
     // B: x2 = x1 + w D^-1 (b - A x1)
-    for (int k = tid; k < BW * BH; k += NT) {
-        const int r = k / BW, c = k - r * BW;
-        const int i = bi0 + c, j = bj0 + r;
-        T x2 = T(0);
-        if (OP::kInner || (i >= 0 && i < N && j >= 0 && j < N)) {
+    if constexpr (OP::kInner) {
+        for (int k = tid; k < BW * BH; k += NT) {
+            const int r = k / BW, c = k - r * BW;
+            const int i = bi0 + c, j = bj0 + r;
             const int ka = (r + HA - HB) * AW + (c + HA - HB);
             const T Au = op.template au<AW>(U, ai0, aj0, i, j);
-            x2 = U[ka] + op.corr(i, j, Bt[ka] - Au, omega, use_diag);
+            V[k] = U[ka] + op.corr(i, j, Bt[ka] - Au, omega, use_diag);
         }
-        V[k] = x2;
+    } else {
+        region_points<NT, BW, BH>(N, OP::R, bi0, bj0, [&](auto in_, int i, int j, int k) {
+            constexpr bool INT = decltype(in_)::value;
+            const int ka = (j - aj0) * AW + (i - ai0);
+            const T Au = op.template au<AW, INT>(U, ai0, aj0, i, j);
+            V[k] = U[ka] + op.template corr<INT>(i, j, Bt[ka] - Au, omega, use_diag);
+        }, [&](int k) { V[k] = T(0); });
     }
     __syncthreads();
     // x2 at the fine points this block owns
@@ -215,13 +287,17 @@ __device__ __forceinline__ bool sr_body(OP& op, int N, const TG* __restrict__ b,
     }
     // C: residual r = b - A x2 into U's storage (x1 is no longer read)
     T* R = U;
-    for (int k = tid; k < TI * JR; k += NT) {
-        const int r = k / TI, c = k - r * TI;
-        const int i = ci0 + c, j = cj0 + r;
-        T rv = T(0);
-        if (OP::kInner || (i >= 0 && i < N && j >= 0 && j < N))
-            rv = Bt[(r + HA) * AW + (c + HA)] - op.template au<BW>(V, bi0, bj0, i, j);
-        R[k] = rv;
+    if constexpr (OP::kInner) {
+        for (int k = tid; k < TI * JR; k += NT) {
+            const int r = k / TI, c = k - r * TI;
+            const int i = ci0 + c, j = cj0 + r;
+            R[k] = Bt[(r + HA) * AW + (c + HA)] - op.template au<BW>(V, bi0, bj0, i, j);
+        }
+    } else {
+        region_points<NT, TI, JR>(N, OP::R, ci0, cj0, [&](auto in_, int i, int j, int k) {
+            constexpr bool INT = decltype(in_)::value;
+            R[k] = Bt[(j - aj0) * AW + (i - ai0)] - op.template au<BW, INT>(V, bi0, bj0, i, j);
+        }, [&](int k) { R[k] = T(0); });
     }
     __syncthreads();
     // Full weighting [1 2 1; 2 4 2; 1 2 1] / 16 (times scale) of the owned coarse points.
@@ -370,15 +446,19 @@ __device__ __forceinline__ bool ps_body(OP& op, int N, const TG* __restrict__ x,
     }
     __syncthreads();
     // B: v1 = u + w D^-1 (b - A u)
-    for (int k = tid; k < BW * BH; k += NT) {
-        const int r = k / BW, c = k - r * BW;
-        const int i = bi0 + c, j = bj0 + r;
-        T v1 = T(0);
-        if (OP::kInner || (i >= 0 && i < N && j >= 0 && j < N)) {
+    if constexpr (OP::kInner) {
+        for (int k = tid; k < BW * BH; k += NT) {
+            const int r = k / BW, c = k - r * BW;
+            const int i = bi0 + c, j = bj0 + r;
             const T Au = op.template au<AW>(U, ai0, aj0, i, j);
-            v1 = U[(r + HA - HB) * AW + (c + HA - HB)] + op.corr(i, j, Bt[k] - Au, omega, use_diag);
+            V[k] = U[(r + HA - HB) * AW + (c + HA - HB)] + op.corr(i, j, Bt[k] - Au, omega, use_diag);
         }
-        V[k] = v1;
+    } else {
+        region_points<NT, BW, BH>(N, OP::R, bi0, bj0, [&](auto in_, int i, int j, int k) {
+            constexpr bool INT = decltype(in_)::value;
+            const T Au = op.template au<AW, INT>(U, ai0, aj0, i, j);
+            V[k] = U[(j - aj0) * AW + (i - ai0)] + op.template corr<INT>(i, j, Bt[k] - Au, omega, use_diag);
+        }, [&](int k) { V[k] = T(0); });
     }
     __syncthreads();
     // C: out = v1 + w D^-1 (b - A v1) on the owned points
@@ -387,15 +467,25 @@ __device__ __forceinline__ bool ps_body(OP& op, int N, const TG* __restrict__ x,
         const int oi0 = bx * TI, oj0 = by * J;
         const int oi1 = oi0 + TI < N ? oi0 + TI : N, oj1 = oj0 + J < N ? oj0 + J : N;
         const int ow = oi1 - oi0;
-        for (int k = tid; k < ow * (oj1 - oj0); k += NT) {
-            const int r = k / ow, c = k - r * ow;
-            const int i = oi0 + c, j = oj0 + r;
+        auto point = [&](auto in_, int i, int j) {
+            constexpr bool INT = decltype(in_)::value;
             const int kb = (j - bj0) * BW + (i - bi0);
-            const T Av = op.template au<BW>(V, bi0, bj0, i, j);
-            const T o = V[kb] + op.corr(i, j, Bt[kb] - Av, omega, use_diag);
+            const T Av = op.template au<BW, INT>(V, bi0, bj0, i, j);
+            const T o = V[kb] + op.template corr<INT>(i, j, Bt[kb] - Av, omega, use_diag);
             const size_t g = gidx(i, j, N);
             out[g] = (TG)o;
             if (dotv) acc += (T)dotv[g] * o;
+        };
+        if constexpr (OP::kInner) {
+            for (int k = tid; k < ow * (oj1 - oj0); k += NT) {
+                const int r = k / ow, c = k - r * ow;
+                point(std::false_type{}, oi0 + c, oj0 + r);
+            }
+        } else {
+            // the owned points (the block's whole TI x J tile, cut at the grid's
+            // far faces: the points past them go to z, which ignores them)
+            region_points<NT, TI, J>(N, OP::R, oi0, oj0,
+                                     [&](auto in_, int i, int j, int) { point(in_, i, j); }, [](int) {});
         }
     }
     return true;

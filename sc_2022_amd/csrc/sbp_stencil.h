@@ -122,6 +122,16 @@ template <> struct Stencil<10> {
 
 __device__ __forceinline__ size_t gidx(int i, int j, int N) { return (size_t)j * (size_t)N + (size_t)i; }
 
+// m / d for 0 <= m < 2^20, 1 <= d <= 2^10, given rd = 1.0f / d (a float
+// estimate, corrected by one): the index arithmetic of point loops whose
+// widths are known only at run time.
+__device__ __forceinline__ int div_small(int m, int d, float rd) {
+    int r = (int)((float)m * rd);
+    if (r * d > m) --r;
+    else if ((r + 1) * d <= m) ++r;
+    return r;
+}
+
 // Storage location of grid point (i, j): the offset of its row and its column
 // (0 <= i < N where it is accessed).  The policies read and write grid
 // functions only through ld/st, so the addressing form is chosen here.

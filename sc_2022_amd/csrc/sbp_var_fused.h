@@ -104,16 +104,19 @@ struct FusedOpV {
     }
     // w (b - A u) / diag(A) at (i, j) (the variable-coefficient smoother is
     // always damped Jacobi).
+    // (INT: the caller knows (i, j) is off the closure rows of both axes.)
+    template <bool INT = false>
     __device__ __forceinline__ double corr(int i, int j, double r, double omega, int) const {
-        const double d = INNER ? kdiag(j) + mdiag(i) : hw(i) * kdiag(j) + hw(j) * mdiag(i);
+        const double d = (INNER || INT) ? bd[H * AH + (j - aj0)] + bd[NQ * AH + H * AW + (i - ai0)]
+                                        : hw(i) * kdiag(j) + hw(j) * mdiag(i);
         return omega * r / d;
     }
     // (A u)(i, j), u in an LDS array holding columns [i0, i0 + W), rows [j0, ...).
-    template <int W>
+    template <int W, bool INT = false>
     __device__ __forceinline__ double au(const double* a, int i0, int j0, int i, int j) const {
         const int k = (j - j0) * W + (i - i0);
         double kp = 0.0, mp = 0.0;
-        if (INNER || (j >= R && j < N - R)) {
+        if (INNER || INT || (j >= R && j < N - R)) {
             const double* c = bd + (j - aj0);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) kp += c[q * AH] * a[k + (q - H) * W];
@@ -124,7 +127,7 @@ struct FusedOpV {
             for (int c = 0; c < C; ++c)
                 kp += (lo ? lt->k.lo[rr][c] : lt->k.hi[rr][c]) * a[((lo ? c : N - 1 - c) - j0) * W + (i - i0)];
         }
-        if (INNER || (i >= R && i < N - R)) {
+        if (INNER || INT || (i >= R && i < N - R)) {
             const double* c = bd + NQ * AH + (i - ai0);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) mp += c[q * AW] * a[k + q - H];
@@ -135,7 +138,7 @@ struct FusedOpV {
             for (int c = 0; c < C; ++c)
                 mp += (lo ? lt->m.lo[rr][c] : lt->m.hi[rr][c]) * a[(j - j0) * W + ((lo ? c : N - 1 - c) - i0)];
         }
-        if (INNER) return kp + mp;
+        if (INNER || INT) return kp + mp;
         return hw(i) * kp + hw(j) * mp;
     }
 };
