@@ -142,11 +142,31 @@ __device__ __forceinline__ void region_points(int N, int R, int x0, int y0, F&& 
             f(std::false_type{}, i, j, (j - y0) * W + (i - x0));
         }
     }
-    if (gw < W || gh < H) {
-        for (int k = tid; k < W * H; k += NT) {
-            const int r = k / W, c = k - r * W;
-            const int i = x0 + c, j = y0 + r;
-            if (i < 0 || i >= N || j < 0 || j >= N) z(k);
+    // outside the grid: whole local rows [0, top) and [H - bot, H), then the
+    // local columns [0, lw) and [W - rw, W) of the rows between
+    const int top = clampi(-y0, 0, H), bot = clampi(y0 + H - N, 0, H - top);
+    const int lw = clampi(-x0, 0, W), rw = clampi(x0 + W - N, 0, W - lw);
+    const int hm = H - top - bot;
+    const int z1 = top * W, z2 = z1 + bot * W, z3 = z2 + hm * lw, z4 = z3 + hm * rw;
+    if (z4 > 0) {
+        const float rlw = lw > 0 ? 1.0f / (float)lw : 0.0f, rrw = rw > 0 ? 1.0f / (float)rw : 0.0f;
+        for (int m = tid; m < z4; m += NT) {
+            int r, c;
+            if (m < z2) {
+                const int mm = m < z1 ? m : m - z1;
+                r = mm / W;
+                c = mm - r * W;
+                if (m >= z1) r += H - bot;
+            } else if (m < z3) {
+                const int mm = m - z2, q = div_small(mm, lw, rlw);
+                r = top + q;
+                c = mm - q * lw;
+            } else {
+                const int mm = m - z3, q = div_small(mm, rw, rrw);
+                r = top + q;
+                c = W - rw + (mm - q * rw);
+            }
+            z(r * W + c);
         }
     }
 }
@@ -242,16 +262,30 @@ __device__ __forceinline__ bool sr_body(OP& op, int N, const TG* __restrict__ b,
         }
         if (dflag) return false;
         op.store();
-        if constexpr (OP::kStage) __syncthreads();
+        if constexpr (OP::kInner) {
+            if constexpr (OP::kStage) __syncthreads();
 #pragma unroll
-        for (int m = 0; m < NA; ++m) {
-            const int k = tid + m * NT;
-            if (k < AW * AH) {
-                const int r = k / AW, c = k - r * AW;
-                const int i = ai0 + c, j = aj0 + r;
-                Bt[k] = v[m];
-                U[k] = (i >= 0 && i < N && j >= 0 && j < N) ? op.corr(i, j, v[m], omega, use_diag) : T(0);
+            for (int m = 0; m < NA; ++m) {
+                const int k = tid + m * NT;
+                if (k < AW * AH) {
+                    const int r = k / AW, c = k - r * AW;
+                    Bt[k] = v[m];
+                    U[k] = op.corr(ai0 + c, aj0 + r, v[m], omega, use_diag);
+                }
             }
+        } else {
+            // face block: b to LDS first, then x1 with the closure points
+            // (a division by their diagonal each) gathered into a few waves
+#pragma unroll
+            for (int m = 0; m < NA; ++m) {
+                const int k = tid + m * NT;
+                if (k < AW * AH) Bt[k] = v[m];
+            }
+            __syncthreads();
+            region_points<NT, AW, AH>(N, OP::R, ai0, aj0, [&](auto in_, int i, int j, int k) {
+                constexpr bool INT = decltype(in_)::value;
+                U[k] = op.template corr<INT>(i, j, Bt[k], omega, use_diag);
+            }, [](int) {});   // (x1 is read at grid points only)
         }
     }
     __syncthreads();
@@ -270,7 +304,7 @@ __device__ __forceinline__ bool sr_body(OP& op, int N, const TG* __restrict__ b,
             const int ka = (j - aj0) * AW + (i - ai0);
             const T Au = op.template au<AW, INT>(U, ai0, aj0, i, j);
             V[k] = U[ka] + op.template corr<INT>(i, j, Bt[ka] - Au, omega, use_diag);
-        }, [&](int k) { V[k] = T(0); });
+        }, [](int) {});   // (V is read at grid points only: no fill outside the grid)
     }
     __syncthreads();
     // x2 at the fine points this block owns
@@ -458,7 +492,7 @@ __device__ __forceinline__ bool ps_body(OP& op, int N, const TG* __restrict__ x,
             constexpr bool INT = decltype(in_)::value;
             const T Au = op.template au<AW, INT>(U, ai0, aj0, i, j);
             V[k] = U[(j - aj0) * AW + (i - ai0)] + op.template corr<INT>(i, j, Bt[k] - Au, omega, use_diag);
-        }, [&](int k) { V[k] = T(0); });
+        }, [](int) {});   // (V is read at grid points only: no fill outside the grid)
     }
     __syncthreads();
     // C: out = v1 + w D^-1 (b - A v1) on the owned points
