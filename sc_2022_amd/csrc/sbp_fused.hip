@@ -9,6 +9,14 @@ namespace sbp {
 
 namespace {
 constexpr int kFusedNT = 256;   // threads per block
+// Launches of at most kFusedWideBlocks blocks (the levels of side <= 257 of
+// one problem) run the 64-wide shapes with twice the threads: there every
+// block is one of a few per CU and the kernel's time is its blocks' chain of
+// phases, which more lanes per phase shorten (129^2: 7.8 against 9.7 us per
+// launch); with more blocks than CUs the extra lanes only fill the CUs sooner
+// (513^2: no change).
+constexpr int kFusedWideNT = 512;
+constexpr int kFusedWideBlocks = 512;
 
 // LDS of the two kernels' tiles (sbp_fused.h: smooth2_restrict holds b and
 // x1 on region A and x2 on region B; prolong_smooth2 the operand on A, the
@@ -36,15 +44,31 @@ template <int P, int TI, int J> constexpr bool fused_compiled() {
 template <int P, int TI, int J, typename T>
 void sr_launch(const OpTab& tf, const T* b, T* x2, T* rc, T omega, int use_diag, T scale, FusedCtl c,
                hipStream_t st) {
-    hipLaunchKernelGGL((smooth2_restrict<P, TI, J, kFusedNT, T>), dim3(c.nbx * c.nby * c.bt.nprob), dim3(kFusedNT),
-                       0, st, b, x2, rc, omega, use_diag, scale, tf, c);
+    const int nb = c.nbx * c.nby * c.bt.nprob;
+    if constexpr (TI == 64) {
+        if (nb <= kFusedWideBlocks) {
+            hipLaunchKernelGGL((smooth2_restrict<P, TI, J, kFusedWideNT, T>), dim3(nb), dim3(kFusedWideNT), 0, st, b,
+                               x2, rc, omega, use_diag, scale, tf, c);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((smooth2_restrict<P, TI, J, kFusedNT, T>), dim3(nb), dim3(kFusedNT), 0, st, b, x2, rc, omega,
+                       use_diag, scale, tf, c);
 }
 
 template <int P, int TI, int J, typename T>
 void ps_launch(const OpTab& tf, const T* x, const T* ec, const T* b, T* out, const T* dotv, T omega, int use_diag,
                FusedCtl c, hipStream_t st) {
-    hipLaunchKernelGGL((prolong_smooth2<P, TI, J, kFusedNT, T>), dim3(c.nbx * c.nby * c.bt.nprob), dim3(kFusedNT), 0,
-                       st, x, ec, b, out, dotv, omega, use_diag, tf, c);
+    const int nb = c.nbx * c.nby * c.bt.nprob;
+    if constexpr (TI == 64) {
+        if (nb <= kFusedWideBlocks) {
+            hipLaunchKernelGGL((prolong_smooth2<P, TI, J, kFusedWideNT, T>), dim3(nb), dim3(kFusedWideNT), 0, st, x,
+                               ec, b, out, dotv, omega, use_diag, tf, c);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((prolong_smooth2<P, TI, J, kFusedNT, T>), dim3(nb), dim3(kFusedNT), 0, st, x, ec, b, out,
+                       dotv, omega, use_diag, tf, c);
 }
 }  // namespace
 
