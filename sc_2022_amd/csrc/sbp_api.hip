@@ -1506,9 +1506,11 @@ int create_impl(int N, int p, int nprob, const sbp_params* prm, bool no_mg, sbp_
             } else {
                 default_tile(L.N, p, q, &L.ti[q], &L.tj[q], &L.ta[q]);
             }
-            // fused two-sweep blocks: small blocks keep the per-block chain short
+            // fused two-sweep blocks: small blocks keep the per-block chain
+            // short; from 257^2 points up 64 x 8 (half the blocks; 257^2: 8.0
+            // against 8.9 us per pre-smoothing launch, 513^2: 10.2 against 11.7)
             L.fti[q] = 64;
-            L.ftj[q] = (long long)L.n * h->np[l] >= 1025ll * 1025 ? 8 : 4;
+            L.ftj[q] = (long long)L.n * h->np[l] >= 257ll * 257 ? 8 : 4;
         }
     }
     int dev = 0, khz = 0;
@@ -1853,7 +1855,7 @@ int var_vcycle(sbp_handle h, int l, const T* b, T* x, const Scalars* sc, int* np
     // temporally blocked kernels, x2 = S(S(0)) with the restricted residual,
     // then x = S(S(x2 + P e)) (and the b'x partials).
     if (nu1 == 2 && nu2 == 2 && (long long)V.n <= (long long)h->prm.fuse_n * h->prm.fuse_n) {
-        const int ti = 64, tj = V.n >= (size_t)1025 * 1025 ? 8 : 4;
+        const int ti = 64, tj = V.n >= (size_t)257 * 257 ? 8 : 4;   // as the constant-coefficient levels
         const size_t nb = (size_t)(h->p + 1) * V.N;
         const VarFusedArgs a{V.N, V.dtab, V.band, V.band + nb, om, h->prm.restrict_scale, sc ? &sc->done : nullptr};
         if (!var_fused_shape_ok(ti, tj)) return SBP_ERR_INVALID_ARG;

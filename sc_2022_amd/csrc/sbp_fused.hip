@@ -9,14 +9,15 @@ namespace sbp {
 
 namespace {
 constexpr int kFusedNT = 256;   // threads per block
-// Launches of at most kFusedWideBlocks blocks (the levels of side <= 257 of
+// Launches of at most kFusedWideBlocks blocks (the levels of side <= 513 of
 // one problem) run the 64-wide shapes with twice the threads: there every
 // block is one of a few per CU and the kernel's time is its blocks' chain of
-// phases, which more lanes per phase shorten (129^2: 7.8 against 9.7 us per
-// launch); with more blocks than CUs the extra lanes only fill the CUs sooner
-// (513^2: no change).
+// phases, which more lanes per phase shorten (pre-smoothing + restriction,
+// 129^2: 7.8 against 9.7 us per launch; 513^2 on 64 x 8: 10.2 against 11.5);
+// with several blocks per CU the extra lanes gain nothing (1025^2 on 64 x 8,
+// 2 193 blocks: 16.7 against 16.2).
 constexpr int kFusedWideNT = 512;
-constexpr int kFusedWideBlocks = 512;
+constexpr int kFusedWideBlocks = 1024;
 
 // LDS of the two kernels' tiles (sbp_fused.h: smooth2_restrict holds b and
 // x1 on region A and x2 on region B; prolong_smooth2 the operand on A, the

@@ -10,21 +10,36 @@ namespace sbp {
 
 namespace {
 constexpr int kVarFusedNT = 256;   // threads per block
+// Launches of at most kVarFusedWideBlocks blocks run with twice the threads
+// (as the constant-coefficient two-sweep kernels do): few blocks per CU, a
+// time set by each block's chain of phases.
+constexpr int kVarFusedWideNT = 512;
+constexpr int kVarFusedWideBlocks = 1024;
 
 // Compiled block shapes (output columns x fine rows).
 #define SBP_VAR_FUSED_SHAPES(X) X(64, 4) X(64, 8)
 
 template <int P, int TI, int J, typename TG>
 void vsr_launch(const VarFusedArgs& a, const TG* b, TG* x2, TG* rc, FusedCtl c, hipStream_t st) {
-    hipLaunchKernelGGL((var_smooth2_restrict<P, TI, J, kVarFusedNT, TG>), dim3(c.nbx * c.nby), dim3(kVarFusedNT), 0,
-                       st, b, x2, rc, a.omega, a.scale, a.tab, a.kband, a.mband, a.N, c);
+    const int nb = c.nbx * c.nby;
+    if (nb <= kVarFusedWideBlocks)
+        hipLaunchKernelGGL((var_smooth2_restrict<P, TI, J, kVarFusedWideNT, TG>), dim3(nb), dim3(kVarFusedWideNT), 0,
+                           st, b, x2, rc, a.omega, a.scale, a.tab, a.kband, a.mband, a.N, c);
+    else
+        hipLaunchKernelGGL((var_smooth2_restrict<P, TI, J, kVarFusedNT, TG>), dim3(nb), dim3(kVarFusedNT), 0, st, b,
+                           x2, rc, a.omega, a.scale, a.tab, a.kband, a.mband, a.N, c);
 }
 
 template <int P, int TI, int J, typename TG>
 void vps_launch(const VarFusedArgs& a, const TG* x, const TG* ec, const TG* b, TG* out, const TG* dotv, FusedCtl c,
                 hipStream_t st) {
-    hipLaunchKernelGGL((var_prolong_smooth2<P, TI, J, kVarFusedNT, TG>), dim3(c.nbx * c.nby), dim3(kVarFusedNT), 0,
-                       st, x, ec, b, out, dotv, a.omega, a.tab, a.kband, a.mband, a.N, c);
+    const int nb = c.nbx * c.nby;
+    if (nb <= kVarFusedWideBlocks)
+        hipLaunchKernelGGL((var_prolong_smooth2<P, TI, J, kVarFusedWideNT, TG>), dim3(nb), dim3(kVarFusedWideNT), 0,
+                           st, x, ec, b, out, dotv, a.omega, a.tab, a.kband, a.mband, a.N, c);
+    else
+        hipLaunchKernelGGL((var_prolong_smooth2<P, TI, J, kVarFusedNT, TG>), dim3(nb), dim3(kVarFusedNT), 0, st, x,
+                           ec, b, out, dotv, a.omega, a.tab, a.kband, a.mband, a.N, c);
 }
 }  // namespace
 
