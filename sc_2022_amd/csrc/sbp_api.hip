@@ -98,6 +98,7 @@ struct sbp_handle_s {
     double clock_khz = 0.0;      // wall_clock64 rate
     // Captured Krylov iterations: [precision][0 = CG, 1 = MG-PCG]
     hipStream_t cap = nullptr;   // capture stream (default priority)
+    hipEvent_t pev = nullptr;    // end of a poll's scalar copy (poll_start)
     // [precision][0 = CG, 1 = MG-PCG][parity of the iteration]
     hipGraphExec_t gexec[2][2][2] = {};
     hipGraph_t graph[2][2][2] = {};
@@ -556,6 +557,24 @@ int poll(sbp_handle h, const Scalars* sc, bool* all_done = nullptr) {
     return SBP_OK;
 }
 
+// A poll in two halves: poll_start queues the scalar blocks' copy to the
+// pinned mirror and an event after it; poll_wait waits for that event only,
+// so work queued between the two runs on while the host waits.
+int poll_start(sbp_handle h, const Scalars* sc) {
+    if (!h->pev) HIPCHK(hipEventCreateWithFlags(&h->pev, hipEventDisableTiming));
+    HIPCHK(hipMemcpyAsync(h->sc_host, sc, sizeof(Scalars) * h->nprob, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipEventRecord(h->pev, h->st));
+    return SBP_OK;
+}
+
+int poll_wait(sbp_handle h, bool* all_done) {
+    HIPCHK(hipEventSynchronize(h->pev));
+    bool d = true;
+    for (int q = 0; q < h->nprob; ++q) d = d && h->sc_host[q].done;
+    *all_done = d;
+    return SBP_OK;
+}
+
 int status_of(const Scalars& s) {
     if (s.status == 5) return SBP_ERR_NONFINITE;
     if (s.status == 6) return SBP_ERR_BREAKDOWN;
@@ -858,21 +877,35 @@ int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use
     // Host polls: every poll_every iterations, or (poll_every = 0) after two
     // iterations and then where the measured convergence rate predicts the
     // last problem converges, so that few launches run for finished problems.
+    // The first adaptive poll only measures the rate: one more iteration is
+    // queued before the host waits for the scalars, so the device does not
+    // idle through the round trip (that iteration's kernels fall through on
+    // the done flags if every problem has already converged).
     const bool adaptive = h->prm.poll_every == 0;
     int next = adaptive ? 2 : h->prm.poll_every;
-    for (int it = 0; it < maxiter;) {
-        const int par = (it & 1) ? 0 : 1;   // iteration 0 has parity 1
+    auto launch = [&](int k) -> int {
+        const int par = (k & 1) ? 0 : 1;   // iteration 0 has parity 1
         if (ex[par]) {
             HIPCHK(hipGraphLaunch(ex[par], st));
-        } else {
-            STCHK(issue_iteration<T>(h, v, sc, use_mg, par));
+            return SBP_OK;
         }
+        return issue_iteration<T>(h, v, sc, use_mg, par);
+    };
+    bool first = true, all = false;
+    for (int it = 0; it < maxiter && !all;) {
+        STCHK(launch(it));
         ++it;
-        if (it >= next && it < maxiter) {
-            bool all = false;
-            STCHK(poll(h, sc, &all));
+        while (it >= next && it < maxiter) {
+            const int seen = it;   // iterations the copied scalars reflect
+            STCHK(poll_start(h, sc));
+            if (adaptive && first) {
+                STCHK(launch(it));
+                ++it;
+            }
+            first = false;
+            STCHK(poll_wait(h, &all));
             if (all) break;
-            next = it + (adaptive ? poll_ahead(h) : h->prm.poll_every);
+            next = seen + (adaptive ? poll_ahead(h) : h->prm.poll_every);
         }
     }
     if (h->ragged) {
@@ -1371,6 +1404,7 @@ int sbp_destroy(sbp_handle h) {
     if (h->vs) (void)rocsparse_destroy_handle(h->vs);
     if (h->vs_buf) (void)hipFree(h->vs_buf);
     if (h->cap) (void)hipStreamDestroy(h->cap);
+    if (h->pev) (void)hipEventDestroy(h->pev);
     if (h->child) (void)sbp_destroy(h->child);
     if (h->cexact) (void)sbp_destroy(h->cexact);
     if (h->vtab) (void)hipFree(h->vtab);
