@@ -393,6 +393,44 @@ def test_graph_replay_matches_eager(SP, p):
 
 
 @pytest.mark.parametrize("p", [2, 4])
+def test_adaptive_polls_match_polling_every_iteration(SP, p):
+    """The adaptive poll schedule (the first poll after two iterations with a
+    third already queued, later polls where the measured rate predicts
+    convergence) stops at the same iteration as polling after every one, with
+    the same history and a bit-identical iterate: for tolerances met first
+    at iterations 1..4 (converged before, at and after the queued third
+    iteration) and for maxiter 1..3 with an unreachable tolerance."""
+    from sc_2022_amd import _capi as capi
+    level = 7
+    N = 2 ** level + 1
+    A, b, _, _ = problem(level, p, "fourier")
+    ha = SP(N, p)                  # poll_every = 0: adaptive
+    h1 = SP(N, p, poll_every=1)
+    _, ref = h1.mgpcg(T(b), abstol=1e-14 * np.linalg.norm(b), history=True)
+    hist = ref.history
+    for k in (1, 2, 3, 4):
+        tol = 0.5 * (hist[k - 1] + hist[k])      # first met at iteration k
+        xa, ra = ha.mgpcg(T(b), abstol=tol, history=True)
+        x1, r1 = h1.mgpcg(T(b), abstol=tol, history=True)
+        assert ra.converged and ra.iters == r1.iters == k and ra.history == r1.history
+        assert torch.equal(xa, x1)
+    for m in (1, 2, 3):
+        xa, ra = ha.mgpcg(T(b), abstol=0.0, maxiter=m, raise_on_fail=False)
+        x1, r1 = h1.mgpcg(T(b), abstol=0.0, maxiter=m, raise_on_fail=False)
+        assert ra.status == r1.status == capi.SBP_ERR_NOT_CONVERGED and ra.iters == r1.iters == m
+        assert torch.equal(xa, x1)
+        xa, ra = ha.cg(T(b), abstol=0.0, maxiter=m, raise_on_fail=False)
+        x1, r1 = h1.cg(T(b), abstol=0.0, maxiter=m, raise_on_fail=False)
+        assert ra.iters == r1.iters == m and torch.equal(xa, x1)
+    for rt in (1e-4, 1e-12):
+        xa, ra = ha.mgpcg_mixed(T(b), abstol=rt * np.linalg.norm(b))
+        x1, r1 = h1.mgpcg_mixed(T(b), abstol=rt * np.linalg.norm(b))
+        assert ra.converged and (ra.iters, ra.inner_iters) == (r1.iters, r1.inner_iters)
+        assert torch.equal(xa, x1)
+    ha.close(); h1.close()
+
+
+@pytest.mark.parametrize("p", [2, 4])
 def test_mgpcg_bottom_kernel_matches_tiled_levels(SP, p):
     """The single-workgroup bottom of the V-cycle and the all-tiled V-cycle
     give the same MG-PCG iteration count and solution to rounding."""
