@@ -826,6 +826,40 @@ int poll_ahead(const sbp_handle h) {
     return need;
 }
 
+// Issues up to maxiter Krylov iterations (issue(k) queues iteration k, from
+// 0) with the host polls between them: every poll_every iterations, or
+// (poll_every = 0) after two iterations and then where the measured
+// convergence rate predicts the last problem converges, so that few launches
+// run for finished problems.  The first adaptive poll only measures the
+// rate: one more iteration is queued before the host waits for the scalars,
+// so the device does not idle through the round trip (that iteration's
+// kernels fall through on the done flags if every problem has already
+// converged).  Returns once every problem is done or maxiter iterations are
+// queued; the caller's last poll drains the stream.
+template <class F>
+int krylov_iterate(sbp_handle h, const Scalars* sc, int maxiter, F&& issue) {
+    const bool adaptive = h->prm.poll_every == 0;
+    int next = adaptive ? 2 : h->prm.poll_every;
+    bool first = true, all = false;
+    for (int it = 0; it < maxiter && !all;) {
+        STCHK(issue(it));
+        ++it;
+        while (it >= next && it < maxiter) {
+            const int seen = it;   // iterations the copied scalars reflect
+            STCHK(poll_start(h, sc));
+            if (adaptive && first) {
+                STCHK(issue(it));
+                ++it;
+            }
+            first = false;
+            STCHK(poll_wait(h, &all));
+            if (all) break;
+            next = seen + (adaptive ? poll_ahead(h) : h->prm.poll_every);
+        }
+    }
+    return SBP_OK;
+}
+
 // Preconditioned CG body shared by FP64 and FP32: solves A x = b for every
 // problem given the initial residuals in v.r (scalar blocks already set by the
 // caller's init kernel); the preconditioner is the V-cycle when use_mg, else
@@ -874,40 +908,14 @@ int pcg_loop(sbp_handle h, const KVecs<T>& v, Scalars* sc, int maxiter, bool use
         STCHK(iteration_graph<T>(h, v, sc, use_mg, 1, &ex[1]));
         STCHK(iteration_graph<T>(h, v, sc, use_mg, 0, &ex[0]));
     }
-    // Host polls: every poll_every iterations, or (poll_every = 0) after two
-    // iterations and then where the measured convergence rate predicts the
-    // last problem converges, so that few launches run for finished problems.
-    // The first adaptive poll only measures the rate: one more iteration is
-    // queued before the host waits for the scalars, so the device does not
-    // idle through the round trip (that iteration's kernels fall through on
-    // the done flags if every problem has already converged).
-    const bool adaptive = h->prm.poll_every == 0;
-    int next = adaptive ? 2 : h->prm.poll_every;
-    auto launch = [&](int k) -> int {
+    STCHK(krylov_iterate(h, sc, maxiter, [&](int k) -> int {
         const int par = (k & 1) ? 0 : 1;   // iteration 0 has parity 1
         if (ex[par]) {
             HIPCHK(hipGraphLaunch(ex[par], st));
             return SBP_OK;
         }
         return issue_iteration<T>(h, v, sc, use_mg, par);
-    };
-    bool first = true, all = false;
-    for (int it = 0; it < maxiter && !all;) {
-        STCHK(launch(it));
-        ++it;
-        while (it >= next && it < maxiter) {
-            const int seen = it;   // iterations the copied scalars reflect
-            STCHK(poll_start(h, sc));
-            if (adaptive && first) {
-                STCHK(launch(it));
-                ++it;
-            }
-            first = false;
-            STCHK(poll_wait(h, &all));
-            if (all) break;
-            next = seen + (adaptive ? poll_ahead(h) : h->prm.poll_every);
-        }
-    }
+    }));
     if (h->ragged) {
         for (int l = 0; l < h->nlev; ++l) {
             const int nj = joiners(h, l);
@@ -2004,9 +2012,7 @@ int var_pcg_f32vc(sbp_handle h, const double* b, double* x, double abstol, int m
                        h->pv, n);
     HIPCHK(hipGetLastError());
     r32_ready = true;
-    const bool adaptive = h->prm.poll_every == 0;
-    int next = adaptive ? 2 : h->prm.poll_every;
-    for (int it = 0; it < maxiter; ++it) {
+    STCHK(krylov_iterate(h, sc, maxiter, [&](int) -> int {
         STCHK(launch_var(h, 1, h->pv, h->Ap, h->pv, 1.0, sc, &np));
         hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, np, sc, 1);
         hipLaunchKernelGGL((update_xr_r32<kEwThreads>), g, dim3(kEwThreads), 0, st, x, h->r, h->r32,
@@ -2018,13 +2024,8 @@ int var_pcg_f32vc(sbp_handle h, const double* b, double* x, double abstol, int m
         hipLaunchKernelGGL((update_p<double, kEwThreads, float>), g, dim3(kEwThreads), 0, st, h->pv,
                            (const float*)h->z32, (const Scalars*)sc, n);
         HIPCHK(hipGetLastError());
-        if (it + 1 >= next && it + 1 < maxiter) {
-            bool all = false;
-            STCHK(poll(h, sc, &all));
-            if (all) break;
-            next = it + 1 + (adaptive ? poll_ahead(h) : h->prm.poll_every);
-        }
-    }
+        return SBP_OK;
+    }));
     return poll(h, sc);
 }
 }  // extern "C++"
@@ -2452,11 +2453,9 @@ int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, const double
     hipLaunchKernelGGL((copy_vec<double, kEwThreads>), dim3(ew_blocks(n), 1), dim3(kEwThreads), 0, st,
                        (const double*)h->r, h->pv, (size_t)n, (const Scalars*)sc);
     HIPCHK(hipGetLastError());
-    const bool adaptive = h->prm.poll_every == 0;
-    int next = adaptive ? 2 : h->prm.poll_every;
     const dim3 g(ew_blocks(n), 1);
     const int nbv = std::min(kCsrMaxBlocks, (n + 256 / kCsrLanes - 1) / (256 / kCsrLanes));
-    for (int it = 0; it < maxiter; ++it) {
+    STCHK(krylov_iterate(h, sc, maxiter, [&](int) -> int {
         hipLaunchKernelGGL(csr_spmv, dim3(nbv), dim3(256), 0, st, rp, col, val, (const double*)h->pv, h->Ap, n,
                            h->partial, (const Scalars*)sc);
         hipLaunchKernelGGL(krylov_alpha, dim3(1), dim3(kRedThreads), 0, st, h->partial, nbv, sc, 0);
@@ -2466,13 +2465,8 @@ int sbp_csr_cg(sbp_handle h, const int32_t* rp, const int32_t* col, const double
         hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)h->r,
                            (const Scalars*)sc, (size_t)n);
         HIPCHK(hipGetLastError());
-        if (it + 1 >= next && it + 1 < maxiter) {
-            bool all = false;
-            STCHK(poll(h, sc, &all));
-            if (all) break;
-            next = it + 1 + (adaptive ? poll_ahead(h) : h->prm.poll_every);
-        }
-    }
+        return SBP_OK;
+    }));
     STCHK(poll(h, sc));
     get_results(h, iters, resnorm, nullptr);
     return batch_status(h);
@@ -2558,9 +2552,7 @@ int sbp_csr_vendor_cg(sbp_handle h, const double* b, double* x, double abstol, i
     hipLaunchKernelGGL((copy_vec<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->r, h->pv, n,
                        (const Scalars*)sc);
     HIPCHK(hipGetLastError());
-    const bool adaptive = h->prm.poll_every == 0;
-    int next = adaptive ? 2 : h->prm.poll_every;
-    for (int it = 0; it < maxiter; ++it) {
+    STCHK(krylov_iterate(h, sc, maxiter, [&](int) -> int {
         STCHK(vendor_spmv(h, h->pv, h->Ap));
         hipLaunchKernelGGL((dot_partial<double, kEwThreads>), g, dim3(kEwThreads), 0, st, (const double*)h->pv,
                            (const double*)h->Ap, n, h->partial);
@@ -2571,13 +2563,8 @@ int sbp_csr_vendor_cg(sbp_handle h, const double* b, double* x, double abstol, i
         hipLaunchKernelGGL((update_p<double, kEwThreads>), g, dim3(kEwThreads), 0, st, h->pv, (const double*)h->r,
                            (const Scalars*)sc, n);
         HIPCHK(hipGetLastError());
-        if (it + 1 >= next && it + 1 < maxiter) {
-            bool all = false;
-            STCHK(poll(h, sc, &all));
-            if (all) break;
-            next = it + 1 + (adaptive ? poll_ahead(h) : h->prm.poll_every);
-        }
-    }
+        return SBP_OK;
+    }));
     STCHK(poll(h, sc));
     get_results(h, iters, resnorm, nullptr);
     return batch_status(h);
